@@ -1,0 +1,96 @@
+"""ctypes binding of libcufcm_fil.so (declarations in include/cufcm_fil.h).
+
+Loading fails loudly when the library has not been built: there is no CPU
+fallback anywhere in this package.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcufcm_fil.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "cufcm_fil.h")
+
+CFM_OK, CFM_EINVAL, CFM_EHIP, CFM_ENOMEM, CFM_ESTATE, CFM_ENOCONV, CFM_ENONFINITE = range(7)
+CFM_FP64, CFM_FP32C = 0, 1
+CFM_SS, CFM_SB, CFM_BB, CFM_BS, CFM_FULL = range(5)
+
+
+class CfmError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[cfm error {code}] {msg}")
+        self.code = code
+
+
+class cfm_config(ctypes.Structure):
+    _fields_ = [
+        ("n_seg", ctypes.c_int32),
+        ("n_blob", ctypes.c_int32),
+        ("seg_dof", ctypes.c_int32),
+        ("wall", ctypes.c_int32),
+        ("a_seg", ctypes.c_double),
+        ("a_blob", ctypes.c_double),
+        ("mu", ctypes.c_double),
+        ("precision", ctypes.c_int32),
+        ("device", ctypes.c_int32),
+        ("tile_split", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+    ]
+
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_vp = ctypes.c_void_p
+_lib = None
+
+
+def header_symbols() -> list[str]:
+    """Function names declared in include/cufcm_fil.h."""
+    with open(HEADER) as fh:
+        text = fh.read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(cfm_[a-z0-9_]+)\s*\(", text)))
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is not built; run `python -m cufcm_fil_amd.build` "
+            "(there is no CPU fallback)")
+    L = ctypes.CDLL(LIB_PATH)
+    L.cfm_abi_version.restype = ctypes.c_int
+    L.cfm_last_error.restype = ctypes.c_char_p
+    L.cfm_last_error.argtypes = [_vp]
+    L.cfm_create.argtypes = [ctypes.POINTER(cfm_config), ctypes.POINTER(_vp)]
+    L.cfm_destroy.argtypes = [_vp]
+    L.cfm_set_positions.argtypes = [_vp, _dp, _dp]
+    L.cfm_set_precision.argtypes = [_vp, ctypes.c_int32]
+    L.cfm_apply.argtypes = [_vp, ctypes.c_int32, _dp, _dp, _dp, _dp]
+    L.cfm_time_apply.argtypes = [_vp, ctypes.c_int32, ctypes.c_int32, _dp]
+    _lib = L
+    return L
+
+
+def ptr(a):
+    if a is None:
+        return None
+    return a.ctypes.data_as(_dp)
+
+
+def f64(a, n=None):
+    a = np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+    if n is not None and a.size != n:
+        raise ValueError(f"expected {n} values, got {a.size}")
+    return a
+
+
+def check(ctx, code):
+    if code != CFM_OK:
+        msg = lib().cfm_last_error(ctx)
+        raise CfmError(code, msg.decode() if msg else "")

@@ -1,0 +1,306 @@
+// api.hip -- C ABI entry points (declared in include/cufcm_fil.h).
+//
+// Host side of the mobility back-end: owns the HIP stream, the device particle
+// buffers and the pinned staging buffers, converts between the reference's
+// interleaved host layout and the device structure-of-arrays layout, and turns
+// every failure into a status code plus a message (the reference's
+// rpy_mobility_solver checks no return codes and exits on fatal paths).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+
+#include "ctx.hpp"
+
+namespace cfm {
+
+Status launch_apply(cfm_ctx* c, const ApplyPlan& plan);
+int plan_tiles(const cfm_ctx* c, const ApplyPlan& plan);
+
+// AoS [dof*cnt] (stage) -> SoA rows k of dst (dst + k*ld + off), k < dof.
+__global__ void aos_to_soa_kernel(const double* __restrict__ src, double* __restrict__ dst,
+                                  int cnt, int dof, int ld, int off) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= cnt) return;
+  for (int k = 0; k < dof; ++k) dst[(size_t)k * ld + off + i] = src[(size_t)dof * i + k];
+}
+
+__global__ void soa_to_aos_kernel(const double* __restrict__ src, double* __restrict__ dst,
+                                  int cnt, int dof, int ld, int off) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= cnt) return;
+  for (int k = 0; k < dof; ++k) dst[(size_t)dof * i + k] = src[(size_t)k * ld + off + i];
+}
+
+static Status ensure_pinned(cfm_ctx* c, size_t n) {
+  if (n <= c->host_pinned_n && c->host_pinned) return {};
+  if (c->host_pinned) hipHostFree(c->host_pinned);
+  c->host_pinned = nullptr;
+  c->host_pinned_n = 0;
+  CFM_HIP(hipHostMalloc((void**)&c->host_pinned, n * sizeof(double), hipHostMallocDefault));
+  c->host_pinned_n = n;
+  return {};
+}
+
+// host AoS -> device SoA rows [0, dof) of `dst` (leading dimension ld) at offset off
+Status upload_aos(cfm_ctx* c, const double* host, int cnt, int dof, double* dst, int ld,
+                  int off) {
+  if (cnt == 0) return {};
+  const size_t m = (size_t)cnt * dof;
+  CFM_TRY(ensure_pinned(c, m));
+  CFM_TRY(c->stage.alloc(std::max<size_t>(m, (size_t)6 * c->p.n)));
+  std::memcpy(c->host_pinned, host, m * sizeof(double));
+  CFM_HIP(hipMemcpyAsync(c->stage.p, c->host_pinned, m * sizeof(double), hipMemcpyHostToDevice,
+                         c->stream));
+  hipLaunchKernelGGL(aos_to_soa_kernel, dim3((cnt + 255) / 256), dim3(256), 0, c->stream,
+                     c->stage.p, dst, cnt, dof, ld, off);
+  CFM_CHECK_LAUNCH();
+  CFM_HIP(hipStreamSynchronize(c->stream));  // pinned buffer is reused by the next transfer
+  return {};
+}
+
+Status download_aos(cfm_ctx* c, double* host, int cnt, int dof, const double* src, int ld,
+                    int off) {
+  if (cnt == 0) return {};
+  const size_t m = (size_t)cnt * dof;
+  CFM_TRY(ensure_pinned(c, m));
+  CFM_TRY(c->stage.alloc(std::max<size_t>(m, (size_t)6 * c->p.n)));
+  hipLaunchKernelGGL(soa_to_aos_kernel, dim3((cnt + 255) / 256), dim3(256), 0, c->stream, src,
+                     c->stage.p, cnt, dof, ld, off);
+  CFM_CHECK_LAUNCH();
+  CFM_HIP(hipMemcpyAsync(c->host_pinned, c->stage.p, m * sizeof(double), hipMemcpyDeviceToHost,
+                         c->stream));
+  CFM_HIP(hipStreamSynchronize(c->stream));
+  std::memcpy(host, c->host_pinned, m * sizeof(double));
+  return {};
+}
+
+static bool all_finite(const double* a, size_t n) {
+  for (size_t i = 0; i < n; ++i)
+    if (!std::isfinite(a[i])) return false;
+  return true;
+}
+
+ApplyPlan plan_for(const cfm_ctx* c, int which) {
+  ApplyPlan pl;
+  const int ns = c->p.n_seg, n = c->p.n;
+  switch (which) {
+    case CFM_SS: pl.tgt_begin = 0; pl.tgt_end = ns; pl.src_seg = true; break;
+    case CFM_SB: pl.tgt_begin = 0; pl.tgt_end = ns; pl.src_blob = true; pl.acc_seg = true; break;
+    case CFM_BB: pl.tgt_begin = ns; pl.tgt_end = n; pl.src_blob = true; break;
+    case CFM_BS: pl.tgt_begin = ns; pl.tgt_end = n; pl.src_seg = true; break;
+    default: pl.tgt_begin = 0; pl.tgt_end = n; pl.src_seg = true; pl.src_blob = true; break;
+  }
+  return pl;
+}
+
+// Source split across workgroups so that a launch has enough workgroups to fill
+// 256 CUs (about 4 per CU), bounded by the number of 256-source tiles.
+int choose_split(const cfm_ctx* c, const ApplyPlan& pl) {
+  if (c->cfg.tile_split > 0) return c->cfg.tile_split;
+  const int tiles = std::max(1, plan_tiles(c, pl));
+  int nsrc = 0;
+  if (pl.src_seg) nsrc += c->p.n_seg;
+  if (pl.src_blob) nsrc += c->p.n_blob;
+  const int stiles = std::max(1, (nsrc + 255) / 256);
+  int s = (1024 + tiles - 1) / tiles;
+  s = std::min(s, std::max(1, stiles / 2));
+  return std::max(1, std::min(s, 64));
+}
+
+}  // namespace cfm
+
+using namespace cfm;
+
+static thread_local std::string g_create_error;
+
+static int fail(cfm_ctx* c, const Status& s) {
+  if (c) c->err = s.msg;
+  else g_create_error = s.msg;
+  return s.code;
+}
+
+extern "C" {
+
+int cfm_abi_version(void) { return CFM_ABI_VERSION; }
+
+const char* cfm_last_error(const cfm_ctx* ctx) {
+  return ctx ? ctx->err.c_str() : g_create_error.c_str();
+}
+
+int cfm_create(const cfm_config* cfg, cfm_ctx** out) {
+  if (!cfg || !out) return fail(nullptr, make_status(CFM_EINVAL, "null argument"));
+  *out = nullptr;
+  if (cfg->n_seg < 0 || cfg->n_blob < 0 || cfg->n_seg + cfg->n_blob <= 0)
+    return fail(nullptr, make_status(CFM_EINVAL, "need n_seg >= 0, n_blob >= 0, n > 0"));
+  if (cfg->seg_dof != 3 && cfg->seg_dof != 6)
+    return fail(nullptr, make_status(CFM_EINVAL, "seg_dof must be 3 or 6 (got %d)", cfg->seg_dof));
+  if (!(cfg->mu > 0.0) || !(cfg->a_seg > 0.0) || !(cfg->a_blob > 0.0))
+    return fail(nullptr, make_status(CFM_EINVAL, "mu and radii must be positive"));
+  if (cfg->wall && (cfg->a_seg != 1.0 || cfg->a_blob != 1.0))
+    return fail(nullptr, make_status(CFM_EINVAL,
+                                     "the wall-corrected branch assumes unit radii (as the "
+                                     "reference's cuda_functions.cu:22)"));
+  if (cfg->precision != CFM_FP64 && cfg->precision != CFM_FP32C)
+    return fail(nullptr, make_status(CFM_EINVAL, "unknown precision %d", cfg->precision));
+  if ((int64_t)cfg->n_seg + cfg->n_blob > (int64_t)1 << 28)
+    return fail(nullptr, make_status(CFM_EINVAL, "too many particles"));
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+    return fail(nullptr, make_status(CFM_EHIP, "no HIP device available"));
+  if (cfg->device < 0 || cfg->device >= ndev)
+    return fail(nullptr, make_status(CFM_EINVAL, "device %d out of range (%d)", cfg->device, ndev));
+  cfm_ctx* c = new (std::nothrow) cfm_ctx();
+  if (!c) return fail(nullptr, make_status(CFM_ENOMEM, "context allocation failed"));
+  c->cfg = *cfg;
+  c->device = cfg->device;
+  auto init = [&]() -> Status {
+    CFM_HIP(hipSetDevice(c->device));
+    CFM_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    CFM_HIP(hipEventCreate(&c->ev0));
+    CFM_HIP(hipEventCreate(&c->ev1));
+    Particles& p = c->p;
+    p.n_seg = cfg->n_seg;
+    p.n_blob = cfg->n_blob;
+    p.n = p.n_seg + p.n_blob;
+    p.seg_dof = cfg->seg_dof;
+    CFM_TRY(p.x.alloc(p.n));
+    CFM_TRY(p.y.alloc(p.n));
+    CFM_TRY(p.z.alloc(p.n));
+    CFM_TRY(p.f.alloc((size_t)6 * p.n));
+    CFM_TRY(p.v.alloc((size_t)6 * p.n));
+    CFM_HIP(hipMemsetAsync(p.f.p, 0, sizeof(double) * 6 * p.n, c->stream));
+    CFM_HIP(hipMemsetAsync(p.v.p, 0, sizeof(double) * 6 * p.n, c->stream));
+    CFM_HIP(hipStreamSynchronize(c->stream));
+    return {};
+  };
+  Status s = init();
+  if (!s.ok()) {
+    cfm_destroy(c);
+    return fail(nullptr, s);
+  }
+  *out = c;
+  return CFM_OK;
+}
+
+int cfm_destroy(cfm_ctx* c) {
+  if (!c) return CFM_OK;
+  hipSetDevice(c->device);
+  if (c->stream) hipStreamSynchronize(c->stream);
+  c->p.x.release();
+  c->p.y.release();
+  c->p.z.release();
+  c->p.f.release();
+  c->p.v.release();
+  c->p.partial.release();
+  c->stage.release();
+  if (c->host_pinned) hipHostFree(c->host_pinned);
+  if (c->ev0) hipEventDestroy(c->ev0);
+  if (c->ev1) hipEventDestroy(c->ev1);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+  return CFM_OK;
+}
+
+int cfm_set_positions(cfm_ctx* c, const double* x_seg, const double* x_blob) {
+  if (!c) return CFM_EINVAL;
+  Particles& p = c->p;
+  if ((p.n_seg > 0 && !x_seg) || (p.n_blob > 0 && !x_blob))
+    return fail(c, make_status(CFM_EINVAL, "null position array"));
+  if ((p.n_seg && !all_finite(x_seg, 3 * (size_t)p.n_seg)) ||
+      (p.n_blob && !all_finite(x_blob, 3 * (size_t)p.n_blob)))
+    return fail(c, make_status(CFM_ENONFINITE, "non-finite position"));
+  if (c->cfg.wall) {
+    for (int i = 0; i < p.n_seg; ++i)
+      if (!(x_seg[3 * i + 2] > 0.0))
+        return fail(c, make_status(CFM_EINVAL, "segment %d is not above the wall (z = %g)", i,
+                                   x_seg[3 * i + 2]));
+    for (int i = 0; i < p.n_blob; ++i)
+      if (!(x_blob[3 * i + 2] > 0.0))
+        return fail(c, make_status(CFM_EINVAL, "blob %d is not above the wall (z = %g)", i,
+                                   x_blob[3 * i + 2]));
+  }
+  hipSetDevice(c->device);
+  // The three coordinate arrays are allocated separately; stage them as rows of
+  // a 3 x n block, then scatter.
+  auto up = [&](const double* h, int cnt, int off) -> Status {
+    if (cnt == 0) return {};
+    CFM_TRY(c->stage.alloc((size_t)6 * p.n + (size_t)3 * cnt));
+    double* rows = c->stage.p + (size_t)6 * p.n;  // 3 x cnt scratch after the AoS stage
+    CFM_TRY(upload_aos(c, h, cnt, 3, rows, cnt, 0));
+    CFM_HIP(hipMemcpyAsync(p.x.p + off, rows, sizeof(double) * cnt, hipMemcpyDeviceToDevice, c->stream));
+    CFM_HIP(hipMemcpyAsync(p.y.p + off, rows + cnt, sizeof(double) * cnt, hipMemcpyDeviceToDevice, c->stream));
+    CFM_HIP(hipMemcpyAsync(p.z.p + off, rows + 2 * (size_t)cnt, sizeof(double) * cnt, hipMemcpyDeviceToDevice, c->stream));
+    CFM_HIP(hipStreamSynchronize(c->stream));
+    return {};
+  };
+  Status s = up(x_seg, p.n_seg, 0);
+  if (s.ok()) s = up(x_blob, p.n_blob, p.n_seg);
+  if (!s.ok()) return fail(c, s);
+  c->have_positions = true;
+  return CFM_OK;
+}
+
+int cfm_set_precision(cfm_ctx* c, int32_t precision) {
+  if (!c) return CFM_EINVAL;
+  if (precision != CFM_FP64 && precision != CFM_FP32C)
+    return fail(c, make_status(CFM_EINVAL, "unknown precision %d", precision));
+  c->cfg.precision = precision;
+  return CFM_OK;
+}
+
+int cfm_apply(cfm_ctx* c, int32_t which, const double* f_seg, const double* f_blob,
+              double* v_seg, double* v_blob) {
+  if (!c) return CFM_EINVAL;
+  if (which < CFM_SS || which > CFM_FULL)
+    return fail(c, make_status(CFM_EINVAL, "unknown block id %d", which));
+  if (!c->have_positions) return fail(c, make_status(CFM_ESTATE, "positions not set"));
+  Particles& p = c->p;
+  const ApplyPlan pl = plan_for(c, which);
+  const int sd = p.seg_dof;
+  const bool tgt_seg = pl.tgt_begin < p.n_seg && p.n_seg > 0;
+  const bool tgt_blob = pl.tgt_end > p.n_seg && p.n_blob > 0;
+  if ((pl.src_seg && p.n_seg && !f_seg) || (pl.src_blob && p.n_blob && !f_blob) ||
+      (tgt_seg && !v_seg) || (tgt_blob && !v_blob))
+    return fail(c, make_status(CFM_EINVAL, "null load or velocity array for block %d", which));
+  if ((pl.src_seg && p.n_seg && !all_finite(f_seg, (size_t)sd * p.n_seg)) ||
+      (pl.src_blob && p.n_blob && !all_finite(f_blob, 3 * (size_t)p.n_blob)))
+    return fail(c, make_status(CFM_ENONFINITE, "non-finite load"));
+  hipSetDevice(c->device);
+  auto run = [&]() -> Status {
+    if (pl.src_seg) CFM_TRY(upload_aos(c, f_seg, p.n_seg, sd, p.f.p, p.n, 0));
+    if (pl.src_blob) CFM_TRY(upload_aos(c, f_blob, p.n_blob, 3, p.f.p, p.n, p.n_seg));
+    if (pl.acc_seg && tgt_seg) CFM_TRY(upload_aos(c, v_seg, p.n_seg, sd, p.v.p, p.n, 0));
+    if (pl.acc_blob && tgt_blob) CFM_TRY(upload_aos(c, v_blob, p.n_blob, 3, p.v.p, p.n, p.n_seg));
+    c->split = choose_split(c, pl);
+    CFM_TRY(launch_apply(c, pl));
+    if (tgt_seg) CFM_TRY(download_aos(c, v_seg, p.n_seg, sd, p.v.p, p.n, 0));
+    if (tgt_blob) CFM_TRY(download_aos(c, v_blob, p.n_blob, 3, p.v.p, p.n, p.n_seg));
+    return {};
+  };
+  Status s = run();
+  return s.ok() ? CFM_OK : fail(c, s);
+}
+
+int cfm_time_apply(cfm_ctx* c, int32_t which, int32_t reps, double* ms_per_apply) {
+  if (!c || !ms_per_apply || reps <= 0) return CFM_EINVAL;
+  if (!c->have_positions) return fail(c, make_status(CFM_ESTATE, "positions not set"));
+  hipSetDevice(c->device);
+  const ApplyPlan pl = plan_for(c, which);
+  auto run = [&]() -> Status {
+    c->split = choose_split(c, pl);
+    CFM_TRY(launch_apply(c, pl));  // warm-up (also allocates split partials)
+    CFM_HIP(hipEventRecord(c->ev0, c->stream));
+    for (int r = 0; r < reps; ++r) CFM_TRY(launch_apply(c, pl));
+    CFM_HIP(hipEventRecord(c->ev1, c->stream));
+    CFM_HIP(hipEventSynchronize(c->ev1));
+    float ms = 0.f;
+    CFM_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    *ms_per_apply = double(ms) / reps;
+    return {};
+  };
+  Status s = run();
+  return s.ok() ? CFM_OK : fail(c, s);
+}
+
+}  // extern "C"

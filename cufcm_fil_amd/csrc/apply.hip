@@ -1,0 +1,349 @@
+// apply.hip -- tiled all-pairs wall-RPY mobility products on gfx950.
+//
+// Replaces Mss/Mbb/Msb/Mbs_mult (src/cilia/cuda_functions.cu:362-757).
+//
+// Layout: positions x,y,z and loads f[k] are structure-of-arrays in HBM over the
+// combined particle index (segments, then blobs).  A workgroup of 256 threads
+// (4 wave64s) owns 64 targets, one per lane.  Sources are streamed through LDS in
+// tiles of 256 (one coalesced load per thread); within a tile wave w evaluates
+// sources [64w, 64w+64) against its 64 targets with the source record broadcast
+// from LDS.  At the end the four waves' partial sums are reduced through LDS in a
+// fixed order (deterministic).  With split > 1 the source tiles are additionally
+// dealt round-robin to `split` workgroups per target tile and a second kernel adds
+// the partials in slice order.
+//
+// FP64 path: pair arithmetic and accumulation in double.
+// FP32C path: separations are formed in double and narrowed, the pair block is
+// evaluated in float, each 64-source tile is summed in float and folded into a
+// per-target two-sum (Kahan-Babuska) accumulator; the final value is s + c in
+// double.
+//
+// The self term of the wall branch is keyed on the global particle id and is
+// evaluated in double during finalisation.
+#include "ctx.hpp"
+#include "rpy_pair.hpp"
+
+namespace cfm {
+
+constexpr int kWG = 256;    // threads per workgroup
+constexpr int kTT = 64;     // targets per workgroup
+constexpr int kTS = 256;    // sources per LDS tile
+constexpr int kWaves = kWG / 64;
+
+template <typename T> struct alignas(16) SrcRec;
+template <> struct alignas(16) SrcRec<float> {
+  double x, y, z;
+  float zf;
+  float f[6];
+  float pad_;
+};
+template <> struct alignas(16) SrcRec<double> {
+  double x, y, z, pad_;
+  double f[6];
+};
+
+struct KArgs {
+  const double* __restrict__ x;
+  const double* __restrict__ y;
+  const double* __restrict__ z;
+  const double* __restrict__ f;   // 6 x n
+  double* __restrict__ v;         // 6 x n
+  double* __restrict__ partial;   // split x 6 x n
+  int n, n_seg, n_blob, seg_dof;
+  int tgt_begin, tgt_end;
+  int seg_tiles;                  // target tiles that hold segments
+  int src_seg, src_blob;
+  int acc_seg, acc_blob;
+  int split;
+  int wall;
+  double mu, a_seg, a_blob;
+};
+
+// Two-sum accumulation of a float partial into (s, c).
+__device__ __forceinline__ void two_sum_add(float& s, float& c, float x) {
+  const float t = s + x;
+  const float bp = t - s;
+  const float err = (s - (t - bp)) + (x - bp);
+  s = t;
+  c += err;
+}
+
+template <typename T, int TD, int FD, bool WALL, bool CHECK>
+__device__ __forceinline__ void tile_sweep(const SrcRec<T>* __restrict__ rec, int cnt, int j0,
+                                           int iid, double xi, double yi, double zi, T zif,
+                                           T ai, T aj, T mu, T* __restrict__ acc) {
+#pragma unroll 2
+  for (int jj = 0; jj < cnt; ++jj) {
+    const SrcRec<T>& s = rec[jj];
+    T dx = T(xi - s.x), dy = T(yi - s.y), dz = T(zi - s.z);
+    T F[FD];
+#pragma unroll
+    for (int k = 0; k < FD; ++k) F[k] = s.f[k];
+    if constexpr (WALL) {
+      T zj;
+      if constexpr (sizeof(T) == 4) zj = s.zf; else zj = s.z;
+      if constexpr (CHECK) {
+        if (j0 + jj == iid) {  // same particle: handled by the self term
+          dx = T(1);
+          dy = T(0);
+          dz = T(0);
+#pragma unroll
+          for (int k = 0; k < FD; ++k) F[k] = T(0);
+        }
+      }
+      wall_pair<T, TD, FD>(dx, dy, dz, zif + zj, zj, F, acc);
+    } else {
+      free_pair<T, TD, FD>(dx, dy, dz, ai, aj, mu, F, acc);
+    }
+  }
+}
+
+// Loops the source tiles of one species [s_begin, s_end) (ids) for this workgroup.
+template <typename T, int TD, int FD, bool WALL>
+__device__ __forceinline__ void sweep_species(const KArgs& a, SrcRec<T>* tile, int s_begin,
+                                              int s_end, int slice, int t_lo, int t_hi,
+                                              int iid, double xi, double yi, double zi, T ai,
+                                              T aj, double* accd, float* s32, float* c32) {
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int n = a.n;
+  const T zif = T(zi);
+  const int ntiles = (s_end - s_begin + kTS - 1) / kTS;
+  for (int tix = slice; tix < ntiles; tix += a.split) {
+    const int t0 = s_begin + tix * kTS;
+    const int j = t0 + threadIdx.x;
+    __syncthreads();
+    if (j < s_end) {
+      SrcRec<T> r;
+      r.x = a.x[j];
+      r.y = a.y[j];
+      r.z = a.z[j];
+      if constexpr (sizeof(T) == 4) r.zf = float(r.z);
+#pragma unroll
+      for (int k = 0; k < FD; ++k) r.f[k] = T(a.f[(size_t)k * n + j]);
+#pragma unroll
+      for (int k = FD; k < 6; ++k) r.f[k] = T(0);
+      tile[threadIdx.x] = r;
+    }
+    __syncthreads();
+    const int w0 = t0 + wave * 64;
+    const int cnt = min(64, s_end - w0);
+    if (cnt <= 0) continue;
+    const SrcRec<T>* rec = tile + wave * 64;
+    // the only tiles that may contain the target itself
+    const bool check = WALL && (w0 < t_hi) && (w0 + cnt > t_lo);
+    if constexpr (sizeof(T) == 4) {
+      T part[TD];
+#pragma unroll
+      for (int k = 0; k < TD; ++k) part[k] = T(0);
+      if (check)
+        tile_sweep<T, TD, FD, WALL, true>(rec, cnt, w0, iid, xi, yi, zi, zif, ai, aj, T(a.mu), part);
+      else
+        tile_sweep<T, TD, FD, WALL, false>(rec, cnt, w0, iid, xi, yi, zi, zif, ai, aj, T(a.mu), part);
+#pragma unroll
+      for (int k = 0; k < TD; ++k) two_sum_add(s32[k], c32[k], part[k]);
+    } else {
+      T part[TD];
+#pragma unroll
+      for (int k = 0; k < TD; ++k) part[k] = T(0);
+      if (check)
+        tile_sweep<T, TD, FD, WALL, true>(rec, cnt, w0, iid, xi, yi, zi, zif, ai, aj, T(a.mu), part);
+      else
+        tile_sweep<T, TD, FD, WALL, false>(rec, cnt, w0, iid, xi, yi, zi, zif, ai, aj, T(a.mu), part);
+#pragma unroll
+      for (int k = 0; k < TD; ++k) accd[k] += double(part[k]);
+    }
+    (void)lane;
+  }
+}
+
+template <int TD>
+__device__ __forceinline__ void add_self(const KArgs& a, int i, double zi, double* out) {
+  // self mobility (wall branch), only when the target's own species is a source
+  const bool is_seg = i < a.n_seg;
+  const bool own = is_seg ? a.src_seg : a.src_blob;
+  if (!a.wall || !own) return;
+  double F[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+  for (int k = 0; k < TD; ++k) F[k] = a.f[(size_t)k * a.n + i];
+  wall_self<TD, TD>(zi, F, out);
+}
+
+template <typename T, int TD, int SD, bool WALL>
+__device__ void apply_body(const KArgs& a, SrcRec<T>* tile, double (*red)[kTT], int t_lo,
+                           int t_hi, int slice) {
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int i = t_lo + lane;
+  const bool valid = i < t_hi;
+  const int ic = valid ? i : t_lo;
+  const double xi = a.x[ic], yi = a.y[ic], zi = a.z[ic];
+  const bool tgt_seg = t_lo < a.n_seg;
+  const T ai = T(tgt_seg ? a.a_seg : a.a_blob);
+  double accd[TD];
+  float s32[TD], c32[TD];
+#pragma unroll
+  for (int k = 0; k < TD; ++k) {
+    accd[k] = 0.0;
+    s32[k] = 0.0f;
+    c32[k] = 0.0f;
+  }
+  const int iid = valid ? i : -1;
+  if (a.src_seg && a.n_seg > 0)
+    sweep_species<T, TD, SD, WALL>(a, tile, 0, a.n_seg, slice, t_lo, t_hi, iid, xi, yi, zi, ai,
+                                   T(a.a_seg), accd, s32, c32);
+  if (a.src_blob && a.n_blob > 0)
+    sweep_species<T, TD, 3, WALL>(a, tile, a.n_seg, a.n, slice, t_lo, t_hi, iid, xi, yi, zi, ai,
+                                  T(a.a_blob), accd, s32, c32);
+  // cross-wave reduction in a fixed order
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < TD; ++k)
+    red[wave * 6 + k][lane] = (sizeof(T) == 4) ? double(s32[k]) + double(c32[k]) : accd[k];
+  __syncthreads();
+  if (wave != 0 || !valid) return;
+  double out[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+  for (int k = 0; k < TD; ++k) {
+    double s = red[k][lane];
+#pragma unroll
+    for (int w = 1; w < kWaves; ++w) s += red[w * 6 + k][lane];
+    out[k] = s;
+  }
+  const size_t n = a.n;
+  if (a.split > 1) {
+#pragma unroll
+    for (int k = 0; k < TD; ++k) a.partial[((size_t)slice * 6 + k) * n + i] = out[k];
+    return;
+  }
+  if (slice == 0) add_self<TD>(a, i, zi, out);
+  const double scale = a.wall ? 1.0 / a.mu : 1.0;
+  const bool acc = tgt_seg ? a.acc_seg : a.acc_blob;
+#pragma unroll
+  for (int k = 0; k < TD; ++k) {
+    double* dst = a.v + (size_t)k * n + i;
+    *dst = acc ? *dst + scale * out[k] : scale * out[k];
+  }
+}
+
+template <typename T, int SD, bool WALL>
+__global__ __launch_bounds__(kWG) void rpy_apply_kernel(KArgs a) {
+  __shared__ SrcRec<T> tile[kTS];
+  __shared__ double red[kWaves * 6][kTT];
+  const int t = blockIdx.x;
+  const int slice = blockIdx.y;
+  int t_lo, t_hi;
+  // target tiles: segment tiles first (never straddling the species boundary)
+  const int seg_lo = max(a.tgt_begin, 0), seg_hi = min(a.tgt_end, a.n_seg);
+  if (t < a.seg_tiles) {
+    t_lo = seg_lo + t * kTT;
+    t_hi = min(t_lo + kTT, seg_hi);
+    apply_body<T, SD, SD, WALL>(a, tile, red, t_lo, t_hi, slice);
+  } else {
+    const int blob_lo = max(a.tgt_begin, a.n_seg), blob_hi = min(a.tgt_end, a.n);
+    t_lo = blob_lo + (t - a.seg_tiles) * kTT;
+    t_hi = min(t_lo + kTT, blob_hi);
+    apply_body<T, 3, SD, WALL>(a, tile, red, t_lo, t_hi, slice);
+  }
+}
+
+// Sum the split partials in slice order, add the self term, scale and store.
+__global__ __launch_bounds__(256) void rpy_finalize_kernel(KArgs a) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = a.tgt_begin + idx;
+  if (i >= a.tgt_end) return;
+  const bool is_seg = i < a.n_seg;
+  const int td = is_seg ? a.seg_dof : 3;
+  const size_t n = a.n;
+  double out[6] = {0, 0, 0, 0, 0, 0};
+  for (int s = 0; s < a.split; ++s)
+    for (int k = 0; k < td; ++k) out[k] += a.partial[((size_t)s * 6 + k) * n + i];
+  const double zi = a.z[i];
+  if (td == 6) add_self<6>(a, i, zi, out); else add_self<3>(a, i, zi, out);
+  const double scale = a.wall ? 1.0 / a.mu : 1.0;
+  const bool acc = is_seg ? a.acc_seg : a.acc_blob;
+  for (int k = 0; k < td; ++k) {
+    double* dst = a.v + (size_t)k * n + i;
+    *dst = acc ? *dst + scale * out[k] : scale * out[k];
+  }
+}
+
+template <typename T, int SD, bool WALL>
+static Status launch_t(const KArgs& a, int tiles, int split, hipStream_t st) {
+  dim3 grid(tiles, split);
+  hipLaunchKernelGGL((rpy_apply_kernel<T, SD, WALL>), grid, dim3(kWG), 0, st, a);
+  CFM_CHECK_LAUNCH();
+  return {};
+}
+
+Status launch_apply(cfm_ctx* c, const ApplyPlan& plan) {
+  Particles& p = c->p;
+  KArgs a{};
+  a.x = p.x.p;
+  a.y = p.y.p;
+  a.z = p.z.p;
+  a.f = p.f.p;
+  a.v = p.v.p;
+  a.partial = p.partial.p;
+  a.n = p.n;
+  a.n_seg = p.n_seg;
+  a.n_blob = p.n_blob;
+  a.seg_dof = p.seg_dof;
+  a.tgt_begin = plan.tgt_begin;
+  a.tgt_end = plan.tgt_end;
+  a.src_seg = plan.src_seg;
+  a.src_blob = plan.src_blob;
+  a.acc_seg = plan.acc_seg;
+  a.acc_blob = plan.acc_blob;
+  a.wall = c->cfg.wall;
+  a.mu = c->cfg.mu;
+  a.a_seg = c->cfg.a_seg;
+  a.a_blob = c->cfg.a_blob;
+  const int seg_lo = std::max(plan.tgt_begin, 0), seg_hi = std::min(plan.tgt_end, p.n_seg);
+  const int blob_lo = std::max(plan.tgt_begin, p.n_seg), blob_hi = std::min(plan.tgt_end, p.n);
+  a.seg_tiles = seg_hi > seg_lo ? (seg_hi - seg_lo + kTT - 1) / kTT : 0;
+  const int blob_tiles = blob_hi > blob_lo ? (blob_hi - blob_lo + kTT - 1) / kTT : 0;
+  const int tiles = a.seg_tiles + blob_tiles;
+  if (tiles == 0) return {};
+  int split = std::max(1, c->split);
+  a.split = split;
+  if (split > 1) {
+    CFM_TRY(p.partial.alloc((size_t)split * 6 * p.n));
+    a.partial = p.partial.p;
+  }
+  const bool fp32 = c->cfg.precision == CFM_FP32C;
+  const bool wall = c->cfg.wall != 0;
+  const int sd = p.seg_dof;
+  Status s;
+  if (fp32) {
+    if (sd == 6) s = wall ? launch_t<float, 6, true>(a, tiles, split, c->stream)
+                          : launch_t<float, 6, false>(a, tiles, split, c->stream);
+    else s = wall ? launch_t<float, 3, true>(a, tiles, split, c->stream)
+                  : launch_t<float, 3, false>(a, tiles, split, c->stream);
+  } else {
+    if (sd == 6) s = wall ? launch_t<double, 6, true>(a, tiles, split, c->stream)
+                          : launch_t<double, 6, false>(a, tiles, split, c->stream);
+    else s = wall ? launch_t<double, 3, true>(a, tiles, split, c->stream)
+                  : launch_t<double, 3, false>(a, tiles, split, c->stream);
+  }
+  CFM_TRY(s);
+  if (split > 1) {
+    const int cnt = plan.tgt_end - plan.tgt_begin;
+    hipLaunchKernelGGL(rpy_finalize_kernel, dim3((cnt + 255) / 256), dim3(256), 0, c->stream, a);
+    CFM_CHECK_LAUNCH();
+  }
+  return {};
+}
+
+// Number of workgroups launched for a plan (used to choose the split).
+int plan_tiles(const cfm_ctx* c, const ApplyPlan& plan) {
+  const Particles& p = c->p;
+  const int seg_lo = std::max(plan.tgt_begin, 0), seg_hi = std::min(plan.tgt_end, p.n_seg);
+  const int blob_lo = std::max(plan.tgt_begin, p.n_seg), blob_hi = std::min(plan.tgt_end, p.n);
+  int t = 0;
+  if (seg_hi > seg_lo) t += (seg_hi - seg_lo + kTT - 1) / kTT;
+  if (blob_hi > blob_lo) t += (blob_hi - blob_lo + kTT - 1) / kTT;
+  return t;
+}
+
+}  // namespace cfm

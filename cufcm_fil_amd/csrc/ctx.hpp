@@ -1,0 +1,106 @@
+// ctx.hpp -- library context, error plumbing and shared device-buffer layout.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/cufcm_fil.h"
+
+namespace cfm {
+
+// Status carrier used inside the library; converted to int at the C boundary.
+struct Status {
+  int code = CFM_OK;
+  std::string msg;
+  bool ok() const { return code == CFM_OK; }
+};
+
+inline Status make_status(int code, const char* fmt, ...) {
+  Status s;
+  s.code = code;
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  s.msg = buf;
+  return s;
+}
+
+#define CFM_HIP(call)                                                                   \
+  do {                                                                                  \
+    hipError_t e_ = (call);                                                             \
+    if (e_ != hipSuccess)                                                               \
+      return ::cfm::make_status(CFM_EHIP, "%s failed: %s (%s:%d)", #call,               \
+                                hipGetErrorString(e_), __FILE__, __LINE__);             \
+  } while (0)
+
+#define CFM_TRY(expr)              \
+  do {                             \
+    ::cfm::Status s_ = (expr);     \
+    if (!s_.ok()) return s_;       \
+  } while (0)
+
+#define CFM_CHECK_LAUNCH() CFM_HIP(hipGetLastError())
+
+// Device buffer owned by a context.
+template <typename T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  Status alloc(size_t count) {
+    if (count <= n && p) return {};
+    if (p) hipFree(p);
+    p = nullptr;
+    n = 0;
+    if (count == 0) return {};
+    hipError_t e = hipMalloc(&p, count * sizeof(T));
+    if (e != hipSuccess)
+      return make_status(CFM_ENOMEM, "hipMalloc(%zu bytes): %s", count * sizeof(T),
+                         hipGetErrorString(e));
+    n = count;
+    return {};
+  }
+  void release() {
+    if (p) hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+};
+
+// Particle storage on the device, structure of arrays over the combined index
+// space [segments 0..n_seg) ++ [blobs n_seg..n_seg+n_blob).
+struct Particles {
+  int n_seg = 0, n_blob = 0, n = 0;
+  int seg_dof = 6;
+  DevBuf<double> x, y, z;     // positions
+  DevBuf<double> f;           // 6 x n (component-major: f + k*n); blobs use k < 3
+  DevBuf<double> v;           // 6 x n
+  DevBuf<double> partial;     // split x 6 x n partial sums
+};
+
+// Kernel-side launch description of one mobility product.
+struct ApplyPlan {
+  int tgt_begin = 0, tgt_end = 0;    // target range (combined ids)
+  bool src_seg = false, src_blob = false;
+  bool acc_seg = false, acc_blob = false;  // accumulate into v instead of overwrite
+};
+
+}  // namespace cfm
+
+struct cfm_ctx {
+  cfm_config cfg{};
+  int device = 0;
+  hipStream_t stream = nullptr;
+  cfm::Particles p;
+  bool have_positions = false;
+  int split = 1;                        // source split across workgroups
+  cfm::DevBuf<double> stage;            // AoS staging for host transfers
+  double* host_pinned = nullptr;        // pinned AoS staging on the host
+  size_t host_pinned_n = 0;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::string err;
+};

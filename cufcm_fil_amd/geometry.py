@@ -1,0 +1,126 @@
+"""Synthetic particle configurations of the benchmark workloads (seeded, numpy).
+
+All lengths are in sphere radii (a = 1); the no-slip wall is the plane z = 0.
+Segment positions are returned in the reference's AoS layout, filament after
+filament, base segment first (the ordering of x_segs_host,
+src/cilia/cilia_sim_main.cu:274-279).
+
+* ``tilted_lattice``  -- 64 filaments x 20 segments, 8x8 lattice (spacing 10),
+  bases at z = 1.1, straight chains of spacing 2.2 tilted by a uniform polar
+  angle <= 0.3 rad and a uniform azimuth (1,280 segments).
+* ``wave_lattice``    -- 640 filaments x 20 segments on a 32x20 lattice (spacing
+  8), planar bends of tip amplitude 0.2 L whose phase is 2 pi x / lambda with
+  lambda = 8 lattice spacings (12,800 segments).
+* ``sphere_over_wall`` -- a rigid sphere of radius 40 carrying 6,000 surface blobs
+  (Fibonacci lattice) and 1,300 radial filaments x 20 segments seeded on a second
+  Fibonacci lattice, bent by a metachronal phase field; the lowest point of the
+  sphere is 50 above the wall (about 32,000 particles).
+* ``ensemble``        -- independent wall systems with segment counts uniform in
+  [100, 4000] (multiples of 20), each a scaled ``tilted_lattice``.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+DL = 2.2     # segment spacing used by every workload
+NSEG = 20
+
+
+def _chain(base, tangents, dl=DL):
+    """Segment centres from a base point and per-segment unit tangents (robot arm,
+    src/cilia/filament.cpp:629-651)."""
+    x = np.empty((len(tangents), 3))
+    x[0] = base
+    for k in range(1, len(tangents)):
+        x[k] = x[k - 1] + 0.5 * dl * (tangents[k - 1] + tangents[k])
+    return x
+
+
+def loads(n, dof, seed):
+    """i.i.d. N(0,1) forces (and torques) in AoS layout."""
+    return np.random.default_rng(seed).standard_normal(dof * n)
+
+
+def tilted_lattice(nx=8, ny=8, spacing=10.0, nseg=NSEG, z0=1.1, max_polar=0.3, seed=0):
+    rng = np.random.default_rng(seed)
+    fils = []
+    for ix in range(nx):
+        for iy in range(ny):
+            th = rng.uniform(0.0, max_polar)
+            ph = rng.uniform(0.0, 2.0 * math.pi)
+            t = np.array([math.sin(th) * math.cos(ph), math.sin(th) * math.sin(ph), math.cos(th)])
+            base = np.array([ix * spacing, iy * spacing, z0])
+            fils.append(_chain(base, np.repeat(t[None], nseg, axis=0)))
+    return np.concatenate(fils).reshape(-1)
+
+
+def _bent_tangents(nseg, amp_frac, phase, axis_u, axis_n, dl=DL):
+    """Tangents of a planar bend in the (axis_u, axis_n) plane whose tip is
+    displaced by about amp_frac * L * cos(phase) along axis_n (L = nseg * dl)."""
+    L = nseg * dl
+    s = (np.arange(nseg) + 0.5) * dl
+    theta = 2.0 * amp_frac * (s / L) * math.cos(phase)
+    return np.cos(theta)[:, None] * axis_u[None] + np.sin(theta)[:, None] * axis_n[None]
+
+
+def wave_lattice(nx=32, ny=20, spacing=8.0, nseg=NSEG, z0=1.1, amp_frac=0.2, wavelength=None):
+    wavelength = 8.0 * spacing if wavelength is None else wavelength
+    ez, ex = np.array([0.0, 0.0, 1.0]), np.array([1.0, 0.0, 0.0])
+    fils = []
+    for ix in range(nx):
+        for iy in range(ny):
+            base = np.array([ix * spacing, iy * spacing, z0])
+            phase = 2.0 * math.pi * base[0] / wavelength
+            fils.append(_chain(base, _bent_tangents(nseg, amp_frac, phase, ez, ex)))
+    return np.concatenate(fils).reshape(-1)
+
+
+def fibonacci_sphere(n, radius=1.0, offset=0.0):
+    """n nearly uniform points on a sphere (golden-angle spiral)."""
+    k = np.arange(n) + 0.5
+    z = 1.0 - 2.0 * k / n
+    r = np.sqrt(1.0 - z * z)
+    phi = math.pi * (3.0 - math.sqrt(5.0)) * k + offset
+    return radius * np.stack([r * np.cos(phi), r * np.sin(phi), z], axis=1)
+
+
+def sphere_over_wall(n_blob=6000, n_fil=1300, nseg=NSEG, radius=40.0, gap=50.0,
+                     amp_frac=0.08, wavelength=None):
+    """Returns (x_seg AoS, x_blob AoS, centre, filament base directions)."""
+    centre = np.array([0.0, 0.0, gap + radius])
+    blobs = fibonacci_sphere(n_blob, radius) + centre
+    dirs = fibonacci_sphere(n_fil, 1.0, offset=0.5)
+    wavelength = 0.5 * math.pi * radius if wavelength is None else wavelength
+    fils = []
+    for d in dirs:
+        # beat plane spanned by the normal and the local polar direction
+        pol = np.array([0.0, 0.0, -1.0]) + d[2] * d   # tangential part of -z
+        nrm = np.linalg.norm(pol)
+        pol = pol / nrm if nrm > 1e-12 else np.array([1.0, 0.0, 0.0])
+        polar_angle = math.acos(max(-1.0, min(1.0, d[2])))
+        phase = 2.0 * math.pi * radius * polar_angle / wavelength   # metachronal wave
+        base = centre + (radius + DL) * d
+        fils.append(_chain(base, _bent_tangents(nseg, amp_frac, phase, d, pol)))
+    return np.concatenate(fils).reshape(-1), blobs.reshape(-1), centre, dirs
+
+
+def ensemble(n_sys=64, lo=100, hi=4000, seed=3):
+    """List of (n_seg, x_seg) for independent wall systems of uneven size."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for s in range(n_sys):
+        nfil = int(rng.integers(lo // NSEG, hi // NSEG + 1))
+        side = int(math.ceil(math.sqrt(nfil)))
+        x = tilted_lattice(nx=side, ny=side, seed=1000 + s).reshape(-1, NSEG, 3)[:nfil]
+        out.append((nfil * NSEG, x.reshape(-1)))
+    return out
+
+
+def min_separation(x):
+    """Smallest centre distance (brute force; small systems only)."""
+    p = np.asarray(x).reshape(-1, 3)
+    d = np.sqrt(((p[:, None, :] - p[None, :, :]) ** 2).sum(-1))
+    np.fill_diagonal(d, np.inf)
+    return d.min()

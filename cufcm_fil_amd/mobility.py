@@ -1,0 +1,116 @@
+"""Host-side mirror of the reference's mobility-solver plugin interface.
+
+``RpyMobilitySolver`` keeps the method names of the reference's abstract
+``mobility_solver`` / ``rpy_mobility_solver`` (src/cilia/mobility/mobility_solver.hpp:26-46,
+rpy_mobility_solver.cu:14-338) and holds the same host buffers (x/f/v of
+segments and blobs, reference AoS layout), while every product runs in the HIP
+library through the C ABI of include/cufcm_fil.h.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib as L
+
+
+class RpyMobilitySolver:
+    """Wall-corrected (or unbounded) RPY mobility of segments + blobs on one GPU."""
+
+    def __init__(self, n_seg: int, n_blob: int = 0, *, seg_dof: int = 6, wall: bool = True,
+                 a_seg: float = 1.0, a_blob: float = 1.0, mu: float = 1.0,
+                 precision: str = "fp64", device: int = 0, tile_split: int = 0):
+        lib = L.lib()
+        cfg = L.cfm_config(n_seg=n_seg, n_blob=n_blob, seg_dof=seg_dof, wall=int(wall),
+                           a_seg=a_seg, a_blob=a_blob, mu=mu,
+                           precision=self._prec(precision), device=device,
+                           tile_split=tile_split, reserved=0)
+        h = ctypes.c_void_p()
+        code = lib.cfm_create(ctypes.byref(cfg), ctypes.byref(h))
+        if code != L.CFM_OK:
+            raise L.CfmError(code, lib.cfm_last_error(None).decode())
+        self._h = h
+        self.n_seg, self.n_blob, self.seg_dof = n_seg, n_blob, seg_dof
+        self.wall, self.mu = wall, mu
+        # host buffers of the reference interface (mobility_solver.hpp:57-63)
+        self.x_segs_host = np.zeros(3 * n_seg)
+        self.x_blobs_host = np.zeros(3 * n_blob)
+        self.f_segs_host = np.zeros(seg_dof * n_seg)
+        self.f_blobs_host = np.zeros(3 * n_blob)
+        self.v_segs_host = np.zeros(seg_dof * n_seg)
+        self.v_blobs_host = np.zeros(3 * n_blob)
+
+    @staticmethod
+    def _prec(p):
+        if isinstance(p, int):
+            return p
+        return {"fp64": L.CFM_FP64, "fp32c": L.CFM_FP32C, "fp32": L.CFM_FP32C}[p.lower()]
+
+    # -- lifetime -----------------------------------------------------------
+    def close(self):
+        if getattr(self, "_h", None):
+            L.lib().cfm_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def handle(self):
+        return self._h
+
+    def set_precision(self, precision):
+        L.check(self._h, L.lib().cfm_set_precision(self._h, self._prec(precision)))
+
+    # -- transfers (copy_*_positions_to_device) -----------------------------
+    def set_positions(self, x_seg=None, x_blob=None):
+        if x_seg is not None:
+            self.x_segs_host[:] = L.f64(x_seg, 3 * self.n_seg)
+        if x_blob is not None:
+            self.x_blobs_host[:] = L.f64(x_blob, 3 * self.n_blob)
+        L.check(self._h, L.lib().cfm_set_positions(
+            self._h, L.ptr(self.x_segs_host), L.ptr(self.x_blobs_host)))
+
+    copy_segment_positions_to_device = set_positions
+
+    # -- products -------------------------------------------------------------
+    def apply(self, which: int, f_seg=None, f_blob=None):
+        if f_seg is not None:
+            self.f_segs_host[:] = L.f64(f_seg, self.seg_dof * self.n_seg)
+        if f_blob is not None:
+            self.f_blobs_host[:] = L.f64(f_blob, 3 * self.n_blob)
+        L.check(self._h, L.lib().cfm_apply(self._h, int(which), L.ptr(self.f_segs_host),
+                                           L.ptr(self.f_blobs_host), L.ptr(self.v_segs_host),
+                                           L.ptr(self.v_blobs_host)))
+        return self.v_segs_host, self.v_blobs_host
+
+    def evaluate_segment_segment_mobility(self, f_seg=None):
+        return self.apply(L.CFM_SS, f_seg=f_seg)[0].copy()
+
+    def evaluate_segment_blob_mobility(self, f_blob=None):
+        return self.apply(L.CFM_SB, f_blob=f_blob)[0].copy()
+
+    def evaluate_blob_blob_mobility(self, f_blob=None):
+        return self.apply(L.CFM_BB, f_blob=f_blob)[1].copy()
+
+    def evaluate_blob_segment_mobility(self, f_seg=None):
+        return self.apply(L.CFM_BS, f_seg=f_seg)[1].copy()
+
+    def evaluate_full_mobility(self, f_seg=None, f_blob=None):
+        v_s, v_b = self.apply(L.CFM_FULL, f_seg=f_seg, f_blob=f_blob)
+        return v_s.copy(), v_b.copy()
+
+    def time_apply(self, which: int, reps: int = 10) -> float:
+        ms = ctypes.c_double()
+        L.check(self._h, L.lib().cfm_time_apply(self._h, int(which), int(reps), ctypes.byref(ms)))
+        return ms.value

@@ -1,0 +1,99 @@
+/* cufcm_fil.h -- C ABI of the MI355X wall-RPY mobility library.
+ *
+ * One context = one set of particles (filament segments, then rigid-body blobs)
+ * on one HIP device with one HIP stream.  Every call is synchronous on return,
+ * returns 0 on success and a nonzero CFM_E* code on failure (the message is
+ * available from cfm_last_error).  Host arrays are caller-owned, double, in the
+ * reference's interleaved (AoS) layout; the library copies them and keeps no
+ * caller pointer.  A context must not be used from two threads at once;
+ * separate contexts are independent.
+ *
+ * Reference interface each entry point replaces (racksa/CUFCM_FIL):
+ *   mobility_solver (src/cilia/mobility/mobility_solver.hpp:26-46) pure virtuals,
+ *   implemented by rpy_mobility_solver (src/cilia/mobility/rpy_mobility_solver.cu).
+ */
+#ifndef CUFCM_FIL_H
+#define CUFCM_FIL_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CFM_ABI_VERSION 1
+
+/* status codes */
+#define CFM_OK 0
+#define CFM_EINVAL 1    /* bad argument or configuration */
+#define CFM_EHIP 2      /* HIP runtime error */
+#define CFM_ENOMEM 3    /* allocation failed */
+#define CFM_ESTATE 4    /* call out of order (e.g. apply before positions) */
+#define CFM_ENOCONV 5   /* iterative solver did not reach its tolerance */
+#define CFM_ENONFINITE 6 /* NaN/Inf in inputs or iterates */
+
+/* precision of the O(N^2) pair sums */
+#define CFM_FP64 0      /* FP64 pair arithmetic */
+#define CFM_FP32C 1     /* FP32 pair arithmetic, compensated (two-sum) accumulation */
+
+/* which block of the grand mobility to apply (cfm_apply) */
+#define CFM_SS 0   /* v_seg  = Mss f_seg   (evaluate_segment_segment_mobility, rpy_mobility_solver.cu:258-265) */
+#define CFM_SB 1   /* v_seg += Msb f_blob  (evaluate_segment_blob_mobility,    rpy_mobility_solver.cu:267-283) */
+#define CFM_BB 2   /* v_blob = Mbb f_blob  (evaluate_blob_blob_mobility,       rpy_mobility_solver.cu:285-301) */
+#define CFM_BS 3   /* v_blob = Mbs f_seg   (evaluate_blob_segment_mobility,    rpy_mobility_solver.cu:303-319) */
+#define CFM_FULL 4 /* both species from both species (evaluate_full_mobility, rpy_mobility_solver.cu:330-338) */
+
+typedef struct cfm_ctx cfm_ctx;
+
+typedef struct cfm_config {
+  int32_t n_seg;      /* filament segments (NSWIM*NFIL*NSEG in the reference) */
+  int32_t n_blob;     /* rigid-body blobs (NSWIM*NBLOB) */
+  int32_t seg_dof;    /* 6: force+torque per segment; 3: force only (PRESCRIBED_CILIA) */
+  int32_t wall;       /* 1: no-slip wall at z = 0 (INFINITE_PLANE_WALL), 0: unbounded */
+  double a_seg;       /* segment radius (RSEG); must be 1 with wall = 1 */
+  double a_blob;      /* blob radius (RBLOB); must be 1 with wall = 1 */
+  double mu;          /* viscosity (MU); velocities scale as 1/mu */
+  int32_t precision;  /* CFM_FP64 or CFM_FP32C */
+  int32_t device;     /* HIP device ordinal */
+  int32_t tile_split; /* source-range split across workgroups; 0 = choose automatically */
+  int32_t reserved;
+} cfm_config;
+
+/* Library ABI version (CFM_ABI_VERSION). */
+int cfm_abi_version(void);
+
+/* Create / destroy a context.  Replaces allocate_host_memory/allocate_device_memory
+ * and free_host_memory/free_device_memory (rpy_mobility_solver.cu:14-117). */
+int cfm_create(const cfm_config* cfg, cfm_ctx** out);
+int cfm_destroy(cfm_ctx* ctx);
+
+/* Message of the last failure on this context ("" if none).  ctx may be NULL
+ * for failures of cfm_create. */
+const char* cfm_last_error(const cfm_ctx* ctx);
+
+/* Upload positions: x_seg[3*n_seg], x_blob[3*n_blob] (either may be NULL when its
+ * count is 0).  Replaces copy_segment_positions_to_device / copy_blob_positions_to_device
+ * (rpy_mobility_solver.cu:119-148). */
+int cfm_set_positions(cfm_ctx* ctx, const double* x_seg, const double* x_blob);
+
+/* Switch the pair-sum precision of an existing context (CFM_FP64 / CFM_FP32C). */
+int cfm_set_precision(cfm_ctx* ctx, int32_t precision);
+
+/* Apply a mobility block: f_seg[seg_dof*n_seg], f_blob[3*n_blob] in,
+ * v_seg[seg_dof*n_seg], v_blob[3*n_blob] out, with the write/accumulate rule of
+ * the reference call named next to each CFM_* block id.  Replaces
+ * copy_*_forces_to_device + evaluate_*_mobility + copy_*_velocities_to_host +
+ * wait_for_device (rpy_mobility_solver.cu:150-338). */
+int cfm_apply(cfm_ctx* ctx, int32_t which, const double* f_seg, const double* f_blob,
+              double* v_seg, double* v_blob);
+
+/* Time `reps` launches of the pair kernel(s) for `which` on the context's stream
+ * with HIP events, using the forces most recently passed to cfm_apply.  Writes the
+ * mean milliseconds per launch.  (Measurement helper; no reference counterpart.) */
+int cfm_time_apply(cfm_ctx* ctx, int32_t which, int32_t reps, double* ms_per_apply);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CUFCM_FIL_H */

@@ -1,0 +1,52 @@
+"""CPU checks of the C-ABI library: it loads, exports every declared symbol and
+rejects bad configurations before touching a device."""
+import ctypes
+import os
+import subprocess
+
+import pytest
+
+from cufcm_fil_amd import _lib as L
+
+
+def test_library_exports_every_header_symbol():
+    syms = L.header_symbols()
+    assert len(syms) >= 8
+    lib = L.lib()
+    missing = [s for s in syms if not hasattr(lib, s)]
+    assert not missing, missing
+    out = subprocess.run(["nm", "-D", "--defined-only", L.LIB_PATH], capture_output=True,
+                         text=True, check=True).stdout
+    exported = {ln.split()[-1] for ln in out.splitlines() if " T " in ln}
+    assert set(syms) <= exported
+    assert lib.cfm_abi_version() == 1
+
+
+def _cfg(**kw):
+    base = dict(n_seg=40, n_blob=0, seg_dof=6, wall=1, a_seg=1.0, a_blob=1.0, mu=1.0,
+                precision=0, device=0, tile_split=0, reserved=0)
+    base.update(kw)
+    return L.cfm_config(**base)
+
+
+@pytest.mark.parametrize("bad", [dict(seg_dof=4), dict(n_seg=-1), dict(n_seg=0, n_blob=0),
+                                 dict(a_seg=2.0), dict(mu=0.0), dict(precision=7)])
+def test_create_rejects_bad_config(bad):
+    h = ctypes.c_void_p()
+    code = L.lib().cfm_create(ctypes.byref(_cfg(**bad)), ctypes.byref(h))
+    assert code == L.CFM_EINVAL
+    assert h.value is None
+    assert L.lib().cfm_last_error(None)
+
+
+@pytest.mark.skipif(os.path.exists("/dev/kfd"), reason="a GPU is present")
+def test_create_without_gpu_reports_hip_error():
+    h = ctypes.c_void_p()
+    code = L.lib().cfm_create(ctypes.byref(_cfg()), ctypes.byref(h))
+    assert code == L.CFM_EHIP
+    assert b"device" in L.lib().cfm_last_error(None).lower()
+
+
+def test_null_context_is_rejected():
+    assert L.lib().cfm_set_positions(None, None, None) == L.CFM_EINVAL
+    assert L.lib().cfm_destroy(None) == L.CFM_OK
