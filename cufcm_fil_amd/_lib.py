@@ -18,6 +18,9 @@ HEADER = os.path.join(os.path.dirname(HERE), "include", "cufcm_fil.h")
 CFM_OK, CFM_EINVAL, CFM_EHIP, CFM_ENOMEM, CFM_ESTATE, CFM_ENOCONV, CFM_ENONFINITE = range(7)
 CFM_FP64, CFM_FP32C = 0, 1
 CFM_SS, CFM_SB, CFM_BB, CFM_BS, CFM_FULL = range(5)
+CFM_SYS_SEG, CFM_SYS_BLOB = 0, 1
+CFM_PC_NONE, CFM_PC_DIAG, CFM_PC_SELF = 0, 1, 2
+CFM_SOLVE_FP64, CFM_SOLVE_MIXED = 0, 1
 
 
 class CfmError(RuntimeError):
@@ -40,6 +43,32 @@ class cfm_config(ctypes.Structure):
         ("tile_split", ctypes.c_int32),
         ("reserved", ctypes.c_int32),
     ]
+
+
+class cfm_solve_opts(ctypes.Structure):
+    _fields_ = [
+        ("system", ctypes.c_int32),
+        ("precond", ctypes.c_int32),
+        ("mode", ctypes.c_int32),
+        ("restart", ctypes.c_int32),
+        ("max_iter", ctypes.c_int32),
+        ("use_guess", ctypes.c_int32),
+        ("rtol", ctypes.c_double),
+    ]
+
+
+class cfm_solve_stats(ctypes.Structure):
+    _fields_ = [
+        ("iterations", ctypes.c_int32),
+        ("refinements", ctypes.c_int32),
+        ("converged", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("rel_residual", ctypes.c_double),
+        ("initial_rel_residual", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
 
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -73,6 +102,9 @@ def lib():
     L.cfm_set_precision.argtypes = [_vp, ctypes.c_int32]
     L.cfm_apply.argtypes = [_vp, ctypes.c_int32, _dp, _dp, _dp, _dp]
     L.cfm_time_apply.argtypes = [_vp, ctypes.c_int32, ctypes.c_int32, _dp]
+    L.cfm_gmres.argtypes = [_vp, ctypes.POINTER(cfm_solve_opts), _dp, _dp,
+                            ctypes.POINTER(cfm_solve_stats)]
+    L.cfm_apply_preconditioner.argtypes = [_vp, ctypes.c_int32, ctypes.c_int32, _dp, _dp]
     _lib = L
     return L
 

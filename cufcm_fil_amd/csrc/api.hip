@@ -11,6 +11,7 @@
 #include <new>
 
 #include "ctx.hpp"
+#include "krylov.hpp"
 
 namespace cfm {
 
@@ -34,7 +35,7 @@ __global__ void soa_to_aos_kernel(const double* __restrict__ src, double* __rest
 
 static Status ensure_pinned(cfm_ctx* c, size_t n) {
   if (n <= c->host_pinned_n && c->host_pinned) return {};
-  if (c->host_pinned) hipHostFree(c->host_pinned);
+  if (c->host_pinned) (void)hipHostFree(c->host_pinned);
   c->host_pinned = nullptr;
   c->host_pinned_n = 0;
   CFM_HIP(hipHostMalloc((void**)&c->host_pinned, n * sizeof(double), hipHostMallocDefault));
@@ -185,8 +186,8 @@ int cfm_create(const cfm_config* cfg, cfm_ctx** out) {
 
 int cfm_destroy(cfm_ctx* c) {
   if (!c) return CFM_OK;
-  hipSetDevice(c->device);
-  if (c->stream) hipStreamSynchronize(c->stream);
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
   c->p.x.release();
   c->p.y.release();
   c->p.z.release();
@@ -194,10 +195,16 @@ int cfm_destroy(cfm_ctx* c) {
   c->p.v.release();
   c->p.partial.release();
   c->stage.release();
-  if (c->host_pinned) hipHostFree(c->host_pinned);
-  if (c->ev0) hipEventDestroy(c->ev0);
-  if (c->ev1) hipEventDestroy(c->ev1);
-  if (c->stream) hipStreamDestroy(c->stream);
+  c->vec_a.release();
+  c->vec_b.release();
+  if (c->krylov) {
+    c->krylov->release();
+    delete c->krylov;
+  }
+  if (c->host_pinned) (void)hipHostFree(c->host_pinned);
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return CFM_OK;
 }
@@ -220,7 +227,7 @@ int cfm_set_positions(cfm_ctx* c, const double* x_seg, const double* x_blob) {
         return fail(c, make_status(CFM_EINVAL, "blob %d is not above the wall (z = %g)", i,
                                    x_blob[3 * i + 2]));
   }
-  hipSetDevice(c->device);
+  (void)hipSetDevice(c->device);
   // The three coordinate arrays are allocated separately; stage them as rows of
   // a 3 x n block, then scatter.
   auto up = [&](const double* h, int cnt, int off) -> Status {
@@ -266,7 +273,7 @@ int cfm_apply(cfm_ctx* c, int32_t which, const double* f_seg, const double* f_bl
   if ((pl.src_seg && p.n_seg && !all_finite(f_seg, (size_t)sd * p.n_seg)) ||
       (pl.src_blob && p.n_blob && !all_finite(f_blob, 3 * (size_t)p.n_blob)))
     return fail(c, make_status(CFM_ENONFINITE, "non-finite load"));
-  hipSetDevice(c->device);
+  (void)hipSetDevice(c->device);
   auto run = [&]() -> Status {
     if (pl.src_seg) CFM_TRY(upload_aos(c, f_seg, p.n_seg, sd, p.f.p, p.n, 0));
     if (pl.src_blob) CFM_TRY(upload_aos(c, f_blob, p.n_blob, 3, p.f.p, p.n, p.n_seg));
@@ -285,7 +292,7 @@ int cfm_apply(cfm_ctx* c, int32_t which, const double* f_seg, const double* f_bl
 int cfm_time_apply(cfm_ctx* c, int32_t which, int32_t reps, double* ms_per_apply) {
   if (!c || !ms_per_apply || reps <= 0) return CFM_EINVAL;
   if (!c->have_positions) return fail(c, make_status(CFM_ESTATE, "positions not set"));
-  hipSetDevice(c->device);
+  (void)hipSetDevice(c->device);
   const ApplyPlan pl = plan_for(c, which);
   auto run = [&]() -> Status {
     c->split = choose_split(c, pl);
@@ -301,6 +308,64 @@ int cfm_time_apply(cfm_ctx* c, int32_t which, int32_t reps, double* ms_per_apply
   };
   Status s = run();
   return s.ok() ? CFM_OK : fail(c, s);
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int cfm_gmres(cfm_ctx* c, const cfm_solve_opts* o, const double* rhs, double* x,
+              cfm_solve_stats* stats) {
+  if (!c || !o || !rhs || !x) return c ? fail(c, make_status(CFM_EINVAL, "null argument")) : CFM_EINVAL;
+  if (o->system != CFM_SYS_SEG && o->system != CFM_SYS_BLOB)
+    return fail(c, make_status(CFM_EINVAL, "unknown system %d", o->system));
+  if (o->precond < CFM_PC_NONE || o->precond > CFM_PC_SELF)
+    return fail(c, make_status(CFM_EINVAL, "unknown preconditioner %d", o->precond));
+  if (o->mode != CFM_SOLVE_FP64 && o->mode != CFM_SOLVE_MIXED)
+    return fail(c, make_status(CFM_EINVAL, "unknown solve mode %d", o->mode));
+  if (!(o->rtol > 0.0) || o->restart < 0 || o->max_iter < 0)
+    return fail(c, make_status(CFM_EINVAL, "need rtol > 0, restart >= 0, max_iter >= 0"));
+  if (!c->have_positions) return fail(c, make_status(CFM_ESTATE, "positions not set"));
+  const SystemDesc s = system_desc(c, o->system);
+  if (s.m == 0) return fail(c, make_status(CFM_EINVAL, "empty system"));
+  if (!all_finite(rhs, s.L) || (o->use_guess && !all_finite(x, s.L)))
+    return fail(c, make_status(CFM_ENONFINITE, "non-finite rhs or guess"));
+  (void)hipSetDevice(c->device);
+  if (!c->krylov) c->krylov = new Krylov();
+  auto run = [&]() -> Status {
+    CFM_TRY(c->vec_a.alloc(s.L));
+    CFM_TRY(c->vec_b.alloc(s.L));
+    CFM_TRY(upload_aos(c, rhs, s.m, s.dof, c->vec_a.p, s.m, 0));
+    CFM_TRY(upload_aos(c, x, s.m, s.dof, c->vec_b.p, s.m, 0));
+    Status st = solve(c, *c->krylov, s, *o, c->vec_a.p, c->vec_b.p, stats, nullptr);
+    if (st.code != CFM_OK && st.code != CFM_ENOCONV) return st;
+    CFM_TRY(download_aos(c, x, s.m, s.dof, c->vec_b.p, s.m, 0));
+    return st;
+  };
+  Status st = run();
+  return st.ok() ? CFM_OK : fail(c, st);
+}
+
+int cfm_apply_preconditioner(cfm_ctx* c, int32_t system, int32_t precond, const double* in,
+                             double* out) {
+  if (!c || !in || !out) return c ? fail(c, make_status(CFM_EINVAL, "null argument")) : CFM_EINVAL;
+  if (system != CFM_SYS_SEG && system != CFM_SYS_BLOB)
+    return fail(c, make_status(CFM_EINVAL, "unknown system %d", system));
+  if (precond < CFM_PC_NONE || precond > CFM_PC_SELF)
+    return fail(c, make_status(CFM_EINVAL, "unknown preconditioner %d", precond));
+  if (!c->have_positions) return fail(c, make_status(CFM_ESTATE, "positions not set"));
+  const SystemDesc s = system_desc(c, system);
+  (void)hipSetDevice(c->device);
+  auto run = [&]() -> Status {
+    CFM_TRY(c->vec_a.alloc(s.L));
+    CFM_TRY(c->vec_b.alloc(s.L));
+    CFM_TRY(upload_aos(c, in, s.m, s.dof, c->vec_a.p, s.m, 0));
+    CFM_TRY(system_precond(c, s, precond, c->vec_a.p, c->vec_b.p));
+    CFM_TRY(download_aos(c, out, s.m, s.dof, c->vec_b.p, s.m, 0));
+    return {};
+  };
+  Status st = run();
+  return st.ok() ? CFM_OK : fail(c, st);
 }
 
 }  // extern "C"

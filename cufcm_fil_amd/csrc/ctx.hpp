@@ -53,7 +53,7 @@ struct DevBuf {
   size_t n = 0;
   Status alloc(size_t count) {
     if (count <= n && p) return {};
-    if (p) hipFree(p);
+    if (p) (void)hipFree(p);
     p = nullptr;
     n = 0;
     if (count == 0) return {};
@@ -65,7 +65,7 @@ struct DevBuf {
     return {};
   }
   void release() {
-    if (p) hipFree(p);
+    if (p) (void)hipFree(p);
     p = nullptr;
     n = 0;
   }
@@ -91,6 +91,8 @@ struct ApplyPlan {
 
 }  // namespace cfm
 
+namespace cfm { struct Krylov; }
+
 struct cfm_ctx {
   cfm_config cfg{};
   int device = 0;
@@ -102,5 +104,7 @@ struct cfm_ctx {
   double* host_pinned = nullptr;        // pinned AoS staging on the host
   size_t host_pinned_n = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  cfm::Krylov* krylov = nullptr;        // GMRES workspace (created on first solve)
+  cfm::DevBuf<double> vec_a, vec_b;     // system-vector staging
   std::string err;
 };

@@ -1,0 +1,476 @@
+// gmres.hip -- right-preconditioned restarted GMRES on the device, FP64 Krylov
+// arithmetic, with FP64 iterative refinement around an FP32C inner solve.
+//
+// Restates the algorithm of mobility_solver::solve_linear_system
+// (src/cilia/mobility/mobility_solver.cpp:2187-2669): right preconditioning
+// (config.hpp:377), initial guess rejected when its residual exceeds ||rhs||
+// (:2406-2417), Givens rotations in the reference's form (:2467-2506), back
+// substitution and x += P^-1 Q y (:2520-2559).  Differences: Krylov cycles are
+// restarted after `restart` vectors (the reference keeps min(n, 350) vectors),
+// orthogonalisation is classical Gram-Schmidt applied twice (two block products
+// per step instead of j sequential dot/axpy pairs), and every reduction runs on
+// the GPU in a fixed order.  The residual estimate |beta_{j+1}| is the only value
+// read back per step (to decide termination).
+//
+// Vectors are structure-of-arrays [dof][m] over the particles of the system.
+#include <algorithm>
+#include <cmath>
+
+#include "ctx.hpp"
+#include "krylov.hpp"
+
+namespace cfm {
+
+Status launch_apply(cfm_ctx* c, const ApplyPlan& plan);
+int choose_split(const cfm_ctx* c, const ApplyPlan& pl);
+ApplyPlan plan_for(const cfm_ctx* c, int which);
+
+constexpr int kRB = 256;  // threads per reduction block
+constexpr int kChunk = kRB * 8;
+
+// partial[b*k + i] = sum over chunk b of Q[i][r] * w[r]
+__global__ __launch_bounds__(kRB) void dots_partial_kernel(const double* __restrict__ Q,
+                                                           size_t ldq, int k,
+                                                           const double* __restrict__ w, int L,
+                                                           double* __restrict__ partial) {
+  __shared__ double red[kRB];
+  const int i = blockIdx.x;
+  const int b = blockIdx.y;
+  const double* q = Q + (size_t)i * ldq;
+  const int r0 = b * kChunk;
+  const int r1 = min(L, r0 + kChunk);
+  double s = 0.0;
+  for (int r = r0 + threadIdx.x; r < r1; r += kRB) s += q[r] * w[r];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int off = kRB / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[(size_t)b * k + i] = red[0];
+}
+
+// out[i] (+)= sum_b partial[b*k + i], fixed order
+__global__ void dots_finish_kernel(const double* __restrict__ partial, int nb, int k,
+                                   double* __restrict__ out, int accumulate) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= k) return;
+  double s = 0.0;
+  for (int b = 0; b < nb; ++b) s += partial[(size_t)b * k + i];
+  out[i] = accumulate ? out[i] + s : s;
+}
+
+// w[r] -= sum_{i<k} h[i] Q[i][r]
+__global__ void gemv_sub_kernel(const double* __restrict__ Q, size_t ldq, int k,
+                                const double* __restrict__ h, double* __restrict__ w, int L) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= L) return;
+  double s = 0.0;
+  for (int i = 0; i < k; ++i) s += h[i] * Q[(size_t)i * ldq + r];
+  w[r] -= s;
+}
+
+// out[r] = sum_{i<k} y[i] Q[i][r]
+__global__ void gemv_kernel(const double* __restrict__ Q, size_t ldq, int k,
+                            const double* __restrict__ y, double* __restrict__ out, int L) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= L) return;
+  double s = 0.0;
+  for (int i = 0; i < k; ++i) s += y[i] * Q[(size_t)i * ldq + r];
+  out[r] = s;
+}
+
+__global__ void axpby_kernel(double a, const double* __restrict__ x, double b,
+                             const double* __restrict__ y, double* __restrict__ out, int L) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < L) out[r] = a * x[r] + b * y[r];
+}
+
+// out = x / s[0] (s on device)
+__global__ void scale_by_inv_kernel(const double* __restrict__ x, const double* __restrict__ s,
+                                    double* __restrict__ out, int L) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  const double d = s[0];
+  if (r < L) out[r] = d != 0.0 ? x[r] / d : 0.0;
+}
+
+__global__ void fill_kernel(double* __restrict__ x, double v, int L) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < L) x[r] = v;
+}
+
+__global__ void sqrt_kernel(double* x) { x[0] = sqrt(fmax(x[0], 0.0)); }
+
+// Copies `dof` rows of length m between two row-major blocks.
+__global__ void copy_rows_kernel(const double* __restrict__ src, size_t src_ld, int src_off,
+                                 double* __restrict__ dst, size_t dst_ld, int dst_off, int dof,
+                                 int m) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  for (int k = 0; k < dof; ++k) dst[(size_t)k * dst_ld + dst_off + i] = src[(size_t)k * src_ld + src_off + i];
+}
+
+// One Givens step on column j of H (ld = m+1 rows, column-major), the reference's
+// formulas (mobility_solver.cpp:2467-2506); writes |beta_{j+1}| to res.
+__global__ void givens_kernel(double* __restrict__ H, int ldh, double* __restrict__ cs,
+                              double* __restrict__ sn, double* __restrict__ beta,
+                              const double* __restrict__ hcol, const double* __restrict__ hnorm,
+                              int j, double* __restrict__ res) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double* col = H + (size_t)j * ldh;
+  for (int i = 0; i <= j; ++i) col[i] = hcol[i];
+  col[j + 1] = hnorm[0];
+  for (int i = 1; i <= j; ++i) {
+    const double t = cs[i - 1] * col[i - 1] + sn[i - 1] * col[i];
+    col[i] = cs[i - 1] * col[i] - sn[i - 1] * col[i - 1];
+    col[i - 1] = t;
+  }
+  const double a = col[j], b = col[j + 1];
+  double c, s;
+  if (b == 0.0) {
+    c = 1.0;
+    s = 0.0;
+  } else if (fabs(b) > fabs(a)) {
+    const double t = a / b;
+    s = ((b > 0.0) ? 1.0 : -1.0) / sqrt(1.0 + t * t);
+    c = t * s;
+  } else {
+    const double t = b / a;
+    c = ((a > 0.0) ? 1.0 : -1.0) / sqrt(1.0 + t * t);
+    s = t * c;
+  }
+  cs[j] = c;
+  sn[j] = s;
+  col[j] = c * a + s * b;
+  col[j + 1] = 0.0;
+  beta[j + 1] = -s * beta[j];
+  beta[j] = c * beta[j];
+  res[0] = fabs(beta[j + 1]);
+}
+
+__global__ void backsub_kernel(const double* __restrict__ H, int ldh,
+                               const double* __restrict__ beta, int k, double* __restrict__ y) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  for (int i = k - 1; i >= 0; --i) {
+    double s = beta[i];
+    for (int jj = i + 1; jj < k; ++jj) s -= H[(size_t)jj * ldh + i] * y[jj];
+    const double d = H[(size_t)i * ldh + i];
+    y[i] = d != 0.0 ? s / d : 0.0;
+  }
+}
+
+__global__ void beta_init_kernel(double* __restrict__ beta, int n, const double* __restrict__ b0) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) beta[i] = (i == 0) ? b0[0] : 0.0;
+}
+
+// ---- preconditioners --------------------------------------------------------
+// y = P^-1 x over the particles [off, off+m) of the combined index space.
+__global__ void precond_kernel(int kind, int wall, int dof, int m, int off,
+                               const double* __restrict__ z, double a, double mu,
+                               const double* __restrict__ x, double* __restrict__ y) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  double v[6] = {0, 0, 0, 0, 0, 0};
+  for (int k = 0; k < dof; ++k) v[k] = x[(size_t)k * m + i];
+  double o[6] = {0, 0, 0, 0, 0, 0};
+  const double pi = 3.14159265358979323846;
+  if (kind == CFM_PC_NONE) {
+    for (int k = 0; k < dof; ++k) o[k] = v[k];
+  } else if (kind == CFM_PC_DIAG || !wall) {
+    const double ct = 6.0 * pi * mu * a, cr = 8.0 * pi * mu * a * a * a;
+    for (int k = 0; k < dof; ++k) o[k] = (k < 3 ? ct : cr) * v[k];
+  } else {
+    // inverse of the wall self-mobility block (a = 1): couples (F_x, T_y) and (F_y, T_x)
+    const double c6 = 1.0 / (6.0 * pi), c8 = 1.0 / (8.0 * pi);
+    const double h1 = 1.0 / z[off + i];
+    const double h3 = h1 * h1 * h1, h4 = h3 * h1, h5 = h4 * h1;
+    const double par = c6 * (1.0 - (9.0 * h1 - 2.0 * h3 + h5) * 0.0625) / mu;
+    const double perp = c6 * (1.0 - (9.0 * h1 - 4.0 * h3 + h5) * 0.125) / mu;
+    o[2] = v[2] / perp;
+    if (dof == 3) {
+      o[0] = v[0] / par;
+      o[1] = v[1] / par;
+    } else {
+      const double cpl = c6 * 0.09375 * h4 / mu;
+      const double rpar = c8 * (1.0 - 0.3125 * h3) / mu;
+      const double rperp = c8 * (1.0 - 0.125 * h3) / mu;
+      const double det = par * rpar - cpl * cpl;
+      // [vx, wy] = [[par, cpl], [cpl, rpar]] [Fx, Ty]
+      o[0] = (rpar * v[0] - cpl * v[4]) / det;
+      o[4] = (par * v[4] - cpl * v[0]) / det;
+      // [vy, wx] = [[par, -cpl], [-cpl, rpar]] [Fy, Tx]
+      o[1] = (rpar * v[1] + cpl * v[3]) / det;
+      o[3] = (par * v[3] + cpl * v[1]) / det;
+      o[5] = v[5] / rperp;
+    }
+  }
+  for (int k = 0; k < dof; ++k) y[(size_t)k * m + i] = o[k];
+}
+
+static inline dim3 g1(int L) { return dim3((L + 255) / 256); }
+
+// ---------------------------------------------------------------------------
+Status Krylov::reserve(int L_, int m_) {
+  if (L_ <= L && m_ <= m && Q.p) return {};
+  L = std::max(L, L_);
+  m = std::max(m, m_);
+  CFM_TRY(Q.alloc((size_t)(m + 1) * L));
+  CFM_TRY(H.alloc((size_t)(m + 1) * (m + 1)));
+  CFM_TRY(cs.alloc(m + 1));
+  CFM_TRY(sn.alloc(m + 1));
+  CFM_TRY(beta.alloc(m + 2));
+  CFM_TRY(h.alloc(m + 2));
+  CFM_TRY(h2.alloc(m + 2));
+  CFM_TRY(y.alloc(m + 2));
+  CFM_TRY(scal.alloc(8));
+  CFM_TRY(r.alloc(L));
+  CFM_TRY(w.alloc(L));
+  CFM_TRY(z.alloc(L));
+  CFM_TRY(t.alloc(L));
+  CFM_TRY(d.alloc(L));
+  const int nb = (L + kChunk - 1) / kChunk;
+  CFM_TRY(partial.alloc((size_t)nb * (m + 2)));
+  if (!host_res) CFM_HIP(hipHostMalloc((void**)&host_res, sizeof(double) * 4, hipHostMallocDefault));
+  return {};
+}
+
+void Krylov::release() {
+  Q.release(); H.release(); cs.release(); sn.release(); beta.release(); h.release();
+  h2.release(); y.release(); scal.release(); r.release(); w.release(); z.release();
+  t.release(); d.release(); partial.release();
+  if (host_res) (void)hipHostFree(host_res);
+  host_res = nullptr;
+  L = m = 0;
+}
+
+// out[0..k) = Q[0..k)^T w
+Status dots(cfm_ctx* c, Krylov& K, const double* Q, size_t ldq, int k, const double* w, int L,
+            double* out, bool accumulate) {
+  const int nb = (L + kChunk - 1) / kChunk;
+  hipLaunchKernelGGL(dots_partial_kernel, dim3(k, nb), dim3(kRB), 0, c->stream, Q, ldq, k, w, L,
+                     K.partial.p);
+  CFM_CHECK_LAUNCH();
+  hipLaunchKernelGGL(dots_finish_kernel, dim3((k + 63) / 64), dim3(64), 0, c->stream,
+                     K.partial.p, nb, k, out, accumulate ? 1 : 0);
+  CFM_CHECK_LAUNCH();
+  return {};
+}
+
+Status norm2(cfm_ctx* c, Krylov& K, const double* w, int L, double* out) {
+  CFM_TRY(dots(c, K, w, 0, 1, w, L, out, false));
+  hipLaunchKernelGGL(sqrt_kernel, dim3(1), dim3(1), 0, c->stream, out);
+  CFM_CHECK_LAUNCH();
+  return {};
+}
+
+Status read_scalar(cfm_ctx* c, Krylov& K, const double* dev, double* host) {
+  CFM_HIP(hipMemcpyAsync(K.host_res, dev, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  CFM_HIP(hipStreamSynchronize(c->stream));
+  *host = K.host_res[0];
+  return {};
+}
+
+SystemDesc system_desc(const cfm_ctx* c, int system) {
+  SystemDesc s;
+  if (system == CFM_SYS_SEG) {
+    s.m = c->p.n_seg;
+    s.dof = c->p.seg_dof;
+    s.off = 0;
+    s.which = CFM_SS;
+    s.a = c->cfg.a_seg;
+  } else {
+    s.m = c->p.n_blob;
+    s.dof = 3;
+    s.off = c->p.n_seg;
+    s.which = CFM_BB;
+    s.a = c->cfg.a_blob;
+  }
+  s.L = s.m * s.dof;
+  return s;
+}
+
+// y = A x for a system vector (layout [dof][m]); precision CFM_FP64 / CFM_FP32C
+Status system_apply(cfm_ctx* c, const SystemDesc& s, int precision, const double* x, double* y) {
+  Particles& p = c->p;
+  hipLaunchKernelGGL(copy_rows_kernel, g1(s.m), dim3(256), 0, c->stream, x, (size_t)s.m, 0, p.f.p,
+                     (size_t)p.n, s.off, s.dof, s.m);
+  CFM_CHECK_LAUNCH();
+  const int saved = c->cfg.precision;
+  c->cfg.precision = precision;
+  const ApplyPlan pl = plan_for(c, s.which);
+  c->split = choose_split(c, pl);
+  Status st = launch_apply(c, pl);
+  c->cfg.precision = saved;
+  CFM_TRY(st);
+  hipLaunchKernelGGL(copy_rows_kernel, g1(s.m), dim3(256), 0, c->stream, p.v.p, (size_t)p.n, s.off,
+                     y, (size_t)s.m, 0, s.dof, s.m);
+  CFM_CHECK_LAUNCH();
+  return {};
+}
+
+Status system_precond(cfm_ctx* c, const SystemDesc& s, int kind, const double* x, double* y) {
+  hipLaunchKernelGGL(precond_kernel, g1(s.m), dim3(256), 0, c->stream, kind, c->cfg.wall, s.dof,
+                     s.m, s.off, c->p.z.p, s.a, c->cfg.mu, x, y);
+  CFM_CHECK_LAUNCH();
+  return {};
+}
+
+// Restarted GMRES on A (pair precision `prec`) for b (device), x in/out (device).
+// Stops when the residual estimate reaches `target_abs`, after max_iter steps in
+// total, or after `max_cycles` cycles.  On return K.r holds b - A x (operator
+// `prec`) and *res_abs its norm.
+Status gmres_core(cfm_ctx* c, Krylov& K, const SystemDesc& s, int prec, int pc, const double* b,
+                  double* x, double target_abs, int restart, int max_iter, int* iters,
+                  double* res_abs, const GmresHooks* hooks) {
+  const int L = s.L;
+  const int m = restart;
+  CFM_TRY(K.reserve(L, m));
+  const size_t ldq = (size_t)K.L;
+  const int ldh = m + 1;
+  double nr = 0.0;
+  // r = b - A x
+  CFM_TRY(system_apply(c, s, prec, x, K.w.p));
+  hipLaunchKernelGGL(axpby_kernel, g1(L), dim3(256), 0, c->stream, 1.0, b, -1.0, K.w.p, K.r.p, L);
+  CFM_CHECK_LAUNCH();
+  CFM_TRY(norm2(c, K, K.r.p, L, K.scal.p));
+  CFM_TRY(read_scalar(c, K, K.scal.p, &nr));
+  if (!std::isfinite(nr)) return make_status(CFM_ENONFINITE, "non-finite GMRES residual");
+  while (nr > target_abs && *iters < max_iter) {
+    // q0 = r / ||r||
+    hipLaunchKernelGGL(scale_by_inv_kernel, g1(L), dim3(256), 0, c->stream, K.r.p, K.scal.p, K.Q.p, L);
+    hipLaunchKernelGGL(beta_init_kernel, dim3(1), dim3(64), 0, c->stream, K.beta.p, m + 1, K.scal.p);
+    CFM_CHECK_LAUNCH();
+    int k = 0;
+    double est = nr;
+    for (int j = 0; j < m; ++j) {
+      double* qj = K.Q.p + (size_t)j * ldq;
+      double* qn = K.Q.p + (size_t)(j + 1) * ldq;
+      CFM_TRY(system_precond(c, s, pc, qj, K.z.p));
+      if (hooks && hooks->precond_correction) CFM_TRY(hooks->precond_correction(c, s, qj, K.z.p, hooks->user));
+      CFM_TRY(system_apply(c, s, prec, K.z.p, K.w.p));
+      // classical Gram-Schmidt, twice
+      CFM_TRY(dots(c, K, K.Q.p, ldq, j + 1, K.w.p, L, K.h.p, false));
+      hipLaunchKernelGGL(gemv_sub_kernel, g1(L), dim3(256), 0, c->stream, K.Q.p, ldq, j + 1, K.h.p, K.w.p, L);
+      CFM_CHECK_LAUNCH();
+      CFM_TRY(dots(c, K, K.Q.p, ldq, j + 1, K.w.p, L, K.h2.p, false));
+      hipLaunchKernelGGL(gemv_sub_kernel, g1(L), dim3(256), 0, c->stream, K.Q.p, ldq, j + 1, K.h2.p, K.w.p, L);
+      hipLaunchKernelGGL(axpby_kernel, dim3(1), dim3(256), 0, c->stream, 1.0, K.h.p, 1.0, K.h2.p, K.h.p, j + 1);
+      CFM_CHECK_LAUNCH();
+      CFM_TRY(norm2(c, K, K.w.p, L, K.scal.p + 1));
+      hipLaunchKernelGGL(scale_by_inv_kernel, g1(L), dim3(256), 0, c->stream, K.w.p, K.scal.p + 1, qn, L);
+      hipLaunchKernelGGL(givens_kernel, dim3(1), dim3(1), 0, c->stream, K.H.p, ldh, K.cs.p, K.sn.p,
+                         K.beta.p, K.h.p, K.scal.p + 1, j, K.scal.p + 2);
+      CFM_CHECK_LAUNCH();
+      CFM_TRY(read_scalar(c, K, K.scal.p + 2, &est));
+      ++*iters;
+      k = j + 1;
+      if (!std::isfinite(est)) return make_status(CFM_ENONFINITE, "non-finite GMRES iterate");
+      if (est <= target_abs || *iters >= max_iter) break;
+    }
+    // x += P^-1 Q y
+    hipLaunchKernelGGL(backsub_kernel, dim3(1), dim3(1), 0, c->stream, K.H.p, ldh, K.beta.p, k, K.y.p);
+    hipLaunchKernelGGL(gemv_kernel, g1(L), dim3(256), 0, c->stream, K.Q.p, ldq, k, K.y.p, K.t.p, L);
+    CFM_CHECK_LAUNCH();
+    CFM_TRY(system_precond(c, s, pc, K.t.p, K.z.p));
+    if (hooks && hooks->precond_correction) CFM_TRY(hooks->precond_correction(c, s, K.t.p, K.z.p, hooks->user));
+    hipLaunchKernelGGL(axpby_kernel, g1(L), dim3(256), 0, c->stream, 1.0, x, 1.0, K.z.p, x, L);
+    CFM_CHECK_LAUNCH();
+    // true residual with this operator
+    CFM_TRY(system_apply(c, s, prec, x, K.w.p));
+    hipLaunchKernelGGL(axpby_kernel, g1(L), dim3(256), 0, c->stream, 1.0, b, -1.0, K.w.p, K.r.p, L);
+    CFM_CHECK_LAUNCH();
+    CFM_TRY(norm2(c, K, K.r.p, L, K.scal.p));
+    CFM_TRY(read_scalar(c, K, K.scal.p, &nr));
+    if (!std::isfinite(nr)) return make_status(CFM_ENONFINITE, "non-finite GMRES residual");
+  }
+  *res_abs = nr;
+  return {};
+}
+
+// Full solve with the reference's initial-guess rule and optional mixed precision.
+Status solve(cfm_ctx* c, Krylov& K, const SystemDesc& s, const cfm_solve_opts& o, const double* b,
+             double* x, cfm_solve_stats* st, const GmresHooks* hooks) {
+  const int L = s.L;
+  const int restart = o.restart > 0 ? o.restart : 50;
+  const int max_iter = o.max_iter > 0 ? o.max_iter : 1000;
+  CFM_TRY(K.reserve(L, restart));
+  cfm_solve_stats S{};
+  double nb = 0.0;
+  CFM_TRY(norm2(c, K, b, L, K.scal.p + 3));
+  CFM_TRY(read_scalar(c, K, K.scal.p + 3, &nb));
+  if (!std::isfinite(nb)) return make_status(CFM_ENONFINITE, "non-finite right-hand side");
+  if (nb == 0.0) {  // zero rhs: zero solution (mobility_solver.cpp:2221-2294)
+    hipLaunchKernelGGL(fill_kernel, g1(L), dim3(256), 0, c->stream, x, 0.0, L);
+    CFM_CHECK_LAUNCH();
+    S.converged = 1;
+    if (st) *st = S;
+    return {};
+  }
+  if (!o.use_guess) {
+    hipLaunchKernelGGL(fill_kernel, g1(L), dim3(256), 0, c->stream, x, 0.0, L);
+    CFM_CHECK_LAUNCH();
+  }
+  const double target = o.rtol * nb;
+  // FP64 residual of the guess; reject a guess worse than zero
+  double nr = 0.0;
+  CFM_TRY(system_apply(c, s, CFM_FP64, x, K.w.p));
+  hipLaunchKernelGGL(axpby_kernel, g1(L), dim3(256), 0, c->stream, 1.0, b, -1.0, K.w.p, K.d.p, L);
+  CFM_CHECK_LAUNCH();
+  CFM_TRY(norm2(c, K, K.d.p, L, K.scal.p));
+  CFM_TRY(read_scalar(c, K, K.scal.p, &nr));
+  if (!std::isfinite(nr)) return make_status(CFM_ENONFINITE, "non-finite residual of the guess");
+  if (nr > nb) {
+    hipLaunchKernelGGL(fill_kernel, g1(L), dim3(256), 0, c->stream, x, 0.0, L);
+    hipLaunchKernelGGL(axpby_kernel, g1(L), dim3(256), 0, c->stream, 1.0, b, 0.0, b, K.d.p, L);
+    CFM_CHECK_LAUNCH();
+    nr = nb;
+  }
+  S.initial_rel_residual = nr / nb;
+  int iters = 0;
+  if (o.mode == CFM_SOLVE_FP64) {
+    double res = nr;
+    if (nr > target) CFM_TRY(gmres_core(c, K, s, CFM_FP64, o.precond, b, x, target, restart,
+                                        max_iter, &iters, &res, hooks));
+    nr = res;
+  } else {
+    // iterative refinement: d solves A32 d = r to a relative 2e-6 (or what is left
+    // to reach the target), x += d, r = b - A64 x.
+    int refinements = 0;
+    while (nr > target && iters < max_iter && refinements < 12) {
+      const double rel_in = std::max(2e-6, 0.5 * target / nr);
+      hipLaunchKernelGGL(fill_kernel, g1(L), dim3(256), 0, c->stream, K.t.p, 0.0, L);
+      CFM_CHECK_LAUNCH();
+      // K.d holds r; the inner solve overwrites K.r/K.w/K.z, so keep r in K.d
+      double inner_res = 0.0;
+      DevBuf<double>& dvec = K.dx;
+      CFM_TRY(dvec.alloc(L));
+      hipLaunchKernelGGL(fill_kernel, g1(L), dim3(256), 0, c->stream, dvec.p, 0.0, L);
+      CFM_CHECK_LAUNCH();
+      CFM_TRY(K.rhs.alloc(L));
+      CFM_HIP(hipMemcpyAsync(K.rhs.p, K.d.p, sizeof(double) * L, hipMemcpyDeviceToDevice, c->stream));
+      CFM_TRY(gmres_core(c, K, s, CFM_FP32C, o.precond, K.rhs.p, dvec.p, rel_in * nr, restart,
+                         max_iter, &iters, &inner_res, hooks));
+      hipLaunchKernelGGL(axpby_kernel, g1(L), dim3(256), 0, c->stream, 1.0, x, 1.0, dvec.p, x, L);
+      CFM_CHECK_LAUNCH();
+      CFM_TRY(system_apply(c, s, CFM_FP64, x, K.w.p));
+      hipLaunchKernelGGL(axpby_kernel, g1(L), dim3(256), 0, c->stream, 1.0, b, -1.0, K.w.p, K.d.p, L);
+      CFM_CHECK_LAUNCH();
+      CFM_TRY(norm2(c, K, K.d.p, L, K.scal.p));
+      CFM_TRY(read_scalar(c, K, K.scal.p, &nr));
+      if (!std::isfinite(nr)) return make_status(CFM_ENONFINITE, "non-finite refined residual");
+      ++refinements;
+    }
+    S.refinements = refinements;
+  }
+  S.iterations = iters;
+  S.rel_residual = nr / nb;
+  S.converged = S.rel_residual <= o.rtol;
+  if (st) *st = S;
+  if (!S.converged)
+    return make_status(CFM_ENOCONV, "GMRES stopped at relative residual %.3e > %.3e after %d steps",
+                       S.rel_residual, o.rtol, iters);
+  return {};
+}
+
+}  // namespace cfm

@@ -1,0 +1,43 @@
+// krylov.hpp -- device workspace and entry points of the GMRES solver.
+#pragma once
+#include "ctx.hpp"
+
+namespace cfm {
+
+// Krylov workspace (device), sized for vectors of length L and m basis vectors.
+struct Krylov {
+  int L = 0, m = 0;
+  DevBuf<double> Q, H, cs, sn, beta, h, h2, y, scal, r, w, z, t, d, dx, rhs, partial;
+  double* host_res = nullptr;  // pinned scalar read-back
+  Status reserve(int L_, int m_);
+  void release();
+};
+
+// One linear system over a contiguous particle range of the combined index space.
+struct SystemDesc {
+  int m = 0;      // particles
+  int dof = 3;    // unknowns per particle
+  int off = 0;    // first particle (combined id)
+  int L = 0;      // m * dof
+  int which = 0;  // CFM_SS or CFM_BB
+  double a = 1.0;
+};
+
+// Optional correction added to the preconditioner output (learned near-field term).
+struct GmresHooks {
+  Status (*precond_correction)(cfm_ctx*, const SystemDesc&, const double* in, double* out,
+                               void* user) = nullptr;
+  void* user = nullptr;
+};
+
+SystemDesc system_desc(const cfm_ctx* c, int system);
+Status system_apply(cfm_ctx* c, const SystemDesc& s, int precision, const double* x, double* y);
+Status system_precond(cfm_ctx* c, const SystemDesc& s, int kind, const double* x, double* y);
+Status solve(cfm_ctx* c, Krylov& K, const SystemDesc& s, const cfm_solve_opts& o, const double* b,
+             double* x, cfm_solve_stats* st, const GmresHooks* hooks);
+Status dots(cfm_ctx* c, Krylov& K, const double* Q, size_t ldq, int k, const double* w, int L,
+            double* out, bool accumulate);
+Status norm2(cfm_ctx* c, Krylov& K, const double* w, int L, double* out);
+Status read_scalar(cfm_ctx* c, Krylov& K, const double* dev, double* host);
+
+}  // namespace cfm

@@ -114,3 +114,36 @@ class RpyMobilitySolver:
         ms = ctypes.c_double()
         L.check(self._h, L.lib().cfm_time_apply(self._h, int(which), int(reps), ctypes.byref(ms)))
         return ms.value
+
+    # -- linear solves (solve_linear_system / apply_preconditioner_new) -------
+    _SYS = {"seg": L.CFM_SYS_SEG, "blob": L.CFM_SYS_BLOB}
+    _PC = {"none": L.CFM_PC_NONE, "diag": L.CFM_PC_DIAG, "self": L.CFM_PC_SELF}
+    _MODE = {"fp64": L.CFM_SOLVE_FP64, "mixed": L.CFM_SOLVE_MIXED}
+
+    def _sys_size(self, system):
+        return self.seg_dof * self.n_seg if self._SYS[system] == L.CFM_SYS_SEG else 3 * self.n_blob
+
+    def solve(self, system: str, rhs, x0=None, *, rtol=1e-8, restart=50, max_iter=1000,
+              precond="self", mode="fp64", raise_on_noconv=True):
+        """GMRES solve of M x = rhs for one species; returns (x, stats dict)."""
+        n = self._sys_size(system)
+        b = L.f64(rhs, n)
+        x = np.zeros(n) if x0 is None else L.f64(x0, n).copy()
+        o = L.cfm_solve_opts(system=self._SYS[system], precond=self._PC[precond],
+                             mode=self._MODE[mode], restart=restart, max_iter=max_iter,
+                             use_guess=int(x0 is not None), rtol=rtol)
+        st = L.cfm_solve_stats()
+        code = L.lib().cfm_gmres(self._h, ctypes.byref(o), L.ptr(b), L.ptr(x), ctypes.byref(st))
+        if code != L.CFM_OK and (raise_on_noconv or code != L.CFM_ENOCONV):
+            L.check(self._h, code)
+        return x, st.as_dict()
+
+    solve_linear_system = solve
+
+    def apply_preconditioner(self, system: str, vec, precond="self"):
+        n = self._sys_size(system)
+        v = L.f64(vec, n)
+        out = np.zeros(n)
+        L.check(self._h, L.lib().cfm_apply_preconditioner(self._h, self._SYS[system],
+                                                           self._PC[precond], L.ptr(v), L.ptr(out)))
+        return out

@@ -92,6 +92,56 @@ int cfm_apply(cfm_ctx* ctx, int32_t which, const double* f_seg, const double* f_
  * mean milliseconds per launch.  (Measurement helper; no reference counterpart.) */
 int cfm_time_apply(cfm_ctx* ctx, int32_t which, int32_t reps, double* ms_per_apply);
 
+
+/* ---- linear solves (GMRES) ------------------------------------------------ */
+
+/* linear systems */
+#define CFM_SYS_SEG 0   /* Mss f_seg = v_seg  (segment resistance problem) */
+#define CFM_SYS_BLOB 1  /* Mbb f_blob = v_blob (rigid-body blob forces, body held fixed) */
+
+/* right preconditioners P^-1 */
+#define CFM_PC_NONE 0
+#define CFM_PC_DIAG 1   /* scalar 6 pi mu a (and 8 pi mu a^3 on torques): mobility_solver.cpp:1108-1114,1550-1556 */
+#define CFM_PC_SELF 2   /* inverse of each particle's own (wall-corrected) self-mobility block */
+
+/* arithmetic of the solve */
+#define CFM_SOLVE_FP64 0   /* FP64 pair sums throughout */
+#define CFM_SOLVE_MIXED 1  /* FP32C pair sums inside FP64 iterative refinement */
+
+typedef struct cfm_solve_opts {
+  int32_t system;     /* CFM_SYS_SEG or CFM_SYS_BLOB */
+  int32_t precond;    /* CFM_PC_* */
+  int32_t mode;       /* CFM_SOLVE_FP64 or CFM_SOLVE_MIXED */
+  int32_t restart;    /* Krylov vectors per cycle (0: 50) */
+  int32_t max_iter;   /* cap on Arnoldi steps over all cycles (0: 1000) */
+  int32_t use_guess;  /* 1: x holds the initial guess, rejected if its residual exceeds
+                         ||rhs|| (mobility_solver.cpp:2296-2417); 0: start from zero */
+  double rtol;        /* target ||rhs - A x|| / ||rhs||, checked with an FP64 product */
+} cfm_solve_opts;
+
+typedef struct cfm_solve_stats {
+  int32_t iterations;   /* Arnoldi steps (matrix products inside Krylov cycles) */
+  int32_t refinements;  /* FP64 residual evaluations after the first */
+  int32_t converged;    /* 1 if rel_residual <= rtol */
+  int32_t reserved;
+  double rel_residual;  /* final ||rhs - A x|| / ||rhs|| from an FP64 product */
+  double initial_rel_residual; /* same for the initial guess (1 for a zero guess) */
+} cfm_solve_stats;
+
+/* Solve A x = rhs by right-preconditioned restarted GMRES on the device (FP64
+ * Arnoldi with classical Gram-Schmidt applied twice, Givens rotations, back
+ * substitution).  rhs and x are host AoS arrays of dof*n values (dof = seg_dof for
+ * CFM_SYS_SEG, 3 for CFM_SYS_BLOB).  Returns CFM_ENOCONV (with x holding the best
+ * iterate) when rtol is not reached within max_iter.  Replaces
+ * mobility_solver::solve_linear_system (mobility_solver.cpp:2187-2669). */
+int cfm_gmres(cfm_ctx* ctx, const cfm_solve_opts* opts, const double* rhs, double* x,
+              cfm_solve_stats* stats);
+
+/* out = P^-1 in for one system (host AoS arrays).  Replaces
+ * mobility_solver::apply_preconditioner_new (mobility_solver.cpp:1102-1594). */
+int cfm_apply_preconditioner(cfm_ctx* ctx, int32_t system, int32_t precond, const double* in,
+                             double* out);
+
 #ifdef __cplusplus
 }
 #endif
