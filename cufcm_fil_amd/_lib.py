@@ -71,6 +71,59 @@ class cfm_solve_stats(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
 
 
+CFM_GUESS_PREVIOUS, CFM_GUESS_LEARNED = 0, 1
+
+
+class cfm_fil_params(ctypes.Structure):
+    _fields_ = [
+        ("nseg", ctypes.c_int32),
+        ("max_broyden", ctypes.c_int32),
+        ("num_euler_steps", ctypes.c_int32),
+        ("barrier", ctypes.c_int32),
+        ("dl", ctypes.c_double),
+        ("dt", ctypes.c_double),
+        ("kb", ctypes.c_double),
+        ("kt", ctypes.c_double),
+        ("end_force", ctypes.c_double),
+        ("repulsive_factor", ctypes.c_double),
+        ("tol", ctypes.c_double),
+        ("gamma", ctypes.c_double),
+        ("pair_precision", ctypes.c_int32),
+        ("guess", ctypes.c_int32),
+        ("blob_solve", cfm_solve_opts),
+    ]
+
+
+class cfm_step_stats(ctypes.Structure):
+    _fields_ = [
+        ("broyden_iterations", ctypes.c_int32),
+        ("gmres_iterations", ctypes.c_int32),
+        ("gmres_solves", ctypes.c_int32),
+        ("converged", ctypes.c_int32),
+        ("final_error", ctypes.c_double),
+        ("max_gmres_rel_residual", ctypes.c_double),
+        ("seconds", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class cfm_profile_stats(ctypes.Structure):
+    _fields_ = [
+        ("launches", ctypes.c_int64),
+        ("kernel_ms", ctypes.c_double),
+        ("alg_flops", ctypes.c_double),
+        ("alg_bytes", ctypes.c_double),
+        ("launches_fp32", ctypes.c_int64),
+        ("kernel_ms_fp32", ctypes.c_double),
+        ("alg_flops_fp32", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 _dp = ctypes.POINTER(ctypes.c_double)
 _vp = ctypes.c_void_p
 _lib = None
@@ -105,6 +158,18 @@ def lib():
     L.cfm_gmres.argtypes = [_vp, ctypes.POINTER(cfm_solve_opts), _dp, _dp,
                             ctypes.POINTER(cfm_solve_stats)]
     L.cfm_apply_preconditioner.argtypes = [_vp, ctypes.c_int32, ctypes.c_int32, _dp, _dp]
+    L.cfm_fil_init.argtypes = [_vp, ctypes.POINTER(cfm_fil_params), _dp, _dp]
+    L.cfm_fil_step.argtypes = [_vp, ctypes.c_int32, ctypes.POINTER(cfm_step_stats)]
+    L.cfm_fil_get_state.argtypes = [_vp, _dp, _dp, _dp, _dp, _dp]
+    L.cfm_fil_set_state.argtypes = [_vp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
+    L.cfm_fil_internal_forces.argtypes = [_vp, ctypes.c_int32, _dp]
+    L.cfm_barrier_forces.argtypes = [_vp, _dp]
+    L.cfm_fil_jacobian_inverse.argtypes = [_vp, ctypes.c_int32, _dp]
+    L.cfm_fil_errors.argtypes = [_vp, ctypes.c_int32, _dp, _dp, _dp]
+    L.cfm_fil_update.argtypes = [_vp, _dp]
+    L.cfm_fil_initial_guess.argtypes = [_vp, ctypes.c_int32]
+    L.cfm_profile.argtypes = [_vp, ctypes.c_int32]
+    L.cfm_profile_read.argtypes = [_vp, ctypes.POINTER(cfm_profile_stats)]
     _lib = L
     return L
 

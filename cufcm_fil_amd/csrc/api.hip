@@ -115,6 +115,9 @@ using namespace cfm;
 
 static thread_local std::string g_create_error;
 
+void cfm_fil_release(cfm_ctx* c);
+void cfm_gnn_release(cfm_ctx* c);
+
 static int fail(cfm_ctx* c, const Status& s) {
   if (c) c->err = s.msg;
   else g_create_error = s.msg;
@@ -187,6 +190,8 @@ int cfm_create(const cfm_config* cfg, cfm_ctx** out) {
 int cfm_destroy(cfm_ctx* c) {
   if (!c) return CFM_OK;
   (void)hipSetDevice(c->device);
+  cfm_fil_release(c);
+  cfm_gnn_release(c);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   c->p.x.release();
   c->p.y.release();
@@ -202,6 +207,7 @@ int cfm_destroy(cfm_ctx* c) {
     delete c->krylov;
   }
   if (c->host_pinned) (void)hipHostFree(c->host_pinned);
+  for (hipEvent_t e : c->prof_events) (void)hipEventDestroy(e);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -366,6 +372,48 @@ int cfm_apply_preconditioner(cfm_ctx* c, int32_t system, int32_t precond, const 
   };
   Status st = run();
   return st.ok() ? CFM_OK : fail(c, st);
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int cfm_profile(cfm_ctx* c, int32_t enable) {
+  if (!c) return CFM_EINVAL;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  c->prof_on = enable != 0;
+  if (enable) {
+    c->prof_used = 0;
+    c->prof_flops.clear();
+    c->prof_bytes.clear();
+    c->prof_fp32.clear();
+  }
+  return CFM_OK;
+}
+
+int cfm_profile_read(cfm_ctx* c, cfm_profile_stats* st) {
+  if (!c || !st) return CFM_EINVAL;
+  (void)hipSetDevice(c->device);
+  hipError_t e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return fail(c, make_status(CFM_EHIP, "%s", hipGetErrorString(e)));
+  cfm_profile_stats S{};
+  for (size_t k = 0; k < c->prof_flops.size(); ++k) {
+    float ms = 0.f;
+    e = hipEventElapsedTime(&ms, c->prof_events[2 * k], c->prof_events[2 * k + 1]);
+    if (e != hipSuccess) return fail(c, make_status(CFM_EHIP, "%s", hipGetErrorString(e)));
+    S.launches += 1;
+    S.kernel_ms += ms;
+    S.alg_flops += c->prof_flops[k];
+    S.alg_bytes += c->prof_bytes[k];
+    if (c->prof_fp32[k]) {
+      S.launches_fp32 += 1;
+      S.kernel_ms_fp32 += ms;
+      S.alg_flops_fp32 += c->prof_flops[k];
+    }
+  }
+  *st = S;
+  return CFM_OK;
 }
 
 }  // extern "C"

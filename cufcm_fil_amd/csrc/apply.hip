@@ -276,6 +276,41 @@ static Status launch_t(const KArgs& a, int tiles, int split, hipStream_t st) {
   return {};
 }
 
+// Algorithmic flops and bytes of one product (pairs counted over the full
+// target x source rectangle, self pairs included).
+static void plan_cost(const cfm_ctx* c, const ApplyPlan& plan, bool fp32, double* flops,
+                      double* bytes) {
+  const Particles& p = c->p;
+  const double nts = std::max(0, std::min(plan.tgt_end, p.n_seg) - std::max(plan.tgt_begin, 0));
+  const double ntb = std::max(0, std::min(plan.tgt_end, p.n) - std::max(plan.tgt_begin, p.n_seg));
+  const double nss = plan.src_seg ? p.n_seg : 0, nsb = plan.src_blob ? p.n_blob : 0;
+  const bool s6 = p.seg_dof == 6;
+  const double f_ss = s6 ? CFM_FLOPS_PAIR_66 : CFM_FLOPS_PAIR_33;
+  const double f_sb = s6 ? CFM_FLOPS_PAIR_63 : CFM_FLOPS_PAIR_33;
+  const double f_bs = s6 ? CFM_FLOPS_PAIR_36 : CFM_FLOPS_PAIR_33;
+  *flops = nts * nss * f_ss + nts * nsb * f_sb + ntb * nss * f_bs + ntb * nsb * CFM_FLOPS_PAIR_33;
+  const double w = fp32 ? 4.0 : 8.0;
+  // sources: positions (3 FP64) + loads; targets: positions (3 FP64) + velocities written
+  *bytes = (nss + nsb + nts + ntb) * 3.0 * 8.0 + (nss * p.seg_dof + nsb * 3.0) * 8.0 +
+           (nts * p.seg_dof + ntb * 3.0) * 8.0;
+  (void)w;
+}
+
+static Status prof_begin(cfm_ctx* c, hipEvent_t* e0, hipEvent_t* e1) {
+  if (c->prof_used + 2 > c->prof_events.size()) {
+    for (int k = 0; k < 64; ++k) {
+      hipEvent_t e;
+      CFM_HIP(hipEventCreate(&e));
+      c->prof_events.push_back(e);
+    }
+  }
+  *e0 = c->prof_events[c->prof_used];
+  *e1 = c->prof_events[c->prof_used + 1];
+  c->prof_used += 2;
+  CFM_HIP(hipEventRecord(*e0, c->stream));
+  return {};
+}
+
 Status launch_apply(cfm_ctx* c, const ApplyPlan& plan) {
   Particles& p = c->p;
   KArgs a{};
@@ -314,6 +349,8 @@ Status launch_apply(cfm_ctx* c, const ApplyPlan& plan) {
   const bool fp32 = c->cfg.precision == CFM_FP32C;
   const bool wall = c->cfg.wall != 0;
   const int sd = p.seg_dof;
+  hipEvent_t pe0 = nullptr, pe1 = nullptr;
+  if (c->prof_on) CFM_TRY(prof_begin(c, &pe0, &pe1));
   Status s;
   if (fp32) {
     if (sd == 6) s = wall ? launch_t<float, 6, true>(a, tiles, split, c->stream)
@@ -327,6 +364,14 @@ Status launch_apply(cfm_ctx* c, const ApplyPlan& plan) {
                   : launch_t<double, 3, false>(a, tiles, split, c->stream);
   }
   CFM_TRY(s);
+  if (c->prof_on) {
+    CFM_HIP(hipEventRecord(pe1, c->stream));
+    double fl = 0.0, by = 0.0;
+    plan_cost(c, plan, fp32, &fl, &by);
+    c->prof_flops.push_back(fl);
+    c->prof_bytes.push_back(by);
+    c->prof_fp32.push_back(fp32 ? 1 : 0);
+  }
   if (split > 1) {
     const int cnt = plan.tgt_end - plan.tgt_begin;
     hipLaunchKernelGGL(rpy_finalize_kernel, dim3((cnt + 255) / 256), dim3(256), 0, c->stream, a);

@@ -106,5 +106,13 @@ struct cfm_ctx {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   cfm::Krylov* krylov = nullptr;        // GMRES workspace (created on first solve)
   cfm::DevBuf<double> vec_a, vec_b;     // system-vector staging
+  void* fil = nullptr;                  // filament state of the implicit step (step.hip)
+  // pair-kernel profiling (HIP events around each launch)
+  bool prof_on = false;
+  std::vector<hipEvent_t> prof_events;  // pool, two per launch
+  size_t prof_used = 0;
+  std::vector<double> prof_flops, prof_bytes;
+  std::vector<int> prof_fp32;
+  void* gnn = nullptr;                  // learned initial guess (gnn.hip)
   std::string err;
 };

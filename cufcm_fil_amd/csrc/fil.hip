@@ -1,0 +1,639 @@
+// fil.hip -- elastic-filament mechanics of the implicit step on the device (FP64).
+//
+// Restates, for tethered filaments with a clamped base and a follower end force
+// (the reference's INSTABILITY_CILIA model):
+//   initial guess      filament.cpp:1336-1663 + segment.cpp:38-79 + robot_arm :629-651
+//   internal loads     filament.cpp:1729-1853
+//   BDF residual       mobility_solver.cpp:2760-2853 (filament rows)
+//   approx. Jacobian   filament.cpp:9-60, :1855-2172 (built and inverted per filament)
+//   J0^-1 product      swimmer.cpp:919-931
+//   state update       filament.cpp:2174-2208, segment.cpp:81-90
+//   barrier forces     cuda_functions.cu:1825-1930 (segments; wall repulsion)
+//
+// Threading: kinematics and loads use one thread per filament (the chain recursions
+// are sequential along a filament); the residual uses one thread per segment; the
+// 6 NSEG x 6 NSEG approximate Jacobian of a filament is assembled in LDS by one
+// workgroup (a thread per segment block row) and inverted in place by Gauss-Jordan
+// elimination with partial pivoting; its transpose is stored so that the inverse
+// products read HBM coalesced.
+#include "ctx.hpp"
+#include "fil.hpp"
+#include "fil_device.hpp"
+
+namespace cfm {
+using namespace fil;
+
+__device__ __forceinline__ Q4 ldq(const double* q, size_t n, int s) {
+  return {q[s], q[n + s], q[2 * n + s], q[3 * n + s]};
+}
+__device__ __forceinline__ void stq(double* q, size_t n, int s, Q4 v) {
+  q[s] = v.s;
+  q[n + s] = v.x;
+  q[2 * n + s] = v.y;
+  q[3 * n + s] = v.z;
+}
+__device__ __forceinline__ V3 ld3(const double* a, size_t n, int s) {
+  return {a[s], a[n + s], a[2 * n + s]};
+}
+__device__ __forceinline__ void st3(double* a, size_t n, int s, V3 v) {
+  a[s] = v.x;
+  a[n + s] = v.y;
+  a[2 * n + s] = v.z;
+}
+
+// robot arm from the base segment of filament f
+__device__ void robot_arm(const FilK& K, int f) {
+  const size_t n = K.nseg_total;
+  const int s0 = f * K.nseg;
+  V3 t1 = tangent(ldq(K.q, n, s0));
+  V3 x = {K.x[s0], K.y[s0], K.z[s0]};
+  for (int i = 1; i < K.nseg; ++i) {
+    const V3 t2 = tangent(ldq(K.q, n, s0 + i));
+    x = x + (0.5 * K.dl) * (t1 + t2);
+    K.x[s0 + i] = x.x;
+    K.y[s0 + i] = x.y;
+    K.z[s0 + i] = x.z;
+    t1 = t2;
+  }
+}
+
+__global__ void fil_initial_guess_kernel(FilK K, int nt) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= K.nfil) return;
+  const size_t n = K.nseg_total;
+  const int s0 = f * K.nseg;
+  for (int i = 0; i < K.nseg; ++i) {
+    const int s = s0 + i;
+    const V3 x = {K.x[s], K.y[s], K.z[s]};
+    st3(K.xm2, n, s, ld3(K.xm1, n, s));
+    st3(K.xm1, n, s, x);
+    V3 u = ld3(K.u, n, s);
+    if (nt <= K.num_euler) {
+      st3(K.um1, n, s, u);
+    } else {
+      const V3 um1 = ld3(K.um1, n, s);
+      st3(K.um1, n, s, u);
+      u = 2.0 * u - um1;
+      st3(K.u, n, s, u);
+    }
+    const Q4 q = ldq(K.q, n, s);
+    stq(K.qm1, n, s, q);
+    stq(K.q, n, s, qmul(lie_exp(u), q));
+  }
+  // Lagrange-multiplier extrapolation (row 0 of each filament is the tether)
+  auto extrap = [&](double* a, double* a1, double* a2, size_t ld, int idx) {
+    for (int c = 0; c < 3; ++c) {
+      double& v = a[c * ld + idx];
+      double& v1 = a1[c * ld + idx];
+      double& v2 = a2[c * ld + idx];
+      if (nt == 1) {
+        v1 = v;
+      } else if (nt == 2) {
+        v2 = v1;
+        v1 = v;
+        v = 2.0 * v1 - v2;
+      } else if (nt > 2) {
+        const double t = 3.0 * (v - v1) + v2;
+        v2 = v1;
+        v1 = v;
+        v = t;
+      }
+    }
+  };
+  for (int i = 0; i < K.nseg; ++i) extrap(K.lam, K.lam_m1, K.lam_m2, n, s0 + i);
+  extrap(K.clamp, K.clamp_m1, K.clamp_m2, (size_t)K.nfil, f);
+  // base follows the (fixed) anchor: position reset, Lie algebra element zero
+  K.x[s0] = K.base[f];
+  K.y[s0] = K.base[K.nfil + f];
+  K.z[s0] = K.base[2 * K.nfil + f];
+  st3(K.u, n, s0, v3(0.0, 0.0, 0.0));
+  stq(K.q, n, s0, ldq(K.qm1, n, s0));
+  robot_arm(K, f);
+}
+
+__device__ V3 body_frame_moment(const FilK& K, Q4 q1, Q4 q2) {
+  Q4 m = qconj(midpoint(q1, q2));
+  const double il = 1.0 / K.dl;
+  const Q4 dq = {(q2.s - q1.s) * il, (q2.x - q1.x) * il, (q2.y - q1.y) * il, (q2.z - q1.z) * il};
+  m = qmul(m, dq);
+  return {K.kt * 2.0 * m.x, K.kb * 2.0 * m.y, K.kb * 2.0 * m.z};
+}
+
+__global__ void fil_forces_kernel(FilK K, int nt, double* __restrict__ fout, int ld) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= K.nfil) return;
+  const size_t n = K.nseg_total;
+  const int s0 = f * K.nseg;
+  double F[6] = {0, 0, 0, 0, 0, 0};   // loads of segment i (being accumulated)
+  Q4 qi = ldq(K.q, n, s0);
+  V3 ti = tangent(qi);
+  for (int i = 0; i < K.nseg; ++i) {
+    double G[6] = {0, 0, 0, 0, 0, 0};  // contributions to segment i+1
+    if (i < K.nseg - 1) {
+      const Q4 qn = ldq(K.q, n, s0 + i + 1);
+      const V3 tn = tangent(qn);
+      const V3 lam = ld3(K.lam, n, s0 + i + 1);
+      const V3 c1 = cross(ti, lam), c2 = cross(tn, lam);
+      const V3 M = mv(rot_mat(midpoint(qi, qn)), body_frame_moment(K, qi, qn));
+      F[0] -= lam.x; F[1] -= lam.y; F[2] -= lam.z;
+      G[0] += lam.x; G[1] += lam.y; G[2] += lam.z;
+      F[3] += -0.5 * K.dl * c1.x + M.x;
+      F[4] += -0.5 * K.dl * c1.y + M.y;
+      F[5] += -0.5 * K.dl * c1.z + M.z;
+      G[3] += -0.5 * K.dl * c2.x - M.x;
+      G[4] += -0.5 * K.dl * c2.y - M.y;
+      G[5] += -0.5 * K.dl * c2.z - M.z;
+      if (i == 0) {
+        // tether force and clamping torque on the base segment
+        const V3 tl = ld3(K.lam, n, s0);
+        F[0] += tl.x; F[1] += tl.y; F[2] += tl.z;
+        const V3 ct = dexpinv_transpose(ld3(K.u, n, s0), ld3(K.clamp, K.nfil, f));
+        F[3] += ct.x; F[4] += ct.y; F[5] += ct.z;
+      }
+      qi = qn;
+      ti = tn;
+    } else {
+      if (i == 0) {
+        const V3 tl = ld3(K.lam, n, s0);
+        F[0] += tl.x; F[1] += tl.y; F[2] += tl.z;
+        const V3 ct = dexpinv_transpose(ld3(K.u, n, s0), ld3(K.clamp, K.nfil, f));
+        F[3] += ct.x; F[4] += ct.y; F[5] += ct.z;
+      }
+      // follower force at the tip
+      F[0] -= K.end_force * ti.x;
+      F[1] -= K.end_force * ti.y;
+      F[2] -= K.end_force * ti.z;
+    }
+    if (nt == 0 && i == 4 && K.nseg > 4) {
+      F[0] += K.pert[f];
+      F[1] += K.pert[K.nfil + f];
+      F[2] += K.pert[2 * K.nfil + f];
+    }
+    const int s = s0 + i;
+    for (int c = 0; c < 6; ++c) {
+      K.fint[c * n + s] = F[c];
+      fout[(size_t)c * ld + s] = F[c];
+      F[c] = G[c];
+    }
+  }
+}
+
+// error vector (reference layout per filament) and the per-segment badness
+// max(|e_x|/(DL TOL), |e_u|/TOL) (NaN propagates).
+__global__ void fil_errors_kernel(FilK K, int nt, const double* __restrict__ v, int ld,
+                                  double* __restrict__ err, double* __restrict__ bad) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  double b = 0.0;
+  if (s < (int)K.nseg_total) {
+    const size_t n = K.nseg_total;
+    const int f = s / K.nseg, j = s - f * K.nseg;
+    const V3 x = {K.x[s], K.y[s], K.z[s]};
+    const V3 xm1 = ld3(K.xm1, n, s), xm2 = ld3(K.xm2, n, s);
+    const V3 vel = {v[s], v[ld + s], v[2 * (size_t)ld + s]};
+    const V3 om = {v[3 * (size_t)ld + s], v[4 * (size_t)ld + s], v[5 * (size_t)ld + s]};
+    const V3 u = ld3(K.u, n, s), um1 = ld3(K.um1, n, s);
+    const V3 W = dexpinv(u, om);
+    V3 ex, eu;
+    if (nt < K.num_euler) {
+      ex = x - xm1 - K.dt * vel;
+      eu = u - K.dt * W;
+    } else {
+      ex = x + (-4.0 / 3.0) * xm1 + (1.0 / 3.0) * xm2 + (-2.0 / 3.0 * K.dt) * vel;
+      eu = u - (1.0 / 3.0) * um1 + (-2.0 / 3.0 * K.dt) * W;
+    }
+    double* e = err + (size_t)f * 6 * K.nseg;
+    e[3 * j] = ex.x;
+    e[3 * j + 1] = ex.y;
+    e[3 * j + 2] = ex.z;
+    e[3 * K.nseg + 3 * j] = eu.x;
+    e[3 * K.nseg + 3 * j + 1] = eu.y;
+    e[3 * K.nseg + 3 * j + 2] = eu.z;
+    const double sx = 1.0 / (K.dl * K.tol), su = 1.0 / K.tol;
+    b = fmax(fmax(fabs(ex.x), fmax(fabs(ex.y), fabs(ex.z))) * sx,
+             fmax(fabs(eu.x), fmax(fabs(eu.y), fabs(eu.z))) * su);
+    if (!(ex.x == ex.x && ex.y == ex.y && ex.z == ex.z && eu.x == eu.x && eu.y == eu.y &&
+          eu.z == eu.z))
+      b = __builtin_nan("");
+  }
+  __shared__ double red[256];
+  red[threadIdx.x] = b;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (threadIdx.x < off) {
+      const double a = red[threadIdx.x], c = red[threadIdx.x + off];
+      red[threadIdx.x] = (a != a || c != c) ? __builtin_nan("") : fmax(a, c);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) bad[blockIdx.x] = red[0];
+}
+
+__global__ void max_finish_kernel(const double* __restrict__ part, int nb, double* __restrict__ out) {
+  __shared__ double red[256];
+  double b = 0.0;
+  for (int i = threadIdx.x; i < nb; i += 256) {
+    const double a = part[i];
+    b = (a != a || b != b) ? __builtin_nan("") : fmax(a, b);
+  }
+  red[threadIdx.x] = b;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (threadIdx.x < off) {
+      const double a = red[threadIdx.x], c = red[threadIdx.x + off];
+      red[threadIdx.x] = (a != a || c != c) ? __builtin_nan("") : fmax(a, c);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = red[0];
+}
+
+// ---- approximate Jacobian -----------------------------------------------------
+struct M43 {
+  double a[4][3];
+};
+__device__ __forceinline__ M43 psi_mat(Q4 q) {
+  M43 P;
+  P.a[0][0] = -0.5 * q.x; P.a[0][1] = -0.5 * q.y; P.a[0][2] = -0.5 * q.z;
+  const M3 r = rcross(v3(q.x, q.y, q.z));
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) P.a[i + 1][j] = 0.5 * (r.a[i][j] + (i == j ? q.s : 0.0));
+  return P;
+}
+// rows 1..3 of left_mult_mat(q) * A
+__device__ __forceinline__ M3 lmul_rows(Q4 q, const M43& A) {
+  // left_mult_mat rows 1..3: [v_i, -rcross(v) + s I]
+  const M3 r = rcross(v3(q.x, q.y, q.z));
+  const double v[3] = {q.x, q.y, q.z};
+  M3 out;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      double s = v[i] * A.a[0][j];
+      for (int k = 0; k < 3; ++k) s += (-r.a[i][k] + (i == k ? q.s : 0.0)) * A.a[k + 1][j];
+      out.a[i][j] = s;
+    }
+  return out;
+}
+
+__device__ M3 moment_lie_derivative(const FilK& K, Q4 q1, Q4 q2, bool upper) {
+  M43 A = upper ? psi_mat(q2) : psi_mat(q1);
+  const double sA = upper ? 1.0 / K.dl : -1.0 / K.dl;
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 3; ++j) A.a[i][j] *= sA;
+  const Q4 mq = qconj(midpoint(q1, q2));
+  M3 d = lmul_rows(mq, A);
+  const double il = 1.0 / K.dl;
+  const Q4 dq = {(q2.s - q1.s) * il, (q2.x - q1.x) * il, (q2.y - q1.y) * il, (q2.z - q1.z) * il};
+  const double sB = upper ? 0.5 * K.dl : -0.5 * K.dl;
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 3; ++j) A.a[i][j] *= sB;
+  d = madd(d, lmul_rows(dq, A));
+  for (int j = 0; j < 3; ++j) {
+    d.a[0][j] *= 2.0 * K.kt;
+    d.a[1][j] *= 2.0 * K.kb;
+    d.a[2][j] *= 2.0 * K.kb;
+  }
+  return d;
+}
+
+__device__ __forceinline__ void put_block(double* J, int N, int r0, int c0, const M3& B) {
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) J[(r0 + i) * N + c0 + j] = B.a[i][j];
+}
+
+__device__ __forceinline__ M3 frame_sum(V3 Kv, Q4 q) {
+  // Kv0 rcross(t) + Kv1 rcross(n) + Kv2 rcross(b)
+  M3 A = mscale(rcross(tangent(q)), Kv.x);
+  A = madd(A, rcross(normal(q)), Kv.y);
+  return madd(A, rcross(binormal(q)), Kv.z);
+}
+
+__global__ __launch_bounds__(256) void fil_jacobian_kernel(FilK K, int nt, int* __restrict__ status) {
+  extern __shared__ double J[];
+  __shared__ double colk[128];
+  __shared__ int piv[128];
+  __shared__ double pval;
+  __shared__ int prow;
+  const int f = blockIdx.x;
+  const int n = K.nseg, N = 6 * n;
+  const size_t ns = K.nseg_total;
+  const int s0 = f * n;
+  const int tid = threadIdx.x, nth = blockDim.x;
+  const bool euler = nt < K.num_euler;
+  const double pi = 3.14159265358979323846;
+  const double a3 = K.a * K.a * K.a;
+  for (int e = tid; e < N * N; e += nth) J[e] = 0.0;
+  __syncthreads();
+  const double fac = euler ? -K.dt / (6.0 * pi * K.mu * K.a) : -K.dt / (9.0 * pi * K.mu * K.a);
+  for (int r = tid; r < 3 * n; r += nth) {
+    J[r * N + r] = fac;
+    if (r < 3 * n - 3) J[r * N + r + 3] = -fac;
+  }
+  __syncthreads();
+  if (tid < n) {
+    const int j = tid;
+    if (j >= 1) {
+      const M3 temp = mscale(rcross(tangent(ldq(K.q, ns, s0 + j))), K.dl);
+      put_block(J, N, 3 * j, 3 * (n + j), mscale(temp, 0.5));
+      for (int i = j + 1; i < n; ++i) put_block(J, N, 3 * i, 3 * (n + j), temp);
+      if (j == n - 1) {
+        const M3 tip = mscale(rcross(tangent(ldq(K.q, ns, s0 + n - 1))), fac * K.end_force);
+        for (int a = 0; a < 3; ++a)
+          for (int b = 0; b < 3; ++b) J[(3 * (n - 1) + a) * N + 3 * (2 * n - 1) + b] -= tip.a[a][b];
+      }
+      const double fac2 = euler ? K.dl * K.dt / (16.0 * pi * K.mu * a3) : K.dl * K.dt / (24.0 * pi * K.mu * a3);
+      for (int kk = j - 1; kk <= j; ++kk) {
+        const M3 um = rcross(ld3(K.u, ns, s0 + kk));
+        const M3 tm = mscale(rcross(tangent(ldq(K.q, ns, s0 + kk))), -1.0);
+        M3 B = madd(tm, mm(um, tm), 0.5);
+        B = madd(B, mm(um, mm(um, tm)), 1.0 / 12.0);
+        put_block(J, N, 3 * (n + kk), 3 * j, mscale(B, fac2));
+      }
+    }
+    // Lie-algebra rows of segment k = j
+    const int k = j;
+    const double tsf = euler ? K.dt : 2.0 * K.dt / 3.0;
+    const double Tfac = 1.0 / (8.0 * pi * K.mu * a3);
+    const V3 uk = ld3(K.u, ns, s0 + k);
+    const M3 Lie = mscale(rcross(uk), -1.0);
+    const V3 om = {Tfac * K.fint[3 * ns + s0 + k], Tfac * K.fint[4 * ns + s0 + k],
+                   Tfac * K.fint[5 * ns + s0 + k]};
+    const M3 Om = rcross(om);
+    const M3 LL = mm(Lie, Lie);
+    auto diag_block = [&](const M3& dO) {
+      M3 B = madd(dO, madd(Om, mm(Lie, dO)), -0.5);
+      M3 C = madd(madd(rcross(mv(Om, uk)), mm(Lie, Om)), mm(LL, dO));
+      B = madd(B, C, 1.0 / 12.0);
+      return mscale(B, -tsf);
+    };
+    auto off_block = [&](const M3& dO) {
+      M3 B = madd(dO, mm(Lie, dO), -0.5);
+      B = madd(B, mm(LL, dO), 1.0 / 12.0);
+      return mscale(B, -tsf);
+    };
+    const Q4 qk = ldq(K.q, ns, s0 + k);
+    const V3 tk = tangent(qk);
+    M3 D = zero3(), Lb = zero3(), Rb = zero3();
+    M3 Cp;
+    {
+      V3 lamvec = v3(0, 0, 0);
+      if (k >= 1) lamvec = lamvec + ld3(K.lam, ns, s0 + k);
+      if (k <= n - 2) lamvec = lamvec + ld3(K.lam, ns, s0 + k + 1);
+      Cp = mscale(mm(rcross(lamvec), rcross(tk)), -0.5 * K.dl);
+    }
+    M3 Eld = zero3();  // elastic part of the diagonal derivative
+    if (k <= n - 2) {
+      const Q4 qp = ldq(K.q, ns, s0 + k + 1);
+      const V3 Kph = body_frame_moment(K, qk, qp);
+      const M3 Rph = rot_mat(midpoint(qk, qp));
+      Eld = madd(Eld, madd(mscale(frame_sum(Kph, qk), 0.5), mm(Rph, moment_lie_derivative(K, qk, qp, false))));
+      const M3 Er = madd(mscale(frame_sum(Kph, qp), 0.5), mm(Rph, moment_lie_derivative(K, qk, qp, true)));
+      Rb = off_block(mscale(Er, Tfac));
+    }
+    if (k >= 1) {
+      const Q4 qm = ldq(K.q, ns, s0 + k - 1);
+      const V3 Kmh = body_frame_moment(K, qm, qk);
+      const M3 Rmh = rot_mat(midpoint(qm, qk));
+      Eld = madd(Eld, madd(mscale(frame_sum(Kmh, qk), 0.5), mm(Rmh, moment_lie_derivative(K, qm, qk, true))), -1.0);
+      const M3 El = madd(mscale(frame_sum(Kmh, qm), -0.5), mm(Rmh, moment_lie_derivative(K, qm, qk, false)), -1.0);
+      Lb = off_block(mscale(El, Tfac));
+    }
+    D = diag_block(mscale(madd(Eld, Cp), Tfac));
+    D = madd(D, eye3());
+    const int r0 = 3 * (n + k);
+    if (k == 0) {
+      // clamped base: the clamp multiplier replaces the base Lie-algebra unknowns
+      const double fac3 = euler ? -K.dt / (8.0 * pi * K.mu * a3) : -K.dt / (12.0 * pi * K.mu * a3);
+      const M3 um = rcross(uk);
+      const M3 blk = madd(madd(eye3(), um, 0.5), mm(um, um), 1.0 / 12.0);
+      const M3 blkT = madd(madd(eye3(), um, -0.5), mm(um, um), 1.0 / 12.0);
+      D = mscale(mm(blk, blkT), fac3);
+    }
+    put_block(J, N, r0, 3 * (n + k), D);
+    if (k >= 1) put_block(J, N, r0, 3 * (n + k - 1), k == 1 ? zero3() : Lb);
+    if (k <= n - 2) put_block(J, N, r0, 3 * (n + k + 1), Rb);
+  }
+  __syncthreads();
+
+  // ---- in-place Gauss-Jordan inversion with partial pivoting ----
+  for (int k = 0; k < N; ++k) {
+    if (tid < 64) {
+      double best = -1.0;
+      int bi = k;
+      for (int r = k + tid; r < N; r += 64) {
+        const double a = fabs(J[r * N + k]);
+        if (a > best) {
+          best = a;
+          bi = r;
+        }
+      }
+      for (int off = 32; off > 0; off >>= 1) {
+        const double ob = __shfl_xor(best, off, 64);
+        const int oi = __shfl_xor(bi, off, 64);
+        if (ob > best || (ob == best && oi < bi)) {
+          best = ob;
+          bi = oi;
+        }
+      }
+      if (tid == 0) {
+        prow = bi;
+        piv[k] = bi;
+      }
+    }
+    __syncthreads();
+    const int p = prow;
+    if (p != k)
+      for (int c = tid; c < N; c += nth) {
+        const double t = J[k * N + c];
+        J[k * N + c] = J[p * N + c];
+        J[p * N + c] = t;
+      }
+    __syncthreads();
+    if (tid == 0) pval = J[k * N + k];
+    for (int r = tid; r < N; r += nth) colk[r] = J[r * N + k];
+    __syncthreads();
+    const double d = pval;
+    if (tid == 0 && !(fabs(d) > 0.0)) status[0] = 1;  // singular
+    for (int c = tid; c < N; c += nth) J[k * N + c] = (c == k) ? 1.0 / d : J[k * N + c] / d;
+    __syncthreads();
+    const double inv = J[k * N + k];
+    for (int e = tid; e < N * N; e += nth) {
+      const int r = e / N, c = e - r * N;
+      if (r == k) continue;
+      J[e] = (c == k) ? -colk[r] * inv : J[e] - colk[r] * J[k * N + c];
+    }
+    __syncthreads();
+  }
+  for (int k = N - 1; k >= 0; --k) {
+    const int p = piv[k];
+    if (p != k)
+      for (int r = tid; r < N; r += nth) {
+        const double t = J[r * N + k];
+        J[r * N + k] = J[r * N + p];
+        J[r * N + p] = t;
+      }
+    __syncthreads();
+  }
+  // store the transpose: jinvT[f][c][r] = Jinv[r][c]
+  double* out = K.jinv + (size_t)f * N * N;
+  for (int e = tid; e < N * N; e += nth) {
+    const int c = e / N, r = e - c * N;
+    out[e] = J[r * N + c];
+  }
+}
+
+// out_f = scale * J0^-1 in_f for every filament (vectors in the reference layout)
+__global__ __launch_bounds__(128) void fil_jinv_mult_kernel(FilK K, const double* __restrict__ in,
+                                                            double* __restrict__ out, double scale) {
+  __shared__ double x[128];
+  const int f = blockIdx.x;
+  const int N = 6 * K.nseg;
+  const double* v = in + (size_t)f * N;
+  for (int i = threadIdx.x; i < N; i += blockDim.x) x[i] = v[i];
+  __syncthreads();
+  const double* JT = K.jinv + (size_t)f * N * N;
+  for (int r = threadIdx.x; r < N; r += blockDim.x) {
+    double s = 0.0;
+    for (int c = 0; c < N; ++c) s += JT[(size_t)c * N + r] * x[c];
+    out[(size_t)f * N + r] = scale * s;
+  }
+}
+
+__global__ void fil_update_kernel(FilK K, const double* __restrict__ upd) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= K.nfil) return;
+  const size_t ns = K.nseg_total;
+  const int n = K.nseg, s0 = f * n;
+  const double* u = upd + (size_t)f * 6 * n;
+  for (int i = 0; i < n; ++i) {
+    const int s = s0 + i;
+    st3(K.lam, ns, s, ld3(K.lam, ns, s) + v3(u[3 * i], u[3 * i + 1], u[3 * i + 2]));
+  }
+  st3(K.clamp, K.nfil, f, ld3(K.clamp, K.nfil, f) + v3(u[3 * n], u[3 * n + 1], u[3 * n + 2]));
+  for (int i = 1; i < n; ++i) {
+    const int s = s0 + i;
+    const V3 un = ld3(K.u, ns, s) + v3(u[3 * (n + i)], u[3 * (n + i) + 1], u[3 * (n + i) + 2]);
+    st3(K.u, ns, s, un);
+    stq(K.q, ns, s, qmul(lie_exp(un), ldq(K.qm1, ns, s)));
+  }
+  robot_arm(K, f);
+}
+
+// Barrier (short-range repulsion) forces added to the segment loads in fout.
+__global__ __launch_bounds__(256) void barrier_kernel(FilK K, const double* __restrict__ bx,
+                                                      const double* __restrict__ by,
+                                                      const double* __restrict__ bz, int nblob,
+                                                      int wall, double* __restrict__ fout, int ld) {
+  __shared__ double sx[256], sy[256], sz[256];
+  const int ns = (int)K.nseg_total;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = i < ns;
+  const int ic = valid ? i : 0;
+  const double xi = K.x[ic], yi = K.y[ic], zi = K.z[ic];
+  const int fi = ic / K.nseg;
+  const double fmag = K.rep_factor * K.end_force;
+  double fx = 0.0, fy = 0.0, fz = 0.0;
+  if (wall && zi < 0.5 * K.dl) {
+    double w = fmin(1.0, 1.0 - (zi - K.a) / (0.5 * K.dl - K.a));
+    fz += fmag * w * w * w * w;
+  }
+  const int ntot = ns + nblob;
+  for (int t0 = 0; t0 < ntot; t0 += 256) {
+    const int j = t0 + threadIdx.x;
+    __syncthreads();
+    if (j < ns) {
+      sx[threadIdx.x] = K.x[j];
+      sy[threadIdx.x] = K.y[j];
+      sz[threadIdx.x] = K.z[j];
+    } else if (j < ntot) {
+      sx[threadIdx.x] = bx[j - ns];
+      sy[threadIdx.x] = by[j - ns];
+      sz[threadIdx.x] = bz[j - ns];
+    }
+    __syncthreads();
+    const int cnt = min(256, ntot - t0);
+    const double a_sum = 2.0 * K.a;
+    const double cut = 1.1 * a_sum, chi = 10.0 / a_sum;
+    for (int jj = 0; jj < cnt; ++jj) {
+      const int j2 = t0 + jj;
+      const double dx = xi - sx[jj], dy = yi - sy[jj], dz = zi - sz[jj];
+      const double d2 = dx * dx + dy * dy + dz * dz;
+      if (d2 < cut * cut) {
+        if (j2 < ns && j2 / K.nseg == fi && abs(ic - j2) <= 1) continue;
+        const double dist = sqrt(d2);
+        double fac = fmin(1.0, 1.0 - chi * (dist - a_sum));
+        fac = fmag * fac * fac * fac * fac / dist;
+        fx += fac * dx;
+        fy += fac * dy;
+        fz += fac * dz;
+      }
+    }
+  }
+  if (valid) {
+    fout[i] += fx;
+    fout[(size_t)ld + i] += fy;
+    fout[2 * (size_t)ld + i] += fz;
+  }
+}
+
+// ---- launchers -------------------------------------------------------------------
+static inline dim3 gsz(int n, int b) { return dim3((n + b - 1) / b); }
+
+Status fil_initial_guess(cfm_ctx* c, const FilK& K, int nt) {
+  hipLaunchKernelGGL(fil_initial_guess_kernel, gsz(K.nfil, 64), dim3(64), 0, c->stream, K, nt);
+  CFM_CHECK_LAUNCH();
+  return {};
+}
+
+Status fil_forces(cfm_ctx* c, const FilK& K, int nt) {
+  hipLaunchKernelGGL(fil_forces_kernel, gsz(K.nfil, 64), dim3(64), 0, c->stream, K, nt, c->p.f.p,
+                     c->p.n);
+  CFM_CHECK_LAUNCH();
+  return {};
+}
+
+Status fil_barrier(cfm_ctx* c, const FilK& K) {
+  const Particles& p = c->p;
+  hipLaunchKernelGGL(barrier_kernel, gsz((int)K.nseg_total, 256), dim3(256), 0, c->stream, K,
+                     p.x.p + p.n_seg, p.y.p + p.n_seg, p.z.p + p.n_seg, p.n_blob, c->cfg.wall,
+                     p.f.p, p.n);
+  CFM_CHECK_LAUNCH();
+  return {};
+}
+
+Status fil_errors(cfm_ctx* c, const FilK& K, int nt, double* err, double* bad_part,
+                  double* bad_out) {
+  const int nb = ((int)K.nseg_total + 255) / 256;
+  hipLaunchKernelGGL(fil_errors_kernel, dim3(nb), dim3(256), 0, c->stream, K, nt, c->p.v.p,
+                     c->p.n, err, bad_part);
+  hipLaunchKernelGGL(max_finish_kernel, dim3(1), dim3(256), 0, c->stream, bad_part, nb, bad_out);
+  CFM_CHECK_LAUNCH();
+  return {};
+}
+
+Status fil_jacobian(cfm_ctx* c, const FilK& K, int nt, int* status_dev) {
+  const int N = 6 * K.nseg;
+  const size_t lds = (size_t)N * N * sizeof(double);
+  static bool attr_set = false;
+  if (!attr_set) {
+    CFM_HIP(hipFuncSetAttribute((const void*)fil_jacobian_kernel,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    attr_set = true;
+  }
+  hipLaunchKernelGGL(fil_jacobian_kernel, dim3(K.nfil), dim3(256), lds, c->stream, K, nt, status_dev);
+  CFM_CHECK_LAUNCH();
+  return {};
+}
+
+Status fil_jinv_mult(cfm_ctx* c, const FilK& K, const double* in, double* out, double scale) {
+  hipLaunchKernelGGL(fil_jinv_mult_kernel, dim3(K.nfil), dim3(128), 0, c->stream, K, in, out, scale);
+  CFM_CHECK_LAUNCH();
+  return {};
+}
+
+Status fil_update(cfm_ctx* c, const FilK& K, const double* upd) {
+  hipLaunchKernelGGL(fil_update_kernel, gsz(K.nfil, 64), dim3(64), 0, c->stream, K, upd);
+  CFM_CHECK_LAUNCH();
+  return {};
+}
+
+}  // namespace cfm

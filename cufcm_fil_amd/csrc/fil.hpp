@@ -1,0 +1,40 @@
+// fil.hpp -- device state of the filaments and the kernels of the implicit step.
+#pragma once
+#include "ctx.hpp"
+
+namespace cfm {
+
+// Kernel-side view of the filament state.  Per-segment arrays are structure of
+// arrays over the nseg_total segments (component-major, e.g. q + k*nseg_total);
+// per-filament arrays over nfil.  lam row 0 of each filament holds the tether
+// multiplier, row k >= 1 the inextensibility multiplier of link (k-1, k).
+struct FilK {
+  int nseg = 0, nfil = 0;
+  size_t nseg_total = 0;
+  int num_euler = 1;
+  double dl = 2.2, dt = 0.1, kb = 1800.0, kt = 1800.0, mu = 1.0, a = 1.0;
+  double end_force = 0.0, rep_factor = 2.0, tol = 1e-4;
+  double* x = nullptr;   // segment positions (the context's particle arrays)
+  double* y = nullptr;
+  double* z = nullptr;
+  double *q = nullptr, *qm1 = nullptr;           // 4 x nseg_total
+  double *u = nullptr, *um1 = nullptr;           // 3 x nseg_total
+  double *xm1 = nullptr, *xm2 = nullptr;         // 3 x nseg_total
+  double *lam = nullptr, *lam_m1 = nullptr, *lam_m2 = nullptr;  // 3 x nseg_total
+  double *clamp = nullptr, *clamp_m1 = nullptr, *clamp_m2 = nullptr;  // 3 x nfil
+  double* base = nullptr;  // 3 x nfil
+  double* pert = nullptr;  // 3 x nfil
+  double* fint = nullptr;  // 6 x nseg_total internal loads (no barrier)
+  double* jinv = nullptr;  // nfil x (6 nseg)^2, transposed inverse Jacobians
+};
+
+Status fil_initial_guess(cfm_ctx* c, const FilK& K, int nt);
+Status fil_forces(cfm_ctx* c, const FilK& K, int nt);
+Status fil_barrier(cfm_ctx* c, const FilK& K);
+Status fil_errors(cfm_ctx* c, const FilK& K, int nt, double* err, double* bad_part,
+                  double* bad_out);
+Status fil_jacobian(cfm_ctx* c, const FilK& K, int nt, int* status_dev);
+Status fil_jinv_mult(cfm_ctx* c, const FilK& K, const double* in, double* out, double scale);
+Status fil_update(cfm_ctx* c, const FilK& K, const double* upd);
+
+}  // namespace cfm

@@ -474,3 +474,53 @@ Status solve(cfm_ctx* c, Krylov& K, const SystemDesc& s, const cfm_solve_opts& o
 }
 
 }  // namespace cfm
+
+namespace cfm {
+
+// ---- BLAS-1/2 helpers shared with the Broyden driver -------------------------
+Status dev_dots(cfm_ctx* c, double* partial, const double* Q, size_t ldq, int k, const double* w,
+                int L, double* out) {
+  const int nb = (L + kChunk - 1) / kChunk;
+  hipLaunchKernelGGL(dots_partial_kernel, dim3(k, nb), dim3(kRB), 0, c->stream, Q, ldq, k, w, L, partial);
+  CFM_CHECK_LAUNCH();
+  hipLaunchKernelGGL(dots_finish_kernel, dim3((k + 63) / 64), dim3(64), 0, c->stream, partial, nb, k, out, 0);
+  CFM_CHECK_LAUNCH();
+  return {};
+}
+size_t dev_dots_partial_size(int L, int k) { return (size_t)((L + kChunk - 1) / kChunk) * k; }
+
+Status dev_gemv_sub(cfm_ctx* c, const double* Q, size_t ldq, int k, const double* h, double* w, int L) {
+  hipLaunchKernelGGL(gemv_sub_kernel, g1(L), dim3(256), 0, c->stream, Q, ldq, k, h, w, L);
+  CFM_CHECK_LAUNCH();
+  return {};
+}
+Status dev_axpby(cfm_ctx* c, double a, const double* x, double b, const double* y, double* out, int L) {
+  hipLaunchKernelGGL(axpby_kernel, g1(L), dim3(256), 0, c->stream, a, x, b, y, out, L);
+  CFM_CHECK_LAUNCH();
+  return {};
+}
+Status dev_fill(cfm_ctx* c, double* x, double v, int L) {
+  hipLaunchKernelGGL(fill_kernel, g1(L), dim3(256), 0, c->stream, x, v, L);
+  CFM_CHECK_LAUNCH();
+  return {};
+}
+Status dev_scale_by_inv(cfm_ctx* c, const double* x, const double* s, double* out, int L) {
+  hipLaunchKernelGGL(scale_by_inv_kernel, g1(L), dim3(256), 0, c->stream, x, s, out, L);
+  CFM_CHECK_LAUNCH();
+  return {};
+}
+Status dev_norm2(cfm_ctx* c, double* partial, const double* w, int L, double* out) {
+  CFM_TRY(dev_dots(c, partial, w, 0, 1, w, L, out));
+  hipLaunchKernelGGL(sqrt_kernel, dim3(1), dim3(1), 0, c->stream, out);
+  CFM_CHECK_LAUNCH();
+  return {};
+}
+Status dev_copy_rows(cfm_ctx* c, const double* src, size_t src_ld, int src_off, double* dst,
+                     size_t dst_ld, int dst_off, int dof, int m) {
+  hipLaunchKernelGGL(copy_rows_kernel, g1(m), dim3(256), 0, c->stream, src, src_ld, src_off, dst,
+                     dst_ld, dst_off, dof, m);
+  CFM_CHECK_LAUNCH();
+  return {};
+}
+
+}  // namespace cfm

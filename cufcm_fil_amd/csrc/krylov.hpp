@@ -41,3 +41,16 @@ Status norm2(cfm_ctx* c, Krylov& K, const double* w, int L, double* out);
 Status read_scalar(cfm_ctx* c, Krylov& K, const double* dev, double* host);
 
 }  // namespace cfm
+
+namespace cfm {
+Status dev_dots(cfm_ctx* c, double* partial, const double* Q, size_t ldq, int k, const double* w,
+                int L, double* out);
+size_t dev_dots_partial_size(int L, int k);
+Status dev_gemv_sub(cfm_ctx* c, const double* Q, size_t ldq, int k, const double* h, double* w, int L);
+Status dev_axpby(cfm_ctx* c, double a, const double* x, double b, const double* y, double* out, int L);
+Status dev_fill(cfm_ctx* c, double* x, double v, int L);
+Status dev_scale_by_inv(cfm_ctx* c, const double* x, const double* s, double* out, int L);
+Status dev_norm2(cfm_ctx* c, double* partial, const double* w, int L, double* out);
+Status dev_copy_rows(cfm_ctx* c, const double* src, size_t src_ld, int src_off, double* dst,
+                     size_t dst_ld, int dst_off, int dof, int m);
+}  // namespace cfm

@@ -1,0 +1,445 @@
+// step.hip -- implicit (BDF + Broyden) filament step driven on the device.
+//
+// Host side restates the time-step loop of src/cilia/cilia_sim_main.cu:405-552:
+//   initial guess -> loads -> velocities -> residual; if too large: J0^-1, then
+//   Broyden iterations (find_update, update, loads, velocities, residual,
+//   end_of_iter) until the residual meets TOL.
+// Velocities follow mobility_solver::compute_velocities (mobility_solver.cpp:2673-2758)
+// for a body held fixed: v_seg = Mss f_seg; v_blob = Mbs f_seg; GMRES for the blob
+// forces with Mbb f_blob = -v_blob; v_seg += Msb f_blob.  The Broyden low-rank
+// matrices C, D (broyden_solver.cpp:54-174) live in HBM as max_broyden columns of
+// length 6 n_seg; every vector operation runs on the GPU and only the scaled
+// residual norm (and GMRES residual estimates) come back to the host.
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+#include "ctx.hpp"
+#include "fil.hpp"
+#include "krylov.hpp"
+
+namespace cfm {
+
+Status launch_apply(cfm_ctx* c, const ApplyPlan& plan);
+int choose_split(const cfm_ctx* c, const ApplyPlan& pl);
+Status upload_aos(cfm_ctx* c, const double* host, int cnt, int dof, double* dst, int ld, int off);
+Status download_aos(cfm_ctx* c, double* host, int cnt, int dof, const double* src, int ld, int off);
+Status gnn_guess(cfm_ctx* c, const double* rhs, double* x0, bool* used);
+
+struct FilState {
+  cfm_fil_params prm{};
+  FilK K;
+  int nbroy = 0;
+  DevBuf<double> q, qm1, u, um1, xm1, xm2, lam, lam_m1, lam_m2, clamp, clamp_m1, clamp_m2, base,
+      pert, fint, jinv;
+  DevBuf<double> err, nerr, upd, C, D, coef, partial, bad_part, scal;
+  DevBuf<double> fb_prev, rhs_b;  // blob-force guess and blob rhs
+  DevBuf<int> status;
+  bool have_jinv = false;
+  double* host = nullptr;  // pinned read-back
+};
+
+static Status ensure_host(FilState* F) {
+  if (!F->host) CFM_HIP(hipHostMalloc((void**)&F->host, sizeof(double) * 8, hipHostMallocDefault));
+  return {};
+}
+
+static Status read_dev(cfm_ctx* c, FilState* F, const double* dev, double* out) {
+  CFM_TRY(ensure_host(F));
+  CFM_HIP(hipMemcpyAsync(F->host, dev, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  CFM_HIP(hipStreamSynchronize(c->stream));
+  *out = F->host[0];
+  return {};
+}
+
+static Status apply_plan(cfm_ctx* c, const ApplyPlan& pl, int prec) {
+  const int saved = c->cfg.precision;
+  c->cfg.precision = prec;
+  c->split = choose_split(c, pl);
+  Status s = launch_apply(c, pl);
+  c->cfg.precision = saved;
+  return s;
+}
+
+// v = M f for the current loads (segments from the filament loads, blob forces by GMRES)
+static Status velocities(cfm_ctx* c, FilState* F, cfm_step_stats* st) {
+  Particles& p = c->p;
+  const int prec = F->prm.pair_precision;
+  if (F->prm.barrier) CFM_TRY(fil_barrier(c, F->K));
+  if (p.n_blob == 0) {
+    ApplyPlan pl;
+    pl.tgt_begin = 0;
+    pl.tgt_end = p.n_seg;
+    pl.src_seg = true;
+    return apply_plan(c, pl, prec);
+  }
+  // segments and blobs from segment loads, one launch
+  ApplyPlan pl;
+  pl.tgt_begin = 0;
+  pl.tgt_end = p.n;
+  pl.src_seg = true;
+  CFM_TRY(apply_plan(c, pl, prec));
+  // blob forces: Mbb f_b = -v_b (body held fixed, so K V_body = 0)
+  const SystemDesc s = system_desc(c, CFM_SYS_BLOB);
+  CFM_TRY(F->rhs_b.alloc(s.L));
+  CFM_TRY(F->fb_prev.alloc(s.L));
+  CFM_TRY(dev_copy_rows(c, p.v.p, (size_t)p.n, p.n_seg, F->rhs_b.p, (size_t)s.m, 0, 3, s.m));
+  CFM_TRY(dev_axpby(c, -1.0, F->rhs_b.p, 0.0, F->rhs_b.p, F->rhs_b.p, s.L));
+  if (F->prm.guess == CFM_GUESS_LEARNED) {
+    bool used = false;
+    CFM_TRY(gnn_guess(c, F->rhs_b.p, F->fb_prev.p, &used));
+  }
+  cfm_solve_opts o = F->prm.blob_solve;
+  o.system = CFM_SYS_BLOB;
+  o.use_guess = 1;
+  if (!c->krylov) c->krylov = new Krylov();
+  cfm_solve_stats ss{};
+  // keep the segment velocities: the blob solve overwrites only blob rows of p.v
+  CFM_TRY(solve(c, *c->krylov, s, o, F->rhs_b.p, F->fb_prev.p, &ss, nullptr));
+  if (st) {
+    st->gmres_iterations += ss.iterations;
+    st->gmres_solves += 1;
+    st->max_gmres_rel_residual = std::max(st->max_gmres_rel_residual, ss.rel_residual);
+  }
+  // v_seg += Msb f_b
+  CFM_TRY(dev_copy_rows(c, F->fb_prev.p, (size_t)s.m, 0, p.f.p, (size_t)p.n, p.n_seg, 3, s.m));
+  ApplyPlan sb;
+  sb.tgt_begin = 0;
+  sb.tgt_end = p.n_seg;
+  sb.src_blob = true;
+  sb.acc_seg = true;
+  return apply_plan(c, sb, prec);
+}
+
+static Status residual(cfm_ctx* c, FilState* F, int nt, double* e, double* bad) {
+  CFM_TRY(fil_errors(c, F->K, nt, e, F->bad_part.p, F->scal.p));
+  return read_dev(c, F, F->scal.p, bad);
+}
+
+// out = -gamma J0^-1 e - sum_i (e . D_i) C_i over the first `it` columns
+static Status low_rank_apply(cfm_ctx* c, FilState* F, const double* e, double* out, int it) {
+  const int L = F->nbroy;
+  CFM_TRY(fil_jinv_mult(c, F->K, e, out, -F->prm.gamma));
+  if (it > 0) {
+    CFM_TRY(dev_dots(c, F->partial.p, F->D.p, (size_t)L, it, e, L, F->coef.p));
+    CFM_TRY(dev_gemv_sub(c, F->C.p, (size_t)L, it, F->coef.p, out, L));
+  }
+  return {};
+}
+
+static Status step(cfm_ctx* c, FilState* F, int nt, cfm_step_stats* st) {
+  const auto t0 = std::chrono::steady_clock::now();
+  cfm_step_stats S{};
+  const int L = F->nbroy;
+  const int maxit = F->prm.max_broyden;
+  CFM_TRY(fil_initial_guess(c, F->K, nt));
+  CFM_TRY(fil_forces(c, F->K, nt));
+  CFM_TRY(velocities(c, F, &S));
+  double bad = 0.0;
+  CFM_TRY(residual(c, F, nt, F->err.p, &bad));
+  if (!std::isfinite(bad)) return make_status(CFM_ENONFINITE, "non-finite residual at step %d", nt);
+  bool too_large = bad > 1.0;
+  if (too_large) {
+    CFM_HIP(hipMemsetAsync(F->status.p, 0, sizeof(int), c->stream));
+    CFM_TRY(fil_jacobian(c, F->K, nt, F->status.p));
+  }
+  int it = 0;
+  while (too_large && it < maxit) {
+    CFM_TRY(low_rank_apply(c, F, F->err.p, F->upd.p, it));
+    CFM_TRY(fil_update(c, F->K, F->upd.p));
+    CFM_TRY(fil_forces(c, F->K, nt));
+    CFM_TRY(velocities(c, F, &S));
+    CFM_TRY(residual(c, F, nt, F->nerr.p, &bad));
+    if (!std::isfinite(bad))
+      return make_status(CFM_ENONFINITE, "Broyden's method diverged after %d iterations during step %d",
+                         it + 1, nt + 1);
+    too_large = bad > 1.0;
+    if (too_large && it < maxit - 1) {
+      double* Ci = F->C.p + (size_t)it * L;
+      double* Di = F->D.p + (size_t)it * L;
+      CFM_TRY(low_rank_apply(c, F, F->nerr.p, Ci, it));
+      CFM_TRY(dev_axpby(c, 1.0, F->nerr.p, -1.0, F->err.p, Di, L));
+      CFM_TRY(dev_norm2(c, F->partial.p, Di, L, F->scal.p + 1));
+      CFM_TRY(dev_scale_by_inv(c, Ci, F->scal.p + 1, Ci, L));
+      CFM_TRY(dev_scale_by_inv(c, Di, F->scal.p + 1, Di, L));
+      std::swap(F->err.p, F->nerr.p);
+    }
+    ++it;
+  }
+  CFM_HIP(hipStreamSynchronize(c->stream));
+  S.broyden_iterations = it;
+  S.converged = !too_large;
+  S.final_error = bad;
+  S.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  if (st) *st = S;
+  return {};
+}
+
+static FilState* fil_of(cfm_ctx* c) { return reinterpret_cast<FilState*>(c->fil); }
+
+}  // namespace cfm
+
+using namespace cfm;
+
+static int fail_c(cfm_ctx* c, const Status& s) {
+  c->err = s.msg;
+  return s.code;
+}
+
+static bool finite_all(const double* a, size_t n) {
+  for (size_t i = 0; i < n; ++i)
+    if (!std::isfinite(a[i])) return false;
+  return true;
+}
+
+void cfm_fil_release(cfm_ctx* c) {
+  FilState* F = fil_of(c);
+  if (!F) return;
+  DevBuf<double>* bufs[] = {&F->q, &F->qm1, &F->u, &F->um1, &F->xm1, &F->xm2, &F->lam,
+                            &F->lam_m1, &F->lam_m2, &F->clamp, &F->clamp_m1, &F->clamp_m2,
+                            &F->base, &F->pert, &F->fint, &F->jinv, &F->err, &F->nerr, &F->upd,
+                            &F->C, &F->D, &F->coef, &F->partial, &F->bad_part, &F->scal,
+                            &F->fb_prev, &F->rhs_b};
+  for (auto* b : bufs) b->release();
+  F->status.release();
+  if (F->host) (void)hipHostFree(F->host);
+  delete F;
+  c->fil = nullptr;
+}
+
+extern "C" {
+
+int cfm_fil_init(cfm_ctx* c, const cfm_fil_params* prm, const double* q_seg, const double* pert) {
+  if (!c || !prm || !q_seg) return c ? fail_c(c, make_status(CFM_EINVAL, "null argument")) : CFM_EINVAL;
+  if (!c->have_positions) return fail_c(c, make_status(CFM_ESTATE, "positions not set"));
+  Particles& p = c->p;
+  if (p.seg_dof != 6) return fail_c(c, make_status(CFM_EINVAL, "filaments need seg_dof = 6"));
+  if (prm->nseg < 2 || prm->nseg > 21 || p.n_seg % prm->nseg != 0)
+    return fail_c(c, make_status(CFM_EINVAL, "nseg must be in [2, 21] and divide n_seg (%d)", p.n_seg));
+  if (prm->max_broyden < 1 || !(prm->dt > 0) || !(prm->dl > 0) || !(prm->tol > 0))
+    return fail_c(c, make_status(CFM_EINVAL, "need max_broyden >= 1 and dt, dl, tol > 0"));
+  if (prm->pair_precision != CFM_FP64 && prm->pair_precision != CFM_FP32C)
+    return fail_c(c, make_status(CFM_EINVAL, "unknown pair precision"));
+  if (!finite_all(q_seg, 4 * (size_t)p.n_seg)) return fail_c(c, make_status(CFM_ENONFINITE, "non-finite q"));
+  (void)hipSetDevice(c->device);
+  cfm_fil_release(c);
+  FilState* F = new FilState();
+  c->fil = F;
+  F->prm = *prm;
+  FilK& K = F->K;
+  K.nseg = prm->nseg;
+  K.nfil = p.n_seg / prm->nseg;
+  K.nseg_total = p.n_seg;
+  K.num_euler = prm->num_euler_steps;
+  K.dl = prm->dl;
+  K.dt = prm->dt;
+  K.kb = prm->kb;
+  K.kt = prm->kt;
+  K.mu = c->cfg.mu;
+  K.a = c->cfg.a_seg;
+  K.end_force = prm->end_force;
+  K.rep_factor = prm->repulsive_factor;
+  K.tol = prm->tol;
+  const size_t ns = p.n_seg, nf = K.nfil;
+  F->nbroy = 6 * p.n_seg;
+  auto run = [&]() -> Status {
+    CFM_TRY(F->q.alloc(4 * ns)); CFM_TRY(F->qm1.alloc(4 * ns));
+    CFM_TRY(F->u.alloc(3 * ns)); CFM_TRY(F->um1.alloc(3 * ns));
+    CFM_TRY(F->xm1.alloc(3 * ns)); CFM_TRY(F->xm2.alloc(3 * ns));
+    CFM_TRY(F->lam.alloc(3 * ns)); CFM_TRY(F->lam_m1.alloc(3 * ns)); CFM_TRY(F->lam_m2.alloc(3 * ns));
+    CFM_TRY(F->clamp.alloc(3 * nf)); CFM_TRY(F->clamp_m1.alloc(3 * nf)); CFM_TRY(F->clamp_m2.alloc(3 * nf));
+    CFM_TRY(F->base.alloc(3 * nf)); CFM_TRY(F->pert.alloc(3 * nf));
+    CFM_TRY(F->fint.alloc(6 * ns));
+    CFM_TRY(F->jinv.alloc((size_t)nf * 36 * K.nseg * K.nseg));
+    const int L = F->nbroy;
+    CFM_TRY(F->err.alloc(L)); CFM_TRY(F->nerr.alloc(L)); CFM_TRY(F->upd.alloc(L));
+    CFM_TRY(F->C.alloc((size_t)L * prm->max_broyden));
+    CFM_TRY(F->D.alloc((size_t)L * prm->max_broyden));
+    CFM_TRY(F->coef.alloc(prm->max_broyden + 1));
+    CFM_TRY(F->partial.alloc(dev_dots_partial_size(L, prm->max_broyden + 1)));
+    CFM_TRY(F->bad_part.alloc((p.n_seg + 255) / 256 + 1));
+    CFM_TRY(F->scal.alloc(8));
+    CFM_TRY(F->status.alloc(4));
+    if (p.n_blob) {
+      CFM_TRY(F->fb_prev.alloc(3 * (size_t)p.n_blob));
+      CFM_TRY(dev_fill(c, F->fb_prev.p, 0.0, 3 * p.n_blob));
+    }
+    DevBuf<double>* zero[] = {&F->u, &F->um1, &F->lam, &F->lam_m1, &F->lam_m2, &F->clamp,
+                              &F->clamp_m1, &F->clamp_m2, &F->pert};
+    for (auto* b : zero) CFM_TRY(dev_fill(c, b->p, 0.0, (int)b->n));
+    CFM_TRY(upload_aos(c, q_seg, p.n_seg, 4, F->q.p, p.n_seg, 0));
+    CFM_HIP(hipMemcpyAsync(F->qm1.p, F->q.p, sizeof(double) * 4 * ns, hipMemcpyDeviceToDevice, c->stream));
+    for (int k = 0; k < 3; ++k) {
+      const double* src = (k == 0 ? p.x.p : k == 1 ? p.y.p : p.z.p);
+      CFM_HIP(hipMemcpyAsync(F->xm1.p + k * ns, src, sizeof(double) * ns, hipMemcpyDeviceToDevice, c->stream));
+      CFM_HIP(hipMemcpyAsync(F->xm2.p + k * ns, src, sizeof(double) * ns, hipMemcpyDeviceToDevice, c->stream));
+    }
+    // base of filament f = position of its first segment
+    for (int k = 0; k < 3; ++k) {
+      const double* src = (k == 0 ? p.x.p : k == 1 ? p.y.p : p.z.p);
+      CFM_HIP(hipMemcpy2DAsync(F->base.p + k * nf, sizeof(double), src, sizeof(double) * K.nseg,
+                               sizeof(double), nf, hipMemcpyDeviceToDevice, c->stream));
+    }
+    if (pert) CFM_TRY(upload_aos(c, pert, (int)nf, 3, F->pert.p, (int)nf, 0));
+    CFM_HIP(hipStreamSynchronize(c->stream));
+    return {};
+  };
+  Status s = run();
+  if (!s.ok()) {
+    cfm_fil_release(c);
+    return fail_c(c, s);
+  }
+  K.x = p.x.p; K.y = p.y.p; K.z = p.z.p;
+  K.q = F->q.p; K.qm1 = F->qm1.p; K.u = F->u.p; K.um1 = F->um1.p;
+  K.xm1 = F->xm1.p; K.xm2 = F->xm2.p;
+  K.lam = F->lam.p; K.lam_m1 = F->lam_m1.p; K.lam_m2 = F->lam_m2.p;
+  K.clamp = F->clamp.p; K.clamp_m1 = F->clamp_m1.p; K.clamp_m2 = F->clamp_m2.p;
+  K.base = F->base.p; K.pert = F->pert.p; K.fint = F->fint.p; K.jinv = F->jinv.p;
+  return CFM_OK;
+}
+
+#define FIL_OR_FAIL(c)                                                              \
+  if (!c) return CFM_EINVAL;                                                        \
+  FilState* F = fil_of(c);                                                          \
+  if (!F) return fail_c(c, make_status(CFM_ESTATE, "cfm_fil_init has not been called")); \
+  (void)hipSetDevice(c->device)
+
+int cfm_fil_step(cfm_ctx* c, int32_t nt, cfm_step_stats* st) {
+  FIL_OR_FAIL(c);
+  if (nt < 0) return fail_c(c, make_status(CFM_EINVAL, "nt must be >= 0"));
+  Status s = step(c, F, nt, st);
+  return s.ok() ? CFM_OK : fail_c(c, s);
+}
+
+int cfm_fil_get_state(cfm_ctx* c, double* x, double* q, double* u, double* lam, double* clamp) {
+  FIL_OR_FAIL(c);
+  const int ns = c->p.n_seg;
+  auto run = [&]() -> Status {
+    if (x) {
+      // positions are the context's particle rows x, y, z (separate arrays)
+      std::vector<double> tmp(3 * (size_t)ns);
+      CFM_HIP(hipMemcpyAsync(tmp.data(), c->p.x.p, sizeof(double) * ns, hipMemcpyDeviceToHost, c->stream));
+      CFM_HIP(hipMemcpyAsync(tmp.data() + ns, c->p.y.p, sizeof(double) * ns, hipMemcpyDeviceToHost, c->stream));
+      CFM_HIP(hipMemcpyAsync(tmp.data() + 2 * ns, c->p.z.p, sizeof(double) * ns, hipMemcpyDeviceToHost, c->stream));
+      CFM_HIP(hipStreamSynchronize(c->stream));
+      for (int i = 0; i < ns; ++i)
+        for (int k = 0; k < 3; ++k) x[3 * i + k] = tmp[(size_t)k * ns + i];
+    }
+    if (q) CFM_TRY(download_aos(c, q, ns, 4, F->q.p, ns, 0));
+    if (u) CFM_TRY(download_aos(c, u, ns, 3, F->u.p, ns, 0));
+    if (lam) CFM_TRY(download_aos(c, lam, ns, 3, F->lam.p, ns, 0));
+    if (clamp) CFM_TRY(download_aos(c, clamp, F->K.nfil, 3, F->clamp.p, F->K.nfil, 0));
+    return {};
+  };
+  Status s = run();
+  return s.ok() ? CFM_OK : fail_c(c, s);
+}
+
+int cfm_fil_set_state(cfm_ctx* c, const double* q, const double* qm1, const double* u,
+                      const double* um1, const double* lam, const double* clamp,
+                      const double* xm1, const double* xm2) {
+  FIL_OR_FAIL(c);
+  const int ns = c->p.n_seg;
+  auto run = [&]() -> Status {
+    if (q) CFM_TRY(upload_aos(c, q, ns, 4, F->q.p, ns, 0));
+    if (qm1) CFM_TRY(upload_aos(c, qm1, ns, 4, F->qm1.p, ns, 0));
+    if (u) CFM_TRY(upload_aos(c, u, ns, 3, F->u.p, ns, 0));
+    if (um1) CFM_TRY(upload_aos(c, um1, ns, 3, F->um1.p, ns, 0));
+    if (lam) CFM_TRY(upload_aos(c, lam, ns, 3, F->lam.p, ns, 0));
+    if (clamp) CFM_TRY(upload_aos(c, clamp, F->K.nfil, 3, F->clamp.p, F->K.nfil, 0));
+    if (xm1) CFM_TRY(upload_aos(c, xm1, ns, 3, F->xm1.p, ns, 0));
+    if (xm2) CFM_TRY(upload_aos(c, xm2, ns, 3, F->xm2.p, ns, 0));
+    return {};
+  };
+  Status s = run();
+  return s.ok() ? CFM_OK : fail_c(c, s);
+}
+
+int cfm_fil_internal_forces(cfm_ctx* c, int32_t nt, double* f_seg) {
+  FIL_OR_FAIL(c);
+  if (!f_seg) return fail_c(c, make_status(CFM_EINVAL, "null output"));
+  auto run = [&]() -> Status {
+    CFM_TRY(fil_forces(c, F->K, nt));
+    return download_aos(c, f_seg, c->p.n_seg, 6, F->fint.p, c->p.n_seg, 0);
+  };
+  Status s = run();
+  return s.ok() ? CFM_OK : fail_c(c, s);
+}
+
+int cfm_barrier_forces(cfm_ctx* c, double* f_seg) {
+  FIL_OR_FAIL(c);
+  if (!f_seg) return fail_c(c, make_status(CFM_EINVAL, "null output"));
+  auto run = [&]() -> Status {
+    CFM_TRY(dev_fill(c, c->p.f.p, 0.0, 3 * c->p.n));
+    CFM_TRY(fil_barrier(c, F->K));
+    return download_aos(c, f_seg, c->p.n_seg, 3, c->p.f.p, c->p.n, 0);
+  };
+  Status s = run();
+  return s.ok() ? CFM_OK : fail_c(c, s);
+}
+
+int cfm_fil_jacobian_inverse(cfm_ctx* c, int32_t nt, double* jinv) {
+  FIL_OR_FAIL(c);
+  auto run = [&]() -> Status {
+    CFM_TRY(fil_forces(c, F->K, nt));
+    CFM_HIP(hipMemsetAsync(F->status.p, 0, sizeof(int), c->stream));
+    CFM_TRY(fil_jacobian(c, F->K, nt, F->status.p));
+    int stv = 0;
+    CFM_HIP(hipMemcpyAsync(&stv, F->status.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    CFM_HIP(hipStreamSynchronize(c->stream));
+    if (stv) return make_status(CFM_ENONFINITE, "singular approximate Jacobian");
+    if (jinv) {
+      const int N = 6 * F->K.nseg;
+      const size_t tot = (size_t)F->K.nfil * N * N;
+      std::vector<double> t(tot);
+      CFM_HIP(hipMemcpy(t.data(), F->jinv.p, sizeof(double) * tot, hipMemcpyDeviceToHost));
+      for (int f = 0; f < F->K.nfil; ++f)
+        for (int r = 0; r < N; ++r)
+          for (int cc = 0; cc < N; ++cc)
+            jinv[(size_t)f * N * N + (size_t)r * N + cc] = t[(size_t)f * N * N + (size_t)cc * N + r];
+    }
+    return {};
+  };
+  Status s = run();
+  return s.ok() ? CFM_OK : fail_c(c, s);
+}
+
+int cfm_fil_errors(cfm_ctx* c, int32_t nt, const double* v_seg, double* err, double* badness) {
+  FIL_OR_FAIL(c);
+  if (!v_seg || !err) return fail_c(c, make_status(CFM_EINVAL, "null argument"));
+  auto run = [&]() -> Status {
+    CFM_TRY(upload_aos(c, v_seg, c->p.n_seg, 6, c->p.v.p, c->p.n, 0));
+    double bad = 0.0;
+    CFM_TRY(residual(c, F, nt, F->err.p, &bad));
+    if (badness) *badness = bad;
+    CFM_HIP(hipMemcpy(err, F->err.p, sizeof(double) * F->nbroy, hipMemcpyDeviceToHost));
+    return {};
+  };
+  Status s = run();
+  return s.ok() ? CFM_OK : fail_c(c, s);
+}
+
+int cfm_fil_update(cfm_ctx* c, const double* upd) {
+  FIL_OR_FAIL(c);
+  if (!upd) return fail_c(c, make_status(CFM_EINVAL, "null argument"));
+  auto run = [&]() -> Status {
+    CFM_HIP(hipMemcpy(F->upd.p, upd, sizeof(double) * F->nbroy, hipMemcpyHostToDevice));
+    CFM_TRY(fil_update(c, F->K, F->upd.p));
+    CFM_HIP(hipStreamSynchronize(c->stream));
+    return {};
+  };
+  Status s = run();
+  return s.ok() ? CFM_OK : fail_c(c, s);
+}
+
+int cfm_fil_initial_guess(cfm_ctx* c, int32_t nt) {
+  FIL_OR_FAIL(c);
+  Status s = fil_initial_guess(c, F->K, nt);
+  if (s.ok()) {
+    hipError_t e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) s = make_status(CFM_EHIP, "%s", hipGetErrorString(e));
+  }
+  return s.ok() ? CFM_OK : fail_c(c, s);
+}
+
+}  // extern "C"

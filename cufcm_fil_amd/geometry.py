@@ -43,17 +43,21 @@ def loads(n, dof, seed):
     return np.random.default_rng(seed).standard_normal(dof * n)
 
 
-def tilted_lattice(nx=8, ny=8, spacing=10.0, nseg=NSEG, z0=1.1, max_polar=0.3, seed=0):
+def tilted_lattice(nx=8, ny=8, spacing=10.0, nseg=NSEG, z0=1.1, max_polar=0.3, seed=0,
+                   return_tangents=False):
     rng = np.random.default_rng(seed)
-    fils = []
+    fils, tans = [], []
     for ix in range(nx):
         for iy in range(ny):
             th = rng.uniform(0.0, max_polar)
             ph = rng.uniform(0.0, 2.0 * math.pi)
             t = np.array([math.sin(th) * math.cos(ph), math.sin(th) * math.sin(ph), math.cos(th)])
             base = np.array([ix * spacing, iy * spacing, z0])
-            fils.append(_chain(base, np.repeat(t[None], nseg, axis=0)))
-    return np.concatenate(fils).reshape(-1)
+            tt = np.repeat(t[None], nseg, axis=0)
+            fils.append(_chain(base, tt))
+            tans.append(tt)
+    x = np.concatenate(fils).reshape(-1)
+    return (x, np.concatenate(tans)) if return_tangents else x
 
 
 def _bent_tangents(nseg, amp_frac, phase, axis_u, axis_n, dl=DL):
@@ -65,16 +69,20 @@ def _bent_tangents(nseg, amp_frac, phase, axis_u, axis_n, dl=DL):
     return np.cos(theta)[:, None] * axis_u[None] + np.sin(theta)[:, None] * axis_n[None]
 
 
-def wave_lattice(nx=32, ny=20, spacing=8.0, nseg=NSEG, z0=1.1, amp_frac=0.2, wavelength=None):
+def wave_lattice(nx=32, ny=20, spacing=8.0, nseg=NSEG, z0=1.1, amp_frac=0.2, wavelength=None,
+                 return_tangents=False):
     wavelength = 8.0 * spacing if wavelength is None else wavelength
     ez, ex = np.array([0.0, 0.0, 1.0]), np.array([1.0, 0.0, 0.0])
-    fils = []
+    fils, tans = [], []
     for ix in range(nx):
         for iy in range(ny):
             base = np.array([ix * spacing, iy * spacing, z0])
             phase = 2.0 * math.pi * base[0] / wavelength
-            fils.append(_chain(base, _bent_tangents(nseg, amp_frac, phase, ez, ex)))
-    return np.concatenate(fils).reshape(-1)
+            tt = _bent_tangents(nseg, amp_frac, phase, ez, ex)
+            fils.append(_chain(base, tt))
+            tans.append(tt)
+    x = np.concatenate(fils).reshape(-1)
+    return (x, np.concatenate(tans)) if return_tangents else x
 
 
 def fibonacci_sphere(n, radius=1.0, offset=0.0):
@@ -87,13 +95,13 @@ def fibonacci_sphere(n, radius=1.0, offset=0.0):
 
 
 def sphere_over_wall(n_blob=6000, n_fil=1300, nseg=NSEG, radius=40.0, gap=50.0,
-                     amp_frac=0.08, wavelength=None):
-    """Returns (x_seg AoS, x_blob AoS, centre, filament base directions)."""
+                     amp_frac=0.08, wavelength=None, return_tangents=False):
+    """Returns (x_seg AoS, x_blob AoS, centre, filament base directions[, tangents])."""
     centre = np.array([0.0, 0.0, gap + radius])
     blobs = fibonacci_sphere(n_blob, radius) + centre
     dirs = fibonacci_sphere(n_fil, 1.0, offset=0.5)
     wavelength = 0.5 * math.pi * radius if wavelength is None else wavelength
-    fils = []
+    fils, tans = [], []
     for d in dirs:
         # beat plane spanned by the normal and the local polar direction
         pol = np.array([0.0, 0.0, -1.0]) + d[2] * d   # tangential part of -z
@@ -102,8 +110,11 @@ def sphere_over_wall(n_blob=6000, n_fil=1300, nseg=NSEG, radius=40.0, gap=50.0,
         polar_angle = math.acos(max(-1.0, min(1.0, d[2])))
         phase = 2.0 * math.pi * radius * polar_angle / wavelength   # metachronal wave
         base = centre + (radius + DL) * d
-        fils.append(_chain(base, _bent_tangents(nseg, amp_frac, phase, d, pol)))
-    return np.concatenate(fils).reshape(-1), blobs.reshape(-1), centre, dirs
+        tt = _bent_tangents(nseg, amp_frac, phase, d, pol)
+        fils.append(_chain(base, tt))
+        tans.append(tt)
+    out = (np.concatenate(fils).reshape(-1), blobs.reshape(-1), centre, dirs)
+    return out + (np.concatenate(tans),) if return_tangents else out
 
 
 def ensemble(n_sys=64, lo=100, hi=4000, seed=3):
@@ -124,3 +135,40 @@ def min_separation(x):
     d = np.sqrt(((p[:, None, :] - p[None, :, :]) ** 2).sum(-1))
     np.fill_diagonal(d, np.inf)
     return d.min()
+
+
+def _quat_mul(p, q):
+    return np.array([p[0] * q[0] - p[1:] @ q[1:],
+                     *(p[0] * q[1:] + q[0] * p[1:] + np.cross(p[1:], q[1:]))])
+
+
+def quat_from_x_to(t):
+    """Unit quaternion (s, v) of the rotation taking the x axis to the unit vector t
+    (the construction of filament::initial_setup, filament.cpp: qtemp.sqrt_in_place_ortho)."""
+    t = np.asarray(t, float) / np.linalg.norm(t)
+    q = np.array([t[0], 0.0, -t[2], t[1]])
+    if q[0] < -0.999999:
+        q[0] = -0.9999999
+    s = math.sqrt(0.5 * (1.0 + q[0]))
+    q = np.concatenate([[s], q[1:] / (2.0 * s)])
+    return q / np.linalg.norm(q)
+
+
+def segment_quaternions(tangents, nseg=NSEG):
+    """Segment orientations whose first frame vector is the given tangent: the base
+    from quat_from_x_to, each next one by the minimal rotation between successive
+    tangents (no twist), AoS (s, v0, v1, v2) per segment."""
+    T = np.asarray(tangents, float).reshape(-1, nseg, 3)
+    out = np.empty((T.shape[0], nseg, 4))
+    for f in range(T.shape[0]):
+        q = quat_from_x_to(T[f, 0])
+        out[f, 0] = q
+        for k in range(1, nseg):
+            a = T[f, k - 1] / np.linalg.norm(T[f, k - 1])
+            b = T[f, k] / np.linalg.norm(T[f, k])
+            r = np.concatenate([[1.0 + a @ b], np.cross(a, b)])
+            r /= np.linalg.norm(r)
+            q = _quat_mul(r, q)
+            q /= np.linalg.norm(q)
+            out[f, k] = q
+    return out.reshape(-1)

@@ -147,3 +147,12 @@ class RpyMobilitySolver:
         L.check(self._h, L.lib().cfm_apply_preconditioner(self._h, self._SYS[system],
                                                            self._PC[precond], L.ptr(v), L.ptr(out)))
         return out
+
+    # -- measurement --------------------------------------------------------------
+    def profile(self, enable=True):
+        L.check(self._h, L.lib().cfm_profile(self._h, int(bool(enable))))
+
+    def profile_read(self):
+        st = L.cfm_profile_stats()
+        L.check(self._h, L.lib().cfm_profile_read(self._h, ctypes.byref(st)))
+        return st.as_dict()

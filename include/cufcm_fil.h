@@ -142,6 +142,103 @@ int cfm_gmres(cfm_ctx* ctx, const cfm_solve_opts* opts, const double* rhs, doubl
 int cfm_apply_preconditioner(cfm_ctx* ctx, int32_t system, int32_t precond, const double* in,
                              double* out);
 
+/* ---- implicit filament step (Broyden) ------------------------------------- */
+/* Tethered elastic filaments with a clamped base and a follower force at the tip
+ * (the reference's INSTABILITY_CILIA model), optionally next to a rigid body that
+ * is held fixed (prescribed zero velocity, the reference's default
+ * BODY_VELOCITY_TYPE 1).  Segments are numbered filament by filament, base first;
+ * n_seg = nfil * nseg. */
+
+#define CFM_GUESS_PREVIOUS 0   /* blob-force GMRES guess = previous solution (mobility_solver.cpp:2296-2388) */
+#define CFM_GUESS_LEARNED 1    /* learned initial guess (cfm_gnn_*), previous solution as fallback */
+
+typedef struct cfm_fil_params {
+  int32_t nseg;            /* segments per filament (NSEG), <= 21 */
+  int32_t max_broyden;     /* MAX_BROYDEN_ITER (config.hpp:357) */
+  int32_t num_euler_steps; /* NUM_EULER_STEPS (config.hpp:382) */
+  int32_t barrier;         /* 1: add barrier forces before each product (apply_interparticle_forces) */
+  double dl;               /* DL: segment spacing */
+  double dt;               /* DT: time step */
+  double kb, kt;           /* bending and twist moduli (config.hpp:320-321) */
+  double end_force;        /* END_FORCE_MAGNITUDE (follower force at the tip) */
+  double repulsive_factor; /* REPULSIVE_FORCE_FACTOR (config.hpp:462) */
+  double tol;              /* Broyden TOL (config.hpp:358) */
+  double gamma;            /* JACOBIAN_CONFIDENCE_FACTOR (config.hpp:355) */
+  int32_t pair_precision;  /* CFM_FP64 / CFM_FP32C for the velocity products */
+  int32_t guess;           /* CFM_GUESS_* for the blob-force GMRES */
+  cfm_solve_opts blob_solve; /* inner GMRES on the body blobs (system field ignored) */
+} cfm_fil_params;
+
+typedef struct cfm_step_stats {
+  int32_t broyden_iterations;
+  int32_t gmres_iterations;  /* Arnoldi steps summed over the step's blob solves */
+  int32_t gmres_solves;
+  int32_t converged;         /* 1 if the Broyden residual met TOL */
+  double final_error;        /* max of |e_x|/(DL TOL), |e_u|/TOL at exit (<= 1 when converged) */
+  double max_gmres_rel_residual;  /* worst FP64 relative residual of the step's blob solves */
+  double seconds;            /* host wall time of the step (stream synchronised) */
+} cfm_step_stats;
+
+/* Set up the filament state on a context whose segment positions were given with
+ * cfm_set_positions: q_seg[4*n_seg] unit quaternions (s, v0, v1, v2) per segment,
+ * pert[3*nfil] (or NULL) the force added to segment 4 of each filament at nt = 0.
+ * Replaces filament::initial_setup (filament.cpp:62-627, elastic branch). */
+int cfm_fil_init(cfm_ctx* ctx, const cfm_fil_params* params, const double* q_seg,
+                 const double* pert);
+
+/* One implicit BDF time step nt (backward Euler for nt < num_euler_steps) solved by
+ * Broyden's method: the loop of cilia_sim_main.cu:405-552 with compute_velocities
+ * (mobility_solver.cpp:2673-2758), compute_errors (:2760-3045), find_update /
+ * end_of_iter (broyden_solver.cpp:71-174) and the filament updates, all on the
+ * device.  Returns CFM_ENONFINITE if the iteration diverges. */
+int cfm_fil_step(cfm_ctx* ctx, int32_t nt, cfm_step_stats* stats);
+
+/* Read / overwrite the filament state (any pointer may be NULL).  Arrays are AoS:
+ * x[3*n_seg], q/qm1[4*n_seg], u/um1/xm1/xm2/lam[3*n_seg] (lam row 0 of a filament
+ * is its tether multiplier), clamp[3*nfil]. */
+int cfm_fil_get_state(cfm_ctx* ctx, double* x, double* q, double* u, double* lam, double* clamp);
+int cfm_fil_set_state(cfm_ctx* ctx, const double* q, const double* qm1, const double* u,
+                      const double* um1, const double* lam, const double* clamp,
+                      const double* xm1, const double* xm2);
+
+/* Pieces of the step, exposed for parity checks:
+ *   internal loads f_seg[6*n_seg] of the current state (internal_forces_and_torques),
+ *   barrier forces added to zero loads f_seg[3*n_seg] (apply_interparticle_forces),
+ *   inverse approximate Jacobians jinv[nfil*(6 nseg)^2], row-major per filament
+ *     (filament::invert_approx_jacobian; uses the current internal loads),
+ *   BDF residual err[6*n_seg] for given velocities v_seg[6*n_seg] (compute_errors)
+ *     and its scaled max norm,
+ *   state update by upd[6*n_seg] (swimmer::update), initial guess of step nt. */
+int cfm_fil_internal_forces(cfm_ctx* ctx, int32_t nt, double* f_seg);
+int cfm_barrier_forces(cfm_ctx* ctx, double* f_seg);
+int cfm_fil_jacobian_inverse(cfm_ctx* ctx, int32_t nt, double* jinv);
+int cfm_fil_errors(cfm_ctx* ctx, int32_t nt, const double* v_seg, double* err, double* badness);
+int cfm_fil_update(cfm_ctx* ctx, const double* upd);
+int cfm_fil_initial_guess(cfm_ctx* ctx, int32_t nt);
+
+/* ---- measurement --------------------------------------------------------- */
+typedef struct cfm_profile_stats {
+  int64_t launches;        /* pair-kernel launches recorded since the last reset */
+  double kernel_ms;        /* summed HIP-event durations of those launches */
+  double alg_flops;        /* algorithmic flops of those launches (FLOPS_PAIR_* x pairs) */
+  double alg_bytes;        /* algorithmic HBM bytes (positions + loads in, velocities out) */
+  int64_t launches_fp32;   /* of which FP32C */
+  double kernel_ms_fp32;
+  double alg_flops_fp32;
+} cfm_profile_stats;
+
+/* Algorithmic flops per pair of the wall branch (add/sub/mul = 1, FMA = 2,
+ * rsqrt = 1, div = 1), counted on the canonical formula of cuda_functions.cu:65-201. */
+#define CFM_FLOPS_PAIR_66 321
+#define CFM_FLOPS_PAIR_63 197
+#define CFM_FLOPS_PAIR_36 203
+#define CFM_FLOPS_PAIR_33 136
+
+/* Record every pair-kernel launch between HIP events on the context's stream
+ * (enable = 1), stop (0).  Enabling resets the counters. (No reference counterpart.) */
+int cfm_profile(cfm_ctx* ctx, int32_t enable);
+int cfm_profile_read(cfm_ctx* ctx, cfm_profile_stats* stats);
+
 #ifdef __cplusplus
 }
 #endif

@@ -483,3 +483,54 @@ class Broyden:
             error = new_error
         self.iter += 1
         return error
+
+
+# ---------------------------------------------------------------- full step
+def badness(e, P: Params, nfil):
+    """max(|e_x| / (DL TOL), |e_u| / TOL) over the reference-layout error vector."""
+    E = np.asarray(e).reshape(nfil, 2, -1)
+    return max(np.abs(E[:, 0]).max() / (P.dl * P.tol), np.abs(E[:, 1]).max() / P.tol)
+
+
+def implicit_step(fils, nt, velocity_fn, barrier=True, x_blob=None, max_iter=400):
+    """One BDF/Broyden step (cilia_sim_main.cu:405-552) over a list of filaments.
+    velocity_fn(x_seg (n,3), F_seg (n,6)) -> (n,6) velocities.  Returns iterations."""
+    P = fils[0].P
+    n = P.nseg
+
+    def loads():
+        F = np.concatenate([f.internal_forces(nt) for f in fils])
+        if barrier:
+            xs = np.concatenate([f.x for f in fils])
+            F[:, :3] += barrier_forces(xs, n, P, x_blob)
+        return F
+
+    def resid():
+        xs = np.concatenate([f.x for f in fils])
+        V = velocity_fn(xs, loads())
+        return np.concatenate([f.errors(V[i * n:(i + 1) * n], nt) for i, f in enumerate(fils)])
+
+    for f in fils:
+        f.initial_guess(nt)
+    e = resid()
+    big = badness(e, P, len(fils)) > 1.0
+    jinvs = None
+    if big:
+        jinvs = [np.linalg.inv(f.approx_jacobian(nt, f.internal_forces(nt))) for f in fils]
+
+    def jm(r):
+        return np.concatenate([J @ r[i * 6 * n:(i + 1) * 6 * n] for i, J in enumerate(jinvs)])
+
+    B = Broyden(P.gamma, max_iter)
+    it = 0
+    while big and it < max_iter:
+        upd = B.find_update(jm, e)
+        for i, f in enumerate(fils):
+            f.update(upd[i * 6 * n:(i + 1) * 6 * n])
+        e_new = resid()
+        if not np.all(np.isfinite(e_new)):
+            raise FloatingPointError("Broyden diverged")
+        big = badness(e_new, P, len(fils)) > 1.0
+        e = B.end_of_iter(jm, e, e_new, big)
+        it += 1
+    return it
