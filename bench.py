@@ -9,9 +9,11 @@ velocities of all segments, with the blob forces from a GMRES solve to a relativ
 FP64 residual of 1e-10.
 
 Two paths are timed back to back on the same initial state:
-  * plain   -- FP64 pair sums, FP64 GMRES, previous blob forces as initial guess;
-  * mixed   -- FP32 compensated pair sums inside FP64 iterative refinement (and the
-               learned initial guess when --learned is given).
+  * plain   -- FP64 pair sums, FP64 GMRES with the reference's 6 pi mu a diagonal
+               preconditioner, previous blob forces as initial guess;
+  * mixed   -- FP32 compensated pair sums inside FP64 iterative refinement, the
+               near-field block preconditioner on the blob solve (and the learned
+               correction / initial guess when --learned is given).
 `value` is the steps/s of the mixed path; `plain_steps_per_s` and the ratio are
 reported beside it.
 
@@ -36,7 +38,7 @@ sys.path.insert(0, ROOT)
 FP32_VALU_PEAK_TFLOPS = 157.3   # MI355X vector FP32 (published)
 
 
-def build_system(args, precision, solve_mode, guess):
+def build_system(args, precision, solve_mode, guess, precond):
     import numpy as np
 
     from cufcm_fil_amd import geometry as geo
@@ -50,11 +52,11 @@ def build_system(args, precision, solve_mode, guess):
     pert = (0.01 * ef * d).reshape(-1)
     return FilamentSystem(xs, tans, x_blob=xb, precision=precision, solve_mode=solve_mode,
                           guess=guess, pert=pert, blob_rtol=1e-10, blob_restart=50,
-                          blob_max_iter=5000, blob_precond="diag")
+                          blob_max_iter=5000, blob_precond=precond)
 
 
-def run_path(args, label, precision, solve_mode, guess):
-    S = build_system(args, precision, solve_mode, guess)
+def run_path(args, label, precision, solve_mode, guess, precond):
+    S = build_system(args, precision, solve_mode, guess, precond)
     try:
         for nt in range(args.warmup):
             st = S.step(nt)
@@ -139,8 +141,8 @@ def main():
         sys.exit(2)
 
     guess = "learned" if args.learned else "previous"
-    mixed = run_path(args, "mixed", "fp32c", "mixed", guess)
-    plain = None if args.skip_plain else run_path(args, "plain", "fp64", "fp64", "previous")
+    mixed = run_path(args, "mixed", "fp32c", "mixed", guess, "near")
+    plain = None if args.skip_plain else run_path(args, "plain", "fp64", "fp64", "previous", "diag")
 
     prof = mixed["profile"]
     kms = prof["kernel_ms_fp32"] / max(1, prof["launches_fp32"])

@@ -19,7 +19,7 @@ CFM_OK, CFM_EINVAL, CFM_EHIP, CFM_ENOMEM, CFM_ESTATE, CFM_ENOCONV, CFM_ENONFINIT
 CFM_FP64, CFM_FP32C = 0, 1
 CFM_SS, CFM_SB, CFM_BB, CFM_BS, CFM_FULL = range(5)
 CFM_SYS_SEG, CFM_SYS_BLOB = 0, 1
-CFM_PC_NONE, CFM_PC_DIAG, CFM_PC_SELF = 0, 1, 2
+CFM_PC_NONE, CFM_PC_DIAG, CFM_PC_SELF, CFM_PC_NEAR = 0, 1, 2, 3
 CFM_SOLVE_FP64, CFM_SOLVE_MIXED = 0, 1
 
 

@@ -82,6 +82,23 @@ struct Particles {
   DevBuf<double> partial;     // split x 6 x n partial sums
 };
 
+// Near-field block preconditioner data (nearfield.hip): NF_K nearest blobs of each
+// blob and the 3x3 blocks of its local inverse, [m][NF_K+1][3][3].
+#define NF_K 16
+struct NearField {
+  bool valid = false;
+  int m = 0;
+  DevBuf<int> nbr;
+  DevBuf<double> W;
+  DevBuf<int> status;
+  void release() {
+    nbr.release();
+    W.release();
+    status.release();
+    valid = false;
+  }
+};
+
 // Kernel-side launch description of one mobility product.
 struct ApplyPlan {
   int tgt_begin = 0, tgt_end = 0;    // target range (combined ids)
@@ -107,6 +124,8 @@ struct cfm_ctx {
   cfm::Krylov* krylov = nullptr;        // GMRES workspace (created on first solve)
   cfm::DevBuf<double> vec_a, vec_b;     // system-vector staging
   void* fil = nullptr;                  // filament state of the implicit step (step.hip)
+  cfm::NearField nf;                    // near-field preconditioner of the blob system
+  const double* nf_corr = nullptr;      // learned correction blocks (same layout as nf.W) or null
   // pair-kernel profiling (HIP events around each launch)
   bool prof_on = false;
   std::vector<hipEvent_t> prof_events;  // pool, two per launch

@@ -18,6 +18,7 @@
 
 #include "ctx.hpp"
 #include "krylov.hpp"
+#include "nearfield.hpp"
 
 namespace cfm {
 
@@ -310,6 +311,11 @@ Status system_apply(cfm_ctx* c, const SystemDesc& s, int precision, const double
 }
 
 Status system_precond(cfm_ctx* c, const SystemDesc& s, int kind, const double* x, double* y) {
+  if (kind == CFM_PC_NEAR) {
+    if (s.which != CFM_BB) return make_status(CFM_EINVAL, "CFM_PC_NEAR applies to the blob system only");
+    if (!c->nf.valid) CFM_TRY(nearfield_build(c, c->nf));
+    return nearfield_apply(c, c->nf, c->nf_corr, x, y);
+  }
   hipLaunchKernelGGL(precond_kernel, g1(s.m), dim3(256), 0, c->stream, kind, c->cfg.wall, s.dof,
                      s.m, s.off, c->p.z.p, s.a, c->cfg.mu, x, y);
   CFM_CHECK_LAUNCH();

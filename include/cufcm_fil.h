@@ -103,6 +103,9 @@ int cfm_time_apply(cfm_ctx* ctx, int32_t which, int32_t reps, double* ms_per_app
 #define CFM_PC_NONE 0
 #define CFM_PC_DIAG 1   /* scalar 6 pi mu a (and 8 pi mu a^3 on torques): mobility_solver.cpp:1108-1114,1550-1556 */
 #define CFM_PC_SELF 2   /* inverse of each particle's own (wall-corrected) self-mobility block */
+#define CFM_PC_NEAR 3   /* blob system only: local inverse over each blob and its 16 nearest
+                           blobs (near-field block preconditioner), plus the learned correction
+                           when one is loaded (cfm_gnn_*) */
 
 /* arithmetic of the solve */
 #define CFM_SOLVE_FP64 0   /* FP64 pair sums throughout */
