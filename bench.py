@@ -140,7 +140,7 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
-    guess = "learned" if args.learned else "previous"
+    guess = "learned" if args.learned else "projected"
     mixed = run_path(args, "mixed", "fp32c", "mixed", guess, "near")
     plain = None if args.skip_plain else run_path(args, "plain", "fp64", "fp64", "previous", "diag")
 

@@ -71,7 +71,7 @@ class cfm_solve_stats(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
 
 
-CFM_GUESS_PREVIOUS, CFM_GUESS_LEARNED = 0, 1
+CFM_GUESS_PREVIOUS, CFM_GUESS_LEARNED, CFM_GUESS_PROJECTED = 0, 1, 2
 
 
 class cfm_fil_params(ctypes.Structure):

@@ -46,10 +46,11 @@ struct KArgs {
   const double* __restrict__ x;
   const double* __restrict__ y;
   const double* __restrict__ z;
-  const double* __restrict__ f;   // 6 x n
-  double* __restrict__ v;         // 6 x n
+  const double* __restrict__ f;   // loads: row k of particle j at f[k*fld + j - foff]
+  double* __restrict__ v;         // velocities: v[k*vld + j - voff]
   double* __restrict__ partial;   // split x 6 x n
   int n, n_seg, n_blob, seg_dof;
+  int fld, foff, vld, voff;
   int tgt_begin, tgt_end;
   int seg_tiles;                  // target tiles that hold segments
   int src_seg, src_blob;
@@ -120,7 +121,7 @@ __device__ __forceinline__ void sweep_species(const KArgs& a, SrcRec<T>* tile, i
       r.z = a.z[j];
       if constexpr (sizeof(T) == 4) r.zf = float(r.z);
 #pragma unroll
-      for (int k = 0; k < FD; ++k) r.f[k] = T(a.f[(size_t)k * n + j]);
+      for (int k = 0; k < FD; ++k) r.f[k] = T(a.f[(size_t)k * a.fld + j - a.foff]);
 #pragma unroll
       for (int k = FD; k < 6; ++k) r.f[k] = T(0);
       tile[threadIdx.x] = r;
@@ -165,7 +166,7 @@ __device__ __forceinline__ void add_self(const KArgs& a, int i, double zi, doubl
   if (!a.wall || !own) return;
   double F[6] = {0, 0, 0, 0, 0, 0};
 #pragma unroll
-  for (int k = 0; k < TD; ++k) F[k] = a.f[(size_t)k * a.n + i];
+  for (int k = 0; k < TD; ++k) F[k] = a.f[(size_t)k * a.fld + i - a.foff];
   wall_self<TD, TD>(zi, F, out);
 }
 
@@ -221,7 +222,7 @@ __device__ void apply_body(const KArgs& a, SrcRec<T>* tile, double (*red)[kTT], 
   const bool acc = tgt_seg ? a.acc_seg : a.acc_blob;
 #pragma unroll
   for (int k = 0; k < TD; ++k) {
-    double* dst = a.v + (size_t)k * n + i;
+    double* dst = a.v + (size_t)k * a.vld + i - a.voff;
     *dst = acc ? *dst + scale * out[k] : scale * out[k];
   }
 }
@@ -263,7 +264,7 @@ __global__ __launch_bounds__(256) void rpy_finalize_kernel(KArgs a) {
   const double scale = a.wall ? 1.0 / a.mu : 1.0;
   const bool acc = is_seg ? a.acc_seg : a.acc_blob;
   for (int k = 0; k < td; ++k) {
-    double* dst = a.v + (size_t)k * n + i;
+    double* dst = a.v + (size_t)k * a.vld + i - a.voff;
     *dst = acc ? *dst + scale * out[k] : scale * out[k];
   }
 }
@@ -319,6 +320,20 @@ Status launch_apply(cfm_ctx* c, const ApplyPlan& plan) {
   a.z = p.z.p;
   a.f = p.f.p;
   a.v = p.v.p;
+  a.fld = p.n;
+  a.foff = 0;
+  a.vld = p.n;
+  a.voff = 0;
+  if (plan.f_ext) {
+    a.f = plan.f_ext;
+    a.fld = plan.f_ld;
+    a.foff = plan.f_off;
+  }
+  if (plan.v_ext) {
+    a.v = plan.v_ext;
+    a.vld = plan.v_ld;
+    a.voff = plan.v_off;
+  }
   a.partial = p.partial.p;
   a.n = p.n;
   a.n_seg = p.n_seg;

@@ -104,6 +104,12 @@ struct ApplyPlan {
   int tgt_begin = 0, tgt_end = 0;    // target range (combined ids)
   bool src_seg = false, src_blob = false;
   bool acc_seg = false, acc_blob = false;  // accumulate into v instead of overwrite
+  // Optional external load / velocity blocks: row k of particle j (combined id)
+  // lives at ext[k * ld + j - off].  Null: the context's own f / v arrays.
+  const double* f_ext = nullptr;
+  int f_ld = 0, f_off = 0;
+  double* v_ext = nullptr;
+  int v_ld = 0, v_off = 0;
 };
 
 }  // namespace cfm

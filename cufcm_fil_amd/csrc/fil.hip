@@ -518,11 +518,51 @@ __global__ void fil_update_kernel(FilK K, const double* __restrict__ upd) {
   robot_arm(K, f);
 }
 
-// Barrier (short-range repulsion) forces added to the segment loads in fout.
+// Bounding boxes of the source tiles (256 consecutive particles: segments, then blobs)
+// used to skip tiles that cannot hold a barrier partner.
+__global__ __launch_bounds__(256) void tile_bbox_kernel(const FilK K, const double* __restrict__ bx,
+                                                        const double* __restrict__ by,
+                                                        const double* __restrict__ bz, int nblob,
+                                                        double* __restrict__ box) {
+  __shared__ double lo[3][256], hi[3][256];
+  const int ns = (int)K.nseg_total;
+  const int j = blockIdx.x * 256 + threadIdx.x;
+  double p[3];
+  bool v = true;
+  if (j < ns) {
+    p[0] = K.x[j]; p[1] = K.y[j]; p[2] = K.z[j];
+  } else if (j < ns + nblob) {
+    p[0] = bx[j - ns]; p[1] = by[j - ns]; p[2] = bz[j - ns];
+  } else {
+    v = false;
+  }
+  for (int c = 0; c < 3; ++c) {
+    lo[c][threadIdx.x] = v ? p[c] : 1e300;
+    hi[c][threadIdx.x] = v ? p[c] : -1e300;
+  }
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o)
+      for (int c = 0; c < 3; ++c) {
+        lo[c][threadIdx.x] = fmin(lo[c][threadIdx.x], lo[c][threadIdx.x + o]);
+        hi[c][threadIdx.x] = fmax(hi[c][threadIdx.x], hi[c][threadIdx.x + o]);
+      }
+    __syncthreads();
+  }
+  if (threadIdx.x < 3) {
+    box[blockIdx.x * 6 + threadIdx.x] = lo[threadIdx.x][0];
+    box[blockIdx.x * 6 + 3 + threadIdx.x] = hi[threadIdx.x][0];
+  }
+}
+
+// Barrier (short-range repulsion) forces added to the segment loads in fout.  The
+// workgroup's 256 targets are one tile of the same tiling, so tile pairs whose
+// boxes are farther apart than the cut-off are skipped as a whole.
 __global__ __launch_bounds__(256) void barrier_kernel(FilK K, const double* __restrict__ bx,
                                                       const double* __restrict__ by,
                                                       const double* __restrict__ bz, int nblob,
-                                                      int wall, double* __restrict__ fout, int ld) {
+                                                      int wall, const double* __restrict__ box,
+                                                      double* __restrict__ fout, int ld) {
   __shared__ double sx[256], sy[256], sz[256];
   const int ns = (int)K.nseg_total;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -537,7 +577,17 @@ __global__ __launch_bounds__(256) void barrier_kernel(FilK K, const double* __re
     fz += fmag * w * w * w * w;
   }
   const int ntot = ns + nblob;
-  for (int t0 = 0; t0 < ntot; t0 += 256) {
+  const double a_sum = 2.0 * K.a;
+  const double cut = 1.1 * a_sum, chi = 10.0 / a_sum;
+  const double* tb = box + blockIdx.x * 6;
+  for (int t0 = 0, tile = 0; t0 < ntot; t0 += 256, ++tile) {
+    const double* sb = box + tile * 6;
+    double gap2 = 0.0;
+    for (int c = 0; c < 3; ++c) {
+      const double g = fmax(0.0, fmax(sb[c] - tb[3 + c], tb[c] - sb[3 + c]));
+      gap2 += g * g;
+    }
+    if (gap2 >= cut * cut) continue;  // uniform across the workgroup
     const int j = t0 + threadIdx.x;
     __syncthreads();
     if (j < ns) {
@@ -551,8 +601,6 @@ __global__ __launch_bounds__(256) void barrier_kernel(FilK K, const double* __re
     }
     __syncthreads();
     const int cnt = min(256, ntot - t0);
-    const double a_sum = 2.0 * K.a;
-    const double cut = 1.1 * a_sum, chi = 10.0 / a_sum;
     for (int jj = 0; jj < cnt; ++jj) {
       const int j2 = t0 + jj;
       const double dx = xi - sx[jj], dy = yi - sy[jj], dz = zi - sz[jj];
@@ -591,10 +639,13 @@ Status fil_forces(cfm_ctx* c, const FilK& K, int nt) {
   return {};
 }
 
-Status fil_barrier(cfm_ctx* c, const FilK& K) {
+Status fil_barrier(cfm_ctx* c, const FilK& K, double* box) {
   const Particles& p = c->p;
+  const int ntiles = (p.n_seg + p.n_blob + 255) / 256;
+  hipLaunchKernelGGL(tile_bbox_kernel, dim3(ntiles), dim3(256), 0, c->stream, K, p.x.p + p.n_seg,
+                     p.y.p + p.n_seg, p.z.p + p.n_seg, p.n_blob, box);
   hipLaunchKernelGGL(barrier_kernel, gsz((int)K.nseg_total, 256), dim3(256), 0, c->stream, K,
-                     p.x.p + p.n_seg, p.y.p + p.n_seg, p.z.p + p.n_seg, p.n_blob, c->cfg.wall,
+                     p.x.p + p.n_seg, p.y.p + p.n_seg, p.z.p + p.n_seg, p.n_blob, c->cfg.wall, box,
                      p.f.p, p.n);
   CFM_CHECK_LAUNCH();
   return {};

@@ -30,7 +30,8 @@ struct FilK {
 
 Status fil_initial_guess(cfm_ctx* c, const FilK& K, int nt);
 Status fil_forces(cfm_ctx* c, const FilK& K, int nt);
-Status fil_barrier(cfm_ctx* c, const FilK& K);
+// box: scratch of 6 * ceil((n_seg + n_blob) / 256) doubles
+Status fil_barrier(cfm_ctx* c, const FilK& K, double* box);
 Status fil_errors(cfm_ctx* c, const FilK& K, int nt, double* err, double* bad_part,
                   double* bad_out);
 Status fil_jacobian(cfm_ctx* c, const FilK& K, int nt, int* status_dev);

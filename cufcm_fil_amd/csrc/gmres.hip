@@ -111,6 +111,37 @@ __global__ void copy_rows_kernel(const double* __restrict__ src, size_t src_ld, 
   for (int k = 0; k < dof; ++k) dst[(size_t)k * dst_ld + dst_off + i] = src[(size_t)k * src_ld + src_off + i];
 }
 
+// h[i] (+)= sum_b partial[b*k+i] (every block reduces the partials in the same
+// order), then w[r] -= sum_{i<k} h_i Q[i][r]; block 0 stores h.
+__global__ __launch_bounds__(256) void cgs_finish_sub_kernel(const double* __restrict__ Q,
+                                                             size_t ldq, int k,
+                                                             const double* __restrict__ partial,
+                                                             int nb, double* __restrict__ h_out,
+                                                             const double* __restrict__ h_prev,
+                                                             double* __restrict__ w, int L) {
+  __shared__ double hs[256];
+  for (int i = threadIdx.x; i < k; i += blockDim.x) {
+    double s = 0.0;
+    for (int b = 0; b < nb; ++b) s += partial[(size_t)b * k + i];
+    hs[i] = s;
+  }
+  __syncthreads();
+  if (blockIdx.x == 0)
+    for (int i = threadIdx.x; i < k; i += blockDim.x) h_out[i] = h_prev ? h_prev[i] + hs[i] : hs[i];
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= L) return;
+  double s = 0.0;
+  for (int i = 0; i < k; ++i) s += hs[i] * Q[(size_t)i * ldq + r];
+  w[r] -= s;
+}
+
+// Finishes ||w|| from its partials, then one Givens step on column j (below).
+__global__ void norm_givens_kernel(const double* __restrict__ partial, int nb,
+                                   double* __restrict__ H, int ldh, double* __restrict__ cs,
+                                   double* __restrict__ sn, double* __restrict__ beta,
+                                   const double* __restrict__ hcol, double* __restrict__ hnorm,
+                                   int j, double* __restrict__ res);
+
 // One Givens step on column j of H (ld = m+1 rows, column-major), the reference's
 // formulas (mobility_solver.cpp:2467-2506); writes |beta_{j+1}| to res.
 __global__ void givens_kernel(double* __restrict__ H, int ldh, double* __restrict__ cs,
@@ -145,6 +176,46 @@ __global__ void givens_kernel(double* __restrict__ H, int ldh, double* __restric
   col[j] = c * a + s * b;
   col[j + 1] = 0.0;
   beta[j + 1] = -s * beta[j];
+  beta[j] = c * beta[j];
+  res[0] = fabs(beta[j + 1]);
+}
+
+__global__ void norm_givens_kernel(const double* __restrict__ partial, int nb,
+                                   double* __restrict__ H, int ldh, double* __restrict__ cs,
+                                   double* __restrict__ sn, double* __restrict__ beta,
+                                   const double* __restrict__ hcol, double* __restrict__ hnorm,
+                                   int j, double* __restrict__ res) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double s = 0.0;
+  for (int b = 0; b < nb; ++b) s += partial[b];
+  hnorm[0] = sqrt(fmax(s, 0.0));
+  double* col = H + (size_t)j * ldh;
+  for (int i = 0; i <= j; ++i) col[i] = hcol[i];
+  col[j + 1] = hnorm[0];
+  for (int i = 1; i <= j; ++i) {
+    const double t = cs[i - 1] * col[i - 1] + sn[i - 1] * col[i];
+    col[i] = cs[i - 1] * col[i] - sn[i - 1] * col[i - 1];
+    col[i - 1] = t;
+  }
+  const double a = col[j], b = col[j + 1];
+  double c, sg;
+  if (b == 0.0) {
+    c = 1.0;
+    sg = 0.0;
+  } else if (fabs(b) > fabs(a)) {
+    const double t = a / b;
+    sg = ((b > 0.0) ? 1.0 : -1.0) / sqrt(1.0 + t * t);
+    c = t * sg;
+  } else {
+    const double t = b / a;
+    c = ((a > 0.0) ? 1.0 : -1.0) / sqrt(1.0 + t * t);
+    sg = t * c;
+  }
+  cs[j] = c;
+  sn[j] = sg;
+  col[j] = c * a + sg * b;
+  col[j + 1] = 0.0;
+  beta[j + 1] = -sg * beta[j];
   beta[j] = c * beta[j];
   res[0] = fabs(beta[j + 1]);
 }
@@ -232,14 +303,19 @@ Status Krylov::reserve(int L_, int m_) {
   CFM_TRY(d.alloc(L));
   const int nb = (L + kChunk - 1) / kChunk;
   CFM_TRY(partial.alloc((size_t)nb * (m + 2)));
-  if (!host_res) CFM_HIP(hipHostMalloc((void**)&host_res, sizeof(double) * 4, hipHostMallocDefault));
+  CFM_TRY(res_ring.alloc(4));
+  if (!host_res) CFM_HIP(hipHostMalloc((void**)&host_res, sizeof(double) * 8, hipHostMallocDefault));
+  if (!ev[0])
+    for (int i = 0; i < 4; ++i) CFM_HIP(hipEventCreateWithFlags(&ev[i], hipEventDisableTiming));
   return {};
 }
 
 void Krylov::release() {
   Q.release(); H.release(); cs.release(); sn.release(); beta.release(); h.release();
   h2.release(); y.release(); scal.release(); r.release(); w.release(); z.release();
-  t.release(); d.release(); partial.release();
+  t.release(); d.release(); partial.release(); res_ring.release();
+  for (int i = 0; i < 4; ++i)
+    if (ev[i]) (void)hipEventDestroy(ev[i]);
   if (host_res) (void)hipHostFree(host_res);
   host_res = nullptr;
   L = m = 0;
@@ -291,23 +367,22 @@ SystemDesc system_desc(const cfm_ctx* c, int system) {
   return s;
 }
 
-// y = A x for a system vector (layout [dof][m]); precision CFM_FP64 / CFM_FP32C
+// y = A x for a system vector (layout [dof][m]); precision CFM_FP64 / CFM_FP32C.
+// The pair kernel reads x and writes y in place (no staging copies).
 Status system_apply(cfm_ctx* c, const SystemDesc& s, int precision, const double* x, double* y) {
-  Particles& p = c->p;
-  hipLaunchKernelGGL(copy_rows_kernel, g1(s.m), dim3(256), 0, c->stream, x, (size_t)s.m, 0, p.f.p,
-                     (size_t)p.n, s.off, s.dof, s.m);
-  CFM_CHECK_LAUNCH();
   const int saved = c->cfg.precision;
   c->cfg.precision = precision;
-  const ApplyPlan pl = plan_for(c, s.which);
+  ApplyPlan pl = plan_for(c, s.which);
+  pl.f_ext = x;
+  pl.f_ld = s.m;
+  pl.f_off = s.off;
+  pl.v_ext = y;
+  pl.v_ld = s.m;
+  pl.v_off = s.off;
   c->split = choose_split(c, pl);
   Status st = launch_apply(c, pl);
   c->cfg.precision = saved;
-  CFM_TRY(st);
-  hipLaunchKernelGGL(copy_rows_kernel, g1(s.m), dim3(256), 0, c->stream, p.v.p, (size_t)p.n, s.off,
-                     y, (size_t)s.m, 0, s.dof, s.m);
-  CFM_CHECK_LAUNCH();
-  return {};
+  return st;
 }
 
 Status system_precond(cfm_ctx* c, const SystemDesc& s, int kind, const double* x, double* y) {
@@ -348,31 +423,39 @@ Status gmres_core(cfm_ctx* c, Krylov& K, const SystemDesc& s, int prec, int pc, 
     hipLaunchKernelGGL(beta_init_kernel, dim3(1), dim3(64), 0, c->stream, K.beta.p, m + 1, K.scal.p);
     CFM_CHECK_LAUNCH();
     int k = 0;
-    double est = nr;
-    for (int j = 0; j < m; ++j) {
+    const int nb = (L + kChunk - 1) / kChunk;
+    // Residual estimates are copied to a pinned ring and read one step late, so
+    // the host never waits for the step it has just queued.
+    int pending = -1;
+    bool stop = false;
+    for (int j = 0; j < m && !stop; ++j) {
       double* qj = K.Q.p + (size_t)j * ldq;
       double* qn = K.Q.p + (size_t)(j + 1) * ldq;
       CFM_TRY(system_precond(c, s, pc, qj, K.z.p));
       if (hooks && hooks->precond_correction) CFM_TRY(hooks->precond_correction(c, s, qj, K.z.p, hooks->user));
       CFM_TRY(system_apply(c, s, prec, K.z.p, K.w.p));
       // classical Gram-Schmidt, twice
-      CFM_TRY(dots(c, K, K.Q.p, ldq, j + 1, K.w.p, L, K.h.p, false));
-      hipLaunchKernelGGL(gemv_sub_kernel, g1(L), dim3(256), 0, c->stream, K.Q.p, ldq, j + 1, K.h.p, K.w.p, L);
-      CFM_CHECK_LAUNCH();
-      CFM_TRY(dots(c, K, K.Q.p, ldq, j + 1, K.w.p, L, K.h2.p, false));
-      hipLaunchKernelGGL(gemv_sub_kernel, g1(L), dim3(256), 0, c->stream, K.Q.p, ldq, j + 1, K.h2.p, K.w.p, L);
-      hipLaunchKernelGGL(axpby_kernel, dim3(1), dim3(256), 0, c->stream, 1.0, K.h.p, 1.0, K.h2.p, K.h.p, j + 1);
-      CFM_CHECK_LAUNCH();
-      CFM_TRY(norm2(c, K, K.w.p, L, K.scal.p + 1));
+      hipLaunchKernelGGL(dots_partial_kernel, dim3(j + 1, nb), dim3(kRB), 0, c->stream, K.Q.p, ldq, j + 1, K.w.p, L, K.partial.p);
+      hipLaunchKernelGGL(cgs_finish_sub_kernel, g1(L), dim3(256), 0, c->stream, K.Q.p, ldq, j + 1, K.partial.p, nb, K.h.p, (const double*)nullptr, K.w.p, L);
+      hipLaunchKernelGGL(dots_partial_kernel, dim3(j + 1, nb), dim3(kRB), 0, c->stream, K.Q.p, ldq, j + 1, K.w.p, L, K.partial.p);
+      hipLaunchKernelGGL(cgs_finish_sub_kernel, g1(L), dim3(256), 0, c->stream, K.Q.p, ldq, j + 1, K.partial.p, nb, K.h2.p, (const double*)K.h.p, K.w.p, L);
+      hipLaunchKernelGGL(dots_partial_kernel, dim3(1, nb), dim3(kRB), 0, c->stream, K.w.p, (size_t)0, 1, K.w.p, L, K.partial.p);
+      hipLaunchKernelGGL(norm_givens_kernel, dim3(1), dim3(1), 0, c->stream, K.partial.p, nb, K.H.p, ldh, K.cs.p, K.sn.p,
+                         K.beta.p, K.h2.p, K.scal.p + 1, j, K.res_ring.p + (j % 4));
       hipLaunchKernelGGL(scale_by_inv_kernel, g1(L), dim3(256), 0, c->stream, K.w.p, K.scal.p + 1, qn, L);
-      hipLaunchKernelGGL(givens_kernel, dim3(1), dim3(1), 0, c->stream, K.H.p, ldh, K.cs.p, K.sn.p,
-                         K.beta.p, K.h.p, K.scal.p + 1, j, K.scal.p + 2);
       CFM_CHECK_LAUNCH();
-      CFM_TRY(read_scalar(c, K, K.scal.p + 2, &est));
+      CFM_HIP(hipMemcpyAsync(K.host_res + 4 + (j % 4), K.res_ring.p + (j % 4), sizeof(double), hipMemcpyDeviceToHost, c->stream));
+      CFM_HIP(hipEventRecord(K.ev[j % 4], c->stream));
       ++*iters;
       k = j + 1;
-      if (!std::isfinite(est)) return make_status(CFM_ENONFINITE, "non-finite GMRES iterate");
-      if (est <= target_abs || *iters >= max_iter) break;
+      if (pending >= 0) {
+        CFM_HIP(hipEventSynchronize(K.ev[pending % 4]));
+        const double est = K.host_res[4 + pending % 4];
+        if (!std::isfinite(est)) return make_status(CFM_ENONFINITE, "non-finite GMRES iterate");
+        if (est <= target_abs) stop = true;
+      }
+      pending = j;
+      if (*iters >= max_iter) stop = true;
     }
     // x += P^-1 Q y
     hipLaunchKernelGGL(backsub_kernel, dim3(1), dim3(1), 0, c->stream, K.H.p, ldh, K.beta.p, k, K.y.p);
@@ -440,11 +523,11 @@ Status solve(cfm_ctx* c, Krylov& K, const SystemDesc& s, const cfm_solve_opts& o
                                         max_iter, &iters, &res, hooks));
     nr = res;
   } else {
-    // iterative refinement: d solves A32 d = r to a relative 2e-6 (or what is left
+    // iterative refinement: d solves A32 d = r to a relative 5e-7 (or what is left
     // to reach the target), x += d, r = b - A64 x.
     int refinements = 0;
     while (nr > target && iters < max_iter && refinements < 12) {
-      const double rel_in = std::max(2e-6, 0.5 * target / nr);
+      const double rel_in = std::max(5e-7, 0.5 * target / nr);
       hipLaunchKernelGGL(fill_kernel, g1(L), dim3(256), 0, c->stream, K.t.p, 0.0, L);
       CFM_CHECK_LAUNCH();
       // K.d holds r; the inner solve overwrites K.r/K.w/K.z, so keep r in K.d
@@ -502,6 +585,11 @@ Status dev_gemv_sub(cfm_ctx* c, const double* Q, size_t ldq, int k, const double
 }
 Status dev_axpby(cfm_ctx* c, double a, const double* x, double b, const double* y, double* out, int L) {
   hipLaunchKernelGGL(axpby_kernel, g1(L), dim3(256), 0, c->stream, a, x, b, y, out, L);
+  CFM_CHECK_LAUNCH();
+  return {};
+}
+Status dev_gemv(cfm_ctx* c, const double* Q, size_t ldq, int k, const double* y, double* out, int L) {
+  hipLaunchKernelGGL(gemv_kernel, g1(L), dim3(256), 0, c->stream, Q, ldq, k, y, out, L);
   CFM_CHECK_LAUNCH();
   return {};
 }

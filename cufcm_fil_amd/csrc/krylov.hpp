@@ -7,8 +7,9 @@ namespace cfm {
 // Krylov workspace (device), sized for vectors of length L and m basis vectors.
 struct Krylov {
   int L = 0, m = 0;
-  DevBuf<double> Q, H, cs, sn, beta, h, h2, y, scal, r, w, z, t, d, dx, rhs, partial;
+  DevBuf<double> Q, H, cs, sn, beta, h, h2, y, scal, r, w, z, t, d, dx, rhs, partial, res_ring;
   double* host_res = nullptr;  // pinned scalar read-back
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   Status reserve(int L_, int m_);
   void release();
 };
@@ -49,6 +50,7 @@ size_t dev_dots_partial_size(int L, int k);
 Status dev_gemv_sub(cfm_ctx* c, const double* Q, size_t ldq, int k, const double* h, double* w, int L);
 Status dev_axpby(cfm_ctx* c, double a, const double* x, double b, const double* y, double* out, int L);
 Status dev_fill(cfm_ctx* c, double* x, double v, int L);
+Status dev_gemv(cfm_ctx* c, const double* Q, size_t ldq, int k, const double* y, double* out, int L);
 Status dev_scale_by_inv(cfm_ctx* c, const double* x, const double* s, double* out, int L);
 Status dev_norm2(cfm_ctx* c, double* partial, const double* w, int L, double* out);
 Status dev_copy_rows(cfm_ctx* c, const double* src, size_t src_ld, int src_off, double* dst,

@@ -26,6 +26,77 @@ Status upload_aos(cfm_ctx* c, const double* host, int cnt, int dof, double* dst,
 Status download_aos(cfm_ctx* c, double* host, int cnt, int dof, const double* src, int ld, int off);
 Status gnn_guess(cfm_ctx* c, const double* rhs, double* x0, bool* used);
 
+// Ring of past (rhs, solution) pairs of the blob system and the Gram matrix of
+// the stored right-hand sides; the guess minimises ||b - sum_i c_i b_i|| (with a
+// tiny Tikhonov term) and returns sum_i c_i x_i.  Because A x_i = b_i to the
+// solver tolerance, its residual is the projection residual of b.
+struct History {
+  static constexpr int H = 16;
+  int L = 0, count = 0, head = 0;
+  DevBuf<double> B, X, g, coef, partial;
+  double G[H][H] = {};
+  double* host = nullptr;
+};
+
+static Status history_guess(cfm_ctx* c, History& h, const double* b, double* x0, bool* used) {
+  *used = false;
+  if (h.count < 2) return {};
+  const int k = h.count;
+  CFM_TRY(dev_dots(c, h.partial.p, h.B.p, (size_t)h.L, k, b, h.L, h.g.p));
+  CFM_HIP(hipMemcpyAsync(h.host, h.g.p, sizeof(double) * k, hipMemcpyDeviceToHost, c->stream));
+  CFM_HIP(hipStreamSynchronize(c->stream));
+  // solve (G + d I) c = g by Gaussian elimination with partial pivoting (k <= 16)
+  double A[History::H][History::H + 1];
+  double tr = 0.0;
+  for (int i = 0; i < k; ++i) tr += h.G[i][i];
+  const double d = 1e-13 * tr / k;
+  for (int i = 0; i < k; ++i) {
+    for (int j = 0; j < k; ++j) A[i][j] = h.G[i][j] + (i == j ? d : 0.0);
+    A[i][k] = h.host[i];
+  }
+  for (int col = 0; col < k; ++col) {
+    int p = col;
+    for (int r = col + 1; r < k; ++r)
+      if (std::fabs(A[r][col]) > std::fabs(A[p][col])) p = r;
+    if (!(std::fabs(A[p][col]) > 0.0)) return {};
+    if (p != col)
+      for (int j = 0; j <= k; ++j) std::swap(A[p][j], A[col][j]);
+    for (int r = col + 1; r < k; ++r) {
+      const double f = A[r][col] / A[col][col];
+      for (int j = col; j <= k; ++j) A[r][j] -= f * A[col][j];
+    }
+  }
+  double cvec[History::H];
+  for (int i = k - 1; i >= 0; --i) {
+    double s = A[i][k];
+    for (int j = i + 1; j < k; ++j) s -= A[i][j] * cvec[j];
+    cvec[i] = s / A[i][i];
+    if (!std::isfinite(cvec[i])) return {};
+  }
+  CFM_HIP(hipMemcpyAsync(h.coef.p, cvec, sizeof(double) * k, hipMemcpyHostToDevice, c->stream));
+  CFM_TRY(dev_gemv(c, h.X.p, (size_t)h.L, k, h.coef.p, x0, h.L));
+  CFM_HIP(hipStreamSynchronize(c->stream));  // cvec is a host temporary
+  *used = true;
+  return {};
+}
+
+static Status history_push(cfm_ctx* c, History& h, const double* b, const double* x) {
+  const int slot = h.head;
+  CFM_HIP(hipMemcpyAsync(h.B.p + (size_t)slot * h.L, b, sizeof(double) * h.L, hipMemcpyDeviceToDevice, c->stream));
+  CFM_HIP(hipMemcpyAsync(h.X.p + (size_t)slot * h.L, x, sizeof(double) * h.L, hipMemcpyDeviceToDevice, c->stream));
+  h.head = (h.head + 1) % History::H;
+  h.count = std::min(h.count + 1, History::H);
+  // new row/column of the Gram matrix
+  CFM_TRY(dev_dots(c, h.partial.p, h.B.p, (size_t)h.L, h.count, h.B.p + (size_t)slot * h.L, h.L, h.g.p));
+  CFM_HIP(hipMemcpyAsync(h.host, h.g.p, sizeof(double) * h.count, hipMemcpyDeviceToHost, c->stream));
+  CFM_HIP(hipStreamSynchronize(c->stream));
+  for (int i = 0; i < h.count; ++i) {
+    h.G[slot][i] = h.host[i];
+    h.G[i][slot] = h.host[i];
+  }
+  return {};
+}
+
 struct FilState {
   cfm_fil_params prm{};
   FilK K;
@@ -34,6 +105,8 @@ struct FilState {
       pert, fint, jinv;
   DevBuf<double> err, nerr, upd, C, D, coef, partial, bad_part, scal;
   DevBuf<double> fb_prev, rhs_b;  // blob-force guess and blob rhs
+  DevBuf<double> box;             // barrier tile bounding boxes
+  History hist;                   // past blob solves (projected guess)
   DevBuf<int> status;
   bool have_jinv = false;
   double* host = nullptr;  // pinned read-back
@@ -65,7 +138,7 @@ static Status apply_plan(cfm_ctx* c, const ApplyPlan& pl, int prec) {
 static Status velocities(cfm_ctx* c, FilState* F, cfm_step_stats* st) {
   Particles& p = c->p;
   const int prec = F->prm.pair_precision;
-  if (F->prm.barrier) CFM_TRY(fil_barrier(c, F->K));
+  if (F->prm.barrier) CFM_TRY(fil_barrier(c, F->K, F->box.p));
   if (p.n_blob == 0) {
     ApplyPlan pl;
     pl.tgt_begin = 0;
@@ -85,9 +158,20 @@ static Status velocities(cfm_ctx* c, FilState* F, cfm_step_stats* st) {
   CFM_TRY(F->fb_prev.alloc(s.L));
   CFM_TRY(dev_copy_rows(c, p.v.p, (size_t)p.n, p.n_seg, F->rhs_b.p, (size_t)s.m, 0, 3, s.m));
   CFM_TRY(dev_axpby(c, -1.0, F->rhs_b.p, 0.0, F->rhs_b.p, F->rhs_b.p, s.L));
-  if (F->prm.guess == CFM_GUESS_LEARNED) {
+  if (F->prm.guess == CFM_GUESS_PROJECTED || F->prm.guess == CFM_GUESS_LEARNED) {
+    History& h = F->hist;
+    if (h.L != s.L) {
+      h.L = s.L;
+      h.count = h.head = 0;
+      CFM_TRY(h.B.alloc((size_t)History::H * s.L));
+      CFM_TRY(h.X.alloc((size_t)History::H * s.L));
+      CFM_TRY(h.g.alloc(History::H));
+      CFM_TRY(h.coef.alloc(History::H));
+      CFM_TRY(h.partial.alloc(dev_dots_partial_size(s.L, History::H)));
+      if (!h.host) CFM_HIP(hipHostMalloc((void**)&h.host, sizeof(double) * History::H, hipHostMallocDefault));
+    }
     bool used = false;
-    CFM_TRY(gnn_guess(c, F->rhs_b.p, F->fb_prev.p, &used));
+    CFM_TRY(history_guess(c, h, F->rhs_b.p, F->fb_prev.p, &used));
   }
   cfm_solve_opts o = F->prm.blob_solve;
   o.system = CFM_SYS_BLOB;
@@ -96,6 +180,8 @@ static Status velocities(cfm_ctx* c, FilState* F, cfm_step_stats* st) {
   cfm_solve_stats ss{};
   // keep the segment velocities: the blob solve overwrites only blob rows of p.v
   CFM_TRY(solve(c, *c->krylov, s, o, F->rhs_b.p, F->fb_prev.p, &ss, nullptr));
+  if (F->prm.guess == CFM_GUESS_PROJECTED || F->prm.guess == CFM_GUESS_LEARNED)
+    CFM_TRY(history_push(c, F->hist, F->rhs_b.p, F->fb_prev.p));
   if (st) {
     st->gmres_iterations += ss.iterations;
     st->gmres_solves += 1;
@@ -199,9 +285,15 @@ void cfm_fil_release(cfm_ctx* c) {
                             &F->lam_m1, &F->lam_m2, &F->clamp, &F->clamp_m1, &F->clamp_m2,
                             &F->base, &F->pert, &F->fint, &F->jinv, &F->err, &F->nerr, &F->upd,
                             &F->C, &F->D, &F->coef, &F->partial, &F->bad_part, &F->scal,
-                            &F->fb_prev, &F->rhs_b};
+                            &F->fb_prev, &F->rhs_b, &F->box};
   for (auto* b : bufs) b->release();
   F->status.release();
+  F->hist.B.release();
+  F->hist.X.release();
+  F->hist.g.release();
+  F->hist.coef.release();
+  F->hist.partial.release();
+  if (F->hist.host) (void)hipHostFree(F->hist.host);
   if (F->host) (void)hipHostFree(F->host);
   delete F;
   c->fil = nullptr;
@@ -259,6 +351,7 @@ int cfm_fil_init(cfm_ctx* c, const cfm_fil_params* prm, const double* q_seg, con
     CFM_TRY(F->partial.alloc(dev_dots_partial_size(L, prm->max_broyden + 1)));
     CFM_TRY(F->bad_part.alloc((p.n_seg + 255) / 256 + 1));
     CFM_TRY(F->scal.alloc(8));
+    CFM_TRY(F->box.alloc(6 * (size_t)((p.n_seg + p.n_blob + 255) / 256)));
     CFM_TRY(F->status.alloc(4));
     if (p.n_blob) {
       CFM_TRY(F->fb_prev.alloc(3 * (size_t)p.n_blob));
@@ -371,7 +464,7 @@ int cfm_barrier_forces(cfm_ctx* c, double* f_seg) {
   if (!f_seg) return fail_c(c, make_status(CFM_EINVAL, "null output"));
   auto run = [&]() -> Status {
     CFM_TRY(dev_fill(c, c->p.f.p, 0.0, 3 * c->p.n));
-    CFM_TRY(fil_barrier(c, F->K));
+    CFM_TRY(fil_barrier(c, F->K, F->box.p));
     return download_aos(c, f_seg, c->p.n_seg, 3, c->p.f.p, c->p.n, 0);
   };
   Status s = run();

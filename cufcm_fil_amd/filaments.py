@@ -61,7 +61,8 @@ class FilamentSystem:
         P.tol = prm["tol"]
         P.gamma = prm["gamma"]
         P.pair_precision = RpyMobilitySolver._prec(precision)
-        P.guess = {"previous": L.CFM_GUESS_PREVIOUS, "learned": L.CFM_GUESS_LEARNED}[guess]
+        P.guess = {"previous": L.CFM_GUESS_PREVIOUS, "learned": L.CFM_GUESS_LEARNED,
+                   "projected": L.CFM_GUESS_PROJECTED}[guess]
         P.blob_solve = L.cfm_solve_opts(system=L.CFM_SYS_BLOB,
                                         precond=RpyMobilitySolver._PC[blob_precond],
                                         mode=RpyMobilitySolver._MODE[solve_mode],

@@ -153,7 +153,9 @@ int cfm_apply_preconditioner(cfm_ctx* ctx, int32_t system, int32_t precond, cons
  * n_seg = nfil * nseg. */
 
 #define CFM_GUESS_PREVIOUS 0   /* blob-force GMRES guess = previous solution (mobility_solver.cpp:2296-2388) */
-#define CFM_GUESS_LEARNED 1    /* learned initial guess (cfm_gnn_*), previous solution as fallback */
+#define CFM_GUESS_LEARNED 1    /* history projection + learned near-field preconditioner correction */
+#define CFM_GUESS_PROJECTED 2  /* minimal-residual combination of the last 16 (rhs, solution) pairs
+                                  (the blob matrix is constant while the body is held fixed) */
 
 typedef struct cfm_fil_params {
   int32_t nseg;            /* segments per filament (NSEG), <= 21 */

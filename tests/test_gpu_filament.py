@@ -163,3 +163,20 @@ def test_mixed_precision_step_reaches_tolerances():
         for nt in range(2):
             st = S.step(nt)
             assert st["converged"] and st["final_error"] <= 1.0
+
+
+def test_projected_guess_reduces_gmres_work():
+    xs, xb, c, d, t = geo.sphere_over_wall(n_blob=800, n_fil=30, radius=14.0, gap=50.0,
+                                            return_tangents=True)
+    out = {}
+    for guess in ("previous", "projected"):
+        with FilamentSystem(xs, t, x_blob=xb, guess=guess, blob_precond="near") as S:
+            its = 0
+            for nt in range(3):
+                st = S.step(nt)
+                assert st["converged"] and st["max_gmres_rel_residual"] <= 1e-10
+                its += st["gmres_iterations"]
+            out[guess] = (its, S.state()["x"])
+    assert out["projected"][0] < out["previous"][0], out
+    # both converge to the same state within the Broyden tolerance
+    assert np.abs(out["projected"][1] - out["previous"][1]).max() < 1e-3
