@@ -168,6 +168,21 @@ def lib():
     L.cfm_fil_errors.argtypes = [_vp, ctypes.c_int32, _dp, _dp, _dp]
     L.cfm_fil_update.argtypes = [_vp, _dp]
     L.cfm_fil_initial_guess.argtypes = [_vp, ctypes.c_int32]
+    _fp = ctypes.POINTER(ctypes.c_float)
+    _ip = ctypes.POINTER(ctypes.c_int32)
+    L.cfm_gnn_init.argtypes = [_vp, ctypes.c_uint32, ctypes.c_int32]
+    L.cfm_gnn_num_params.argtypes = [_vp]
+    L.cfm_gnn_record.argtypes = [_vp, ctypes.c_int32]
+    L.cfm_gnn_samples.argtypes = [_vp]
+    L.cfm_gnn_add_samples.argtypes = [_vp, ctypes.c_int32, _dp, _dp]
+    L.cfm_gnn_train.argtypes = [_vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_double, _dp, _dp]
+    L.cfm_gnn_enable.argtypes = [_vp, ctypes.c_int32]
+    L.cfm_gnn_get_params.argtypes = [_vp, _fp]
+    L.cfm_gnn_set_params.argtypes = [_vp, _fp]
+    L.cfm_gnn_correction.argtypes = [_vp, _dp, _ip]
+    L.cfm_gnn_gradient.argtypes = [_vp, ctypes.c_int32, ctypes.c_int32, _fp, _dp]
+    L.cfm_gemm_bf16.argtypes = [_vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _fp,
+                                ctypes.c_int32, _fp, ctypes.c_int32, _fp]
     L.cfm_profile.argtypes = [_vp, ctypes.c_int32]
     L.cfm_profile_read.argtypes = [_vp, ctypes.POINTER(cfm_profile_stats)]
     _lib = L
@@ -191,3 +206,11 @@ def check(ctx, code):
     if code != CFM_OK:
         msg = lib().cfm_last_error(ctx)
         raise CfmError(code, msg.decode() if msg else "")
+
+
+def fptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def iptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))

@@ -25,6 +25,7 @@ int choose_split(const cfm_ctx* c, const ApplyPlan& pl);
 Status upload_aos(cfm_ctx* c, const double* host, int cnt, int dof, double* dst, int ld, int off);
 Status download_aos(cfm_ctx* c, double* host, int cnt, int dof, const double* src, int ld, int off);
 Status gnn_guess(cfm_ctx* c, const double* rhs, double* x0, bool* used);
+Status gnn_record(cfm_ctx* c, const double* b, const double* x);
 
 // Ring of past (rhs, solution) pairs of the blob system and the Gram matrix of
 // the stored right-hand sides; the guess minimises ||b - sum_i c_i b_i|| (with a
@@ -182,6 +183,7 @@ static Status velocities(cfm_ctx* c, FilState* F, cfm_step_stats* st) {
   CFM_TRY(solve(c, *c->krylov, s, o, F->rhs_b.p, F->fb_prev.p, &ss, nullptr));
   if (F->prm.guess == CFM_GUESS_PROJECTED || F->prm.guess == CFM_GUESS_LEARNED)
     CFM_TRY(history_push(c, F->hist, F->rhs_b.p, F->fb_prev.p));
+  CFM_TRY(gnn_record(c, F->rhs_b.p, F->fb_prev.p));
   if (st) {
     st->gmres_iterations += ss.iterations;
     st->gmres_solves += 1;

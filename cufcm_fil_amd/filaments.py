@@ -107,6 +107,10 @@ class FilamentSystem:
         a = [None if v is None else L.f64(v) for v in (q, qm1, u, um1, lam, clamp, xm1, xm2)]
         L.check(self.h, L.lib().cfm_fil_set_state(self.h, *[L.ptr(v) for v in a]))
 
+    def blob_rhs_solution_pairs(self):
+        """Number of (rhs, solution) pairs recorded by the learned correction."""
+        return self.mob.gnn_samples()
+
     # -- pieces (parity checks) ---------------------------------------------------
     def internal_forces(self, nt):
         f = np.zeros(6 * self.n_seg)

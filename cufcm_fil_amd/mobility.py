@@ -157,3 +157,61 @@ class RpyMobilitySolver:
         st = L.cfm_profile_stats()
         L.check(self._h, L.lib().cfm_profile_read(self._h, ctypes.byref(st)))
         return st.as_dict()
+
+    # -- learned near-field correction of the blob preconditioner ----------------
+    def gnn_init(self, seed=0, capacity=256):
+        L.check(self._h, L.lib().cfm_gnn_init(self._h, int(seed), int(capacity)))
+        return L.lib().cfm_gnn_num_params(self._h)
+
+    def gnn_record(self, on=True):
+        L.check(self._h, L.lib().cfm_gnn_record(self._h, int(bool(on))))
+
+    def gnn_samples(self):
+        return L.lib().cfm_gnn_samples(self._h)
+
+    def gnn_add_samples(self, rhs, sol=None):
+        rhs = L.f64(rhs)
+        n = rhs.size // (3 * self.n_blob)
+        sol = None if sol is None else L.f64(sol, rhs.size)
+        L.check(self._h, L.lib().cfm_gnn_add_samples(self._h, n, L.ptr(rhs), L.ptr(sol)))
+
+    def gnn_train(self, epochs=20, batch=8, lr=1e-3):
+        l0, l1 = ctypes.c_double(), ctypes.c_double()
+        L.check(self._h, L.lib().cfm_gnn_train(self._h, int(epochs), int(batch), float(lr),
+                                               ctypes.byref(l0), ctypes.byref(l1)))
+        return l0.value, l1.value
+
+    def gnn_enable(self, on=True):
+        L.check(self._h, L.lib().cfm_gnn_enable(self._h, int(bool(on))))
+
+    def gnn_params(self):
+        th = np.zeros(L.lib().cfm_gnn_num_params(self._h), dtype=np.float32)
+        L.check(self._h, L.lib().cfm_gnn_get_params(self._h, L.fptr(th)))
+        return th
+
+    def gnn_set_params(self, theta):
+        th = np.ascontiguousarray(theta, dtype=np.float32)
+        L.check(self._h, L.lib().cfm_gnn_set_params(self._h, L.fptr(th)))
+
+    def gnn_correction(self):
+        corr = np.zeros(self.n_blob * 17 * 9)
+        nbr = np.zeros(self.n_blob * 16, dtype=np.int32)
+        L.check(self._h, L.lib().cfm_gnn_correction(self._h, L.ptr(corr), L.iptr(nbr)))
+        return corr.reshape(self.n_blob, 17, 3, 3), nbr.reshape(self.n_blob, 16)
+
+    def gnn_gradient(self, s0, s1):
+        g = np.zeros(L.lib().cfm_gnn_num_params(self._h), dtype=np.float32)
+        loss = ctypes.c_double()
+        L.check(self._h, L.lib().cfm_gnn_gradient(self._h, int(s0), int(s1), L.fptr(g),
+                                                  ctypes.byref(loss)))
+        return loss.value, g
+
+    def gemm_bf16(self, A, B, transA=False, transB=False):
+        A = np.ascontiguousarray(A, dtype=np.float32)
+        B = np.ascontiguousarray(B, dtype=np.float32)
+        M, K = (A.shape[1], A.shape[0]) if transA else A.shape
+        N = B.shape[0] if transB else B.shape[1]
+        C = np.zeros((M, N), dtype=np.float32)
+        L.check(self._h, L.lib().cfm_gemm_bf16(self._h, M, N, K, L.fptr(A), int(transA), L.fptr(B),
+                                               int(transB), L.fptr(C)))
+        return C

@@ -244,6 +244,29 @@ typedef struct cfm_profile_stats {
 int cfm_profile(cfm_ctx* ctx, int32_t enable);
 int cfm_profile_read(cfm_ctx* ctx, cfm_profile_stats* stats);
 
+/* ---- learned near-field correction (no reference counterpart) -------------- */
+/* A message-passing network on the blobs' 16-nearest-neighbour graph predicts a
+ * 3x3 correction block per edge; with it enabled, CFM_PC_NEAR applies the analytic
+ * near-field local inverse plus the learned correction.  Dense layers run on the
+ * matrix cores (BF16 operands, FP32 accumulation); training minimises the
+ * preconditioned residual mean_t ||b_t - A P^-1 b_t||^2 / ||b_t||^2 over recorded
+ * (rhs, solution) pairs of the plain solver.  Plain GMRES stays the judge: every
+ * solve still checks its FP64 residual against rtol. */
+int cfm_gnn_init(cfm_ctx* ctx, uint32_t seed, int32_t capacity);   /* capacity: training pairs kept */
+int cfm_gnn_num_params(cfm_ctx* ctx);
+int cfm_gnn_record(cfm_ctx* ctx, int32_t on);          /* record every blob solve of cfm_fil_step */
+int cfm_gnn_samples(cfm_ctx* ctx);                     /* recorded pairs available */
+int cfm_gnn_add_samples(cfm_ctx* ctx, int32_t n, const double* rhs, const double* sol); /* AoS [n][3 n_blob] */
+int cfm_gnn_train(cfm_ctx* ctx, int32_t epochs, int32_t batch, double lr, double* loss_before,
+                  double* loss_after);
+int cfm_gnn_enable(cfm_ctx* ctx, int32_t on);          /* add the correction inside CFM_PC_NEAR */
+int cfm_gnn_get_params(cfm_ctx* ctx, float* params);   /* cfm_gnn_num_params floats */
+int cfm_gnn_set_params(cfm_ctx* ctx, const float* params);
+int cfm_gnn_correction(cfm_ctx* ctx, double* corr, int32_t* nbr); /* corr [n_blob][17][9], nbr [n_blob][16] */
+int cfm_gnn_gradient(cfm_ctx* ctx, int32_t s0, int32_t s1, float* grad, double* loss);
+int cfm_gemm_bf16(cfm_ctx* ctx, int32_t M, int32_t N, int32_t K, const float* A, int32_t transA,
+                  const float* B, int32_t transB, float* C);  /* C = op(A) op(B), row-major */
+
 #ifdef __cplusplus
 }
 #endif
