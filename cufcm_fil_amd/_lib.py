@@ -19,7 +19,7 @@ CFM_OK, CFM_EINVAL, CFM_EHIP, CFM_ENOMEM, CFM_ESTATE, CFM_ENOCONV, CFM_ENONFINIT
 CFM_FP64, CFM_FP32C = 0, 1
 CFM_SS, CFM_SB, CFM_BB, CFM_BS, CFM_FULL = range(5)
 CFM_SYS_SEG, CFM_SYS_BLOB = 0, 1
-CFM_PC_NONE, CFM_PC_DIAG, CFM_PC_SELF, CFM_PC_NEAR = 0, 1, 2, 3
+CFM_PC_NONE, CFM_PC_DIAG, CFM_PC_SELF, CFM_PC_NEAR, CFM_PC_DENSE = 0, 1, 2, 3, 4
 CFM_SOLVE_FP64, CFM_SOLVE_MIXED = 0, 1
 
 
@@ -124,6 +124,41 @@ class cfm_profile_stats(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class cfm_batch_config(ctypes.Structure):
+    _fields_ = [
+        ("n_sys", ctypes.c_int32),
+        ("seg_dof", ctypes.c_int32),
+        ("wall", ctypes.c_int32),
+        ("device", ctypes.c_int32),
+        ("a_seg", ctypes.c_double),
+        ("mu", ctypes.c_double),
+        ("reserved", ctypes.c_int32 * 4),
+    ]
+
+
+class cfm_batch_tuning(ctypes.Structure):
+    _fields_ = [
+        ("max_sources", ctypes.c_int32),
+        ("grid", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 2),
+        ("ms", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return {"max_sources": self.max_sources, "grid": self.grid, "ms": self.ms}
+
+
+class cfm_batch_sys_stats(ctypes.Structure):
+    _fields_ = [
+        ("iterations", ctypes.c_int32),
+        ("converged", ctypes.c_int32),
+        ("reserved0", ctypes.c_int32),
+        ("reserved1", ctypes.c_int32),
+        ("rel_residual", ctypes.c_double),
+        ("latency_ms", ctypes.c_double),
+    ]
+
+
 _dp = ctypes.POINTER(ctypes.c_double)
 _vp = ctypes.c_void_p
 _lib = None
@@ -183,6 +218,18 @@ def lib():
     L.cfm_gnn_gradient.argtypes = [_vp, ctypes.c_int32, ctypes.c_int32, _fp, _dp]
     L.cfm_gemm_bf16.argtypes = [_vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _fp,
                                 ctypes.c_int32, _fp, ctypes.c_int32, _fp]
+    L.cfm_batch_create.argtypes = [ctypes.POINTER(cfm_batch_config), _ip, ctypes.POINTER(_vp)]
+    L.cfm_batch_destroy.argtypes = [_vp]
+    L.cfm_batch_last_error.restype = ctypes.c_char_p
+    L.cfm_batch_last_error.argtypes = [_vp]
+    L.cfm_batch_set_positions.argtypes = [_vp, _dp]
+    L.cfm_batch_apply.argtypes = [_vp, ctypes.c_int32, _dp, _dp]
+    L.cfm_batch_set_tuning.argtypes = [_vp, ctypes.POINTER(cfm_batch_tuning)]
+    L.cfm_batch_get_tuning.argtypes = [_vp, ctypes.POINTER(cfm_batch_tuning)]
+    L.cfm_batch_tune.argtypes = [_vp, ctypes.c_int32, ctypes.POINTER(cfm_batch_tuning),
+                                 ctypes.POINTER(cfm_batch_tuning), ctypes.c_int32, _ip]
+    L.cfm_batch_gmres.argtypes = [_vp, _dp, _dp, ctypes.POINTER(cfm_solve_opts),
+                                  ctypes.POINTER(cfm_batch_sys_stats), _dp]
     L.cfm_profile.argtypes = [_vp, ctypes.c_int32]
     L.cfm_profile_read.argtypes = [_vp, ctypes.POINTER(cfm_profile_stats)]
     _lib = L
@@ -205,6 +252,12 @@ def f64(a, n=None):
 def check(ctx, code):
     if code != CFM_OK:
         msg = lib().cfm_last_error(ctx)
+        raise CfmError(code, msg.decode() if msg else "")
+
+
+def check_batch(b, code):
+    if code != CFM_OK:
+        msg = lib().cfm_batch_last_error(b)
         raise CfmError(code, msg.decode() if msg else "")
 
 

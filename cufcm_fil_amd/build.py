@@ -14,7 +14,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libcufcm_fil.so")
-SOURCES = ["api.hip", "apply.hip", "gmres.hip", "fil.hip", "step.hip", "gnn.hip", "nearfield.hip"]
+SOURCES = ["api.hip", "apply.hip", "gmres.hip", "fil.hip", "step.hip", "gnn.hip", "nearfield.hip",
+           "densepc.hip", "batch.hip"]
 ARCH = os.environ.get("CFM_OFFLOAD_ARCH", "gfx950")
 
 FLAGS = [
@@ -22,13 +23,15 @@ FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-result",
 ]
+# rocSOLVER factorises the blob mobility once for the resident-inverse preconditioner
+LIBS = ["-L/opt/rocm/lib", "-lrocsolver", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"]
 
 
 def build(verbose: bool = False, extra=None) -> str:
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
     srcs = [os.path.join(CSRC, s) for s in SOURCES]
     tmp = LIB + ".tmp"
-    cmd = [hipcc, *FLAGS, *(extra or []), "-o", tmp, *srcs]
+    cmd = [hipcc, *FLAGS, *(extra or []), "-o", tmp, *srcs, *LIBS]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

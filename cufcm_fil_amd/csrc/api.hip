@@ -201,6 +201,7 @@ int cfm_destroy(cfm_ctx* c) {
   c->p.partial.release();
   c->stage.release();
   c->nf.release();
+  c->dinv.release();
   c->vec_a.release();
   c->vec_b.release();
   if (c->krylov) {
@@ -252,7 +253,13 @@ int cfm_set_positions(cfm_ctx* c, const double* x_seg, const double* x_blob) {
   if (s.ok()) s = up(x_blob, p.n_blob, p.n_seg);
   if (!s.ok()) return fail(c, s);
   c->have_positions = true;
-  c->nf.valid = false;  // near-field operator depends on the blob positions
+  // the near-field blocks and the resident inverse depend on the blob positions only
+  const size_t nb3 = (size_t)3 * p.n_blob;
+  if (c->blob_pos.size() != nb3 || (nb3 && std::memcmp(c->blob_pos.data(), x_blob, sizeof(double) * nb3))) {
+    c->blob_pos.assign(x_blob, x_blob + nb3);
+    c->nf.valid = false;
+    c->dinv.valid = false;
+  }
   return CFM_OK;
 }
 
@@ -327,7 +334,7 @@ int cfm_gmres(cfm_ctx* c, const cfm_solve_opts* o, const double* rhs, double* x,
   if (!c || !o || !rhs || !x) return c ? fail(c, make_status(CFM_EINVAL, "null argument")) : CFM_EINVAL;
   if (o->system != CFM_SYS_SEG && o->system != CFM_SYS_BLOB)
     return fail(c, make_status(CFM_EINVAL, "unknown system %d", o->system));
-  if (o->precond < CFM_PC_NONE || o->precond > CFM_PC_NEAR)
+  if (o->precond < CFM_PC_NONE || o->precond > CFM_PC_DENSE)
     return fail(c, make_status(CFM_EINVAL, "unknown preconditioner %d", o->precond));
   if (o->mode != CFM_SOLVE_FP64 && o->mode != CFM_SOLVE_MIXED)
     return fail(c, make_status(CFM_EINVAL, "unknown solve mode %d", o->mode));
@@ -359,7 +366,7 @@ int cfm_apply_preconditioner(cfm_ctx* c, int32_t system, int32_t precond, const 
   if (!c || !in || !out) return c ? fail(c, make_status(CFM_EINVAL, "null argument")) : CFM_EINVAL;
   if (system != CFM_SYS_SEG && system != CFM_SYS_BLOB)
     return fail(c, make_status(CFM_EINVAL, "unknown system %d", system));
-  if (precond < CFM_PC_NONE || precond > CFM_PC_NEAR)
+  if (precond < CFM_PC_NONE || precond > CFM_PC_DENSE)
     return fail(c, make_status(CFM_EINVAL, "unknown preconditioner %d", precond));
   if (!c->have_positions) return fail(c, make_status(CFM_ESTATE, "positions not set"));
   const SystemDesc s = system_desc(c, system);

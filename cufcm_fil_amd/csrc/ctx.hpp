@@ -99,6 +99,24 @@ struct NearField {
   }
 };
 
+// Resident inverse of the blob-blob mobility (densepc.hip): an n x n FP32 matrix,
+// n = 3 n_blob, rows and columns in the blob-system vector order [component][blob],
+// leading dimension ld (a multiple of 64 floats, padding zero).  Built once per set
+// of blob positions from an FP64 Cholesky factorisation.
+struct DenseInverse {
+  bool valid = false;
+  int n = 0;
+  size_t ld = 0;
+  DevBuf<float> P;
+  double build_seconds = 0.0;
+  void release() {
+    P.release();
+    valid = false;
+    n = 0;
+    ld = 0;
+  }
+};
+
 // Kernel-side launch description of one mobility product.
 struct ApplyPlan {
   int tgt_begin = 0, tgt_end = 0;    // target range (combined ids)
@@ -132,6 +150,8 @@ struct cfm_ctx {
   void* fil = nullptr;                  // filament state of the implicit step (step.hip)
   cfm::NearField nf;                    // near-field preconditioner of the blob system
   const double* nf_corr = nullptr;      // learned correction blocks (same layout as nf.W) or null
+  cfm::DenseInverse dinv;               // resident inverse of the blob system (CFM_PC_DENSE)
+  std::vector<double> blob_pos;         // blob positions (AoS) of the last cfm_set_positions
   // pair-kernel profiling (HIP events around each launch)
   bool prof_on = false;
   std::vector<hipEvent_t> prof_events;  // pool, two per launch

@@ -518,109 +518,166 @@ __global__ void fil_update_kernel(FilK K, const double* __restrict__ upd) {
   robot_arm(K, f);
 }
 
-// Bounding boxes of the source tiles (256 consecutive particles: segments, then blobs)
-// used to skip tiles that cannot hold a barrier partner.
-__global__ __launch_bounds__(256) void tile_bbox_kernel(const FilK K, const double* __restrict__ bx,
-                                                        const double* __restrict__ by,
-                                                        const double* __restrict__ bz, int nblob,
-                                                        double* __restrict__ box) {
-  __shared__ double lo[3][256], hi[3][256];
-  const int ns = (int)K.nseg_total;
-  const int j = blockIdx.x * 256 + threadIdx.x;
-  double p[3];
-  bool v = true;
-  if (j < ns) {
-    p[0] = K.x[j]; p[1] = K.y[j]; p[2] = K.z[j];
-  } else if (j < ns + nblob) {
-    p[0] = bx[j - ns]; p[1] = by[j - ns]; p[2] = bz[j - ns];
-  } else {
-    v = false;
+// ---- barrier forces over a hashed cell list -----------------------------------------
+// Cells have the edge of the barrier cut-off (1.1 x 2a), so every partner of a
+// segment lies in its own or one of the 26 adjacent cells.  Cells are hashed into a
+// power-of-two table; a bucket can hold particles of several cells, so a visit keeps
+// only the particles whose own cell is the cell being visited (no double counting).
+
+__device__ __forceinline__ int cell_coord(double v, double inv_h) {
+  return (int)floor(fmin(fmax(v * inv_h, -1.0e9), 1.0e9));
+}
+
+__device__ __forceinline__ int cell_bucket(int cx, int cy, int cz, int mask) {
+  const unsigned h = ((unsigned)cx * 73856093u) ^ ((unsigned)cy * 19349663u) ^
+                     ((unsigned)cz * 83492791u);
+  return (int)(h & (unsigned)mask);
+}
+
+struct GridPos {
+  const double *sx, *sy, *sz;  // segments
+  const double *bx, *by, *bz;  // blobs
+  int ns, n;
+  __device__ __forceinline__ void get(int j, double& x, double& y, double& z) const {
+    if (j < ns) {
+      x = sx[j]; y = sy[j]; z = sz[j];
+    } else {
+      x = bx[j - ns]; y = by[j - ns]; z = bz[j - ns];
+    }
   }
-  for (int c = 0; c < 3; ++c) {
-    lo[c][threadIdx.x] = v ? p[c] : 1e300;
-    hi[c][threadIdx.x] = v ? p[c] : -1e300;
-  }
+};
+
+__global__ void grid_clear_kernel(int* __restrict__ a, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = 0;
+}
+
+__global__ void grid_count_kernel(GridPos P, double inv_h, int mask, int* __restrict__ key,
+                                  int* __restrict__ count) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= P.n) return;
+  double x, y, z;
+  P.get(j, x, y, z);
+  const int b = cell_bucket(cell_coord(x, inv_h), cell_coord(y, inv_h), cell_coord(z, inv_h), mask);
+  key[j] = b;
+  atomicAdd(count + b, 1);
+}
+
+// Exclusive scan of the bucket counts (one workgroup of 1024 threads); writes the
+// offsets to start[0..nb] and a copy of start[0..nb) to cursor.
+__global__ __launch_bounds__(1024) void grid_scan_kernel(int* __restrict__ start,
+                                                         int* __restrict__ cursor, int nb) {
+  __shared__ int part[1024];
+  const int t = threadIdx.x;
+  const int per = (nb + 1023) / 1024;
+  const int b0 = t * per, b1 = min(nb, b0 + per);
+  int s = 0;
+  for (int b = b0; b < b1; ++b) s += start[b];
+  part[t] = s;
   __syncthreads();
-  for (int o = 128; o > 0; o >>= 1) {
-    if (threadIdx.x < o)
-      for (int c = 0; c < 3; ++c) {
-        lo[c][threadIdx.x] = fmin(lo[c][threadIdx.x], lo[c][threadIdx.x + o]);
-        hi[c][threadIdx.x] = fmax(hi[c][threadIdx.x], hi[c][threadIdx.x + o]);
-      }
+  for (int o = 1; o < 1024; o <<= 1) {
+    const int v = t >= o ? part[t - o] : 0;
+    __syncthreads();
+    part[t] += v;
     __syncthreads();
   }
-  if (threadIdx.x < 3) {
-    box[blockIdx.x * 6 + threadIdx.x] = lo[threadIdx.x][0];
-    box[blockIdx.x * 6 + 3 + threadIdx.x] = hi[threadIdx.x][0];
+  int run = part[t] - s;  // exclusive prefix of this thread's chunk
+  for (int b = b0; b < b1; ++b) {
+    const int cnt = start[b];
+    start[b] = run;
+    cursor[b] = run;
+    run += cnt;
+  }
+  if (t == 1023) start[nb] = part[1023];
+}
+
+__global__ void grid_scatter_kernel(int n, const int* __restrict__ key, int* __restrict__ cursor,
+                                    int* __restrict__ ids) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  ids[atomicAdd(cursor + key[j], 1)] = j;
+}
+
+// Ascending ids inside every bucket, so the visit order (and the rounding of the
+// force sums) does not depend on the order the atomics handed out slots.
+__global__ void grid_sort_kernel(int nb, const int* __restrict__ start, int* __restrict__ ids) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  const int s0 = start[b], s1 = start[b + 1];
+  for (int i = s0 + 1; i < s1; ++i) {
+    const int v = ids[i];
+    int k = i - 1;
+    while (k >= s0 && ids[k] > v) {
+      ids[k + 1] = ids[k];
+      --k;
+    }
+    ids[k + 1] = v;
   }
 }
 
-// Barrier (short-range repulsion) forces added to the segment loads in fout.  The
-// workgroup's 256 targets are one tile of the same tiling, so tile pairs whose
-// boxes are farther apart than the cut-off are skipped as a whole.
-__global__ __launch_bounds__(256) void barrier_kernel(FilK K, const double* __restrict__ bx,
-                                                      const double* __restrict__ by,
-                                                      const double* __restrict__ bz, int nblob,
-                                                      int wall, const double* __restrict__ box,
+// Barrier (short-range repulsion) forces added to the segment loads in fout
+// (cuda_functions.cu:1825-1930): wall repulsion below half a segment length, and a
+// (1 - chi (r - 2a))^4 / r repulsion from every particle within 1.1 x 2a except the
+// segment itself and its neighbours along the same filament.
+__global__ __launch_bounds__(256) void barrier_kernel(FilK K, GridPos P, double inv_h, int mask,
+                                                      const int* __restrict__ start,
+                                                      const int* __restrict__ ids, int wall,
                                                       double* __restrict__ fout, int ld) {
-  __shared__ double sx[256], sy[256], sz[256];
-  const int ns = (int)K.nseg_total;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool valid = i < ns;
-  const int ic = valid ? i : 0;
-  const double xi = K.x[ic], yi = K.y[ic], zi = K.z[ic];
-  const int fi = ic / K.nseg;
+  if (i >= (int)K.nseg_total) return;
+  const double xi = K.x[i], yi = K.y[i], zi = K.z[i];
+  const int fi = i / K.nseg;
   const double fmag = K.rep_factor * K.end_force;
   double fx = 0.0, fy = 0.0, fz = 0.0;
   if (wall && zi < 0.5 * K.dl) {
     double w = fmin(1.0, 1.0 - (zi - K.a) / (0.5 * K.dl - K.a));
     fz += fmag * w * w * w * w;
   }
-  const int ntot = ns + nblob;
   const double a_sum = 2.0 * K.a;
   const double cut = 1.1 * a_sum, chi = 10.0 / a_sum;
-  const double* tb = box + blockIdx.x * 6;
-  for (int t0 = 0, tile = 0; t0 < ntot; t0 += 256, ++tile) {
-    const double* sb = box + tile * 6;
-    double gap2 = 0.0;
-    for (int c = 0; c < 3; ++c) {
-      const double g = fmax(0.0, fmax(sb[c] - tb[3 + c], tb[c] - sb[3 + c]));
-      gap2 += g * g;
-    }
-    if (gap2 >= cut * cut) continue;  // uniform across the workgroup
-    const int j = t0 + threadIdx.x;
-    __syncthreads();
-    if (j < ns) {
-      sx[threadIdx.x] = K.x[j];
-      sy[threadIdx.x] = K.y[j];
-      sz[threadIdx.x] = K.z[j];
-    } else if (j < ntot) {
-      sx[threadIdx.x] = bx[j - ns];
-      sy[threadIdx.x] = by[j - ns];
-      sz[threadIdx.x] = bz[j - ns];
-    }
-    __syncthreads();
-    const int cnt = min(256, ntot - t0);
-    for (int jj = 0; jj < cnt; ++jj) {
-      const int j2 = t0 + jj;
-      const double dx = xi - sx[jj], dy = yi - sy[jj], dz = zi - sz[jj];
-      const double d2 = dx * dx + dy * dy + dz * dz;
-      if (d2 < cut * cut) {
-        if (j2 < ns && j2 / K.nseg == fi && abs(ic - j2) <= 1) continue;
-        const double dist = sqrt(d2);
-        double fac = fmin(1.0, 1.0 - chi * (dist - a_sum));
-        fac = fmag * fac * fac * fac * fac / dist;
-        fx += fac * dx;
-        fy += fac * dy;
-        fz += fac * dz;
+  const int cx = cell_coord(xi, inv_h), cy = cell_coord(yi, inv_h), cz = cell_coord(zi, inv_h);
+  for (int dz = -1; dz <= 1; ++dz)
+    for (int dy = -1; dy <= 1; ++dy)
+      for (int dx = -1; dx <= 1; ++dx) {
+        const int nx = cx + dx, ny = cy + dy, nz = cz + dz;
+        const int b = cell_bucket(nx, ny, nz, mask);
+        const int t1 = start[b + 1];
+        for (int t = start[b]; t < t1; ++t) {
+          const int j = ids[t];
+          double xj, yj, zj;
+          P.get(j, xj, yj, zj);
+          if (cell_coord(xj, inv_h) != nx || cell_coord(yj, inv_h) != ny ||
+              cell_coord(zj, inv_h) != nz)
+            continue;
+          if (j < P.ns && j / K.nseg == fi && abs(i - j) <= 1) continue;
+          const double ex = xi - xj, ey = yi - yj, ez = zi - zj;
+          const double d2 = ex * ex + ey * ey + ez * ez;
+          if (d2 < cut * cut) {
+            const double dist = sqrt(d2);
+            double fac = fmin(1.0, 1.0 - chi * (dist - a_sum));
+            fac = fmag * fac * fac * fac * fac / dist;
+            fx += fac * ex;
+            fy += fac * ey;
+            fz += fac * ez;
+          }
+        }
       }
-    }
+  fout[i] += fx;
+  fout[(size_t)ld + i] += fy;
+  fout[2 * (size_t)ld + i] += fz;
+}
+
+Status BarrierGrid::ensure(int n_particles) {
+  int nb = 1024;
+  while (nb < 2 * n_particles) nb <<= 1;
+  if (nb != nbucket) {
+    CFM_TRY(start.alloc((size_t)nb + 1));
+    CFM_TRY(cursor.alloc((size_t)nb));
+    nbucket = nb;
   }
-  if (valid) {
-    fout[i] += fx;
-    fout[(size_t)ld + i] += fy;
-    fout[2 * (size_t)ld + i] += fz;
-  }
+  CFM_TRY(ids.alloc((size_t)std::max(1, n_particles)));
+  CFM_TRY(key.alloc((size_t)std::max(1, n_particles)));
+  return {};
 }
 
 // ---- launchers -------------------------------------------------------------------
@@ -639,14 +696,22 @@ Status fil_forces(cfm_ctx* c, const FilK& K, int nt) {
   return {};
 }
 
-Status fil_barrier(cfm_ctx* c, const FilK& K, double* box) {
+Status fil_barrier(cfm_ctx* c, const FilK& K, BarrierGrid& g) {
   const Particles& p = c->p;
-  const int ntiles = (p.n_seg + p.n_blob + 255) / 256;
-  hipLaunchKernelGGL(tile_bbox_kernel, dim3(ntiles), dim3(256), 0, c->stream, K, p.x.p + p.n_seg,
-                     p.y.p + p.n_seg, p.z.p + p.n_seg, p.n_blob, box);
-  hipLaunchKernelGGL(barrier_kernel, gsz((int)K.nseg_total, 256), dim3(256), 0, c->stream, K,
-                     p.x.p + p.n_seg, p.y.p + p.n_seg, p.z.p + p.n_seg, p.n_blob, c->cfg.wall, box,
-                     p.f.p, p.n);
+  const int n = p.n_seg + p.n_blob;
+  CFM_TRY(g.ensure(n));
+  const GridPos P{K.x, K.y, K.z, p.x.p + p.n_seg, p.y.p + p.n_seg, p.z.p + p.n_seg, p.n_seg, n};
+  const double inv_h = 1.0 / (1.1 * 2.0 * K.a);
+  const int nb = g.nbucket, mask = nb - 1;
+  hipLaunchKernelGGL(grid_clear_kernel, gsz(nb + 1, 256), dim3(256), 0, c->stream, g.start.p, nb + 1);
+  hipLaunchKernelGGL(grid_count_kernel, gsz(n, 256), dim3(256), 0, c->stream, P, inv_h, mask,
+                     g.key.p, g.start.p);
+  hipLaunchKernelGGL(grid_scan_kernel, dim3(1), dim3(1024), 0, c->stream, g.start.p, g.cursor.p, nb);
+  hipLaunchKernelGGL(grid_scatter_kernel, gsz(n, 256), dim3(256), 0, c->stream, n, g.key.p,
+                     g.cursor.p, g.ids.p);
+  hipLaunchKernelGGL(grid_sort_kernel, gsz(nb, 256), dim3(256), 0, c->stream, nb, g.start.p, g.ids.p);
+  hipLaunchKernelGGL(barrier_kernel, gsz((int)K.nseg_total, 256), dim3(256), 0, c->stream, K, P,
+                     inv_h, mask, g.start.p, g.ids.p, c->cfg.wall, p.f.p, p.n);
   CFM_CHECK_LAUNCH();
   return {};
 }

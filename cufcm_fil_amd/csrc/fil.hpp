@@ -28,10 +28,26 @@ struct FilK {
   double* jinv = nullptr;  // nfil x (6 nseg)^2, transposed inverse Jacobians
 };
 
+// Hashed uniform cell list over segments and blobs for the barrier: cells of edge
+// equal to the barrier cut-off, hashed into nbucket (power of two) buckets; ids holds
+// the particle ids grouped by bucket, ascending inside each bucket, and start the
+// bucket offsets (nbucket + 1 entries).
+struct BarrierGrid {
+  int nbucket = 0;
+  DevBuf<int> start, cursor, ids, key;
+  Status ensure(int n_particles);
+  void release() {
+    start.release();
+    cursor.release();
+    ids.release();
+    key.release();
+    nbucket = 0;
+  }
+};
+
 Status fil_initial_guess(cfm_ctx* c, const FilK& K, int nt);
 Status fil_forces(cfm_ctx* c, const FilK& K, int nt);
-// box: scratch of 6 * ceil((n_seg + n_blob) / 256) doubles
-Status fil_barrier(cfm_ctx* c, const FilK& K, double* box);
+Status fil_barrier(cfm_ctx* c, const FilK& K, BarrierGrid& grid);
 Status fil_errors(cfm_ctx* c, const FilK& K, int nt, double* err, double* bad_part,
                   double* bad_out);
 Status fil_jacobian(cfm_ctx* c, const FilK& K, int nt, int* status_dev);

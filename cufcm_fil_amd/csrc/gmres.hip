@@ -19,6 +19,8 @@
 #include "ctx.hpp"
 #include "krylov.hpp"
 #include "nearfield.hpp"
+#include "densepc.hpp"
+#include "precond.hpp"
 
 namespace cfm {
 
@@ -249,37 +251,7 @@ __global__ void precond_kernel(int kind, int wall, int dof, int m, int off,
   double v[6] = {0, 0, 0, 0, 0, 0};
   for (int k = 0; k < dof; ++k) v[k] = x[(size_t)k * m + i];
   double o[6] = {0, 0, 0, 0, 0, 0};
-  const double pi = 3.14159265358979323846;
-  if (kind == CFM_PC_NONE) {
-    for (int k = 0; k < dof; ++k) o[k] = v[k];
-  } else if (kind == CFM_PC_DIAG || !wall) {
-    const double ct = 6.0 * pi * mu * a, cr = 8.0 * pi * mu * a * a * a;
-    for (int k = 0; k < dof; ++k) o[k] = (k < 3 ? ct : cr) * v[k];
-  } else {
-    // inverse of the wall self-mobility block (a = 1): couples (F_x, T_y) and (F_y, T_x)
-    const double c6 = 1.0 / (6.0 * pi), c8 = 1.0 / (8.0 * pi);
-    const double h1 = 1.0 / z[off + i];
-    const double h3 = h1 * h1 * h1, h4 = h3 * h1, h5 = h4 * h1;
-    const double par = c6 * (1.0 - (9.0 * h1 - 2.0 * h3 + h5) * 0.0625) / mu;
-    const double perp = c6 * (1.0 - (9.0 * h1 - 4.0 * h3 + h5) * 0.125) / mu;
-    o[2] = v[2] / perp;
-    if (dof == 3) {
-      o[0] = v[0] / par;
-      o[1] = v[1] / par;
-    } else {
-      const double cpl = c6 * 0.09375 * h4 / mu;
-      const double rpar = c8 * (1.0 - 0.3125 * h3) / mu;
-      const double rperp = c8 * (1.0 - 0.125 * h3) / mu;
-      const double det = par * rpar - cpl * cpl;
-      // [vx, wy] = [[par, cpl], [cpl, rpar]] [Fx, Ty]
-      o[0] = (rpar * v[0] - cpl * v[4]) / det;
-      o[4] = (par * v[4] - cpl * v[0]) / det;
-      // [vy, wx] = [[par, -cpl], [-cpl, rpar]] [Fy, Tx]
-      o[1] = (rpar * v[1] + cpl * v[3]) / det;
-      o[3] = (par * v[3] + cpl * v[1]) / det;
-      o[5] = v[5] / rperp;
-    }
-  }
+  particle_precond(kind, wall, dof, z[off + i], a, mu, v, o);
   for (int k = 0; k < dof; ++k) y[(size_t)k * m + i] = o[k];
 }
 
@@ -394,6 +366,11 @@ Status system_precond(cfm_ctx* c, const SystemDesc& s, int kind, const double* x
     if (!c->nf.valid) CFM_TRY(nearfield_build(c, c->nf));
     return nearfield_apply(c, c->nf, c->nf_corr, x, y);
   }
+  if (kind == CFM_PC_DENSE) {
+    if (s.which != CFM_BB) return make_status(CFM_EINVAL, "CFM_PC_DENSE applies to the blob system only");
+    if (!c->dinv.valid) CFM_TRY(dense_build(c, c->dinv));
+    return dense_apply(c, c->dinv, x, y);
+  }
   hipLaunchKernelGGL(precond_kernel, g1(s.m), dim3(256), 0, c->stream, kind, c->cfg.wall, s.dof,
                      s.m, s.off, c->p.z.p, s.a, c->cfg.mu, x, y);
   CFM_CHECK_LAUNCH();
@@ -403,10 +380,11 @@ Status system_precond(cfm_ctx* c, const SystemDesc& s, int kind, const double* x
 // Restarted GMRES on A (pair precision `prec`) for b (device), x in/out (device).
 // Stops when the residual estimate reaches `target_abs`, after max_iter steps in
 // total, or after `max_cycles` cycles.  On return K.r holds b - A x (operator
-// `prec`) and *res_abs its norm.
+// `prec`) and *res_abs its norm, except after max_cycles: then *res_abs is the last
+// residual estimate and K.r is stale.
 Status gmres_core(cfm_ctx* c, Krylov& K, const SystemDesc& s, int prec, int pc, const double* b,
-                  double* x, double target_abs, int restart, int max_iter, int* iters,
-                  double* res_abs, const GmresHooks* hooks) {
+                  double* x, double target_abs, int restart, int max_iter, int max_cycles,
+                  int* iters, double* res_abs, const GmresHooks* hooks) {
   const int L = s.L;
   const int m = restart;
   CFM_TRY(K.reserve(L, m));
@@ -420,6 +398,8 @@ Status gmres_core(cfm_ctx* c, Krylov& K, const SystemDesc& s, int prec, int pc, 
   CFM_TRY(norm2(c, K, K.r.p, L, K.scal.p));
   CFM_TRY(read_scalar(c, K, K.scal.p, &nr));
   if (!std::isfinite(nr)) return make_status(CFM_ENONFINITE, "non-finite GMRES residual");
+  int cycles = 0;
+  double last_est = nr;
   while (nr > target_abs && *iters < max_iter) {
     if (s.which == CFM_BB && gnn_recording(c)) CFM_TRY(gnn_record_rhs(c, K.r.p));
     // q0 = r / ||r||
@@ -460,6 +440,7 @@ Status gmres_core(cfm_ctx* c, Krylov& K, const SystemDesc& s, int prec, int pc, 
         CFM_HIP(hipEventSynchronize(K.ev[pending % 4]));
         const double est = K.host_res[4 + pending % 4];
         if (!std::isfinite(est)) return make_status(CFM_ENONFINITE, "non-finite GMRES iterate");
+        last_est = est;
         if (est <= target_abs) stop = true;
       }
       pending = j;
@@ -473,6 +454,10 @@ Status gmres_core(cfm_ctx* c, Krylov& K, const SystemDesc& s, int prec, int pc, 
     if (hooks && hooks->precond_correction) CFM_TRY(hooks->precond_correction(c, s, K.t.p, K.z.p, hooks->user));
     hipLaunchKernelGGL(axpby_kernel, g1(L), dim3(256), 0, c->stream, 1.0, x, 1.0, K.z.p, x, L);
     CFM_CHECK_LAUNCH();
+    if (++cycles >= max_cycles) {  // caller recomputes the residual itself
+      nr = last_est;
+      break;
+    }
     // true residual with this operator
     CFM_TRY(system_apply(c, s, prec, x, K.w.p));
     hipLaunchKernelGGL(axpby_kernel, g1(L), dim3(256), 0, c->stream, 1.0, b, -1.0, K.w.p, K.r.p, L);
@@ -528,13 +513,13 @@ Status solve(cfm_ctx* c, Krylov& K, const SystemDesc& s, const cfm_solve_opts& o
   if (o.mode == CFM_SOLVE_FP64) {
     double res = nr;
     if (nr > target) CFM_TRY(gmres_core(c, K, s, CFM_FP64, o.precond, b, x, target, restart,
-                                        max_iter, &iters, &res, hooks));
+                                        max_iter, 1 << 30, &iters, &res, hooks));
     nr = res;
   } else {
     // iterative refinement: d solves A32 d = r to a relative 5e-7 (or what is left
     // to reach the target), x += d, r = b - A64 x.
     int refinements = 0;
-    while (nr > target && iters < max_iter && refinements < 12) {
+    while (nr > target && iters < max_iter && refinements < 40) {
       const double rel_in = std::max(5e-7, 0.5 * target / nr);
       hipLaunchKernelGGL(fill_kernel, g1(L), dim3(256), 0, c->stream, K.t.p, 0.0, L);
       CFM_CHECK_LAUNCH();
@@ -546,8 +531,11 @@ Status solve(cfm_ctx* c, Krylov& K, const SystemDesc& s, const cfm_solve_opts& o
       CFM_CHECK_LAUNCH();
       CFM_TRY(K.rhs.alloc(L));
       CFM_HIP(hipMemcpyAsync(K.rhs.p, K.d.p, sizeof(double) * L, hipMemcpyDeviceToDevice, c->stream));
+      // one restart cycle per refinement: the FP32C operator is linear only to its
+      // rounding, so an inner solve can stall short of rel_in; the FP64 residual
+      // below decides what is left.
       CFM_TRY(gmres_core(c, K, s, CFM_FP32C, o.precond, K.rhs.p, dvec.p, rel_in * nr, restart,
-                         max_iter, &iters, &inner_res, hooks));
+                         max_iter, 1, &iters, &inner_res, hooks));
       hipLaunchKernelGGL(axpby_kernel, g1(L), dim3(256), 0, c->stream, 1.0, x, 1.0, dvec.p, x, L);
       CFM_CHECK_LAUNCH();
       CFM_TRY(system_apply(c, s, CFM_FP64, x, K.w.p));

@@ -106,7 +106,7 @@ struct FilState {
       pert, fint, jinv;
   DevBuf<double> err, nerr, upd, C, D, coef, partial, bad_part, scal;
   DevBuf<double> fb_prev, rhs_b;  // blob-force guess and blob rhs
-  DevBuf<double> box;             // barrier tile bounding boxes
+  BarrierGrid grid;               // barrier cell list
   History hist;                   // past blob solves (projected guess)
   DevBuf<int> status;
   bool have_jinv = false;
@@ -139,7 +139,7 @@ static Status apply_plan(cfm_ctx* c, const ApplyPlan& pl, int prec) {
 static Status velocities(cfm_ctx* c, FilState* F, cfm_step_stats* st) {
   Particles& p = c->p;
   const int prec = F->prm.pair_precision;
-  if (F->prm.barrier) CFM_TRY(fil_barrier(c, F->K, F->box.p));
+  if (F->prm.barrier) CFM_TRY(fil_barrier(c, F->K, F->grid));
   if (p.n_blob == 0) {
     ApplyPlan pl;
     pl.tgt_begin = 0;
@@ -287,9 +287,10 @@ void cfm_fil_release(cfm_ctx* c) {
                             &F->lam_m1, &F->lam_m2, &F->clamp, &F->clamp_m1, &F->clamp_m2,
                             &F->base, &F->pert, &F->fint, &F->jinv, &F->err, &F->nerr, &F->upd,
                             &F->C, &F->D, &F->coef, &F->partial, &F->bad_part, &F->scal,
-                            &F->fb_prev, &F->rhs_b, &F->box};
+                            &F->fb_prev, &F->rhs_b};
   for (auto* b : bufs) b->release();
   F->status.release();
+  F->grid.release();
   F->hist.B.release();
   F->hist.X.release();
   F->hist.g.release();
@@ -353,7 +354,7 @@ int cfm_fil_init(cfm_ctx* c, const cfm_fil_params* prm, const double* q_seg, con
     CFM_TRY(F->partial.alloc(dev_dots_partial_size(L, prm->max_broyden + 1)));
     CFM_TRY(F->bad_part.alloc((p.n_seg + 255) / 256 + 1));
     CFM_TRY(F->scal.alloc(8));
-    CFM_TRY(F->box.alloc(6 * (size_t)((p.n_seg + p.n_blob + 255) / 256)));
+    CFM_TRY(F->grid.ensure(p.n_seg + p.n_blob));
     CFM_TRY(F->status.alloc(4));
     if (p.n_blob) {
       CFM_TRY(F->fb_prev.alloc(3 * (size_t)p.n_blob));
@@ -466,7 +467,7 @@ int cfm_barrier_forces(cfm_ctx* c, double* f_seg) {
   if (!f_seg) return fail_c(c, make_status(CFM_EINVAL, "null output"));
   auto run = [&]() -> Status {
     CFM_TRY(dev_fill(c, c->p.f.p, 0.0, 3 * c->p.n));
-    CFM_TRY(fil_barrier(c, F->K, F->box.p));
+    CFM_TRY(fil_barrier(c, F->K, F->grid));
     return download_aos(c, f_seg, c->p.n_seg, 3, c->p.f.p, c->p.n, 0);
   };
   Status s = run();

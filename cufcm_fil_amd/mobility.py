@@ -118,7 +118,7 @@ class RpyMobilitySolver:
     # -- linear solves (solve_linear_system / apply_preconditioner_new) -------
     _SYS = {"seg": L.CFM_SYS_SEG, "blob": L.CFM_SYS_BLOB}
     _PC = {"none": L.CFM_PC_NONE, "diag": L.CFM_PC_DIAG, "self": L.CFM_PC_SELF,
-           "near": L.CFM_PC_NEAR}
+           "near": L.CFM_PC_NEAR, "dense": L.CFM_PC_DENSE}
     _MODE = {"fp64": L.CFM_SOLVE_FP64, "mixed": L.CFM_SOLVE_MIXED}
 
     def _sys_size(self, system):

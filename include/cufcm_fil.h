@@ -106,6 +106,9 @@ int cfm_time_apply(cfm_ctx* ctx, int32_t which, int32_t reps, double* ms_per_app
 #define CFM_PC_NEAR 3   /* blob system only: local inverse over each blob and its 16 nearest
                            blobs (near-field block preconditioner), plus the learned correction
                            when one is loaded (cfm_gnn_*) */
+#define CFM_PC_DENSE 4  /* blob system only: resident FP32 inverse of the whole blob-blob
+                           mobility (FP64 Cholesky, built on first use and kept until the
+                           blob positions change; 4 (3 n_blob)^2 bytes of device memory) */
 
 /* arithmetic of the solve */
 #define CFM_SOLVE_FP64 0   /* FP64 pair sums throughout */
@@ -266,6 +269,65 @@ int cfm_gnn_correction(cfm_ctx* ctx, double* corr, int32_t* nbr); /* corr [n_blo
 int cfm_gnn_gradient(cfm_ctx* ctx, int32_t s0, int32_t s1, float* grad, double* loss);
 int cfm_gemm_bf16(cfm_ctx* ctx, int32_t M, int32_t N, int32_t K, const float* A, int32_t transA,
                   const float* B, int32_t transB, float* C);  /* C = op(A) op(B), row-major */
+
+/* ---- batched ensembles of independent wall systems ----------------------------------
+ * No counterpart inside the reference's mobility_solver: the reference runs each
+ * member of an ensemble (parameter sweep, bisection, periodic-orbit search) as its
+ * own process (pyfile/driver/func.py:544 launches one executable per parameter set).
+ * A batch holds n_sys independent systems of segments (sizes n_seg[s], any mix);
+ * particles interact only within their own system.  Host arrays concatenate the
+ * systems in order, AoS as everywhere else: positions 3 per particle, loads and
+ * velocities seg_dof per particle. */
+typedef struct cfm_batch cfm_batch;
+
+typedef struct {
+  int32_t n_sys;
+  int32_t seg_dof;    /* 6 (force + torque) or 3 */
+  int32_t wall;       /* no-slip wall z = 0 (requires a_seg == 1) */
+  int32_t device;
+  double a_seg, mu;
+  int32_t reserved[4];
+} cfm_batch_config;
+
+/* Work decomposition of the persistent pair kernel: items are 64-target tiles times
+ * slices of at most max_sources sources of the same system (0: whole system, a
+ * multiple of 256 otherwise), pulled largest first by `grid` resident workgroups. */
+typedef struct {
+  int32_t max_sources;
+  int32_t grid;
+  int32_t reserved[2];
+  double ms;          /* cfm_batch_tune: measured time of one batched product */
+} cfm_batch_tuning;
+
+typedef struct {
+  int32_t iterations;   /* Arnoldi steps taken by this system */
+  int32_t converged;
+  int32_t reserved0, reserved1;
+  double rel_residual;  /* FP64 ||b - M x|| / ||b|| at return */
+  double latency_ms;    /* from the start of the call to the end of the GMRES step at
+                           which the system's residual estimate met its target */
+} cfm_batch_sys_stats;
+
+int cfm_batch_create(const cfm_batch_config* cfg, const int32_t* n_seg, cfm_batch** out);
+int cfm_batch_destroy(cfm_batch* b);
+const char* cfm_batch_last_error(const cfm_batch* b);
+int cfm_batch_set_positions(cfm_batch* b, const double* x /* 3 sum(n_seg) */);
+/* v = M f per system, pair sums in `precision` (CFM_FP64 / CFM_FP32C) */
+int cfm_batch_apply(cfm_batch* b, int32_t precision, const double* f, double* v);
+int cfm_batch_set_tuning(cfm_batch* b, const cfm_batch_tuning* t);
+int cfm_batch_get_tuning(cfm_batch* b, cfm_batch_tuning* t);
+/* Autotuner: times the batched product for every candidate decomposition on the
+ * batch's own positions and keeps the fastest; tried[] (optional, max_tried
+ * entries) receives every candidate with its time. */
+int cfm_batch_tune(cfm_batch* b, int32_t precision, cfm_batch_tuning* chosen,
+                   cfm_batch_tuning* tried, int32_t max_tried, int32_t* n_tried);
+/* Every system's segment resistance problem M_s x_s = rhs_s, solved together by
+ * restarted right-preconditioned GMRES in lockstep (mobility_solver.cpp:2187-2669
+ * per system).  opts->system is ignored; precond NONE, DIAG or SELF; mode FP64 or
+ * MIXED (FP32C pair sums inside FP64 refinement).  One Arnoldi step of the batch is
+ * one hipGraph replay.  per_sys (optional) has n_sys entries. */
+int cfm_batch_gmres(cfm_batch* b, const double* rhs, double* x /* in: guess if use_guess */,
+                    const cfm_solve_opts* opts, cfm_batch_sys_stats* per_sys, double* total_ms);
 
 #ifdef __cplusplus
 }

@@ -50,3 +50,29 @@ def test_create_without_gpu_reports_hip_error():
 def test_null_context_is_rejected():
     assert L.lib().cfm_set_positions(None, None, None) == L.CFM_EINVAL
     assert L.lib().cfm_destroy(None) == L.CFM_OK
+
+
+def _bcfg(**kw):
+    base = dict(n_sys=3, seg_dof=6, wall=1, device=0, a_seg=1.0, mu=1.0)
+    base.update(kw)
+    return L.cfm_batch_config(**base)
+
+
+@pytest.mark.parametrize("bad,sizes", [(dict(n_sys=0), [20]), (dict(seg_dof=5), [20, 20, 20]),
+                                       (dict(a_seg=2.0), [20, 20, 20]), (dict(mu=-1.0), [20, 20, 20]),
+                                       (dict(), [20, 0, 20])])
+def test_batch_create_rejects_bad_config(bad, sizes):
+    h = ctypes.c_void_p()
+    n = (ctypes.c_int32 * len(sizes))(*sizes)
+    code = L.lib().cfm_batch_create(ctypes.byref(_bcfg(**bad)), n, ctypes.byref(h))
+    assert code == L.CFM_EINVAL and h.value is None
+    assert L.lib().cfm_batch_last_error(None)
+
+
+@pytest.mark.skipif(os.path.exists("/dev/kfd"), reason="a GPU is present")
+def test_batch_create_without_gpu_reports_hip_error():
+    h = ctypes.c_void_p()
+    n = (ctypes.c_int32 * 3)(20, 40, 60)
+    assert L.lib().cfm_batch_create(ctypes.byref(_bcfg()), n, ctypes.byref(h)) == L.CFM_EHIP
+    assert b"device" in L.lib().cfm_batch_last_error(None).lower()
+    assert L.lib().cfm_batch_destroy(None) == L.CFM_OK

@@ -116,3 +116,33 @@ def test_nearfield_preconditioner_cuts_iterations():
         f, sn = m.solve("blob", rhs, rtol=1e-10, precond="near", max_iter=3000)
     assert sn["converged"] and _oracle_rel_res(xb, 3, f, rhs) < 1e-9
     assert sn["iterations"] * 2 < sd["iterations"], (sn, sd)
+
+
+def test_dense_preconditioner_is_the_rounded_inverse():
+    """CFM_PC_DENSE applies the FP32-rounded inverse of the oracle's blob mobility
+    with FP64 sums: every entry within one FP32 rounding of the exact inverse."""
+    xs, xb, c, d = geo.sphere_over_wall(n_blob=300, n_fil=6, radius=8.0, gap=50.0)
+    ns, nb = xs.size // 3, xb.size // 3
+    Minv = np.linalg.inv(oracle.mobility_matrix(xb, 3))
+    v = geo.loads(nb, 3, 8)
+    with RpyMobilitySolver(ns, nb) as m:
+        m.set_positions(xs, xb)
+        got = m.apply_preconditioner("blob", v, "dense")
+        again = m.apply_preconditioner("blob", v, "dense")   # built once, reused
+    ref = Minv @ v
+    bound = 1.2e-7 * (np.abs(Minv) @ np.abs(v)) + 1e-12 * np.abs(ref).max()
+    assert np.all(np.abs(got - ref) <= bound), np.abs(got - ref).max()
+    assert np.array_equal(got, again)
+
+
+@pytest.mark.parametrize("mode", ["fp64", "mixed"])
+def test_dense_preconditioner_solves_in_few_iterations(mode):
+    xs, xb, c, d = geo.sphere_over_wall(n_blob=1500, n_fil=40, radius=20.0, gap=50.0)
+    ns, nb = xs.size // 3, xb.size // 3
+    rhs = geo.loads(nb, 3, 5)
+    with RpyMobilitySolver(ns, nb) as m:
+        m.set_positions(xs, xb)
+        f, st = m.solve("blob", rhs, rtol=1e-10, precond="dense", mode=mode, max_iter=200)
+        _, sn = m.solve("blob", rhs, rtol=1e-10, precond="near", mode=mode, max_iter=3000)
+    assert st["converged"] and _oracle_rel_res(xb, 3, f, rhs) < 1e-9
+    assert st["iterations"] <= 10 and st["iterations"] * 3 < sn["iterations"], (st, sn)
