@@ -195,9 +195,9 @@ def main():
     ap.add_argument("--skip-batch", action="store_true")
     ap.add_argument("--batch-reps", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--precond", default="near", choices=["near", "dense"],
+    ap.add_argument("--precond", default="dense", choices=["near", "dense"],
                     help="blob-solve preconditioner of the fast path")
-    ap.add_argument("--mode", default="mixed", choices=["mixed", "fp64"],
+    ap.add_argument("--mode", default="fp64", choices=["mixed", "fp64"],
                     help="blob-solve arithmetic of the fast path")
     args = ap.parse_args()
     if args.gpus != 1:

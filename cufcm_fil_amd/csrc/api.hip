@@ -10,6 +10,8 @@
 #include <cstring>
 #include <new>
 
+#include <rocblas/rocblas.h>
+
 #include "ctx.hpp"
 #include "krylov.hpp"
 
@@ -97,8 +99,9 @@ ApplyPlan plan_for(const cfm_ctx* c, int which) {
 
 // Source split across workgroups so that a launch has enough workgroups to fill
 // 256 CUs (about 4 per CU), bounded by the number of 256-source tiles.
-int choose_split(const cfm_ctx* c, const ApplyPlan& pl) {
-  if (c->cfg.tile_split > 0) return c->cfg.tile_split;
+Status tune_split(cfm_ctx* c, const ApplyPlan& pl, int heuristic, int* best);
+
+static int heuristic_split(const cfm_ctx* c, const ApplyPlan& pl) {
   const int tiles = std::max(1, plan_tiles(c, pl));
   int nsrc = 0;
   if (pl.src_seg) nsrc += c->p.n_seg;
@@ -107,6 +110,20 @@ int choose_split(const cfm_ctx* c, const ApplyPlan& pl) {
   int s = (1024 + tiles - 1) / tiles;
   s = std::min(s, std::max(1, stiles / 2));
   return std::max(1, std::min(s, 64));
+}
+
+int choose_split(cfm_ctx* c, const ApplyPlan& pl) {
+  if (c->cfg.tile_split > 0) return c->cfg.tile_split;
+  const int heur = heuristic_split(c, pl);
+  if (c->cfg.tile_split == 0) return heur;
+  const std::array<long long, 6> key = {pl.tgt_begin, pl.tgt_end, pl.src_seg, pl.src_blob,
+                                        c->cfg.precision, c->p.n};
+  auto it = c->split_cache.find(key);
+  if (it != c->split_cache.end()) return it->second;
+  int best = heur;
+  if (!tune_split(c, pl, heur, &best).ok()) best = heur;  // timing failed: keep the heuristic
+  c->split_cache[key] = best;
+  return best;
 }
 
 }  // namespace cfm
@@ -123,6 +140,23 @@ static int fail(cfm_ctx* c, const Status& s) {
   else g_create_error = s.msg;
   return s.code;
 }
+
+namespace cfm {
+Status ctx_rocblas(cfm_ctx* c, void** handle) {
+  if (!c->rocblas) {
+    rocblas_handle h = nullptr;
+    if (rocblas_create_handle(&h) != rocblas_status_success)
+      return make_status(CFM_EHIP, "rocblas_create_handle failed");
+    if (rocblas_set_stream(h, c->stream) != rocblas_status_success) {
+      (void)rocblas_destroy_handle(h);
+      return make_status(CFM_EHIP, "rocblas_set_stream failed");
+    }
+    c->rocblas = h;
+  }
+  *handle = c->rocblas;
+  return {};
+}
+}  // namespace cfm
 
 extern "C" {
 
@@ -147,6 +181,8 @@ int cfm_create(const cfm_config* cfg, cfm_ctx** out) {
                                      "reference's cuda_functions.cu:22)"));
   if (cfg->precision != CFM_FP64 && cfg->precision != CFM_FP32C)
     return fail(nullptr, make_status(CFM_EINVAL, "unknown precision %d", cfg->precision));
+  if (cfg->tile_split < -1 || cfg->tile_split > 64)
+    return fail(nullptr, make_status(CFM_EINVAL, "tile_split must be -1 (autotune), 0 or 1..64"));
   if ((int64_t)cfg->n_seg + cfg->n_blob > (int64_t)1 << 28)
     return fail(nullptr, make_status(CFM_EINVAL, "too many particles"));
   int ndev = 0;
@@ -193,6 +229,10 @@ int cfm_destroy(cfm_ctx* c) {
   cfm_fil_release(c);
   cfm_gnn_release(c);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->rocblas) (void)rocblas_destroy_handle((rocblas_handle)c->rocblas);
+  c->lu_piv.release();
+  c->lu_info.release();
+  c->tune_v.release();
   c->p.x.release();
   c->p.y.release();
   c->p.z.release();

@@ -20,6 +20,9 @@
 //
 // The self term of the wall branch is keyed on the global particle id and is
 // evaluated in double during finalisation.
+#include <algorithm>
+#include <vector>
+
 #include "ctx.hpp"
 #include "apply_tile.hpp"
 #include "rpy_pair.hpp"
@@ -348,6 +351,49 @@ int plan_tiles(const cfm_ctx* c, const ApplyPlan& plan) {
   if (seg_hi > seg_lo) t += (seg_hi - seg_lo + kTT - 1) / kTT;
   if (blob_hi > blob_lo) t += (blob_hi - blob_lo + kTT - 1) / kTT;
   return t;
+}
+
+// Autotuner of the source split: every candidate writes a scratch output (the
+// product is never accumulated twice) and the fastest of two timed launches wins.
+Status tune_split(cfm_ctx* c, const ApplyPlan& pl, int heuristic, int* best) {
+  const Particles& p = c->p;
+  int nsrc = (pl.src_seg ? p.n_seg : 0) + (pl.src_blob ? p.n_blob : 0);
+  const int stiles = std::max(1, (nsrc + kTS - 1) / kTS);
+  std::vector<int> cand = {heuristic};
+  for (int s : {1, 2, 3, 4, 6, 8, 12, 16, 24, 32})
+    if (s <= stiles && s != heuristic) cand.push_back(s);
+  CFM_TRY(c->tune_v.alloc((size_t)6 * p.n));
+  ApplyPlan t = pl;
+  t.acc_seg = t.acc_blob = false;
+  t.v_ext = c->tune_v.p;
+  t.v_ld = p.n;
+  t.v_off = 0;
+  const bool prof = c->prof_on;
+  c->prof_on = false;
+  float best_ms = 1e30f;
+  Status st;
+  for (int s : cand) {
+    c->split = s;
+    st = launch_apply(c, t);  // warm-up (sizes the split partials)
+    if (!st.ok()) break;
+    if (hipEventRecord(c->ev0, c->stream) != hipSuccess) { st = make_status(CFM_EHIP, "event"); break; }
+    st = launch_apply(c, t);
+    if (st.ok()) st = launch_apply(c, t);
+    if (!st.ok()) break;
+    if (hipEventRecord(c->ev1, c->stream) != hipSuccess || hipEventSynchronize(c->ev1) != hipSuccess) {
+      st = make_status(CFM_EHIP, "event");
+      break;
+    }
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    if (ms < best_ms) {
+      best_ms = ms;
+      *best = s;
+    }
+  }
+  c->prof_on = prof;
+  c->split = *best;
+  return st;
 }
 
 }  // namespace cfm

@@ -18,6 +18,8 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <numeric>
@@ -893,7 +895,15 @@ int cfm_batch_destroy(cfm_batch* b) {
   if (b->host) (void)hipHostFree(b->host);
   if (b->host_flags) (void)hipHostFree(b->host_flags);
   if (b->stream) (void)hipStreamDestroy(b->stream);
-  delete b;  // DevBuf members free their memory
+  for (auto* d : {&b->x, &b->y, &b->z, &b->f, &b->v, &b->partial, &b->Q, &b->H, &b->cs, &b->sn,
+                  &b->beta, &b->h, &b->h2, &b->yv, &b->kpart, &b->sinfo, &b->r, &b->w, &b->zt,
+                  &b->xs, &b->dv, &b->rhs, &b->otarget})
+    d->release();
+  for (auto* d : {&b->sys_of, &b->nslice, &b->counters, &b->chunk_begin, &b->iinfo, &b->jdev})
+    d->release();
+  b->items.release();
+  b->chunks.release();
+  delete b;
   return CFM_OK;
 }
 
@@ -1047,6 +1057,7 @@ int cfm_batch_gmres(cfm_batch* b, const double* rhs, double* xout, const cfm_sol
   const int max_iter = o->max_iter > 0 ? o->max_iter : 1000;
   const int pc = o->precond;
   const bool mixed = o->mode == CFM_SOLVE_MIXED;
+  const bool debug = std::getenv("CFM_BATCH_DEBUG") != nullptr;
   double elapsed = 0.0;
   auto solve = [&]() -> Status {
     CFM_TRY(reserve_krylov(b, restart));
@@ -1088,6 +1099,17 @@ int cfm_batch_gmres(cfm_batch* b, const double* rhs, double* xout, const cfm_sol
       CFM_HIP(hipMemcpyAsync(b->host_flags, b->jdev.p, sizeof(int) * 4, hipMemcpyDeviceToHost, b->stream));
       CFM_HIP(hipStreamSynchronize(b->stream));
       std::memcpy(jd, b->host_flags, sizeof(jd));
+      if (debug) {
+        std::vector<double> si((size_t)8 * b->n_sys);
+        std::vector<int> ii((size_t)8 * b->n_sys);
+        CFM_HIP(hipMemcpy(si.data(), b->sinfo.p, sizeof(double) * si.size(), hipMemcpyDeviceToHost));
+        CFM_HIP(hipMemcpy(ii.data(), b->iinfo.p, sizeof(int) * ii.size(), hipMemcpyDeviceToHost));
+        std::fprintf(stderr, "[batch] pass %d running %d j %d steps %d\n", pass, jd[2], jd[0], jd[1]);
+        for (int q = 0; q < b->n_sys; ++q)
+          std::fprintf(stderr, "  sys %d st %d it %d k %d nr %.3e tgt %.3e est %.3e nb %.3e\n", q,
+                       ii[8 * q], ii[8 * q + II_ITERS], ii[8 * q + II_K], si[8 * q + SI_NR],
+                       si[8 * q + SI_TARGET], si[8 * q + SI_EST], si[8 * q + SI_NB]);
+      }
       if (jd[2] == 0) break;  // nobody left running
       CFM_TRY(run_cycle(b, g, &nev, b->m));
       if (mixed) {

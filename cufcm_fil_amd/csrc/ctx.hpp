@@ -3,7 +3,9 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdarg>
+#include <array>
 #include <cstdio>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -133,6 +135,11 @@ struct ApplyPlan {
 }  // namespace cfm
 
 namespace cfm { struct Krylov; }
+struct cfm_ctx;
+namespace cfm {
+// The context's rocBLAS handle (rocSOLVER calls), bound to the context's stream.
+Status ctx_rocblas(cfm_ctx* c, void** handle);
+}
 
 struct cfm_ctx {
   cfm_config cfg{};
@@ -152,6 +159,11 @@ struct cfm_ctx {
   const double* nf_corr = nullptr;      // learned correction blocks (same layout as nf.W) or null
   cfm::DenseInverse dinv;               // resident inverse of the blob system (CFM_PC_DENSE)
   std::vector<double> blob_pos;         // blob positions (AoS) of the last cfm_set_positions
+  void* rocblas = nullptr;              // rocblas_handle on `stream` (created on first use)
+  // autotuned source splits (tile_split = -1), keyed on the product's shape
+  std::map<std::array<long long, 6>, int> split_cache;
+  cfm::DevBuf<double> tune_v;           // scratch output of the tuning launches
+  cfm::DevBuf<int> lu_piv, lu_info;     // batched LU pivots / status (filament Jacobians)
   // pair-kernel profiling (HIP events around each launch)
   bool prof_on = false;
   std::vector<hipEvent_t> prof_events;  // pool, two per launch

@@ -110,13 +110,6 @@ __global__ __launch_bounds__(256) void dense_gemv_kernel(int n, size_t ld,
   }
 }
 
-struct RocblasHandle {
-  rocblas_handle h = nullptr;
-  ~RocblasHandle() {
-    if (h) (void)rocblas_destroy_handle(h);
-  }
-};
-
 }  // namespace
 
 Status dense_build(cfm_ctx* c, DenseInverse& d) {
@@ -137,25 +130,30 @@ Status dense_build(cfm_ctx* c, DenseInverse& d) {
   d.release();
   DevBuf<double> A;
   DevBuf<int> info;
+  struct Release {  // the FP64 factor is scratch: freed on every return
+    DevBuf<double>& a;
+    DevBuf<int>& i;
+    ~Release() {
+      a.release();
+      i.release();
+    }
+  } release_scratch{A, info};
   CFM_TRY(A.alloc(lda * n));
   CFM_TRY(info.alloc(1));
   hipLaunchKernelGGL(dense_assemble_kernel, dim3((m + 255) / 256, m), dim3(256), 0, c->stream, p.x.p,
                      p.y.p, p.z.p, p.n_seg, m, c->cfg.wall, c->cfg.mu, c->cfg.a_blob, A.p, lda);
   CFM_CHECK_LAUNCH();
-  RocblasHandle rb;
-  if (rocblas_create_handle(&rb.h) != rocblas_status_success)
-    return make_status(CFM_EHIP, "rocblas_create_handle failed");
-  if (rocblas_set_stream(rb.h, c->stream) != rocblas_status_success)
-    return make_status(CFM_EHIP, "rocblas_set_stream failed");
+  rocblas_handle hb = nullptr;
+  CFM_TRY(ctx_rocblas(c, (void**)&hb));
   int hinfo = 0;
-  if (rocsolver_dpotrf(rb.h, rocblas_fill_lower, n, A.p, (rocblas_int)lda, info.p) !=
+  if (rocsolver_dpotrf(hb, rocblas_fill_lower, n, A.p, (rocblas_int)lda, info.p) !=
       rocblas_status_success)
     return make_status(CFM_EHIP, "rocsolver_dpotrf failed");
   CFM_HIP(hipMemcpyAsync(&hinfo, info.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
   CFM_HIP(hipStreamSynchronize(c->stream));
   if (hinfo != 0)
     return make_status(CFM_ENONFINITE, "blob mobility not positive definite (dpotrf info %d)", hinfo);
-  if (rocsolver_dpotri(rb.h, rocblas_fill_lower, n, A.p, (rocblas_int)lda, info.p) !=
+  if (rocsolver_dpotri(hb, rocblas_fill_lower, n, A.p, (rocblas_int)lda, info.p) !=
       rocblas_status_success)
     return make_status(CFM_EHIP, "rocsolver_dpotri failed");
   CFM_HIP(hipMemcpyAsync(&hinfo, info.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
@@ -166,7 +164,6 @@ Status dense_build(cfm_ctx* c, DenseInverse& d) {
                      c->stream, A.p, lda, n, d.P.p, ld);
   CFM_CHECK_LAUNCH();
   CFM_HIP(hipStreamSynchronize(c->stream));
-  A.release();
   d.n = n;
   d.ld = ld;
   d.valid = true;

@@ -17,6 +17,9 @@
 // elimination with partial pivoting; its transpose is stored so that the inverse
 // products read HBM coalesced.
 #include "ctx.hpp"
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
 #include "fil.hpp"
 #include "fil_device.hpp"
 
@@ -307,12 +310,11 @@ __device__ __forceinline__ M3 frame_sum(V3 Kv, Q4 q) {
   return madd(A, rcross(binormal(q)), Kv.z);
 }
 
-__global__ __launch_bounds__(256) void fil_jacobian_kernel(FilK K, int nt, int* __restrict__ status) {
+// Approximate Jacobian of one filament (filament.cpp:1855-2037 blocks, swimmer.cpp:
+// 539-560 assembly), built in LDS and stored column-major in K.jinv; the batched LU
+// inversion (fil_jacobian) then overwrites it with the inverse.
+__global__ __launch_bounds__(256) void fil_jacobian_kernel(FilK K, int nt) {
   extern __shared__ double J[];
-  __shared__ double colk[128];
-  __shared__ int piv[128];
-  __shared__ double pval;
-  __shared__ int prow;
   const int f = blockIdx.x;
   const int n = K.nseg, N = 6 * n;
   const size_t ns = K.nseg_total;
@@ -413,72 +415,17 @@ __global__ __launch_bounds__(256) void fil_jacobian_kernel(FilK K, int nt, int* 
     if (k <= n - 2) put_block(J, N, r0, 3 * (n + k + 1), Rb);
   }
   __syncthreads();
-
-  // ---- in-place Gauss-Jordan inversion with partial pivoting ----
-  for (int k = 0; k < N; ++k) {
-    if (tid < 64) {
-      double best = -1.0;
-      int bi = k;
-      for (int r = k + tid; r < N; r += 64) {
-        const double a = fabs(J[r * N + k]);
-        if (a > best) {
-          best = a;
-          bi = r;
-        }
-      }
-      for (int off = 32; off > 0; off >>= 1) {
-        const double ob = __shfl_xor(best, off, 64);
-        const int oi = __shfl_xor(bi, off, 64);
-        if (ob > best || (ob == best && oi < bi)) {
-          best = ob;
-          bi = oi;
-        }
-      }
-      if (tid == 0) {
-        prow = bi;
-        piv[k] = bi;
-      }
-    }
-    __syncthreads();
-    const int p = prow;
-    if (p != k)
-      for (int c = tid; c < N; c += nth) {
-        const double t = J[k * N + c];
-        J[k * N + c] = J[p * N + c];
-        J[p * N + c] = t;
-      }
-    __syncthreads();
-    if (tid == 0) pval = J[k * N + k];
-    for (int r = tid; r < N; r += nth) colk[r] = J[r * N + k];
-    __syncthreads();
-    const double d = pval;
-    if (tid == 0 && !(fabs(d) > 0.0)) status[0] = 1;  // singular
-    for (int c = tid; c < N; c += nth) J[k * N + c] = (c == k) ? 1.0 / d : J[k * N + c] / d;
-    __syncthreads();
-    const double inv = J[k * N + k];
-    for (int e = tid; e < N * N; e += nth) {
-      const int r = e / N, c = e - r * N;
-      if (r == k) continue;
-      J[e] = (c == k) ? -colk[r] * inv : J[e] - colk[r] * J[k * N + c];
-    }
-    __syncthreads();
-  }
-  for (int k = N - 1; k >= 0; --k) {
-    const int p = piv[k];
-    if (p != k)
-      for (int r = tid; r < N; r += nth) {
-        const double t = J[r * N + k];
-        J[r * N + k] = J[r * N + p];
-        J[r * N + p] = t;
-      }
-    __syncthreads();
-  }
-  // store the transpose: jinvT[f][c][r] = Jinv[r][c]
+  // column-major copy (element (r, c) at c N + r) for the batched LU inversion
   double* out = K.jinv + (size_t)f * N * N;
   for (int e = tid; e < N * N; e += nth) {
     const int c = e / N, r = e - c * N;
     out[e] = J[r * N + c];
   }
+}
+
+__global__ void info_any_kernel(const int* __restrict__ info, int n, int* __restrict__ status) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && info[i] != 0) status[0] = 1;
 }
 
 // out_f = scale * J0^-1 in_f for every filament (vectors in the reference layout)
@@ -563,32 +510,55 @@ __global__ void grid_count_kernel(GridPos P, double inv_h, int mask, int* __rest
   atomicAdd(count + b, 1);
 }
 
-// Exclusive scan of the bucket counts (one workgroup of 1024 threads); writes the
-// offsets to start[0..nb] and a copy of start[0..nb) to cursor.
-__global__ __launch_bounds__(1024) void grid_scan_kernel(int* __restrict__ start,
-                                                         int* __restrict__ cursor, int nb) {
-  __shared__ int part[1024];
-  const int t = threadIdx.x;
-  const int per = (nb + 1023) / 1024;
-  const int b0 = t * per, b1 = min(nb, b0 + per);
-  int s = 0;
-  for (int b = b0; b < b1; ++b) s += start[b];
-  part[t] = s;
+// Exclusive scan of the bucket counts in two passes over blocks of 1024 buckets
+// (nb is a power of two >= 1024).  Pass 1: each block scans its buckets in place
+// (int4 loads, wave prefix sums by lane shuffles) and writes its total.
+__global__ __launch_bounds__(256) void grid_scan_blocks_kernel(int* __restrict__ start,
+                                                               int* __restrict__ blocksum) {
+  __shared__ int wsum[4];
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  int4* p = reinterpret_cast<int4*>(start + (size_t)blockIdx.x * 1024) + t;
+  int4 v = *p;
+  const int tot = v.x + v.y + v.z + v.w;
+  int inc = tot;  // inclusive prefix over the wave
+  for (int o = 1; o < 64; o <<= 1) {
+    const int u = __shfl_up(inc, o, 64);
+    if (lane >= o) inc += u;
+  }
+  if (lane == 63) wsum[wave] = inc;
   __syncthreads();
-  for (int o = 1; o < 1024; o <<= 1) {
-    const int v = t >= o ? part[t - o] : 0;
-    __syncthreads();
-    part[t] += v;
-    __syncthreads();
+  int base = 0;
+  for (int w = 0; w < wave; ++w) base += wsum[w];
+  const int ex = base + inc - tot;
+  int4 o;
+  o.x = ex;
+  o.y = ex + v.x;
+  o.z = o.y + v.y;
+  o.w = o.z + v.z;
+  *p = o;
+  if (t == 0) blocksum[blockIdx.x] = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+}
+
+// Pass 2: add the totals of the preceding blocks, copy to the scatter cursors and
+// close the table with start[nb] = number of particles.
+__global__ __launch_bounds__(256) void grid_scan_add_kernel(int* __restrict__ start,
+                                                            int* __restrict__ cursor,
+                                                            const int* __restrict__ blocksum,
+                                                            int nb) {
+  __shared__ int off;
+  if (threadIdx.x == 0) {
+    int acc = 0;
+    for (int b = 0; b < (int)blockIdx.x; ++b) acc += blocksum[b];
+    off = acc;
+    if (blockIdx.x == gridDim.x - 1) start[nb] = acc + blocksum[blockIdx.x];
   }
-  int run = part[t] - s;  // exclusive prefix of this thread's chunk
-  for (int b = b0; b < b1; ++b) {
-    const int cnt = start[b];
-    start[b] = run;
-    cursor[b] = run;
-    run += cnt;
+  __syncthreads();
+  for (int e = threadIdx.x; e < 1024; e += 256) {
+    const size_t i = (size_t)blockIdx.x * 1024 + e;
+    const int v = start[i] + off;
+    start[i] = v;
+    cursor[i] = v;
   }
-  if (t == 1023) start[nb] = part[1023];
 }
 
 __global__ void grid_scatter_kernel(int n, const int* __restrict__ key, int* __restrict__ cursor,
@@ -673,6 +643,7 @@ Status BarrierGrid::ensure(int n_particles) {
   if (nb != nbucket) {
     CFM_TRY(start.alloc((size_t)nb + 1));
     CFM_TRY(cursor.alloc((size_t)nb));
+    CFM_TRY(blocksum.alloc((size_t)nb / 1024));
     nbucket = nb;
   }
   CFM_TRY(ids.alloc((size_t)std::max(1, n_particles)));
@@ -706,7 +677,10 @@ Status fil_barrier(cfm_ctx* c, const FilK& K, BarrierGrid& g) {
   hipLaunchKernelGGL(grid_clear_kernel, gsz(nb + 1, 256), dim3(256), 0, c->stream, g.start.p, nb + 1);
   hipLaunchKernelGGL(grid_count_kernel, gsz(n, 256), dim3(256), 0, c->stream, P, inv_h, mask,
                      g.key.p, g.start.p);
-  hipLaunchKernelGGL(grid_scan_kernel, dim3(1), dim3(1024), 0, c->stream, g.start.p, g.cursor.p, nb);
+  hipLaunchKernelGGL(grid_scan_blocks_kernel, dim3(nb / 1024), dim3(256), 0, c->stream, g.start.p,
+                     g.blocksum.p);
+  hipLaunchKernelGGL(grid_scan_add_kernel, dim3(nb / 1024), dim3(256), 0, c->stream, g.start.p,
+                     g.cursor.p, g.blocksum.p, nb);
   hipLaunchKernelGGL(grid_scatter_kernel, gsz(n, 256), dim3(256), 0, c->stream, n, g.key.p,
                      g.cursor.p, g.ids.p);
   hipLaunchKernelGGL(grid_sort_kernel, gsz(nb, 256), dim3(256), 0, c->stream, nb, g.start.p, g.ids.p);
@@ -735,7 +709,26 @@ Status fil_jacobian(cfm_ctx* c, const FilK& K, int nt, int* status_dev) {
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     attr_set = true;
   }
-  hipLaunchKernelGGL(fil_jacobian_kernel, dim3(K.nfil), dim3(256), lds, c->stream, K, nt, status_dev);
+  hipLaunchKernelGGL(fil_jacobian_kernel, dim3(K.nfil), dim3(256), lds, c->stream, K, nt);
+  CFM_CHECK_LAUNCH();
+  // J0^-1 of every filament in place: LU with partial pivoting, then the inverse
+  // from the factors (the reference inverts each block with LAPACK getrf/getri,
+  // matrix.cpp:538-571)
+  rocblas_handle h = nullptr;
+  CFM_TRY(ctx_rocblas(c, (void**)&h));
+  CFM_TRY(c->lu_piv.alloc((size_t)K.nfil * N));
+  CFM_TRY(c->lu_info.alloc((size_t)K.nfil));
+  const rocblas_stride sA = (rocblas_stride)N * N, sP = N;
+  if (rocsolver_dgetrf_strided_batched(h, N, N, K.jinv, N, sA, c->lu_piv.p, sP, c->lu_info.p,
+                                       K.nfil) != rocblas_status_success)
+    return make_status(CFM_EHIP, "rocsolver_dgetrf_strided_batched failed");
+  hipLaunchKernelGGL(info_any_kernel, gsz(K.nfil, 256), dim3(256), 0, c->stream, c->lu_info.p,
+                     K.nfil, status_dev);
+  if (rocsolver_dgetri_strided_batched(h, N, K.jinv, N, sA, c->lu_piv.p, sP, c->lu_info.p,
+                                       K.nfil) != rocblas_status_success)
+    return make_status(CFM_EHIP, "rocsolver_dgetri_strided_batched failed");
+  hipLaunchKernelGGL(info_any_kernel, gsz(K.nfil, 256), dim3(256), 0, c->stream, c->lu_info.p,
+                     K.nfil, status_dev);
   CFM_CHECK_LAUNCH();
   return {};
 }

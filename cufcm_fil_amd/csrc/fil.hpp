@@ -34,9 +34,10 @@ struct FilK {
 // bucket offsets (nbucket + 1 entries).
 struct BarrierGrid {
   int nbucket = 0;
-  DevBuf<int> start, cursor, ids, key;
+  DevBuf<int> start, cursor, ids, key, blocksum;
   Status ensure(int n_particles);
   void release() {
+    blocksum.release();
     start.release();
     cursor.release();
     ids.release();

@@ -25,7 +25,7 @@
 namespace cfm {
 
 Status launch_apply(cfm_ctx* c, const ApplyPlan& plan);
-int choose_split(const cfm_ctx* c, const ApplyPlan& pl);
+int choose_split(cfm_ctx* c, const ApplyPlan& pl);
 ApplyPlan plan_for(const cfm_ctx* c, int which);
 
 bool gnn_recording(cfm_ctx* c);
@@ -263,6 +263,7 @@ Status Krylov::reserve(int L_, int m_) {
   L = std::max(L, L_);
   m = std::max(m, m_);
   CFM_TRY(Q.alloc((size_t)(m + 1) * L));
+  CFM_TRY(Z.alloc((size_t)m * L));
   CFM_TRY(H.alloc((size_t)(m + 1) * (m + 1)));
   CFM_TRY(cs.alloc(m + 1));
   CFM_TRY(sn.alloc(m + 1));
@@ -286,9 +287,9 @@ Status Krylov::reserve(int L_, int m_) {
 }
 
 void Krylov::release() {
-  Q.release(); H.release(); cs.release(); sn.release(); beta.release(); h.release();
+  Q.release(); Z.release(); H.release(); cs.release(); sn.release(); beta.release(); h.release();
   h2.release(); y.release(); scal.release(); r.release(); w.release(); z.release();
-  t.release(); d.release(); partial.release(); res_ring.release();
+  t.release(); d.release(); dx.release(); rhs.release(); partial.release(); res_ring.release();
   for (int i = 0; i < 4; ++i)
     if (ev[i]) (void)hipEventDestroy(ev[i]);
   if (host_res) (void)hipHostFree(host_res);
@@ -381,25 +382,32 @@ Status system_precond(cfm_ctx* c, const SystemDesc& s, int kind, const double* x
 // Stops when the residual estimate reaches `target_abs`, after max_iter steps in
 // total, or after `max_cycles` cycles.  On return K.r holds b - A x (operator
 // `prec`) and *res_abs its norm, except after max_cycles: then *res_abs is the last
-// residual estimate and K.r is stale.
+// residual estimate and K.r is stale.  With r_given the caller has already put
+// b - A x into K.r and its norm into *res_abs (no initial product).
 Status gmres_core(cfm_ctx* c, Krylov& K, const SystemDesc& s, int prec, int pc, const double* b,
                   double* x, double target_abs, int restart, int max_iter, int max_cycles,
-                  int* iters, double* res_abs, const GmresHooks* hooks) {
+                  int* iters, double* res_abs, const GmresHooks* hooks, bool r_given) {
   const int L = s.L;
   const int m = restart;
   CFM_TRY(K.reserve(L, m));
   const size_t ldq = (size_t)K.L;
   const int ldh = m + 1;
   double nr = 0.0;
-  // r = b - A x
-  CFM_TRY(system_apply(c, s, prec, x, K.w.p));
-  hipLaunchKernelGGL(axpby_kernel, g1(L), dim3(256), 0, c->stream, 1.0, b, -1.0, K.w.p, K.r.p, L);
-  CFM_CHECK_LAUNCH();
-  CFM_TRY(norm2(c, K, K.r.p, L, K.scal.p));
-  CFM_TRY(read_scalar(c, K, K.scal.p, &nr));
+  if (r_given) {
+    nr = *res_abs;
+    CFM_TRY(norm2(c, K, K.r.p, L, K.scal.p));  // device copy of ||r|| for q0
+  } else {
+    // r = b - A x
+    CFM_TRY(system_apply(c, s, prec, x, K.w.p));
+    hipLaunchKernelGGL(axpby_kernel, g1(L), dim3(256), 0, c->stream, 1.0, b, -1.0, K.w.p, K.r.p, L);
+    CFM_CHECK_LAUNCH();
+    CFM_TRY(norm2(c, K, K.r.p, L, K.scal.p));
+    CFM_TRY(read_scalar(c, K, K.scal.p, &nr));
+  }
   if (!std::isfinite(nr)) return make_status(CFM_ENONFINITE, "non-finite GMRES residual");
   int cycles = 0;
   double last_est = nr;
+  const bool sync_check = pc == CFM_PC_DENSE;
   while (nr > target_abs && *iters < max_iter) {
     if (s.which == CFM_BB && gnn_recording(c)) CFM_TRY(gnn_record_rhs(c, K.r.p));
     // q0 = r / ||r||
@@ -415,9 +423,10 @@ Status gmres_core(cfm_ctx* c, Krylov& K, const SystemDesc& s, int prec, int pc, 
     for (int j = 0; j < m && !stop; ++j) {
       double* qj = K.Q.p + (size_t)j * ldq;
       double* qn = K.Q.p + (size_t)(j + 1) * ldq;
-      CFM_TRY(system_precond(c, s, pc, qj, K.z.p));
-      if (hooks && hooks->precond_correction) CFM_TRY(hooks->precond_correction(c, s, qj, K.z.p, hooks->user));
-      CFM_TRY(system_apply(c, s, prec, K.z.p, K.w.p));
+      double* zj = K.Z.p + (size_t)j * ldq;
+      CFM_TRY(system_precond(c, s, pc, qj, zj));
+      if (hooks && hooks->precond_correction) CFM_TRY(hooks->precond_correction(c, s, qj, zj, hooks->user));
+      CFM_TRY(system_apply(c, s, prec, zj, K.w.p));
       // classical Gram-Schmidt, twice
       hipLaunchKernelGGL(dots_partial_kernel, dim3(j + 1, nb), dim3(kRB), 0, c->stream, K.Q.p, ldq, j + 1, K.w.p, L, K.partial.p);
       hipLaunchKernelGGL(cgs_finish_sub_kernel, g1(L), dim3(256), 0, c->stream, K.Q.p, ldq, j + 1, K.partial.p, nb, K.h.p, (const double*)nullptr, K.w.p, L);
@@ -436,7 +445,14 @@ Status gmres_core(cfm_ctx* c, Krylov& K, const SystemDesc& s, int prec, int pc, 
       CFM_HIP(hipEventRecord(K.ev[j % 4], c->stream));
       ++*iters;
       k = j + 1;
-      if (pending >= 0) {
+      if (sync_check) {
+        // few, expensive steps (resident inverse): read this step's estimate now
+        CFM_HIP(hipEventSynchronize(K.ev[j % 4]));
+        const double est = K.host_res[4 + j % 4];
+        if (!std::isfinite(est)) return make_status(CFM_ENONFINITE, "non-finite GMRES iterate");
+        last_est = est;
+        if (est <= target_abs) stop = true;
+      } else if (pending >= 0) {
         CFM_HIP(hipEventSynchronize(K.ev[pending % 4]));
         const double est = K.host_res[4 + pending % 4];
         if (!std::isfinite(est)) return make_status(CFM_ENONFINITE, "non-finite GMRES iterate");
@@ -446,15 +462,20 @@ Status gmres_core(cfm_ctx* c, Krylov& K, const SystemDesc& s, int prec, int pc, 
       pending = j;
       if (*iters >= max_iter) stop = true;
     }
-    // x += P^-1 Q y
+    // x += P^-1 Q y = Z y
     hipLaunchKernelGGL(backsub_kernel, dim3(1), dim3(1), 0, c->stream, K.H.p, ldh, K.beta.p, k, K.y.p);
-    hipLaunchKernelGGL(gemv_kernel, g1(L), dim3(256), 0, c->stream, K.Q.p, ldq, k, K.y.p, K.t.p, L);
-    CFM_CHECK_LAUNCH();
-    CFM_TRY(system_precond(c, s, pc, K.t.p, K.z.p));
-    if (hooks && hooks->precond_correction) CFM_TRY(hooks->precond_correction(c, s, K.t.p, K.z.p, hooks->user));
+    hipLaunchKernelGGL(gemv_kernel, g1(L), dim3(256), 0, c->stream, K.Z.p, ldq, k, K.y.p, K.z.p, L);
     hipLaunchKernelGGL(axpby_kernel, g1(L), dim3(256), 0, c->stream, 1.0, x, 1.0, K.z.p, x, L);
     CFM_CHECK_LAUNCH();
     if (++cycles >= max_cycles) {  // caller recomputes the residual itself
+      nr = last_est;
+      break;
+    }
+    // With the FP64 operator and the estimate of the final step in hand (read
+    // synchronously), an estimate at most half the target ends the
+    // solve without another product: the recurrence residual and b - A x then agree
+    // to rounding (the reference stops on the estimate alone, :2510).
+    if (sync_check && prec == CFM_FP64 && last_est <= 0.5 * target_abs) {
       nr = last_est;
       break;
     }
@@ -512,8 +533,12 @@ Status solve(cfm_ctx* c, Krylov& K, const SystemDesc& s, const cfm_solve_opts& o
   int iters = 0;
   if (o.mode == CFM_SOLVE_FP64) {
     double res = nr;
-    if (nr > target) CFM_TRY(gmres_core(c, K, s, CFM_FP64, o.precond, b, x, target, restart,
-                                        max_iter, 1 << 30, &iters, &res, hooks));
+    if (nr > target) {
+      // K.d holds b - A x of the (possibly rejected) guess
+      CFM_HIP(hipMemcpyAsync(K.r.p, K.d.p, sizeof(double) * L, hipMemcpyDeviceToDevice, c->stream));
+      CFM_TRY(gmres_core(c, K, s, CFM_FP64, o.precond, b, x, target, restart, max_iter, 1 << 30,
+                         &iters, &res, hooks, true));
+    }
     nr = res;
   } else {
     // iterative refinement: d solves A32 d = r to a relative 5e-7 (or what is left
@@ -534,8 +559,11 @@ Status solve(cfm_ctx* c, Krylov& K, const SystemDesc& s, const cfm_solve_opts& o
       // one restart cycle per refinement: the FP32C operator is linear only to its
       // rounding, so an inner solve can stall short of rel_in; the FP64 residual
       // below decides what is left.
+      // d starts at zero, so the inner residual is r itself
+      CFM_HIP(hipMemcpyAsync(K.r.p, K.d.p, sizeof(double) * L, hipMemcpyDeviceToDevice, c->stream));
+      inner_res = nr;
       CFM_TRY(gmres_core(c, K, s, CFM_FP32C, o.precond, K.rhs.p, dvec.p, rel_in * nr, restart,
-                         max_iter, 1, &iters, &inner_res, hooks));
+                         max_iter, 1, &iters, &inner_res, hooks, true));
       hipLaunchKernelGGL(axpby_kernel, g1(L), dim3(256), 0, c->stream, 1.0, x, 1.0, dvec.p, x, L);
       CFM_CHECK_LAUNCH();
       CFM_TRY(system_apply(c, s, CFM_FP64, x, K.w.p));

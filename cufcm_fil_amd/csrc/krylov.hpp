@@ -7,7 +7,9 @@ namespace cfm {
 // Krylov workspace (device), sized for vectors of length L and m basis vectors.
 struct Krylov {
   int L = 0, m = 0;
-  DevBuf<double> Q, H, cs, sn, beta, h, h2, y, scal, r, w, z, t, d, dx, rhs, partial, res_ring;
+  // Q: Arnoldi basis; Z: its preconditioned images P^-1 q_j, kept so the update
+  // x += P^-1 Q y is the product Z y (no further preconditioner application)
+  DevBuf<double> Q, Z, H, cs, sn, beta, h, h2, y, scal, r, w, z, t, d, dx, rhs, partial, res_ring;
   double* host_res = nullptr;  // pinned scalar read-back
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   Status reserve(int L_, int m_);

@@ -21,7 +21,7 @@
 namespace cfm {
 
 Status launch_apply(cfm_ctx* c, const ApplyPlan& plan);
-int choose_split(const cfm_ctx* c, const ApplyPlan& pl);
+int choose_split(cfm_ctx* c, const ApplyPlan& pl);
 Status upload_aos(cfm_ctx* c, const double* host, int cnt, int dof, double* dst, int ld, int off);
 Status download_aos(cfm_ctx* c, double* host, int cnt, int dof, const double* src, int ld, int off);
 Status gnn_guess(cfm_ctx* c, const double* rhs, double* x0, bool* used);

@@ -44,18 +44,39 @@ def loads(n, dof, seed):
 
 
 def tilted_lattice(nx=8, ny=8, spacing=10.0, nseg=NSEG, z0=1.1, max_polar=0.3, seed=0,
-                   return_tangents=False):
+                   return_tangents=False, min_sep=None, count=None):
+    """Straight filaments on an nx x ny lattice, each tilted by a random polar angle
+    in [0, max_polar) and azimuth in [0, 2 pi).  With min_sep, a filament's tilt is
+    redrawn until none of its segments is closer than min_sep to a segment of an
+    earlier filament (tilted neighbours can otherwise cross).  count keeps only the
+    first `count` lattice sites."""
     rng = np.random.default_rng(seed)
     fils, tans = [], []
-    for ix in range(nx):
-        for iy in range(ny):
+    sites = [(ix, iy) for ix in range(nx) for iy in range(ny)]
+    if count is not None:
+        sites = sites[:count]
+    for ix, iy in sites:
+        base = np.array([ix * spacing, iy * spacing, z0])
+        placed = np.concatenate(fils) if (min_sep and fils) else None
+        for attempt in range(1000):
             th = rng.uniform(0.0, max_polar)
             ph = rng.uniform(0.0, 2.0 * math.pi)
             t = np.array([math.sin(th) * math.cos(ph), math.sin(th) * math.sin(ph), math.cos(th)])
-            base = np.array([ix * spacing, iy * spacing, z0])
             tt = np.repeat(t[None], nseg, axis=0)
-            fils.append(_chain(base, tt))
-            tans.append(tt)
+            chain = _chain(base, tt)
+            if placed is None:
+                break
+            near = placed[np.abs(placed[:, 0] - base[0]) + np.abs(placed[:, 1] - base[1])
+                          < 2.0 * nseg * DL]
+            if near.size == 0:
+                break
+            d2 = ((chain[:, None, :] - near[None, :, :]) ** 2).sum(-1)
+            if d2.min() >= min_sep * min_sep:
+                break
+        else:
+            raise RuntimeError("could not place a non-overlapping filament")
+        fils.append(chain)
+        tans.append(tt)
     x = np.concatenate(fils).reshape(-1)
     return (x, np.concatenate(tans)) if return_tangents else x
 
@@ -118,14 +139,16 @@ def sphere_over_wall(n_blob=6000, n_fil=1300, nseg=NSEG, radius=40.0, gap=50.0,
 
 
 def ensemble(n_sys=64, lo=100, hi=4000, seed=3):
-    """List of (n_seg, x_seg) for independent wall systems of uneven size."""
+    """List of (n_seg, x_seg) for independent wall systems of uneven size: lattices of
+    tilted filaments as tilted_lattice, with no two segments of different filaments
+    closer than one segment spacing."""
     rng = np.random.default_rng(seed)
     out = []
     for s in range(n_sys):
         nfil = int(rng.integers(lo // NSEG, hi // NSEG + 1))
         side = int(math.ceil(math.sqrt(nfil)))
-        x = tilted_lattice(nx=side, ny=side, seed=1000 + s).reshape(-1, NSEG, 3)[:nfil]
-        out.append((nfil * NSEG, x.reshape(-1)))
+        x = tilted_lattice(nx=side, ny=side, seed=1000 + s, min_sep=2.2, count=nfil)
+        out.append((nfil * NSEG, x))
     return out
 
 

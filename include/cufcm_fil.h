@@ -55,7 +55,9 @@ typedef struct cfm_config {
   double mu;          /* viscosity (MU); velocities scale as 1/mu */
   int32_t precision;  /* CFM_FP64 or CFM_FP32C */
   int32_t device;     /* HIP device ordinal */
-  int32_t tile_split; /* source-range split across workgroups; 0 = choose automatically */
+  int32_t tile_split; /* source-range split across workgroups: > 0 fixed; 0 = size heuristic
+                         (deterministic); -1 = autotuned: the first product of each shape
+                         times the candidate splits on the device and keeps the fastest */
   int32_t reserved;
 } cfm_config;
 

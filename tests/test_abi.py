@@ -30,7 +30,8 @@ def _cfg(**kw):
 
 
 @pytest.mark.parametrize("bad", [dict(seg_dof=4), dict(n_seg=-1), dict(n_seg=0, n_blob=0),
-                                 dict(a_seg=2.0), dict(mu=0.0), dict(precision=7)])
+                                 dict(a_seg=2.0), dict(mu=0.0), dict(precision=7),
+                                 dict(tile_split=-2), dict(tile_split=65)])
 def test_create_rejects_bad_config(bad):
     h = ctypes.c_void_p()
     code = L.lib().cfm_create(ctypes.byref(_cfg(**bad)), ctypes.byref(h))

@@ -110,3 +110,28 @@ def test_mu_scaling_and_linearity():
     rs, rb = oracle.apply_full(xs, fs, xb, fb, mu=2.5)
     assert np.abs(a_s - rs).max() / np.abs(rs).max() < TOL64
     assert np.allclose(b_s, 2 * a_s, rtol=1e-13, atol=0) and np.allclose(b_b, 2 * a_b, rtol=1e-13)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32c"])
+def test_autotuned_split_matches_oracle(prec):
+    """tile_split = -1: the first product of each shape times candidate splits into
+    scratch memory; accumulating blocks (Msb adds into v_seg) must add exactly once."""
+    xs, xb, fs, fb = _system(seed=4)
+    ns, nb = xs.size // 3, xb.size // 3
+    ids, idb = np.arange(ns), np.arange(ns, ns + nb)
+    with RpyMobilitySolver(ns, nb, precision=prec, tile_split=-1) as m:
+        m.set_positions(xs, xb)
+        ref_ss = oracle.apply(xs, 6, xs, 6, fs)
+        ref_sb = oracle.apply(xs, 6, xb, 3, fb, id_tgt=ids, id_src=idb)
+        for _ in range(2):      # tuned on the first call, cached on the second
+            v_ss = m.evaluate_segment_segment_mobility(fs)
+            e, t = _err(v_ss, ref_ss, prec)
+            assert e < t, ("SS", e)
+            v_sb = m.evaluate_segment_blob_mobility(fb)
+            e, t = _err(v_sb, ref_ss + ref_sb, prec)
+            assert e < t, ("SB", e)
+        vs, vb = m.evaluate_full_mobility(fs, fb)
+        rs, rb = oracle.apply_full(xs, fs, xb, fb)
+        for got, ref in ((vs, rs), (vb, rb)):
+            e, t = _err(got, ref, prec)
+            assert e < t, ("FULL", e)
