@@ -52,7 +52,7 @@ def build_system(args, precision, solve_mode, guess, precond):
     pert = (0.01 * ef * d).reshape(-1)
     return FilamentSystem(xs, tans, x_blob=xb, precision=precision, solve_mode=solve_mode,
                           guess=guess, pert=pert, blob_rtol=1e-10, blob_restart=50,
-                          blob_max_iter=5000, blob_precond=precond)
+                          blob_max_iter=5000, blob_precond=precond, tile_split=-1)
 
 
 def run_path(args, label, precision, solve_mode, guess, precond):
@@ -149,6 +149,53 @@ def run_batched(args):
     }
 
 
+def run_learned(args):
+    """Learned near-field correction on the 32,000-particle workload: recorded from
+    the plain solver's blob solves, trained on the GPU, then judged by the GMRES
+    iterations of held-out blob solves with the near-field preconditioner (plain
+    GMRES stays the judge: every solve must reach the same 1e-10 residual)."""
+    import numpy as np
+    S = build_system(args, "fp64", "fp64", "previous", "diag")
+    try:
+        m = S.mob
+        m.gnn_init(seed=0, capacity=384)
+        m.gnn_record(True)
+        for nt in range(2):
+            S.step(nt)
+        m.gnn_record(False)
+        rng = np.random.default_rng(11)
+        held = [-m.evaluate_blob_segment_mobility(rng.standard_normal(6 * S.n_seg)) for _ in range(3)]
+
+        def solves():
+            its, res = [], []
+            t0 = time.perf_counter()
+            for b in held:
+                _, st = m.solve("blob", b, rtol=1e-10, precond="near", mode="fp64", max_iter=3000)
+                its.append(st["iterations"])
+                res.append(st["rel_residual"])
+            return its, res, 1e3 * (time.perf_counter() - t0) / len(held)
+
+        its0, res0, ms0 = solves()
+        t0 = time.perf_counter()
+        l0, l1 = m.gnn_train(epochs=8, batch=16, lr=1e-3)
+        train_s = time.perf_counter() - t0
+        m.gnn_enable(True)
+        its1, res1, ms1 = solves()
+        samples = m.gnn_samples()
+    finally:
+        S.close()
+    print(f"[bench] learned: near {its0} -> near+learned {its1}", file=sys.stderr, flush=True)
+    return {
+        "workload": "3 held-out blob solves of the 32,000-particle system, near-field block "
+                    "preconditioner with and without the learned correction, rtol 1e-10",
+        "training_samples": samples, "loss_before": l0, "loss_after": l1,
+        "train_seconds": round(train_s, 3),
+        "near_iterations": its0, "near_learned_iterations": its1,
+        "near_ms_per_solve": round(ms0, 3), "near_learned_ms_per_solve": round(ms1, 3),
+        "max_rel_residual": max(res0 + res1),
+    }
+
+
 def cpu_baseline(args, flops_per_step):
     """FP64 OpenMP oracle apply on a bounded sample of targets, extrapolated."""
     import numpy as np
@@ -193,13 +240,26 @@ def main():
     ap.add_argument("--skip-plain", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-batch", action="store_true")
+    ap.add_argument("--skip-learned", action="store_true")
     ap.add_argument("--batch-reps", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--precond", default="dense", choices=["near", "dense"],
                     help="blob-solve preconditioner of the fast path")
     ap.add_argument("--mode", default="fp64", choices=["mixed", "fp64"],
                     help="blob-solve arithmetic of the fast path")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="only run one device-to-device copy of BYTES bytes (calibrates the "
+                         "FETCH_SIZE / WRITE_SIZE counters against a copy of known size)")
     args = ap.parse_args()
+    if args.copy_probe > 0:
+        import torch
+        a = torch.ones(args.copy_probe // 4, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": a.numel() * 4}), flush=True)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build runs the implicit step on a "
               "single MI355X (multi-GPU sharding is a design note, not a run path).",
@@ -220,6 +280,7 @@ def main():
                 "launches": prof["launches_fp32"], "avg_ms": round(kms, 4),
                 "alg_flops_per_launch": flops_launch}
     batched = None if args.skip_batch else run_batched(args)
+    learned = None if args.skip_learned else run_learned(args)
     cpu = None
     if not args.skip_cpu:
         ref = plain if plain else mixed
@@ -252,6 +313,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "batched": batched,
+        "learned": learned,
     }
     print(json.dumps(out), flush=True)
 

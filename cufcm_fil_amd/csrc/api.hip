@@ -107,7 +107,8 @@ static int heuristic_split(const cfm_ctx* c, const ApplyPlan& pl) {
   if (pl.src_seg) nsrc += c->p.n_seg;
   if (pl.src_blob) nsrc += c->p.n_blob;
   const int stiles = std::max(1, (nsrc + 255) / 256);
-  int s = (1024 + tiles - 1) / tiles;
+  // about 8k workgroups: several per CU slot, so the partial last round is short
+  int s = (8192 + tiles - 1) / tiles;
   s = std::min(s, std::max(1, stiles / 2));
   return std::max(1, std::min(s, 64));
 }

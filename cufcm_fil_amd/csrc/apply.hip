@@ -68,7 +68,8 @@ __device__ __forceinline__ void sweep_species(const KArgs& a, SrcRec<T>* tile, i
       r.z = a.z[j];
       if constexpr (sizeof(T) == 4) r.zf = float(r.z);
 #pragma unroll
-      for (int k = 0; k < FD; ++k) r.f[k] = T(a.f[(size_t)k * a.fld + j - a.foff]);
+      for (int k = 0; k < FD; ++k)
+        r.f[k] = T((WALL ? kStageScale : 1.0) * a.f[(size_t)k * a.fld + j - a.foff]);
 #pragma unroll
       for (int k = FD; k < 6; ++k) r.f[k] = T(0);
       tile[threadIdx.x] = r;

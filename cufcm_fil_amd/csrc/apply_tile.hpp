@@ -10,6 +10,9 @@ constexpr int kTT = 64;     // targets per workgroup
 constexpr int kTS = 256;    // sources per LDS tile
 constexpr int kWaves = kWG / 64;
 
+// Wall-branch sources are staged as (1/(6 pi)) F (the form wall_pair_scaled takes).
+constexpr double kStageScale = 0.053051647697298445;
+
 template <typename T> struct alignas(16) SrcRec;
 template <> struct alignas(16) SrcRec<float> {
   double x, y, z;
@@ -55,7 +58,7 @@ __device__ __forceinline__ void tile_sweep(const SrcRec<T>* __restrict__ rec, in
           for (int k = 0; k < FD; ++k) F[k] = T(0);
         }
       }
-      wall_pair<T, TD, FD>(dx, dy, dz, zif + zj, zj, F, acc);
+      wall_pair_scaled<T, TD, FD>(dx, dy, dz, zif + zj, zj, F, acc);
     } else {
       free_pair<T, TD, FD>(dx, dy, dz, ai, aj, mu, F, acc);
     }

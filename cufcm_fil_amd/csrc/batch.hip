@@ -109,7 +109,7 @@ __global__ __launch_bounds__(kWG) void batch_apply_kernel(BArgs a) {
         r.z = a.z[j];
         if constexpr (sizeof(T) == 4) r.zf = float(r.z);
 #pragma unroll
-        for (int k = 0; k < D; ++k) r.f[k] = T(a.f[(size_t)D * j + k]);
+        for (int k = 0; k < D; ++k) r.f[k] = T((WALL ? kStageScale : 1.0) * a.f[(size_t)D * j + k]);
 #pragma unroll
         for (int k = D; k < 6; ++k) r.f[k] = T(0);
         tile[threadIdx.x] = r;

@@ -127,6 +127,150 @@ __device__ __forceinline__ void wall_pair(T dx, T dy, T dz, T Rz, T zj, const T*
   }
 }
 
+// Hot-loop form of wall_pair for loads already multiplied by 1/(6 pi) (the tiled
+// products scale each source once when they stage it): the same blocks, regrouped
+// so that every velocity component is accumulated by fused multiply-adds and the
+// products shared between blocks (k2 r_hat, B0 e, (e x t)_z) are formed once.
+template <typename T, int VD, int FD>
+__device__ __forceinline__ void wall_pair_scaled(T dx, T dy, T dz, T Rz, T zj,
+                                                 const T* __restrict__ F, T* __restrict__ V) {
+  const T fx = F[0], fy = F[1], fz = F[2];
+  const T tx = (FD > 3) ? F[3] : T(0), ty = (FD > 3) ? F[4] : T(0), tz = (FD > 3) ? F[5] : T(0);
+  const T hxy = dx * dx + dy * dy;
+
+  // ---- free-space part (1/(8 pi) = (6/8) / (6 pi)) ----
+  {
+    const T is = rsq<T>(hxy + dz * dz);
+    const T rx = dx * is, ry = dy * is, rz = dz * is;
+    const T is2 = is * is;
+    const T k1 = T(0.75) * is;
+    const T rf = rx * fx + ry * fy + rz * fz;
+    const T iso = k1 * (T(1) + T(2.0 / 3.0) * is2);
+    const T an = k1 * (T(1) - T(2) * is2) * rf;
+    V[0] += iso * fx;
+    V[0] += an * rx;
+    V[1] += iso * fy;
+    V[1] += an * ry;
+    V[2] += iso * fz;
+    V[2] += an * rz;
+    if (FD > 3 || VD > 3) {
+      const T k2 = k1 * is;
+      const T qx = k2 * rx, qy = k2 * ry, qz = k2 * rz;
+      if (FD > 3) {  // k2 (t x r_hat)
+        V[0] += ty * qz;
+        V[0] -= tz * qy;
+        V[1] += tz * qx;
+        V[1] -= tx * qz;
+        V[2] += tx * qy;
+        V[2] -= ty * qx;
+      }
+      if (VD > 3) {  // k2 (f x r_hat)
+        V[3] += fy * qz;
+        V[3] -= fz * qy;
+        V[4] += fz * qx;
+        V[4] -= fx * qz;
+        V[5] += fx * qy;
+        V[5] -= fy * qx;
+        if (FD > 3) {
+          const T k3 = T(0.5) * k2 * is;
+          const T rt3 = T(3) * k3 * (rx * tx + ry * ty + rz * tz);
+          V[3] += rt3 * rx;
+          V[3] -= k3 * tx;
+          V[4] += rt3 * ry;
+          V[4] -= k3 * ty;
+          V[5] += rt3 * rz;
+          V[5] -= k3 * tz;
+        }
+      }
+    }
+  }
+
+  // ---- image system ----
+  const T p1 = rsq<T>(hxy + Rz * Rz);
+  const T p2 = p1 * p1, p3 = p2 * p1, p4 = p2 * p2, p5 = p4 * p1;
+  const T ex = dx * p1, ey = dy * p1, ez = Rz * p1;
+  const T u = ez * ez;
+  const T h = zj * rcp<T>(Rz);
+  const T hh = h * (T(1) - h);
+  const T ef = ex * fx + ey * fy + ez * fz;
+  const T q3 = T(1) - T(5) * u;
+  {
+    const T A0 = -(T(0.75) * p1 * (T(1) + T(2) * hh * u) +
+                   T(0.5) * (p3 * (T(1) - T(3) * u) - p5 * q3));
+    const T A1 = -(T(0.75) * p1 * (T(1) - T(6) * hh * u) - T(1.5) * p3 * q3 +
+                   T(2.5) * p5 * (T(1) - T(7) * u)) * ef;
+    const T A2 = ez * fz *
+                 (T(1.5) * h * p1 * (T(1) - T(6) * (T(1) - h) * u) - T(3) * p3 * q3 +
+                  T(5) * p5 * (T(2) - T(7) * u));
+    const T A3 = (T(1.5) * h * p1 - T(5) * p5) * ef;
+    const T A4 = -((T(3) * h * h * p1 + T(3) * p3) * u + (T(2) - T(15) * u) * p5) * fz;
+    const T A12 = A1 + A2;
+    V[0] += A0 * fx;
+    V[0] += A12 * ex;
+    V[1] += A0 * fy;
+    V[1] += A12 * ey;
+    V[2] += A0 * fz;
+    V[2] += (A12 + A3) * ez;
+    V[2] += A4;
+  }
+  if (VD > 3 || FD > 3) {
+    const T B0 = T(0.75) * p2;  // also D0 of the torque block
+    const T bx = B0 * ex, by = B0 * ey, bz = B0 * ez;
+    const T hp2 = h * p2;
+    const T w4 = (T(1.5) - T(15) * u) * p4;
+    const T cz = ex * ty - ey * tx;  // (e x t)_z
+    if (VD > 3) {
+      const T B1 = fz * (T(9) * u * hp2 + w4);
+      const T B2 = -ez * ef * (T(4.5) * hp2 - T(7.5) * p4);
+      const T B3 = -T(1.5) * ez * (hp2 - p4);
+      const T B12 = B1 + B2;
+      const T s3 = bz + B3;
+      V[3] -= fy * s3;
+      V[3] += fz * by;
+      V[3] += B12 * ey;
+      V[4] -= fz * bx;
+      V[4] += fx * s3;
+      V[4] -= B12 * ex;
+      V[5] -= fx * by;
+      V[5] += fy * bx;
+      if (FD > 3) {
+        const T et = ex * tx + ey * ty + ez * tz;
+        const T p3et = p3 * et;
+        const T C0 = p3 * (T(0.375) - T(2.25) * u);
+        const T C1 = -T(1.125) * p3et;
+        const T C2 = T(2.25) * ez * p3et;
+        const T C3 = T(2.25) * p3 * cz;
+        V[3] += C0 * tx;
+        V[3] += C1 * ex;
+        V[3] -= C3 * ey;
+        V[4] += C0 * ty;
+        V[4] += C1 * ey;
+        V[4] += C3 * ex;
+        V[5] += C0 * tz;
+        V[5] += C1 * ez;
+        V[5] += C2;
+      }
+    }
+    if (FD > 3) {  // translation <- torque (transpose of rotation <- force, h -> 1 - h)
+      const T gp2 = p2 - hp2;
+      const T D1 = -(T(9) * u * gp2 + w4) * cz;
+      const T D2 = ez * (T(4.5) * gp2 - T(7.5) * p4) * cz;
+      const T D3 = -T(1.5) * ez * (gp2 - p4);
+      const T s0 = bz + D3;
+      V[0] += tz * by;
+      V[0] += ty * s0;
+      V[0] += D2 * ex;
+      V[1] -= tx * s0;
+      V[1] -= tz * bx;
+      V[1] += D2 * ey;
+      V[2] += ty * bx;
+      V[2] -= tx * by;
+      V[2] -= D2 * ez;
+      V[2] -= D1;
+    }
+  }
+}
+
 // Self mobility of one sphere above the wall (a = 1, mu = 1); double only.
 template <int VD, int FD>
 __device__ __forceinline__ void wall_self(double z, const double* __restrict__ F,
