@@ -151,9 +151,10 @@ def run_batched(args):
 
 def run_learned(args):
     """Learned near-field correction on the 32,000-particle workload: recorded from
-    the plain solver's blob solves, trained on the GPU, then judged by the GMRES
-    iterations of held-out blob solves with the near-field preconditioner (plain
-    GMRES stays the judge: every solve must reach the same 1e-10 residual)."""
+    the plain solver's blob solves, trained on the GPU, kept only if it lowers the
+    GMRES iterations of two validation solves, and judged on three held-out blob
+    solves with the near-field preconditioner, with and without it (plain GMRES stays
+    the judge: every solve must reach the same 1e-10 residual)."""
     import numpy as np
     S = build_system(args, "fp64", "fp64", "previous", "diag")
     try:
@@ -164,7 +165,8 @@ def run_learned(args):
             S.step(nt)
         m.gnn_record(False)
         rng = np.random.default_rng(11)
-        held = [-m.evaluate_blob_segment_mobility(rng.standard_normal(6 * S.n_seg)) for _ in range(3)]
+        rhs = [-m.evaluate_blob_segment_mobility(rng.standard_normal(6 * S.n_seg)) for _ in range(5)]
+        val, held = rhs[:2], rhs[2:]
 
         def solves():
             its, res = [], []
@@ -177,10 +179,11 @@ def run_learned(args):
 
         its0, res0, ms0 = solves()
         t0 = time.perf_counter()
-        l0, l1 = m.gnn_train(epochs=8, batch=16, lr=1e-3)
+        gate = m.gnn_train_gated(val, epochs=8, batch=16, lr=1e-3)
         train_s = time.perf_counter() - t0
         m.gnn_enable(True)
         its1, res1, ms1 = solves()
+        m.gnn_enable(gate["kept"])
         samples = m.gnn_samples()
     finally:
         S.close()
@@ -188,8 +191,9 @@ def run_learned(args):
     return {
         "workload": "3 held-out blob solves of the 32,000-particle system, near-field block "
                     "preconditioner with and without the learned correction, rtol 1e-10",
-        "training_samples": samples, "loss_before": l0, "loss_after": l1,
-        "train_seconds": round(train_s, 3),
+        "training_samples": samples, "loss_before": gate["loss_before"],
+        "loss_after": gate["loss_after"], "train_seconds": round(train_s, 3),
+        "kept_after_validation": gate["kept"],
         "near_iterations": its0, "near_learned_iterations": its1,
         "near_ms_per_solve": round(ms0, 3), "near_learned_ms_per_solve": round(ms1, 3),
         "max_rel_residual": max(res0 + res1),

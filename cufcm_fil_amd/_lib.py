@@ -124,6 +124,21 @@ class cfm_profile_stats(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+CFM_BEAT_FULFORD_BLAKE = 1
+
+
+class cfm_cilia_params(ctypes.Structure):
+    _fields_ = [
+        ("n_cilia", ctypes.c_int32),
+        ("nseg", ctypes.c_int32),
+        ("beat", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("dl", ctypes.c_double),
+        ("dt", ctypes.c_double),
+        ("solve", cfm_solve_opts),
+    ]
+
+
 class cfm_batch_config(ctypes.Structure):
     _fields_ = [
         ("n_sys", ctypes.c_int32),
@@ -218,6 +233,9 @@ def lib():
     L.cfm_gnn_gradient.argtypes = [_vp, ctypes.c_int32, ctypes.c_int32, _fp, _dp]
     L.cfm_gemm_bf16.argtypes = [_vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _fp,
                                 ctypes.c_int32, _fp, ctypes.c_int32, _fp]
+    L.cfm_cilia_init.argtypes = [_vp, ctypes.POINTER(cfm_cilia_params), _dp, _dp, _dp]
+    L.cfm_cilia_step.argtypes = [_vp, ctypes.c_int32, _dp, _dp, ctypes.POINTER(cfm_solve_stats)]
+    L.cfm_cilia_get_state.argtypes = [_vp, _dp, _dp, _dp, _dp, _dp, _dp]
     L.cfm_batch_create.argtypes = [ctypes.POINTER(cfm_batch_config), _ip, ctypes.POINTER(_vp)]
     L.cfm_batch_destroy.argtypes = [_vp]
     L.cfm_batch_last_error.restype = ctypes.c_char_p

@@ -1,0 +1,73 @@
+"""Prescribed-shape cilia above the wall (the reference's CILIA_TYPE 3 path in its
+generalised-force mode): fitted Fulford-Blake beat at prescribed phase speeds, one
+wall-RPY resistance solve per step, phase and angle generalised forces per cilium.
+
+Mirrors filament::initial_setup / initial_guess (src/cilia/filament.cpp:225-470,
+1336-1490) and the time-loop body of src/cilia/cilia_sim_main.cu:405-438.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+
+import numpy as np
+
+from . import _lib as L
+from .mobility import RpyMobilitySolver
+
+
+def wall_lattice(nx, ny, spacing, dl=2.6):
+    """Bases on a rectangular lattice centred at the origin, DL/2 above the wall
+    (the reference's rectangular seeding, swimmer.cpp:173-200, config.hpp:397)."""
+    im, jm = 0.5 * (nx - 1), 0.5 * (ny - 1)
+    b = [((i - im) * spacing, (j - jm) * spacing, 0.5 * dl) for i in range(nx) for j in range(ny)]
+    return np.array(b, dtype=np.float64).reshape(-1)
+
+
+class PrescribedCilia:
+    """n cilia of nseg segments; base (3 n), phase and phase_dot (n)."""
+
+    def __init__(self, base, phase=None, phase_dot=None, *, nseg=20, dl=2.6, dt=1.0 / 300,
+                 precision="fp64", solve_mode="fp64", precond="self", rtol=1e-10, restart=50,
+                 max_iter=1000, device=0, tile_split=0):
+        base = L.f64(base)
+        n = base.size // 3
+        self.n_cilia, self.nseg = n, nseg
+        phase = np.zeros(n) if phase is None else L.f64(phase, n)
+        phase_dot = np.full(n, 2.0 * math.pi) if phase_dot is None else L.f64(phase_dot, n)
+        self.mob = RpyMobilitySolver(n * nseg, 0, seg_dof=3, wall=True, precision=precision,
+                                     device=device, tile_split=tile_split)
+        o = L.cfm_solve_opts(system=L.CFM_SYS_SEG, precond=RpyMobilitySolver._PC[precond],
+                             mode=RpyMobilitySolver._MODE[solve_mode], restart=restart,
+                             max_iter=max_iter, use_guess=1, rtol=rtol)
+        self.prm = L.cfm_cilia_params(n_cilia=n, nseg=nseg, beat=L.CFM_BEAT_FULFORD_BLAKE,
+                                      dl=dl, dt=dt, solve=o)
+        L.check(self.mob._h, L.lib().cfm_cilia_init(self.mob._h, ctypes.byref(self.prm),
+                                                    L.ptr(base), L.ptr(phase),
+                                                    L.ptr(phase_dot)))
+
+    def close(self):
+        self.mob.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def step(self, nt):
+        """One time step; returns (q_phase, q_angle, solve stats)."""
+        qp, qa = np.zeros(self.n_cilia), np.zeros(self.n_cilia)
+        st = L.cfm_solve_stats()
+        L.check(self.mob._h, L.lib().cfm_cilia_step(self.mob._h, int(nt), L.ptr(qp), L.ptr(qa),
+                                                    ctypes.byref(st)))
+        return qp, qa, st.as_dict()
+
+    def state(self):
+        ns = self.n_cilia * self.nseg
+        x, vp, va, f = (np.zeros(3 * ns) for _ in range(4))
+        s, ph = np.zeros(ns), np.zeros(self.n_cilia)
+        L.check(self.mob._h, L.lib().cfm_cilia_get_state(self.mob._h, L.ptr(x), L.ptr(vp),
+                                                         L.ptr(va), L.ptr(s), L.ptr(ph), L.ptr(f)))
+        return {"x": x, "vel_dir_phase": vp, "vel_dir_angle": va,
+                "s_values": s.reshape(self.n_cilia, self.nseg), "phase": ph, "f": f}

@@ -135,6 +135,7 @@ static thread_local std::string g_create_error;
 
 void cfm_fil_release(cfm_ctx* c);
 void cfm_gnn_release(cfm_ctx* c);
+void cfm_cilia_release(cfm_ctx* c);
 
 static int fail(cfm_ctx* c, const Status& s) {
   if (c) c->err = s.msg;
@@ -229,6 +230,7 @@ int cfm_destroy(cfm_ctx* c) {
   (void)hipSetDevice(c->device);
   cfm_fil_release(c);
   cfm_gnn_release(c);
+  cfm_cilia_release(c);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   if (c->rocblas) (void)rocblas_destroy_handle((rocblas_handle)c->rocblas);
   c->lu_piv.release();

@@ -171,5 +171,6 @@ struct cfm_ctx {
   std::vector<double> prof_flops, prof_bytes;
   std::vector<int> prof_fp32;
   void* gnn = nullptr;                  // learned initial guess (gnn.hip)
+  void* cilia = nullptr;                // prescribed-shape cilia (cilia.hip)
   std::string err;
 };

@@ -184,6 +184,33 @@ class RpyMobilitySolver:
     def gnn_enable(self, on=True):
         L.check(self._h, L.lib().cfm_gnn_enable(self._h, int(bool(on))))
 
+    def gnn_train_gated(self, validation_rhs, epochs=8, batch=16, lr=1e-3, rtol=1e-10,
+                        max_iter=3000):
+        """Train on the recorded samples, then keep the correction only if it lowers
+        the total GMRES iterations of the validation blob solves (near-field
+        preconditioner, FP64, rtol); otherwise the analytic preconditioner alone stays
+        in use.  Every validation solve must reach rtol either way."""
+        def iterations():
+            its = []
+            for b in validation_rhs:
+                _, st = self.solve("blob", b, rtol=rtol, precond="near", mode="fp64",
+                                   max_iter=max_iter)
+                if not st["converged"]:
+                    raise L.CfmError(L.CFM_ENOCONV, "validation solve did not converge")
+                its.append(st["iterations"])
+            return its
+
+        self.gnn_enable(False)
+        its0 = iterations()
+        l0, l1 = self.gnn_train(epochs=epochs, batch=batch, lr=lr)
+        self.gnn_enable(True)
+        its1 = iterations()
+        keep = sum(its1) < sum(its0)
+        if not keep:
+            self.gnn_enable(False)
+        return {"loss_before": l0, "loss_after": l1, "iterations_without": its0,
+                "iterations_with": its1, "kept": keep}
+
     def gnn_params(self):
         th = np.zeros(L.lib().cfm_gnn_num_params(self._h), dtype=np.float32)
         L.check(self._h, L.lib().cfm_gnn_get_params(self._h, L.fptr(th)))

@@ -272,6 +272,42 @@ int cfm_gnn_gradient(cfm_ctx* ctx, int32_t s0, int32_t s1, float* grad, double* 
 int cfm_gemm_bf16(cfm_ctx* ctx, int32_t M, int32_t N, int32_t K, const float* A, int32_t transA,
                   const float* B, int32_t transB, float* C);  /* C = op(A) op(B), row-major */
 
+/* ---- prescribed-shape cilia above the wall ------------------------------------------
+ * The reference's CILIA_TYPE 3 path in the mode that writes its generalised-force
+ * reference files (WRITE_GENERALISED_FORCES): every cilium follows the fitted
+ * Fulford-Blake beat (filament.cpp:684-1160) at a prescribed phase speed; each step
+ * places the segments (filament.cpp:1336-1490), solves the wall-RPY resistance problem
+ * M F = phase_dot vel_dir_phase over all segments (mobility_solver.cpp:301-420 rhs,
+ * GMRES), and returns each cilium's phase and angle generalised forces
+ * sum_n vel_dir . F_n (cilia_sim_main.cu:405-438).  The context must have the wall,
+ * seg_dof = 3, no blobs and n_seg = n_cilia * nseg (cilium c owns segments
+ * [c nseg, (c+1) nseg)). */
+#define CFM_BEAT_FULFORD_BLAKE 1   /* Ito, Omori & Ishikawa (2019) fit, filament.cpp:684-758 */
+
+typedef struct {
+  int32_t n_cilia;
+  int32_t nseg;          /* segments per cilium, 2..64 */
+  int32_t beat;          /* CFM_BEAT_* */
+  int32_t reserved;
+  double dl;             /* segment spacing (filament length dl (nseg - 1)) */
+  double dt;             /* time step: phase += phase_dot dt at every step after the first */
+  cfm_solve_opts solve;  /* segment solve: precond NONE/DIAG/SELF, mode, rtol, ... */
+} cfm_cilia_params;
+
+/* filament::initial_setup, prescribed branch (filament.cpp:225-470); base 3 n_cilia
+ * (AoS, above the wall), phase and phase_dot n_cilia.  Places the segments. */
+int cfm_cilia_init(cfm_ctx* ctx, const cfm_cilia_params* p, const double* base,
+                   const double* phase, const double* phase_dot);
+/* One step nt (the reference's time-loop body): q_phase, q_angle (n_cilia each, may be
+ * null) receive the generalised forces; st (may be null) the solve statistics. */
+int cfm_cilia_step(cfm_ctx* ctx, int32_t nt, double* q_phase, double* q_angle,
+                   cfm_solve_stats* st);
+/* Current geometry and forces (each pointer may be null): positions and velocity
+ * directions 3 n_seg (AoS), segment parameters n_cilia nseg, phases n_cilia, segment
+ * forces of the last step 3 n_seg. */
+int cfm_cilia_get_state(cfm_ctx* ctx, double* x, double* vel_dir_phase, double* vel_dir_angle,
+                        double* s_values, double* phase, double* f_seg);
+
 /* ---- batched ensembles of independent wall systems ----------------------------------
  * No counterpart inside the reference's mobility_solver: the reference runs each
  * member of an ensemble (parameter sweep, bisection, periodic-orbit search) as its

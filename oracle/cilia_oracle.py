@@ -1,0 +1,150 @@
+"""numpy restatement of the prescribed-cilium path -- TEST INFRASTRUCTURE ONLY.
+
+Follows the reference's fitted-beat cilium (CILIA_TYPE 3, SHAPE_SEQUENCE 1):
+  * Fourier/polynomial beat of Fulford & Blake as fitted by Ito, Omori & Ishikawa
+    (2019, Table 1), rescaled to unit mean length with the zero mode halved
+    (filament.cpp:684-758 coefficients, :964-1045 shape, tangent and phase derivative);
+  * arc-length parameters s_n of the segments: trapezoid curve length on 10 NSEG
+    panels and bisection to within 0.1/NSEG of n/(NSEG-1) (filament.cpp:1047-1160);
+  * placement above the wall: base at (0, 0, DL/2), reference x axis along the wall
+    normal (the quaternion of filament.cpp:225-226 for dir = e_z), segment n at
+    base + FIL_LENGTH R shape(s_n, psi) with FIL_LENGTH = DL (NSEG-1), velocity
+    directions FIL_LENGTH R d shape/d psi and FIL_LENGTH R (e_z x shape)
+    (filament.cpp:1336-1490, config.hpp:397,579);
+  * the resistance solve M F = phase_dot vel_dir_phase over the segments (3-DOF,
+    wall-corrected RPY) and the generalised forces sum_n vel_dir . F_n
+    (mobility_solver.cpp:301-420, cilia_sim_main.cu:418-438).
+
+Parity status: pinned by the reference's own output files
+input/forcing/fulford_and_blake_reference_{phase,angle}_generalised_forces_NSEG=20_SEP=2.600000.dat
+(copied under tests/golden/), by the arc-length property of the s-values and by
+finite differences of the shape in the phase (tests/test_cilia_oracle.py).
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+# Ito, Omori & Ishikawa (2019) Table 1 as used by the reference: rows = Fourier
+# mode, columns = polynomial degree (s, s^2, s^3); x is the filament axis, y the
+# beat direction.
+AX = np.array([[9.7204e-01, -2.8315e-01, 4.9243e-02],
+               [-1.8466e-02, -1.2926e-01, 2.6981e-01],
+               [1.6209e-01, -3.4983e-01, 1.9082e-01],
+               [1.0259e-02, 3.5907e-02, -6.8736e-02]])
+AY = np.array([[-3.3547e-01, 4.0369e-01, 1.0362e-01],
+               [4.0318e-01, -1.5553e+00, 7.3455e-01],
+               [-9.9513e-02, 3.2829e-02, -1.2106e-01],
+               [8.1046e-02, -3.0982e-01, 1.4568e-01]])
+BX = np.array([[1.9697e-01, -5.1193e-01, 3.4778e-01],
+               [-5.1295e-02, 4.3396e-01, -3.3547e-01],
+               [1.2311e-02, 1.4157e-01, -1.1695e-01]])
+BY = np.array([[2.9136e-01, 1.0721e+00, -1.0433e+00],
+               [6.1554e-03, 3.2521e-01, -2.8315e-01],
+               [-6.0528e-02, 2.3185e-01, -2.0108e-01]])
+
+
+def _modes(psi):
+    cv = np.array([1.0] + [math.cos(n * psi) for n in range(1, 4)])
+    sv = np.array([math.sin(n * psi) for n in range(1, 4)])
+    return cv, sv
+
+
+def tangent(s, psi):
+    """d shape / d s in the beat plane (filament.cpp:964-988)."""
+    cv, sv = _modes(psi)
+    p = np.array([1.0, 2.0 * s, 3.0 * s * s])
+    return (cv @ AX + sv @ BX) @ p, (cv @ AY + sv @ BY) @ p
+
+
+def shape(s, psi):
+    """Reference-frame position at parameter s, unit length (filament.cpp:990-1019)."""
+    cv, sv = _modes(psi)
+    p = np.array([s, s * s, s ** 3])
+    return np.array([(cv @ AX + sv @ BX) @ p, (cv @ AY + sv @ BY) @ p, 0.0])
+
+
+def shape_phase_derivative(s, psi):
+    """d shape / d psi (filament.cpp:1021-1045)."""
+    dc = np.array([0.0] + [-n * math.sin(n * psi) for n in range(1, 4)])
+    ds = np.array([n * math.cos(n * psi) for n in range(1, 4)])
+    p = np.array([s, s * s, s ** 3])
+    return np.array([(dc @ AX + ds @ BX) @ p, (dc @ AY + ds @ BY) @ p, 0.0])
+
+
+def curve_length(s, psi, nseg=20):
+    """Trapezoid arc length from 0 to s on 10 nseg panels, the last panel
+    interpolated (filament.cpp:1047-1094)."""
+    if s <= 0.0:
+        return 0.0
+    dl = 1.0 / (10 * nseg)
+    f0 = math.hypot(*tangent(0.0, psi))
+    fl = math.floor(s / dl)
+    length = 0.0
+    for n in range(1, int(fl) + 1):
+        f1 = math.hypot(*tangent(n * dl, psi))
+        length += 0.5 * dl * (f0 + f1)
+        f0 = f1
+    ce = math.ceil(s / dl)
+    if ce > fl:
+        f1 = math.hypot(*tangent(ce * dl, psi))
+        length += ((s / dl - fl) / (ce - fl)) * 0.5 * dl * (f0 + f1)
+    return length
+
+
+def s_values(psi, nseg=20):
+    """Segment parameters by bisection on the arc-length fraction
+    (filament.cpp:1096-1160)."""
+    s = [0.0] * nseg
+    s[-1] = 1.0
+    total = curve_length(1.0, psi, nseg)
+    for n in range(1, nseg - 1):
+        target = n / (nseg - 1)
+        lo, hi = s[n - 1], 1.0
+        est = 0.5 * (lo + hi)
+        frac = curve_length(est, psi, nseg) / total
+        while abs(frac - target) > 0.1 / nseg:
+            if frac > target:
+                hi = est
+            else:
+                lo = est
+            est = 0.5 * (lo + hi)
+            frac = curve_length(est, psi, nseg) / total
+        s[n] = est
+    return np.array(s)
+
+
+# Rotation taking the reference x axis to the wall normal e_z: the square root of
+# the quaternion (0, 0, -1, 0) (filament.cpp:225-226 with dir = e_z).
+FRAME = np.array([[0.0, 0.0, -1.0],
+                  [0.0, 1.0, 0.0],
+                  [1.0, 0.0, 0.0]])
+
+
+def placement(psi, base, nseg=20, dl=2.6, angle=0.0):
+    """Positions, phase and angle velocity directions of one cilium (AoS, 3 nseg)."""
+    L = dl * (nseg - 1)
+    s = s_values(psi, nseg)
+    ca, sa = math.cos(angle), math.sin(angle)
+    Rshape = np.array([[ca, -sa, 0.0], [sa, ca, 0.0], [0.0, 0.0, 1.0]])
+    base = np.asarray(base, dtype=np.float64)
+    x = np.zeros((nseg, 3))
+    vphase = np.zeros((nseg, 3))
+    vangle = np.zeros((nseg, 3))
+    x[0] = base
+    for n in range(1, nseg):
+        ref = L * (Rshape @ shape(s[n], psi))
+        x[n] = base + FRAME @ ref
+        vangle[n] = FRAME @ np.array([-ref[1], ref[0], 0.0])   # e_z x ref
+        vphase[n] = FRAME @ (L * (Rshape @ shape_phase_derivative(s[n], psi)))
+    return x.reshape(-1), vphase.reshape(-1), vangle.reshape(-1), s
+
+
+def generalised_forces(mobility, psi, phase_dot=2.0 * math.pi, base=(0.0, 0.0, 1.3), nseg=20,
+                       dl=2.6):
+    """Phase and angle generalised forces of one cilium: F = M^-1 (phase_dot vel_dir_phase)
+    with M = mobility(x) the 3-DOF wall mobility matrix of its segments."""
+    x, vp, va, _ = placement(psi, base, nseg, dl)
+    f = np.linalg.solve(mobility(x), phase_dot * vp)
+    return float(vp @ f), float(va @ f)
