@@ -36,9 +36,24 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 FP32_VALU_PEAK_TFLOPS = 157.3   # MI355X vector FP32 (published)
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "rpy_apply_fp32c_sphere32k_pmc.md")
+
+
+def measured_traffic():
+    """HBM bytes per launch of the FP32C pair kernel from the committed counter
+    summary (FETCH_SIZE and WRITE_SIZE runs calibrated on a device copy)."""
+    try:
+        with open(PMC_SUMMARY) as fh:
+            for line in fh:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":")[1].split()[0])
+    except OSError:
+        pass
+    return None
 
 
 def build_system(args, precision, solve_mode, guess, precond):
+    split = getattr(args, "tile_split", -1)
     import numpy as np
 
     from cufcm_fil_amd import geometry as geo
@@ -52,7 +67,7 @@ def build_system(args, precision, solve_mode, guess, precond):
     pert = (0.01 * ef * d).reshape(-1)
     return FilamentSystem(xs, tans, x_blob=xb, precision=precision, solve_mode=solve_mode,
                           guess=guess, pert=pert, blob_rtol=1e-10, blob_restart=50,
-                          blob_max_iter=5000, blob_precond=precond, tile_split=-1)
+                          blob_max_iter=5000, blob_precond=precond, tile_split=split)
 
 
 def run_path(args, label, precision, solve_mode, guess, precond):
@@ -200,6 +215,40 @@ def run_learned(args):
     }
 
 
+def run_cilia(args):
+    """Prescribed-shape cilia: a 16 x 16 lattice of Fulford-Blake cilia (20 segments,
+    spacing 2.6a, lattice spacing 60a) above the wall, beating in phase at 2 pi per
+    unit time with 300 steps per beat; one wall-RPY resistance solve per step to a
+    relative FP64 residual of 1e-10 and the generalised forces of every cilium."""
+    import numpy as np
+
+    from cufcm_fil_amd.cilia import PrescribedCilia, wall_lattice
+    base = wall_lattice(16, 16, spacing=60.0)
+    n = base.size // 3
+    steps = 20
+    with PrescribedCilia(base, np.zeros(n), precision="fp32c", solve_mode="mixed",
+                         tile_split=-1) as C:
+        for nt in range(3):
+            C.step(nt)
+        its, res = 0, 0.0
+        t0 = time.perf_counter()
+        for nt in range(3, 3 + steps):
+            _, _, st = C.step(nt)
+            its += st["iterations"]
+            res = max(res, st["rel_residual"])
+        el = time.perf_counter() - t0
+    print(f"[bench] cilia: {1e3 * el / steps:.2f} ms/step", file=sys.stderr, flush=True)
+    return {
+        "workload": "16 x 16 lattice of Fulford-Blake cilia (20 segments, seg_sep 2.6a, "
+                    "spacing 60a) above the wall, in phase, 300 steps per beat; one "
+                    "wall-RPY resistance solve per step (rtol 1e-10, FP32C pair sums in "
+                    "FP64 refinement, self-block preconditioner)",
+        "cilia": n, "segments": 20 * n,
+        "steps_per_s": round(steps / el, 3), "ms_per_step": round(1e3 * el / steps, 3),
+        "gmres_iterations_per_step": its / steps, "max_rel_residual": res,
+    }
+
+
 def cpu_baseline(args, flops_per_step):
     """FP64 OpenMP oracle apply on a bounded sample of targets, extrapolated."""
     import numpy as np
@@ -245,12 +294,15 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-batch", action="store_true")
     ap.add_argument("--skip-learned", action="store_true")
+    ap.add_argument("--skip-cilia", action="store_true")
     ap.add_argument("--batch-reps", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--precond", default="dense", choices=["near", "dense"],
                     help="blob-solve preconditioner of the fast path")
     ap.add_argument("--mode", default="fp64", choices=["mixed", "fp64"],
                     help="blob-solve arithmetic of the fast path")
+    ap.add_argument("--tile-split", type=int, default=-1,
+                    help="source split of the pair kernel (-1: autotuned, 0: heuristic)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run one device-to-device copy of BYTES bytes (calibrates the "
                          "FETCH_SIZE / WRITE_SIZE counters against a copy of known size)")
@@ -280,11 +332,12 @@ def main():
     achieved = flops_launch / (kms * 1e-3) / 1e12 if kms > 0 else 0.0
     roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": FP32_VALU_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(achieved / FP32_VALU_PEAK_TFLOPS, 4),
-                "traffic": None, "kernel": "rpy_apply_kernel<float,6,true>",
+                "traffic": measured_traffic(), "kernel": "rpy_apply_kernel<float,6,true>",
                 "launches": prof["launches_fp32"], "avg_ms": round(kms, 4),
                 "alg_flops_per_launch": flops_launch}
     batched = None if args.skip_batch else run_batched(args)
     learned = None if args.skip_learned else run_learned(args)
+    cilia = None if args.skip_cilia else run_cilia(args)
     cpu = None
     if not args.skip_cpu:
         ref = plain if plain else mixed
@@ -318,6 +371,7 @@ def main():
         "cpu_baseline": cpu,
         "batched": batched,
         "learned": learned,
+        "prescribed_cilia": cilia,
     }
     print(json.dumps(out), flush=True)
 

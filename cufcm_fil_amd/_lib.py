@@ -236,6 +236,9 @@ def lib():
     L.cfm_cilia_init.argtypes = [_vp, ctypes.POINTER(cfm_cilia_params), _dp, _dp, _dp]
     L.cfm_cilia_step.argtypes = [_vp, ctypes.c_int32, _dp, _dp, ctypes.POINTER(cfm_solve_stats)]
     L.cfm_cilia_get_state.argtypes = [_vp, _dp, _dp, _dp, _dp, _dp, _dp]
+    L.cfm_cilia_set_driving.argtypes = [_vp, _dp, ctypes.c_int32, _dp, ctypes.c_int32,
+                                        ctypes.c_double]
+    L.cfm_cilia_get_speeds.argtypes = [_vp, _dp, _dp, _dp, _dp]
     L.cfm_batch_create.argtypes = [ctypes.POINTER(cfm_batch_config), _ip, ctypes.POINTER(_vp)]
     L.cfm_batch_destroy.argtypes = [_vp]
     L.cfm_batch_last_error.restype = ctypes.c_char_p

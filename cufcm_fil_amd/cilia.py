@@ -55,6 +55,22 @@ class PrescribedCilia:
     def __exit__(self, *a):
         self.close()
 
+    def set_driving(self, q_phase_table, q_angle_table=None, spring=0.0):
+        """Dynamic phase (and shape-angle) evolution driven by recorded generalised
+        forces of one beat (tables sampled at phase 2 pi k / n)."""
+        qp = L.f64(q_phase_table)
+        qa = None if q_angle_table is None else L.f64(q_angle_table)
+        L.check(self.mob._h, L.lib().cfm_cilia_set_driving(
+            self.mob._h, L.ptr(qp), qp.size, L.ptr(qa), 0 if qa is None else qa.size,
+            float(spring)))
+
+    def speeds(self):
+        n = self.n_cilia
+        ph, pd, an, ad = (np.zeros(n) for _ in range(4))
+        L.check(self.mob._h, L.lib().cfm_cilia_get_speeds(self.mob._h, L.ptr(ph), L.ptr(pd),
+                                                          L.ptr(an), L.ptr(ad)))
+        return {"phase": ph, "phase_dot": pd, "angle": an, "angle_dot": ad}
+
     def step(self, nt):
         """One time step; returns (q_phase, q_angle, solve stats)."""
         qp, qa = np.zeros(self.n_cilia), np.zeros(self.n_cilia)

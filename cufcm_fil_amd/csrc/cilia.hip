@@ -1,10 +1,14 @@
 // cilia.hip -- prescribed-shape cilia above the wall (fitted Fulford-Blake beat).
 //
-// Restates the reference's CILIA_TYPE 3 path with WRITE_GENERALISED_FORCES (the
-// mode that wrote its input/forcing reference files): every cilium follows the
-// fitted beat at a prescribed phase speed, the segment forces solve the wall-RPY
-// resistance problem M F = phase_dot * vel_dir_phase (3 unknowns per segment), and
-// each cilium reports its phase and angle generalised forces sum_n vel_dir . F_n.
+// Restates the reference's CILIA_TYPE 3 path.  With WRITE_GENERALISED_FORCES (the
+// mode that wrote its input/forcing reference files) every cilium follows the fitted
+// beat at a prescribed phase speed, the segment forces solve the wall-RPY resistance
+// problem M F = phase_dot * vel_dir_phase (3 unknowns per segment), and each cilium
+// reports its phase and angle generalised forces sum_n vel_dir . F_n.  With
+// DYNAMIC_PHASE_EVOLUTION (and DYNAMIC_SHAPE_ROTATION) the phase (and shape-angle)
+// speeds are unknowns of a coupled system driven by recorded generalised forces
+// (mobility_solver.cpp:242-356 driving, :1891-2115 operator, :1104-1244 block
+// preconditioner).
 //   beat coefficients      filament.cpp:684-758 (Ito, Omori & Ishikawa 2019, Table 1)
 //   shape / derivatives    filament.cpp:964-1045
 //   segment parameters     filament.cpp:1047-1160 (trapezoid arc length, bisection)
@@ -22,6 +26,7 @@
 
 #include "ctx.hpp"
 #include "krylov.hpp"
+#include "precond.hpp"
 
 namespace cfm {
 
@@ -231,11 +236,163 @@ __global__ void cilia_advance_kernel(int ncil, double dt, const double* __restri
   if (c < ncil) phase[c] += pdot[c] * dt;
 }
 
+// ---- coupled system: unknowns [F (3 x n_seg)] ++ [phase speeds] ++ [angle speeds] ----
+
+struct CoupleK {
+  int ncil, nseg, m;        // m = n_seg of the context
+  int dyn_angle;
+  const double* vphase;     // 3 x m
+  const double* vangle;     // 3 x m
+};
+
+// y_F -= phi_dot K + psi_dot A (own segments); y_phi = -K . F, y_psi = -A . F
+// (mobility_solver.cpp:2002-2050 with the body velocities prescribed).
+__global__ void cilia_couple_kernel(CoupleK k, const double* __restrict__ x, double* __restrict__ y) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= k.ncil) return;
+  const size_t m = k.m;
+  const double wp = x[3 * m + c];
+  const double wa = k.dyn_angle ? x[3 * m + k.ncil + c] : 0.0;
+  double dp = 0.0, da = 0.0;
+  for (int i = 0; i < k.nseg; ++i) {
+    const size_t g = (size_t)c * k.nseg + i;
+    for (int d = 0; d < 3; ++d) {
+      const double vp = k.vphase[d * m + g], va = k.vangle[d * m + g];
+      const double fv = x[d * m + g];
+      dp += vp * fv;
+      da += va * fv;
+      y[d * m + g] -= wp * vp + (k.dyn_angle ? wa * va : 0.0);
+    }
+  }
+  y[3 * m + c] = -dp;
+  if (k.dyn_angle) y[3 * m + k.ncil + c] = -da;
+}
+
+// Block preconditioner: F = P (r1 + K w), (K^T P K) w = -r2 - K^T P r1 with P the
+// particle preconditioner (6 pi mu a: exactly mobility_solver.cpp:1104-1200; or the
+// wall self-block inverse), one 1x1 or 2x2 Schur block per cilium.
+__global__ void cilia_precond_kernel(CoupleK k, int kind, double mu, const double* __restrict__ z,
+                                     const double* __restrict__ r, double* __restrict__ out) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= k.ncil) return;
+  const size_t m = k.m;
+  double s11 = 0.0, s12 = 0.0, s22 = 0.0, dp = 0.0, da = 0.0;
+  for (int i = 0; i < k.nseg; ++i) {
+    const size_t g = (size_t)c * k.nseg + i;
+    double rv[3], vp[3], va[3], pr[3], pp[3], pa[3];
+    for (int d = 0; d < 3; ++d) {
+      rv[d] = r[d * m + g];
+      vp[d] = k.vphase[d * m + g];
+      va[d] = k.vangle[d * m + g];
+    }
+    particle_precond(kind, 1, 3, z[g], 1.0, mu, rv, pr);
+    particle_precond(kind, 1, 3, z[g], 1.0, mu, vp, pp);
+    particle_precond(kind, 1, 3, z[g], 1.0, mu, va, pa);
+    for (int d = 0; d < 3; ++d) {
+      s11 += vp[d] * pp[d];
+      s12 += vp[d] * pa[d];
+      s22 += va[d] * pa[d];
+      dp += vp[d] * pr[d];
+      da += va[d] * pr[d];
+      out[d * m + g] = pr[d];
+    }
+  }
+  const double tp = -r[3 * m + c] - dp;
+  double wp, wa = 0.0;
+  if (k.dyn_angle) {
+    const double ta = -r[3 * m + k.ncil + c] - da;
+    const double det = s11 * s22 - s12 * s12;
+    wp = (s22 * tp - s12 * ta) / det;
+    wa = (s11 * ta - s12 * tp) / det;
+    out[3 * m + k.ncil + c] = wa;
+  } else {
+    wp = tp / s11;
+  }
+  out[3 * m + c] = wp;
+  for (int i = 0; i < k.nseg; ++i) {
+    const size_t g = (size_t)c * k.nseg + i;
+    double vp[3], va[3], pp[3], pa[3];
+    for (int d = 0; d < 3; ++d) {
+      vp[d] = k.vphase[d * m + g];
+      va[d] = k.vangle[d * m + g];
+    }
+    particle_precond(kind, 1, 3, z[g], 1.0, mu, vp, pp);
+    particle_precond(kind, 1, 3, z[g], 1.0, mu, va, pa);
+    for (int d = 0; d < 3; ++d) out[d * m + g] += wp * pp[d] + (k.dyn_angle ? wa * pa[d] : 0.0);
+  }
+}
+
+// Periodic cubic Hermite interpolation of a table sampled at 2 pi k / n with
+// central-difference slopes (mobility_solver.cpp:242-281).
+__device__ double cyclic_hermite(const double* tab, int n, double phase) {
+  double x = 0.5 * phase / 3.14159265358979323846;
+  x = (x - floor(x)) * n;
+  const int lo = (int)x;
+  if (lo == n) return tab[0];
+  const int hi = (lo == n - 1) ? 0 : lo + 1;
+  const double d_lo = (lo == 0) ? (tab[1] - tab[0]) : 0.5 * (tab[hi] - tab[lo - 1]);
+  const double d_hi = (hi == 0) ? (tab[0] - tab[n - 1])
+                                : 0.5 * (tab[(hi == n - 1) ? 0 : hi + 1] - tab[lo]);
+  const double t = x - lo, t2 = t * t, t3 = t2 * t;
+  return tab[lo] * (2.0 * t3 - 3.0 * t2 + 1.0) + d_lo * (t3 - 2.0 * t2 + t) +
+         tab[hi] * (3.0 * t2 - 2.0 * t3) + d_hi * (t3 - t2);
+}
+
+// rhs = [0 ; -q_phase ; -q_angle] with the recorded driving forces scaled by the
+// cilium's natural frequency and the linear torsional spring on the shape angle
+// (mobility_solver.cpp:301-356, FORCE_NOISE_MAG = 0, FENE_MODEL = 0).
+__global__ void cilia_drive_kernel(int ncil, int m, int dyn_angle, double len, double spring,
+                                   const double* __restrict__ phase, const double* __restrict__ angle,
+                                   const double* __restrict__ omega0,
+                                   const double* __restrict__ tab_p, int np,
+                                   const double* __restrict__ tab_a, int na,
+                                   double* __restrict__ rhs) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int L = 3 * m + ncil * (dyn_angle ? 2 : 1);
+  if (i >= L) return;
+  if (i < 3 * m) {
+    rhs[i] = 0.0;
+    return;
+  }
+  const int c = (i - 3 * m) % ncil;
+  const double sc = 0.5 * omega0[c] / 3.14159265358979323846;
+  if (i < 3 * m + ncil) {
+    rhs[i] = -cyclic_hermite(tab_p, np, phase[c]) * sc;
+  } else {
+    const double q = cyclic_hermite(tab_a, na, phase[c]) * sc - spring * omega0[c] * len * len * len * angle[c];
+    rhs[i] = -q;
+  }
+}
+
+// speeds from the solution; F copied to the force array
+__global__ void cilia_unpack_kernel(int ncil, int m, int dyn_angle, const double* __restrict__ z,
+                                    double* __restrict__ pdot, double* __restrict__ adot,
+                                    double* __restrict__ f) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < 3 * m) f[i] = z[i];
+  if (i < ncil) {
+    pdot[i] = z[3 * m + i];
+    adot[i] = dyn_angle ? z[3 * m + ncil + i] : 0.0;
+  }
+}
+
+__global__ void cilia_advance_angle_kernel(int ncil, double dt, const double* __restrict__ adot,
+                                           double* __restrict__ angle) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < ncil) angle[c] += adot[c] * dt;
+}
+
 struct CiliaState {
   cfm_cilia_params prm{};
   DevBuf<double> base, phase, pdot, angle, vphase, vangle, svals, rhs, f, q;
   double* host = nullptr;  // pinned read-back of q
   bool have_geometry = false;
+  // dynamic evolution (cfm_cilia_set_driving): phase (and shape-angle) speeds are
+  // unknowns of the coupled system, driven by recorded generalised forces
+  bool dynamic = false, dyn_angle = false;
+  double spring = 0.0;
+  DevBuf<double> omega0, adot, tab_phase, tab_angle, z, zrhs;
+  int n_tab_phase = 0, n_tab_angle = 0;
 };
 
 static CiliaState* cil_of(cfm_ctx* c) { return reinterpret_cast<CiliaState*>(c->cilia); }
@@ -270,6 +427,37 @@ static Status geometry(cfm_ctx* c, CiliaState* S) {
   return {};
 }
 
+static CoupleK couple_args(cfm_ctx* c, CiliaState* S) {
+  CoupleK k{};
+  k.ncil = S->prm.n_cilia;
+  k.nseg = S->prm.nseg;
+  k.m = c->p.n_seg;
+  k.dyn_angle = S->dyn_angle ? 1 : 0;
+  k.vphase = S->vphase.p;
+  k.vangle = S->vangle.p;
+  return k;
+}
+
+Status cilia_couple_apply(cfm_ctx* c, const double* x, double* y) {
+  CiliaState* S = cil_of(c);
+  if (!S || !S->dynamic) return make_status(CFM_ESTATE, "no coupled cilia system");
+  hipLaunchKernelGGL(cilia_couple_kernel, gc(S->prm.n_cilia), dim3(64), 0, c->stream,
+                     couple_args(c, S), x, y);
+  CFM_CHECK_LAUNCH();
+  return {};
+}
+
+Status cilia_precond(cfm_ctx* c, int kind, const double* x, double* y) {
+  CiliaState* S = cil_of(c);
+  if (!S || !S->dynamic) return make_status(CFM_ESTATE, "no coupled cilia system");
+  if (kind != CFM_PC_DIAG && kind != CFM_PC_SELF)
+    return make_status(CFM_EINVAL, "the coupled cilia system takes CFM_PC_DIAG or CFM_PC_SELF");
+  hipLaunchKernelGGL(cilia_precond_kernel, gc(S->prm.n_cilia), dim3(64), 0, c->stream,
+                     couple_args(c, S), kind, c->cfg.mu, c->p.z.p, x, y);
+  CFM_CHECK_LAUNCH();
+  return {};
+}
+
 }  // namespace cfm
 
 using namespace cfm;
@@ -278,7 +466,8 @@ void cfm_cilia_release(cfm_ctx* c) {
   CiliaState* S = cil_of(c);
   if (!S) return;
   for (auto* b : {&S->base, &S->phase, &S->pdot, &S->angle, &S->vphase, &S->vangle, &S->svals,
-                  &S->rhs, &S->f, &S->q})
+                  &S->rhs, &S->f, &S->q, &S->omega0, &S->adot, &S->tab_phase, &S->tab_angle,
+                  &S->z, &S->zrhs})
     b->release();
   if (S->host) (void)hipHostFree(S->host);
   delete S;
@@ -341,6 +530,10 @@ int cfm_cilia_init(cfm_ctx* c, const cfm_cilia_params* prm, const double* base,
     CFM_TRY(upload_aos(c, base, nc, 3, S->base.p, nc, 0));
     CFM_TRY(upload_aos(c, phase, nc, 1, S->phase.p, nc, 0));
     CFM_TRY(upload_aos(c, phase_dot, nc, 1, S->pdot.p, nc, 0));
+    CFM_TRY(S->omega0.alloc(nc));
+    CFM_TRY(S->adot.alloc(nc));
+    CFM_HIP(hipMemcpyAsync(S->omega0.p, S->pdot.p, sizeof(double) * nc, hipMemcpyDeviceToDevice, c->stream));
+    CFM_TRY(dev_fill(c, S->adot.p, 0.0, nc));
     CFM_TRY(dev_fill(c, S->angle.p, 0.0, nc));
     CFM_TRY(dev_fill(c, S->f.p, 0.0, 3 * p.n_seg));
     CFM_TRY(geometry(c, S));
@@ -362,11 +555,45 @@ int cfm_cilia_step(cfm_ctx* c, int32_t nt, double* q_phase, double* q_angle, cfm
   (void)hipSetDevice(c->device);
   const int nc = S->prm.n_cilia;
   auto run = [&]() -> Status {
-    if (nt > 0)  // phase += phase_dot dt before the step (filament.cpp:1340-1343)
+    if (nt > 0) {  // phase += phase_dot dt before the step (filament.cpp:1340-1356)
       hipLaunchKernelGGL(cilia_advance_kernel, gc(nc), dim3(64), 0, c->stream, nc, S->prm.dt,
                          S->pdot.p, S->phase.p);
+      if (S->dyn_angle)
+        hipLaunchKernelGGL(cilia_advance_angle_kernel, gc(nc), dim3(64), 0, c->stream, nc,
+                           S->prm.dt, S->adot.p, S->angle.p);
+    }
     CFM_TRY(geometry(c, S));
-    const SystemDesc sd = system_desc(c, CFM_SYS_SEG);
+    SystemDesc sd = system_desc(c, CFM_SYS_SEG);
+    if (S->dynamic) {
+      // coupled system: forces and speeds, driven by the recorded generalised forces
+      sd.cilia = true;
+      sd.extra = nc * (S->dyn_angle ? 2 : 1);
+      sd.L = 3 * c->p.n_seg + sd.extra;
+      hipLaunchKernelGGL(cilia_drive_kernel, dim3((sd.L + 255) / 256), dim3(256), 0, c->stream, nc,
+                         c->p.n_seg, S->dyn_angle ? 1 : 0, S->prm.dl * (S->prm.nseg - 1), S->spring,
+                         S->phase.p, S->angle.p, S->omega0.p, S->tab_phase.p, S->n_tab_phase,
+                         S->tab_angle.p, S->n_tab_angle, S->zrhs.p);
+      CFM_CHECK_LAUNCH();
+      cfm_solve_opts o = S->prm.solve;
+      o.system = CFM_SYS_SEG;
+      o.use_guess = 1;  // previous step's forces and speeds
+      if (!c->krylov) c->krylov = new Krylov();
+      cfm_solve_stats ss{};
+      CFM_TRY(solve(c, *c->krylov, sd, o, S->zrhs.p, S->z.p, &ss, nullptr));
+      hipLaunchKernelGGL(cilia_unpack_kernel, dim3((3 * c->p.n_seg + 255) / 256), dim3(256), 0,
+                         c->stream, nc, c->p.n_seg, S->dyn_angle ? 1 : 0, S->z.p, S->pdot.p,
+                         S->adot.p, S->f.p);
+      CFM_CHECK_LAUNCH();
+      hipLaunchKernelGGL(cilia_genforce_kernel, gc(nc), dim3(64), 0, c->stream, nc, S->prm.nseg,
+                         c->p.n_seg, S->vphase.p, S->vangle.p, S->f.p, S->q.p);
+      CFM_CHECK_LAUNCH();
+      CFM_HIP(hipMemcpyAsync(S->host, S->q.p, sizeof(double) * 2 * nc, hipMemcpyDeviceToHost, c->stream));
+      CFM_HIP(hipStreamSynchronize(c->stream));
+      if (q_phase) std::memcpy(q_phase, S->host, sizeof(double) * nc);
+      if (q_angle) std::memcpy(q_angle, S->host + nc, sizeof(double) * nc);
+      if (st) *st = ss;
+      return {};
+    }
     hipLaunchKernelGGL(cilia_rhs_kernel, dim3((c->p.n_seg + 255) / 256), dim3(256), 0, c->stream,
                        c->p.n_seg, S->prm.nseg, S->pdot.p, S->vphase.p, S->rhs.p);
     CFM_CHECK_LAUNCH();
@@ -384,6 +611,65 @@ int cfm_cilia_step(cfm_ctx* c, int32_t nt, double* q_phase, double* q_angle, cfm
     if (q_phase) std::memcpy(q_phase, S->host, sizeof(double) * nc);
     if (q_angle) std::memcpy(q_angle, S->host + nc, sizeof(double) * nc);
     if (st) *st = ss;
+    return {};
+  };
+  Status s = run();
+  return s.ok() ? CFM_OK : cfail(c, s);
+}
+
+int cfm_cilia_set_driving(cfm_ctx* c, const double* q_phase_table, int32_t n_phase,
+                          const double* q_angle_table, int32_t n_angle, double spring) {
+  if (!c) return CFM_EINVAL;
+  CiliaState* S = cil_of(c);
+  if (!S) return cfail(c, make_status(CFM_ESTATE, "cfm_cilia_init has not been called"));
+  if (!q_phase_table || n_phase < 2 || (q_angle_table && n_angle < 2) || !std::isfinite(spring))
+    return cfail(c, make_status(CFM_EINVAL, "driving tables need at least 2 values"));
+  for (int i = 0; i < n_phase; ++i)
+    if (!std::isfinite(q_phase_table[i])) return cfail(c, make_status(CFM_EINVAL, "non-finite table"));
+  if (q_angle_table)
+    for (int i = 0; i < n_angle; ++i)
+      if (!std::isfinite(q_angle_table[i])) return cfail(c, make_status(CFM_EINVAL, "non-finite table"));
+  (void)hipSetDevice(c->device);
+  const int nc = S->prm.n_cilia;
+  auto run = [&]() -> Status {
+    CFM_TRY(S->tab_phase.alloc(n_phase));
+    CFM_HIP(hipMemcpy(S->tab_phase.p, q_phase_table, sizeof(double) * n_phase, hipMemcpyHostToDevice));
+    S->n_tab_phase = n_phase;
+    S->dyn_angle = q_angle_table != nullptr;
+    if (S->dyn_angle) {
+      CFM_TRY(S->tab_angle.alloc(n_angle));
+      CFM_HIP(hipMemcpy(S->tab_angle.p, q_angle_table, sizeof(double) * n_angle, hipMemcpyHostToDevice));
+      S->n_tab_angle = n_angle;
+    }
+    S->spring = spring;
+    const size_t L = 3 * (size_t)c->p.n_seg + (size_t)nc * (S->dyn_angle ? 2 : 1);
+    CFM_TRY(S->z.alloc(L));
+    CFM_TRY(S->zrhs.alloc(L));
+    CFM_TRY(dev_fill(c, S->z.p, 0.0, (int)L));
+    // initial guess of the speeds: the natural frequencies, no shape rotation
+    CFM_HIP(hipMemcpyAsync(S->z.p + 3 * (size_t)c->p.n_seg, S->omega0.p, sizeof(double) * nc,
+                           hipMemcpyDeviceToDevice, c->stream));
+    CFM_HIP(hipStreamSynchronize(c->stream));
+    S->dynamic = true;
+    return {};
+  };
+  Status s = run();
+  return s.ok() ? CFM_OK : cfail(c, s);
+}
+
+int cfm_cilia_get_speeds(cfm_ctx* c, double* phase, double* phase_dot, double* angle,
+                         double* angle_dot) {
+  if (!c) return CFM_EINVAL;
+  CiliaState* S = cil_of(c);
+  if (!S) return cfail(c, make_status(CFM_ESTATE, "cfm_cilia_init has not been called"));
+  (void)hipSetDevice(c->device);
+  const int nc = S->prm.n_cilia;
+  auto run = [&]() -> Status {
+    CFM_HIP(hipStreamSynchronize(c->stream));
+    if (phase) CFM_HIP(hipMemcpy(phase, S->phase.p, sizeof(double) * nc, hipMemcpyDeviceToHost));
+    if (phase_dot) CFM_HIP(hipMemcpy(phase_dot, S->pdot.p, sizeof(double) * nc, hipMemcpyDeviceToHost));
+    if (angle) CFM_HIP(hipMemcpy(angle, S->angle.p, sizeof(double) * nc, hipMemcpyDeviceToHost));
+    if (angle_dot) CFM_HIP(hipMemcpy(angle_dot, S->adot.p, sizeof(double) * nc, hipMemcpyDeviceToHost));
     return {};
   };
   Status s = run();

@@ -358,10 +358,12 @@ Status system_apply(cfm_ctx* c, const SystemDesc& s, int precision, const double
   c->split = choose_split(c, pl);
   Status st = launch_apply(c, pl);
   c->cfg.precision = saved;
+  if (st.ok() && s.cilia) st = cilia_couple_apply(c, x, y);
   return st;
 }
 
 Status system_precond(cfm_ctx* c, const SystemDesc& s, int kind, const double* x, double* y) {
+  if (s.cilia) return cilia_precond(c, kind, x, y);
   if (kind == CFM_PC_NEAR) {
     if (s.which != CFM_BB) return make_status(CFM_EINVAL, "CFM_PC_NEAR applies to the blob system only");
     if (!c->nf.valid) CFM_TRY(nearfield_build(c, c->nf));

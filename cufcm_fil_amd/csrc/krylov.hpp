@@ -24,7 +24,15 @@ struct SystemDesc {
   int L = 0;      // m * dof
   int which = 0;  // CFM_SS or CFM_BB
   double a = 1.0;
+  // Coupled prescribed-cilia system (cilia.hip): `extra` scalar unknowns (phase and
+  // shape-angle speeds) follow the [dof][m] force block, L = m dof + extra.
+  bool cilia = false;
+  int extra = 0;
 };
+
+// Coupling rows / columns and block preconditioner of the coupled cilia system.
+Status cilia_couple_apply(cfm_ctx* c, const double* x, double* y);
+Status cilia_precond(cfm_ctx* c, int kind, const double* x, double* y);
 
 // Optional correction added to the preconditioner output (learned near-field term).
 struct GmresHooks {

@@ -302,6 +302,22 @@ int cfm_cilia_init(cfm_ctx* ctx, const cfm_cilia_params* p, const double* base,
  * null) receive the generalised forces; st (may be null) the solve statistics. */
 int cfm_cilia_step(cfm_ctx* ctx, int32_t nt, double* q_phase, double* q_angle,
                    cfm_solve_stats* st);
+/* Dynamic evolution (DYNAMIC_PHASE_EVOLUTION, and DYNAMIC_SHAPE_ROTATION when an angle
+ * table is given): from the next step on, the phase speeds (and shape-angle speeds)
+ * are unknowns of the coupled system of mobility_solver.cpp:1891-2115, driven by the
+ * recorded generalised forces of one beat (tables of n values at phase 2 pi k / n,
+ * e.g. the reference's input/forcing files), interpolated periodically
+ * (mobility_solver.cpp:242-281) and scaled by phase_dot / (2 pi) of cfm_cilia_init
+ * (each cilium's natural frequency); the shape angle feels the linear torsional spring
+ * spring * phase_dot * L^3 * angle (mobility_solver.cpp:306-337, FENE_MODEL 0).
+ * Preconditioner: the block form of mobility_solver.cpp:1104-1244 with the
+ * particle preconditioner of opts.precond (CFM_PC_DIAG reproduces it exactly). */
+int cfm_cilia_set_driving(cfm_ctx* ctx, const double* q_phase_table, int32_t n_phase,
+                          const double* q_angle_table /* may be null */, int32_t n_angle,
+                          double spring);
+/* Phases, phase speeds, shape angles and their speeds (n_cilia each, may be null). */
+int cfm_cilia_get_speeds(cfm_ctx* ctx, double* phase, double* phase_dot, double* angle,
+                         double* angle_dot);
 /* Current geometry and forces (each pointer may be null): positions and velocity
  * directions 3 n_seg (AoS), segment parameters n_cilia nseg, phases n_cilia, segment
  * forces of the last step 3 n_seg. */

@@ -13,12 +13,17 @@ Follows the reference's fitted-beat cilium (CILIA_TYPE 3, SHAPE_SEQUENCE 1):
     (filament.cpp:1336-1490, config.hpp:397,579);
   * the resistance solve M F = phase_dot vel_dir_phase over the segments (3-DOF,
     wall-corrected RPY) and the generalised forces sum_n vel_dir . F_n
-    (mobility_solver.cpp:301-420, cilia_sim_main.cu:418-438).
+    (mobility_solver.cpp:301-420, cilia_sim_main.cu:418-438);
+  * the coupled system with phase (and shape-angle) speeds as unknowns, driven by the
+    recorded generalised forces through periodic cubic Hermite interpolation
+    (mobility_solver.cpp:242-356, 1891-2115).
 
 Parity status: pinned by the reference's own output files
 input/forcing/fulford_and_blake_reference_{phase,angle}_generalised_forces_NSEG=20_SEP=2.600000.dat
-(copied under tests/golden/), by the arc-length property of the s-values and by
-finite differences of the shape in the phase (tests/test_cilia_oracle.py).
+(copied under tests/golden/) -- both directly and in closed loop (driving a cilium
+with its own recorded forces returns its beat speed) -- by the arc-length property of
+the s-values, by finite differences of the shape in the phase, and by node, periodicity
+and slope properties of the interpolation (tests/test_cilia_oracle.py).
 """
 from __future__ import annotations
 
@@ -148,3 +153,61 @@ def generalised_forces(mobility, psi, phase_dot=2.0 * math.pi, base=(0.0, 0.0, 1
     x, vp, va, _ = placement(psi, base, nseg, dl)
     f = np.linalg.solve(mobility(x), phase_dot * vp)
     return float(vp @ f), float(va @ f)
+
+
+# ---- dynamic phase / shape-angle evolution (the coupled prescribed system) -----------
+
+def cyclic_hermite(table, phase):
+    """Periodic cubic Hermite interpolation of a table sampled at phase = 2 pi k / n
+    with central-difference slopes (a one-sided slope at node 0), as
+    mobility_solver.cpp:242-281 does."""
+    n = len(table)
+    x = 0.5 * phase / math.pi
+    x = (x - math.floor(x)) * n
+    lo = int(x)
+    if lo == n:
+        return float(table[0])
+    hi = 0 if lo == n - 1 else lo + 1
+    d_lo = (table[1] - table[0]) if lo == 0 else 0.5 * (table[hi] - table[lo - 1])
+    d_hi = (table[0] - table[n - 1]) if hi == 0 else 0.5 * (table[0 if hi == n - 1 else hi + 1] - table[lo])
+    t = x - lo
+    t2, t3 = t * t, t * t * t
+    return float(table[lo] * (2 * t3 - 3 * t2 + 1) + d_lo * (t3 - 2 * t2 + t)
+                 + table[hi] * (3 * t2 - 2 * t3) + d_hi * (t3 - t2))
+
+
+def driving_forces(phase, angle, omega0, q_phase_table, q_angle_table, spring=0.0, dl=2.6,
+                   nseg=20):
+    """Generalised driving forces of one cilium (mobility_solver.cpp:306-356 with
+    FORCE_NOISE_MAG = 0 and the linear torsional spring)."""
+    L = dl * (nseg - 1)
+    qp = cyclic_hermite(q_phase_table, phase) * 0.5 * omega0 / math.pi
+    qa = None
+    if q_angle_table is not None:
+        qa = cyclic_hermite(q_angle_table, phase) * 0.5 * omega0 / math.pi
+        qa -= spring * omega0 * L ** 3 * angle
+    return qp, qa
+
+
+def coupled_solve(M, vphase, vangle, q_phase, q_angle=None):
+    """Dense solve of the coupled system of mobility_solver.cpp:1891-2115 for cilia
+    above the wall (body velocities prescribed):
+        M F - sum_c (phi_dot_c K_c + psi_dot_c A_c) = 0,
+        -K_c . F = -q_phase_c,   -A_c . F = -q_angle_c,
+    with K_c, A_c the phase and angle velocity directions of cilium c (3 nseg each,
+    zero outside its own segments).  Returns (F, phi_dot, psi_dot or None)."""
+    n = M.shape[0]
+    nc = len(vphase)
+    cols = list(vphase) + (list(vangle) if q_angle is not None else [])
+    k = len(cols)
+    A = np.zeros((n + k, n + k))
+    A[:n, :n] = M
+    for j, v in enumerate(cols):
+        A[:n, n + j] = -v
+        A[n + j, :n] = -v
+    rhs = np.zeros(n + k)
+    rhs[n:n + nc] = -np.asarray(q_phase)
+    if q_angle is not None:
+        rhs[n + nc:] = -np.asarray(q_angle)
+    z = np.linalg.solve(A, rhs)
+    return z[:n], z[n:n + nc], (z[n + nc:] if q_angle is not None else None)

@@ -74,3 +74,40 @@ def test_placement_geometry():
     assert not vp[:3].any() and not va[:3].any()
     r = p - p[0]
     assert np.allclose(va.reshape(-1, 3), np.cross([-1.0, 0.0, 0.0], r))
+
+
+def test_cyclic_hermite_interpolation():
+    rng = np.random.default_rng(1)
+    table = rng.standard_normal(40)
+    n = table.size
+    for k in (0, 1, 17, n - 1):
+        assert co.cyclic_hermite(table, 2 * math.pi * k / n) == pytest.approx(table[k], abs=1e-14)
+    # periodic in the phase, and exact for a constant table
+    assert co.cyclic_hermite(table, 0.3) == pytest.approx(co.cyclic_hermite(table, 0.3 + 2 * math.pi), abs=1e-12)
+    assert co.cyclic_hermite(np.full(7, 2.5), 1.234) == pytest.approx(2.5, abs=1e-14)
+    # interior slopes are central differences: the derivative at node 5 is continuous
+    h = 1e-7 * 2 * math.pi / n
+    x5 = 2 * math.pi * 5 / n
+    left = (co.cyclic_hermite(table, x5) - co.cyclic_hermite(table, x5 - h)) / h
+    right = (co.cyclic_hermite(table, x5 + h) - co.cyclic_hermite(table, x5)) / h
+    assert left == pytest.approx(right, rel=1e-4)
+    assert right == pytest.approx(0.5 * (table[6] - table[4]) * n / (2 * math.pi), rel=1e-4)
+
+
+@pytest.mark.parametrize("with_angle", [False, True])
+def test_driving_a_cilium_with_its_own_forces_recovers_the_beat_speed(with_angle):
+    """Closed loop through the reference's own files: they hold the generalised forces
+    of a cilium beating at 2 pi with a fixed shape angle, so driving the coupled system
+    with them must return phase speed 2 pi (and zero angle speed) to the files'
+    single-precision / 1e-4 accuracy."""
+    qp_tab = _golden("fb_phase_generalised_forces_nseg20_sep2.6.dat")
+    qa_tab = _golden("fb_angle_generalised_forces_nseg20_sep2.6.dat") if with_angle else None
+    for nt in (0, 61, 150, 233):
+        psi = 2 * math.pi * nt / 300
+        x, vp, va, _ = co.placement(psi, (0.0, 0.0, 1.3))
+        qp, qa = co.driving_forces(psi, 0.0, 2 * math.pi, qp_tab, qa_tab)
+        M = oracle.mobility_matrix(x, 3)
+        F, phid, psid = co.coupled_solve(M, [vp], [va], [qp], None if qa is None else [qa])
+        assert phid[0] == pytest.approx(2 * math.pi, rel=2e-4)
+        if with_angle:
+            assert abs(psid[0]) <= 2e-4 * 2 * math.pi

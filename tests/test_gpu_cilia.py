@@ -84,3 +84,61 @@ def test_init_rejects_wrong_contexts():
         assert L.lib().cfm_cilia_init(m._h, ctypes.byref(prm), L.ptr(b), L.ptr(p), L.ptr(w)) == L.CFM_EINVAL
     with RpyMobilitySolver(20, 0, seg_dof=3) as m:
         assert L.lib().cfm_cilia_step(m._h, 0, None, None, None) == L.CFM_ESTATE
+
+
+@pytest.mark.parametrize("with_angle,precond", [(False, "diag"), (True, "diag"), (True, "self")])
+def test_driving_with_recorded_forces_recovers_the_beat(with_angle, precond):
+    """Closed loop through the reference's files: a cilium driven by its own recorded
+    generalised forces beats at 2 pi with no shape rotation, step after step (the
+    files hold single-precision values from solves to 1e-4)."""
+    qp = _golden("fb_phase_generalised_forces_nseg20_sep2.6.dat")
+    qa = _golden("fb_angle_generalised_forces_nseg20_sep2.6.dat") if with_angle else None
+    with PrescribedCilia([0.0, 0.0, 1.3], precond=precond) as C:
+        C.set_driving(qp, qa, spring=0.0)
+        for nt in range(60):
+            p, a, st = C.step(nt)
+            assert st["converged"] and st["rel_residual"] <= 1e-10
+            sp = C.speeds()
+            assert sp["phase_dot"][0] == pytest.approx(2 * math.pi, rel=3e-4), nt
+            assert abs(sp["angle_dot"][0]) <= 3e-4 * 2 * math.pi
+            # the generalised force balances the drive
+            assert p[0] == pytest.approx(np.interp(sp["phase"][0], 2 * math.pi * np.arange(300) / 300, qp), rel=1e-3)
+
+
+def test_coupled_lattice_matches_dense_solve():
+    base = wall_lattice(2, 3, spacing=30.0)
+    n = base.size // 3
+    rng = np.random.default_rng(2)
+    phase = rng.uniform(0.0, 2.0 * math.pi, n)
+    omega0 = 2.0 * math.pi * (1.0 + 0.05 * rng.standard_normal(n))
+    qp_tab = _golden("fb_phase_generalised_forces_nseg20_sep2.6.dat")
+    qa_tab = _golden("fb_angle_generalised_forces_nseg20_sep2.6.dat")
+    spring = 0.01
+    with PrescribedCilia(base, phase, omega0, precond="self") as C:
+        C.set_driving(qp_tab, qa_tab, spring=spring)
+        C.step(0)
+        state, sp = C.state(), C.speeds()
+    xs, vps, vas, qps, qas = [], [], [], [], []
+    L = 2.6 * 19
+    for c in range(n):
+        x, vp, va, _ = co.placement(phase[c], base[3 * c:3 * c + 3])
+        z = np.zeros(60 * n)
+        xs.append(x)
+        zp, za = np.zeros(60 * n), np.zeros(60 * n)
+        zp[60 * c:60 * c + 60], za[60 * c:60 * c + 60] = vp, va
+        vps.append(zp), vas.append(za)
+        q1, q2 = co.driving_forces(phase[c], 0.0, omega0[c], qp_tab, qa_tab, spring=spring)
+        qps.append(q1), qas.append(q2)
+    M = oracle.mobility_matrix(np.concatenate(xs), 3)
+    F, phid, psid = co.coupled_solve(M, vps, vas, qps, qas)
+    # residual of the GPU solution in the oracle's coupled operator (the solve's own
+    # criterion, rtol 1e-10), then the solution itself to the saddle system's conditioning
+    m = M.shape[0]
+    K = np.stack(vps + vas, axis=1)
+    A = np.block([[M, -K], [-K.T, np.zeros((2 * n, 2 * n))]])
+    b = np.concatenate([np.zeros(m), -np.asarray(qps), -np.asarray(qas)])
+    zg = np.concatenate([state["f"], sp["phase_dot"], sp["angle_dot"]])
+    assert np.linalg.norm(A @ zg - b) <= 2e-10 * np.linalg.norm(b)
+    assert np.abs(state["f"] - F).max() <= 1e-6 * np.abs(F).max()
+    assert np.allclose(sp["phase_dot"], phid, rtol=1e-6, atol=0)
+    assert np.allclose(sp["angle_dot"], psid, rtol=0, atol=1e-6 * np.abs(phid).max())
