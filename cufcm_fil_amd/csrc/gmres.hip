@@ -317,6 +317,13 @@ Status norm2(cfm_ctx* c, Krylov& K, const double* w, int L, double* out) {
   return {};
 }
 
+// Copies n <= 4 consecutive device doubles to K.host_res[0..n) with one wait.
+Status read_scalars(cfm_ctx* c, Krylov& K, const double* dev, int n) {
+  CFM_HIP(hipMemcpyAsync(K.host_res, dev, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+  CFM_HIP(hipStreamSynchronize(c->stream));
+  return {};
+}
+
 Status read_scalar(cfm_ctx* c, Krylov& K, const double* dev, double* host) {
   CFM_HIP(hipMemcpyAsync(K.host_res, dev, sizeof(double), hipMemcpyDeviceToHost, c->stream));
   CFM_HIP(hipStreamSynchronize(c->stream));
@@ -501,29 +508,36 @@ Status solve(cfm_ctx* c, Krylov& K, const SystemDesc& s, const cfm_solve_opts& o
   const int max_iter = o.max_iter > 0 ? o.max_iter : 1000;
   CFM_TRY(K.reserve(L, restart));
   cfm_solve_stats S{};
-  double nb = 0.0;
+  // ||b|| and the FP64 residual of the guess come back with one wait
+  double nb = 0.0, nr = 0.0;
   CFM_TRY(norm2(c, K, b, L, K.scal.p + 3));
-  CFM_TRY(read_scalar(c, K, K.scal.p + 3, &nb));
+  if (o.use_guess) {
+    CFM_TRY(system_apply(c, s, CFM_FP64, x, K.w.p));
+    hipLaunchKernelGGL(axpby_kernel, g1(L), dim3(256), 0, c->stream, 1.0, b, -1.0, K.w.p, K.d.p, L);
+    CFM_CHECK_LAUNCH();
+    CFM_TRY(norm2(c, K, K.d.p, L, K.scal.p));
+    CFM_TRY(read_scalars(c, K, K.scal.p, 4));
+    nr = K.host_res[0];
+    nb = K.host_res[3];
+  } else {
+    CFM_TRY(read_scalar(c, K, K.scal.p + 3, &nb));
+    nr = nb;
+  }
   if (!std::isfinite(nb)) return make_status(CFM_ENONFINITE, "non-finite right-hand side");
-  if (nb == 0.0) {  // zero rhs: zero solution (mobility_solver.cpp:2221-2294)
+  if (nb == 0.0 || !o.use_guess) {
     hipLaunchKernelGGL(fill_kernel, g1(L), dim3(256), 0, c->stream, x, 0.0, L);
     CFM_CHECK_LAUNCH();
+  }
+  if (nb == 0.0) {  // zero rhs: zero solution (mobility_solver.cpp:2221-2294)
     S.converged = 1;
     if (st) *st = S;
     return {};
   }
-  if (!o.use_guess) {
-    hipLaunchKernelGGL(fill_kernel, g1(L), dim3(256), 0, c->stream, x, 0.0, L);
-    CFM_CHECK_LAUNCH();
-  }
   const double target = o.rtol * nb;
-  // FP64 residual of the guess; reject a guess worse than zero
-  double nr = 0.0;
-  CFM_TRY(system_apply(c, s, CFM_FP64, x, K.w.p));
-  hipLaunchKernelGGL(axpby_kernel, g1(L), dim3(256), 0, c->stream, 1.0, b, -1.0, K.w.p, K.d.p, L);
-  CFM_CHECK_LAUNCH();
-  CFM_TRY(norm2(c, K, K.d.p, L, K.scal.p));
-  CFM_TRY(read_scalar(c, K, K.scal.p, &nr));
+  if (!o.use_guess) {  // x = 0: the residual is b itself
+    CFM_HIP(hipMemcpyAsync(K.d.p, b, sizeof(double) * L, hipMemcpyDeviceToDevice, c->stream));
+  }
+  // reject a guess worse than zero
   if (!std::isfinite(nr)) return make_status(CFM_ENONFINITE, "non-finite residual of the guess");
   if (nr > nb) {
     hipLaunchKernelGGL(fill_kernel, g1(L), dim3(256), 0, c->stream, x, 0.0, L);

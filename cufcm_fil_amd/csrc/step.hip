@@ -34,10 +34,21 @@ Status gnn_record(cfm_ctx* c, const double* b, const double* x);
 struct History {
   static constexpr int H = 16;
   int L = 0, count = 0, head = 0;
-  DevBuf<double> B, X, g, coef, partial;
+  DevBuf<double> B, X, g, grow, coef, partial;
   double G[H][H] = {};
-  double* host = nullptr;
+  double* host = nullptr;  // pinned: [0,H) dots with b, [H,2H) new Gram row, [2H,3H) coefficients
+  int pending = -1;        // slot whose Gram row is computed on the device but not yet read
 };
+
+// Folds a Gram row that has reached h.host[H..] into G.
+static void history_fold_row(History& h) {
+  if (h.pending < 0) return;
+  for (int i = 0; i < h.count; ++i) {
+    h.G[h.pending][i] = h.host[History::H + i];
+    h.G[i][h.pending] = h.host[History::H + i];
+  }
+  h.pending = -1;
+}
 
 static Status history_guess(cfm_ctx* c, History& h, const double* b, double* x0, bool* used) {
   *used = false;
@@ -45,7 +56,10 @@ static Status history_guess(cfm_ctx* c, History& h, const double* b, double* x0,
   const int k = h.count;
   CFM_TRY(dev_dots(c, h.partial.p, h.B.p, (size_t)h.L, k, b, h.L, h.g.p));
   CFM_HIP(hipMemcpyAsync(h.host, h.g.p, sizeof(double) * k, hipMemcpyDeviceToHost, c->stream));
+  if (h.pending >= 0)  // the last push's Gram row comes back with the same wait
+    CFM_HIP(hipMemcpyAsync(h.host + History::H, h.grow.p, sizeof(double) * k, hipMemcpyDeviceToHost, c->stream));
   CFM_HIP(hipStreamSynchronize(c->stream));
+  history_fold_row(h);
   // solve (G + d I) c = g by Gaussian elimination with partial pivoting (k <= 16)
   double A[History::H][History::H + 1];
   double tr = 0.0;
@@ -74,27 +88,29 @@ static Status history_guess(cfm_ctx* c, History& h, const double* b, double* x0,
     cvec[i] = s / A[i][i];
     if (!std::isfinite(cvec[i])) return {};
   }
-  CFM_HIP(hipMemcpyAsync(h.coef.p, cvec, sizeof(double) * k, hipMemcpyHostToDevice, c->stream));
+  // pinned staging: rewritten only after the next wait on this stream
+  double* pc = h.host + 2 * History::H;
+  for (int i = 0; i < k; ++i) pc[i] = cvec[i];
+  CFM_HIP(hipMemcpyAsync(h.coef.p, pc, sizeof(double) * k, hipMemcpyHostToDevice, c->stream));
   CFM_TRY(dev_gemv(c, h.X.p, (size_t)h.L, k, h.coef.p, x0, h.L));
-  CFM_HIP(hipStreamSynchronize(c->stream));  // cvec is a host temporary
   *used = true;
   return {};
 }
 
 static Status history_push(cfm_ctx* c, History& h, const double* b, const double* x) {
+  if (h.pending >= 0) {  // a row no guess has read yet: fetch it before it is overwritten
+    CFM_HIP(hipMemcpyAsync(h.host + History::H, h.grow.p, sizeof(double) * h.count, hipMemcpyDeviceToHost, c->stream));
+    CFM_HIP(hipStreamSynchronize(c->stream));
+    history_fold_row(h);
+  }
   const int slot = h.head;
   CFM_HIP(hipMemcpyAsync(h.B.p + (size_t)slot * h.L, b, sizeof(double) * h.L, hipMemcpyDeviceToDevice, c->stream));
   CFM_HIP(hipMemcpyAsync(h.X.p + (size_t)slot * h.L, x, sizeof(double) * h.L, hipMemcpyDeviceToDevice, c->stream));
   h.head = (h.head + 1) % History::H;
   h.count = std::min(h.count + 1, History::H);
-  // new row/column of the Gram matrix
-  CFM_TRY(dev_dots(c, h.partial.p, h.B.p, (size_t)h.L, h.count, h.B.p + (size_t)slot * h.L, h.L, h.g.p));
-  CFM_HIP(hipMemcpyAsync(h.host, h.g.p, sizeof(double) * h.count, hipMemcpyDeviceToHost, c->stream));
-  CFM_HIP(hipStreamSynchronize(c->stream));
-  for (int i = 0; i < h.count; ++i) {
-    h.G[slot][i] = h.host[i];
-    h.G[i][slot] = h.host[i];
-  }
+  // new row/column of the Gram matrix, read back by the next guess
+  CFM_TRY(dev_dots(c, h.partial.p, h.B.p, (size_t)h.L, h.count, h.B.p + (size_t)slot * h.L, h.L, h.grow.p));
+  h.pending = slot;
   return {};
 }
 
@@ -167,9 +183,11 @@ static Status velocities(cfm_ctx* c, FilState* F, cfm_step_stats* st) {
       CFM_TRY(h.B.alloc((size_t)History::H * s.L));
       CFM_TRY(h.X.alloc((size_t)History::H * s.L));
       CFM_TRY(h.g.alloc(History::H));
+      CFM_TRY(h.grow.alloc(History::H));
       CFM_TRY(h.coef.alloc(History::H));
       CFM_TRY(h.partial.alloc(dev_dots_partial_size(s.L, History::H)));
-      if (!h.host) CFM_HIP(hipHostMalloc((void**)&h.host, sizeof(double) * History::H, hipHostMallocDefault));
+      h.pending = -1;
+      if (!h.host) CFM_HIP(hipHostMalloc((void**)&h.host, sizeof(double) * 3 * History::H, hipHostMallocDefault));
     }
     bool used = false;
     CFM_TRY(history_guess(c, h, F->rhs_b.p, F->fb_prev.p, &used));
@@ -294,6 +312,7 @@ void cfm_fil_release(cfm_ctx* c) {
   F->hist.B.release();
   F->hist.X.release();
   F->hist.g.release();
+  F->hist.grow.release();
   F->hist.coef.release();
   F->hist.partial.release();
   if (F->hist.host) (void)hipHostFree(F->hist.host);
