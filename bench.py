@@ -28,6 +28,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -249,6 +250,63 @@ def run_cilia(args):
     }
 
 
+def run_free_body(args):
+    """Free rigid body: the 6,000-blob sphere of the headline workload (radius 40a,
+    centre 90a above the wall) as a free body.  Times make_body_reference_matrices
+    (six unit-load solves) and the free-body resistance solve (blob forces and body
+    velocities for a body load plus a background velocity at the blobs) with the
+    reference's block preconditioner in FP64 and with the exact Schur complement
+    around the resident inverse."""
+    import numpy as np
+
+    from cufcm_fil_amd import RpyMobilitySolver
+    from cufcm_fil_amd import geometry as geo
+    R, h, nb = 40.0, 90.0, 6000
+    ref = geo.fibonacci_sphere(nb, R)
+    q = np.array([0.9, 0.1, -0.3, 0.2])
+    q /= np.linalg.norm(q)
+    rng = np.random.default_rng(3)
+    rhs = np.concatenate([0.01 * rng.standard_normal(3 * nb), rng.standard_normal(6)])
+    out = {"workload": "sphere R=40a of 6000 blobs, centre 90a above the wall, free body; "
+                       "make_body_reference_matrices (6 solves) and the free-body "
+                       "resistance solve (3 x 6000 + 6 unknowns), rtol 1e-10"}
+    with RpyMobilitySolver(0, nb, wall=True, precision="fp32c", tile_split=-1) as m:
+        m.body_init(ref, 1)
+        m.body_set_state([0.0, 0.0, h], q)
+        t0 = time.perf_counter()
+        _, N, st = m.make_body_reference_matrices(0, precond="dense", mode="fp64", rtol=1e-10)
+        out["reference_matrices_s"] = round(time.perf_counter() - t0, 3)
+        out["reference_matrices_iterations"] = st["iterations"]
+        # lab-frame translational mobility relative to 1/(6 pi mu R) beside Faxen's
+        # (parallel) and Brenner's (normal) wall series; N comes back in the body frame
+        w, a, b, c = q
+        Rq = np.array([[1 - 2 * (b * b + c * c), 2 * (a * b - w * c), 2 * (a * c + w * b)],
+                       [2 * (a * b + w * c), 1 - 2 * (a * a + c * c), 2 * (b * c - w * a)],
+                       [2 * (a * c - w * b), 2 * (b * c + w * a), 1 - 2 * (a * a + b * b)]])
+        Ntt = Rq @ N[:3, :3] @ Rq.T
+        s6 = 6.0 * math.pi * R
+        sr = R / h
+        out["mobility_parallel_x6piR"] = round(0.5 * (Ntt[0, 0] + Ntt[1, 1]) * s6, 5)
+        out["mobility_normal_x6piR"] = round(Ntt[2, 2] * s6, 5)
+        out["faxen_parallel"] = round(1 - 9 / 16 * sr + sr ** 3 / 8 - 45 / 256 * sr ** 4 - sr ** 5 / 16, 5)
+        out["brenner_normal"] = round(1 - 9 / 8 * sr + sr ** 3 / 2 - 0.57 * sr ** 4 + 0.2 * sr ** 5, 5)
+        for name, pc, mode, reps in [("diag_fp64", "diag", "fp64", 2), ("dense_fp64", "dense", "fp64", 5),
+                                     ("dense_mixed", "dense", "mixed", 5)]:
+            m.solve("body", rhs, rtol=1e-10, precond=pc, mode=mode, max_iter=5000)  # warm-up / build
+            its, res = 0, 0.0
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                _, st = m.solve("body", rhs, rtol=1e-10, precond=pc, mode=mode, max_iter=5000)
+                its += st["iterations"]
+                res = max(res, st["rel_residual"])
+            el = (time.perf_counter() - t0) / reps
+            out[name] = {"ms_per_solve": round(1e3 * el, 3), "iterations": its / reps,
+                         "rel_residual": res}
+            print(f"[bench] free body {name}: {1e3 * el:.1f} ms/solve, {its / reps:.0f} iterations",
+                  file=sys.stderr, flush=True)
+    return out
+
+
 def cpu_baseline(args, flops_per_step):
     """FP64 OpenMP oracle apply on a bounded sample of targets, extrapolated."""
     import numpy as np
@@ -295,6 +353,7 @@ def main():
     ap.add_argument("--skip-batch", action="store_true")
     ap.add_argument("--skip-learned", action="store_true")
     ap.add_argument("--skip-cilia", action="store_true")
+    ap.add_argument("--skip-body", action="store_true")
     ap.add_argument("--batch-reps", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--precond", default="dense", choices=["near", "dense"],
@@ -338,6 +397,7 @@ def main():
     batched = None if args.skip_batch else run_batched(args)
     learned = None if args.skip_learned else run_learned(args)
     cilia = None if args.skip_cilia else run_cilia(args)
+    body = None if args.skip_body else run_free_body(args)
     cpu = None
     if not args.skip_cpu:
         ref = plain if plain else mixed
@@ -372,6 +432,7 @@ def main():
         "batched": batched,
         "learned": learned,
         "prescribed_cilia": cilia,
+        "free_body": body,
     }
     print(json.dumps(out), flush=True)
 

@@ -18,7 +18,7 @@ HEADER = os.path.join(os.path.dirname(HERE), "include", "cufcm_fil.h")
 CFM_OK, CFM_EINVAL, CFM_EHIP, CFM_ENOMEM, CFM_ESTATE, CFM_ENOCONV, CFM_ENONFINITE = range(7)
 CFM_FP64, CFM_FP32C = 0, 1
 CFM_SS, CFM_SB, CFM_BB, CFM_BS, CFM_FULL = range(5)
-CFM_SYS_SEG, CFM_SYS_BLOB = 0, 1
+CFM_SYS_SEG, CFM_SYS_BLOB, CFM_SYS_BODY = 0, 1, 2
 CFM_PC_NONE, CFM_PC_DIAG, CFM_PC_SELF, CFM_PC_NEAR, CFM_PC_DENSE = 0, 1, 2, 3, 4
 CFM_SOLVE_FP64, CFM_SOLVE_MIXED = 0, 1
 
@@ -208,6 +208,13 @@ def lib():
     L.cfm_gmres.argtypes = [_vp, ctypes.POINTER(cfm_solve_opts), _dp, _dp,
                             ctypes.POINTER(cfm_solve_stats)]
     L.cfm_apply_preconditioner.argtypes = [_vp, ctypes.c_int32, ctypes.c_int32, _dp, _dp]
+    L.cfm_system_apply.argtypes = [_vp, ctypes.c_int32, ctypes.c_int32, _dp, _dp]
+    L.cfm_body_init.argtypes = [_vp, ctypes.c_int32, _dp]
+    L.cfm_body_set_state.argtypes = [_vp, _dp, _dp]
+    L.cfm_body_reference_matrices.argtypes = [_vp, ctypes.c_int32, ctypes.POINTER(cfm_solve_opts),
+                                              _dp, _dp, ctypes.POINTER(cfm_solve_stats)]
+    L.cfm_body_set_reference_matrices.argtypes = [_vp, _dp, _dp]
+    L.cfm_body_get_reference_matrices.argtypes = [_vp, _dp, _dp]
     L.cfm_fil_init.argtypes = [_vp, ctypes.POINTER(cfm_fil_params), _dp, _dp]
     L.cfm_fil_step.argtypes = [_vp, ctypes.c_int32, ctypes.POINTER(cfm_step_stats)]
     L.cfm_fil_get_state.argtypes = [_vp, _dp, _dp, _dp, _dp, _dp]

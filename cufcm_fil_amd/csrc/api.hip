@@ -84,6 +84,36 @@ static bool all_finite(const double* a, size_t n) {
   return true;
 }
 
+// Host <-> device system vectors: the particle block is AoS on the host and [dof][m]
+// on the device; the extra entries (6 per free body) follow in the same order on both.
+static Status resolve_system(cfm_ctx* c, int system, SystemDesc* s) {
+  if (system == CFM_SYS_BODY) return body_system_desc(c, s);
+  if (system != CFM_SYS_SEG && system != CFM_SYS_BLOB)
+    return make_status(CFM_EINVAL, "unknown system %d", system);
+  *s = system_desc(c, system);
+  return {};
+}
+
+static Status upload_system(cfm_ctx* c, const SystemDesc& s, const double* host, double* dev) {
+  CFM_TRY(upload_aos(c, host, s.m, s.dof, dev, s.m, 0));
+  if (s.extra > 0) {
+    const size_t n0 = (size_t)s.m * s.dof;
+    CFM_HIP(hipMemcpyAsync(dev + n0, host + n0, sizeof(double) * s.extra, hipMemcpyHostToDevice, c->stream));
+    CFM_HIP(hipStreamSynchronize(c->stream));
+  }
+  return {};
+}
+
+static Status download_system(cfm_ctx* c, const SystemDesc& s, const double* dev, double* host) {
+  CFM_TRY(download_aos(c, host, s.m, s.dof, dev, s.m, 0));
+  if (s.extra > 0) {
+    const size_t n0 = (size_t)s.m * s.dof;
+    CFM_HIP(hipMemcpyAsync(host + n0, dev + n0, sizeof(double) * s.extra, hipMemcpyDeviceToHost, c->stream));
+    CFM_HIP(hipStreamSynchronize(c->stream));
+  }
+  return {};
+}
+
 ApplyPlan plan_for(const cfm_ctx* c, int which) {
   ApplyPlan pl;
   const int ns = c->p.n_seg, n = c->p.n;
@@ -136,6 +166,10 @@ static thread_local std::string g_create_error;
 void cfm_fil_release(cfm_ctx* c);
 void cfm_gnn_release(cfm_ctx* c);
 void cfm_cilia_release(cfm_ctx* c);
+void cfm_body_release(cfm_ctx* c);
+namespace cfm {
+void body_positions_changed(cfm_ctx* c);
+}
 
 static int fail(cfm_ctx* c, const Status& s) {
   if (c) c->err = s.msg;
@@ -231,6 +265,7 @@ int cfm_destroy(cfm_ctx* c) {
   cfm_fil_release(c);
   cfm_gnn_release(c);
   cfm_cilia_release(c);
+  cfm_body_release(c);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   if (c->rocblas) (void)rocblas_destroy_handle((rocblas_handle)c->rocblas);
   c->lu_piv.release();
@@ -296,6 +331,7 @@ int cfm_set_positions(cfm_ctx* c, const double* x_seg, const double* x_blob) {
   if (s.ok()) s = up(x_blob, p.n_blob, p.n_seg);
   if (!s.ok()) return fail(c, s);
   c->have_positions = true;
+  body_positions_changed(c);
   // the near-field blocks and the resident inverse depend on the blob positions only
   const size_t nb3 = (size_t)3 * p.n_blob;
   if (c->blob_pos.size() != nb3 || (nb3 && std::memcmp(c->blob_pos.data(), x_blob, sizeof(double) * nb3))) {
@@ -375,8 +411,6 @@ extern "C" {
 int cfm_gmres(cfm_ctx* c, const cfm_solve_opts* o, const double* rhs, double* x,
               cfm_solve_stats* stats) {
   if (!c || !o || !rhs || !x) return c ? fail(c, make_status(CFM_EINVAL, "null argument")) : CFM_EINVAL;
-  if (o->system != CFM_SYS_SEG && o->system != CFM_SYS_BLOB)
-    return fail(c, make_status(CFM_EINVAL, "unknown system %d", o->system));
   if (o->precond < CFM_PC_NONE || o->precond > CFM_PC_DENSE)
     return fail(c, make_status(CFM_EINVAL, "unknown preconditioner %d", o->precond));
   if (o->mode != CFM_SOLVE_FP64 && o->mode != CFM_SOLVE_MIXED)
@@ -384,8 +418,12 @@ int cfm_gmres(cfm_ctx* c, const cfm_solve_opts* o, const double* rhs, double* x,
   if (!(o->rtol > 0.0) || o->restart < 0 || o->max_iter < 0)
     return fail(c, make_status(CFM_EINVAL, "need rtol > 0, restart >= 0, max_iter >= 0"));
   if (!c->have_positions) return fail(c, make_status(CFM_ESTATE, "positions not set"));
-  const SystemDesc s = system_desc(c, o->system);
+  SystemDesc s;
+  Status sd = resolve_system(c, o->system, &s);
+  if (!sd.ok()) return fail(c, sd);
   if (s.m == 0) return fail(c, make_status(CFM_EINVAL, "empty system"));
+  if (s.body && o->precond == CFM_PC_NEAR)
+    return fail(c, make_status(CFM_EINVAL, "the body system takes CFM_PC_NONE, _DIAG, _SELF or _DENSE"));
   if (!all_finite(rhs, s.L) || (o->use_guess && !all_finite(x, s.L)))
     return fail(c, make_status(CFM_ENONFINITE, "non-finite rhs or guess"));
   (void)hipSetDevice(c->device);
@@ -393,11 +431,11 @@ int cfm_gmres(cfm_ctx* c, const cfm_solve_opts* o, const double* rhs, double* x,
   auto run = [&]() -> Status {
     CFM_TRY(c->vec_a.alloc(s.L));
     CFM_TRY(c->vec_b.alloc(s.L));
-    CFM_TRY(upload_aos(c, rhs, s.m, s.dof, c->vec_a.p, s.m, 0));
-    CFM_TRY(upload_aos(c, x, s.m, s.dof, c->vec_b.p, s.m, 0));
+    CFM_TRY(upload_system(c, s, rhs, c->vec_a.p));
+    CFM_TRY(upload_system(c, s, x, c->vec_b.p));
     Status st = solve(c, *c->krylov, s, *o, c->vec_a.p, c->vec_b.p, stats, nullptr);
     if (st.code != CFM_OK && st.code != CFM_ENOCONV) return st;
-    CFM_TRY(download_aos(c, x, s.m, s.dof, c->vec_b.p, s.m, 0));
+    CFM_TRY(download_system(c, s, c->vec_b.p, x));
     return st;
   };
   Status st = run();
@@ -407,19 +445,41 @@ int cfm_gmres(cfm_ctx* c, const cfm_solve_opts* o, const double* rhs, double* x,
 int cfm_apply_preconditioner(cfm_ctx* c, int32_t system, int32_t precond, const double* in,
                              double* out) {
   if (!c || !in || !out) return c ? fail(c, make_status(CFM_EINVAL, "null argument")) : CFM_EINVAL;
-  if (system != CFM_SYS_SEG && system != CFM_SYS_BLOB)
-    return fail(c, make_status(CFM_EINVAL, "unknown system %d", system));
   if (precond < CFM_PC_NONE || precond > CFM_PC_DENSE)
     return fail(c, make_status(CFM_EINVAL, "unknown preconditioner %d", precond));
   if (!c->have_positions) return fail(c, make_status(CFM_ESTATE, "positions not set"));
-  const SystemDesc s = system_desc(c, system);
+  SystemDesc s;
+  Status sd = resolve_system(c, system, &s);
+  if (!sd.ok()) return fail(c, sd);
   (void)hipSetDevice(c->device);
   auto run = [&]() -> Status {
     CFM_TRY(c->vec_a.alloc(s.L));
     CFM_TRY(c->vec_b.alloc(s.L));
-    CFM_TRY(upload_aos(c, in, s.m, s.dof, c->vec_a.p, s.m, 0));
+    CFM_TRY(upload_system(c, s, in, c->vec_a.p));
     CFM_TRY(system_precond(c, s, precond, c->vec_a.p, c->vec_b.p));
-    CFM_TRY(download_aos(c, out, s.m, s.dof, c->vec_b.p, s.m, 0));
+    CFM_TRY(download_system(c, s, c->vec_b.p, out));
+    return {};
+  };
+  Status st = run();
+  return st.ok() ? CFM_OK : fail(c, st);
+}
+
+int cfm_system_apply(cfm_ctx* c, int32_t system, int32_t precision, const double* x, double* y) {
+  if (!c || !x || !y) return c ? fail(c, make_status(CFM_EINVAL, "null argument")) : CFM_EINVAL;
+  if (precision != CFM_FP64 && precision != CFM_FP32C)
+    return fail(c, make_status(CFM_EINVAL, "unknown precision %d", precision));
+  if (!c->have_positions) return fail(c, make_status(CFM_ESTATE, "positions not set"));
+  SystemDesc s;
+  Status sd = resolve_system(c, system, &s);
+  if (!sd.ok()) return fail(c, sd);
+  if (!all_finite(x, s.L)) return fail(c, make_status(CFM_ENONFINITE, "non-finite input"));
+  (void)hipSetDevice(c->device);
+  auto run = [&]() -> Status {
+    CFM_TRY(c->vec_a.alloc(s.L));
+    CFM_TRY(c->vec_b.alloc(s.L));
+    CFM_TRY(upload_system(c, s, x, c->vec_a.p));
+    CFM_TRY(system_apply(c, s, precision, c->vec_a.p, c->vec_b.p));
+    CFM_TRY(download_system(c, s, c->vec_b.p, y));
     return {};
   };
   Status st = run();

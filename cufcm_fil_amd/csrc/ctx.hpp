@@ -111,6 +111,7 @@ struct DenseInverse {
   size_t ld = 0;
   DevBuf<float> P;
   double build_seconds = 0.0;
+  int generation = 0;  // bumped by every build: caches derived from P compare it
   void release() {
     P.release();
     valid = false;
@@ -172,5 +173,6 @@ struct cfm_ctx {
   std::vector<int> prof_fp32;
   void* gnn = nullptr;                  // learned initial guess (gnn.hip)
   void* cilia = nullptr;                // prescribed-shape cilia (cilia.hip)
+  void* body = nullptr;                 // free rigid bodies made of the blobs (body.hip)
   std::string err;
 };

@@ -167,6 +167,7 @@ Status dense_build(cfm_ctx* c, DenseInverse& d) {
   d.n = n;
   d.ld = ld;
   d.valid = true;
+  ++d.generation;
   d.build_seconds =
       std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   return {};

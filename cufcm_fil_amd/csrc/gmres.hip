@@ -366,11 +366,13 @@ Status system_apply(cfm_ctx* c, const SystemDesc& s, int precision, const double
   Status st = launch_apply(c, pl);
   c->cfg.precision = saved;
   if (st.ok() && s.cilia) st = cilia_couple_apply(c, x, y);
+  if (st.ok() && s.body) st = body_couple_apply(c, x, y);
   return st;
 }
 
 Status system_precond(cfm_ctx* c, const SystemDesc& s, int kind, const double* x, double* y) {
   if (s.cilia) return cilia_precond(c, kind, x, y);
+  if (s.body) return body_precond(c, kind, x, y);
   if (kind == CFM_PC_NEAR) {
     if (s.which != CFM_BB) return make_status(CFM_EINVAL, "CFM_PC_NEAR applies to the blob system only");
     if (!c->nf.valid) CFM_TRY(nearfield_build(c, c->nf));
@@ -418,7 +420,7 @@ Status gmres_core(cfm_ctx* c, Krylov& K, const SystemDesc& s, int prec, int pc, 
   double last_est = nr;
   const bool sync_check = pc == CFM_PC_DENSE;
   while (nr > target_abs && *iters < max_iter) {
-    if (s.which == CFM_BB && gnn_recording(c)) CFM_TRY(gnn_record_rhs(c, K.r.p));
+    if (s.which == CFM_BB && !s.body && gnn_recording(c)) CFM_TRY(gnn_record_rhs(c, K.r.p));
     // q0 = r / ||r||
     hipLaunchKernelGGL(scale_by_inv_kernel, g1(L), dim3(256), 0, c->stream, K.r.p, K.scal.p, K.Q.p, L);
     hipLaunchKernelGGL(beta_init_kernel, dim3(1), dim3(64), 0, c->stream, K.beta.p, m + 1, K.scal.p);
@@ -449,7 +451,7 @@ Status gmres_core(cfm_ctx* c, Krylov& K, const SystemDesc& s, int prec, int pc, 
       CFM_HIP(hipMemcpyAsync(K.host_res + 4 + (j % 4), K.res_ring.p + (j % 4), sizeof(double), hipMemcpyDeviceToHost, c->stream));
       // training data for the learned near-field correction: Krylov directions of
       // the blob system visited by this solve
-      if (s.which == CFM_BB && (j == 3 || j == 9 || j == 19 || j == 39) && gnn_recording(c))
+      if (s.which == CFM_BB && !s.body && (j == 3 || j == 9 || j == 19 || j == 39) && gnn_recording(c))
         CFM_TRY(gnn_record_rhs(c, qn));
       CFM_HIP(hipEventRecord(K.ev[j % 4], c->stream));
       ++*iters;

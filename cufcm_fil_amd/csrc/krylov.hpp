@@ -28,11 +28,19 @@ struct SystemDesc {
   // shape-angle speeds) follow the [dof][m] force block, L = m dof + extra.
   bool cilia = false;
   int extra = 0;
+  // Free rigid bodies (body.hip): 6 unknowns per body (V, Omega) follow the [3][m]
+  // blob-force block, L = 3 m + 6 n_body.
+  bool body = false;
 };
 
 // Coupling rows / columns and block preconditioner of the coupled cilia system.
 Status cilia_couple_apply(cfm_ctx* c, const double* x, double* y);
 Status cilia_precond(cfm_ctx* c, int kind, const double* x, double* y);
+
+// Coupling rows / columns, block preconditioners and description of the free-body system.
+Status body_couple_apply(cfm_ctx* c, const double* x, double* y);
+Status body_precond(cfm_ctx* c, int kind, const double* x, double* y);
+Status body_system_desc(cfm_ctx* c, SystemDesc* s);
 
 // Optional correction added to the preconditioner output (learned near-field term).
 struct GmresHooks {

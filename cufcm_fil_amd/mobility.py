@@ -116,13 +116,18 @@ class RpyMobilitySolver:
         return ms.value
 
     # -- linear solves (solve_linear_system / apply_preconditioner_new) -------
-    _SYS = {"seg": L.CFM_SYS_SEG, "blob": L.CFM_SYS_BLOB}
+    _SYS = {"seg": L.CFM_SYS_SEG, "blob": L.CFM_SYS_BLOB, "body": L.CFM_SYS_BODY}
     _PC = {"none": L.CFM_PC_NONE, "diag": L.CFM_PC_DIAG, "self": L.CFM_PC_SELF,
            "near": L.CFM_PC_NEAR, "dense": L.CFM_PC_DENSE}
     _MODE = {"fp64": L.CFM_SOLVE_FP64, "mixed": L.CFM_SOLVE_MIXED}
 
     def _sys_size(self, system):
-        return self.seg_dof * self.n_seg if self._SYS[system] == L.CFM_SYS_SEG else 3 * self.n_blob
+        k = self._SYS[system]
+        if k == L.CFM_SYS_SEG:
+            return self.seg_dof * self.n_seg
+        if k == L.CFM_SYS_BODY:
+            return 3 * self.n_blob + 6 * getattr(self, "_n_body", 0)
+        return 3 * self.n_blob
 
     def solve(self, system: str, rhs, x0=None, *, rtol=1e-8, restart=50, max_iter=1000,
               precond="self", mode="fp64", raise_on_noconv=True):
@@ -148,6 +153,51 @@ class RpyMobilitySolver:
         L.check(self._h, L.lib().cfm_apply_preconditioner(self._h, self._SYS[system],
                                                            self._PC[precond], L.ptr(v), L.ptr(out)))
         return out
+
+    def system_apply(self, system: str, x, precision="fp64"):
+        """y = A x of a whole system (system_matrix_mult, mobility_solver.cpp:1596-1881)."""
+        n = self._sys_size(system)
+        v = L.f64(x, n)
+        out = np.zeros(n)
+        L.check(self._h, L.lib().cfm_system_apply(self._h, self._SYS[system],
+                                                   self._prec(precision), L.ptr(v), L.ptr(out)))
+        return out
+
+    # -- free rigid bodies made of the blobs (rigid_body.cpp, mobility_solver.cpp) --
+    def body_init(self, ref, n_body=1):
+        """n_body bodies sharing the body-frame blob offsets ref (3 n_blob / n_body)."""
+        r = L.f64(ref, 3 * (self.n_blob // max(int(n_body), 1)))
+        L.check(self._h, L.lib().cfm_body_init(self._h, int(n_body), L.ptr(r)))
+        self._n_body = int(n_body)
+
+    def body_set_state(self, x, q):
+        """Body centres (3 n_body) and unit quaternions (w, x, y, z) (4 n_body)."""
+        nb = self._n_body
+        xs, qs = L.f64(x, 3 * nb), L.f64(q, 4 * nb)
+        L.check(self._h, L.lib().cfm_body_set_state(self._h, L.ptr(xs), L.ptr(qs)))
+
+    def make_body_reference_matrices(self, body=0, *, rtol=1e-10, precond="dense", mode="fp64",
+                                     restart=50, max_iter=1000):
+        """(K^T K)^-1 and the 6 x 6 body mobility, body frame (mobility_solver.cpp:3049-3146)."""
+        o = L.cfm_solve_opts(system=L.CFM_SYS_BODY, precond=self._PC[precond],
+                             mode=self._MODE[mode], restart=restart, max_iter=max_iter,
+                             use_guess=0, rtol=rtol)
+        ktk, mob = np.zeros(36), np.zeros(36)
+        st = L.cfm_solve_stats()
+        L.check(self._h, L.lib().cfm_body_reference_matrices(
+            self._h, int(body), ctypes.byref(o), L.ptr(ktk), L.ptr(mob), ctypes.byref(st)))
+        return ktk.reshape(6, 6), mob.reshape(6, 6), st.as_dict()
+
+    def body_set_reference_matrices(self, ktk_inv=None, mobility=None):
+        a = None if ktk_inv is None else L.f64(ktk_inv, 36)
+        b = None if mobility is None else L.f64(mobility, 36)
+        L.check(self._h, L.lib().cfm_body_set_reference_matrices(
+            self._h, None if a is None else L.ptr(a), None if b is None else L.ptr(b)))
+
+    def body_reference_matrices(self):
+        ktk, mob = np.zeros(36), np.zeros(36)
+        L.check(self._h, L.lib().cfm_body_get_reference_matrices(self._h, L.ptr(ktk), L.ptr(mob)))
+        return ktk.reshape(6, 6), mob.reshape(6, 6)
 
     # -- measurement --------------------------------------------------------------
     def profile(self, enable=True):

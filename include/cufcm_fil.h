@@ -100,6 +100,7 @@ int cfm_time_apply(cfm_ctx* ctx, int32_t which, int32_t reps, double* ms_per_app
 /* linear systems */
 #define CFM_SYS_SEG 0   /* Mss f_seg = v_seg  (segment resistance problem) */
 #define CFM_SYS_BLOB 1  /* Mbb f_blob = v_blob (rigid-body blob forces, body held fixed) */
+#define CFM_SYS_BODY 2  /* free rigid bodies: blob forces and body velocities (cfm_body_*) */
 
 /* right preconditioners P^-1 */
 #define CFM_PC_NONE 0
@@ -149,6 +150,10 @@ int cfm_gmres(cfm_ctx* ctx, const cfm_solve_opts* opts, const double* rhs, doubl
  * mobility_solver::apply_preconditioner_new (mobility_solver.cpp:1102-1594). */
 int cfm_apply_preconditioner(cfm_ctx* ctx, int32_t system, int32_t precond, const double* in,
                              double* out);
+/* y = A x for a whole system (mobility_solver.cpp:1596-1881 system_matrix_mult): the pair
+ * products of the system in `precision` (CFM_FP64 / CFM_FP32C) plus, for CFM_SYS_BODY,
+ * the coupling rows and columns -K U and -K^T F in FP64.  Vectors as for cfm_gmres. */
+int cfm_system_apply(cfm_ctx* ctx, int32_t system, int32_t precision, const double* x, double* y);
 
 /* ---- implicit filament step (Broyden) ------------------------------------- */
 /* Tethered elastic filaments with a clamped base and a follower force at the tip
@@ -323,6 +328,39 @@ int cfm_cilia_get_speeds(cfm_ctx* ctx, double* phase, double* phase_dot, double*
  * forces of the last step 3 n_seg. */
 int cfm_cilia_get_state(cfm_ctx* ctx, double* x, double* vel_dir_phase, double* vel_dir_angle,
                         double* s_values, double* phase, double* f_seg);
+
+/* ---- free rigid bodies made of the blobs ---------------------------------------------
+ * The reference's rigid-body blob system (USE_BROYDEN_FOR_EVERYTHING 0,
+ * PRESCRIBED_BODY_VELOCITIES 0): n_body bodies of nb = n_blob / n_body blobs each (body b
+ * owns blobs [b nb, (b+1) nb)), all with the same body-frame offsets `ref` (3 nb, AoS;
+ * rigid_body blob_references), centre x_b and unit quaternion q_b = (w, x, y, z):
+ * blob i of body b sits at x_b + R(q_b) ref_i (rigid_body.cpp:497-515).
+ * System CFM_SYS_BODY (cfm_gmres, cfm_apply_preconditioner, cfm_system_apply): unknowns
+ * F (3 n_blob blob forces, AoS) then U (6 per body: V, Omega), in the reference's order;
+ *     M_bb F - K U = rhs_blob,   -K^T F = rhs_body,   (K U)_i = V_b + Omega_b x r_i
+ * (mobility_solver.cpp:1822-1880).  rhs_blob is minus the velocity the segments induce
+ * at the blobs and rhs_body minus the total force and torque the body must balance
+ * (mobility_solver.cpp:455-632).  Preconditioners: CFM_PC_DIAG is the reference's block
+ * form with 6 pi mu a on the blobs and the rotated reference matrices
+ * (mobility_solver.cpp:1518-1590); CFM_PC_SELF the same with each blob's wall-corrected
+ * self block; CFM_PC_DENSE the exact Schur complement around the resident FP32 inverse
+ * of M_bb (at most 128 bodies).  cfm_set_positions un-places the bodies. */
+int cfm_body_init(cfm_ctx* ctx, int32_t n_body, const double* ref);
+/* Places every body (x 3 n_body, q 4 n_body); with segments in the context, their
+ * positions must have been set first. */
+int cfm_body_set_state(cfm_ctx* ctx, const double* x, const double* q);
+/* make_body_reference_matrices (mobility_solver.cpp:3049-3146): (K^T K)^-1 of one body
+ * and the 6 x 6 mobility of body `body` alone (six solves of CFM_SYS_BODY with a unit
+ * force or torque, options `opts`; the reference's sphere check compares it with
+ * 1/(6 pi mu R), 1/(8 pi mu R^3)).  Both are returned (36 each, row-major, may be null)
+ * in the body frame and kept for CFM_PC_DIAG / _SELF, rotated by R(q_b) for each body;
+ * st (may be null) sums the six solves' steps and keeps their worst residual. */
+int cfm_body_reference_matrices(cfm_ctx* ctx, int32_t body, const cfm_solve_opts* opts,
+                                double* ktk_inv, double* mobility, cfm_solve_stats* st);
+/* The reference matrices without the solves (the reference copies them between swimmers,
+ * cilia_sim_main.cu:293-297); either pointer may be null to keep the current one. */
+int cfm_body_set_reference_matrices(cfm_ctx* ctx, const double* ktk_inv, const double* mobility);
+int cfm_body_get_reference_matrices(cfm_ctx* ctx, double* ktk_inv, double* mobility);
 
 /* ---- batched ensembles of independent wall systems ----------------------------------
  * No counterpart inside the reference's mobility_solver: the reference runs each

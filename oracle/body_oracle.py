@@ -1,0 +1,143 @@
+"""numpy restatement of the free rigid body made of blobs -- TEST INFRASTRUCTURE ONLY
+(imported by tests/ and never by the product).
+
+Follows the reference's rigid-body blob system (USE_BROYDEN_FOR_EVERYTHING 0,
+PRESCRIBED_BODY_VELOCITIES 0):
+  * blob placement x_b + R(q_b) ref_i           rigid_body.cpp:497-515
+  * the saddle-point product
+        [ M_bb  -K ] [F]      M_bb F - K U
+        [ -K^T   0 ] [U]  =   -K^T F
+    with (K U)_i = V_b + Omega_b x r_i, r_i = R ref_i   mobility_solver.cpp:1822-1880
+  * the block preconditioner with the scalar blob approximation 6 pi mu a, the
+    reference (K^T K)^-1 and body mobility rotated by R      mobility_solver.cpp:1518-1590
+  * make_body_reference_matrices: K^T K from the body-frame offsets, and the 6 x 6
+    body mobility from six resistance solves with a unit force or torque on the
+    body (column 5 - n from right-hand side entry -1 at row 6 n_body - 1 - n)
+                                                      mobility_solver.cpp:3049-3146
+Unknown ordering as the reference: 3 n_blob blob forces (AoS), then 6 per body
+(V then Omega).
+
+Parity status: pinned in tests/test_body_oracle.py by what the mathematics fixes --
+K^T K against the Gram matrix of an explicitly assembled K, the preconditioner against
+the exact inverse of the saddle matrix with M_bb replaced by its scalar
+approximation, the body mobility against (K^T M_bb^-1 K)^-1 from a dense solve, its
+symmetry, Stokes drag of a blob sphere (1/(6 pi mu R), 1/(8 pi mu R^3)) and the
+Faxen/Brenner wall corrections of a sphere's translational mobility.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from filament_oracle import rcross, rot_mat
+
+
+def place(ref, x, q):
+    """Blob positions (n_body nb, 3) and rotated offsets of bodies at centres x
+    (n_body, 3) with quaternions q (n_body, 4) (rigid_body.cpp:497-515)."""
+    ref = np.asarray(ref, dtype=np.float64).reshape(-1, 3)
+    pos, off = [], []
+    for xb, qb in zip(np.asarray(x).reshape(-1, 3), np.asarray(q).reshape(-1, 4)):
+        r = ref @ rot_mat(qb).T
+        off.append(r)
+        pos.append(xb + r)
+    return np.concatenate(pos), np.concatenate(off)
+
+
+def k_matrix(offsets, n_body):
+    """Dense K (3 n_blob x 6 n_body): column block b maps (V, Omega) of body b to the
+    velocities V + Omega x r_i of its blobs."""
+    r = np.asarray(offsets).reshape(-1, 3)
+    n = r.shape[0]
+    nb = n // n_body
+    K = np.zeros((3 * n, 6 * n_body))
+    for i in range(n):
+        b = i // nb
+        K[3 * i:3 * i + 3, 6 * b:6 * b + 3] = np.eye(3)
+        # Omega x r = rcross(r) Omega   (rcross(v) w = w x v)
+        K[3 * i:3 * i + 3, 6 * b + 3:6 * b + 6] = rcross(r[i])
+    return K
+
+
+def ktk_reference(ref):
+    """K^T K of one body from its body-frame offsets (mobility_solver.cpp:3052-3070)."""
+    ref = np.asarray(ref).reshape(-1, 3)
+    A = np.zeros((6, 6))
+    A[:3, :3] = ref.shape[0] * np.eye(3)
+    for r in ref:
+        C = rcross(r)
+        A[:3, 3:] += C
+        A[3:, :3] -= C
+        A[3:, 3:] -= C @ C
+    return A
+
+
+def saddle_matrix(M, K):
+    """[[M, -K], [-K^T, 0]] in the reference's unknown order."""
+    n, k = K.shape
+    A = np.zeros((n + k, n + k))
+    A[:n, :n] = M
+    A[:n, n:] = -K
+    A[n:, :n] = -K.T
+    return A
+
+
+def rotate6(A, R):
+    """Each 3 x 3 block of a 6 x 6 matrix conjugated by R (mobility_solver.cpp:1535-1545)."""
+    B = np.array(A, dtype=np.float64)
+    for i in (0, 3):
+        for j in (0, 3):
+            B[i:i + 3, j:j + 3] = R @ A[i:i + 3, j:j + 3] @ R.T
+    return B
+
+
+def precondition(v, offsets, q, ktk_inv_ref, mob_ref, mu=1.0, a=1.0):
+    """The reference's free-body preconditioner (mobility_solver.cpp:1518-1590)."""
+    q = np.asarray(q).reshape(-1, 4)
+    nbody = q.shape[0]
+    r = np.asarray(offsets).reshape(-1, 3)
+    n = r.shape[0]
+    nb = n // nbody
+    minv = 6.0 * math.pi * mu * a
+    out = np.zeros_like(v)
+    for b in range(nbody):
+        R = rot_mat(q[b])
+        Kb = k_matrix(r[b * nb:(b + 1) * nb], 1)
+        fb = v[3 * b * nb:3 * (b + 1) * nb]
+        ub = v[3 * n + 6 * b:3 * n + 6 * b + 6]
+        rhs = -Kb.T @ fb
+        vel = rotate6(ktk_inv_ref, R) @ rhs - rotate6(mob_ref, R) @ ub
+        out[3 * n + 6 * b:3 * n + 6 * b + 6] = vel
+        out[3 * b * nb:3 * (b + 1) * nb] = minv * (fb + Kb @ vel)
+    return out
+
+
+def body_mobility(M, K):
+    """(K^T M^-1 K)^-1: the grand body mobility of the bodies whose blob-blob mobility is
+    M (what six saddle solves with unit body loads return, column by column)."""
+    return np.linalg.inv(K.T @ np.linalg.solve(M, K))
+
+
+def reference_mobility_by_solves(M, K):
+    """make_body_reference_matrices' six solves for a single body: right-hand side zero
+    except -1 at row (rows - 1 - n); column 5 - n of the result is the body velocity."""
+    A = saddle_matrix(M, K)
+    rows = A.shape[0]
+    N = np.zeros((6, 6))
+    for n in range(6):
+        rhs = np.zeros(rows)
+        rhs[rows - 1 - n] = -1.0
+        z = np.linalg.solve(A, rhs)
+        N[:, 5 - n] = z[-6:]
+    return N
+
+
+def faxen_brenner_ratios(R, h):
+    """Translational mobility of a sphere of radius R whose centre is h above a no-slip
+    wall, relative to 1/(6 pi mu R): parallel (Faxen) and normal (Brenner, leading
+    terms of the method-of-reflections series)."""
+    s = R / h
+    par = 1.0 - 9.0 / 16.0 * s + 1.0 / 8.0 * s ** 3 - 45.0 / 256.0 * s ** 4 - 1.0 / 16.0 * s ** 5
+    nor = 1.0 - 9.0 / 8.0 * s + 1.0 / 2.0 * s ** 3 - 57.0 / 100.0 * s ** 4 + 1.0 / 5.0 * s ** 5
+    return par, nor

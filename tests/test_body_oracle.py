@@ -1,0 +1,104 @@
+"""CPU pins of the free-rigid-body oracle (oracle/body_oracle.py): what the mathematics
+fixes about K, the reference's block preconditioner, the body mobility and a blob
+sphere's hydrodynamics (Stokes drag, Faxen/Brenner wall corrections)."""
+import math
+
+import numpy as np
+import pytest
+
+import body_oracle as bo
+import oracle
+from cufcm_fil_amd import geometry as geo
+from filament_oracle import rot_mat
+
+
+def _quat(seed):
+    q = np.random.default_rng(seed).normal(size=4)
+    return q / np.linalg.norm(q)
+
+
+def test_ktk_closed_form_is_the_gram_matrix_of_k():
+    ref = np.random.default_rng(1).normal(size=(37, 3)) * 3.0
+    K = bo.k_matrix(ref, 1)
+    assert np.allclose(bo.ktk_reference(ref), K.T @ K, rtol=0, atol=1e-12)
+    # K maps (V, Omega) to V + Omega x r
+    u = np.random.default_rng(2).normal(size=6)
+    v = (K @ u).reshape(-1, 3)
+    assert np.allclose(v, u[:3] + np.cross(u[3:], ref), atol=1e-13)
+
+
+def test_placement_rotates_offsets_about_the_centre():
+    ref = np.random.default_rng(3).normal(size=(10, 3))
+    x = np.array([[1.0, -2.0, 7.0], [0.5, 0.5, 9.0]])
+    q = np.stack([_quat(4), _quat(5)])
+    pos, off = bo.place(ref, x, q)
+    for b in range(2):
+        R = rot_mat(q[b])
+        assert np.allclose(R @ R.T, np.eye(3), atol=1e-14)
+        assert np.allclose(off[10 * b:10 * b + 10], ref @ R.T, atol=1e-14)
+        assert np.allclose(pos[10 * b:10 * b + 10] - x[b], off[10 * b:10 * b + 10], atol=1e-13)
+
+
+def test_reference_preconditioner_inverts_the_scalar_saddle_matrix():
+    """With M_bb replaced by its scalar approximation m I (m = 1/(6 pi mu a)) and the
+    body mobility by m (K^T K)^-1, mobility_solver.cpp:1518-1590 is the exact inverse
+    of the saddle matrix -- for rotated bodies too (the reference matrices are body
+    frame and rotated by R)."""
+    mu, a = 0.7, 1.3
+    m = 1.0 / (6.0 * math.pi * mu * a)
+    ref = geo.fibonacci_sphere(40, 4.0)
+    x = np.array([[0.0, 0.0, 10.0], [12.0, 0.0, 10.0]])
+    q = np.stack([_quat(7), _quat(8)])
+    pos, off = bo.place(ref, x, q)
+    K = bo.k_matrix(off, 2)
+    A = bo.saddle_matrix(m * np.eye(3 * len(pos)), K)
+    kinv = np.linalg.inv(bo.ktk_reference(ref))
+    v = np.random.default_rng(9).normal(size=A.shape[0])
+    got = bo.precondition(v, off, q, kinv, m * kinv, mu=mu, a=a)
+    assert np.allclose(got, np.linalg.solve(A, v), rtol=0, atol=1e-10 * np.abs(got).max())
+
+
+@pytest.mark.parametrize("wall", [False, True])
+def test_body_mobility_from_six_solves(wall):
+    """make_body_reference_matrices' solves (rhs -1 at row rows-1-n -> column 5-n)
+    give (K^T M^-1 K)^-1, a symmetric positive-definite 6 x 6."""
+    ref = geo.fibonacci_sphere(60, 4.5)
+    pos, off = bo.place(ref, np.array([[0.0, 0.0, 9.0]]), _quat(11)[None])
+    K = bo.k_matrix(off, 1)
+    M = oracle.mobility_matrix(pos, 3, wall=wall)
+    N = bo.reference_mobility_by_solves(M, K)
+    assert np.allclose(N, bo.body_mobility(M, K), rtol=0, atol=1e-12 * np.abs(N).max())
+    assert np.allclose(N, N.T, rtol=0, atol=1e-12 * np.abs(N).max())
+    assert np.linalg.eigvalsh(0.5 * (N + N.T)).min() > 0.0
+
+
+def test_blob_sphere_converges_to_stokes_drag():
+    """The reference's sphere check (mobility_solver.cpp:3116-3132): the body mobility
+    of a blob sphere of radius R approaches diag(1/(6 pi mu R), 1/(8 pi mu R^3)) as
+    the blobs fill its surface (relative error 10.5% at 100 blobs, 4.7% at 400)."""
+    errs = []
+    for n, R in [(100, 4.0), (400, 8.0)]:
+        pos, off = bo.place(geo.fibonacci_sphere(n, R), np.zeros((1, 3)), np.array([[1.0, 0, 0, 0]]))
+        N = bo.body_mobility(oracle.mobility_matrix(pos, 3, wall=False), bo.k_matrix(off, 1))
+        Nt = np.diag([1 / (6 * math.pi * R)] * 3 + [1 / (8 * math.pi * R ** 3)] * 3)
+        errs.append(np.linalg.norm(N - Nt) / np.linalg.norm(Nt))
+        # isotropic to the seeding's regularity, no translation-rotation coupling
+        assert np.abs(np.diag(N)[:3] / N[0, 0] - 1).max() < 0.01
+        assert np.abs(N[:3, 3:]).max() < 0.02 * math.sqrt(N[0, 0] * N[3, 3])
+    assert 0.09 < errs[0] < 0.12 and 0.04 < errs[1] < 0.055
+
+
+def test_blob_sphere_near_the_wall_follows_faxen_and_brenner():
+    """Translational mobility of the blob sphere at h = 3R above the wall relative to
+    its unbounded value: Faxen's parallel series to 1%, Brenner's normal series to 3%
+    (a dropped or mis-signed image term moves these by 20-60%)."""
+    R, h = 8.0, 24.0
+    ref = geo.fibonacci_sphere(400, R)
+    pos, off = bo.place(ref, np.zeros((1, 3)), np.array([[1.0, 0, 0, 0]]))
+    K = bo.k_matrix(off, 1)
+    N0 = bo.body_mobility(oracle.mobility_matrix(pos, 3, wall=False), K)
+    Nw = bo.body_mobility(oracle.mobility_matrix(pos + [0.0, 0.0, h], 3, wall=True), K)
+    par, nor = bo.faxen_brenner_ratios(R, h)
+    assert abs(Nw[0, 0] / N0[0, 0] / par - 1) < 0.01
+    assert abs(Nw[1, 1] / N0[1, 1] / par - 1) < 0.01
+    assert abs(Nw[2, 2] / N0[2, 2] / nor - 1) < 0.03
