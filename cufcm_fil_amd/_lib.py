@@ -125,6 +125,7 @@ class cfm_profile_stats(ctypes.Structure):
 
 
 CFM_BEAT_FULFORD_BLAKE = 1
+CFM_BEAT_FULFORD_BLAKE_ORIGINAL = 2
 
 
 class cfm_cilia_params(ctypes.Structure):
@@ -135,6 +136,7 @@ class cfm_cilia_params(ctypes.Structure):
         ("reserved", ctypes.c_int32),
         ("dl", ctypes.c_double),
         ("dt", ctypes.c_double),
+        ("s_tol", ctypes.c_double),
         ("solve", cfm_solve_opts),
     ]
 
@@ -246,6 +248,7 @@ def lib():
     L.cfm_cilia_set_driving.argtypes = [_vp, _dp, ctypes.c_int32, _dp, ctypes.c_int32,
                                         ctypes.c_double]
     L.cfm_cilia_get_speeds.argtypes = [_vp, _dp, _dp, _dp, _dp]
+    L.cfm_cilia_set_s_table.argtypes = [_vp, _dp, ctypes.c_int32, ctypes.c_int32]
     L.cfm_batch_create.argtypes = [ctypes.POINTER(cfm_batch_config), _ip, ctypes.POINTER(_vp)]
     L.cfm_batch_destroy.argtypes = [_vp]
     L.cfm_batch_last_error.restype = ctypes.c_char_p

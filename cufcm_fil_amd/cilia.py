@@ -29,7 +29,7 @@ class PrescribedCilia:
 
     def __init__(self, base, phase=None, phase_dot=None, *, nseg=20, dl=2.6, dt=1.0 / 300,
                  precision="fp64", solve_mode="fp64", precond="self", rtol=1e-10, restart=50,
-                 max_iter=1000, device=0, tile_split=0):
+                 max_iter=1000, device=0, tile_split=0, beat="fulford_blake", s_tol=0.0):
         base = L.f64(base)
         n = base.size // 3
         self.n_cilia, self.nseg = n, nseg
@@ -40,8 +40,10 @@ class PrescribedCilia:
         o = L.cfm_solve_opts(system=L.CFM_SYS_SEG, precond=RpyMobilitySolver._PC[precond],
                              mode=RpyMobilitySolver._MODE[solve_mode], restart=restart,
                              max_iter=max_iter, use_guess=1, rtol=rtol)
-        self.prm = L.cfm_cilia_params(n_cilia=n, nseg=nseg, beat=L.CFM_BEAT_FULFORD_BLAKE,
-                                      dl=dl, dt=dt, solve=o)
+        beats = {"fulford_blake": L.CFM_BEAT_FULFORD_BLAKE,
+                 "fulford_blake_original": L.CFM_BEAT_FULFORD_BLAKE_ORIGINAL}
+        self.prm = L.cfm_cilia_params(n_cilia=n, nseg=nseg, beat=beats[beat], dl=dl, dt=dt,
+                                      s_tol=s_tol, solve=o)
         L.check(self.mob._h, L.lib().cfm_cilia_init(self.mob._h, ctypes.byref(self.prm),
                                                     L.ptr(base), L.ptr(phase),
                                                     L.ptr(phase_dot)))
@@ -63,6 +65,17 @@ class PrescribedCilia:
         L.check(self.mob._h, L.lib().cfm_cilia_set_driving(
             self.mob._h, L.ptr(qp), qp.size, L.ptr(qa), 0 if qa is None else qa.size,
             float(spring)))
+
+    def set_s_table(self, table=None):
+        """Segment parameters from a recorded table (rows: phases 2 pi k / rows), as the
+        reference reads its s_values file; None returns to the bisection."""
+        if table is None:
+            L.check(self.mob._h, L.lib().cfm_cilia_set_s_table(self.mob._h, None, 0, 0))
+            return
+        t = np.ascontiguousarray(table, dtype=np.float64)
+        rows, cols = t.shape
+        L.check(self.mob._h, L.lib().cfm_cilia_set_s_table(self.mob._h, L.ptr(t.reshape(-1)),
+                                                           rows, cols))
 
     def speeds(self):
         n = self.n_cilia

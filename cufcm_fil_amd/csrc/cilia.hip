@@ -36,13 +36,14 @@ Status download_aos(cfm_ctx* c, double* host, int cnt, int dof, const double* sr
 constexpr int kMaxCiliaSeg = 64;
 
 // Fourier modes x polynomial degree (s, s^2, s^3); x along the filament axis of the
-// reference frame, y along the beat direction.  Values of Ito, Omori & Ishikawa
-// (2019) Table 1 as the reference uses them (rescaled to unit mean length, zero
-// mode halved).
+// reference frame, y along the beat direction; cos0 weights the zero cosine mode.
 struct BeatCoeffs {
   double ax[4][3], ay[4][3], bx[3][3], by[3][3];
+  double cos0;
 };
 
+// Ito, Omori & Ishikawa (2019) Table 1 as SHAPE_SEQUENCE 1 uses it: rescaled to unit
+// mean length, zero mode halved into the table (filament.cpp:684-758).
 static BeatCoeffs fulford_blake() {
   BeatCoeffs b{};
   const double ax[4][3] = {{9.7204e-01, -2.8315e-01, 4.9243e-02},
@@ -63,6 +64,25 @@ static BeatCoeffs fulford_blake() {
   std::memcpy(b.ay, ay, sizeof ay);
   std::memcpy(b.bx, bx, sizeof bx);
   std::memcpy(b.by, by, sizeof by);
+  b.cos0 = 1.0;
+  return b;
+}
+
+// The unaltered Table 1 values (SHAPE_SEQUENCE 4, filament.cpp:813-868) with the zero
+// cosine mode weighted 0.5 at evaluation (filament.cpp:977-979, 1006-1008).
+static BeatCoeffs fulford_blake_original() {
+  BeatCoeffs b{};
+  const double ax[4][3] = {{1.895, -0.552, 0.096}, {-0.018, -0.126, 0.263}, {0.158, -0.341, 0.186},
+                           {0.010, 0.035, -0.067}};
+  const double ay[4][3] = {{-0.654, 0.787, 0.202}, {0.393, -1.516, 0.716}, {-0.097, 0.032, -0.118},
+                           {0.079, -0.302, 0.142}};
+  const double bx[3][3] = {{0.192, -0.499, 0.339}, {-0.050, 0.423, -0.327}, {0.012, 0.138, -0.114}};
+  const double by[3][3] = {{0.284, 1.045, -1.017}, {0.006, 0.317, -0.276}, {-0.059, 0.226, -0.196}};
+  std::memcpy(b.ax, ax, sizeof ax);
+  std::memcpy(b.ay, ay, sizeof ay);
+  std::memcpy(b.bx, bx, sizeof bx);
+  std::memcpy(b.by, by, sizeof by);
+  b.cos0 = 0.5;
   return b;
 }
 
@@ -70,6 +90,9 @@ struct CiliaK {
   BeatCoeffs cf;
   int ncil, nseg, nseg_total;
   double dl, len;  // segment spacing, filament length dl (nseg - 1)
+  double s_tol;    // bisection tolerance on the arc-length fraction
+  const double* s_table;  // recorded segment parameters (rows x cols) or null
+  int s_rows, s_cols;
   const double* base;    // 3 x ncil (SoA)
   const double* phase;   // ncil
   const double* angle;   // ncil (rotation of the beat about the reference z axis)
@@ -79,9 +102,9 @@ struct CiliaK {
   double* svals;         // ncil x nseg
 };
 
-// Mode vectors (1, cos psi, cos 2psi, cos 3psi) and (sin psi, sin 2psi, sin 3psi).
-__device__ inline void modes(double psi, double cv[4], double sv[3]) {
-  cv[0] = 1.0;
+// Mode vectors (cos0, cos psi, cos 2psi, cos 3psi) and (sin psi, sin 2psi, sin 3psi).
+__device__ inline void modes(double psi, double cos0, double cv[4], double sv[3]) {
+  cv[0] = cos0;
   for (int n = 1; n < 4; ++n) {
     cv[n] = cos(n * psi);
     sv[n - 1] = sin(n * psi);
@@ -136,26 +159,48 @@ __global__ void cilia_geometry_kernel(CiliaK k) {
   const int n = k.nseg, s0 = c * n;
   const double psi = k.phase[c];
   double cv[4], sv[3];
-  modes(psi, cv, sv);
-  // segment parameters: bisection on the arc-length fraction
+  modes(psi, k.cf.cos0, cv, sv);
   double s[kMaxCiliaSeg];
   s[0] = 0.0;
   s[n - 1] = 1.0;
-  const double total = curve_length(k.cf, cv, sv, 1.0, n);
-  for (int i = 1; i < n - 1; ++i) {
-    const double target = double(i) / double(n - 1);
-    double lo = s[i - 1], hi = 1.0;
-    double est = 0.5 * (lo + hi);
-    double frac = curve_length(k.cf, cv, sv, est, n) / total;
-    for (int it = 0; it < 200 && fabs(frac - target) > 0.1 / n; ++it) {
-      if (frac > target)
-        hi = est;
-      else
-        lo = est;
-      est = 0.5 * (lo + hi);
-      frac = curve_length(k.cf, cv, sv, est, n) / total;
+  if (k.s_table) {
+    // recorded parameters, linear in phase between the table's rows and linear in
+    // segment position between its columns (filament.cpp:1190-1235)
+    double x = 0.5 * psi / 3.14159265358979323846;
+    x = (x - floor(x)) * k.s_rows;
+    const int lo = (int)x;
+    const double *a = k.s_table, *b = k.s_table;
+    double w = 0.0;
+    if (lo < k.s_rows) {  // lo == s_rows only when rounding lands x on s_rows: row 0
+      a = k.s_table + (size_t)lo * k.s_cols;
+      b = k.s_table + (size_t)(lo == k.s_rows - 1 ? 0 : lo + 1) * k.s_cols;
+      w = x - lo;
     }
-    s[i] = est;
+    for (int i = 1; i < n - 1; ++i) {
+      const double si = (double)(k.s_cols - 1) * i / (double)(n - 1);
+      const int j = (int)si;
+      const double vj = a[j] + (b[j] - a[j]) * w;
+      const double vj1 = a[j + 1] + (b[j + 1] - a[j + 1]) * w;
+      s[i] = vj + (si - j) * (vj1 - vj);
+    }
+  } else {
+    // bisection on the arc-length fraction (filament.cpp:1096-1185)
+    const double total = curve_length(k.cf, cv, sv, 1.0, n);
+    for (int i = 1; i < n - 1; ++i) {
+      const double target = double(i) / double(n - 1);
+      double lo = s[i - 1], hi = 1.0;
+      double est = 0.5 * (lo + hi);
+      double frac = curve_length(k.cf, cv, sv, est, n) / total;
+      for (int it = 0; it < 200 && fabs(frac - target) > k.s_tol; ++it) {
+        if (frac > target)
+          hi = est;
+        else
+          lo = est;
+        est = 0.5 * (lo + hi);
+        frac = curve_length(k.cf, cv, sv, est, n) / total;
+      }
+      s[i] = est;
+    }
   }
   // placement: reference x -> lab z, reference y -> lab y, reference z -> lab -x
   const double bxp = k.base[c], byp = k.base[k.ncil + c], bzp = k.base[2 * k.ncil + c];
@@ -393,13 +438,20 @@ struct CiliaState {
   double spring = 0.0;
   DevBuf<double> omega0, adot, tab_phase, tab_angle, z, zrhs;
   int n_tab_phase = 0, n_tab_angle = 0;
+  // recorded segment parameters (cfm_cilia_set_s_table)
+  DevBuf<double> s_table;
+  int s_rows = 0, s_cols = 0;
 };
 
 static CiliaState* cil_of(cfm_ctx* c) { return reinterpret_cast<CiliaState*>(c->cilia); }
 
 static CiliaK kargs(cfm_ctx* c, CiliaState* S) {
   CiliaK k{};
-  k.cf = fulford_blake();
+  k.cf = S->prm.beat == CFM_BEAT_FULFORD_BLAKE_ORIGINAL ? fulford_blake_original() : fulford_blake();
+  k.s_tol = S->prm.s_tol > 0.0 ? S->prm.s_tol : 0.1 / S->prm.nseg;
+  k.s_table = S->s_rows > 0 ? S->s_table.p : nullptr;
+  k.s_rows = S->s_rows;
+  k.s_cols = S->s_cols;
   k.ncil = S->prm.n_cilia;
   k.nseg = S->prm.nseg;
   k.nseg_total = c->p.n_seg;
@@ -467,7 +519,7 @@ void cfm_cilia_release(cfm_ctx* c) {
   if (!S) return;
   for (auto* b : {&S->base, &S->phase, &S->pdot, &S->angle, &S->vphase, &S->vangle, &S->svals,
                   &S->rhs, &S->f, &S->q, &S->omega0, &S->adot, &S->tab_phase, &S->tab_angle,
-                  &S->z, &S->zrhs})
+                  &S->z, &S->zrhs, &S->s_table})
     b->release();
   if (S->host) (void)hipHostFree(S->host);
   delete S;
@@ -487,8 +539,10 @@ int cfm_cilia_init(cfm_ctx* c, const cfm_cilia_params* prm, const double* base,
   if (!prm || !base || !phase || !phase_dot)
     return cfail(c, make_status(CFM_EINVAL, "null argument"));
   const Particles& p = c->p;
-  if (prm->beat != CFM_BEAT_FULFORD_BLAKE)
+  if (prm->beat != CFM_BEAT_FULFORD_BLAKE && prm->beat != CFM_BEAT_FULFORD_BLAKE_ORIGINAL)
     return cfail(c, make_status(CFM_EINVAL, "unknown beat %d", prm->beat));
+  if (!(prm->s_tol >= 0.0) || prm->s_tol >= 0.5)
+    return cfail(c, make_status(CFM_EINVAL, "s_tol must lie in [0, 0.5) (0: 0.1 / nseg)"));
   if (prm->nseg < 2 || prm->nseg > kMaxCiliaSeg || prm->n_cilia <= 0 ||
       (long long)prm->n_cilia * prm->nseg != p.n_seg)
     return cfail(c, make_status(CFM_EINVAL, "n_cilia x nseg must equal the context's n_seg, 2 <= nseg <= %d",
@@ -655,6 +709,34 @@ int cfm_cilia_set_driving(cfm_ctx* c, const double* q_phase_table, int32_t n_pha
   };
   Status s = run();
   return s.ok() ? CFM_OK : cfail(c, s);
+}
+
+int cfm_cilia_set_s_table(cfm_ctx* c, const double* table, int32_t rows, int32_t cols) {
+  if (!c) return CFM_EINVAL;
+  CiliaState* S = cil_of(c);
+  if (!S) return cfail(c, make_status(CFM_ESTATE, "cfm_cilia_init has not been called"));
+  (void)hipSetDevice(c->device);
+  if (!table) {  // back to the bisection
+    S->s_rows = S->s_cols = 0;
+  } else {
+    if (rows <= 0 || cols < 2)
+      return cfail(c, make_status(CFM_EINVAL, "s table needs rows >= 1 and cols >= 2"));
+    for (long long i = 0; i < (long long)rows * cols; ++i)
+      if (!std::isfinite(table[i])) return cfail(c, make_status(CFM_ENONFINITE, "non-finite s table"));
+    auto run = [&]() -> Status {
+      CFM_TRY(S->s_table.alloc((size_t)rows * cols));
+      CFM_HIP(hipMemcpyAsync(S->s_table.p, table, sizeof(double) * rows * cols, hipMemcpyHostToDevice,
+                             c->stream));
+      CFM_HIP(hipStreamSynchronize(c->stream));
+      return {};
+    };
+    Status st = run();
+    if (!st.ok()) return cfail(c, st);
+    S->s_rows = rows;
+    S->s_cols = cols;
+  }
+  Status st = geometry(c, S);  // re-place the segments with the new parameters
+  return st.ok() ? CFM_OK : cfail(c, st);
 }
 
 int cfm_cilia_get_speeds(cfm_ctx* c, double* phase, double* phase_dot, double* angle,

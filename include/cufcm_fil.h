@@ -288,6 +288,8 @@ int cfm_gemm_bf16(cfm_ctx* ctx, int32_t M, int32_t N, int32_t K, const float* A,
  * seg_dof = 3, no blobs and n_seg = n_cilia * nseg (cilium c owns segments
  * [c nseg, (c+1) nseg)). */
 #define CFM_BEAT_FULFORD_BLAKE 1   /* Ito, Omori & Ishikawa (2019) fit, filament.cpp:684-758 */
+#define CFM_BEAT_FULFORD_BLAKE_ORIGINAL 2 /* the fit's unaltered coefficients, zero cosine mode
+                                             weighted 0.5 (SHAPE_SEQUENCE 4, filament.cpp:813-868,977-1008) */
 
 typedef struct {
   int32_t n_cilia;
@@ -296,6 +298,8 @@ typedef struct {
   int32_t reserved;
   double dl;             /* segment spacing (filament length dl (nseg - 1)) */
   double dt;             /* time step: phase += phase_dot dt at every step after the first */
+  double s_tol;          /* bisection tolerance on the arc-length fraction of the segment
+                            parameters (0: 0.1 / nseg, filament.cpp:1152) */
   cfm_solve_opts solve;  /* segment solve: precond NONE/DIAG/SELF, mode, rtol, ... */
 } cfm_cilia_params;
 
@@ -320,6 +324,12 @@ int cfm_cilia_step(cfm_ctx* ctx, int32_t nt, double* q_phase, double* q_angle,
 int cfm_cilia_set_driving(cfm_ctx* ctx, const double* q_phase_table, int32_t n_phase,
                           const double* q_angle_table /* may be null */, int32_t n_angle,
                           double spring);
+/* Segment parameters from a recorded table instead of the bisection, as the reference
+ * does whenever it does not write its reference files (filament.cpp:1190-1235; the table
+ * is the s_values file of input/forcing, read at cilia_sim_main.cu:339-378): rows at phases
+ * 2 pi k / rows, cols recorded segments (>= 2), linear in phase and in segment position.
+ * A null table returns to the bisection.  Re-places the segments. */
+int cfm_cilia_set_s_table(cfm_ctx* ctx, const double* table, int32_t rows, int32_t cols);
 /* Phases, phase speeds, shape angles and their speeds (n_cilia each, may be null). */
 int cfm_cilia_get_speeds(cfm_ctx* ctx, double* phase, double* phase_dot, double* angle,
                          double* angle_dot);

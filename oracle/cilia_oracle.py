@@ -20,6 +20,9 @@ Follows the reference's fitted-beat cilium (CILIA_TYPE 3, SHAPE_SEQUENCE 1):
 
 Parity status: pinned by the reference's own output files
 input/forcing/fulford_and_blake_reference_{phase,angle}_generalised_forces_NSEG=20_SEP=2.600000.dat
+and, for the unaltered coefficients (SHAPE_SEQUENCE 4),
+input/forcing/fulford_and_blake_original_reference_{phase,angle}_generalised_forces_...dat
+and its recorded segment parameters ..._s_values_NSEG=20_SEP=2.600000.dat
 (copied under tests/golden/) -- both directly and in closed loop (driving a cilium
 with its own recorded forces returns its beat speed) -- by the arc-length property of
 the s-values, by finite differences of the shape in the phase, and by node, periodicity
@@ -49,75 +52,121 @@ BY = np.array([[2.9136e-01, 1.0721e+00, -1.0433e+00],
                [6.1554e-03, 3.2521e-01, -2.8315e-01],
                [-6.0528e-02, 2.3185e-01, -2.0108e-01]])
 
+# The unaltered Table 1 values (SHAPE_SEQUENCE 4, filament.cpp:813-868): the zero
+# cosine mode enters with weight 0.5 (filament.cpp:977-979, 1006-1008) and the mean
+# length is 0.97475 rather than 1.
+AX0 = np.array([[1.895, -0.552, 0.096], [-0.018, -0.126, 0.263], [0.158, -0.341, 0.186],
+                [0.010, 0.035, -0.067]])
+AY0 = np.array([[-0.654, 0.787, 0.202], [0.393, -1.516, 0.716], [-0.097, 0.032, -0.118],
+                [0.079, -0.302, 0.142]])
+BX0 = np.array([[0.192, -0.499, 0.339], [-0.050, 0.423, -0.327], [0.012, 0.138, -0.114]])
+BY0 = np.array([[0.284, 1.045, -1.017], [0.006, 0.317, -0.276], [-0.059, 0.226, -0.196]])
 
-def _modes(psi):
-    cv = np.array([1.0] + [math.cos(n * psi) for n in range(1, 4)])
+# beat name -> (Ax, Ay, Bx, By, weight of the zero cosine mode)
+BEATS = {"fulford_blake": (AX, AY, BX, BY, 1.0),
+         "fulford_blake_original": (AX0, AY0, BX0, BY0, 0.5)}
+
+
+def _modes(psi, c0=1.0):
+    cv = np.array([c0] + [math.cos(n * psi) for n in range(1, 4)])
     sv = np.array([math.sin(n * psi) for n in range(1, 4)])
     return cv, sv
 
 
-def tangent(s, psi):
+def tangent(s, psi, beat="fulford_blake"):
     """d shape / d s in the beat plane (filament.cpp:964-988)."""
-    cv, sv = _modes(psi)
+    ax, ay, bx, by, c0 = BEATS[beat]
+    cv, sv = _modes(psi, c0)
     p = np.array([1.0, 2.0 * s, 3.0 * s * s])
-    return (cv @ AX + sv @ BX) @ p, (cv @ AY + sv @ BY) @ p
+    return (cv @ ax + sv @ bx) @ p, (cv @ ay + sv @ by) @ p
 
 
-def shape(s, psi):
-    """Reference-frame position at parameter s, unit length (filament.cpp:990-1019)."""
-    cv, sv = _modes(psi)
+def shape(s, psi, beat="fulford_blake"):
+    """Reference-frame position at parameter s (filament.cpp:990-1019)."""
+    ax, ay, bx, by, c0 = BEATS[beat]
+    cv, sv = _modes(psi, c0)
     p = np.array([s, s * s, s ** 3])
-    return np.array([(cv @ AX + sv @ BX) @ p, (cv @ AY + sv @ BY) @ p, 0.0])
+    return np.array([(cv @ ax + sv @ bx) @ p, (cv @ ay + sv @ by) @ p, 0.0])
 
 
-def shape_phase_derivative(s, psi):
+def shape_phase_derivative(s, psi, beat="fulford_blake"):
     """d shape / d psi (filament.cpp:1021-1045)."""
+    ax, ay, bx, by, _ = BEATS[beat]
     dc = np.array([0.0] + [-n * math.sin(n * psi) for n in range(1, 4)])
     ds = np.array([n * math.cos(n * psi) for n in range(1, 4)])
     p = np.array([s, s * s, s ** 3])
-    return np.array([(dc @ AX + ds @ BX) @ p, (dc @ AY + ds @ BY) @ p, 0.0])
+    return np.array([(dc @ ax + ds @ bx) @ p, (dc @ ay + ds @ by) @ p, 0.0])
 
 
-def curve_length(s, psi, nseg=20):
+def curve_length(s, psi, nseg=20, beat="fulford_blake"):
     """Trapezoid arc length from 0 to s on 10 nseg panels, the last panel
     interpolated (filament.cpp:1047-1094)."""
     if s <= 0.0:
         return 0.0
     dl = 1.0 / (10 * nseg)
-    f0 = math.hypot(*tangent(0.0, psi))
+    f0 = math.hypot(*tangent(0.0, psi, beat))
     fl = math.floor(s / dl)
     length = 0.0
     for n in range(1, int(fl) + 1):
-        f1 = math.hypot(*tangent(n * dl, psi))
+        f1 = math.hypot(*tangent(n * dl, psi, beat))
         length += 0.5 * dl * (f0 + f1)
         f0 = f1
     ce = math.ceil(s / dl)
     if ce > fl:
-        f1 = math.hypot(*tangent(ce * dl, psi))
+        f1 = math.hypot(*tangent(ce * dl, psi, beat))
         length += ((s / dl - fl) / (ce - fl)) * 0.5 * dl * (f0 + f1)
     return length
 
 
-def s_values(psi, nseg=20):
-    """Segment parameters by bisection on the arc-length fraction
-    (filament.cpp:1096-1160)."""
+def s_values(psi, nseg=20, beat="fulford_blake", tol=None, margins=False):
+    """Segment parameters by bisection on the arc-length fraction to within tol
+    (default 0.1/nseg) (filament.cpp:1096-1160).  With margins=True also returns, per
+    segment, how close any of its bisection comparisons came to flipping (the smaller
+    of | |frac - target| - tol | and |frac - target|)."""
+    tol = 0.1 / nseg if tol is None else tol
     s = [0.0] * nseg
     s[-1] = 1.0
-    total = curve_length(1.0, psi, nseg)
+    m = np.full(nseg, np.inf)
+    total = curve_length(1.0, psi, nseg, beat)
     for n in range(1, nseg - 1):
         target = n / (nseg - 1)
         lo, hi = s[n - 1], 1.0
         est = 0.5 * (lo + hi)
-        frac = curve_length(est, psi, nseg) / total
-        while abs(frac - target) > 0.1 / nseg:
+        frac = curve_length(est, psi, nseg, beat) / total
+        m[n] = min(abs(abs(frac - target) - tol), abs(frac - target))
+        while abs(frac - target) > tol:
             if frac > target:
                 hi = est
             else:
                 lo = est
             est = 0.5 * (lo + hi)
-            frac = curve_length(est, psi, nseg) / total
+            frac = curve_length(est, psi, nseg, beat) / total
+            m[n] = min(m[n], abs(abs(frac - target) - tol), abs(frac - target))
         s[n] = est
-    return np.array(s)
+    return (np.array(s), m) if margins else np.array(s)
+
+
+def s_values_from_table(table, psi, nseg):
+    """Segment parameters read from a recorded table (rows: phases 2 pi k / rows,
+    columns: the recorded segments), bilinear in phase and in segment position
+    (filament.cpp:1190-1235)."""
+    table = np.asarray(table, dtype=np.float64)
+    rows, cols = table.shape
+    x = 0.5 * psi / math.pi
+    x = (x - math.floor(x)) * rows
+    lo = int(x)
+    if lo == rows:
+        svec = table[0]
+    else:
+        up = table[0] if lo == rows - 1 else table[lo + 1]
+        svec = table[lo] + (up - table[lo]) * (x - lo)
+    s = np.zeros(nseg)
+    s[-1] = 1.0
+    for n in range(1, nseg - 1):
+        si = (cols - 1) * n / (nseg - 1)
+        k = int(si)
+        s[n] = svec[k] + (si - k) * (svec[k + 1] - svec[k])
+    return s
 
 
 # Rotation taking the reference x axis to the wall normal e_z: the square root of
@@ -127,10 +176,11 @@ FRAME = np.array([[0.0, 0.0, -1.0],
                   [1.0, 0.0, 0.0]])
 
 
-def placement(psi, base, nseg=20, dl=2.6, angle=0.0):
-    """Positions, phase and angle velocity directions of one cilium (AoS, 3 nseg)."""
+def placement(psi, base, nseg=20, dl=2.6, angle=0.0, beat="fulford_blake", s=None):
+    """Positions, phase and angle velocity directions of one cilium (AoS, 3 nseg);
+    segment parameters by bisection unless given."""
     L = dl * (nseg - 1)
-    s = s_values(psi, nseg)
+    s = s_values(psi, nseg, beat) if s is None else np.asarray(s)
     ca, sa = math.cos(angle), math.sin(angle)
     Rshape = np.array([[ca, -sa, 0.0], [sa, ca, 0.0], [0.0, 0.0, 1.0]])
     base = np.asarray(base, dtype=np.float64)
@@ -139,18 +189,18 @@ def placement(psi, base, nseg=20, dl=2.6, angle=0.0):
     vangle = np.zeros((nseg, 3))
     x[0] = base
     for n in range(1, nseg):
-        ref = L * (Rshape @ shape(s[n], psi))
+        ref = L * (Rshape @ shape(s[n], psi, beat))
         x[n] = base + FRAME @ ref
         vangle[n] = FRAME @ np.array([-ref[1], ref[0], 0.0])   # e_z x ref
-        vphase[n] = FRAME @ (L * (Rshape @ shape_phase_derivative(s[n], psi)))
+        vphase[n] = FRAME @ (L * (Rshape @ shape_phase_derivative(s[n], psi, beat)))
     return x.reshape(-1), vphase.reshape(-1), vangle.reshape(-1), s
 
 
 def generalised_forces(mobility, psi, phase_dot=2.0 * math.pi, base=(0.0, 0.0, 1.3), nseg=20,
-                       dl=2.6):
+                       dl=2.6, beat="fulford_blake"):
     """Phase and angle generalised forces of one cilium: F = M^-1 (phase_dot vel_dir_phase)
     with M = mobility(x) the 3-DOF wall mobility matrix of its segments."""
-    x, vp, va, _ = placement(psi, base, nseg, dl)
+    x, vp, va, _ = placement(psi, base, nseg, dl, beat=beat)
     f = np.linalg.solve(mobility(x), phase_dot * vp)
     return float(vp @ f), float(va @ f)
 

@@ -111,3 +111,66 @@ def test_driving_a_cilium_with_its_own_forces_recovers_the_beat_speed(with_angle
         assert phid[0] == pytest.approx(2 * math.pi, rel=2e-4)
         if with_angle:
             assert abs(psid[0]) <= 2e-4 * 2 * math.pi
+
+
+def _s_table():
+    v = np.loadtxt(os.path.join(GOLD, "fb_original_s_values_nseg20_sep2.6.dat"))
+    assert int(v[0]) == 300 and int(v[1]) == 20 and v.size == 2 + 300 * 20
+    return v[2:].reshape(300, 20)
+
+
+def test_original_beat_s_values_match_reference_file():
+    """The reference's recorded segment parameters of the unaltered coefficient set
+    (SHAPE_SEQUENCE 4) at sampled steps, with the bisection tolerance 0.001/NSEG the
+    file was written with: within 1e-6 (six printed digits plus the single-precision
+    rounding of the reference's midpoints; one bisection decision apart is 1.5e-5 or
+    more) except where the reference's single-precision arithmetic took the other side
+    of a comparison that FP64 puts within 1e-6 of its threshold (steps 20 and 68 here:
+    9 of all 300 steps); the present source's 0.1/NSEG stops earlier (several 1e-3)."""
+    tab = _s_table()
+    flips = 0
+    for k in list(range(0, 300, 23)) + [20, 68]:
+        psi = 2.0 * math.pi * k / 300
+        s, m = co.s_values(psi, 20, "fulford_blake_original", tol=0.001 / 20, margins=True)
+        d = np.abs(s - tab[k])
+        if d.max() > 1e-6:
+            first = int(np.argmax(d > 1e-6))
+            assert m[first] < 1e-6 and d[first] > 1e-5, (k, first, m[first])
+            flips += 1
+    assert flips == 2
+    coarse = co.s_values(0.0, 20, "fulford_blake_original")
+    assert np.abs(coarse - tab[0]).max() > 1e-3
+
+
+def test_rescaled_table_is_the_original_over_its_mean_length():
+    """filament.cpp:684-758 divides Table 1 by 0.97475 and halves the zero cosine mode
+    into the table; both transcriptions agree to the five digits they carry."""
+    w = np.ones((4, 1))
+    w[0] = 0.5
+    for a, a0 in [(co.AX, co.AX0), (co.AY, co.AY0)]:
+        assert np.allclose(a, a0 * w / 0.97475, rtol=6e-3, atol=2e-4)
+    for b, b0 in [(co.BX, co.BX0), (co.BY, co.BY0)]:
+        assert np.allclose(b, b0 / 0.97475, rtol=6e-3, atol=2e-4)
+    # same shape everywhere up to that rescaling
+    for psi in (0.3, 2.0, 4.4):
+        for s in (0.2, 0.7, 1.0):
+            assert np.allclose(co.shape(s, psi), co.shape(s, psi, "fulford_blake_original") / 0.97475,
+                               atol=2e-3)
+
+
+def test_s_table_interpolation_rule():
+    """filament.cpp:1190-1235: rows at phase 2 pi k / rows (periodic), columns at the
+    recorded segments; a table with as many columns as segments returns its rows at the
+    sampled phases, their mean half way between, and row 0 after the last row."""
+    tab = _s_table()
+    for k in (0, 17, 299):
+        assert np.allclose(co.s_values_from_table(tab, 2 * math.pi * k / 300, 20), tab[k], atol=1e-12)
+    mid = co.s_values_from_table(tab, 2 * math.pi * 10.5 / 300, 20)
+    assert np.allclose(mid, 0.5 * (tab[10] + tab[11]), atol=1e-12)
+    wrap = co.s_values_from_table(tab, 2 * math.pi * 299.5 / 300, 20)
+    assert np.allclose(wrap, 0.5 * (tab[299] + tab[0]), atol=1e-12)
+    # a 39-column table (every other column a midpoint) gives back the even columns
+    fine = np.zeros((300, 39))
+    fine[:, ::2] = tab
+    fine[:, 1::2] = 0.5 * (tab[:, :-1] + tab[:, 1:])
+    assert np.allclose(co.s_values_from_table(fine, 2 * math.pi * 5 / 300, 20), tab[5], atol=1e-12)

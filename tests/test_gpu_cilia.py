@@ -142,3 +142,61 @@ def test_coupled_lattice_matches_dense_solve():
     assert np.abs(state["f"] - F).max() <= 1e-6 * np.abs(F).max()
     assert np.allclose(sp["phase_dot"], phid, rtol=1e-6, atol=0)
     assert np.allclose(sp["angle_dot"], psid, rtol=0, atol=1e-6 * np.abs(phid).max())
+
+
+def _s_table():
+    v = np.loadtxt(os.path.join(GOLD, "fb_original_s_values_nseg20_sep2.6.dat"))
+    return v[2:].reshape(300, 20)
+
+
+def test_original_beat_s_values_match_reference_file_every_step():
+    """The reference's recorded segment parameters of the unaltered coefficients
+    (SHAPE_SEQUENCE 4), all 300 steps, with the bisection tolerance 0.001/NSEG the file
+    was written with: every step equals the FP64 oracle's bisection exactly, and the
+    file within 1e-6 (six printed digits plus the single-precision rounding of the
+    reference's midpoints; one decision apart is 1.5e-5 or more) except at steps where
+    the reference's single-precision arithmetic took the other side of a comparison
+    that FP64 puts within 1e-6 of its threshold (9 of 300)."""
+    tab = _s_table()
+    flips = []
+    with PrescribedCilia([0.0, 0.0, 1.3], beat="fulford_blake_original", s_tol=0.001 / 20) as C:
+        for nt in range(300):
+            C.step(nt)
+            st = C.state()
+            d = np.abs(st["s_values"][0] - tab[nt])
+            if d.max() > 1e-6 or nt % 60 == 0:
+                psi = 2.0 * math.pi * nt / 300
+                s, m = co.s_values(psi, 20, "fulford_blake_original", 0.001 / 20, margins=True)
+                assert np.array_equal(st["s_values"][0], s), nt
+                if d.max() > 1e-6:
+                    first = int(np.argmax(d > 1e-6))
+                    assert m[first] < 1e-6 and d[first] > 1e-5, (nt, first, m[first])
+                    flips.append(nt)
+                x, vp, va, _ = co.placement(psi, (0.0, 0.0, 1.3), beat="fulford_blake_original", s=s)
+                assert np.abs(st["x"] - x).max() <= 1e-12 * np.abs(x).max()
+                assert np.abs(st["vel_dir_phase"] - vp).max() <= 1e-12 * np.abs(vp).max()
+    assert len(flips) <= 9, flips
+
+
+def test_s_table_mode_interpolates_the_recorded_parameters():
+    """With the recorded table loaded (the reference's mode whenever it does not write
+    its reference files) the parameters at the sampled phases are the table's rows and
+    in between follow the oracle's interpolation; clearing it returns to bisection."""
+    tab = _s_table()
+    phases = [2.0 * math.pi * 37 / 300, 1.234, 2.0 * math.pi * 299.5 / 300]
+    with PrescribedCilia([0.0, 0.0, 1.3, 40.0, 0.0, 1.3, 0.0, 40.0, 1.3], phase=phases,
+                         beat="fulford_blake_original") as C:
+        C.set_s_table(tab)
+        st = C.state()
+        assert np.abs(st["s_values"][0] - tab[37]).max() <= 1e-12
+        for c, psi in enumerate(phases):
+            ref = co.s_values_from_table(tab, psi, 20)
+            assert np.abs(st["s_values"][c] - ref).max() <= 1e-14, c
+            x, _, _, _ = co.placement(psi, (40.0 * (c == 1), 40.0 * (c == 2), 1.3),
+                                      beat="fulford_blake_original", s=ref)
+            assert np.abs(st["x"][60 * c:60 * c + 60] - x).max() <= 1e-12 * np.abs(x).max()
+        _, _, sst = C.step(0)
+        assert sst["converged"]
+        C.set_s_table(None)
+        s_bis = C.state()["s_values"][1]
+    assert np.array_equal(s_bis, co.s_values(1.234, 20, "fulford_blake_original"))
