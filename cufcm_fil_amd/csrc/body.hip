@@ -28,6 +28,7 @@
 #include <new>
 #include <vector>
 
+#include "body.hpp"
 #include "ctx.hpp"
 #include "densepc.hpp"
 #include "krylov.hpp"
@@ -55,6 +56,8 @@ struct BodyState {
   DevBuf<double> PK;        // P K, 6 n_body columns of 3 n_blob
   DevBuf<double> Sinv;      // (K^T P K)^-1, row-major
   int schur_gen = -1;       // dense-inverse generation PK and Sinv belong to
+  DevBuf<double> ref_dev;   // ref on the device (placement from device-held body states)
+  DevBuf<double> frame;     // body-frame (K^T K)^-1 and N on the device (72)
 };
 
 BodyState* body_of(cfm_ctx* c) { return reinterpret_cast<BodyState*>(c->body); }
@@ -278,6 +281,49 @@ __global__ __launch_bounds__(256) void body_dense_finish_kernel(int n3, int ncol
   out[j] = s;
 }
 
+// Blob positions and rotated offsets from body centres / orientations on the device.
+__global__ __launch_bounds__(256) void body_place_kernel(int m, int nb, const double* __restrict__ ref,
+                                                         const double* __restrict__ bx,
+                                                         const double* __restrict__ bq,
+                                                         double* __restrict__ r, double* __restrict__ px,
+                                                         double* __restrict__ py, double* __restrict__ pz) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const int b = i / nb, k = i - b * nb;
+  const double s = bq[4 * b], a = bq[4 * b + 1], bb = bq[4 * b + 2], cc = bq[4 * b + 3];
+  const double R[9] = {1.0 - 2.0 * (bb * bb + cc * cc), 2.0 * (a * bb - s * cc), 2.0 * (a * cc + s * bb),
+                       2.0 * (a * bb + s * cc), 1.0 - 2.0 * (a * a + cc * cc), 2.0 * (bb * cc - s * a),
+                       2.0 * (a * cc - s * bb), 2.0 * (bb * cc + s * a), 1.0 - 2.0 * (a * a + bb * bb)};
+  const double v0 = ref[3 * k], v1 = ref[3 * k + 1], v2 = ref[3 * k + 2];
+  const double o0 = R[0] * v0 + R[1] * v1 + R[2] * v2;
+  const double o1 = R[3] * v0 + R[4] * v1 + R[5] * v2;
+  const double o2 = R[6] * v0 + R[7] * v1 + R[8] * v2;
+  r[i] = o0;
+  r[m + i] = o1;
+  r[2 * (size_t)m + i] = o2;
+  px[i] = bx[3 * b] + o0;
+  py[i] = bx[3 * b + 1] + o1;
+  pz[i] = bx[3 * b + 2] + o2;
+}
+
+// Per body: each 3 x 3 block of the body-frame matrices conjugated by R(q_b).
+__global__ void body_mats_kernel(int nbody, const double* __restrict__ frame,
+                                 const double* __restrict__ bq, double* __restrict__ mats) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nbody * 72) return;
+  const int b = t / 72, e = t - 72 * b, mat = e / 36, ij = e - 36 * mat;
+  const int i = ij / 6, j = ij - 6 * i, bi = i / 3 * 3, bj = j / 3 * 3;
+  const double s = bq[4 * b], a = bq[4 * b + 1], bb = bq[4 * b + 2], cc = bq[4 * b + 3];
+  const double R[9] = {1.0 - 2.0 * (bb * bb + cc * cc), 2.0 * (a * bb - s * cc), 2.0 * (a * cc + s * bb),
+                       2.0 * (a * bb + s * cc), 1.0 - 2.0 * (a * a + cc * cc), 2.0 * (bb * cc - s * a),
+                       2.0 * (a * cc - s * bb), 2.0 * (bb * cc + s * a), 1.0 - 2.0 * (a * a + bb * bb)};
+  const double* A = frame + 36 * mat;
+  double v = 0.0;
+  for (int k = 0; k < 3; ++k)
+    for (int l = 0; l < 3; ++l) v += R[3 * (i - bi) + k] * A[(bi + k) * 6 + bj + l] * R[3 * (j - bj) + l];
+  mats[72 * (size_t)b + e] = v;
+}
+
 Status upload_mats(cfm_ctx* c, BodyState* S) {
   std::vector<double> h((size_t)72 * S->n_body);
   for (int b = 0; b < S->n_body; ++b) {
@@ -288,6 +334,11 @@ Status upload_mats(cfm_ctx* c, BodyState* S) {
   }
   CFM_TRY(S->mats.alloc(h.size()));
   CFM_HIP(hipMemcpyAsync(S->mats.p, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice, c->stream));
+  double fr[72];
+  std::copy(S->ktk_inv, S->ktk_inv + 36, fr);
+  std::copy(S->mob, S->mob + 36, fr + 36);
+  CFM_TRY(S->frame.alloc(72));
+  CFM_HIP(hipMemcpyAsync(S->frame.p, fr, sizeof(fr), hipMemcpyHostToDevice, c->stream));
   CFM_HIP(hipStreamSynchronize(c->stream));
   return {};
 }
@@ -333,7 +384,8 @@ Status build_schur(cfm_ctx* c, BodyState* S) {
 }
 
 void body_release_state(BodyState* S) {
-  for (auto* b : {&S->r, &S->mats, &S->w, &S->aux, &S->kcol, &S->PK, &S->Sinv}) b->release();
+  for (auto* b : {&S->r, &S->mats, &S->w, &S->aux, &S->kcol, &S->PK, &S->Sinv, &S->ref_dev, &S->frame})
+    b->release();
 }
 
 }  // namespace
@@ -394,6 +446,47 @@ Status body_precond(cfm_ctx* c, int kind, const double* x, double* y) {
     return {};
   }
   return make_status(CFM_EINVAL, "the body system takes CFM_PC_NONE, _DIAG, _SELF or _DENSE");
+}
+
+Status body_info(cfm_ctx* c, BodyInfo* out) {
+  BodyState* S = body_of(c);
+  if (!S) return make_status(CFM_ESTATE, "cfm_body_init has not been called");
+  if (!S->placed) return make_status(CFM_ESTATE, "body state not set (cfm_body_set_state)");
+  out->n_body = S->n_body;
+  out->nb = S->nb;
+  out->x = S->x;
+  out->q = S->q;
+  std::copy(S->mob, S->mob + 36, out->mob);
+  return {};
+}
+
+const double* body_offsets(cfm_ctx* c) {
+  BodyState* S = body_of(c);
+  return S ? S->r.p : nullptr;
+}
+
+Status body_place_device(cfm_ctx* c, const double* bx, const double* bq) {
+  BodyState* S = body_of(c);
+  if (!S) return make_status(CFM_ESTATE, "cfm_body_init has not been called");
+  const int m = c->p.n_blob, off = c->p.n_seg;
+  if (!S->ref_dev.p) {
+    CFM_TRY(S->ref_dev.alloc(S->ref.size()));
+    CFM_HIP(hipMemcpyAsync(S->ref_dev.p, S->ref.data(), sizeof(double) * S->ref.size(),
+                           hipMemcpyHostToDevice, c->stream));
+  }
+  CFM_TRY(S->r.alloc(3 * (size_t)m));
+  hipLaunchKernelGGL(body_place_kernel, g1(m), dim3(256), 0, c->stream, m, S->nb, S->ref_dev.p, bx, bq,
+                     S->r.p, c->p.x.p + off, c->p.y.p + off, c->p.z.p + off);
+  hipLaunchKernelGGL(body_mats_kernel, g1(72 * S->n_body), dim3(256), 0, c->stream, S->n_body, S->frame.p,
+                     bq, S->mats.p);
+  CFM_CHECK_LAUNCH();
+  // cached blob-position products no longer apply
+  c->blob_pos.clear();
+  c->nf.valid = false;
+  c->dinv.valid = false;
+  S->schur_gen = -1;
+  S->placed = true;
+  return {};
 }
 
 // Blob positions changed behind the bodies' back (cfm_set_positions): the body

@@ -1,0 +1,22 @@
+// body.hpp -- what the implicit step needs from the free rigid bodies (body.hip).
+#pragma once
+#include <vector>
+
+#include "ctx.hpp"
+
+namespace cfm {
+
+struct BodyInfo {
+  int n_body = 0, nb = 0;    // bodies, blobs per body
+  std::vector<double> x, q;  // centres (3 n_body), quaternions (4 n_body) of the last placement
+  double mob[36];            // body-frame body mobility (identity until made or set)
+};
+
+Status body_info(cfm_ctx* c, BodyInfo* out);
+// Places every blob from body centres bx (3 n_body) and quaternions bq (4 n_body) held in
+// device memory, with the rotated reference matrices; invalidates caches of blob products.
+Status body_place_device(cfm_ctx* c, const double* bx, const double* bq);
+// Rotated blob offsets R(q_b) ref of the last placement ([3][n_blob], device), or null.
+const double* body_offsets(cfm_ctx* c);
+
+}  // namespace cfm

@@ -44,6 +44,22 @@ __device__ __forceinline__ void st3(double* a, size_t n, int s, V3 v) {
   a[2 * n + s] = v.z;
 }
 
+// accept_state_from_rigid_body (filament.cpp:653-678): the first segment of filament f
+// at its body's base point, turned by the body's Lie-algebra element from its previous
+// orientation.
+__device__ void attach_to_body(const FilK& K, int f) {
+  const size_t n = K.nseg_total;
+  const int s0 = f * K.nseg, b = f / K.fpb;
+  const Q4 qb = {K.bq[4 * b], K.bq[4 * b + 1], K.bq[4 * b + 2], K.bq[4 * b + 3]};
+  const V3 r = mv(rot_mat(qb), v3(K.fref[3 * f], K.fref[3 * f + 1], K.fref[3 * f + 2]));
+  K.x[s0] = K.bx[3 * b] + r.x;
+  K.y[s0] = K.bx[3 * b + 1] + r.y;
+  K.z[s0] = K.bx[3 * b + 2] + r.z;
+  const V3 ub = {K.bu[3 * b], K.bu[3 * b + 1], K.bu[3 * b + 2]};
+  st3(K.u, n, s0, ub);
+  stq(K.q, n, s0, qmul(lie_exp(ub), ldq(K.qm1, n, s0)));
+}
+
 // robot arm from the base segment of filament f
 __device__ void robot_arm(const FilK& K, int f) {
   const size_t n = K.nseg_total;
@@ -105,12 +121,15 @@ __global__ void fil_initial_guess_kernel(FilK K, int nt) {
   };
   for (int i = 0; i < K.nseg; ++i) extrap(K.lam, K.lam_m1, K.lam_m2, n, s0 + i);
   extrap(K.clamp, K.clamp_m1, K.clamp_m2, (size_t)K.nfil, f);
-  // base follows the (fixed) anchor: position reset, Lie algebra element zero
-  K.x[s0] = K.base[f];
-  K.y[s0] = K.base[K.nfil + f];
-  K.z[s0] = K.base[2 * K.nfil + f];
-  st3(K.u, n, s0, v3(0.0, 0.0, 0.0));
-  stq(K.q, n, s0, ldq(K.qm1, n, s0));
+  if (K.bx) {
+    attach_to_body(K, f);
+  } else {  // base follows the (fixed) anchor: position reset, Lie algebra element zero
+    K.x[s0] = K.base[f];
+    K.y[s0] = K.base[K.nfil + f];
+    K.z[s0] = K.base[2 * K.nfil + f];
+    st3(K.u, n, s0, v3(0.0, 0.0, 0.0));
+    stq(K.q, n, s0, ldq(K.qm1, n, s0));
+  }
   robot_arm(K, f);
 }
 
@@ -402,6 +421,12 @@ __global__ __launch_bounds__(256) void fil_jacobian_kernel(FilK K, int nt) {
     D = diag_block(mscale(madd(Eld, Cp), Tfac));
     D = madd(D, eye3());
     const int r0 = 3 * (n + k);
+    if (K.clampblk && k <= 1) {  // how rows 0 and 1 depend on u_0 (elastic_clamping_block1/2)
+      const M3& E = k == 0 ? D : Lb;
+      double* cb = K.clampblk + (size_t)f * 18 + 9 * k;
+      for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) cb[3 * a + b] = E.a[a][b];
+    }
     if (k == 0) {
       // clamped base: the clamp multiplier replaces the base Lie-algebra unknowns
       const double fac3 = euler ? -K.dt / (8.0 * pi * K.mu * a3) : -K.dt / (12.0 * pi * K.mu * a3);
@@ -451,6 +476,7 @@ __global__ void fil_update_kernel(FilK K, const double* __restrict__ upd) {
   const size_t ns = K.nseg_total;
   const int n = K.nseg, s0 = f * n;
   const double* u = upd + (size_t)f * 6 * n;
+  if (K.bx) attach_to_body(K, f);  // the body has moved first (swimmer.cpp:1076-1100)
   for (int i = 0; i < n; ++i) {
     const int s = s0 + i;
     st3(K.lam, ns, s, ld3(K.lam, ns, s) + v3(u[3 * i], u[3 * i + 1], u[3 * i + 2]));
@@ -637,6 +663,55 @@ __global__ __launch_bounds__(256) void barrier_kernel(FilK K, GridPos P, double 
   fout[2 * (size_t)ld + i] += fz;
 }
 
+// Repulsion on blob ib from every segment and every blob of another body within
+// 1.1 x 2a (cuda_functions.cu:1932-2012; the pairs of one body's own blobs cancel in
+// its total force and torque, which is all the body balance uses, and are skipped).
+__global__ __launch_bounds__(256) void blob_barrier_kernel(FilK K, GridPos P, double inv_h,
+                                                           int mask, const int* __restrict__ start,
+                                                           const int* __restrict__ ids,
+                                                           int nb_per_body,
+                                                           double* __restrict__ rep) {
+  const int ib = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nblob = P.n - P.ns;
+  if (ib >= nblob) return;
+  const double xi = P.bx[ib], yi = P.by[ib], zi = P.bz[ib];
+  const int bi = ib / nb_per_body;
+  const double fmag = K.rep_factor * K.end_force;
+  const double a_sum = 2.0 * K.a;
+  const double cut = 1.1 * a_sum, chi = 10.0 / a_sum;
+  double fx = 0.0, fy = 0.0, fz = 0.0;
+  const int cx = cell_coord(xi, inv_h), cy = cell_coord(yi, inv_h), cz = cell_coord(zi, inv_h);
+  for (int dz = -1; dz <= 1; ++dz)
+    for (int dy = -1; dy <= 1; ++dy)
+      for (int dx = -1; dx <= 1; ++dx) {
+        const int nx = cx + dx, ny = cy + dy, nz = cz + dz;
+        const int b = cell_bucket(nx, ny, nz, mask);
+        const int t1 = start[b + 1];
+        for (int t = start[b]; t < t1; ++t) {
+          const int j = ids[t];
+          if (j >= P.ns && (j - P.ns) / nb_per_body == bi) continue;
+          double xj, yj, zj;
+          P.get(j, xj, yj, zj);
+          if (cell_coord(xj, inv_h) != nx || cell_coord(yj, inv_h) != ny ||
+              cell_coord(zj, inv_h) != nz)
+            continue;
+          const double ex = xi - xj, ey = yi - yj, ez = zi - zj;
+          const double d2 = ex * ex + ey * ey + ez * ez;
+          if (d2 < cut * cut) {
+            const double dist = sqrt(d2);
+            double fac = fmin(1.0, 1.0 - chi * (dist - a_sum));
+            fac = fmag * fac * fac * fac * fac / dist;
+            fx += fac * ex;
+            fy += fac * ey;
+            fz += fac * ez;
+          }
+        }
+      }
+  rep[ib] = fx;
+  rep[(size_t)nblob + ib] = fy;
+  rep[2 * (size_t)nblob + ib] = fz;
+}
+
 Status BarrierGrid::ensure(int n_particles) {
   int nb = 1024;
   while (nb < 2 * n_particles) nb <<= 1;
@@ -690,12 +765,25 @@ Status fil_barrier(cfm_ctx* c, const FilK& K, BarrierGrid& g) {
   return {};
 }
 
+Status fil_blob_barrier(cfm_ctx* c, const FilK& K, BarrierGrid& g, int nb_per_body, double* rep) {
+  const Particles& p = c->p;
+  const int n = p.n_seg + p.n_blob;
+  if (p.n_blob == 0) return {};
+  const GridPos P{K.x, K.y, K.z, p.x.p + p.n_seg, p.y.p + p.n_seg, p.z.p + p.n_seg, p.n_seg, n};
+  const double inv_h = 1.0 / (1.1 * 2.0 * K.a);
+  hipLaunchKernelGGL(blob_barrier_kernel, gsz(p.n_blob, 256), dim3(256), 0, c->stream, K, P, inv_h,
+                     g.nbucket - 1, g.start.p, g.ids.p, nb_per_body, rep);
+  CFM_CHECK_LAUNCH();
+  return {};
+}
+
 Status fil_errors(cfm_ctx* c, const FilK& K, int nt, double* err, double* bad_part,
-                  double* bad_out) {
+                  double* bad_out, int extra) {
   const int nb = ((int)K.nseg_total + 255) / 256;
   hipLaunchKernelGGL(fil_errors_kernel, dim3(nb), dim3(256), 0, c->stream, K, nt, c->p.v.p,
                      c->p.n, err, bad_part);
-  hipLaunchKernelGGL(max_finish_kernel, dim3(1), dim3(256), 0, c->stream, bad_part, nb, bad_out);
+  hipLaunchKernelGGL(max_finish_kernel, dim3(1), dim3(256), 0, c->stream, bad_part, nb + extra,
+                     bad_out);
   CFM_CHECK_LAUNCH();
   return {};
 }

@@ -26,6 +26,12 @@ struct FilK {
   double* pert = nullptr;  // 3 x nfil
   double* fint = nullptr;  // 6 x nseg_total internal loads (no barrier)
   double* jinv = nullptr;  // nfil x (6 nseg)^2, transposed inverse Jacobians
+  // Free bodies (null when the bases are fixed anchors): filament f belongs to body
+  // f / fpb, its base sits at bx + R(bq) fref[f] and its first segment carries the
+  // body's Lie-algebra element bu (AoS per body: 3, 4, 3 values).
+  const double *bx = nullptr, *bq = nullptr, *bu = nullptr, *fref = nullptr;
+  int fpb = 0;
+  double* clampblk = nullptr;  // 18 per filament: Lie rows 0 and 1 of J against u_0
 };
 
 // Hashed uniform cell list over segments and blobs for the barrier: cells of edge
@@ -49,8 +55,13 @@ struct BarrierGrid {
 Status fil_initial_guess(cfm_ctx* c, const FilK& K, int nt);
 Status fil_forces(cfm_ctx* c, const FilK& K, int nt);
 Status fil_barrier(cfm_ctx* c, const FilK& K, BarrierGrid& grid);
+// Residual of the filament rows; bad_part[nb .. nb + extra) already hold the badness
+// of further rows (free bodies), nb = ceil(nseg_total / 256).
 Status fil_errors(cfm_ctx* c, const FilK& K, int nt, double* err, double* bad_part,
-                  double* bad_out);
+                  double* bad_out, int extra = 0);
+// Repulsion on every blob from the segments and from the blobs of other bodies, on the
+// cell list fil_barrier built (rep: 3 x n_blob).
+Status fil_blob_barrier(cfm_ctx* c, const FilK& K, BarrierGrid& grid, int nb_per_body, double* rep);
 Status fil_jacobian(cfm_ctx* c, const FilK& K, int nt, int* status_dev);
 Status fil_jinv_mult(cfm_ctx* c, const FilK& K, const double* in, double* out, double scale);
 Status fil_update(cfm_ctx* c, const FilK& K, const double* upd);

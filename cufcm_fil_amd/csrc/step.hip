@@ -15,7 +15,9 @@
 #include <cstring>
 
 #include "ctx.hpp"
+#include "body.hpp"
 #include "fil.hpp"
+#include "swim.hpp"
 #include "krylov.hpp"
 
 namespace cfm {
@@ -127,6 +129,10 @@ struct FilState {
   DevBuf<int> status;
   bool have_jinv = false;
   double* host = nullptr;  // pinned read-back
+  // free swimmers (cfm_fil_attach_bodies): body state among the Broyden unknowns
+  bool swim = false;
+  SwimK S;
+  DevBuf<double> sw_state, sw_fref, sw_fext, sw_mob, sw_BJ, sw_Sf, sw_clamp, sw_rep, sw_tmp;
 };
 
 static Status ensure_host(FilState* F) {
@@ -152,7 +158,10 @@ static Status apply_plan(cfm_ctx* c, const ApplyPlan& pl, int prec) {
 }
 
 // v = M f for the current loads (segments from the filament loads, blob forces by GMRES)
+static Status swim_velocities(cfm_ctx* c, FilState* F, cfm_step_stats* st);
+
 static Status velocities(cfm_ctx* c, FilState* F, cfm_step_stats* st) {
+  if (F->swim) return swim_velocities(c, F, st);
   Particles& p = c->p;
   const int prec = F->prm.pair_precision;
   if (F->prm.barrier) CFM_TRY(fil_barrier(c, F->K, F->grid));
@@ -217,15 +226,97 @@ static Status velocities(cfm_ctx* c, FilState* F, cfm_step_stats* st) {
   return apply_plan(c, sb, prec);
 }
 
+// Free swimmer: the blob system carries the body velocities as unknowns and the
+// body load (swimmer.cpp:470-535 plus the blob repulsion of mobility_solver.cpp:215-236)
+// as its last rows (mobility_solver.cpp:455-632, 2673-2758).
+static Status swim_velocities(cfm_ctx* c, FilState* F, cfm_step_stats* st) {
+  Particles& p = c->p;
+  const int prec = F->prm.pair_precision;
+  const double* rep = nullptr;
+  if (F->prm.barrier) {
+    CFM_TRY(fil_barrier(c, F->K, F->grid));
+    CFM_TRY(fil_blob_barrier(c, F->K, F->grid, F->S.nb, F->sw_rep.p));
+    rep = F->sw_rep.p;
+  }
+  SystemDesc s;
+  CFM_TRY(body_system_desc(c, &s));
+  CFM_TRY(swim_force(c, F->S, F->K, rep, body_offsets(c)));
+  ApplyPlan pl;
+  pl.tgt_begin = 0;
+  pl.tgt_end = p.n;
+  pl.src_seg = true;
+  CFM_TRY(apply_plan(c, pl, prec));
+  // rhs: minus the blob velocities the segments induce, minus the body load
+  CFM_TRY(F->rhs_b.alloc(s.L));
+  CFM_TRY(dev_copy_rows(c, p.v.p, (size_t)p.n, p.n_seg, F->rhs_b.p, (size_t)s.m, 0, 3, s.m));
+  CFM_HIP(hipMemcpyAsync(F->rhs_b.p + 3 * (size_t)s.m, F->S.force, sizeof(double) * s.extra,
+                         hipMemcpyDeviceToDevice, c->stream));
+  CFM_TRY(dev_axpby(c, -1.0, F->rhs_b.p, 0.0, F->rhs_b.p, F->rhs_b.p, s.L));
+  cfm_solve_opts o = F->prm.blob_solve;
+  o.system = CFM_SYS_BODY;
+  o.use_guess = 1;  // previous blob forces and body velocities
+  if (!c->krylov) c->krylov = new Krylov();
+  cfm_solve_stats ss{};
+  CFM_TRY(solve(c, *c->krylov, s, o, F->rhs_b.p, F->fb_prev.p, &ss, nullptr));
+  if (st) {
+    st->gmres_iterations += ss.iterations;
+    st->gmres_solves += 1;
+    st->max_gmres_rel_residual = std::max(st->max_gmres_rel_residual, ss.rel_residual);
+  }
+  CFM_HIP(hipMemcpyAsync(F->S.vel, F->fb_prev.p + 3 * (size_t)s.m, sizeof(double) * s.extra,
+                         hipMemcpyDeviceToDevice, c->stream));
+  // v_seg += Msb f_b
+  CFM_TRY(dev_copy_rows(c, F->fb_prev.p, (size_t)s.m, 0, p.f.p, (size_t)p.n, p.n_seg, 3, s.m));
+  ApplyPlan sb;
+  sb.tgt_begin = 0;
+  sb.tgt_end = p.n_seg;
+  sb.src_blob = true;
+  sb.acc_seg = true;
+  return apply_plan(c, sb, prec);
+}
+
 static Status residual(cfm_ctx* c, FilState* F, int nt, double* e, double* bad) {
-  CFM_TRY(fil_errors(c, F->K, nt, e, F->bad_part.p, F->scal.p));
+  int extra = 0;
+  if (F->swim) {
+    const int nb = (c->p.n_seg + 255) / 256;
+    CFM_TRY(swim_errors(c, F->S, nt, e + 6 * (size_t)c->p.n_seg, F->bad_part.p + nb));
+    extra = F->S.nbody;
+  }
+  CFM_TRY(fil_errors(c, F->K, nt, e, F->bad_part.p, F->scal.p, extra));
   return read_dev(c, F, F->scal.p, bad);
+}
+
+// out = scale J0^-1 in (block diagonal over filaments, plus the body Schur complement)
+static Status jinv_mult(cfm_ctx* c, FilState* F, const double* in, double* out, double scale) {
+  if (F->swim) return swim_jinv_mult(c, F->S, F->K, in, out, scale, F->sw_tmp.p);
+  return fil_jinv_mult(c, F->K, in, out, scale);
+}
+
+static Status jacobian(cfm_ctx* c, FilState* F, int nt) {
+  CFM_HIP(hipMemsetAsync(F->status.p, 0, sizeof(int), c->stream));
+  CFM_TRY(fil_jacobian(c, F->K, nt, F->status.p));
+  if (F->swim) CFM_TRY(swim_jacobian(c, F->S, F->K, nt, F->status.p));
+  return {};
+}
+
+static Status update(cfm_ctx* c, FilState* F, const double* upd) {
+  if (F->swim) CFM_TRY(swim_update(c, F->S, upd + 6 * (size_t)c->p.n_seg));
+  CFM_TRY(fil_update(c, F->K, upd));
+  if (F->swim) CFM_TRY(body_place_device(c, F->S.x, F->S.q));
+  return {};
+}
+
+static Status initial_guess(cfm_ctx* c, FilState* F, int nt) {
+  if (F->swim) CFM_TRY(swim_guess(c, F->S, nt));
+  CFM_TRY(fil_initial_guess(c, F->K, nt));
+  if (F->swim) CFM_TRY(body_place_device(c, F->S.x, F->S.q));
+  return {};
 }
 
 // out = -gamma J0^-1 e - sum_i (e . D_i) C_i over the first `it` columns
 static Status low_rank_apply(cfm_ctx* c, FilState* F, const double* e, double* out, int it) {
   const int L = F->nbroy;
-  CFM_TRY(fil_jinv_mult(c, F->K, e, out, -F->prm.gamma));
+  CFM_TRY(jinv_mult(c, F, e, out, -F->prm.gamma));
   if (it > 0) {
     CFM_TRY(dev_dots(c, F->partial.p, F->D.p, (size_t)L, it, e, L, F->coef.p));
     CFM_TRY(dev_gemv_sub(c, F->C.p, (size_t)L, it, F->coef.p, out, L));
@@ -238,21 +329,18 @@ static Status step(cfm_ctx* c, FilState* F, int nt, cfm_step_stats* st) {
   cfm_step_stats S{};
   const int L = F->nbroy;
   const int maxit = F->prm.max_broyden;
-  CFM_TRY(fil_initial_guess(c, F->K, nt));
+  CFM_TRY(initial_guess(c, F, nt));
   CFM_TRY(fil_forces(c, F->K, nt));
   CFM_TRY(velocities(c, F, &S));
   double bad = 0.0;
   CFM_TRY(residual(c, F, nt, F->err.p, &bad));
   if (!std::isfinite(bad)) return make_status(CFM_ENONFINITE, "non-finite residual at step %d", nt);
   bool too_large = bad > 1.0;
-  if (too_large) {
-    CFM_HIP(hipMemsetAsync(F->status.p, 0, sizeof(int), c->stream));
-    CFM_TRY(fil_jacobian(c, F->K, nt, F->status.p));
-  }
+  if (too_large) CFM_TRY(jacobian(c, F, nt));
   int it = 0;
   while (too_large && it < maxit) {
     CFM_TRY(low_rank_apply(c, F, F->err.p, F->upd.p, it));
-    CFM_TRY(fil_update(c, F->K, F->upd.p));
+    CFM_TRY(update(c, F, F->upd.p));
     CFM_TRY(fil_forces(c, F->K, nt));
     CFM_TRY(velocities(c, F, &S));
     CFM_TRY(residual(c, F, nt, F->nerr.p, &bad));
@@ -305,7 +393,8 @@ void cfm_fil_release(cfm_ctx* c) {
                             &F->lam_m1, &F->lam_m2, &F->clamp, &F->clamp_m1, &F->clamp_m2,
                             &F->base, &F->pert, &F->fint, &F->jinv, &F->err, &F->nerr, &F->upd,
                             &F->C, &F->D, &F->coef, &F->partial, &F->bad_part, &F->scal,
-                            &F->fb_prev, &F->rhs_b};
+                            &F->fb_prev, &F->rhs_b, &F->sw_state, &F->sw_fref, &F->sw_fext,
+                            &F->sw_mob, &F->sw_BJ, &F->sw_Sf, &F->sw_clamp, &F->sw_rep, &F->sw_tmp};
   for (auto* b : bufs) b->release();
   F->status.release();
   F->grid.release();
@@ -539,7 +628,7 @@ int cfm_fil_update(cfm_ctx* c, const double* upd) {
   if (!upd) return fail_c(c, make_status(CFM_EINVAL, "null argument"));
   auto run = [&]() -> Status {
     CFM_HIP(hipMemcpy(F->upd.p, upd, sizeof(double) * F->nbroy, hipMemcpyHostToDevice));
-    CFM_TRY(fil_update(c, F->K, F->upd.p));
+    CFM_TRY(update(c, F, F->upd.p));
     CFM_HIP(hipStreamSynchronize(c->stream));
     return {};
   };
@@ -547,9 +636,147 @@ int cfm_fil_update(cfm_ctx* c, const double* upd) {
   return s.ok() ? CFM_OK : fail_c(c, s);
 }
 
+int cfm_fil_attach_bodies(cfm_ctx* c, const double* f_ext) {
+  FIL_OR_FAIL(c);
+  if (F->swim) return fail_c(c, make_status(CFM_ESTATE, "bodies are already attached"));
+  BodyInfo bi;
+  Status st = body_info(c, &bi);
+  if (!st.ok()) return fail_c(c, st);
+  const int nfil = F->K.nfil, B = bi.n_body;
+  if (nfil % B != 0)
+    return fail_c(c, make_status(CFM_EINVAL, "%d filaments cannot be shared equally by %d bodies", nfil, B));
+  if (c->cfg.a_blob != c->cfg.a_seg)
+    return fail_c(c, make_status(CFM_EINVAL, "free swimmers need equal blob and segment radii"));
+  const int pc = F->prm.blob_solve.precond;
+  if (pc != CFM_PC_NONE && pc != CFM_PC_DIAG && pc != CFM_PC_SELF)
+    return fail_c(c, make_status(CFM_EINVAL, "free swimmers take blob preconditioners NONE, DIAG or SELF "
+                                             "(the blob matrix changes as the bodies move)"));
+  if (f_ext)
+    for (int i = 0; i < 6 * B; ++i)
+      if (!std::isfinite(f_ext[i])) return fail_c(c, make_status(CFM_ENONFINITE, "non-finite external load"));
+  const int ns = c->p.n_seg, m = c->p.n_blob;
+  auto run = [&]() -> Status {
+    // base offsets in the body frame from the current first segments
+    std::vector<double> base(3 * (size_t)nfil), fref(3 * (size_t)nfil);
+    CFM_HIP(hipMemcpy(base.data(), F->base.p, sizeof(double) * base.size(), hipMemcpyDeviceToHost));
+    for (int f = 0; f < nfil; ++f) {
+      const int b = f / (nfil / B);
+      const double* q = &bi.q[4 * (size_t)b];
+      const double s0 = q[0], a = q[1], bb = q[2], cc = q[3];
+      const double R[9] = {1 - 2 * (bb * bb + cc * cc), 2 * (a * bb - s0 * cc), 2 * (a * cc + s0 * bb),
+                           2 * (a * bb + s0 * cc), 1 - 2 * (a * a + cc * cc), 2 * (bb * cc - s0 * a),
+                           2 * (a * cc - s0 * bb), 2 * (bb * cc + s0 * a), 1 - 2 * (a * a + bb * bb)};
+      const double d[3] = {base[f] - bi.x[3 * b], base[nfil + f] - bi.x[3 * b + 1],
+                           base[2 * (size_t)nfil + f] - bi.x[3 * b + 2]};
+      for (int k = 0; k < 3; ++k) fref[3 * (size_t)f + k] = R[k] * d[0] + R[3 + k] * d[1] + R[6 + k] * d[2];
+    }
+    // per body: x, xm1, xm2, u, um1 (3 each), q, qm1 (4 each), Qinit 9, vel 6, force 6,
+    // S0 36, Sinv 36, tb 6
+    const int per = 3 * 5 + 4 * 2 + 9 + 6 + 6 + 36 + 36 + 6;
+    CFM_TRY(F->sw_state.alloc((size_t)per * B));
+    double* d0 = F->sw_state.p;
+    SwimK& S = F->S;
+    size_t o = 0;
+    auto take = [&](int k) { double* p = d0 + o; o += (size_t)k * B; return p; };
+    S.x = take(3); S.xm1 = take(3); S.xm2 = take(3); S.u = take(3); S.um1 = take(3);
+    S.q = take(4); S.qm1 = take(4); S.Qinit = take(9); S.vel = take(6); S.force = take(6);
+    S.S0 = take(36); S.Sinv = take(36); S.tb = take(6);
+    std::vector<double> hx(3 * (size_t)B), hq(4 * (size_t)B), hQ(9 * (size_t)B);
+    for (int b = 0; b < B; ++b) {
+      for (int k = 0; k < 3; ++k) hx[3 * b + k] = bi.x[3 * b + k];
+      for (int k = 0; k < 4; ++k) hq[4 * b + k] = bi.q[4 * b + k];
+      const double* q = &bi.q[4 * (size_t)b];
+      const double s0 = q[0], a = q[1], bb = q[2], cc = q[3];
+      const double R[9] = {1 - 2 * (bb * bb + cc * cc), 2 * (a * bb - s0 * cc), 2 * (a * cc + s0 * bb),
+                           2 * (a * bb + s0 * cc), 1 - 2 * (a * a + cc * cc), 2 * (bb * cc - s0 * a),
+                           2 * (a * cc - s0 * bb), 2 * (bb * cc + s0 * a), 1 - 2 * (a * a + bb * bb)};
+      for (int k = 0; k < 9; ++k) hQ[9 * b + k] = R[k];
+    }
+    CFM_TRY(dev_fill(c, d0, 0.0, per * B));
+    for (double* dst : {S.x, S.xm1, S.xm2})
+      CFM_HIP(hipMemcpyAsync(dst, hx.data(), sizeof(double) * hx.size(), hipMemcpyHostToDevice, c->stream));
+    for (double* dst : {S.q, S.qm1})
+      CFM_HIP(hipMemcpyAsync(dst, hq.data(), sizeof(double) * hq.size(), hipMemcpyHostToDevice, c->stream));
+    CFM_HIP(hipMemcpyAsync(S.Qinit, hQ.data(), sizeof(double) * hQ.size(), hipMemcpyHostToDevice, c->stream));
+    CFM_TRY(F->sw_fref.alloc(fref.size()));
+    CFM_HIP(hipMemcpyAsync(F->sw_fref.p, fref.data(), sizeof(double) * fref.size(), hipMemcpyHostToDevice, c->stream));
+    CFM_TRY(F->sw_fext.alloc(6 * (size_t)B));
+    CFM_TRY(dev_fill(c, F->sw_fext.p, 0.0, 6 * B));
+    if (f_ext)
+      CFM_HIP(hipMemcpyAsync(F->sw_fext.p, f_ext, sizeof(double) * 6 * B, hipMemcpyHostToDevice, c->stream));
+    CFM_TRY(F->sw_mob.alloc(36));
+    CFM_HIP(hipMemcpyAsync(F->sw_mob.p, bi.mob, sizeof(double) * 36, hipMemcpyHostToDevice, c->stream));
+    const int N = 6 * F->K.nseg;
+    CFM_TRY(F->sw_BJ.alloc((size_t)nfil * 6 * N));
+    CFM_TRY(F->sw_Sf.alloc(36 * (size_t)nfil));
+    CFM_TRY(F->sw_clamp.alloc(18 * (size_t)nfil));
+    CFM_TRY(F->sw_rep.alloc(3 * (size_t)m));
+    CFM_TRY(dev_fill(c, F->sw_rep.p, 0.0, 3 * m));
+    CFM_TRY(F->sw_tmp.alloc(6 * (size_t)ns));
+    S.nbody = B;
+    S.fpb = nfil / B;
+    S.nfil = nfil;
+    S.nseg = F->K.nseg;
+    S.nb = bi.nb;
+    S.num_euler = F->K.num_euler;
+    S.dl = F->K.dl;
+    S.dt = F->K.dt;
+    S.tol = F->K.tol;
+    S.end_force = F->K.end_force;
+    S.fref = F->sw_fref.p;
+    S.fext = F->sw_fext.p;
+    S.mob = F->sw_mob.p;
+    S.BJ = F->sw_BJ.p;
+    S.Sf = F->sw_Sf.p;
+    // Broyden vectors grow by 6 per body; the blob solve by the body velocities
+    const int L = 6 * ns + 6 * B;
+    F->nbroy = L;
+    for (auto* buf : {&F->err, &F->nerr, &F->upd}) CFM_TRY(buf->alloc(L));
+    CFM_TRY(F->C.alloc((size_t)L * F->prm.max_broyden));
+    CFM_TRY(F->D.alloc((size_t)L * F->prm.max_broyden));
+    CFM_TRY(F->partial.alloc(dev_dots_partial_size(L, F->prm.max_broyden + 1)));
+    CFM_TRY(F->bad_part.alloc((ns + 255) / 256 + 1 + B));
+    CFM_TRY(F->fb_prev.alloc(3 * (size_t)m + 6 * B));
+    CFM_TRY(dev_fill(c, F->fb_prev.p, 0.0, 3 * m + 6 * B));
+    F->K.bx = S.x;
+    F->K.bq = S.q;
+    F->K.bu = S.u;
+    F->K.fref = F->sw_fref.p;
+    F->K.fpb = S.fpb;
+    F->K.clampblk = F->sw_clamp.p;
+    F->prm.guess = CFM_GUESS_PREVIOUS;  // the blob matrix changes as the bodies move
+    F->swim = true;
+    CFM_TRY(body_place_device(c, S.x, S.q));
+    CFM_HIP(hipStreamSynchronize(c->stream));
+    return {};
+  };
+  st = run();
+  if (!st.ok()) {
+    F->swim = false;
+    return fail_c(c, st);
+  }
+  return CFM_OK;
+}
+
+int cfm_fil_get_body_state(cfm_ctx* c, double* x, double* q, double* u, double* vel) {
+  FIL_OR_FAIL(c);
+  if (!F->swim) return fail_c(c, make_status(CFM_ESTATE, "no bodies attached (cfm_fil_attach_bodies)"));
+  const int B = F->S.nbody;
+  auto run = [&]() -> Status {
+    if (x) CFM_HIP(hipMemcpyAsync(x, F->S.x, sizeof(double) * 3 * B, hipMemcpyDeviceToHost, c->stream));
+    if (q) CFM_HIP(hipMemcpyAsync(q, F->S.q, sizeof(double) * 4 * B, hipMemcpyDeviceToHost, c->stream));
+    if (u) CFM_HIP(hipMemcpyAsync(u, F->S.u, sizeof(double) * 3 * B, hipMemcpyDeviceToHost, c->stream));
+    if (vel) CFM_HIP(hipMemcpyAsync(vel, F->S.vel, sizeof(double) * 6 * B, hipMemcpyDeviceToHost, c->stream));
+    CFM_HIP(hipStreamSynchronize(c->stream));
+    return {};
+  };
+  Status st = run();
+  return st.ok() ? CFM_OK : fail_c(c, st);
+}
+
 int cfm_fil_initial_guess(cfm_ctx* c, int32_t nt) {
   FIL_OR_FAIL(c);
-  Status s = fil_initial_guess(c, F->K, nt);
+  Status s = initial_guess(c, F, nt);
   if (s.ok()) {
     hipError_t e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) s = make_status(CFM_EHIP, "%s", hipGetErrorString(e));

@@ -1,0 +1,447 @@
+// swim.hip -- free swimmers: rigid bodies of blobs carrying the elastic filaments, with
+// the body position and orientation among the Broyden unknowns (BODY_VELOCITY_TYPE 0,
+// blob forces and body velocities by GMRES).
+//
+//   body initial guess / update     rigid_body.cpp:410-458, 517-544
+//   body load of the swimmer        swimmer.cpp:470-535, mobility_solver.cpp:215-236
+//   body rows of the BDF residual   mobility_solver.cpp:3003-3040
+//   body Schur complement of J0     swimmer.cpp:688-909 (no sphere-specific correction)
+//   J0^-1 with the body             swimmer.cpp:1022-1068
+// Unknown order of the Broyden vector: the filaments' 6 NSEG blocks, then 6 per body
+// (position, Lie algebra).  Filament f belongs to body f / fpb.
+#include <hip/hip_runtime.h>
+
+#include "fil.hpp"
+#include "fil_device.hpp"
+#include "swim.hpp"
+
+namespace cfm {
+using namespace fil;
+
+namespace {
+
+__device__ __forceinline__ V3 ldv(const double* a) { return {a[0], a[1], a[2]}; }
+__device__ __forceinline__ void stv(double* a, V3 v) {
+  a[0] = v.x;
+  a[1] = v.y;
+  a[2] = v.z;
+}
+__device__ __forceinline__ Q4 ldq4(const double* a) { return {a[0], a[1], a[2], a[3]}; }
+__device__ __forceinline__ Q4 segq(const FilK& K, int s) {
+  const size_t n = K.nseg_total;
+  return {K.q[s], K.q[n + s], K.q[2 * n + s], K.q[3 * n + s]};
+}
+__device__ __forceinline__ void stq4(double* a, Q4 q) {
+  a[0] = q.s;
+  a[1] = q.x;
+  a[2] = q.y;
+  a[3] = q.z;
+}
+__device__ __forceinline__ M3 ldm(const double* a) {  // row-major 3 x 3
+  M3 m;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) m.a[i][j] = a[3 * i + j];
+  return m;
+}
+__device__ __forceinline__ M3 transpose(const M3& A) {
+  M3 T;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) T.a[i][j] = A.a[j][i];
+  return T;
+}
+// 3 x 3 block (bi, bj) of a row-major 6 x 6
+__device__ __forceinline__ M3 blk6(const double* A, int bi, int bj) {
+  M3 m;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) m.a[i][j] = A[(bi + i) * 6 + bj + j];
+  return m;
+}
+
+// rigid_body::initial_guess: extrapolate position and Lie algebra, q = exp(u) q_-1
+__global__ void swim_guess_kernel(SwimK S, int nt) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= S.nbody) return;
+  const V3 x = ldv(S.x + 3 * b), xm1 = ldv(S.xm1 + 3 * b), xm2 = ldv(S.xm2 + 3 * b);
+  const V3 u = ldv(S.u + 3 * b), um1 = ldv(S.um1 + 3 * b);
+  V3 gx, gu;
+  if (nt <= S.num_euler) {
+    gx = 2.0 * x - xm1;
+    gu = u;
+  } else {
+    gx = 3.0 * (x - xm1) + xm2;
+    gu = 2.0 * u - um1;
+  }
+  stv(S.xm2 + 3 * b, xm1);
+  stv(S.xm1 + 3 * b, x);
+  stv(S.x + 3 * b, gx);
+  stv(S.um1 + 3 * b, u);
+  stv(S.u + 3 * b, gu);
+  const Q4 q = ldq4(S.q + 4 * b);
+  stq4(S.qm1 + 4 * b, q);
+  const Q4 qn = qmul(lie_exp(gu), q);
+  stq4(S.q + 4 * b, qn);
+  const M3 R = rot_mat(qn);
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) S.Qinit[9 * b + 3 * i + j] = R.a[i][j];
+}
+
+// rigid_body::update from the body block of a Broyden increment
+__global__ void swim_update_kernel(SwimK S, const double* __restrict__ upd) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= S.nbody) return;
+  const double* d = upd + 6 * b;
+  stv(S.x + 3 * b, ldv(S.x + 3 * b) + ldv(d));
+  const V3 u = ldv(S.u + 3 * b) + ldv(d + 3);
+  stv(S.u + 3 * b, u);
+  stq4(S.q + 4 * b, qmul(lie_exp(u), ldq4(S.qm1 + 4 * b)));
+}
+
+// Body load: minus the internal loads of its segments about the centre, the repulsion on
+// its blobs about the centre, plus the external load.  One workgroup per body.
+__global__ __launch_bounds__(256) void swim_force_kernel(SwimK S, FilK K, const double* __restrict__ rep,
+                                                         const double* __restrict__ boff, int nblob) {
+  __shared__ double red[4][6];
+  const int b = blockIdx.x;
+  const size_t ns = K.nseg_total;
+  double s[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  const V3 xb = ldv(S.x + 3 * b);
+  const int s1 = (b + 1) * S.fpb * S.nseg;
+  for (int i = b * S.fpb * S.nseg + threadIdx.x; i < s1; i += 256) {
+    const V3 F = {K.fint[i], K.fint[ns + i], K.fint[2 * ns + i]};
+    const V3 T = {K.fint[3 * ns + i], K.fint[4 * ns + i], K.fint[5 * ns + i]};
+    const V3 d = v3(K.x[i], K.y[i], K.z[i]) - xb;
+    const V3 tq = T + cross(d, F);
+    s[0] -= F.x;
+    s[1] -= F.y;
+    s[2] -= F.z;
+    s[3] -= tq.x;
+    s[4] -= tq.y;
+    s[5] -= tq.z;
+  }
+  if (rep) {
+    const int j1 = (b + 1) * S.nb;
+    for (int j = b * S.nb + threadIdx.x; j < j1; j += 256) {
+      const V3 F = {rep[j], rep[nblob + j], rep[2 * (size_t)nblob + j]};
+      const V3 r = {boff[j], boff[nblob + j], boff[2 * (size_t)nblob + j]};
+      const V3 tq = cross(r, F);
+      s[0] += F.x;
+      s[1] += F.y;
+      s[2] += F.z;
+      s[3] += tq.x;
+      s[4] += tq.y;
+      s[5] += tq.z;
+    }
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    double v = s[k];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if (lane == 0) red[wave][k] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    const int k = threadIdx.x;
+    S.force[6 * b + k] = ((red[0][k] + red[1][k]) + (red[2][k] + red[3][k])) + S.fext[6 * b + k];
+  }
+}
+
+// Body rows of the BDF residual and their badness max(|e_x| / (DL TOL), |e_u| / TOL).
+__global__ void swim_errors_kernel(SwimK S, int nt, double* __restrict__ err, double* __restrict__ bad) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= S.nbody) return;
+  const V3 x = ldv(S.x + 3 * b), xm1 = ldv(S.xm1 + 3 * b), xm2 = ldv(S.xm2 + 3 * b);
+  const V3 u = ldv(S.u + 3 * b), um1 = ldv(S.um1 + 3 * b);
+  const V3 V = ldv(S.vel + 6 * b), W = dexpinv(u, ldv(S.vel + 6 * b + 3));
+  V3 ex, eu;
+  if (nt < S.num_euler) {
+    ex = x - xm1 - S.dt * V;
+    eu = u - S.dt * W;
+  } else {
+    ex = x + (-4.0 / 3.0) * xm1 + (1.0 / 3.0) * xm2 + (-2.0 / 3.0 * S.dt) * V;
+    eu = u - (1.0 / 3.0) * um1 + (-2.0 / 3.0 * S.dt) * W;
+  }
+  stv(err + 6 * b, ex);
+  stv(err + 6 * b + 3, eu);
+  double v = fmax(fmax(fabs(ex.x), fmax(fabs(ex.y), fabs(ex.z))) / (S.dl * S.tol),
+                  fmax(fabs(eu.x), fmax(fabs(eu.y), fabs(eu.z))) / S.tol);
+  if (!(ex.x == ex.x && ex.y == ex.y && ex.z == ex.z && eu.x == eu.x && eu.y == eu.y && eu.z == eu.z))
+    v = __builtin_nan("");
+  bad[b] = v;
+}
+
+// Rotation part of a position row's dependence on the body: rcross((DL/2 + |d|) d / |d|)
+__device__ __forceinline__ M3 arm_block(const SwimK& S, V3 d) {
+  const double dn = sqrt(dot(d, d));
+  return rcross(((0.5 * S.dl + dn) / dn) * d);
+}
+
+__device__ __forceinline__ V3 base_disp(const SwimK& S, int f) {
+  const int b = f / S.fpb;
+  return mv(ldm(S.Qinit + 9 * b), ldv(S.fref + 3 * f));
+}
+
+// Body-mobility block (bi, bj) rotated by Q_init of body b
+__device__ __forceinline__ M3 rot_mob(const SwimK& S, int b, int bi, int bj) {
+  const M3 Q = ldm(S.Qinit + 9 * b);
+  return mm(mm(Q, blk6(S.mob, bi, bj)), transpose(Q));
+}
+
+// B J^-1 of filament f (swimmer.cpp:744-886 without the sphere correction) and its
+// product with the filament's body columns A (:888-902).  One workgroup per filament.
+__global__ __launch_bounds__(256) void swim_bj_kernel(SwimK S, FilK K, int nt) {
+  extern __shared__ double lds[];
+  const int f = blockIdx.x;
+  const int n = K.nseg, N = 6 * n;
+  const int s0 = f * n, b = f / S.fpb;
+  double* B = lds;          // 6 x N
+  double* BJ = lds + 6 * N; // 6 x N
+  const int tid = threadIdx.x;
+  const double dt_fac = nt < S.num_euler ? -S.dt : -2.0 * S.dt / 3.0;
+  const V3 ub = ldv(S.u + 3 * b);
+  const M3 blc = mscale(rcross(ub), -1.0);
+  const M3 blc2 = mm(blc, blc);
+  const M3 Dinv = madd(madd(eye3(), blc2, 1.0 / 12.0), blc, -0.5);
+  const M3 DinvT = madd(madd(eye3(), blc2, 1.0 / 12.0), blc, 0.5);
+  const V3 d = base_disp(S, f);
+  for (int e = tid; e < 6 * N; e += 256) B[e] = 0.0;
+  __syncthreads();
+  if (tid == 0) {
+    auto put = [&](int r0, int c0, const M3& M) {
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) B[(r0 + i) * N + c0 + j] = M.a[i][j];
+    };
+    put(0, 0, mscale(eye3(), -1.0));
+    put(3, 0, rcross(d));
+    put(3, 3 * n, mscale(DinvT, -1.0));
+    const V3 tl = tangent(segq(K, s0 + n - 1));
+    const M3 E = rcross(S.end_force * tl);
+    put(0, N - 3, E);
+    const V3 diff = v3(K.x[s0 + n - 1], K.y[s0 + n - 1], K.z[s0 + n - 1]) - ldv(S.x + 3 * b);
+    put(3, N - 3, mm(rcross((0.5 * S.dl) * tl - diff), E));
+    for (int m = 1; m < n - 1; ++m) put(3, 3 * (n + m), mm(E, rcross(S.dl * tangent(segq(K, s0 + m)))));
+  }
+  __syncthreads();
+  // B <- dt_fac diag(I, Dinv) N B, column by column
+  if (tid < N) {
+    const int c = tid;
+    double col[6], out[6];
+    for (int r = 0; r < 6; ++r) col[r] = B[r * N + c];
+    for (int bi = 0; bi < 6; bi += 3) {
+      V3 acc = v3(0.0, 0.0, 0.0);
+      for (int bj = 0; bj < 6; bj += 3) acc = acc + mv(rot_mob(S, b, bi, bj), v3(col[bj], col[bj + 1], col[bj + 2]));
+      if (bi == 3) acc = mv(Dinv, acc);
+      out[bi] = dt_fac * acc.x;
+      out[bi + 1] = dt_fac * acc.y;
+      out[bi + 2] = dt_fac * acc.z;
+    }
+    for (int r = 0; r < 6; ++r) B[r * N + c] = out[r];
+  }
+  __syncthreads();
+  // BJ = B J^-1 (J^-1 stored column-major: element (r, c) at c N + r)
+  const double* Ji = K.jinv + (size_t)f * N * N;
+  for (int e = tid; e < 6 * N; e += 256) {
+    const int k = e / N, c = e - k * N;
+    double s = 0.0;
+    for (int r = 0; r < N; ++r) s += B[k * N + r] * Ji[(size_t)c * N + r];
+    BJ[e] = s;
+    S.BJ[(size_t)f * 6 * N + e] = s;
+  }
+  __syncthreads();
+  // S_f = BJ A: position rows take I (translation) and the arm block (rotation); the
+  // Lie rows of segments 0 and 1 take the clamping blocks (rotation)
+  if (tid < 36) {
+    const int k = tid / 6, j = tid - 6 * k;
+    const double* row = BJ + k * N;
+    double s = 0.0;
+    if (j < 3) {
+      for (int m = 0; m < n; ++m) s += row[3 * m + j];
+    } else {
+      const int jj = j - 3;
+      const M3 Dm = arm_block(S, d);
+      for (int m = 0; m < n; ++m)
+        for (int a = 0; a < 3; ++a) s += row[3 * m + a] * Dm.a[a][jj];
+      const double* cb = K.clampblk + (size_t)f * 18;
+      for (int a = 0; a < 3; ++a) s += row[3 * n + a] * cb[3 * a + jj] + row[3 * n + 3 + a] * cb[9 + 3 * a + jj];
+    }
+    S.Sf[36 * (size_t)f + tid] = s;
+  }
+}
+
+// Body block of J0 and its Schur complement (swimmer.cpp:690-742, 902-909), inverted.
+__global__ void swim_schur_kernel(SwimK S, FilK K, int nt, int* __restrict__ status) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= S.nbody) return;
+  const int n = K.nseg;
+  const size_t ns = K.nseg_total;
+  const double dt_fac = nt < S.num_euler ? -S.dt : -2.0 * S.dt / 3.0;
+  const V3 u = ldv(S.u + 3 * b);
+  const M3 blc = mscale(rcross(u), -1.0);
+  const M3 cbo = mscale(blc, -1.0 / dt_fac);
+  double A[36];
+  for (int i = 0; i < 36; ++i) A[i] = (i % 7 == 0) ? 1.0 : 0.0;
+  M3 T = mscale(madd(mscale(mm(blc, cbo), 1.0 / 12.0), cbo, -0.5), dt_fac);
+  M3 old = zero3();
+  for (int f = b * S.fpb; f < (b + 1) * S.fpb; ++f) {
+    const V3 d = base_disp(S, f);
+    const int s0 = f * n;
+    const V3 lt = {K.lam[s0], K.lam[ns + s0], K.lam[2 * ns + s0]};
+    const V3 cl = {K.clamp[f], K.clamp[K.nfil + f], K.clamp[2 * K.nfil + f]};
+    old = madd(old, mm(rcross(lt), rcross(d)), -1.0);
+    old = madd(old, rcross(cl), 0.5);
+    old = madd(old, madd(rcross(cross(u, cl)), mm(blc, rcross(cl))), -1.0 / 12.0);
+    const V3 tl = tangent(segq(K, s0 + n - 1));
+    old = madd(old, mm(rcross(S.end_force * tl), arm_block(S, d)));
+  }
+  old = mm(rot_mob(S, b, 3, 3), old);
+  T = madd(T, mscale(madd(madd(old, mm(blc, old), -0.5), mm(mm(blc, blc), old), 1.0 / 12.0), dt_fac));
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) A[(3 + i) * 6 + 3 + j] += T.a[i][j];
+  for (int i = 0; i < 36; ++i) S.S0[36 * b + i] = A[i];
+  for (int f = b * S.fpb; f < (b + 1) * S.fpb; ++f)
+    for (int i = 0; i < 36; ++i) A[i] -= S.Sf[36 * (size_t)f + i];
+  // Gauss-Jordan with partial pivoting
+  double I6[36];
+  for (int i = 0; i < 36; ++i) I6[i] = (i % 7 == 0) ? 1.0 : 0.0;
+  bool ok = true;
+  for (int c = 0; c < 6 && ok; ++c) {
+    int p = c;
+    for (int r = c + 1; r < 6; ++r)
+      if (fabs(A[r * 6 + c]) > fabs(A[p * 6 + c])) p = r;
+    const double piv = A[p * 6 + c];
+    if (!(fabs(piv) > 0.0) || piv != piv) {
+      ok = false;
+      break;
+    }
+    if (p != c)
+      for (int j = 0; j < 6; ++j) {
+        double t = A[p * 6 + j];
+        A[p * 6 + j] = A[c * 6 + j];
+        A[c * 6 + j] = t;
+        t = I6[p * 6 + j];
+        I6[p * 6 + j] = I6[c * 6 + j];
+        I6[c * 6 + j] = t;
+      }
+    const double ip = 1.0 / piv;
+    for (int j = 0; j < 6; ++j) {
+      A[c * 6 + j] *= ip;
+      I6[c * 6 + j] *= ip;
+    }
+    for (int r = 0; r < 6; ++r) {
+      if (r == c) continue;
+      const double g = A[r * 6 + c];
+      for (int j = 0; j < 6; ++j) {
+        A[r * 6 + j] -= g * A[c * 6 + j];
+        I6[r * 6 + j] -= g * I6[c * 6 + j];
+      }
+    }
+  }
+  if (!ok) status[0] = 1;
+  for (int i = 0; i < 36; ++i) S.Sinv[36 * b + i] = I6[i];
+}
+
+// Body block of J0^-1 in: t = S^-1 (in_b - sum_f BJ_f in_f); out_b = scale t.  One
+// workgroup per body.
+__global__ __launch_bounds__(256) void swim_b2_kernel(SwimK S, int N, const double* __restrict__ in,
+                                                      double* __restrict__ out, double scale) {
+  __shared__ double red[4][6];
+  const int b = blockIdx.x;
+  const size_t L0 = (size_t)S.nfil * N;
+  double s[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  const size_t e0 = (size_t)b * S.fpb * N, e1 = e0 + (size_t)S.fpb * N;
+  for (size_t e = e0 + threadIdx.x; e < e1; e += 256) {
+    const size_t f = e / N, c = e - f * N;
+    const double* bj = S.BJ + f * 6 * N + c;
+    const double v = in[e];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) s[k] += bj[(size_t)k * N] * v;
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    double v = s[k];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if (lane == 0) red[wave][k] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double b2[6];
+    for (int k = 0; k < 6; ++k)
+      b2[k] = in[L0 + 6 * b + k] - ((red[0][k] + red[1][k]) + (red[2][k] + red[3][k]));
+    for (int k = 0; k < 6; ++k) {
+      double t = 0.0;
+      for (int j = 0; j < 6; ++j) t += S.Sinv[36 * b + 6 * k + j] * b2[j];
+      S.tb[6 * b + k] = t;
+      out[L0 + 6 * b + k] = scale * t;
+    }
+  }
+}
+
+// tmp_f = in_f - A_f t_b for every filament row (then J_f^-1 tmp_f).  One thread per segment.
+__global__ void swim_rhs_kernel(SwimK S, FilK K, const double* __restrict__ in, double* __restrict__ tmp) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= (int)K.nseg_total) return;
+  const int n = K.nseg, f = s / n, m = s - f * n, b = f / S.fpb;
+  const size_t base = (size_t)f * 6 * n;
+  const V3 tv = ldv(S.tb + 6 * b), tw = ldv(S.tb + 6 * b + 3);
+  const V3 px = tv + mv(arm_block(S, base_disp(S, f)), tw);
+  const size_t ip = base + 3 * m;
+  tmp[ip] = in[ip] - px.x;
+  tmp[ip + 1] = in[ip + 1] - px.y;
+  tmp[ip + 2] = in[ip + 2] - px.z;
+  const size_t il = base + 3 * n + 3 * m;
+  V3 pl = v3(0.0, 0.0, 0.0);
+  if (m <= 1) pl = mv(ldm(K.clampblk + (size_t)f * 18 + 9 * m), tw);
+  tmp[il] = in[il] - pl.x;
+  tmp[il + 1] = in[il + 1] - pl.y;
+  tmp[il + 2] = in[il + 2] - pl.z;
+}
+
+inline dim3 gs(int n, int b) { return dim3((unsigned)((n + b - 1) / b)); }
+
+}  // namespace
+
+Status swim_guess(cfm_ctx* c, const SwimK& S, int nt) {
+  hipLaunchKernelGGL(swim_guess_kernel, gs(S.nbody, 64), dim3(64), 0, c->stream, S, nt);
+  CFM_CHECK_LAUNCH();
+  return {};
+}
+
+Status swim_update(cfm_ctx* c, const SwimK& S, const double* upd_body) {
+  hipLaunchKernelGGL(swim_update_kernel, gs(S.nbody, 64), dim3(64), 0, c->stream, S, upd_body);
+  CFM_CHECK_LAUNCH();
+  return {};
+}
+
+Status swim_force(cfm_ctx* c, const SwimK& S, const FilK& K, const double* rep, const double* boff) {
+  hipLaunchKernelGGL(swim_force_kernel, dim3(S.nbody), dim3(256), 0, c->stream, S, K, rep, boff,
+                     c->p.n_blob);
+  CFM_CHECK_LAUNCH();
+  return {};
+}
+
+Status swim_errors(cfm_ctx* c, const SwimK& S, int nt, double* err_body, double* bad) {
+  hipLaunchKernelGGL(swim_errors_kernel, gs(S.nbody, 64), dim3(64), 0, c->stream, S, nt, err_body, bad);
+  CFM_CHECK_LAUNCH();
+  return {};
+}
+
+Status swim_jacobian(cfm_ctx* c, const SwimK& S, const FilK& K, int nt, int* status_dev) {
+  const int N = 6 * K.nseg;
+  const size_t lds = sizeof(double) * 12 * N;
+  hipLaunchKernelGGL(swim_bj_kernel, dim3(S.nfil), dim3(256), lds, c->stream, S, K, nt);
+  hipLaunchKernelGGL(swim_schur_kernel, gs(S.nbody, 64), dim3(64), 0, c->stream, S, K, nt, status_dev);
+  CFM_CHECK_LAUNCH();
+  return {};
+}
+
+Status swim_jinv_mult(cfm_ctx* c, const SwimK& S, const FilK& K, const double* in, double* out,
+                      double scale, double* tmp) {
+  const int N = 6 * K.nseg;
+  hipLaunchKernelGGL(swim_b2_kernel, dim3(S.nbody), dim3(256), 0, c->stream, S, N, in, out, scale);
+  hipLaunchKernelGGL(swim_rhs_kernel, gs((int)K.nseg_total, 256), dim3(256), 0, c->stream, S, K, in, tmp);
+  CFM_CHECK_LAUNCH();
+  return fil_jinv_mult(c, K, tmp, out, scale);
+}
+
+}  // namespace cfm

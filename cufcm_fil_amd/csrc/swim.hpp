@@ -1,0 +1,38 @@
+// swim.hpp -- free swimmers: rigid bodies carrying the filaments, the body state among
+// the Broyden unknowns (swim.hip).
+#pragma once
+#include "ctx.hpp"
+#include "fil.hpp"
+
+namespace cfm {
+
+struct SwimK {
+  int nbody = 0, fpb = 0, nfil = 0, nseg = 0;  // bodies, filaments per body, filaments, NSEG
+  int nb = 0;                                  // blobs per body
+  int num_euler = 1;
+  double dl = 2.2, dt = 0.1, tol = 1e-4, end_force = 0.0;
+  // per body, AoS: position 3 (x, x_-1, x_-2), Lie algebra 3 (u, u_-1), quaternion 4
+  // (q, q_-1), R(q) at the step's initial guess 9 (row-major)
+  double *x = nullptr, *xm1 = nullptr, *xm2 = nullptr, *u = nullptr, *um1 = nullptr;
+  double *q = nullptr, *qm1 = nullptr, *Qinit = nullptr;
+  const double* fref = nullptr;  // 3 per filament: base offset in the body frame
+  double* vel = nullptr;         // 6 per body: V, Omega of the last blob solve
+  double* force = nullptr;       // 6 per body: load the blobs must balance
+  const double* fext = nullptr;  // 6 per body: external force and torque
+  const double* mob = nullptr;   // 36: body-frame body mobility (row-major)
+  double* BJ = nullptr;          // per filament 6 x 6 NSEG: B J^-1
+  double* Sf = nullptr;          // 36 per filament: B J^-1 A
+  double* S0 = nullptr;          // 36 per body: body block of J0
+  double* Sinv = nullptr;        // 36 per body: inverse Schur complement
+  double* tb = nullptr;          // 6 per body: body part of the last J0^-1 product
+};
+
+Status swim_guess(cfm_ctx* c, const SwimK& S, int nt);
+Status swim_update(cfm_ctx* c, const SwimK& S, const double* upd_body);
+Status swim_force(cfm_ctx* c, const SwimK& S, const FilK& K, const double* rep, const double* boff);
+Status swim_errors(cfm_ctx* c, const SwimK& S, int nt, double* err_body, double* bad);
+Status swim_jacobian(cfm_ctx* c, const SwimK& S, const FilK& K, int nt, int* status_dev);
+Status swim_jinv_mult(cfm_ctx* c, const SwimK& S, const FilK& K, const double* in, double* out,
+                      double scale, double* tmp);
+
+}  // namespace cfm

@@ -36,7 +36,7 @@ class FilamentSystem:
 
     def __init__(self, x_seg, tangents, *, x_blob=None, precision="fp64", solve_mode="fp64",
                  blob_rtol=1e-10, blob_restart=50, blob_max_iter=2000, blob_precond="diag",
-                 guess="previous", pert=None, device=0, tile_split=0, **params):
+                 guess="previous", pert=None, device=0, tile_split=0, wall=True, **params):
         prm = dict(DEFAULTS)
         prm.update(params)
         self.params = prm
@@ -45,7 +45,7 @@ class FilamentSystem:
         n_seg = x_seg.size // 3
         n_blob = 0 if x_blob is None else L.f64(x_blob).size // 3
         self.n_seg, self.n_blob, self.nseg, self.nfil = n_seg, n_blob, nseg, n_seg // nseg
-        self.mob = RpyMobilitySolver(n_seg, n_blob, seg_dof=6, wall=True, precision=precision,
+        self.mob = RpyMobilitySolver(n_seg, n_blob, seg_dof=6, wall=wall, precision=precision,
                                      device=device, tile_split=tile_split)
         self.mob.set_positions(x_seg, None if x_blob is None else L.f64(x_blob))
         q = geo.segment_quaternions(tangents, nseg)
@@ -76,6 +76,33 @@ class FilamentSystem:
     @property
     def h(self):
         return self.mob.handle
+
+    # -- free swimmers (swimmer.cpp, rigid_body.cpp) ---------------------------------
+    def attach_bodies(self, blob_ref, x_body, q_body, *, f_ext=None, mobility=None,
+                      ref_precond="dense", ref_rtol=1e-10):
+        """The blobs become n_body free rigid bodies (blob_ref: body-frame offsets of one
+        body, x_body 3 n_body, q_body 4 n_body) carrying the filaments (n_fil / n_body
+        per body); their positions and orientations join the Broyden unknowns.  The body
+        mobility for the approximate Jacobian is computed (make_body_reference_matrices)
+        unless given (6 x 6, body frame)."""
+        x_body = L.f64(x_body)
+        n_body = x_body.size // 3
+        self.mob.body_init(blob_ref, n_body)
+        self.mob.body_set_state(x_body, q_body)
+        if mobility is None:
+            self.mob.make_body_reference_matrices(0, precond=ref_precond, rtol=ref_rtol)
+        else:
+            self.mob.body_set_reference_matrices(None, L.f64(mobility, 36))
+        fe = None if f_ext is None else L.f64(f_ext, 6 * n_body)
+        L.check(self.h, L.lib().cfm_fil_attach_bodies(self.h, L.ptr(fe)))
+        self.n_body = n_body
+
+    def body_state(self):
+        nb = self.n_body
+        x, q, u, v = np.zeros(3 * nb), np.zeros(4 * nb), np.zeros(3 * nb), np.zeros(6 * nb)
+        L.check(self.h, L.lib().cfm_fil_get_body_state(self.h, L.ptr(x), L.ptr(q), L.ptr(u),
+                                                        L.ptr(v)))
+        return {"x": x, "q": q, "u": u, "vel": v}
 
     def close(self):
         self.mob.close()
@@ -128,15 +155,20 @@ class FilamentSystem:
         L.check(self.h, L.lib().cfm_fil_jacobian_inverse(self.h, int(nt), L.ptr(J)))
         return J.reshape(self.nfil, N, N)
 
+    @property
+    def n_unknowns(self):
+        """Length of the Broyden vectors: 6 per segment, plus 6 per attached body."""
+        return 6 * self.n_seg + 6 * getattr(self, "n_body", 0)
+
     def errors(self, nt, v_seg):
-        e = np.zeros(6 * self.n_seg)
+        e = np.zeros(self.n_unknowns)
         bad = ctypes.c_double()
         L.check(self.h, L.lib().cfm_fil_errors(self.h, int(nt), L.ptr(L.f64(v_seg, 6 * self.n_seg)),
                                                 L.ptr(e), ctypes.byref(bad)))
         return e, bad.value
 
     def update(self, upd):
-        L.check(self.h, L.lib().cfm_fil_update(self.h, L.ptr(L.f64(upd, 6 * self.n_seg))))
+        L.check(self.h, L.lib().cfm_fil_update(self.h, L.ptr(L.f64(upd, self.n_unknowns))))
 
     def initial_guess(self, nt):
         L.check(self.h, L.lib().cfm_fil_initial_guess(self.h, int(nt)))

@@ -231,6 +231,26 @@ int cfm_fil_errors(cfm_ctx* ctx, int32_t nt, const double* v_seg, double* err, d
 int cfm_fil_update(cfm_ctx* ctx, const double* upd);
 int cfm_fil_initial_guess(cfm_ctx* ctx, int32_t nt);
 
+/* Free swimmers (BODY_VELOCITY_TYPE 0 with the blob system solved by GMRES): the
+ * bodies of cfm_body_init / cfm_body_set_state carry the filaments, n_fil / n_body
+ * consecutive filaments per body, each attached at its current first segment (kept
+ * rigidly in the body frame; the first segment turns with the body,
+ * filament.cpp:653-678).  From the next cfm_fil_step on, each body's position and
+ * orientation are Broyden unknowns (rigid_body.cpp:410-544; body rows of
+ * mobility_solver.cpp:3003-3040), the blob solve is CFM_SYS_BODY with the body load
+ * (swimmer.cpp:470-535: minus the segments' internal loads about the centre, plus the
+ * repulsion on the blobs, plus f_ext) as its last rows, and J0^-1 includes the body
+ * Schur complement (swimmer.cpp:688-909, 1022-1068) built with the body mobility of
+ * cfm_body_reference_matrices.  f_ext: 6 per body (force, torque; may be null).  The
+ * Broyden vectors of cfm_fil_errors / cfm_fil_update then hold 6 n_seg + 6 n_body
+ * values (bodies last: position increment, Lie-algebra increment).  Needs equal blob
+ * and segment radii and a blob preconditioner NONE, DIAG or SELF; the blob-force guess
+ * becomes the previous solution. */
+int cfm_fil_attach_bodies(cfm_ctx* ctx, const double* f_ext);
+/* Body centres (3 n_body), quaternions (4), Lie-algebra elements of the step (3) and the
+ * last solve's (V, Omega) (6); each pointer may be null. */
+int cfm_fil_get_body_state(cfm_ctx* ctx, double* x, double* q, double* u, double* vel);
+
 /* ---- measurement --------------------------------------------------------- */
 typedef struct cfm_profile_stats {
   int64_t launches;        /* pair-kernel launches recorded since the last reset */

@@ -204,7 +204,10 @@ class Filament:
         for i in range(1, self.P.nseg):
             self.x[i] = self.x[i - 1] + 0.5 * dl * (tangent(self.q[i - 1]) + tangent(self.q[i]))
 
-    def initial_guess(self, nt):
+    def initial_guess(self, nt, base=None, ub=None):
+        """filament::initial_guess (filament.cpp:1336-1663): extrapolation, then
+        accept_state_from_rigid_body (:653-678) with the base position and Lie-algebra
+        element of the body (a fixed anchor and zero when not given)."""
         P = self.P
         self.xm2 = self.xm1.copy()
         self.xm1 = self.x.copy()
@@ -236,15 +239,21 @@ class Filament:
             self.clamp_m2 = self.clamp_m1.copy()
             self.clamp_m1 = self.clamp.copy()
             self.clamp = t
-        # accept_state_from_rigid_body with a fixed base (body Lie algebra 0)
-        self.x[0] = self.base
-        du = -self.u[0]
-        self.u[0] = self.u[0] + du
+        self.accept_state_from_rigid_body(base, ub)
+
+    def accept_state_from_rigid_body(self, base=None, ub=None):
+        """Segment 0 at the base with the body's Lie-algebra element (filament.cpp:653-678)."""
+        self.x[0] = self.base if base is None else np.asarray(base, float)
+        self.u[0] = np.zeros(3) if ub is None else np.asarray(ub, float).copy()
         self.q[0] = qmul(lie_exp(self.u[0]), self.qm1[0])
         self.robot_arm()
 
-    def update(self, upd):
+    def update(self, upd, base=None, ub=None):
+        """filament::update (filament.cpp:2174-2208); with a body, first
+        accept_state_from_rigid_body (swimmer.cpp:1076-1100)."""
         n = self.P.nseg
+        if base is not None:
+            self.accept_state_from_rigid_body(base, ub)
         upd = np.asarray(upd, float).reshape(2 * n, 3)
         self.lam[0] += upd[0]
         for i in range(1, n):
