@@ -58,6 +58,12 @@ struct BodyState {
   int schur_gen = -1;       // dense-inverse generation PK and Sinv belong to
   DevBuf<double> ref_dev;   // ref on the device (placement from device-held body states)
   DevBuf<double> frame;     // body-frame (K^T K)^-1 and N on the device (72)
+  // Frozen dense preconditioner (free swimmers): built once and applied in the frame it
+  // was built in, rotated by R(q) R(q_build)^T per body.
+  bool frozen = false, built_frozen = false;
+  DevBuf<double> q_cur, q_build;  // quaternions of the last placement / of the build (4 n_body)
+  DevBuf<double> r_build;         // offsets at the build ([3][n_blob])
+  DevBuf<double> rin, rout;       // rotated system vectors
 };
 
 BodyState* body_of(cfm_ctx* c) { return reinterpret_cast<BodyState*>(c->body); }
@@ -281,6 +287,54 @@ __global__ __launch_bounds__(256) void body_dense_finish_kernel(int n3, int ncol
   out[j] = s;
 }
 
+__device__ __forceinline__ void quat_rot(const double* q, double R[9]) {
+  const double s = q[0], a = q[1], b = q[2], c = q[3];
+  R[0] = 1.0 - 2.0 * (b * b + c * c);
+  R[1] = 2.0 * (a * b - s * c);
+  R[2] = 2.0 * (a * c + s * b);
+  R[3] = 2.0 * (a * b + s * c);
+  R[4] = 1.0 - 2.0 * (a * a + c * c);
+  R[5] = 2.0 * (b * c - s * a);
+  R[6] = 2.0 * (a * c - s * b);
+  R[7] = 2.0 * (b * c + s * a);
+  R[8] = 1.0 - 2.0 * (a * a + b * b);
+}
+
+// out = G in (transpose: G^T in) for a body-system vector, G = R(q_b) R(q0_b)^T on every
+// blob 3-vector and on both 3-vectors (V, Omega) of every body.
+__global__ __launch_bounds__(256) void body_rotate_kernel(int m, int nb, int nbody,
+                                                          const double* __restrict__ qc,
+                                                          const double* __restrict__ q0, int transpose,
+                                                          const double* __restrict__ in,
+                                                          double* __restrict__ out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= m + 2 * nbody) return;
+  int b;
+  size_t i0, st;
+  if (t < m) {
+    b = t / nb;
+    i0 = t;
+    st = m;
+  } else {
+    const int k = t - m;
+    b = k / 2;
+    i0 = 3 * (size_t)m + 6 * b + 3 * (k % 2);
+    st = 1;
+  }
+  double Rc[9], R0[9], G[9];
+  quat_rot(qc + 4 * b, Rc);
+  quat_rot(q0 + 4 * b, R0);
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j)
+      G[3 * i + j] = Rc[3 * i] * R0[3 * j] + Rc[3 * i + 1] * R0[3 * j + 1] + Rc[3 * i + 2] * R0[3 * j + 2];
+  const double v0 = in[i0], v1 = in[i0 + st], v2 = in[i0 + 2 * st];
+  for (int i = 0; i < 3; ++i) {
+    const double o = transpose ? G[i] * v0 + G[3 + i] * v1 + G[6 + i] * v2
+                               : G[3 * i] * v0 + G[3 * i + 1] * v1 + G[3 * i + 2] * v2;
+    out[i0 + i * st] = o;
+  }
+}
+
 // Blob positions and rotated offsets from body centres / orientations on the device.
 __global__ __launch_bounds__(256) void body_place_kernel(int m, int nb, const double* __restrict__ ref,
                                                          const double* __restrict__ bx,
@@ -384,7 +438,8 @@ Status build_schur(cfm_ctx* c, BodyState* S) {
 }
 
 void body_release_state(BodyState* S) {
-  for (auto* b : {&S->r, &S->mats, &S->w, &S->aux, &S->kcol, &S->PK, &S->Sinv, &S->ref_dev, &S->frame})
+  for (auto* b : {&S->r, &S->mats, &S->w, &S->aux, &S->kcol, &S->PK, &S->Sinv, &S->ref_dev, &S->frame,
+                 &S->q_cur, &S->q_build, &S->r_build, &S->rin, &S->rout})
     b->release();
 }
 
@@ -432,16 +487,45 @@ Status body_precond(cfm_ctx* c, int kind, const double* x, double* y) {
     return {};
   }
   if (kind == CFM_PC_DENSE) {
-    if (!c->dinv.valid || S->schur_gen != c->dinv.generation) CFM_TRY(build_schur(c, S));
+    const bool reuse = S->frozen && S->built_frozen;
+    if (!reuse && (!c->dinv.valid || S->schur_gen != c->dinv.generation)) {
+      CFM_TRY(build_schur(c, S));
+      if (S->frozen) {  // remember the frame the preconditioner belongs to
+        CFM_TRY(S->q_build.alloc(4 * (size_t)S->n_body));
+        CFM_TRY(S->r_build.alloc(3 * (size_t)m));
+        CFM_HIP(hipMemcpyAsync(S->q_build.p, S->q_cur.p, sizeof(double) * 4 * S->n_body,
+                               hipMemcpyDeviceToDevice, c->stream));
+        CFM_HIP(hipMemcpyAsync(S->r_build.p, S->r.p, sizeof(double) * 3 * m, hipMemcpyDeviceToDevice,
+                               c->stream));
+        S->built_frozen = true;
+      }
+    }
+    const double* xin = x;
+    double* yout = y;
+    const double* roff = S->r.p;
+    const int nrot = m + 2 * S->n_body;
+    if (S->frozen) {  // into the frame of the build
+      CFM_TRY(S->rin.alloc(L));
+      CFM_TRY(S->rout.alloc(L));
+      hipLaunchKernelGGL(body_rotate_kernel, g1(nrot), dim3(256), 0, c->stream, m, S->nb, S->n_body,
+                         S->q_cur.p, S->q_build.p, 1, x, S->rin.p);
+      xin = S->rin.p;
+      yout = S->rout.p;
+      roff = S->r_build.p;
+    }
     const int nc = 6 * S->n_body;
     CFM_TRY(S->w.alloc(3 * (size_t)m));
     CFM_TRY(S->aux.alloc(nc));
-    CFM_TRY(dense_apply(c, c->dinv, x, S->w.p));  // w = P f
-    hipLaunchKernelGGL(body_kt_kernel, dim3(S->n_body), dim3(256), 0, c->stream, 2, m, S->nb, S->r.p,
-                       S->w.p, xU, (const double*)nullptr, S->aux.p);  // -u - K^T w
-    hipLaunchKernelGGL(body_small_gemv_kernel, g1(nc), dim3(256), 0, c->stream, nc, S->Sinv.p, S->aux.p, yU);
+    CFM_TRY(dense_apply(c, c->dinv, xin, S->w.p));  // w = P f
+    hipLaunchKernelGGL(body_kt_kernel, dim3(S->n_body), dim3(256), 0, c->stream, 2, m, S->nb, roff,
+                       S->w.p, xin + 3 * (size_t)m, (const double*)nullptr, S->aux.p);  // -u - K^T w
+    hipLaunchKernelGGL(body_small_gemv_kernel, g1(nc), dim3(256), 0, c->stream, nc, S->Sinv.p, S->aux.p,
+                       yout + 3 * (size_t)m);
     hipLaunchKernelGGL(body_dense_finish_kernel, g1(3 * m), dim3(256), 0, c->stream, 3 * m, nc, S->w.p,
-                       S->PK.p, yU, y);
+                       S->PK.p, yout + 3 * (size_t)m, yout);
+    if (S->frozen)  // back to the current frame
+      hipLaunchKernelGGL(body_rotate_kernel, g1(nrot), dim3(256), 0, c->stream, m, S->nb, S->n_body,
+                         S->q_cur.p, S->q_build.p, 0, (const double*)S->rout.p, y);
     CFM_CHECK_LAUNCH();
     return {};
   }
@@ -479,14 +563,26 @@ Status body_place_device(cfm_ctx* c, const double* bx, const double* bq) {
                      S->r.p, c->p.x.p + off, c->p.y.p + off, c->p.z.p + off);
   hipLaunchKernelGGL(body_mats_kernel, g1(72 * S->n_body), dim3(256), 0, c->stream, S->n_body, S->frame.p,
                      bq, S->mats.p);
+  CFM_TRY(S->q_cur.alloc(4 * (size_t)S->n_body));
+  CFM_HIP(hipMemcpyAsync(S->q_cur.p, bq, sizeof(double) * 4 * S->n_body, hipMemcpyDeviceToDevice, c->stream));
   CFM_CHECK_LAUNCH();
-  // cached blob-position products no longer apply
+  // cached blob-position products no longer apply (a frozen dense preconditioner is
+  // applied rotated instead)
   c->blob_pos.clear();
   c->nf.valid = false;
-  c->dinv.valid = false;
-  S->schur_gen = -1;
+  if (!(S->frozen && S->built_frozen)) {
+    c->dinv.valid = false;
+    S->schur_gen = -1;
+  }
   S->placed = true;
   return {};
+}
+
+void body_freeze_dense(cfm_ctx* c, bool on) {
+  BodyState* S = body_of(c);
+  if (!S) return;
+  S->frozen = on;
+  S->built_frozen = false;
 }
 
 // Blob positions changed behind the bodies' back (cfm_set_positions): the body
@@ -596,6 +692,8 @@ int cfm_body_set_state(cfm_ctx* c, const double* x, const double* q) {
                            c->stream));
     S->x.assign(x, x + 3 * (size_t)B);
     S->q = qn;
+    CFM_TRY(S->q_cur.alloc(4 * (size_t)B));
+    CFM_HIP(hipMemcpyAsync(S->q_cur.p, qn.data(), sizeof(double) * 4 * B, hipMemcpyHostToDevice, c->stream));
     CFM_TRY(upload_mats(c, S));
     return {};
   };
@@ -608,6 +706,7 @@ int cfm_body_set_state(cfm_ctx* c, const double* x, const double* q) {
   }
   if (c->p.n_seg == 0) c->have_positions = true;
   S->schur_gen = -1;  // K changed with the offsets
+  S->built_frozen = false;
   S->placed = true;
   return CFM_OK;
 }

@@ -16,6 +16,10 @@ Status body_info(cfm_ctx* c, BodyInfo* out);
 // Places every blob from body centres bx (3 n_body) and quaternions bq (4 n_body) held in
 // device memory, with the rotated reference matrices; invalidates caches of blob products.
 Status body_place_device(cfm_ctx* c, const double* bx, const double* bq);
+// Keep CFM_PC_DENSE of the body system across body motions: built once, then applied in
+// the frame it was built in (exact without a wall, where rigid motions leave the blob
+// mobility invariant up to rotation; a preconditioner above one).
+void body_freeze_dense(cfm_ctx* c, bool on);
 // Rotated blob offsets R(q_b) ref of the last placement ([3][n_blob], device), or null.
 const double* body_offsets(cfm_ctx* c);
 

@@ -648,9 +648,9 @@ int cfm_fil_attach_bodies(cfm_ctx* c, const double* f_ext) {
   if (c->cfg.a_blob != c->cfg.a_seg)
     return fail_c(c, make_status(CFM_EINVAL, "free swimmers need equal blob and segment radii"));
   const int pc = F->prm.blob_solve.precond;
-  if (pc != CFM_PC_NONE && pc != CFM_PC_DIAG && pc != CFM_PC_SELF)
-    return fail_c(c, make_status(CFM_EINVAL, "free swimmers take blob preconditioners NONE, DIAG or SELF "
-                                             "(the blob matrix changes as the bodies move)"));
+  if (pc == CFM_PC_NEAR)
+    return fail_c(c, make_status(CFM_EINVAL, "free swimmers take blob preconditioners NONE, DIAG, SELF "
+                                             "or DENSE"));
   if (f_ext)
     for (int i = 0; i < 6 * B; ++i)
       if (!std::isfinite(f_ext[i])) return fail_c(c, make_status(CFM_ENONFINITE, "non-finite external load"));
@@ -745,6 +745,7 @@ int cfm_fil_attach_bodies(cfm_ctx* c, const double* f_ext) {
     F->K.fpb = S.fpb;
     F->K.clampblk = F->sw_clamp.p;
     F->prm.guess = CFM_GUESS_PREVIOUS;  // the blob matrix changes as the bodies move
+    body_freeze_dense(c, pc == CFM_PC_DENSE);
     F->swim = true;
     CFM_TRY(body_place_device(c, S.x, S.q));
     CFM_HIP(hipStreamSynchronize(c->stream));

@@ -29,8 +29,11 @@ def _gpu(sw, P, wall, f_ext=None, precond="self"):
     return S
 
 
-@pytest.mark.parametrize("wall,precond", [(False, "self"), (True, "diag")])
+@pytest.mark.parametrize("wall,precond", [(False, "self"), (True, "diag"), (False, "dense"),
+                                          (True, "dense")])
 def test_free_swimmer_steps_match_oracle(wall, precond):
+    """CFM_PC_DENSE is built once and applied in the body frame: exact without the wall
+    (a couple of GMRES steps per solve), a preconditioner above it."""
     P, sw, _ = _swimmer(wall=wall)
     vel = _velocity(wall)
     with _gpu(sw, P, wall, precond=precond) as S:
@@ -39,6 +42,8 @@ def test_free_swimmer_steps_match_oracle(wall, precond):
             it = so.implicit_step(sw, nt, vel, wall=wall)
             assert st["converged"] and st["broyden_iterations"] == it, (nt, st, it)
             assert st["max_gmres_rel_residual"] <= 1e-12
+            if precond == "dense" and not wall:
+                assert st["gmres_iterations"] <= 3 * st["gmres_solves"], st
             g = S.state()
             xs = np.concatenate([f.x for f in sw.fils])
             assert np.abs(g["x"].reshape(-1, 3) - xs).max() < 1e-7, nt
