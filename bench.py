@@ -307,6 +307,49 @@ def run_free_body(args):
     return out
 
 
+def run_free_swimmer(args):
+    """Free swimmer: the headline workload with the sphere free to move (its position
+    and orientation among the Broyden unknowns, the blob solve carrying the body
+    velocities).  Plain: FP64 pair sums and the reference's block preconditioner (6 pi
+    mu a on the blobs, (K^T K)^-1 and the body mobility).  Fast: FP32C pair sums and the
+    dense Schur preconditioner built once in the body frame and applied rotated."""
+    import numpy as np
+
+    from cufcm_fil_amd import geometry as geo
+    out = {"workload": "sphere R=40a, 6000 blobs + 1300x20 elastic filaments, wall 50a "
+                       "below, body free (force- and torque-free swimmer); BDF2 step by "
+                       "Broyden, blob + body GMRES(50) rtol 1e-10"}
+    xs, xb, centre, dirs, tans = geo.sphere_over_wall(return_tangents=True)
+    ref = xb.reshape(-1, 3) - centre
+    mob = None
+    for label, prec, pc in [("fast", "fp32c", "dense"), ("plain", "fp64", "diag")]:
+        S = build_system(args, prec, "fp64", "previous", pc)
+        try:
+            t0 = time.perf_counter()
+            S.attach_bodies(ref.reshape(-1), centre, [1.0, 0.0, 0.0, 0.0], mobility=mob)
+            if mob is None:
+                mob = S.mob.body_reference_matrices()[1].reshape(-1)
+                out["reference_matrices_s"] = round(time.perf_counter() - t0, 3)
+            steps, stats = 2, []
+            S.step(0)
+            t0 = time.perf_counter()
+            for nt in range(1, 1 + steps):
+                stats.append(S.step(nt))
+            el = time.perf_counter() - t0
+            b = S.body_state()
+        finally:
+            S.close()
+        out[label] = {"steps_per_s": round(steps / el, 4), "ms_per_step": round(1e3 * el / steps, 2),
+                      "broyden_iterations": sum(s["broyden_iterations"] for s in stats),
+                      "gmres_iterations": sum(s["gmres_iterations"] for s in stats),
+                      "max_gmres_rel_residual": max(s["max_gmres_rel_residual"] for s in stats),
+                      "all_converged": all(s["converged"] for s in stats),
+                      "body_velocity": [round(v, 6) for v in b["vel"]]}
+        print(f"[bench] free swimmer {label}: {1e3 * el / steps:.1f} ms/step", file=sys.stderr, flush=True)
+    out["speedup"] = round(out["fast"]["steps_per_s"] / out["plain"]["steps_per_s"], 3)
+    return out
+
+
 def cpu_baseline(args, flops_per_step):
     """FP64 OpenMP oracle apply on a bounded sample of targets, extrapolated."""
     import numpy as np
@@ -354,6 +397,7 @@ def main():
     ap.add_argument("--skip-learned", action="store_true")
     ap.add_argument("--skip-cilia", action="store_true")
     ap.add_argument("--skip-body", action="store_true")
+    ap.add_argument("--skip-swimmer", action="store_true")
     ap.add_argument("--batch-reps", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--precond", default="dense", choices=["near", "dense"],
@@ -398,6 +442,7 @@ def main():
     learned = None if args.skip_learned else run_learned(args)
     cilia = None if args.skip_cilia else run_cilia(args)
     body = None if args.skip_body else run_free_body(args)
+    swimmer = None if args.skip_swimmer else run_free_swimmer(args)
     cpu = None
     if not args.skip_cpu:
         ref = plain if plain else mixed
@@ -433,6 +478,7 @@ def main():
         "learned": learned,
         "prescribed_cilia": cilia,
         "free_body": body,
+        "free_swimmer": swimmer,
     }
     print(json.dumps(out), flush=True)
 
