@@ -251,6 +251,7 @@ def lib():
                                         ctypes.c_double]
     L.cfm_cilia_get_speeds.argtypes = [_vp, _dp, _dp, _dp, _dp]
     L.cfm_cilia_set_s_table.argtypes = [_vp, _dp, ctypes.c_int32, ctypes.c_int32]
+    L.cfm_cilia_set_phases.argtypes = [_vp, _dp, _dp]
     L.cfm_batch_create.argtypes = [ctypes.POINTER(cfm_batch_config), _ip, ctypes.POINTER(_vp)]
     L.cfm_batch_destroy.argtypes = [_vp]
     L.cfm_batch_last_error.restype = ctypes.c_char_p

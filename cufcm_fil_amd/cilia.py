@@ -77,6 +77,21 @@ class PrescribedCilia:
         L.check(self.mob._h, L.lib().cfm_cilia_set_s_table(self.mob._h, L.ptr(t.reshape(-1)),
                                                            rows, cols))
 
+    def set_phases(self, phase=None, angle=None):
+        """Replace phases and / or shape angles (e.g. from a psi.dat state)."""
+        n = self.n_cilia
+        ph = None if phase is None else L.f64(phase, n)
+        an = None if angle is None else L.f64(angle, n)
+        L.check(self.mob._h, L.lib().cfm_cilia_set_phases(self.mob._h, L.ptr(ph), L.ptr(an)))
+
+    def write_outputs(self, writer, step, period=1.0):
+        """One line of the reference's _seg_states.dat (positions) and _true_states.dat
+        (period, phases, shape angles) through a simio.OutputWriter."""
+        st = self.state()
+        sp = self.speeds()
+        writer.states(step, st["x"].reshape(-1, 3))
+        writer.true_states(step, period, sp["phase"], sp["angle"])
+
     def speeds(self):
         n = self.n_cilia
         ph, pd, an, ad = (np.zeros(n) for _ in range(4))

@@ -758,6 +758,26 @@ int cfm_cilia_get_speeds(cfm_ctx* c, double* phase, double* phase_dot, double* a
   return s.ok() ? CFM_OK : cfail(c, s);
 }
 
+int cfm_cilia_set_phases(cfm_ctx* c, const double* phase, const double* angle) {
+  if (!c) return CFM_EINVAL;
+  CiliaState* S = cil_of(c);
+  if (!S) return cfail(c, make_status(CFM_ESTATE, "cfm_cilia_init has not been called"));
+  const int nc = S->prm.n_cilia;
+  for (int i = 0; i < nc; ++i)
+    if ((phase && !std::isfinite(phase[i])) || (angle && !std::isfinite(angle[i])))
+      return cfail(c, make_status(CFM_ENONFINITE, "non-finite phase or angle"));
+  (void)hipSetDevice(c->device);
+  auto run = [&]() -> Status {
+    if (phase) CFM_HIP(hipMemcpyAsync(S->phase.p, phase, sizeof(double) * nc, hipMemcpyHostToDevice, c->stream));
+    if (angle) CFM_HIP(hipMemcpyAsync(S->angle.p, angle, sizeof(double) * nc, hipMemcpyHostToDevice, c->stream));
+    CFM_TRY(geometry(c, S));
+    CFM_HIP(hipStreamSynchronize(c->stream));
+    return {};
+  };
+  Status s = run();
+  return s.ok() ? CFM_OK : cfail(c, s);
+}
+
 int cfm_cilia_get_state(cfm_ctx* c, double* x, double* vel_dir_phase, double* vel_dir_angle,
                         double* s_values, double* phase, double* f_seg) {
   if (!c) return CFM_EINVAL;

@@ -97,6 +97,16 @@ class FilamentSystem:
         L.check(self.h, L.lib().cfm_fil_attach_bodies(self.h, L.ptr(fe)))
         self.n_body = n_body
 
+    def write_outputs(self, writer, step):
+        """One line of the reference's _seg_states.dat (segment quaternions) and, with
+        bodies attached, _body_states.dat (centre, quaternion) through a simio.OutputWriter."""
+        st = self.state()
+        if getattr(self, "n_body", 0):
+            b = self.body_state()
+            writer.states(step, st["q"].reshape(-1, 4), b["x"], b["q"])
+        else:
+            writer.states(step, st["q"].reshape(-1, 4))
+
     def body_state(self):
         nb = self.n_body
         x, q, u, v = np.zeros(3 * nb), np.zeros(4 * nb), np.zeros(3 * nb), np.zeros(6 * nb)

@@ -350,6 +350,10 @@ int cfm_cilia_set_driving(cfm_ctx* ctx, const double* q_phase_table, int32_t n_p
  * 2 pi k / rows, cols recorded segments (>= 2), linear in phase and in segment position.
  * A null table returns to the bisection.  Re-places the segments. */
 int cfm_cilia_set_s_table(cfm_ctx* ctx, const double* table, int32_t rows, int32_t cols);
+/* Replaces the phases and / or shape angles (n_cilia each, either may be null) and
+ * re-places the segments: a run restarted from a recorded state (the psi.dat initial
+ * state the reference's drivers write, read at swimmer.cpp:43-60). */
+int cfm_cilia_set_phases(cfm_ctx* ctx, const double* phase, const double* angle);
 /* Phases, phase speeds, shape angles and their speeds (n_cilia each, may be null). */
 int cfm_cilia_get_speeds(cfm_ctx* ctx, double* phase, double* phase_dot, double* angle,
                          double* angle_dot);

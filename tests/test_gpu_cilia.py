@@ -200,3 +200,29 @@ def test_s_table_mode_interpolates_the_recorded_parameters():
         C.set_s_table(None)
         s_bis = C.state()["s_values"][1]
     assert np.array_equal(s_bis, co.s_values(1.234, 20, "fulford_blake_original"))
+
+
+def test_psi_restart_and_reference_output_files(tmp_path):
+    """A run restarted from a psi.dat state (phases and angles) writes the reference's
+    _seg_states.dat / _true_states.dat lines; they hold the library's own state."""
+    from cufcm_fil_amd import simio
+    base = wall_lattice(2, 2, spacing=30.0)
+    rng = np.random.default_rng(3)
+    ph, an = rng.uniform(0, 2 * math.pi, 4), 0.1 * rng.normal(size=4)
+    simio.write_psi(tmp_path / "psi.dat", 0.0, 1.0, ph, an)
+    k, T, ph2, an2 = simio.read_psi(tmp_path / "psi.dat", 4)
+    w = simio.OutputWriter(str(tmp_path) + "/", "run")
+    with PrescribedCilia(base, np.zeros(4)) as C:
+        C.set_phases(ph2, an2)
+        sp = C.speeds()
+        assert np.array_equal(sp["phase"], ph) and np.array_equal(sp["angle"], an)
+        x0, _, _, _ = co.placement(ph[1], base[3:6], angle=an[1])
+        assert np.abs(C.state()["x"][60:120] - x0).max() <= 1e-12 * np.abs(x0).max()
+        for nt in range(3):
+            C.step(nt)
+            C.write_outputs(w, nt + 1, period=T)
+        x = C.state()["x"]
+    steps, rows = simio.read_lines(w.path("seg_states"))
+    assert list(steps) == [1, 2, 3] and np.allclose(rows[-1], x, rtol=1e-14, atol=1e-14)
+    steps, rows = simio.read_lines(w.path("true_states"))
+    assert rows[-1][0] == T and np.allclose(rows[-1][1:5], ph + 2 * math.pi * 2 / 300, atol=1e-12)

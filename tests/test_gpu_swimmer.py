@@ -66,3 +66,20 @@ def test_external_load_moves_the_swimmer_as_the_oracle():
         b = S.body_state()
     assert np.abs(b["x"] - sw.body.x).max() < 1e-8
     assert b["vel"][2] < 0.0 and b["x"][2] < 40.0
+
+
+def test_swimmer_writes_reference_state_files(tmp_path):
+    from cufcm_fil_amd import simio
+    P, sw, wall = _swimmer()
+    w = simio.OutputWriter(str(tmp_path) + "/", "swim")
+    with _gpu(sw, P, wall) as S:
+        for nt in range(2):
+            S.step(nt)
+            S.write_outputs(w, nt + 1)
+        b = S.body_state()
+        q = S.state()["q"]
+    steps, rows = simio.read_lines(w.path("body_states"))
+    assert list(steps) == [1, 2] and rows[-1].size == 7
+    assert np.allclose(rows[-1], np.concatenate([b["x"], b["q"]]), rtol=1e-14, atol=1e-14)
+    steps, rows = simio.read_lines(w.path("seg_states"))
+    assert rows[-1].size == 4 * P.nseg * len(sw.fils) and np.allclose(rows[-1], q, atol=1e-14)
