@@ -115,3 +115,125 @@ class PrescribedCilia:
                                                          L.ptr(va), L.ptr(s), L.ptr(ph), L.ptr(f)))
         return {"x": x, "vel_dir_phase": vp, "vel_dir_angle": va,
                 "s_values": s.reshape(self.n_cilia, self.nseg), "phase": ph, "f": f}
+
+
+def _quat_from_matrix(R):
+    """Unit quaternion (w, x, y, z), w >= 0, of a rotation matrix (Shepperd's method)."""
+    t = np.trace(R)
+    if t > 0.0:
+        w = 0.5 * math.sqrt(1.0 + t)
+        q = np.array([w, (R[2, 1] - R[1, 2]) / (4 * w), (R[0, 2] - R[2, 0]) / (4 * w),
+                      (R[1, 0] - R[0, 1]) / (4 * w)])
+    else:
+        i = int(np.argmax(np.diag(R)))
+        j, k = (i + 1) % 3, (i + 2) % 3
+        r = math.sqrt(max(1.0 + R[i, i] - R[j, j] - R[k, k], 0.0))
+        v = np.zeros(3)
+        v[i] = 0.5 * r
+        v[j] = (R[j, i] + R[i, j]) / (2 * r)
+        v[k] = (R[k, i] + R[i, k]) / (2 * r)
+        w = (R[k, j] - R[j, k]) / (2 * r)
+        q = np.array([w, *v])
+    q /= np.linalg.norm(q)
+    return q if q[0] >= 0.0 else -q
+
+
+def cilium_frames(directions, tilt=0.0):
+    """Frame quaternions (w, x, y, z) of cilia whose bases point along `directions`
+    (body frame, n x 3): the first axis along the direction (the filament axis), the
+    second along the posterior polar direction -- the projection of -e_z -- turned by
+    `tilt` about the first (filament.cpp:370-432).  At the poles, where the polar
+    direction vanishes, the second axis is e_y."""
+    out = []
+    for d in np.asarray(directions, dtype=np.float64).reshape(-1, 3):
+        d = d / np.linalg.norm(d)
+        polar = np.array([0.0, 0.0, -1.0]) + d[2] * d
+        n = np.linalg.norm(polar)
+        e2 = polar / n if n > 0.0 else np.array([0.0, 1.0, 0.0])
+        e2 = math.cos(tilt) * e2 + math.sin(tilt) * np.cross(d, e2)
+        R = np.column_stack([d, e2, np.cross(d, e2)])
+        out.append(_quat_from_matrix(R))
+    return np.array(out).reshape(-1)
+
+
+class CiliatedBodies:
+    """n_body free rigid bodies of blobs, each carrying the same arrangement of
+    prescribed cilia (the reference's PRESCRIBED_CILIA swimmers with free bodies).
+
+    blob_ref: body-frame blob offsets of one body (3 nb); fil_ref, fil_q: base offsets
+    (3 per cilium) and frame quaternions (4 per cilium, see cilium_frames) of one body's
+    cilia; body_x, body_q: centres and unit quaternions (w, x, y, z).  Phases and phase
+    speeds cover all n_body x per cilia.  Every step solves for the forces, the bodies'
+    velocities and, with set_driving, the phase (and shape-angle) speeds; later steps
+    first move phases and bodies explicitly with the previous step's speeds."""
+
+    def __init__(self, blob_ref, fil_ref, fil_q, body_x, body_q, phase=None, phase_dot=None, *,
+                 nseg=20, dl=2.6, dt=1.0 / 300, wall=False, precision="fp64", solve_mode="fp64",
+                 precond="diag", rtol=1e-10, restart=50, max_iter=2000, device=0, tile_split=0,
+                 beat="fulford_blake", s_tol=0.0):
+        blob_ref = L.f64(blob_ref)
+        fil_ref, fil_q = L.f64(fil_ref), L.f64(fil_q)
+        body_x, body_q = L.f64(body_x), L.f64(body_q)
+        nbody, per, nb = body_x.size // 3, fil_ref.size // 3, blob_ref.size // 3
+        n = nbody * per
+        self.n_body, self.per_body, self.n_cilia, self.nseg, self.nb = nbody, per, n, nseg, nb
+        phase = np.zeros(n) if phase is None else L.f64(phase, n)
+        phase_dot = np.full(n, 2.0 * math.pi) if phase_dot is None else L.f64(phase_dot, n)
+        self.mob = RpyMobilitySolver(n * nseg, nbody * nb, seg_dof=3, wall=wall, precision=precision,
+                                     device=device, tile_split=tile_split)
+        # provisional positions: segments at the bases' bodies, blobs placed
+        R = [_rot(body_q[4 * b:4 * b + 4]) for b in range(nbody)]
+        x_blob = np.concatenate([body_x[3 * b:3 * b + 3] + blob_ref.reshape(-1, 3) @ R[b].T
+                                 for b in range(nbody)])
+        x_seg = np.repeat(body_x.reshape(-1, 3), per * nseg, axis=0)
+        self.mob.set_positions(x_seg, x_blob)
+        self.mob.body_init(blob_ref, nbody)
+        self.mob.body_set_state(body_x, body_q)
+        o = L.cfm_solve_opts(system=L.CFM_SYS_SEG, precond=RpyMobilitySolver._PC[precond],
+                             mode=RpyMobilitySolver._MODE[solve_mode], restart=restart,
+                             max_iter=max_iter, use_guess=1, rtol=rtol)
+        beats = {"fulford_blake": L.CFM_BEAT_FULFORD_BLAKE,
+                 "fulford_blake_original": L.CFM_BEAT_FULFORD_BLAKE_ORIGINAL}
+        self.prm = L.cfm_cilia_params(n_cilia=n, nseg=nseg, beat=beats[beat], dl=dl, dt=dt,
+                                      s_tol=s_tol, solve=o)
+        L.check(self.mob._h, L.lib().cfm_cilia_init_bodies(self.mob._h, ctypes.byref(self.prm),
+                                                           L.ptr(fil_ref), L.ptr(fil_q),
+                                                           L.ptr(phase), L.ptr(phase_dot)))
+
+    def close(self):
+        self.mob.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    set_driving = PrescribedCilia.set_driving
+    set_phases = PrescribedCilia.set_phases
+    speeds = PrescribedCilia.speeds
+    step = PrescribedCilia.step
+    state = PrescribedCilia.state
+
+    def bodies(self):
+        """Body centres, quaternions, velocities (V, Omega) of the last step, blob forces."""
+        nb = self.n_body
+        x, q, v = np.zeros(3 * nb), np.zeros(4 * nb), np.zeros(6 * nb)
+        fb = np.zeros(3 * nb * self.nb)
+        L.check(self.mob._h, L.lib().cfm_cilia_get_bodies(self.mob._h, L.ptr(x), L.ptr(q),
+                                                          L.ptr(v), L.ptr(fb)))
+        return {"x": x, "q": q, "vel": v, "f_blob": fb}
+
+    def write_outputs(self, writer, step, period=1.0):
+        """_seg_states.dat (segment positions), _body_states.dat (centre, quaternion) and
+        _true_states.dat lines through a simio.OutputWriter."""
+        st, sp, b = self.state(), self.speeds(), self.bodies()
+        writer.states(step, st["x"].reshape(-1, 3), b["x"], b["q"])
+        writer.true_states(step, period, sp["phase"], sp["angle"])
+
+
+def _rot(q):
+    s, a, b, c = q
+    return np.array([[1 - 2 * (b * b + c * c), 2 * (a * b - s * c), 2 * (a * c + s * b)],
+                     [2 * (a * b + s * c), 1 - 2 * (a * a + c * c), 2 * (b * c - s * a)],
+                     [2 * (a * c - s * b), 2 * (b * c + s * a), 1 - 2 * (a * a + b * b)]])

@@ -26,6 +26,8 @@
 
 #include "ctx.hpp"
 #include "krylov.hpp"
+#include "body.hpp"
+#include "fil_device.hpp"
 #include "precond.hpp"
 
 namespace cfm {
@@ -93,7 +95,13 @@ struct CiliaK {
   double s_tol;    // bisection tolerance on the arc-length fraction
   const double* s_table;  // recorded segment parameters (rows x cols) or null
   int s_rows, s_cols;
-  const double* base;    // 3 x ncil (SoA)
+  const double* base;    // 3 x ncil (SoA); unused for cilia on bodies
+  // cilia carried by free bodies (null bx: cilia on the wall in the fixed frame)
+  const double* bx;      // body centres, 3 per body
+  const double* bq;      // body quaternions, 4 per body
+  const double* fref;    // base offsets of one body's cilia (body frame), 3 each
+  const double* fq;      // frame quaternions of one body's cilia (body frame), 4 each
+  int per_body;          // cilia per body
   const double* phase;   // ncil
   const double* angle;   // ncil (rotation of the beat about the reference z axis)
   double *x, *y, *z;     // segment positions (the context's particle arrays)
@@ -153,6 +161,20 @@ __device__ double curve_length(const BeatCoeffs& cf, const double cv[4], const d
   return length;
 }
 
+// Rotation matrix (row-major) of a unit quaternion (w, x, y, z) (quaternion.cpp:305-347).
+__device__ inline void quat_to_mat(const double* q, double R[9]) {
+  const double s = q[0], a = q[1], b = q[2], c = q[3];
+  R[0] = 1.0 - 2.0 * (b * b + c * c);
+  R[1] = 2.0 * (a * b - s * c);
+  R[2] = 2.0 * (a * c + s * b);
+  R[3] = 2.0 * (a * b + s * c);
+  R[4] = 1.0 - 2.0 * (a * a + c * c);
+  R[5] = 2.0 * (b * c - s * a);
+  R[6] = 2.0 * (a * c - s * b);
+  R[7] = 2.0 * (b * c + s * a);
+  R[8] = 1.0 - 2.0 * (a * a + b * b);
+}
+
 __global__ void cilia_geometry_kernel(CiliaK k) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= k.ncil) return;
@@ -202,8 +224,30 @@ __global__ void cilia_geometry_kernel(CiliaK k) {
       s[i] = est;
     }
   }
-  // placement: reference x -> lab z, reference y -> lab y, reference z -> lab -x
-  const double bxp = k.base[c], byp = k.base[k.ncil + c], bzp = k.base[2 * k.ncil + c];
+  // frame of the cilium: its first two columns carry the reference x (filament axis)
+  // and y (beat direction) axes.  On the wall: reference x -> lab z, y -> lab y
+  // (filament.cpp:225-226 for dir = e_z).  On a body: R(q_body) R(q_ref) with the base
+  // at x_body + R(q_body) ref (swimmer.cpp:448-462, filament.cpp:653-663).
+  double e1[3] = {0.0, 0.0, 1.0}, e2[3] = {0.0, 1.0, 0.0};
+  double bxp, byp, bzp;
+  if (k.bx) {
+    const int b = c / k.per_body, l = c - b * k.per_body;
+    double Rb[9], Rf[9];
+    quat_to_mat(k.bq + 4 * b, Rb);
+    quat_to_mat(k.fq + 4 * l, Rf);
+    for (int r = 0; r < 3; ++r) {
+      e1[r] = Rb[3 * r] * Rf[0] + Rb[3 * r + 1] * Rf[3] + Rb[3 * r + 2] * Rf[6];
+      e2[r] = Rb[3 * r] * Rf[1] + Rb[3 * r + 1] * Rf[4] + Rb[3 * r + 2] * Rf[7];
+    }
+    const double* o = k.fref + 3 * l;
+    bxp = k.bx[3 * b] + Rb[0] * o[0] + Rb[1] * o[1] + Rb[2] * o[2];
+    byp = k.bx[3 * b + 1] + Rb[3] * o[0] + Rb[4] * o[1] + Rb[5] * o[2];
+    bzp = k.bx[3 * b + 2] + Rb[6] * o[0] + Rb[7] * o[1] + Rb[8] * o[2];
+  } else {
+    bxp = k.base[c];
+    byp = k.base[k.ncil + c];
+    bzp = k.base[2 * k.ncil + c];
+  }
   const double ca = cos(k.angle[c]), sa = sin(k.angle[c]);
   const size_t ns = k.nseg_total;
   double dc[4], ds[3];
@@ -230,19 +274,29 @@ __global__ void cilia_geometry_kernel(CiliaK k) {
     const double sx = combine(k.cf.ax, k.cf.bx, cv, sv, p);
     const double sy = combine(k.cf.ay, k.cf.by, cv, sv, p);
     const double rx = k.len * (ca * sx - sa * sy), ry = k.len * (sa * sx + ca * sy);
-    k.x[g] = bxp;
-    k.y[g] = byp + ry;
-    k.z[g] = bzp + rx;
-    // e_z x ref in the reference frame is (-ry, rx, 0) -> lab (0, rx, -ry)
-    k.vangle[g] = 0.0;
-    k.vangle[ns + g] = rx;
-    k.vangle[2 * ns + g] = -ry;
     const double dx = combine(k.cf.ax, k.cf.bx, dc, ds, p);
     const double dy = combine(k.cf.ay, k.cf.by, dc, ds, p);
     const double ux = k.len * (ca * dx - sa * dy), uy = k.len * (sa * dx + ca * dy);
-    k.vphase[g] = 0.0;
-    k.vphase[ns + g] = uy;
-    k.vphase[2 * ns + g] = ux;
+    if (!k.bx) {  // wall frame written out: lab (0, ry, rx)
+      k.x[g] = bxp;
+      k.y[g] = byp + ry;
+      k.z[g] = bzp + rx;
+      // e_z x ref in the reference frame is (-ry, rx, 0) -> lab (0, rx, -ry)
+      k.vangle[g] = 0.0;
+      k.vangle[ns + g] = rx;
+      k.vangle[2 * ns + g] = -ry;
+      k.vphase[g] = 0.0;
+      k.vphase[ns + g] = uy;
+      k.vphase[2 * ns + g] = ux;
+      continue;
+    }
+    k.x[g] = bxp + e1[0] * rx + e2[0] * ry;
+    k.y[g] = byp + e1[1] * rx + e2[1] * ry;
+    k.z[g] = bzp + e1[2] * rx + e2[2] * ry;
+    for (int d = 0; d < 3; ++d) {
+      k.vangle[d * ns + g] = e2[d] * rx - e1[d] * ry;
+      k.vphase[d * ns + g] = e1[d] * ux + e2[d] * uy;
+    }
   }
 }
 
@@ -386,22 +440,24 @@ __device__ double cyclic_hermite(const double* tab, int n, double phase) {
 // rhs = [0 ; -q_phase ; -q_angle] with the recorded driving forces scaled by the
 // cilium's natural frequency and the linear torsional spring on the shape angle
 // (mobility_solver.cpp:301-356, FORCE_NOISE_MAG = 0, FENE_MODEL = 0).
-__global__ void cilia_drive_kernel(int ncil, int m, int dyn_angle, double len, double spring,
+// The phase slots start at `off` (after the forces, and the body velocities when the
+// cilia ride on free bodies); everything before them is zero.
+__global__ void cilia_drive_kernel(int ncil, int off, int dyn_angle, double len, double spring,
                                    const double* __restrict__ phase, const double* __restrict__ angle,
                                    const double* __restrict__ omega0,
                                    const double* __restrict__ tab_p, int np,
                                    const double* __restrict__ tab_a, int na,
                                    double* __restrict__ rhs) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int L = 3 * m + ncil * (dyn_angle ? 2 : 1);
+  const int L = off + ncil * (dyn_angle ? 2 : 1);
   if (i >= L) return;
-  if (i < 3 * m) {
+  if (i < off) {
     rhs[i] = 0.0;
     return;
   }
-  const int c = (i - 3 * m) % ncil;
+  const int c = (i - off) % ncil;
   const double sc = 0.5 * omega0[c] / 3.14159265358979323846;
-  if (i < 3 * m + ncil) {
+  if (i < off + ncil) {
     rhs[i] = -cyclic_hermite(tab_p, np, phase[c]) * sc;
   } else {
     const double q = cyclic_hermite(tab_a, na, phase[c]) * sc - spring * omega0[c] * len * len * len * angle[c];
@@ -427,6 +483,404 @@ __global__ void cilia_advance_angle_kernel(int ncil, double dt, const double* __
   if (c < ncil) angle[c] += adot[c] * dt;
 }
 
+// ---- cilia carried by free rigid bodies ---------------------------------------------
+// System of the reference's PRESCRIBED_CILIA swimmers with free bodies
+// (mobility_solver.cpp:1891-2115): unknowns [F: 3 x M by component, M = n_seg + n_blob]
+// ++ [V, Omega per body] ++ [phase speeds] ++ [angle speeds];
+//   rows of particle i of body b:  (M F)_i - (V_b + Omega_b x r_i) - phi_dot k_phi,i - theta_dot k_theta,i
+//   rows of body b:                 -sum_i F_i, -sum_i r_i x F_i
+//   rows of cilium c:               -k_phi . F, -k_theta . F
+// with r_i = x_i - x_b.  The block preconditioner inverts this saddle matrix exactly
+// with M^-1 replaced by the particle preconditioner P (6 pi mu a: the reference's
+// mobility_solver.cpp:1250-1515 with make_precon_mat, swimmer.cpp:1175-1347).
+
+struct BodyCiliaK {
+  int ncil, nseg, nseg_total, nb, nbody, per_body;
+  int M;  // particles of the system: segments, then blobs
+  int dynamic, dyn_angle;
+  const double *px, *py, *pz;  // particle positions
+  const double* bx;            // body centres, 3 per body
+  const double* vphase;        // [3][n_seg]
+  const double* vangle;        // [3][n_seg]
+  int pc_kind, wall;
+  double mu, a_seg, a_blob;
+  double* pc_c;  // per cilium (15): S_cc^-1 (i11, i12, i22), B_phi (6), B_theta (6)
+  double* pc_b;  // per body (36): Schur complement inverse, row-major
+};
+
+__device__ inline int cb_body(const BodyCiliaK& k, int i) {
+  return i < k.nseg_total ? (i / k.nseg) / k.per_body : (i - k.nseg_total) / k.nb;
+}
+
+// Diagonal of the 3-DOF particle preconditioner (6 pi mu a, or the wall self-block).
+__device__ inline void cb_pdiag(const BodyCiliaK& k, int i, double p[3]) {
+  const double a = i < k.nseg_total ? k.a_seg : k.a_blob;
+  for (int d = 0; d < 3; ++d) {
+    double e[3] = {0.0, 0.0, 0.0}, o[3];
+    e[d] = 1.0;
+    particle_precond(k.pc_kind, k.wall, 3, k.pz[i], a, k.mu, e, o);
+    p[d] = o[d];
+  }
+}
+
+// Particle i of body b: the body's particles are its cilia's segments, then its blobs.
+__device__ inline int cb_particle(const BodyCiliaK& k, int b, int t) {
+  const int ns = k.per_body * k.nseg;
+  return t < ns ? b * ns + t : k.nseg_total + b * k.nb + (t - ns);
+}
+
+// Sum over a 256-thread block; the result is valid in thread 0.
+template <int NV>
+__device__ inline void cb_block_sum(double (&v)[NV]) {
+  __shared__ double part[4][NV];
+  for (int j = 0; j < NV; ++j)
+    for (int o = 32; o > 0; o >>= 1) v[j] += __shfl_down(v[j], o);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0)
+    for (int j = 0; j < NV; ++j) part[w][j] = v[j];
+  __syncthreads();
+  if (threadIdx.x == 0)
+    for (int j = 0; j < NV; ++j) v[j] = (part[0][j] + part[1][j]) + (part[2][j] + part[3][j]);
+  __syncthreads();
+}
+
+// y_i -= V_b + Omega_b x r_i (+ phi_dot k_phi + theta_dot k_theta on segments)
+__global__ __launch_bounds__(256) void cb_couple_particles_kernel(BodyCiliaK k, const double* __restrict__ x,
+                                                                  double* __restrict__ y) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= k.M) return;
+  const size_t M = k.M;
+  const int b = cb_body(k, i);
+  const double* U = x + 3 * M + 6 * (size_t)b;
+  const double r0 = k.px[i] - k.bx[3 * b], r1 = k.py[i] - k.bx[3 * b + 1], r2 = k.pz[i] - k.bx[3 * b + 2];
+  double u0 = U[0] + (U[4] * r2 - U[5] * r1);
+  double u1 = U[1] + (U[5] * r0 - U[3] * r2);
+  double u2 = U[2] + (U[3] * r1 - U[4] * r0);
+  if (k.dynamic && i < k.nseg_total) {
+    const size_t ph = 3 * M + 6 * (size_t)k.nbody, ns = k.nseg_total;
+    const int c = i / k.nseg;
+    const double wp = x[ph + c];
+    u0 += wp * k.vphase[i];
+    u1 += wp * k.vphase[ns + i];
+    u2 += wp * k.vphase[2 * ns + i];
+    if (k.dyn_angle) {
+      const double wa = x[ph + k.ncil + c];
+      u0 += wa * k.vangle[i];
+      u1 += wa * k.vangle[ns + i];
+      u2 += wa * k.vangle[2 * ns + i];
+    }
+  }
+  y[i] -= u0;
+  y[M + i] -= u1;
+  y[2 * M + i] -= u2;
+}
+
+// y_phi_c = -k_phi . F, y_theta_c = -k_theta . F
+__global__ void cb_couple_cilia_kernel(BodyCiliaK k, const double* __restrict__ x, double* __restrict__ y) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= k.ncil) return;
+  const size_t M = k.M, ns = k.nseg_total, ph = 3 * M + 6 * (size_t)k.nbody;
+  double dp = 0.0, da = 0.0;
+  for (int i = 0; i < k.nseg; ++i) {
+    const size_t g = (size_t)c * k.nseg + i;
+    for (int d = 0; d < 3; ++d) {
+      const double f = x[d * M + g];
+      dp += k.vphase[d * ns + g] * f;
+      da += k.vangle[d * ns + g] * f;
+    }
+  }
+  y[ph + c] = -dp;
+  if (k.dyn_angle) y[ph + k.ncil + c] = -da;
+}
+
+// y_body = -(sum F_i, sum r_i x F_i) over the body's particles (one block per body)
+__global__ __launch_bounds__(256) void cb_couple_body_kernel(BodyCiliaK k, const double* __restrict__ x,
+                                                             double* __restrict__ y) {
+  const int b = blockIdx.x;
+  const size_t M = k.M;
+  const int cnt = k.per_body * k.nseg + k.nb;
+  const double c0 = k.bx[3 * b], c1 = k.bx[3 * b + 1], c2 = k.bx[3 * b + 2];
+  double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int t = threadIdx.x; t < cnt; t += blockDim.x) {
+    const int i = cb_particle(k, b, t);
+    const double f0 = x[i], f1 = x[M + i], f2 = x[2 * M + i];
+    const double r0 = k.px[i] - c0, r1 = k.py[i] - c1, r2 = k.pz[i] - c2;
+    acc[0] += f0;
+    acc[1] += f1;
+    acc[2] += f2;
+    acc[3] += r1 * f2 - r2 * f1;
+    acc[4] += r2 * f0 - r0 * f2;
+    acc[5] += r0 * f1 - r1 * f0;
+  }
+  cb_block_sum<6>(acc);
+  if (threadIdx.x == 0)
+    for (int d = 0; d < 6; ++d) y[3 * M + 6 * (size_t)b + d] = -acc[d];
+}
+
+// Per cilium: S_cc = (k_phi, k_theta)^T P (k_phi, k_theta) inverted, and
+// B = K_b^T P (k_phi, k_theta), K_b^T f = (f, r x f) (make_precon_mat's C^-1, v and u sums).
+__global__ void cb_pc_cilia_prep_kernel(BodyCiliaK k) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= k.ncil) return;
+  const int b = c / k.per_body;
+  const size_t ns = k.nseg_total;
+  double c11 = 0.0, c12 = 0.0, c22 = 0.0, Bp[6] = {0, 0, 0, 0, 0, 0}, Ba[6] = {0, 0, 0, 0, 0, 0};
+  for (int i = 0; i < k.nseg; ++i) {
+    const int g = c * k.nseg + i;
+    double p[3];
+    cb_pdiag(k, g, p);
+    const double r[3] = {k.px[g] - k.bx[3 * b], k.py[g] - k.bx[3 * b + 1], k.pz[g] - k.bx[3 * b + 2]};
+    double kp[3], ka[3], pp[3], pa[3];
+    for (int d = 0; d < 3; ++d) {
+      kp[d] = k.vphase[d * ns + g];
+      ka[d] = k.vangle[d * ns + g];
+      pp[d] = p[d] * kp[d];
+      pa[d] = p[d] * ka[d];
+      c11 += kp[d] * pp[d];
+      c12 += kp[d] * pa[d];
+      c22 += ka[d] * pa[d];
+      Bp[d] += pp[d];
+      Ba[d] += pa[d];
+    }
+    Bp[3] += r[1] * pp[2] - r[2] * pp[1];
+    Bp[4] += r[2] * pp[0] - r[0] * pp[2];
+    Bp[5] += r[0] * pp[1] - r[1] * pp[0];
+    Ba[3] += r[1] * pa[2] - r[2] * pa[1];
+    Ba[4] += r[2] * pa[0] - r[0] * pa[2];
+    Ba[5] += r[0] * pa[1] - r[1] * pa[0];
+  }
+  double* o = k.pc_c + 15 * (size_t)c;
+  if (k.dyn_angle) {
+    const double det = c11 * c22 - c12 * c12;
+    o[0] = c22 / det;
+    o[1] = -c12 / det;
+    o[2] = c11 / det;
+  } else {
+    o[0] = 1.0 / c11;
+    o[1] = 0.0;
+    o[2] = 0.0;
+  }
+  for (int d = 0; d < 6; ++d) {
+    o[3 + d] = Bp[d];
+    o[9 + d] = Ba[d];
+  }
+}
+
+// In-place inverse of a 6 x 6 row-major matrix (Gauss-Jordan, partial pivoting).
+__device__ inline void cb_invert6(double* A) {
+  double I[36];
+  for (int i = 0; i < 36; ++i) I[i] = (i % 7 == 0) ? 1.0 : 0.0;
+  for (int col = 0; col < 6; ++col) {
+    int piv = col;
+    for (int r = col + 1; r < 6; ++r)
+      if (fabs(A[6 * r + col]) > fabs(A[6 * piv + col])) piv = r;
+    if (piv != col)
+      for (int j = 0; j < 6; ++j) {
+        double t = A[6 * col + j];
+        A[6 * col + j] = A[6 * piv + j];
+        A[6 * piv + j] = t;
+        t = I[6 * col + j];
+        I[6 * col + j] = I[6 * piv + j];
+        I[6 * piv + j] = t;
+      }
+    const double inv = 1.0 / A[6 * col + col];
+    for (int j = 0; j < 6; ++j) {
+      A[6 * col + j] *= inv;
+      I[6 * col + j] *= inv;
+    }
+    for (int r = 0; r < 6; ++r) {
+      if (r == col) continue;
+      const double f = A[6 * r + col];
+      for (int j = 0; j < 6; ++j) {
+        A[6 * r + j] -= f * A[6 * col + j];
+        I[6 * r + j] -= f * I[6 * col + j];
+      }
+    }
+  }
+  for (int i = 0; i < 36; ++i) A[i] = I[i];
+}
+
+// Per body: (K_b^T P K_b - sum_c B_c S_cc^-1 B_c^T)^-1 (one block per body).
+__global__ __launch_bounds__(256) void cb_pc_body_prep_kernel(BodyCiliaK k) {
+  const int b = blockIdx.x;
+  const int cnt = k.per_body * k.nseg + k.nb;
+  const double c0 = k.bx[3 * b], c1 = k.bx[3 * b + 1], c2 = k.bx[3 * b + 2];
+  double A[36];
+  for (int j = 0; j < 36; ++j) A[j] = 0.0;
+  for (int t = threadIdx.x; t < cnt; t += blockDim.x) {
+    const int i = cb_particle(k, b, t);
+    double p[3];
+    cb_pdiag(k, i, p);
+    const double r0 = k.px[i] - c0, r1 = k.py[i] - c1, r2 = k.pz[i] - c2;
+    // K_i = [I | -[r]x]: row d holds e_d and -([r]x)_d
+    const double K[3][6] = {{1.0, 0.0, 0.0, 0.0, r2, -r1},
+                            {0.0, 1.0, 0.0, -r2, 0.0, r0},
+                            {0.0, 0.0, 1.0, r1, -r0, 0.0}};
+    for (int a = 0; a < 6; ++a)
+      for (int e = 0; e < 6; ++e)
+        A[6 * a + e] += K[0][a] * p[0] * K[0][e] + K[1][a] * p[1] * K[1][e] + K[2][a] * p[2] * K[2][e];
+  }
+  cb_block_sum<36>(A);
+  if (threadIdx.x != 0) return;
+  if (k.dynamic) {
+    for (int l = 0; l < k.per_body; ++l) {
+      const double* o = k.pc_c + 15 * (size_t)(b * k.per_body + l);
+      const double *Bp = o + 3, *Ba = o + 9;
+      for (int a = 0; a < 6; ++a)
+        for (int e = 0; e < 6; ++e)
+          A[6 * a + e] -= o[0] * Bp[a] * Bp[e] + o[1] * (Bp[a] * Ba[e] + Ba[a] * Bp[e]) + o[2] * Ba[a] * Ba[e];
+    }
+  }
+  cb_invert6(A);
+  for (int j = 0; j < 36; ++j) k.pc_b[36 * (size_t)b + j] = A[j];
+}
+
+// mode 0: out_i = P r_i;  mode 1: out_i += P (V_b + Omega_b x r_i + phi k_phi + theta k_theta)
+__global__ __launch_bounds__(256) void cb_pc_particles_kernel(BodyCiliaK k, int mode, const double* __restrict__ r,
+                                                              double* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= k.M) return;
+  const size_t M = k.M;
+  double p[3];
+  cb_pdiag(k, i, p);
+  if (mode == 0) {
+    for (int d = 0; d < 3; ++d) out[d * M + i] = p[d] * r[d * M + i];
+    return;
+  }
+  const int b = cb_body(k, i);
+  const double* U = out + 3 * M + 6 * (size_t)b;
+  const double r0 = k.px[i] - k.bx[3 * b], r1 = k.py[i] - k.bx[3 * b + 1], r2 = k.pz[i] - k.bx[3 * b + 2];
+  double u[3] = {U[0] + (U[4] * r2 - U[5] * r1), U[1] + (U[5] * r0 - U[3] * r2), U[2] + (U[3] * r1 - U[4] * r0)};
+  if (k.dynamic && i < k.nseg_total) {
+    const size_t ph = 3 * M + 6 * (size_t)k.nbody, ns = k.nseg_total;
+    const int c = i / k.nseg;
+    const double wp = out[ph + c], wa = k.dyn_angle ? out[ph + k.ncil + c] : 0.0;
+    for (int d = 0; d < 3; ++d) u[d] += wp * k.vphase[d * ns + i] + (k.dyn_angle ? wa * k.vangle[d * ns + i] : 0.0);
+  }
+  for (int d = 0; d < 3; ++d) out[d * M + i] += p[d] * u[d];
+}
+
+// mode 0: (phi, theta)_C = S_cc^-1 t with t = -r_c - (k_phi, k_theta) . (P r);
+// mode 1: (phi, theta) = (phi, theta)_C - S_cc^-1 B_c^T U_b
+__global__ void cb_pc_cilia_kernel(BodyCiliaK k, int mode, const double* __restrict__ r, double* __restrict__ out) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= k.ncil) return;
+  const size_t M = k.M, ns = k.nseg_total, ph = 3 * M + 6 * (size_t)k.nbody;
+  const double* o = k.pc_c + 15 * (size_t)c;
+  if (mode == 0) {
+    double tp = -r[ph + c], ta = k.dyn_angle ? -r[ph + k.ncil + c] : 0.0;
+    for (int i = 0; i < k.nseg; ++i) {
+      const size_t g = (size_t)c * k.nseg + i;
+      for (int d = 0; d < 3; ++d) {
+        tp -= k.vphase[d * ns + g] * out[d * M + g];
+        if (k.dyn_angle) ta -= k.vangle[d * ns + g] * out[d * M + g];
+      }
+    }
+    out[ph + c] = o[0] * tp + o[1] * ta;
+    if (k.dyn_angle) out[ph + k.ncil + c] = o[1] * tp + o[2] * ta;
+    return;
+  }
+  const int b = c / k.per_body;
+  const double* U = out + 3 * M + 6 * (size_t)b;
+  double bp = 0.0, ba = 0.0;
+  for (int d = 0; d < 6; ++d) {
+    bp += o[3 + d] * U[d];
+    ba += o[9 + d] * U[d];
+  }
+  out[ph + c] -= o[0] * bp + o[1] * ba;
+  if (k.dyn_angle) out[ph + k.ncil + c] -= o[1] * bp + o[2] * ba;
+}
+
+// U_b = Sinv_b (-r_b - sum_i K_i^T (P r)_i - sum_c B_c (phi, theta)_C) (one block per body)
+__global__ __launch_bounds__(256) void cb_pc_body_kernel(BodyCiliaK k, const double* __restrict__ r,
+                                                         double* __restrict__ out) {
+  const int b = blockIdx.x;
+  const size_t M = k.M;
+  const int cnt = k.per_body * k.nseg + k.nb;
+  const double c0 = k.bx[3 * b], c1 = k.bx[3 * b + 1], c2 = k.bx[3 * b + 2];
+  double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int t = threadIdx.x; t < cnt; t += blockDim.x) {
+    const int i = cb_particle(k, b, t);
+    const double f0 = out[i], f1 = out[M + i], f2 = out[2 * M + i];
+    const double r0 = k.px[i] - c0, r1 = k.py[i] - c1, r2 = k.pz[i] - c2;
+    acc[0] += f0;
+    acc[1] += f1;
+    acc[2] += f2;
+    acc[3] += r1 * f2 - r2 * f1;
+    acc[4] += r2 * f0 - r0 * f2;
+    acc[5] += r0 * f1 - r1 * f0;
+  }
+  cb_block_sum<6>(acc);
+  if (threadIdx.x != 0) return;
+  const size_t ub = 3 * M + 6 * (size_t)b, ph = 3 * M + 6 * (size_t)k.nbody;
+  double t[6];
+  for (int d = 0; d < 6; ++d) t[d] = -r[ub + d] - acc[d];
+  if (k.dynamic) {
+    for (int l = 0; l < k.per_body; ++l) {
+      const int c = b * k.per_body + l;
+      const double* o = k.pc_c + 15 * (size_t)c;
+      const double wp = out[ph + c], wa = k.dyn_angle ? out[ph + k.ncil + c] : 0.0;
+      for (int d = 0; d < 6; ++d) t[d] -= o[3 + d] * wp + o[9 + d] * wa;
+    }
+  }
+  const double* Si = k.pc_b + 36 * (size_t)b;
+  for (int a = 0; a < 6; ++a) {
+    double v = 0.0;
+    for (int e = 0; e < 6; ++e) v += Si[6 * a + e] * t[e];
+    out[ub + a] = v;
+  }
+}
+
+// rhs with prescribed phase speeds: phase_dot k_phi on the segment rows, zero elsewhere
+__global__ void cb_rhs_kernel(int L, int M, int nseg_total, int nseg, const double* __restrict__ pdot,
+                              const double* __restrict__ vphase, double* __restrict__ rhs) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= L) return;
+  double v = 0.0;
+  if (i < 3 * M) {
+    const int d = i / M, j = i - d * M;
+    if (j < nseg_total) v = pdot[j / nseg] * vphase[(size_t)d * nseg_total + j];
+  }
+  rhs[i] = v;
+}
+
+// solution -> segment forces, blob forces, body velocities, phase and angle speeds
+__global__ void cb_unpack_kernel(BodyCiliaK k, const double* __restrict__ z, double* __restrict__ f,
+                                 double* __restrict__ fb, double* __restrict__ vb, double* __restrict__ pdot,
+                                 double* __restrict__ adot) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t M = k.M;
+  if (i < 3 * k.M) {
+    const int d = i / k.M, j = i - d * k.M;
+    if (j < k.nseg_total)
+      f[(size_t)d * k.nseg_total + j] = z[i];
+    else
+      fb[(size_t)d * (k.M - k.nseg_total) + (j - k.nseg_total)] = z[i];
+  }
+  if (i < 6 * k.nbody) vb[i] = z[3 * M + i];
+  if (k.dynamic && i < k.ncil) {
+    const size_t ph = 3 * M + 6 * (size_t)k.nbody;
+    pdot[i] = z[ph + i];
+    adot[i] = k.dyn_angle ? z[ph + k.ncil + i] : 0.0;
+  }
+}
+
+// Explicit body update after a step (cilia_sim_main.cu:724-740):
+// x += dt V, q = exp(dt Omega) q.
+__global__ void cb_advance_bodies_kernel(int nbody, double dt, const double* __restrict__ vb,
+                                         double* __restrict__ bx, double* __restrict__ bq) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nbody) return;
+  for (int d = 0; d < 3; ++d) bx[3 * b + d] += dt * vb[6 * b + d];
+  const fil::Q4 e = fil::lie_exp(fil::v3(dt * vb[6 * b + 3], dt * vb[6 * b + 4], dt * vb[6 * b + 5]));
+  const fil::Q4 q = {bq[4 * b], bq[4 * b + 1], bq[4 * b + 2], bq[4 * b + 3]};
+  const fil::Q4 r = fil::qmul(e, q);
+  bq[4 * b] = r.s;
+  bq[4 * b + 1] = r.x;
+  bq[4 * b + 2] = r.y;
+  bq[4 * b + 3] = r.z;
+}
+
 struct CiliaState {
   cfm_cilia_params prm{};
   DevBuf<double> base, phase, pdot, angle, vphase, vangle, svals, rhs, f, q;
@@ -441,7 +895,23 @@ struct CiliaState {
   // recorded segment parameters (cfm_cilia_set_s_table)
   DevBuf<double> s_table;
   int s_rows = 0, s_cols = 0;
+  // cilia carried by free bodies (cfm_cilia_init_bodies): body centres, quaternions and
+  // velocities, the cilia's body-frame bases and frames, blob forces, and the block
+  // preconditioner's per-cilium and per-body matrices
+  int n_body = 0, per_body = 0;
+  DevBuf<double> bx, bq, vbody, fref, fq, fblob, pc_c, pc_b;
+  bool pc_ready = false;
+  int pc_kind = -1;
 };
+
+// First phase-speed slot of the system vector, and the system length.
+static size_t phase_slot(const cfm_ctx* c, const CiliaState* S) {
+  return S->n_body > 0 ? 3 * (size_t)(c->p.n_seg + c->p.n_blob) + 6 * (size_t)S->n_body
+                       : 3 * (size_t)c->p.n_seg;
+}
+static size_t system_len(const cfm_ctx* c, const CiliaState* S) {
+  return phase_slot(c, S) + (S->dynamic ? (size_t)S->prm.n_cilia * (S->dyn_angle ? 2 : 1) : 0);
+}
 
 static CiliaState* cil_of(cfm_ctx* c) { return reinterpret_cast<CiliaState*>(c->cilia); }
 
@@ -458,6 +928,13 @@ static CiliaK kargs(cfm_ctx* c, CiliaState* S) {
   k.dl = S->prm.dl;
   k.len = S->prm.dl * (S->prm.nseg - 1);
   k.base = S->base.p;
+  if (S->n_body > 0) {
+    k.bx = S->bx.p;
+    k.bq = S->bq.p;
+    k.fref = S->fref.p;
+    k.fq = S->fq.p;
+    k.per_body = S->per_body;
+  }
   k.phase = S->phase.p;
   k.angle = S->angle.p;
   k.x = c->p.x.p;
@@ -476,6 +953,7 @@ static Status geometry(cfm_ctx* c, CiliaState* S) {
   CFM_CHECK_LAUNCH();
   c->have_positions = true;
   S->have_geometry = true;
+  S->pc_ready = false;
   return {};
 }
 
@@ -490,8 +968,69 @@ static CoupleK couple_args(cfm_ctx* c, CiliaState* S) {
   return k;
 }
 
+static BodyCiliaK body_cilia_args(cfm_ctx* c, CiliaState* S, int pc_kind) {
+  BodyCiliaK k{};
+  k.ncil = S->prm.n_cilia;
+  k.nseg = S->prm.nseg;
+  k.nseg_total = c->p.n_seg;
+  k.nbody = S->n_body;
+  k.per_body = S->per_body;
+  k.nb = c->p.n_blob / S->n_body;
+  k.M = c->p.n_seg + c->p.n_blob;
+  k.dynamic = S->dynamic ? 1 : 0;
+  k.dyn_angle = S->dyn_angle ? 1 : 0;
+  k.px = c->p.x.p;
+  k.py = c->p.y.p;
+  k.pz = c->p.z.p;
+  k.bx = S->bx.p;
+  k.vphase = S->vphase.p;
+  k.vangle = S->vangle.p;
+  k.pc_kind = pc_kind;
+  k.wall = c->cfg.wall;
+  k.mu = c->cfg.mu;
+  k.a_seg = c->cfg.a_seg;
+  k.a_blob = c->cfg.a_blob;
+  k.pc_c = S->pc_c.p;
+  k.pc_b = S->pc_b.p;
+  return k;
+}
+
+// The system of cilia on free bodies: forces of every particle, body velocities, speeds.
+static SystemDesc body_cilia_desc(cfm_ctx* c, CiliaState* S) {
+  SystemDesc sd;
+  sd.m = c->p.n_seg + c->p.n_blob;
+  sd.dof = 3;
+  sd.off = 0;
+  sd.which = CFM_FULL;
+  sd.a = c->cfg.a_seg;
+  sd.cilia = true;
+  sd.extra = 6 * S->n_body + (S->dynamic ? S->prm.n_cilia * (S->dyn_angle ? 2 : 1) : 0);
+  sd.L = 3 * sd.m + sd.extra;
+  return sd;
+}
+
+// make_precon_mat (swimmer.cpp:1175-1347) and the per-cilium blocks, for the current
+// geometry and the preconditioner kind of the step's solve.
+static Status body_cilia_prep(cfm_ctx* c, CiliaState* S, int kind) {
+  const BodyCiliaK k = body_cilia_args(c, S, kind);
+  hipLaunchKernelGGL(cb_pc_cilia_prep_kernel, gc(k.ncil), dim3(64), 0, c->stream, k);
+  hipLaunchKernelGGL(cb_pc_body_prep_kernel, dim3(k.nbody), dim3(256), 0, c->stream, k);
+  CFM_CHECK_LAUNCH();
+  S->pc_ready = true;
+  S->pc_kind = kind;
+  return {};
+}
+
 Status cilia_couple_apply(cfm_ctx* c, const double* x, double* y) {
   CiliaState* S = cil_of(c);
+  if (S && S->n_body > 0) {
+    const BodyCiliaK k = body_cilia_args(c, S, CFM_PC_DIAG);
+    hipLaunchKernelGGL(cb_couple_particles_kernel, dim3((k.M + 255) / 256), dim3(256), 0, c->stream, k, x, y);
+    if (k.dynamic) hipLaunchKernelGGL(cb_couple_cilia_kernel, gc(k.ncil), dim3(64), 0, c->stream, k, x, y);
+    hipLaunchKernelGGL(cb_couple_body_kernel, dim3(k.nbody), dim3(256), 0, c->stream, k, x, y);
+    CFM_CHECK_LAUNCH();
+    return {};
+  }
   if (!S || !S->dynamic) return make_status(CFM_ESTATE, "no coupled cilia system");
   hipLaunchKernelGGL(cilia_couple_kernel, gc(S->prm.n_cilia), dim3(64), 0, c->stream,
                      couple_args(c, S), x, y);
@@ -501,12 +1040,81 @@ Status cilia_couple_apply(cfm_ctx* c, const double* x, double* y) {
 
 Status cilia_precond(cfm_ctx* c, int kind, const double* x, double* y) {
   CiliaState* S = cil_of(c);
+  if (S && S->n_body > 0) {
+    if (kind == CFM_PC_NONE) {
+      CFM_HIP(hipMemcpyAsync(y, x, sizeof(double) * system_len(c, S), hipMemcpyDeviceToDevice, c->stream));
+      return {};
+    }
+    if (kind != CFM_PC_DIAG && kind != CFM_PC_SELF)
+      return make_status(CFM_EINVAL, "cilia on free bodies take CFM_PC_NONE, _DIAG or _SELF");
+    if (!S->pc_ready || S->pc_kind != kind) CFM_TRY(body_cilia_prep(c, S, kind));
+    const BodyCiliaK k = body_cilia_args(c, S, kind);
+    const dim3 gp((k.M + 255) / 256);
+    hipLaunchKernelGGL(cb_pc_particles_kernel, gp, dim3(256), 0, c->stream, k, 0, x, y);
+    if (k.dynamic) hipLaunchKernelGGL(cb_pc_cilia_kernel, gc(k.ncil), dim3(64), 0, c->stream, k, 0, x, y);
+    hipLaunchKernelGGL(cb_pc_body_kernel, dim3(k.nbody), dim3(256), 0, c->stream, k, x, y);
+    if (k.dynamic) hipLaunchKernelGGL(cb_pc_cilia_kernel, gc(k.ncil), dim3(64), 0, c->stream, k, 1, x, y);
+    hipLaunchKernelGGL(cb_pc_particles_kernel, gp, dim3(256), 0, c->stream, k, 1, x, y);
+    CFM_CHECK_LAUNCH();
+    return {};
+  }
   if (!S || !S->dynamic) return make_status(CFM_ESTATE, "no coupled cilia system");
   if (kind != CFM_PC_DIAG && kind != CFM_PC_SELF)
     return make_status(CFM_EINVAL, "the coupled cilia system takes CFM_PC_DIAG or CFM_PC_SELF");
   hipLaunchKernelGGL(cilia_precond_kernel, gc(S->prm.n_cilia), dim3(64), 0, c->stream,
                      couple_args(c, S), kind, c->cfg.mu, c->p.z.p, x, y);
   CFM_CHECK_LAUNCH();
+  return {};
+}
+
+// One step of cilia on free bodies: explicit update of phases, shape angles and bodies
+// with the previous step's speeds, geometry, block preconditioner, one solve.
+static Status step_bodies(cfm_ctx* c, CiliaState* S, int nt, double* q_phase, double* q_angle,
+                          cfm_solve_stats* st) {
+  const int nc = S->prm.n_cilia;
+  if (nt > 0) {
+    // phases before the step (filament.cpp:1340-1356); bodies after the previous step
+    // (cilia_sim_main.cu:724-740) -- the same state at this step's solve
+    hipLaunchKernelGGL(cilia_advance_kernel, gc(nc), dim3(64), 0, c->stream, nc, S->prm.dt, S->pdot.p,
+                       S->phase.p);
+    if (S->dyn_angle)
+      hipLaunchKernelGGL(cilia_advance_angle_kernel, gc(nc), dim3(64), 0, c->stream, nc, S->prm.dt,
+                         S->adot.p, S->angle.p);
+    hipLaunchKernelGGL(cb_advance_bodies_kernel, gc(S->n_body), dim3(64), 0, c->stream, S->n_body,
+                       S->prm.dt, S->vbody.p, S->bx.p, S->bq.p);
+    CFM_CHECK_LAUNCH();
+    CFM_TRY(body_place_device(c, S->bx.p, S->bq.p));
+  }
+  CFM_TRY(geometry(c, S));
+  const SystemDesc sd = body_cilia_desc(c, S);
+  cfm_solve_opts o = S->prm.solve;
+  o.use_guess = 1;  // previous step's solution
+  if (o.precond == CFM_PC_DIAG || o.precond == CFM_PC_SELF) CFM_TRY(body_cilia_prep(c, S, o.precond));
+  if (S->dynamic) {
+    hipLaunchKernelGGL(cilia_drive_kernel, dim3((sd.L + 255) / 256), dim3(256), 0, c->stream, nc,
+                       (int)phase_slot(c, S), S->dyn_angle ? 1 : 0, S->prm.dl * (S->prm.nseg - 1),
+                       S->spring, S->phase.p, S->angle.p, S->omega0.p, S->tab_phase.p, S->n_tab_phase,
+                       S->tab_angle.p, S->n_tab_angle, S->zrhs.p);
+  } else {
+    hipLaunchKernelGGL(cb_rhs_kernel, dim3((sd.L + 255) / 256), dim3(256), 0, c->stream, sd.L, sd.m,
+                       c->p.n_seg, S->prm.nseg, S->pdot.p, S->vphase.p, S->zrhs.p);
+  }
+  CFM_CHECK_LAUNCH();
+  if (!c->krylov) c->krylov = new Krylov();
+  cfm_solve_stats ss{};
+  CFM_TRY(solve(c, *c->krylov, sd, o, S->zrhs.p, S->z.p, &ss, nullptr));
+  const BodyCiliaK k = body_cilia_args(c, S, o.precond);
+  const int nu = std::max(3 * k.M, std::max(6 * k.nbody, nc));
+  hipLaunchKernelGGL(cb_unpack_kernel, dim3((nu + 255) / 256), dim3(256), 0, c->stream, k, S->z.p, S->f.p,
+                     S->fblob.p, S->vbody.p, S->pdot.p, S->adot.p);
+  hipLaunchKernelGGL(cilia_genforce_kernel, gc(nc), dim3(64), 0, c->stream, nc, S->prm.nseg, c->p.n_seg,
+                     S->vphase.p, S->vangle.p, S->f.p, S->q.p);
+  CFM_CHECK_LAUNCH();
+  CFM_HIP(hipMemcpyAsync(S->host, S->q.p, sizeof(double) * 2 * nc, hipMemcpyDeviceToHost, c->stream));
+  CFM_HIP(hipStreamSynchronize(c->stream));
+  if (q_phase) std::memcpy(q_phase, S->host, sizeof(double) * nc);
+  if (q_angle) std::memcpy(q_angle, S->host + nc, sizeof(double) * nc);
+  if (st) *st = ss;
   return {};
 }
 
@@ -519,7 +1127,8 @@ void cfm_cilia_release(cfm_ctx* c) {
   if (!S) return;
   for (auto* b : {&S->base, &S->phase, &S->pdot, &S->angle, &S->vphase, &S->vangle, &S->svals,
                   &S->rhs, &S->f, &S->q, &S->omega0, &S->adot, &S->tab_phase, &S->tab_angle,
-                  &S->z, &S->zrhs, &S->s_table})
+                  &S->z, &S->zrhs, &S->s_table, &S->bx, &S->bq, &S->vbody, &S->fref, &S->fq,
+                  &S->fblob, &S->pc_c, &S->pc_b})
     b->release();
   if (S->host) (void)hipHostFree(S->host);
   delete S;
@@ -533,30 +1142,40 @@ static int cfail(cfm_ctx* c, const Status& s) {
 
 extern "C" {
 
+// Parameter checks shared by both initialisations.
+static Status check_params(const cfm_ctx* c, const cfm_cilia_params* prm, const double* phase,
+                           const double* phase_dot) {
+  const Particles& p = c->p;
+  if (prm->beat != CFM_BEAT_FULFORD_BLAKE && prm->beat != CFM_BEAT_FULFORD_BLAKE_ORIGINAL)
+    return make_status(CFM_EINVAL, "unknown beat %d", prm->beat);
+  if (!(prm->s_tol >= 0.0) || prm->s_tol >= 0.5)
+    return make_status(CFM_EINVAL, "s_tol must lie in [0, 0.5) (0: 0.1 / nseg)");
+  if (prm->nseg < 2 || prm->nseg > kMaxCiliaSeg || prm->n_cilia <= 0 ||
+      (long long)prm->n_cilia * prm->nseg != p.n_seg)
+    return make_status(CFM_EINVAL, "n_cilia x nseg must equal the context's n_seg, 2 <= nseg <= %d",
+                       kMaxCiliaSeg);
+  if (p.seg_dof != 3) return make_status(CFM_EINVAL, "prescribed cilia need 3-DOF segments");
+  if (!(prm->dl > 0.0) || !(prm->dt > 0.0)) return make_status(CFM_EINVAL, "dl and dt must be positive");
+  if (prm->solve.precond < CFM_PC_NONE || prm->solve.precond > CFM_PC_SELF)
+    return make_status(CFM_EINVAL, "cilia solves take CFM_PC_NONE, _DIAG or _SELF");
+  for (int i = 0; i < prm->n_cilia; ++i)
+    if (!std::isfinite(phase[i]) || !std::isfinite(phase_dot[i]))
+      return make_status(CFM_EINVAL, "non-finite phase or phase speed");
+  return {};
+}
+
 int cfm_cilia_init(cfm_ctx* c, const cfm_cilia_params* prm, const double* base,
                    const double* phase, const double* phase_dot) {
   if (!c) return CFM_EINVAL;
   if (!prm || !base || !phase || !phase_dot)
     return cfail(c, make_status(CFM_EINVAL, "null argument"));
   const Particles& p = c->p;
-  if (prm->beat != CFM_BEAT_FULFORD_BLAKE && prm->beat != CFM_BEAT_FULFORD_BLAKE_ORIGINAL)
-    return cfail(c, make_status(CFM_EINVAL, "unknown beat %d", prm->beat));
-  if (!(prm->s_tol >= 0.0) || prm->s_tol >= 0.5)
-    return cfail(c, make_status(CFM_EINVAL, "s_tol must lie in [0, 0.5) (0: 0.1 / nseg)"));
-  if (prm->nseg < 2 || prm->nseg > kMaxCiliaSeg || prm->n_cilia <= 0 ||
-      (long long)prm->n_cilia * prm->nseg != p.n_seg)
-    return cfail(c, make_status(CFM_EINVAL, "n_cilia x nseg must equal the context's n_seg, 2 <= nseg <= %d",
-                                kMaxCiliaSeg));
-  if (p.n_blob != 0 || p.seg_dof != 3 || !c->cfg.wall)
+  const Status chk = check_params(c, prm, phase, phase_dot);
+  if (!chk.ok()) return cfail(c, chk);
+  if (p.n_blob != 0 || !c->cfg.wall)
     return cfail(c, make_status(CFM_EINVAL, "prescribed cilia need a wall context with 3-DOF segments and no blobs"));
-  if (!(prm->dl > 0.0) || !(prm->dt > 0.0))
-    return cfail(c, make_status(CFM_EINVAL, "dl and dt must be positive"));
-  if (prm->solve.precond < CFM_PC_NONE || prm->solve.precond > CFM_PC_SELF)
-    return cfail(c, make_status(CFM_EINVAL, "cilia solves take CFM_PC_NONE, _DIAG or _SELF"));
   const int nc = prm->n_cilia;
   for (int i = 0; i < nc; ++i) {
-    if (!std::isfinite(phase[i]) || !std::isfinite(phase_dot[i]))
-      return cfail(c, make_status(CFM_EINVAL, "non-finite phase or phase speed"));
     for (int d = 0; d < 3; ++d)
       if (!std::isfinite(base[3 * i + d])) return cfail(c, make_status(CFM_EINVAL, "non-finite base"));
     if (!(base[3 * i + 2] > 0.0))
@@ -602,6 +1221,109 @@ int cfm_cilia_init(cfm_ctx* c, const cfm_cilia_params* prm, const double* base,
   return CFM_OK;
 }
 
+int cfm_cilia_init_bodies(cfm_ctx* c, const cfm_cilia_params* prm, const double* fil_ref,
+                          const double* fil_q, const double* phase, const double* phase_dot) {
+  if (!c) return CFM_EINVAL;
+  if (!prm || !fil_ref || !fil_q || !phase || !phase_dot)
+    return cfail(c, make_status(CFM_EINVAL, "null argument"));
+  const Status chk = check_params(c, prm, phase, phase_dot);
+  if (!chk.ok()) return cfail(c, chk);
+  BodyInfo bi;
+  const Status bs = body_info(c, &bi);
+  if (!bs.ok()) return cfail(c, bs);
+  const int nc = prm->n_cilia;
+  if (nc % bi.n_body != 0)
+    return cfail(c, make_status(CFM_EINVAL, "n_cilia (%d) must be a multiple of the number of bodies (%d)",
+                                nc, bi.n_body));
+  const int per = nc / bi.n_body;
+  for (int l = 0; l < per; ++l) {
+    double qq = 0.0;
+    for (int d = 0; d < 4; ++d) {
+      if (!std::isfinite(fil_q[4 * l + d]) || (d < 3 && !std::isfinite(fil_ref[3 * l + d])))
+        return cfail(c, make_status(CFM_ENONFINITE, "non-finite cilium base or frame"));
+      qq += fil_q[4 * l + d] * fil_q[4 * l + d];
+    }
+    if (std::fabs(qq - 1.0) > 1e-8)
+      return cfail(c, make_status(CFM_EINVAL, "cilium frame quaternion %d is not a unit quaternion", l));
+  }
+  (void)hipSetDevice(c->device);
+  cfm_cilia_release(c);
+  CiliaState* S = new (std::nothrow) CiliaState();
+  if (!S) return CFM_ENOMEM;
+  c->cilia = S;
+  S->prm = *prm;
+  S->n_body = bi.n_body;
+  S->per_body = per;
+  auto run = [&]() -> Status {
+    const size_t ns = c->p.n_seg, nbody = bi.n_body;
+    CFM_TRY(S->base.alloc(3 * (size_t)nc));
+    CFM_TRY(S->phase.alloc(nc));
+    CFM_TRY(S->pdot.alloc(nc));
+    CFM_TRY(S->angle.alloc(nc));
+    CFM_TRY(S->vphase.alloc(3 * ns));
+    CFM_TRY(S->vangle.alloc(3 * ns));
+    CFM_TRY(S->svals.alloc((size_t)nc * prm->nseg));
+    CFM_TRY(S->f.alloc(3 * ns));
+    CFM_TRY(S->q.alloc(2 * (size_t)nc));
+    CFM_TRY(S->omega0.alloc(nc));
+    CFM_TRY(S->adot.alloc(nc));
+    CFM_TRY(S->bx.alloc(3 * nbody));
+    CFM_TRY(S->bq.alloc(4 * nbody));
+    CFM_TRY(S->vbody.alloc(6 * nbody));
+    CFM_TRY(S->fref.alloc(3 * (size_t)per));
+    CFM_TRY(S->fq.alloc(4 * (size_t)per));
+    CFM_TRY(S->fblob.alloc(3 * (size_t)c->p.n_blob));
+    CFM_TRY(S->pc_c.alloc(15 * (size_t)nc));
+    CFM_TRY(S->pc_b.alloc(36 * nbody));
+    const size_t L = system_len(c, S);
+    CFM_TRY(S->z.alloc(L));
+    CFM_TRY(S->zrhs.alloc(L));
+    CFM_HIP(hipHostMalloc((void**)&S->host, sizeof(double) * 2 * nc, hipHostMallocDefault));
+    CFM_TRY(upload_aos(c, phase, nc, 1, S->phase.p, nc, 0));
+    CFM_TRY(upload_aos(c, phase_dot, nc, 1, S->pdot.p, nc, 0));
+    CFM_HIP(hipMemcpyAsync(S->omega0.p, S->pdot.p, sizeof(double) * nc, hipMemcpyDeviceToDevice, c->stream));
+    CFM_HIP(hipMemcpyAsync(S->bx.p, bi.x.data(), sizeof(double) * 3 * nbody, hipMemcpyHostToDevice, c->stream));
+    CFM_HIP(hipMemcpyAsync(S->bq.p, bi.q.data(), sizeof(double) * 4 * nbody, hipMemcpyHostToDevice, c->stream));
+    CFM_HIP(hipMemcpyAsync(S->fref.p, fil_ref, sizeof(double) * 3 * per, hipMemcpyHostToDevice, c->stream));
+    CFM_HIP(hipMemcpyAsync(S->fq.p, fil_q, sizeof(double) * 4 * per, hipMemcpyHostToDevice, c->stream));
+    CFM_TRY(dev_fill(c, S->base.p, 0.0, 3 * nc));
+    CFM_TRY(dev_fill(c, S->adot.p, 0.0, nc));
+    CFM_TRY(dev_fill(c, S->angle.p, 0.0, nc));
+    CFM_TRY(dev_fill(c, S->f.p, 0.0, (int)(3 * ns)));
+    CFM_TRY(dev_fill(c, S->vbody.p, 0.0, (int)(6 * nbody)));
+    CFM_TRY(dev_fill(c, S->fblob.p, 0.0, 3 * c->p.n_blob));
+    CFM_TRY(dev_fill(c, S->z.p, 0.0, (int)L));
+    CFM_TRY(geometry(c, S));
+    CFM_HIP(hipStreamSynchronize(c->stream));
+    return {};
+  };
+  Status s = run();
+  if (!s.ok()) {
+    cfm_cilia_release(c);
+    return cfail(c, s);
+  }
+  return CFM_OK;
+}
+
+int cfm_cilia_get_bodies(cfm_ctx* c, double* x, double* q, double* vel, double* f_blob) {
+  if (!c) return CFM_EINVAL;
+  CiliaState* S = cil_of(c);
+  if (!S || S->n_body == 0)
+    return cfail(c, make_status(CFM_ESTATE, "cfm_cilia_init_bodies has not been called"));
+  (void)hipSetDevice(c->device);
+  const size_t nb = S->n_body;
+  auto run = [&]() -> Status {
+    CFM_HIP(hipStreamSynchronize(c->stream));
+    if (x) CFM_HIP(hipMemcpy(x, S->bx.p, sizeof(double) * 3 * nb, hipMemcpyDeviceToHost));
+    if (q) CFM_HIP(hipMemcpy(q, S->bq.p, sizeof(double) * 4 * nb, hipMemcpyDeviceToHost));
+    if (vel) CFM_HIP(hipMemcpy(vel, S->vbody.p, sizeof(double) * 6 * nb, hipMemcpyDeviceToHost));
+    if (f_blob) CFM_TRY(download_aos(c, f_blob, c->p.n_blob, 3, S->fblob.p, c->p.n_blob, 0));
+    return {};
+  };
+  Status s = run();
+  return s.ok() ? CFM_OK : cfail(c, s);
+}
+
 int cfm_cilia_step(cfm_ctx* c, int32_t nt, double* q_phase, double* q_angle, cfm_solve_stats* st) {
   if (!c) return CFM_EINVAL;
   CiliaState* S = cil_of(c);
@@ -609,6 +1331,7 @@ int cfm_cilia_step(cfm_ctx* c, int32_t nt, double* q_phase, double* q_angle, cfm
   (void)hipSetDevice(c->device);
   const int nc = S->prm.n_cilia;
   auto run = [&]() -> Status {
+    if (S->n_body > 0) return step_bodies(c, S, nt, q_phase, q_angle, st);
     if (nt > 0) {  // phase += phase_dot dt before the step (filament.cpp:1340-1356)
       hipLaunchKernelGGL(cilia_advance_kernel, gc(nc), dim3(64), 0, c->stream, nc, S->prm.dt,
                          S->pdot.p, S->phase.p);
@@ -624,7 +1347,7 @@ int cfm_cilia_step(cfm_ctx* c, int32_t nt, double* q_phase, double* q_angle, cfm
       sd.extra = nc * (S->dyn_angle ? 2 : 1);
       sd.L = 3 * c->p.n_seg + sd.extra;
       hipLaunchKernelGGL(cilia_drive_kernel, dim3((sd.L + 255) / 256), dim3(256), 0, c->stream, nc,
-                         c->p.n_seg, S->dyn_angle ? 1 : 0, S->prm.dl * (S->prm.nseg - 1), S->spring,
+                         3 * c->p.n_seg, S->dyn_angle ? 1 : 0, S->prm.dl * (S->prm.nseg - 1), S->spring,
                          S->phase.p, S->angle.p, S->omega0.p, S->tab_phase.p, S->n_tab_phase,
                          S->tab_angle.p, S->n_tab_angle, S->zrhs.p);
       CFM_CHECK_LAUNCH();
@@ -696,15 +1419,15 @@ int cfm_cilia_set_driving(cfm_ctx* c, const double* q_phase_table, int32_t n_pha
       S->n_tab_angle = n_angle;
     }
     S->spring = spring;
-    const size_t L = 3 * (size_t)c->p.n_seg + (size_t)nc * (S->dyn_angle ? 2 : 1);
+    S->dynamic = true;
+    const size_t L = system_len(c, S);
     CFM_TRY(S->z.alloc(L));
     CFM_TRY(S->zrhs.alloc(L));
     CFM_TRY(dev_fill(c, S->z.p, 0.0, (int)L));
     // initial guess of the speeds: the natural frequencies, no shape rotation
-    CFM_HIP(hipMemcpyAsync(S->z.p + 3 * (size_t)c->p.n_seg, S->omega0.p, sizeof(double) * nc,
+    CFM_HIP(hipMemcpyAsync(S->z.p + phase_slot(c, S), S->omega0.p, sizeof(double) * nc,
                            hipMemcpyDeviceToDevice, c->stream));
     CFM_HIP(hipStreamSynchronize(c->stream));
-    S->dynamic = true;
     return {};
   };
   Status s = run();

@@ -363,6 +363,34 @@ int cfm_cilia_get_speeds(cfm_ctx* ctx, double* phase, double* phase_dot, double*
 int cfm_cilia_get_state(cfm_ctx* ctx, double* x, double* vel_dir_phase, double* vel_dir_angle,
                         double* s_values, double* phase, double* f_seg);
 
+/* Prescribed cilia carried by free rigid bodies: the reference's PRESCRIBED_CILIA
+ * swimmers with PRESCRIBED_BODY_VELOCITIES 0.  The context's segments are the cilia
+ * (seg_dof = 3, n_seg = n_cilia nseg) and its blobs the bodies, set up first with
+ * cfm_body_init and cfm_body_set_state; every body carries n_cilia / n_body cilia (body
+ * b owns cilia [b per, (b+1) per)).  fil_ref (3 per cilium of one body, AoS) are the
+ * cilia's base offsets and fil_q (4 per cilium, unit) their frames in the body frame (the
+ * reference's filament_references and body_q_ref, written to _fil_references.dat and
+ * _fil_q.dat; swimmer.cpp:448-462, filament.cpp:370-432): cilium c of body b has its
+ * base at x_b + R(q_b) fil_ref_l and its frame R(q_b) R(fil_q_l), whose first two axes
+ * carry the beat's filament axis and beat direction.
+ * Each cfm_cilia_step solves the system of mobility_solver.cpp:1891-2115: unknowns the
+ * forces of all segments and blobs, (V, Omega) of every body, and with
+ * cfm_cilia_set_driving the phase (and shape-angle) speeds; rows
+ *     M F - K U - sum_c (phi_dot_c k_phi,c + theta_dot_c k_theta,c) = rhs,
+ *     -K^T F = 0 (force- and torque-free bodies),  -k_phi,c . F = -q_phase_c, ...
+ * (rhs: phase_dot k_phi with prescribed phase speeds, zero with driving;
+ * mobility_solver.cpp:301-458).  Preconditioner CFM_PC_DIAG is the reference's block
+ * inverse with 6 pi mu a and make_precon_mat's 6 x 6 Schur complement
+ * (mobility_solver.cpp:1104-1515, swimmer.cpp:1175-1347); CFM_PC_SELF the same with each
+ * particle's wall self block.  Steps after the first first advance phases, shape angles
+ * and bodies explicitly with the previous step's speeds: x_b += dt V_b,
+ * q_b = exp(dt Omega_b) q_b (cilia_sim_main.cu:724-740). */
+int cfm_cilia_init_bodies(cfm_ctx* ctx, const cfm_cilia_params* p, const double* fil_ref,
+                          const double* fil_q, const double* phase, const double* phase_dot);
+/* Body centres 3 n_body, quaternions 4 n_body, velocities (V, Omega) 6 n_body of the last
+ * step and blob forces 3 n_blob (AoS); each pointer may be null. */
+int cfm_cilia_get_bodies(cfm_ctx* ctx, double* x, double* q, double* vel, double* f_blob);
+
 /* ---- free rigid bodies made of the blobs ---------------------------------------------
  * The reference's rigid-body blob system (USE_BROYDEN_FOR_EVERYTHING 0,
  * PRESCRIBED_BODY_VELOCITIES 0): n_body bodies of nb = n_blob / n_body blobs each (body b

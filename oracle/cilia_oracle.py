@@ -176,9 +176,11 @@ FRAME = np.array([[0.0, 0.0, -1.0],
                   [1.0, 0.0, 0.0]])
 
 
-def placement(psi, base, nseg=20, dl=2.6, angle=0.0, beat="fulford_blake", s=None):
+def placement(psi, base, nseg=20, dl=2.6, angle=0.0, beat="fulford_blake", s=None, frame=None):
     """Positions, phase and angle velocity directions of one cilium (AoS, 3 nseg);
-    segment parameters by bisection unless given."""
+    segment parameters by bisection unless given; frame: the cilium's rotation (3 x 3,
+    default the wall frame FRAME)."""
+    FRAME_ = FRAME if frame is None else np.asarray(frame, dtype=np.float64)
     L = dl * (nseg - 1)
     s = s_values(psi, nseg, beat) if s is None else np.asarray(s)
     ca, sa = math.cos(angle), math.sin(angle)
@@ -190,9 +192,9 @@ def placement(psi, base, nseg=20, dl=2.6, angle=0.0, beat="fulford_blake", s=Non
     x[0] = base
     for n in range(1, nseg):
         ref = L * (Rshape @ shape(s[n], psi, beat))
-        x[n] = base + FRAME @ ref
-        vangle[n] = FRAME @ np.array([-ref[1], ref[0], 0.0])   # e_z x ref
-        vphase[n] = FRAME @ (L * (Rshape @ shape_phase_derivative(s[n], psi, beat)))
+        x[n] = base + FRAME_ @ ref
+        vangle[n] = FRAME_ @ np.array([-ref[1], ref[0], 0.0])   # e_z x ref
+        vphase[n] = FRAME_ @ (L * (Rshape @ shape_phase_derivative(s[n], psi, beat)))
     return x.reshape(-1), vphase.reshape(-1), vangle.reshape(-1), s
 
 
