@@ -64,6 +64,8 @@ struct BodyState {
   DevBuf<double> q_cur, q_build;  // quaternions of the last placement / of the build (4 n_body)
   DevBuf<double> r_build;         // offsets at the build ([3][n_blob])
   DevBuf<double> rin, rout;       // rotated system vectors
+  DevBuf<double> st;              // staging of an embedded blob block (3 n_blob + 6 n_body)
+  std::vector<double> R0;         // K^T P K at the build, (6 n_body)^2 row-major
 };
 
 BodyState* body_of(cfm_ctx* c) { return reinterpret_cast<BodyState*>(c->body); }
@@ -287,6 +289,14 @@ __global__ __launch_bounds__(256) void body_dense_finish_kernel(int n3, int ncol
   out[j] = s;
 }
 
+// dst[k ld + j] += src[k m + j] for the three components of every blob.
+__global__ __launch_bounds__(256) void body_add_rows_kernel(int m, const double* __restrict__ src,
+                                                            size_t ld, double* __restrict__ dst) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= m) return;
+  for (int k = 0; k < 3; ++k) dst[(size_t)k * ld + j] += src[(size_t)k * m + j];
+}
+
 __device__ __forceinline__ void quat_rot(const double* q, double R[9]) {
   const double s = q[0], a = q[1], b = q[2], c = q[3];
   R[0] = 1.0 - 2.0 * (b * b + c * c);
@@ -430,6 +440,7 @@ Status build_schur(cfm_ctx* c, BodyState* S) {
   for (int i = 0; i < nc; ++i)
     for (int j = 0; j < nc; ++j)
       Sm[(size_t)i * nc + j] = -0.5 * (A[(size_t)j * nc + i] + A[(size_t)i * nc + j]);
+  S->R0 = Sm;
   if (!invert(Sm, nc)) return make_status(CFM_ENONFINITE, "singular body Schur complement K^T P K");
   CFM_HIP(hipMemcpyAsync(S->Sinv.p, Sm.data(), sizeof(double) * Sm.size(), hipMemcpyHostToDevice, c->stream));
   CFM_HIP(hipStreamSynchronize(c->stream));
@@ -439,7 +450,7 @@ Status build_schur(cfm_ctx* c, BodyState* S) {
 
 void body_release_state(BodyState* S) {
   for (auto* b : {&S->r, &S->mats, &S->w, &S->aux, &S->kcol, &S->PK, &S->Sinv, &S->ref_dev, &S->frame,
-                 &S->q_cur, &S->q_build, &S->r_build, &S->rin, &S->rout})
+                 &S->q_cur, &S->q_build, &S->r_build, &S->rin, &S->rout, &S->st})
     b->release();
 }
 
@@ -467,6 +478,134 @@ Status body_couple_apply(cfm_ctx* c, const double* x, double* y) {
   return {};
 }
 
+// The resident inverse, P K and (K^T P K)^-1 for the current placement -- or, frozen,
+// those of the build, applied in its frame.
+static Status ensure_dense(cfm_ctx* c, BodyState* S) {
+  const int m = c->p.n_blob;
+  const bool reuse = S->frozen && S->built_frozen;
+  if (!reuse && (!c->dinv.valid || S->schur_gen != c->dinv.generation)) {
+    CFM_TRY(build_schur(c, S));
+    if (S->frozen) {  // remember the frame the preconditioner belongs to
+      CFM_TRY(S->q_build.alloc(4 * (size_t)S->n_body));
+      CFM_TRY(S->r_build.alloc(3 * (size_t)m));
+      CFM_HIP(hipMemcpyAsync(S->q_build.p, S->q_cur.p, sizeof(double) * 4 * S->n_body,
+                             hipMemcpyDeviceToDevice, c->stream));
+      CFM_HIP(hipMemcpyAsync(S->r_build.p, S->r.p, sizeof(double) * 3 * m, hipMemcpyDeviceToDevice,
+                             c->stream));
+      S->built_frozen = true;
+    }
+  }
+  return {};
+}
+
+Status body_dense_ready(cfm_ctx* c) {
+  BodyState* S = body_of(c);
+  if (!S || !S->placed) return make_status(CFM_ESTATE, "no placed bodies");
+  return ensure_dense(c, S);
+}
+
+// y_b = P x_b for a blob block embedded with row stride ld (component k of blob j at
+// [k ld + j]); frozen: rotated into the frame of the build and back.
+Status body_dense_blob_apply(cfm_ctx* c, const double* x, size_t ldx, double* y, size_t ldy) {
+  BodyState* S = body_of(c);
+  const int m = c->p.n_blob, nrot = m + 2 * S->n_body;
+  const size_t L = 3 * (size_t)m + 6 * (size_t)S->n_body;
+  CFM_TRY(S->st.alloc(L));
+  CFM_TRY(S->rin.alloc(L));
+  CFM_TRY(S->rout.alloc(L));
+  CFM_TRY(dev_copy_rows(c, x, ldx, 0, S->st.p, m, 0, 3, m));
+  CFM_TRY(dev_fill(c, S->st.p + 3 * (size_t)m, 0.0, 6 * S->n_body));
+  const double* in = S->st.p;
+  if (S->frozen) {
+    hipLaunchKernelGGL(body_rotate_kernel, g1(nrot), dim3(256), 0, c->stream, m, S->nb, S->n_body,
+                       S->q_cur.p, S->q_build.p, 1, (const double*)S->st.p, S->rin.p);
+    in = S->rin.p;
+  }
+  CFM_TRY(dense_apply(c, c->dinv, in, S->rout.p));
+  CFM_TRY(dev_fill(c, S->rout.p + 3 * (size_t)m, 0.0, 6 * S->n_body));
+  const double* out = S->rout.p;
+  if (S->frozen) {
+    hipLaunchKernelGGL(body_rotate_kernel, g1(nrot), dim3(256), 0, c->stream, m, S->nb, S->n_body,
+                       S->q_cur.p, S->q_build.p, 0, (const double*)S->rout.p, S->st.p);
+    out = S->st.p;
+  }
+  CFM_CHECK_LAUNCH();
+  return dev_copy_rows(c, out, m, 0, y, ldy, 0, 3, m);
+}
+
+// y_b += P K U (U: 6 per body, in the current frame).
+Status body_dense_rigid_add(cfm_ctx* c, const double* U, double* y, size_t ldy) {
+  BodyState* S = body_of(c);
+  const int m = c->p.n_blob, nrot = m + 2 * S->n_body, nc = 6 * S->n_body;
+  const size_t L = 3 * (size_t)m + nc;
+  CFM_TRY(S->st.alloc(L));
+  CFM_TRY(S->rin.alloc(L));
+  CFM_TRY(S->rout.alloc(L));
+  CFM_TRY(dev_fill(c, S->st.p, 0.0, 3 * m));
+  CFM_HIP(hipMemcpyAsync(S->st.p + 3 * (size_t)m, U, sizeof(double) * nc, hipMemcpyDeviceToDevice, c->stream));
+  const double* in = S->st.p;
+  if (S->frozen) {
+    hipLaunchKernelGGL(body_rotate_kernel, g1(nrot), dim3(256), 0, c->stream, m, S->nb, S->n_body,
+                       S->q_cur.p, S->q_build.p, 1, (const double*)S->st.p, S->rin.p);
+    in = S->rin.p;
+  }
+  // blob part of `in` is zero: out = P K U'
+  hipLaunchKernelGGL(body_dense_finish_kernel, g1(3 * m), dim3(256), 0, c->stream, 3 * m, nc, in, S->PK.p,
+                     in + 3 * (size_t)m, S->rout.p);
+  CFM_TRY(dev_fill(c, S->rout.p + 3 * (size_t)m, 0.0, nc));
+  const double* out = S->rout.p;
+  if (S->frozen) {
+    hipLaunchKernelGGL(body_rotate_kernel, g1(nrot), dim3(256), 0, c->stream, m, S->nb, S->n_body,
+                       S->q_cur.p, S->q_build.p, 0, (const double*)S->rout.p, S->st.p);
+    out = S->st.p;
+  }
+  hipLaunchKernelGGL(body_add_rows_kernel, g1(m), dim3(256), 0, c->stream, m, out, ldy, y);
+  CFM_CHECK_LAUNCH();
+  return {};
+}
+
+// R = K^T P K of the current placement, (6 n_body)^2 row-major on the host; frozen: the
+// build's, rotated by G_b = R(q_b) R(q_build,b)^T on each 3 x 3 block.
+Status body_dense_resistance(cfm_ctx* c, std::vector<double>* R) {
+  BodyState* S = body_of(c);
+  const int nb = S->n_body, n = 6 * nb;
+  *R = S->R0;
+  if (!S->frozen) return {};
+  std::vector<double> qc(4 * (size_t)nb), q0(4 * (size_t)nb);
+  CFM_HIP(hipMemcpyAsync(qc.data(), S->q_cur.p, sizeof(double) * qc.size(), hipMemcpyDeviceToHost, c->stream));
+  CFM_HIP(hipMemcpyAsync(q0.data(), S->q_build.p, sizeof(double) * q0.size(), hipMemcpyDeviceToHost, c->stream));
+  CFM_HIP(hipStreamSynchronize(c->stream));
+  std::vector<double> G(9 * (size_t)nb);
+  for (int b = 0; b < nb; ++b) {
+    double Rc[9], Rb[9];
+    rot_mat(&qc[4 * b], Rc);
+    rot_mat(&q0[4 * b], Rb);
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j)
+        G[9 * b + 3 * i + j] = Rc[3 * i] * Rb[3 * j] + Rc[3 * i + 1] * Rb[3 * j + 1] + Rc[3 * i + 2] * Rb[3 * j + 2];
+  }
+  // block (I, J) of 3 x 3 entries: G_{b(I)} R0_IJ G_{b(J)}^T
+  for (int I = 0; I < n; I += 3)
+    for (int J = 0; J < n; J += 3) {
+      const double* Gi = &G[9 * (I / 6)];
+      const double* Gj = &G[9 * (J / 6)];
+      double T[9];
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+          double v = 0.0;
+          for (int k = 0; k < 3; ++k) v += Gi[3 * i + k] * S->R0[(size_t)(I + k) * n + J + j];
+          T[3 * i + j] = v;
+        }
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+          double v = 0.0;
+          for (int k = 0; k < 3; ++k) v += T[3 * i + k] * Gj[3 * j + k];
+          (*R)[(size_t)(I + i) * n + J + j] = v;
+        }
+    }
+  return {};
+}
+
 Status body_precond(cfm_ctx* c, int kind, const double* x, double* y) {
   BodyState* S = body_of(c);
   if (!S || !S->placed) return make_status(CFM_ESTATE, "no placed bodies");
@@ -487,19 +626,7 @@ Status body_precond(cfm_ctx* c, int kind, const double* x, double* y) {
     return {};
   }
   if (kind == CFM_PC_DENSE) {
-    const bool reuse = S->frozen && S->built_frozen;
-    if (!reuse && (!c->dinv.valid || S->schur_gen != c->dinv.generation)) {
-      CFM_TRY(build_schur(c, S));
-      if (S->frozen) {  // remember the frame the preconditioner belongs to
-        CFM_TRY(S->q_build.alloc(4 * (size_t)S->n_body));
-        CFM_TRY(S->r_build.alloc(3 * (size_t)m));
-        CFM_HIP(hipMemcpyAsync(S->q_build.p, S->q_cur.p, sizeof(double) * 4 * S->n_body,
-                               hipMemcpyDeviceToDevice, c->stream));
-        CFM_HIP(hipMemcpyAsync(S->r_build.p, S->r.p, sizeof(double) * 3 * m, hipMemcpyDeviceToDevice,
-                               c->stream));
-        S->built_frozen = true;
-      }
-    }
+    CFM_TRY(ensure_dense(c, S));
     const double* xin = x;
     double* yout = y;
     const double* roff = S->r.p;

@@ -23,4 +23,15 @@ void body_freeze_dense(cfm_ctx* c, bool on);
 // Rotated blob offsets R(q_b) ref of the last placement ([3][n_blob], device), or null.
 const double* body_offsets(cfm_ctx* c);
 
+// Dense blob block for systems that embed the bodies (cilia on free bodies): the
+// resident inverse P of M_bb, its rigid columns P K and R = K^T P K, built for the
+// current placement or, frozen, once and applied in the frame of the build.
+Status body_dense_ready(cfm_ctx* c);
+// y_b = P x_b; blob blocks with component k of blob j at [k ld + j].
+Status body_dense_blob_apply(cfm_ctx* c, const double* x, size_t ldx, double* y, size_t ldy);
+// y_b += P K U, U: (V, Omega) of every body.
+Status body_dense_rigid_add(cfm_ctx* c, const double* U, double* y, size_t ldy);
+// R = K^T P K, (6 n_body)^2 row-major, on the host.
+Status body_dense_resistance(cfm_ctx* c, std::vector<double>* R);
+
 }  // namespace cfm

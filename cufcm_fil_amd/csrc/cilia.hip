@@ -506,6 +506,11 @@ struct BodyCiliaK {
   double mu, a_seg, a_blob;
   double* pc_c;  // per cilium (15): S_cc^-1 (i11, i12, i22), B_phi (6), B_theta (6)
   double* pc_b;  // per body (36): Schur complement inverse, row-major
+  // CFM_PC_DENSE: the blob block is the resident inverse of M_bb (applied by the body
+  // module); the particle kernels skip blobs, pc_b holds the segment part of the
+  // Schur complement (not inverted) and tU the body right-hand sides
+  int dense;
+  double* tU;
 };
 
 __device__ inline int cb_body(const BodyCiliaK& k, int i) {
@@ -703,7 +708,7 @@ __device__ inline void cb_invert6(double* A) {
 // Per body: (K_b^T P K_b - sum_c B_c S_cc^-1 B_c^T)^-1 (one block per body).
 __global__ __launch_bounds__(256) void cb_pc_body_prep_kernel(BodyCiliaK k) {
   const int b = blockIdx.x;
-  const int cnt = k.per_body * k.nseg + k.nb;
+  const int cnt = k.per_body * k.nseg + (k.dense ? 0 : k.nb);
   const double c0 = k.bx[3 * b], c1 = k.bx[3 * b + 1], c2 = k.bx[3 * b + 2];
   double A[36];
   for (int j = 0; j < 36; ++j) A[j] = 0.0;
@@ -731,7 +736,7 @@ __global__ __launch_bounds__(256) void cb_pc_body_prep_kernel(BodyCiliaK k) {
           A[6 * a + e] -= o[0] * Bp[a] * Bp[e] + o[1] * (Bp[a] * Ba[e] + Ba[a] * Bp[e]) + o[2] * Ba[a] * Ba[e];
     }
   }
-  cb_invert6(A);
+  if (!k.dense) cb_invert6(A);
   for (int j = 0; j < 36; ++j) k.pc_b[36 * (size_t)b + j] = A[j];
 }
 
@@ -739,7 +744,7 @@ __global__ __launch_bounds__(256) void cb_pc_body_prep_kernel(BodyCiliaK k) {
 __global__ __launch_bounds__(256) void cb_pc_particles_kernel(BodyCiliaK k, int mode, const double* __restrict__ r,
                                                               double* __restrict__ out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= k.M) return;
+  if (i >= k.M || (k.dense && i >= k.nseg_total)) return;
   const size_t M = k.M;
   double p[3];
   cb_pdiag(k, i, p);
@@ -823,12 +828,26 @@ __global__ __launch_bounds__(256) void cb_pc_body_kernel(BodyCiliaK k, const dou
       for (int d = 0; d < 6; ++d) t[d] -= o[3 + d] * wp + o[9 + d] * wa;
     }
   }
+  if (k.dense) {  // the bodies couple through P: one full solve (cb_full_gemv_kernel)
+    for (int d = 0; d < 6; ++d) k.tU[6 * (size_t)b + d] = t[d];
+    return;
+  }
   const double* Si = k.pc_b + 36 * (size_t)b;
   for (int a = 0; a < 6; ++a) {
     double v = 0.0;
     for (int e = 0; e < 6; ++e) v += Si[6 * a + e] * t[e];
     out[ub + a] = v;
   }
+}
+
+// U = S^-1 t with the full (6 n_body)^2 inverse (row-major)
+__global__ void cb_full_gemv_kernel(int n, const double* __restrict__ Sinv, const double* __restrict__ t,
+                                    double* __restrict__ U) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  double v = 0.0;
+  for (int j = 0; j < n; ++j) v += Sinv[(size_t)r * n + j] * t[j];
+  U[r] = v;
 }
 
 // rhs with prescribed phase speeds: phase_dot k_phi on the segment rows, zero elsewhere
@@ -900,6 +919,7 @@ struct CiliaState {
   // preconditioner's per-cilium and per-body matrices
   int n_body = 0, per_body = 0;
   DevBuf<double> bx, bq, vbody, fref, fq, fblob, pc_c, pc_b;
+  DevBuf<double> tU, pc_full;  // CFM_PC_DENSE: body right-hand sides, full Schur inverse
   bool pc_ready = false;
   int pc_kind = -1;
 };
@@ -992,6 +1012,8 @@ static BodyCiliaK body_cilia_args(cfm_ctx* c, CiliaState* S, int pc_kind) {
   k.a_blob = c->cfg.a_blob;
   k.pc_c = S->pc_c.p;
   k.pc_b = S->pc_b.p;
+  k.dense = pc_kind == CFM_PC_DENSE ? 1 : 0;
+  k.tU = S->tU.p;
   return k;
 }
 
@@ -1016,6 +1038,51 @@ static Status body_cilia_prep(cfm_ctx* c, CiliaState* S, int kind) {
   hipLaunchKernelGGL(cb_pc_cilia_prep_kernel, gc(k.ncil), dim3(64), 0, c->stream, k);
   hipLaunchKernelGGL(cb_pc_body_prep_kernel, dim3(k.nbody), dim3(256), 0, c->stream, k);
   CFM_CHECK_LAUNCH();
+  if (k.dense) {
+    // S = K_b^T P K_b (resident inverse, all bodies) + per body the segment part less
+    // the cilia's Schur terms; inverted once per geometry on the host
+    const int n = 6 * k.nbody;
+    CFM_TRY(body_dense_ready(c));
+    std::vector<double> R, D(36 * (size_t)k.nbody);
+    CFM_TRY(body_dense_resistance(c, &R));
+    CFM_HIP(hipMemcpyAsync(D.data(), S->pc_b.p, sizeof(double) * D.size(), hipMemcpyDeviceToHost, c->stream));
+    CFM_HIP(hipStreamSynchronize(c->stream));
+    for (int b = 0; b < k.nbody; ++b)
+      for (int i = 0; i < 6; ++i)
+        for (int j = 0; j < 6; ++j) R[(size_t)(6 * b + i) * n + 6 * b + j] += D[36 * (size_t)b + 6 * i + j];
+    // symmetric positive definite: Gauss-Jordan with partial pivoting
+    std::vector<double> I((size_t)n * n, 0.0);
+    for (int i = 0; i < n; ++i) I[(size_t)i * n + i] = 1.0;
+    for (int col = 0; col < n; ++col) {
+      int piv = col;
+      for (int r = col + 1; r < n; ++r)
+        if (std::fabs(R[(size_t)r * n + col]) > std::fabs(R[(size_t)piv * n + col])) piv = r;
+      if (!(std::fabs(R[(size_t)piv * n + col]) > 0.0))
+        return make_status(CFM_ENONFINITE, "singular body Schur complement");
+      for (int j = 0; j < n; ++j) {
+        std::swap(R[(size_t)col * n + j], R[(size_t)piv * n + j]);
+        std::swap(I[(size_t)col * n + j], I[(size_t)piv * n + j]);
+      }
+      const double inv = 1.0 / R[(size_t)col * n + col];
+      for (int j = 0; j < n; ++j) {
+        R[(size_t)col * n + j] *= inv;
+        I[(size_t)col * n + j] *= inv;
+      }
+      for (int r = 0; r < n; ++r) {
+        if (r == col) continue;
+        const double f = R[(size_t)r * n + col];
+        if (f == 0.0) continue;
+        for (int j = 0; j < n; ++j) {
+          R[(size_t)r * n + j] -= f * R[(size_t)col * n + j];
+          I[(size_t)r * n + j] -= f * I[(size_t)col * n + j];
+        }
+      }
+    }
+    CFM_TRY(S->pc_full.alloc((size_t)n * n));
+    CFM_TRY(S->tU.alloc(n));
+    CFM_HIP(hipMemcpyAsync(S->pc_full.p, I.data(), sizeof(double) * I.size(), hipMemcpyHostToDevice, c->stream));
+    CFM_HIP(hipStreamSynchronize(c->stream));
+  }
   S->pc_ready = true;
   S->pc_kind = kind;
   return {};
@@ -1045,16 +1112,22 @@ Status cilia_precond(cfm_ctx* c, int kind, const double* x, double* y) {
       CFM_HIP(hipMemcpyAsync(y, x, sizeof(double) * system_len(c, S), hipMemcpyDeviceToDevice, c->stream));
       return {};
     }
-    if (kind != CFM_PC_DIAG && kind != CFM_PC_SELF)
-      return make_status(CFM_EINVAL, "cilia on free bodies take CFM_PC_NONE, _DIAG or _SELF");
+    if (kind != CFM_PC_DIAG && kind != CFM_PC_SELF && kind != CFM_PC_DENSE)
+      return make_status(CFM_EINVAL, "cilia on free bodies take CFM_PC_NONE, _DIAG, _SELF or _DENSE");
     if (!S->pc_ready || S->pc_kind != kind) CFM_TRY(body_cilia_prep(c, S, kind));
     const BodyCiliaK k = body_cilia_args(c, S, kind);
     const dim3 gp((k.M + 255) / 256);
+    const size_t M = k.M, ns = k.nseg_total;
     hipLaunchKernelGGL(cb_pc_particles_kernel, gp, dim3(256), 0, c->stream, k, 0, x, y);
+    if (k.dense) CFM_TRY(body_dense_blob_apply(c, x + ns, M, y + ns, M));
     if (k.dynamic) hipLaunchKernelGGL(cb_pc_cilia_kernel, gc(k.ncil), dim3(64), 0, c->stream, k, 0, x, y);
     hipLaunchKernelGGL(cb_pc_body_kernel, dim3(k.nbody), dim3(256), 0, c->stream, k, x, y);
+    if (k.dense)
+      hipLaunchKernelGGL(cb_full_gemv_kernel, gc(6 * k.nbody), dim3(64), 0, c->stream, 6 * k.nbody,
+                         S->pc_full.p, S->tU.p, y + 3 * M);
     if (k.dynamic) hipLaunchKernelGGL(cb_pc_cilia_kernel, gc(k.ncil), dim3(64), 0, c->stream, k, 1, x, y);
     hipLaunchKernelGGL(cb_pc_particles_kernel, gp, dim3(256), 0, c->stream, k, 1, x, y);
+    if (k.dense) CFM_TRY(body_dense_rigid_add(c, y + 3 * M, y + ns, M));
     CFM_CHECK_LAUNCH();
     return {};
   }
@@ -1089,7 +1162,8 @@ static Status step_bodies(cfm_ctx* c, CiliaState* S, int nt, double* q_phase, do
   const SystemDesc sd = body_cilia_desc(c, S);
   cfm_solve_opts o = S->prm.solve;
   o.use_guess = 1;  // previous step's solution
-  if (o.precond == CFM_PC_DIAG || o.precond == CFM_PC_SELF) CFM_TRY(body_cilia_prep(c, S, o.precond));
+  if (o.precond == CFM_PC_DIAG || o.precond == CFM_PC_SELF || o.precond == CFM_PC_DENSE)
+    CFM_TRY(body_cilia_prep(c, S, o.precond));
   if (S->dynamic) {
     hipLaunchKernelGGL(cilia_drive_kernel, dim3((sd.L + 255) / 256), dim3(256), 0, c->stream, nc,
                        (int)phase_slot(c, S), S->dyn_angle ? 1 : 0, S->prm.dl * (S->prm.nseg - 1),
@@ -1128,7 +1202,7 @@ void cfm_cilia_release(cfm_ctx* c) {
   for (auto* b : {&S->base, &S->phase, &S->pdot, &S->angle, &S->vphase, &S->vangle, &S->svals,
                   &S->rhs, &S->f, &S->q, &S->omega0, &S->adot, &S->tab_phase, &S->tab_angle,
                   &S->z, &S->zrhs, &S->s_table, &S->bx, &S->bq, &S->vbody, &S->fref, &S->fq,
-                  &S->fblob, &S->pc_c, &S->pc_b})
+                  &S->fblob, &S->pc_c, &S->pc_b, &S->tU, &S->pc_full})
     b->release();
   if (S->host) (void)hipHostFree(S->host);
   delete S;
@@ -1226,7 +1300,10 @@ int cfm_cilia_init_bodies(cfm_ctx* c, const cfm_cilia_params* prm, const double*
   if (!c) return CFM_EINVAL;
   if (!prm || !fil_ref || !fil_q || !phase || !phase_dot)
     return cfail(c, make_status(CFM_EINVAL, "null argument"));
-  const Status chk = check_params(c, prm, phase, phase_dot);
+  cfm_cilia_params pr = *prm;
+  const bool dense = pr.solve.precond == CFM_PC_DENSE;
+  if (dense) pr.solve.precond = CFM_PC_DIAG;  // checked as the others
+  const Status chk = check_params(c, &pr, phase, phase_dot);
   if (!chk.ok()) return cfail(c, chk);
   BodyInfo bi;
   const Status bs = body_info(c, &bi);
@@ -1254,6 +1331,8 @@ int cfm_cilia_init_bodies(cfm_ctx* c, const cfm_cilia_params* prm, const double*
   S->prm = *prm;
   S->n_body = bi.n_body;
   S->per_body = per;
+  // the resident inverse of M_bb is built once and applied in the bodies' frame
+  body_freeze_dense(c, dense);
   auto run = [&]() -> Status {
     const size_t ns = c->p.n_seg, nbody = bi.n_body;
     CFM_TRY(S->base.alloc(3 * (size_t)nc));

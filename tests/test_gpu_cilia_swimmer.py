@@ -46,7 +46,7 @@ def _compare(G, sw, nt, qp_o, qa_o, qp_g, qa_g, tol=1e-8):
     assert np.allclose(sp["phase_dot"], sw.phase_dot, rtol=tol), nt
 
 
-@pytest.mark.parametrize("precond", ["diag", "none"])
+@pytest.mark.parametrize("precond", ["diag", "none", "dense"])
 def test_prescribed_phase_speeds_match_oracle(precond):
     sw = swimmer(n_body=2, per=3, seed=11, dt=0.02)
     with _gpu(sw, precond) as G:
@@ -57,7 +57,8 @@ def test_prescribed_phase_speeds_match_oracle(precond):
             _compare(G, sw, nt, qp_o, qa_o, qp_g, qa_g)
 
 
-def test_dynamic_phases_and_angles_match_oracle():
+@pytest.mark.parametrize("precond", ["diag", "dense"])
+def test_dynamic_phases_and_angles_match_oracle(precond):
     qp = np.loadtxt(os.path.join(GOLD, "fb_phase_generalised_forces_nseg20_sep2.6.dat"))[1:]
     qa = np.loadtxt(os.path.join(GOLD, "fb_angle_generalised_forces_nseg20_sep2.6.dat"))[1:]
     # tables recorded for 20 segments, scaled to these 6-segment cilia
@@ -66,7 +67,7 @@ def test_dynamic_phases_and_angles_match_oracle():
     sw.driving = (qp, qa, 0.05)
     # the right-hand side holds only the driving forces: a tighter GMRES tolerance, and
     # forces within 1e-7 of the dense solve
-    with _gpu(sw, "diag", rtol=1e-13) as G:
+    with _gpu(sw, precond, rtol=1e-13) as G:
         G.set_driving(qp, qa, spring=0.05)
         for nt in range(3):
             qp_g, qa_g, stats = G.step(nt)
@@ -79,7 +80,7 @@ def test_dynamic_phases_and_angles_match_oracle():
             assert np.allclose(sp["angle"], sw.angle, atol=1e-12)
 
 
-def _big(body_q, phase):
+def _big(body_q, phase, precond="diag"):
     R = 20.0
     blob_ref = geo.fibonacci_sphere(1500, R)
     dirs = geo.fibonacci_sphere(40, 1.0).reshape(-1, 3)
@@ -87,7 +88,7 @@ def _big(body_q, phase):
     fil_q = cilium_frames(dirs, 0.0)
     return CiliatedBodies(blob_ref, fil_ref, fil_q, [5.0, -3.0, 2.0], body_q, phase,
                           np.full(40, 2 * math.pi), nseg=20, dl=2.6, dt=0.01, wall=False,
-                          precond="diag", rtol=1e-10)
+                          precond=precond, rtol=1e-10)
 
 
 def test_rotated_swimmer_moves_rotated():
@@ -104,4 +105,15 @@ def test_rotated_swimmer_moves_rotated():
     scale = np.abs(v0).max()
     assert np.abs(v1[:3] - R0 @ v0[:3]).max() < 1e-7 * scale
     assert np.abs(v1[3:] - R0 @ v0[3:]).max() < 1e-7 * scale
-    print("gmres iterations", s0["iterations"], s1["iterations"])
+    # the resident inverse of the blob block (frozen in the body frame) against the
+    # reference's 6 pi mu a: same velocities, a fraction of the GMRES steps, also after
+    # the body has moved
+    with _big(qb, phase, "dense") as G:
+        _, _, s2 = G.step(0)
+        v2 = G.bodies()["vel"]
+        for nt in range(1, 3):
+            _, _, s3 = G.step(nt)
+            assert s3["converged"] and s3["iterations"] <= s2["iterations"] + 2, s3
+    assert np.abs(v2 - v1).max() < 1e-7 * scale
+    print("gmres iterations diag", s0["iterations"], "dense", s2["iterations"], s3["iterations"])
+    assert s2["iterations"] * 1.5 < s1["iterations"]
