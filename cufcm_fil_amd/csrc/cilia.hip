@@ -27,6 +27,7 @@
 #include "ctx.hpp"
 #include "krylov.hpp"
 #include "body.hpp"
+#include "rpy_pair.hpp"
 #include "fil_device.hpp"
 #include "precond.hpp"
 
@@ -511,6 +512,12 @@ struct BodyCiliaK {
   // Schur complement (not inverted) and tU the body right-hand sides
   int dense;
   double* tU;
+  // CFM_PC_NEAR / CFM_PC_DENSE: each cilium's own mobility block inverted (P_c,
+  // 3 nseg x 3 nseg), its images P_c (K columns, k_phi, k_theta) and K^T P_c K
+  int blk;
+  double* pblk;  // per cilium (3 nseg)^2
+  double* pkc;   // per cilium 8 x 3 nseg: P_c times (e_x, e_y, e_z, e_x x r, e_y x r, e_z x r, k_phi, k_theta)
+  double* dcil;  // per cilium 36: K^T P_c K
 };
 
 __device__ inline int cb_body(const BodyCiliaK& k, int i) {
@@ -708,12 +715,13 @@ __device__ inline void cb_invert6(double* A) {
 // Per body: (K_b^T P K_b - sum_c B_c S_cc^-1 B_c^T)^-1 (one block per body).
 __global__ __launch_bounds__(256) void cb_pc_body_prep_kernel(BodyCiliaK k) {
   const int b = blockIdx.x;
-  const int cnt = k.per_body * k.nseg + (k.dense ? 0 : k.nb);
+  const int nsb = k.blk ? 0 : k.per_body * k.nseg;  // segments handled here
+  const int cnt = nsb + (k.dense ? 0 : k.nb);
   const double c0 = k.bx[3 * b], c1 = k.bx[3 * b + 1], c2 = k.bx[3 * b + 2];
   double A[36];
   for (int j = 0; j < 36; ++j) A[j] = 0.0;
   for (int t = threadIdx.x; t < cnt; t += blockDim.x) {
-    const int i = cb_particle(k, b, t);
+    const int i = t < nsb ? cb_particle(k, b, t) : k.nseg_total + b * k.nb + (t - nsb);
     double p[3];
     cb_pdiag(k, i, p);
     const double r0 = k.px[i] - c0, r1 = k.py[i] - c1, r2 = k.pz[i] - c2;
@@ -727,6 +735,9 @@ __global__ __launch_bounds__(256) void cb_pc_body_prep_kernel(BodyCiliaK k) {
   }
   cb_block_sum<36>(A);
   if (threadIdx.x != 0) return;
+  if (k.blk)
+    for (int l = 0; l < k.per_body; ++l)
+      for (int j = 0; j < 36; ++j) A[j] += k.dcil[36 * (size_t)(b * k.per_body + l) + j];
   if (k.dynamic) {
     for (int l = 0; l < k.per_body; ++l) {
       const double* o = k.pc_c + 15 * (size_t)(b * k.per_body + l);
@@ -744,7 +755,7 @@ __global__ __launch_bounds__(256) void cb_pc_body_prep_kernel(BodyCiliaK k) {
 __global__ __launch_bounds__(256) void cb_pc_particles_kernel(BodyCiliaK k, int mode, const double* __restrict__ r,
                                                               double* __restrict__ out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= k.M || (k.dense && i >= k.nseg_total)) return;
+  if (i >= k.M || (k.dense && i >= k.nseg_total) || (k.blk && i < k.nseg_total)) return;
   const size_t M = k.M;
   double p[3];
   cb_pdiag(k, i, p);
@@ -900,6 +911,155 @@ __global__ void cb_advance_bodies_kernel(int nbody, double dt, const double* __r
   bq[4 * b + 3] = r.z;
 }
 
+// ---- per-cilium block preconditioner (CFM_PC_NEAR, CFM_PC_DENSE) ----------------------
+
+// Unit-load response of segment i to a load on segment j, 3 x 3 (row p, column q).
+__device__ inline void cb_seg_block(const BodyCiliaK& k, int i, int j, int q, double V[3]) {
+  double F[3] = {0.0, 0.0, 0.0};
+  F[q] = 1.0;
+  V[0] = V[1] = V[2] = 0.0;
+  if (k.wall) {
+    if (i == j)
+      wall_self<3, 3>(k.pz[i], F, V);
+    else
+      wall_pair<double, 3, 3>(k.px[i] - k.px[j], k.py[i] - k.py[j], k.pz[i] - k.pz[j], k.pz[i] + k.pz[j],
+                              k.pz[j], F, V);
+    for (int p = 0; p < 3; ++p) V[p] /= k.mu;
+  } else {
+    free_pair<double, 3, 3>(k.px[i] - k.px[j], k.py[i] - k.py[j], k.pz[i] - k.pz[j], k.a_seg, k.a_seg, k.mu,
+                            F, V);
+  }
+}
+
+// One workgroup per cilium: P_c = M_cc^-1 by in-place Gauss-Jordan (M_cc is symmetric
+// positive definite: no pivoting), then P_c times the cilium's K columns, k_phi and
+// k_theta, and its contributions S_cc^-1, B and K^T P_c K.
+__global__ __launch_bounds__(256) void cb_block_prep_kernel(BodyCiliaK k) {
+  const int c = blockIdx.x, n = 3 * k.nseg, s0 = c * k.nseg, b = c / k.per_body;
+  double* A = k.pblk + (size_t)c * n * n;
+  for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
+    const int r = e / n, col = e - r * n;
+    double V[3];
+    cb_seg_block(k, s0 + r / 3, s0 + col / 3, col % 3, V);
+    A[e] = V[r % 3];
+  }
+  __syncthreads();
+  __shared__ double rowk[3 * kMaxCiliaSeg], colk[3 * kMaxCiliaSeg];
+  for (int kk = 0; kk < n; ++kk) {
+    const double piv = 1.0 / A[kk * n + kk];
+    for (int j = threadIdx.x; j < n; j += blockDim.x) {
+      rowk[j] = (j == kk) ? 1.0 : A[kk * n + j] * piv;
+      colk[j] = A[j * n + kk];
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
+      const int r = e / n, j = e - r * n;
+      if (r == kk)
+        A[e] = (j == kk) ? piv : rowk[j];
+      else if (j == kk)
+        A[e] = -colk[r] * piv;
+      else
+        A[e] -= colk[r] * rowk[j];
+    }
+    __syncthreads();
+  }
+  // columns of the cilium: K (6) and the velocity directions (2), then their images
+  const double c0 = k.bx[3 * b], c1 = k.bx[3 * b + 1], c2 = k.bx[3 * b + 2];
+  const size_t ns = k.nseg_total;
+  double* W = k.pkc + (size_t)c * 8 * n;
+  for (int e = threadIdx.x; e < 8 * n; e += blockDim.x) {
+    const int col = e / n, r = e - col * n;
+    double acc = 0.0;
+    for (int j = 0; j < n; ++j) {
+      const int g = s0 + j / 3, d = j % 3;
+      double v;
+      if (col < 3) {
+        v = (d == col) ? 1.0 : 0.0;
+      } else if (col < 6) {  // e_a x r_g
+        const double rr[3] = {k.px[g] - c0, k.py[g] - c1, k.pz[g] - c2};
+        const int a = col - 3, a1 = (a + 1) % 3, a2 = (a + 2) % 3;
+        v = (d == a2) ? rr[a1] : (d == a1) ? -rr[a2] : 0.0;
+      } else {
+        v = (col == 6 ? k.vphase : k.vangle)[(size_t)d * ns + g];
+      }
+      acc += A[r * n + j] * v;
+    }
+    W[e] = acc;
+  }
+  __syncthreads();
+  if (threadIdx.x >= 64) return;
+  // K^T P_c K (36 entries), B = K^T P_c (k_phi, k_theta), S_cc (thread 0)
+  const int t = threadIdx.x;
+  auto colv = [&](int col, int j) -> double {
+    const int g = s0 + j / 3, d = j % 3;
+    if (col < 3) return (d == col) ? 1.0 : 0.0;
+    if (col < 6) {
+      const double rr[3] = {k.px[g] - c0, k.py[g] - c1, k.pz[g] - c2};
+      const int a = col - 3, a1 = (a + 1) % 3, a2 = (a + 2) % 3;
+      return (d == a2) ? rr[a1] : (d == a1) ? -rr[a2] : 0.0;
+    }
+    return (col == 6 ? k.vphase : k.vangle)[(size_t)d * ns + g];
+  };
+  if (t < 36) {
+    const int a = t / 6, e = t - 6 * (t / 6);
+    double v = 0.0;
+    for (int j = 0; j < n; ++j) v += colv(a, j) * W[(size_t)e * n + j];
+    k.dcil[36 * (size_t)c + t] = v;
+  } else if (t < 48) {
+    const int q = t - 36, a = q % 6, which = q / 6;  // B_phi (a) or B_theta (a)
+    double v = 0.0;
+    for (int j = 0; j < n; ++j) v += colv(a, j) * W[(size_t)(6 + which) * n + j];
+    k.pc_c[15 * (size_t)c + 3 + 6 * which + a] = v;
+  } else if (t == 48) {
+    double c11 = 0.0, c12 = 0.0, c22 = 0.0;
+    for (int j = 0; j < n; ++j) {
+      c11 += colv(6, j) * W[(size_t)6 * n + j];
+      c12 += colv(6, j) * W[(size_t)7 * n + j];
+      c22 += colv(7, j) * W[(size_t)7 * n + j];
+    }
+    double* o = k.pc_c + 15 * (size_t)c;
+    if (k.dyn_angle) {
+      const double det = c11 * c22 - c12 * c12;
+      o[0] = c22 / det;
+      o[1] = -c12 / det;
+      o[2] = c11 / det;
+    } else {
+      o[0] = 1.0 / c11;
+      o[1] = 0.0;
+      o[2] = 0.0;
+    }
+  }
+}
+
+// mode 0: out_c = P_c r_c;  mode 1: out_c += P_c (K U + phi k_phi + theta k_theta)
+// (one workgroup per cilium, one thread per row)
+__global__ __launch_bounds__(256) void cb_block_apply_kernel(BodyCiliaK k, int mode, const double* __restrict__ r,
+                                                             double* __restrict__ out) {
+  const int c = blockIdx.x, n = 3 * k.nseg, s0 = c * k.nseg;
+  const size_t M = k.M;
+  const int row = threadIdx.x;
+  if (row >= n) return;
+  const int g = s0 + row / 3, d = row % 3;
+  if (mode == 0) {
+    const double* A = k.pblk + (size_t)c * n * n + (size_t)row * n;
+    double v = 0.0;
+    for (int j = 0; j < n; ++j) v += A[j] * r[(size_t)(j % 3) * M + s0 + j / 3];
+    out[(size_t)d * M + g] = v;
+    return;
+  }
+  const int b = c / k.per_body;
+  const double* U = out + 3 * M + 6 * (size_t)b;
+  const double* W = k.pkc + (size_t)c * 8 * n;
+  double v = 0.0;
+  for (int a = 0; a < 6; ++a) v += W[(size_t)a * n + row] * U[a];
+  if (k.dynamic) {
+    const size_t ph = 3 * M + 6 * (size_t)k.nbody;
+    v += W[(size_t)6 * n + row] * out[ph + c];
+    if (k.dyn_angle) v += W[(size_t)7 * n + row] * out[ph + k.ncil + c];
+  }
+  out[(size_t)d * M + g] += v;
+}
+
 struct CiliaState {
   cfm_cilia_params prm{};
   DevBuf<double> base, phase, pdot, angle, vphase, vangle, svals, rhs, f, q;
@@ -920,6 +1080,7 @@ struct CiliaState {
   int n_body = 0, per_body = 0;
   DevBuf<double> bx, bq, vbody, fref, fq, fblob, pc_c, pc_b;
   DevBuf<double> tU, pc_full;  // CFM_PC_DENSE: body right-hand sides, full Schur inverse
+  DevBuf<double> pblk, pkc, dcil;  // CFM_PC_NEAR / _DENSE: per-cilium blocks
   bool pc_ready = false;
   int pc_kind = -1;
 };
@@ -1014,6 +1175,11 @@ static BodyCiliaK body_cilia_args(cfm_ctx* c, CiliaState* S, int pc_kind) {
   k.pc_b = S->pc_b.p;
   k.dense = pc_kind == CFM_PC_DENSE ? 1 : 0;
   k.tU = S->tU.p;
+  k.blk = (pc_kind == CFM_PC_NEAR || pc_kind == CFM_PC_DENSE) ? 1 : 0;
+  k.pblk = S->pblk.p;
+  k.pkc = S->pkc.p;
+  k.dcil = S->dcil.p;
+  if (k.blk) k.pc_kind = CFM_PC_DIAG;  // blobs outside the dense block: 6 pi mu a
   return k;
 }
 
@@ -1034,8 +1200,17 @@ static SystemDesc body_cilia_desc(cfm_ctx* c, CiliaState* S) {
 // make_precon_mat (swimmer.cpp:1175-1347) and the per-cilium blocks, for the current
 // geometry and the preconditioner kind of the step's solve.
 static Status body_cilia_prep(cfm_ctx* c, CiliaState* S, int kind) {
+  if (kind == CFM_PC_NEAR || kind == CFM_PC_DENSE) {
+    const size_t n = 3 * (size_t)S->prm.nseg, nc = S->prm.n_cilia;
+    CFM_TRY(S->pblk.alloc(nc * n * n));
+    CFM_TRY(S->pkc.alloc(nc * 8 * n));
+    CFM_TRY(S->dcil.alloc(nc * 36));
+  }
   const BodyCiliaK k = body_cilia_args(c, S, kind);
-  hipLaunchKernelGGL(cb_pc_cilia_prep_kernel, gc(k.ncil), dim3(64), 0, c->stream, k);
+  if (k.blk)
+    hipLaunchKernelGGL(cb_block_prep_kernel, dim3(k.ncil), dim3(256), 0, c->stream, k);
+  else
+    hipLaunchKernelGGL(cb_pc_cilia_prep_kernel, gc(k.ncil), dim3(64), 0, c->stream, k);
   hipLaunchKernelGGL(cb_pc_body_prep_kernel, dim3(k.nbody), dim3(256), 0, c->stream, k);
   CFM_CHECK_LAUNCH();
   if (k.dense) {
@@ -1112,13 +1287,14 @@ Status cilia_precond(cfm_ctx* c, int kind, const double* x, double* y) {
       CFM_HIP(hipMemcpyAsync(y, x, sizeof(double) * system_len(c, S), hipMemcpyDeviceToDevice, c->stream));
       return {};
     }
-    if (kind != CFM_PC_DIAG && kind != CFM_PC_SELF && kind != CFM_PC_DENSE)
-      return make_status(CFM_EINVAL, "cilia on free bodies take CFM_PC_NONE, _DIAG, _SELF or _DENSE");
+    if (kind != CFM_PC_DIAG && kind != CFM_PC_SELF && kind != CFM_PC_NEAR && kind != CFM_PC_DENSE)
+      return make_status(CFM_EINVAL, "cilia on free bodies take CFM_PC_NONE, _DIAG, _SELF, _NEAR or _DENSE");
     if (!S->pc_ready || S->pc_kind != kind) CFM_TRY(body_cilia_prep(c, S, kind));
     const BodyCiliaK k = body_cilia_args(c, S, kind);
     const dim3 gp((k.M + 255) / 256);
     const size_t M = k.M, ns = k.nseg_total;
     hipLaunchKernelGGL(cb_pc_particles_kernel, gp, dim3(256), 0, c->stream, k, 0, x, y);
+    if (k.blk) hipLaunchKernelGGL(cb_block_apply_kernel, dim3(k.ncil), dim3(256), 0, c->stream, k, 0, x, y);
     if (k.dense) CFM_TRY(body_dense_blob_apply(c, x + ns, M, y + ns, M));
     if (k.dynamic) hipLaunchKernelGGL(cb_pc_cilia_kernel, gc(k.ncil), dim3(64), 0, c->stream, k, 0, x, y);
     hipLaunchKernelGGL(cb_pc_body_kernel, dim3(k.nbody), dim3(256), 0, c->stream, k, x, y);
@@ -1127,6 +1303,7 @@ Status cilia_precond(cfm_ctx* c, int kind, const double* x, double* y) {
                          S->pc_full.p, S->tU.p, y + 3 * M);
     if (k.dynamic) hipLaunchKernelGGL(cb_pc_cilia_kernel, gc(k.ncil), dim3(64), 0, c->stream, k, 1, x, y);
     hipLaunchKernelGGL(cb_pc_particles_kernel, gp, dim3(256), 0, c->stream, k, 1, x, y);
+    if (k.blk) hipLaunchKernelGGL(cb_block_apply_kernel, dim3(k.ncil), dim3(256), 0, c->stream, k, 1, x, y);
     if (k.dense) CFM_TRY(body_dense_rigid_add(c, y + 3 * M, y + ns, M));
     CFM_CHECK_LAUNCH();
     return {};
@@ -1162,8 +1339,7 @@ static Status step_bodies(cfm_ctx* c, CiliaState* S, int nt, double* q_phase, do
   const SystemDesc sd = body_cilia_desc(c, S);
   cfm_solve_opts o = S->prm.solve;
   o.use_guess = 1;  // previous step's solution
-  if (o.precond == CFM_PC_DIAG || o.precond == CFM_PC_SELF || o.precond == CFM_PC_DENSE)
-    CFM_TRY(body_cilia_prep(c, S, o.precond));
+  if (o.precond != CFM_PC_NONE) CFM_TRY(body_cilia_prep(c, S, o.precond));
   if (S->dynamic) {
     hipLaunchKernelGGL(cilia_drive_kernel, dim3((sd.L + 255) / 256), dim3(256), 0, c->stream, nc,
                        (int)phase_slot(c, S), S->dyn_angle ? 1 : 0, S->prm.dl * (S->prm.nseg - 1),
@@ -1202,7 +1378,7 @@ void cfm_cilia_release(cfm_ctx* c) {
   for (auto* b : {&S->base, &S->phase, &S->pdot, &S->angle, &S->vphase, &S->vangle, &S->svals,
                   &S->rhs, &S->f, &S->q, &S->omega0, &S->adot, &S->tab_phase, &S->tab_angle,
                   &S->z, &S->zrhs, &S->s_table, &S->bx, &S->bq, &S->vbody, &S->fref, &S->fq,
-                  &S->fblob, &S->pc_c, &S->pc_b, &S->tU, &S->pc_full})
+                  &S->fblob, &S->pc_c, &S->pc_b, &S->tU, &S->pc_full, &S->pblk, &S->pkc, &S->dcil})
     b->release();
   if (S->host) (void)hipHostFree(S->host);
   delete S;
@@ -1302,7 +1478,7 @@ int cfm_cilia_init_bodies(cfm_ctx* c, const cfm_cilia_params* prm, const double*
     return cfail(c, make_status(CFM_EINVAL, "null argument"));
   cfm_cilia_params pr = *prm;
   const bool dense = pr.solve.precond == CFM_PC_DENSE;
-  if (dense) pr.solve.precond = CFM_PC_DIAG;  // checked as the others
+  if (dense || pr.solve.precond == CFM_PC_NEAR) pr.solve.precond = CFM_PC_DIAG;  // checked as the others
   const Status chk = check_params(c, &pr, phase, phase_dot);
   if (!chk.ok()) return cfail(c, chk);
   BodyInfo bi;

@@ -46,7 +46,7 @@ def _compare(G, sw, nt, qp_o, qa_o, qp_g, qa_g, tol=1e-8):
     assert np.allclose(sp["phase_dot"], sw.phase_dot, rtol=tol), nt
 
 
-@pytest.mark.parametrize("precond", ["diag", "none", "dense"])
+@pytest.mark.parametrize("precond", ["diag", "none", "near", "dense"])
 def test_prescribed_phase_speeds_match_oracle(precond):
     sw = swimmer(n_body=2, per=3, seed=11, dt=0.02)
     with _gpu(sw, precond) as G:
@@ -57,7 +57,7 @@ def test_prescribed_phase_speeds_match_oracle(precond):
             _compare(G, sw, nt, qp_o, qa_o, qp_g, qa_g)
 
 
-@pytest.mark.parametrize("precond", ["diag", "dense"])
+@pytest.mark.parametrize("precond", ["diag", "near", "dense"])
 def test_dynamic_phases_and_angles_match_oracle(precond):
     qp = np.loadtxt(os.path.join(GOLD, "fb_phase_generalised_forces_nseg20_sep2.6.dat"))[1:]
     qa = np.loadtxt(os.path.join(GOLD, "fb_angle_generalised_forces_nseg20_sep2.6.dat"))[1:]
