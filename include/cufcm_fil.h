@@ -382,7 +382,10 @@ int cfm_cilia_get_state(cfm_ctx* ctx, double* x, double* vel_dir_phase, double* 
  * mobility_solver.cpp:301-458).  Preconditioner CFM_PC_DIAG is the reference's block
  * inverse with 6 pi mu a and make_precon_mat's 6 x 6 Schur complement
  * (mobility_solver.cpp:1104-1515, swimmer.cpp:1175-1347); CFM_PC_SELF the same with each
- * particle's wall self block.  Steps after the first first advance phases, shape angles
+ * particle's wall self block; CFM_PC_NEAR replaces 6 pi mu a on each cilium's segments by
+ * the inverse of that cilium's own mobility block; CFM_PC_DENSE also uses the resident
+ * FP32 inverse of the blob block (built once, applied in the bodies' frame) with a full
+ * 6 n_body Schur complement.  Steps after the first first advance phases, shape angles
  * and bodies explicitly with the previous step's speeds: x_b += dt V_b,
  * q_b = exp(dt Omega_b) q_b (cilia_sim_main.cu:724-740). */
 int cfm_cilia_init_bodies(cfm_ctx* ctx, const cfm_cilia_params* p, const double* fil_ref,
