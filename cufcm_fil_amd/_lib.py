@@ -254,6 +254,7 @@ def lib():
     L.cfm_cilia_set_phases.argtypes = [_vp, _dp, _dp]
     L.cfm_cilia_init_bodies.argtypes = [_vp, ctypes.POINTER(cfm_cilia_params), _dp, _dp, _dp, _dp]
     L.cfm_cilia_get_bodies.argtypes = [_vp, _dp, _dp, _dp, _dp]
+    L.cfm_cilia_set_driving_options.argtypes = [_vp, ctypes.c_int32, ctypes.c_double, ctypes.c_uint64]
     L.cfm_batch_create.argtypes = [ctypes.POINTER(cfm_batch_config), _ip, ctypes.POINTER(_vp)]
     L.cfm_batch_destroy.argtypes = [_vp]
     L.cfm_batch_last_error.restype = ctypes.c_char_p

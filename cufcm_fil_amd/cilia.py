@@ -66,6 +66,12 @@ class PrescribedCilia:
             self.mob._h, L.ptr(qp), qp.size, L.ptr(qa), 0 if qa is None else qa.size,
             float(spring)))
 
+    def set_driving_options(self, fene=False, noise=0.0, seed=0):
+        """FENE form of the shape-angle spring and seeded noise on the phase drive
+        (FENE_MODEL, FORCE_NOISE_MAG)."""
+        L.check(self.mob._h, L.lib().cfm_cilia_set_driving_options(self.mob._h, int(bool(fene)),
+                                                                   float(noise), int(seed)))
+
     def set_s_table(self, table=None):
         """Segment parameters from a recorded table (rows: phases 2 pi k / rows), as the
         reference reads its s_values file; None returns to the bisection."""
@@ -210,6 +216,7 @@ class CiliatedBodies:
         self.close()
 
     set_driving = PrescribedCilia.set_driving
+    set_driving_options = PrescribedCilia.set_driving_options
     set_phases = PrescribedCilia.set_phases
     speeds = PrescribedCilia.speeds
     step = PrescribedCilia.step

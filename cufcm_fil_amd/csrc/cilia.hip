@@ -441,13 +441,40 @@ __device__ double cyclic_hermite(const double* tab, int n, double phase) {
 // rhs = [0 ; -q_phase ; -q_angle] with the recorded driving forces scaled by the
 // cilium's natural frequency and the linear torsional spring on the shape angle
 // (mobility_solver.cpp:301-356, FORCE_NOISE_MAG = 0, FENE_MODEL = 0).
-// The phase slots start at `off` (after the forces, and the body velocities when the
-// cilia ride on free bodies); everything before them is zero.
+// Normal deviate of (seed, step, cilium): SplitMix64 mixing of the key into two
+// 53-bit uniforms on (0, 1), then Box-Muller.
+__host__ __device__ inline unsigned long long mix64(unsigned long long z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+__device__ inline double normal_deviate(unsigned long long seed, int nt, int c) {
+  const unsigned long long key = mix64(seed ^ mix64(((unsigned long long)(unsigned)nt << 32) | (unsigned)c));
+  const unsigned long long h1 = mix64(key), h2 = mix64(key + 1ull);
+  const double u1 = ((double)(h1 >> 11) + 0.5) * 0x1.0p-53;
+  const double u2 = ((double)(h2 >> 11) + 0.5) * 0x1.0p-53;
+  return sqrt(-2.0 * log(u1)) * cos(2.0 * 3.14159265358979323846 * u2);
+}
+
+struct DriveOpts {
+  int fene;                 // 1: FENE form of the angle spring
+  double noise;             // FORCE_NOISE_MAG
+  unsigned long long seed;  // of the noise
+  int nt;                   // step
+};
+
+// rhs = [0 ; -q_phase ; -q_angle] with the recorded driving forces scaled by the
+// cilium's natural frequency, the torsional spring on the shape angle (linear, or the
+// FENE form beyond |angle| = 0.78) and optional noise on the phase drive
+// (mobility_solver.cpp:301-356).  The phase slots start at `off` (after the forces,
+// and the body velocities when the cilia ride on free bodies); everything before them
+// is zero.
 __global__ void cilia_drive_kernel(int ncil, int off, int dyn_angle, double len, double spring,
                                    const double* __restrict__ phase, const double* __restrict__ angle,
                                    const double* __restrict__ omega0,
                                    const double* __restrict__ tab_p, int np,
-                                   const double* __restrict__ tab_a, int na,
+                                   const double* __restrict__ tab_a, int na, DriveOpts dopt,
                                    double* __restrict__ rhs) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int L = off + ncil * (dyn_angle ? 2 : 1);
@@ -459,9 +486,14 @@ __global__ void cilia_drive_kernel(int ncil, int off, int dyn_angle, double len,
   const int c = (i - off) % ncil;
   const double sc = 0.5 * omega0[c] / 3.14159265358979323846;
   if (i < off + ncil) {
-    rhs[i] = -cyclic_hermite(tab_p, np, phase[c]) * sc;
+    double q = cyclic_hermite(tab_p, np, phase[c]) * sc;
+    if (dopt.noise != 0.0) q += dopt.noise * normal_deviate(dopt.seed, dopt.nt, c);
+    rhs[i] = -q;
   } else {
-    const double q = cyclic_hermite(tab_a, na, phase[c]) * sc - spring * omega0[c] * len * len * len * angle[c];
+    const double a = angle[c], amax = 0.78;
+    double fac = a;
+    if (dopt.fene && fabs(a) > amax) fac = 0.5 * a * (1.0 + a * a / (amax * amax) * exp(fabs(a) - amax));
+    const double q = cyclic_hermite(tab_a, na, phase[c]) * sc - spring * omega0[c] * len * len * len * fac;
     rhs[i] = -q;
   }
 }
@@ -1069,6 +1101,7 @@ struct CiliaState {
   // unknowns of the coupled system, driven by recorded generalised forces
   bool dynamic = false, dyn_angle = false;
   double spring = 0.0;
+  DriveOpts dopt{0, 0.0, 0ull, 0};  // cfm_cilia_set_driving_options
   DevBuf<double> omega0, adot, tab_phase, tab_angle, z, zrhs;
   int n_tab_phase = 0, n_tab_angle = 0;
   // recorded segment parameters (cfm_cilia_set_s_table)
@@ -1095,6 +1128,12 @@ static size_t system_len(const cfm_ctx* c, const CiliaState* S) {
 }
 
 static CiliaState* cil_of(cfm_ctx* c) { return reinterpret_cast<CiliaState*>(c->cilia); }
+
+static DriveOpts drive_opts(const CiliaState* S, int nt) {
+  DriveOpts d = S->dopt;
+  d.nt = nt;
+  return d;
+}
 
 static CiliaK kargs(cfm_ctx* c, CiliaState* S) {
   CiliaK k{};
@@ -1344,7 +1383,7 @@ static Status step_bodies(cfm_ctx* c, CiliaState* S, int nt, double* q_phase, do
     hipLaunchKernelGGL(cilia_drive_kernel, dim3((sd.L + 255) / 256), dim3(256), 0, c->stream, nc,
                        (int)phase_slot(c, S), S->dyn_angle ? 1 : 0, S->prm.dl * (S->prm.nseg - 1),
                        S->spring, S->phase.p, S->angle.p, S->omega0.p, S->tab_phase.p, S->n_tab_phase,
-                       S->tab_angle.p, S->n_tab_angle, S->zrhs.p);
+                       S->tab_angle.p, S->n_tab_angle, drive_opts(S, nt), S->zrhs.p);
   } else {
     hipLaunchKernelGGL(cb_rhs_kernel, dim3((sd.L + 255) / 256), dim3(256), 0, c->stream, sd.L, sd.m,
                        c->p.n_seg, S->prm.nseg, S->pdot.p, S->vphase.p, S->zrhs.p);
@@ -1604,7 +1643,7 @@ int cfm_cilia_step(cfm_ctx* c, int32_t nt, double* q_phase, double* q_angle, cfm
       hipLaunchKernelGGL(cilia_drive_kernel, dim3((sd.L + 255) / 256), dim3(256), 0, c->stream, nc,
                          3 * c->p.n_seg, S->dyn_angle ? 1 : 0, S->prm.dl * (S->prm.nseg - 1), S->spring,
                          S->phase.p, S->angle.p, S->omega0.p, S->tab_phase.p, S->n_tab_phase,
-                         S->tab_angle.p, S->n_tab_angle, S->zrhs.p);
+                         S->tab_angle.p, S->n_tab_angle, drive_opts(S, nt), S->zrhs.p);
       CFM_CHECK_LAUNCH();
       cfm_solve_opts o = S->prm.solve;
       o.system = CFM_SYS_SEG;
@@ -1687,6 +1726,18 @@ int cfm_cilia_set_driving(cfm_ctx* c, const double* q_phase_table, int32_t n_pha
   };
   Status s = run();
   return s.ok() ? CFM_OK : cfail(c, s);
+}
+
+int cfm_cilia_set_driving_options(cfm_ctx* c, int32_t spring_model, double noise, uint64_t seed) {
+  if (!c) return CFM_EINVAL;
+  CiliaState* S = cil_of(c);
+  if (!S) return cfail(c, make_status(CFM_ESTATE, "cfm_cilia_init has not been called"));
+  if ((spring_model != 0 && spring_model != 1) || !std::isfinite(noise))
+    return cfail(c, make_status(CFM_EINVAL, "spring_model is 0 (linear) or 1 (FENE); noise must be finite"));
+  S->dopt.fene = spring_model;
+  S->dopt.noise = noise;
+  S->dopt.seed = seed;
+  return CFM_OK;
 }
 
 int cfm_cilia_set_s_table(cfm_ctx* c, const double* table, int32_t rows, int32_t cols) {

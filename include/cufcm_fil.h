@@ -344,6 +344,14 @@ int cfm_cilia_step(cfm_ctx* ctx, int32_t nt, double* q_phase, double* q_angle,
 int cfm_cilia_set_driving(cfm_ctx* ctx, const double* q_phase_table, int32_t n_phase,
                           const double* q_angle_table /* may be null */, int32_t n_angle,
                           double spring);
+/* Options of the dynamic driving (mobility_solver.cpp:306-356): spring_model 0 is the
+ * linear torsional spring, 1 the FENE form (FENE_MODEL 1): beyond |angle| = 0.78 the
+ * spring's angle factor becomes 0.5 angle (1 + angle^2 / 0.78^2 exp(|angle| - 0.78)).
+ * noise (FORCE_NOISE_MAG) adds noise * N(0, 1) to each cilium's phase driving force at
+ * every step; the deviate is a counter-based function of (seed, step, cilium) --
+ * SplitMix64 mixing into two 53-bit uniforms, then Box-Muller -- so runs repeat exactly
+ * (the reference draws from an unseeded std::random_device). */
+int cfm_cilia_set_driving_options(cfm_ctx* ctx, int32_t spring_model, double noise, uint64_t seed);
 /* Segment parameters from a recorded table instead of the bisection, as the reference
  * does whenever it does not write its reference files (filament.cpp:1190-1235; the table
  * is the s_values file of input/forcing, read at cilia_sim_main.cu:339-378): rows at phases

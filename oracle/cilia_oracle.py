@@ -228,16 +228,47 @@ def cyclic_hermite(table, phase):
                  + table[hi] * (3 * t2 - 2 * t3) + d_hi * (t3 - t2))
 
 
+_M64 = (1 << 64) - 1
+
+
+def mix64(z):
+    """SplitMix64 finaliser (Steele, Lea & Flood 2014) on 64-bit integers."""
+    z = (z + 0x9E3779B97F4A7C15) & _M64
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & _M64
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & _M64
+    return z ^ (z >> 31)
+
+
+def normal_deviate(seed, nt, c):
+    """N(0, 1) deviate of (seed, step, cilium): two 53-bit uniforms on (0, 1) from the
+    mixed key, then Box-Muller."""
+    key = mix64((seed ^ mix64(((nt & 0xFFFFFFFF) << 32) | (c & 0xFFFFFFFF))) & _M64)
+    h1, h2 = mix64(key), mix64((key + 1) & _M64)
+    u1 = ((h1 >> 11) + 0.5) * 2.0 ** -53
+    u2 = ((h2 >> 11) + 0.5) * 2.0 ** -53
+    return math.sqrt(-2.0 * math.log(u1)) * math.cos(2.0 * math.pi * u2)
+
+
+def fene_factor(angle, max_angle=0.78):
+    """Angle factor of the FENE torsional spring (mobility_solver.cpp:320-330)."""
+    if abs(angle) > max_angle:
+        return 0.5 * angle * (1.0 + angle ** 2 / max_angle ** 2 * math.exp(abs(angle) - max_angle))
+    return angle
+
+
 def driving_forces(phase, angle, omega0, q_phase_table, q_angle_table, spring=0.0, dl=2.6,
-                   nseg=20):
-    """Generalised driving forces of one cilium (mobility_solver.cpp:306-356 with
-    FORCE_NOISE_MAG = 0 and the linear torsional spring)."""
+                   nseg=20, fene=False, noise=0.0, seed=0, nt=0, c=0):
+    """Generalised driving forces of cilium c at step nt (mobility_solver.cpp:306-356):
+    the interpolated recorded forces scaled by omega0 / 2 pi, the torsional spring
+    (linear, or the FENE form) and noise * N(0, 1) on the phase drive."""
     L = dl * (nseg - 1)
     qp = cyclic_hermite(q_phase_table, phase) * 0.5 * omega0 / math.pi
+    if noise != 0.0:
+        qp += noise * normal_deviate(seed, nt, c)
     qa = None
     if q_angle_table is not None:
         qa = cyclic_hermite(q_angle_table, phase) * 0.5 * omega0 / math.pi
-        qa -= spring * omega0 * L ** 3 * angle
+        qa -= spring * omega0 * L ** 3 * (fene_factor(angle) if fene else angle)
     return qp, qa
 
 

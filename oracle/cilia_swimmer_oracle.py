@@ -30,7 +30,7 @@ preconditioner against the exact inverse of the saddle matrix with M replaced by
 (6 pi mu a)^-1 I, make_precon_mat against that matrix's Schur complement, the solve
 against force- and torque-free balance and the rigid-body identities (a swimmer
 translated moves identically; rotated, its velocities rotate), and the dynamic-phase
-solve against cilia_oracle.coupled_solve when the body is held fixed.
+solve by each cilium's generalised force equalling its interpolated drive.
 """
 from __future__ import annotations
 
@@ -86,6 +86,7 @@ class Swimmers:
         self.nseg, self.dl, self.dt, self.beat, self.s_tol = nseg, dl, dt, beat, s_tol
         self.mu, self.a = mu, a
         self.driving = None   # (q_phase_table, q_angle_table or None, spring)
+        self.fene, self.noise, self.seed, self.nt = False, 0.0, 0, 0
 
     def geometry(self):
         """Segment positions (n_seg x 3), blob positions (n_blob x 3) and per-cilium
@@ -156,7 +157,8 @@ class Swimmers:
         qt, qa, spring = self.driving
         for c in range(self.nc):
             qp_, qa_ = co.driving_forces(self.phase[c], self.angle[c], self.omega0[c], qt, qa,
-                                         spring, self.dl, self.nseg)
+                                         spring, self.dl, self.nseg, self.fene, self.noise,
+                                         self.seed, self.nt, c)
             r[nu + c] = -qp_
             if dyn_angle:
                 r[nu + self.nc + c] = -qa_
@@ -307,6 +309,7 @@ class Swimmers:
         """One step (explicit update first for nt > 0); returns (solution, q_phase, q_angle)."""
         dynamic = self.driving is not None
         dyn_angle = dynamic and self.driving[1] is not None
+        self.nt = nt
         if nt > 0:
             self.phase += self.dt * self.phase_dot
             if dyn_angle:

@@ -142,3 +142,22 @@ def test_dynamic_phases_balance_the_driving_force():
     drive = [co.driving_forces(sw.phase[c], 0.0, sw.omega0[c], tab, None)[0] for c in range(sw.nc)]
     assert np.allclose(qp, drive, rtol=1e-9)
     assert np.all(sw.phase_dot > 0)
+
+
+def test_driving_options_fene_spring_and_seeded_noise():
+    # SplitMix64: the first output of the generator seeded with 0
+    assert co.mix64(0) == 0xE220A8397B1DCDAF
+    # FENE factor: linear up to 0.78, continuous there, odd
+    for a in (0.1, -0.5, 0.78):
+        assert co.fene_factor(a) == a
+    for a in (0.78 + 1e-9, 1.0, 1.5):
+        assert abs(co.fene_factor(a) + co.fene_factor(-a)) < 1e-15
+    assert abs(co.fene_factor(0.78 + 1e-9) - 0.78) < 1e-8
+    assert co.fene_factor(1.0) == 0.5 * (1 + 1 / 0.78 ** 2 * math.exp(0.22))
+    # deviates: reproducible, distinct per (seed, step, cilium), standard normal
+    assert co.normal_deviate(7, 3, 5) == co.normal_deviate(7, 3, 5)
+    assert len({co.normal_deviate(7, 3, c) for c in range(50)}) == 50
+    assert co.normal_deviate(7, 3, 5) != co.normal_deviate(8, 3, 5) != co.normal_deviate(7, 4, 5)
+    z = np.array([co.normal_deviate(11, nt, c) for nt in range(200) for c in range(250)])
+    assert abs(z.mean()) < 0.015 and abs(z.var() - 1.0) < 0.02
+    assert abs(np.mean(z ** 4) - 3.0) < 0.08 and abs(np.mean(np.abs(z) < 1.0) - 0.6827) < 0.01

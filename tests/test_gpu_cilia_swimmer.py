@@ -80,6 +80,27 @@ def test_dynamic_phases_and_angles_match_oracle(precond):
             assert np.allclose(sp["angle"], sw.angle, atol=1e-12)
 
 
+def test_fene_spring_and_noise_match_oracle():
+    qp = np.loadtxt(os.path.join(GOLD, "fb_phase_generalised_forces_nseg20_sep2.6.dat"))[1:]
+    qa = np.loadtxt(os.path.join(GOLD, "fb_angle_generalised_forces_nseg20_sep2.6.dat"))[1:]
+    qp, qa = qp * (NSEG / 20.0) ** 3, qa * (NSEG / 20.0) ** 3
+    sw = swimmer(n_body=1, per=4, seed=13, dt=0.02)
+    sw.driving = (qp, qa, 0.05)
+    sw.fene, sw.noise, sw.seed = True, 3.0, 12345
+    sw.angle[:] = [0.9, -1.1, 0.3, 0.8]     # two beyond the FENE threshold
+    with _gpu(sw, "diag", rtol=1e-13) as G:
+        G.set_driving(qp, qa, spring=0.05)
+        G.set_driving_options(fene=True, noise=3.0, seed=12345)
+        G.set_phases(angle=sw.angle)
+        for nt in range(3):
+            qp_g, qa_g, stats = G.step(nt)
+            _, qp_o, qa_o = sw.step(nt)
+            assert stats["converged"], stats
+            _compare(G, sw, nt, qp_o, qa_o, qp_g, qa_g, tol=1e-7)
+            assert np.allclose(G.speeds()["angle_dot"], sw.angle_dot, rtol=1e-7,
+                               atol=1e-7 * np.abs(sw.angle_dot).max())
+
+
 def _big(body_q, phase, precond="diag"):
     R = 20.0
     blob_ref = geo.fibonacci_sphere(1500, R)
