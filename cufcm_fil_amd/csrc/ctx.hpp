@@ -105,18 +105,24 @@ struct NearField {
 // n = 3 n_blob, rows and columns in the blob-system vector order [component][blob],
 // leading dimension ld (a multiple of 64 floats, padding zero).  Built once per set
 // of blob positions from an FP64 Cholesky factorisation.
+// P = M_bb^-1 kept as the upper triangle of square tiles (tile (I, J), I <= J, stored
+// contiguously, row-major inside), FP32.
 struct DenseInverse {
   bool valid = false;
   int n = 0;
-  size_t ld = 0;
-  DevBuf<float> P;
+  int nt = 0;               // tiles per side
+  DevBuf<float> P;          // packed tiles
+  DevBuf<int> tiles;        // (I, J) of every stored tile
+  DevBuf<double> part;      // per (tile row, tile column) partial products
   double build_seconds = 0.0;
   int generation = 0;  // bumped by every build: caches derived from P compare it
   void release() {
     P.release();
+    tiles.release();
+    part.release();
     valid = false;
     n = 0;
-    ld = 0;
+    nt = 0;
   }
 };
 

@@ -4,14 +4,17 @@
 // With the body held fixed the blob positions never change, so M_bb is one matrix
 // for the whole run.  It is assembled once in FP64 (the same pair functions as the
 // apply), factorised and inverted with rocSOLVER (dpotrf + dpotri: M_bb is
-// symmetric positive definite), rounded to FP32 and kept resident in HBM
-// (4 n^2 bytes, n = 3 n_blob: 1.3 GB at 6,000 blobs).  Applying it is one GEMV
-// with FP64 accumulation, so the preconditioner is linear to FP64 rounding and the
+// symmetric positive definite), rounded to FP32 and kept resident in HBM as the upper
+// triangle of 128 x 128 tiles (2 n^2 bytes, n = 3 n_blob: 0.65 GB at 6,000 blobs).
+// Applying it reads every stored tile once: a tile (I, J) gives P_IJ x_J to row block I
+// and, off the diagonal, P_IJ^T x_I to row block J; the partial products are summed
+// in a fixed order, in FP64, so the preconditioner is linear to FP64 rounding and the
 // FP32 rounding of the entries only perturbs A P^-1 around the identity.
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
 
 #include <chrono>
+#include <vector>
 
 #include "ctx.hpp"
 #include "densepc.hpp"
@@ -21,7 +24,8 @@ namespace cfm {
 
 namespace {
 
-constexpr int kRows = 8;  // matrix rows per GEMV workgroup
+constexpr int kT = 128;  // tile side
+constexpr int kG = 16;   // row groups of 32 threads in a tile's workgroup
 
 // A (column-major, leading dimension lda) <- M_bb in the [component][blob] order.
 // Block row i = blob index over threads, block column j = blockIdx.y.
@@ -56,58 +60,92 @@ __global__ __launch_bounds__(256) void dense_assemble_kernel(const double* __res
   }
 }
 
-// P (FP32, rows of length ld, zero padding) <- the full symmetric inverse whose lower
-// triangle dpotri left in A.  Thread r of column c writes P[c * ld + r] (P is
-// symmetric, so this is also row-major).
-__global__ __launch_bounds__(256) void dense_round_kernel(const double* __restrict__ A, size_t lda,
-                                                          int n, float* __restrict__ P, size_t ld) {
-  const size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const size_t c = blockIdx.y;
-  if (r >= ld) return;
-  float v = 0.0f;
-  if (r < (size_t)n) v = (float)(r >= c ? A[r + c * lda] : A[c + r * lda]);
-  P[c * ld + r] = v;
+// Packed tiles <- the symmetric inverse whose lower triangle dpotri left in A; entries
+// beyond n are zero.  Block (J, I) of the grid writes row r = threadIdx.x of tile (I, J)
+// when I <= J.
+__global__ __launch_bounds__(kT) void dense_pack_kernel(const double* __restrict__ A, size_t lda, int n,
+                                                        int nt, float* __restrict__ P) {
+  const int J = blockIdx.x, I = blockIdx.y;
+  if (I > J) return;
+  const size_t t = (size_t)I * nt - (size_t)I * (I - 1) / 2 + (J - I);  // packed index
+  float* T = P + t * kT * kT;
+  for (int r = 0; r < kT; ++r) {
+    const int c = threadIdx.x;
+    const int gi = I * kT + r, gj = J * kT + c;
+    double v = 0.0;
+    if (gi < n && gj < n) v = gi >= gj ? A[gi + (size_t)gj * lda] : A[gj + (size_t)gi * lda];
+    T[(size_t)r * kT + c] = (float)v;
+  }
 }
 
-// y = P x: kRows rows per workgroup, float4 loads of P, FP64 products and sums.
-__global__ __launch_bounds__(256) void dense_gemv_kernel(int n, size_t ld,
+// One workgroup per stored tile (I, J): part[I][J] = P_IJ x_J and, for I < J,
+// part[J][I] = P_IJ^T x_I.  Thread (g, q) reads rows g, g + kG, ... of column quad q
+// (coalesced float4 loads), keeps its column sums in registers; row sums go through LDS.
+__global__ __launch_bounds__(32 * kG) void dense_symv_kernel(int n, int nt, const int* __restrict__ tiles,
                                                          const float* __restrict__ P,
                                                          const double* __restrict__ xv,
-                                                         double* __restrict__ yv) {
-  __shared__ double red[4][kRows];
-  const int r0 = blockIdx.x * kRows;
-  const int nr = min(kRows, n - r0);
-  double acc[kRows];
+                                                         double* __restrict__ part) {
+  __shared__ double rowp[kT][33];
+  __shared__ double colp[kG][kT];
+  const int t = blockIdx.x;
+  const int I = tiles[2 * t], J = tiles[2 * t + 1];
+  const float4* T = reinterpret_cast<const float4*>(P + (size_t)t * kT * kT);
+  const int q = threadIdx.x & 31, g = threadIdx.x >> 5;
+  double xj[4];
 #pragma unroll
-  for (int r = 0; r < kRows; ++r) acc[r] = 0.0;
-  const int n4 = (int)(ld / 4);
-  const float4* P4 = reinterpret_cast<const float4*>(P + (size_t)r0 * ld);
-  for (int c4 = threadIdx.x; c4 < n4; c4 += 256) {
-    const int c = 4 * c4;
-    const double x0 = c < n ? xv[c] : 0.0;
-    const double x1 = c + 1 < n ? xv[c + 1] : 0.0;
-    const double x2 = c + 2 < n ? xv[c + 2] : 0.0;
-    const double x3 = c + 3 < n ? xv[c + 3] : 0.0;
-#pragma unroll
-    for (int r = 0; r < kRows; ++r) {
-      if (r < nr) {
-        const float4 p = P4[(size_t)r * n4 + c4];
-        acc[r] += (double)p.x * x0 + (double)p.y * x1 + (double)p.z * x2 + (double)p.w * x3;
-      }
-    }
+  for (int k = 0; k < 4; ++k) {
+    const int col = J * kT + 4 * q + k;
+    xj[k] = col < n ? xv[col] : 0.0;
   }
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-#pragma unroll
-  for (int r = 0; r < kRows; ++r) {
-    double v = acc[r];
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    if (lane == 0) red[wave][r] = v;
+  double ca[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
+  for (int k = 0; k < kT / kG; ++k) {
+    const int r = g + kG * k;
+    const float4 p = T[r * (kT / 4) + q];
+    const int gr = I * kT + r;
+    const double xi = gr < n ? xv[gr] : 0.0;
+    rowp[r][q] = (double)p.x * xj[0] + (double)p.y * xj[1] + (double)p.z * xj[2] + (double)p.w * xj[3];
+    ca[0] += (double)p.x * xi;
+    ca[1] += (double)p.y * xi;
+    ca[2] += (double)p.z * xi;
+    ca[3] += (double)p.w * xi;
   }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) colp[g][4 * q + k] = ca[k];
   __syncthreads();
-  if (threadIdx.x < nr) {
-    const int r = threadIdx.x;
-    yv[r0 + r] = (red[0][r] + red[1][r]) + (red[2][r] + red[3][r]);
+  const int tid = threadIdx.x;
+  if (tid < kT) {
+    double s = 0.0;
+    for (int k = 0; k < 32; ++k) s += rowp[tid][k];
+    part[((size_t)I * nt + J) * kT + tid] = s;
+  } else if (I != J && tid < 2 * kT) {
+    const int c = tid - kT;
+    double s = 0.0;
+    for (int k = 0; k < kG; ++k) s += colp[k][c];
+    part[((size_t)J * nt + I) * kT + c] = s;
   }
+}
+
+// y_r = sum over tile columns J of part[r / kT][J][r % kT]: a workgroup takes 64 rows
+// of one tile row, wave w the columns J = w mod 4, the four sums added in order.
+__global__ __launch_bounds__(256) void dense_sum_kernel(int n, int nt, const double* __restrict__ part,
+                                                        double* __restrict__ yv) {
+  __shared__ double red[4][64];
+  const int K = blockIdx.x, half = blockIdx.y;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int rr = half * 64 + lane;
+  const double* p = part + (size_t)K * nt * kT + rr;
+  double s0 = 0.0, s1 = 0.0;
+  int J = w;
+  for (; J + 4 < nt; J += 8) {
+    s0 += p[(size_t)J * kT];
+    s1 += p[(size_t)(J + 4) * kT];
+  }
+  if (J < nt) s0 += p[(size_t)J * kT];
+  red[w][lane] = s0 + s1;
+  __syncthreads();
+  const int r = K * kT + rr;
+  if (w == 0 && r < n) yv[r] = (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
 }
 
 }  // namespace
@@ -118,8 +156,9 @@ Status dense_build(cfm_ctx* c, DenseInverse& d) {
   if (m <= 0) return make_status(CFM_EINVAL, "CFM_PC_DENSE needs blobs");
   const int n = 3 * m;
   const size_t lda = (size_t)n;
-  const size_t ld = ((size_t)n + 63) / 64 * 64;
-  const size_t need = sizeof(double) * lda * n + sizeof(float) * ld * n;
+  const int nt = (n + kT - 1) / kT;
+  const size_t ntiles = (size_t)nt * (nt + 1) / 2;
+  const size_t need = sizeof(double) * lda * n + sizeof(float) * ntiles * kT * kT;
   size_t free_b = 0, total_b = 0;
   CFM_HIP(hipMemGetInfo(&free_b, &total_b));
   if (need + need / 8 > free_b)
@@ -159,13 +198,21 @@ Status dense_build(cfm_ctx* c, DenseInverse& d) {
   CFM_HIP(hipMemcpyAsync(&hinfo, info.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
   CFM_HIP(hipStreamSynchronize(c->stream));
   if (hinfo != 0) return make_status(CFM_ENONFINITE, "singular blob mobility (dpotri info %d)", hinfo);
-  CFM_TRY(d.P.alloc(ld * n));
-  hipLaunchKernelGGL(dense_round_kernel, dim3((unsigned)((ld + 255) / 256), n), dim3(256), 0,
-                     c->stream, A.p, lda, n, d.P.p, ld);
+  CFM_TRY(d.P.alloc(ntiles * kT * kT));
+  CFM_TRY(d.tiles.alloc(2 * ntiles));
+  CFM_TRY(d.part.alloc((size_t)nt * nt * kT));
+  std::vector<int> tl(2 * ntiles);
+  for (int I = 0, t = 0; I < nt; ++I)
+    for (int J = I; J < nt; ++J, ++t) {
+      tl[2 * t] = I;
+      tl[2 * t + 1] = J;
+    }
+  CFM_HIP(hipMemcpyAsync(d.tiles.p, tl.data(), sizeof(int) * tl.size(), hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(dense_pack_kernel, dim3(nt, nt), dim3(kT), 0, c->stream, A.p, lda, n, nt, d.P.p);
   CFM_CHECK_LAUNCH();
   CFM_HIP(hipStreamSynchronize(c->stream));
   d.n = n;
-  d.ld = ld;
+  d.nt = nt;
   d.valid = true;
   ++d.generation;
   d.build_seconds =
@@ -174,8 +221,10 @@ Status dense_build(cfm_ctx* c, DenseInverse& d) {
 }
 
 Status dense_apply(cfm_ctx* c, const DenseInverse& d, const double* x, double* y) {
-  hipLaunchKernelGGL(dense_gemv_kernel, dim3((d.n + kRows - 1) / kRows), dim3(256), 0, c->stream,
-                     d.n, d.ld, d.P.p, x, y);
+  const unsigned ntiles = (unsigned)((size_t)d.nt * (d.nt + 1) / 2);
+  hipLaunchKernelGGL(dense_symv_kernel, dim3(ntiles), dim3(32 * kG), 0, c->stream, d.n, d.nt, d.tiles.p,
+                     d.P.p, x, d.part.p);
+  hipLaunchKernelGGL(dense_sum_kernel, dim3(d.nt, kT / 64), dim3(256), 0, c->stream, d.n, d.nt, d.part.p, y);
   CFM_CHECK_LAUNCH();
   return {};
 }
