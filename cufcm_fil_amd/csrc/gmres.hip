@@ -483,10 +483,11 @@ Status gmres_core(cfm_ctx* c, Krylov& K, const SystemDesc& s, int prec, int pc, 
       break;
     }
     // With the FP64 operator and the estimate of the final step in hand (read
-    // synchronously), an estimate at most half the target ends the
-    // solve without another product: the recurrence residual and b - A x then agree
-    // to rounding (the reference stops on the estimate alone, :2510).
-    if (sync_check && prec == CFM_FP64 && last_est <= 0.5 * target_abs) {
+    // synchronously), an estimate within the target (with a 10% margin) ends the
+    // solve without another product: with CGS2 the recurrence residual and b - A x
+    // agree to rounding, far below the target (the reference stops on the estimate
+    // alone, :2510).
+    if (sync_check && prec == CFM_FP64 && last_est <= 0.9 * target_abs) {
       nr = last_est;
       break;
     }
