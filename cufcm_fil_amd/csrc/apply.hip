@@ -196,17 +196,26 @@ __global__ __launch_bounds__(kWG) void rpy_apply_kernel(KArgs a) {
   }
 }
 
-// Sum the split partials in slice order, add the self term, scale and store.
+// Sum the split partials, add the self term, scale and store.  A workgroup takes 64
+// targets; wave w sums the slices s = w mod 4 in order, the four sums are added in
+// wave order (a fixed order for every launch).
 __global__ __launch_bounds__(256) void rpy_finalize_kernel(KArgs a) {
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  const int i = a.tgt_begin + idx;
-  if (i >= a.tgt_end) return;
+  __shared__ double red[kWaves][6][64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int i = a.tgt_begin + blockIdx.x * 64 + lane;
+  const bool valid = i < a.tgt_end;
   const bool is_seg = i < a.n_seg;
   const int td = is_seg ? a.seg_dof : 3;
   const size_t n = a.n;
-  double out[6] = {0, 0, 0, 0, 0, 0};
-  for (int s = 0; s < a.split; ++s)
-    for (int k = 0; k < td; ++k) out[k] += a.partial[((size_t)s * 6 + k) * n + i];
+  double part[6] = {0, 0, 0, 0, 0, 0};
+  if (valid)
+    for (int s = w; s < a.split; s += kWaves)
+      for (int k = 0; k < td; ++k) part[k] += a.partial[((size_t)s * 6 + k) * n + i];
+  for (int k = 0; k < 6; ++k) red[w][k][lane] = part[k];
+  __syncthreads();
+  if (w != 0 || !valid) return;
+  double out[6];
+  for (int k = 0; k < 6; ++k) out[k] = (red[0][k][lane] + red[1][k][lane]) + (red[2][k][lane] + red[3][k][lane]);
   const double zi = a.z[i];
   if (td == 6) add_self<6>(a, i, zi, out); else add_self<3>(a, i, zi, out);
   const double scale = a.wall ? 1.0 / a.mu : 1.0;
@@ -337,7 +346,7 @@ Status launch_apply(cfm_ctx* c, const ApplyPlan& plan) {
   }
   if (split > 1) {
     const int cnt = plan.tgt_end - plan.tgt_begin;
-    hipLaunchKernelGGL(rpy_finalize_kernel, dim3((cnt + 255) / 256), dim3(256), 0, c->stream, a);
+    hipLaunchKernelGGL(rpy_finalize_kernel, dim3((cnt + 63) / 64), dim3(256), 0, c->stream, a);
     CFM_CHECK_LAUNCH();
   }
   return {};
