@@ -91,6 +91,8 @@ class cfm_fil_params(ctypes.Structure):
         ("pair_precision", ctypes.c_int32),
         ("guess", ctypes.c_int32),
         ("blob_solve", cfm_solve_opts),
+        ("overlap", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
 
 

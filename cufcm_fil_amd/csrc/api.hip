@@ -290,6 +290,13 @@ int cfm_destroy(cfm_ctx* c) {
   for (hipEvent_t e : c->prof_events) (void)hipEventDestroy(e);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->side) {
+    (void)hipStreamSynchronize(c->side);
+    (void)hipEventDestroy(c->ev_fork);
+    (void)hipEventDestroy(c->ev_join);
+    (void)hipStreamDestroy(c->side);
+  }
+  c->partial_side.release();
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return CFM_OK;

@@ -254,7 +254,7 @@ static void plan_cost(const cfm_ctx* c, const ApplyPlan& plan, bool fp32, double
   (void)w;
 }
 
-static Status prof_begin(cfm_ctx* c, hipEvent_t* e0, hipEvent_t* e1) {
+static Status prof_begin(cfm_ctx* c, hipEvent_t* e0, hipEvent_t* e1, hipStream_t st) {
   if (c->prof_used + 2 > c->prof_events.size()) {
     for (int k = 0; k < 64; ++k) {
       hipEvent_t e;
@@ -265,7 +265,25 @@ static Status prof_begin(cfm_ctx* c, hipEvent_t* e0, hipEvent_t* e1) {
   *e0 = c->prof_events[c->prof_used];
   *e1 = c->prof_events[c->prof_used + 1];
   c->prof_used += 2;
-  CFM_HIP(hipEventRecord(*e0, c->stream));
+  CFM_HIP(hipEventRecord(*e0, st));
+  return {};
+}
+
+Status side_fork(cfm_ctx* c) {
+  if (!c->side) {
+    CFM_HIP(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+    CFM_HIP(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+    CFM_HIP(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
+  }
+  CFM_HIP(hipEventRecord(c->ev_fork, c->stream));
+  CFM_HIP(hipStreamWaitEvent(c->side, c->ev_fork, 0));
+  return {};
+}
+
+Status side_join(cfm_ctx* c) {
+  if (!c->side) return {};
+  CFM_HIP(hipEventRecord(c->ev_join, c->side));
+  CFM_HIP(hipStreamWaitEvent(c->stream, c->ev_join, 0));
   return {};
 }
 
@@ -314,30 +332,33 @@ Status launch_apply(cfm_ctx* c, const ApplyPlan& plan) {
   if (tiles == 0) return {};
   int split = std::max(1, c->split);
   a.split = split;
+  if (plan.side && !c->side) return make_status(CFM_ESTATE, "side stream launch before side_fork");
+  const hipStream_t stream = plan.side ? c->side : c->stream;
   if (split > 1) {
-    CFM_TRY(p.partial.alloc((size_t)split * 6 * p.n));
-    a.partial = p.partial.p;
+    DevBuf<double>& part = plan.side ? c->partial_side : p.partial;
+    CFM_TRY(part.alloc((size_t)split * 6 * p.n));
+    a.partial = part.p;
   }
   const bool fp32 = c->cfg.precision == CFM_FP32C;
   const bool wall = c->cfg.wall != 0;
   const int sd = p.seg_dof;
   hipEvent_t pe0 = nullptr, pe1 = nullptr;
-  if (c->prof_on) CFM_TRY(prof_begin(c, &pe0, &pe1));
+  if (c->prof_on) CFM_TRY(prof_begin(c, &pe0, &pe1, stream));
   Status s;
   if (fp32) {
-    if (sd == 6) s = wall ? launch_t<float, 6, true>(a, tiles, split, c->stream)
-                          : launch_t<float, 6, false>(a, tiles, split, c->stream);
-    else s = wall ? launch_t<float, 3, true>(a, tiles, split, c->stream)
-                  : launch_t<float, 3, false>(a, tiles, split, c->stream);
+    if (sd == 6) s = wall ? launch_t<float, 6, true>(a, tiles, split, stream)
+                          : launch_t<float, 6, false>(a, tiles, split, stream);
+    else s = wall ? launch_t<float, 3, true>(a, tiles, split, stream)
+                  : launch_t<float, 3, false>(a, tiles, split, stream);
   } else {
-    if (sd == 6) s = wall ? launch_t<double, 6, true>(a, tiles, split, c->stream)
-                          : launch_t<double, 6, false>(a, tiles, split, c->stream);
-    else s = wall ? launch_t<double, 3, true>(a, tiles, split, c->stream)
-                  : launch_t<double, 3, false>(a, tiles, split, c->stream);
+    if (sd == 6) s = wall ? launch_t<double, 6, true>(a, tiles, split, stream)
+                          : launch_t<double, 6, false>(a, tiles, split, stream);
+    else s = wall ? launch_t<double, 3, true>(a, tiles, split, stream)
+                  : launch_t<double, 3, false>(a, tiles, split, stream);
   }
   CFM_TRY(s);
   if (c->prof_on) {
-    CFM_HIP(hipEventRecord(pe1, c->stream));
+    CFM_HIP(hipEventRecord(pe1, stream));
     double fl = 0.0, by = 0.0;
     plan_cost(c, plan, fp32, &fl, &by);
     c->prof_flops.push_back(fl);
@@ -346,7 +367,7 @@ Status launch_apply(cfm_ctx* c, const ApplyPlan& plan) {
   }
   if (split > 1) {
     const int cnt = plan.tgt_end - plan.tgt_begin;
-    hipLaunchKernelGGL(rpy_finalize_kernel, dim3((cnt + 63) / 64), dim3(256), 0, c->stream, a);
+    hipLaunchKernelGGL(rpy_finalize_kernel, dim3((cnt + 63) / 64), dim3(256), 0, stream, a);
     CFM_CHECK_LAUNCH();
   }
   return {};
@@ -374,6 +395,7 @@ Status tune_split(cfm_ctx* c, const ApplyPlan& pl, int heuristic, int* best) {
     if (s <= stiles && s != heuristic) cand.push_back(s);
   CFM_TRY(c->tune_v.alloc((size_t)6 * p.n));
   ApplyPlan t = pl;
+  t.side = false;  // timed on the main stream
   t.acc_seg = t.acc_blob = false;
   t.v_ext = c->tune_v.p;
   t.v_ld = p.n;

@@ -137,6 +137,9 @@ struct ApplyPlan {
   int f_ld = 0, f_off = 0;
   double* v_ext = nullptr;
   int v_ld = 0, v_off = 0;
+  // Launch on the context's side stream (with its own split partials), to overlap
+  // with work queued on the main stream; the caller orders the two with events.
+  bool side = false;
 };
 
 }  // namespace cfm
@@ -152,6 +155,9 @@ struct cfm_ctx {
   cfm_config cfg{};
   int device = 0;
   hipStream_t stream = nullptr;
+  hipStream_t side = nullptr;           // second stream (default priority), created on demand
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  cfm::DevBuf<double> partial_side;     // split partials of side-stream launches
   cfm::Particles p;
   bool have_positions = false;
   int split = 1;                        // source split across workgroups

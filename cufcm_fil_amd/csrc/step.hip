@@ -24,6 +24,9 @@ namespace cfm {
 
 Status launch_apply(cfm_ctx* c, const ApplyPlan& plan);
 int choose_split(cfm_ctx* c, const ApplyPlan& pl);
+ApplyPlan plan_for(const cfm_ctx* c, int which);
+Status side_fork(cfm_ctx* c);
+Status side_join(cfm_ctx* c);
 Status upload_aos(cfm_ctx* c, const double* host, int cnt, int dof, double* dst, int ld, int off);
 Status download_aos(cfm_ctx* c, double* host, int cnt, int dof, const double* src, int ld, int off);
 Status gnn_guess(cfm_ctx* c, const double* rhs, double* x0, bool* used);
@@ -172,52 +175,91 @@ static Status velocities(cfm_ctx* c, FilState* F, cfm_step_stats* st) {
     pl.src_seg = true;
     return apply_plan(c, pl, prec);
   }
-  // segments and blobs from segment loads, one launch
-  ApplyPlan pl;
-  pl.tgt_begin = 0;
-  pl.tgt_end = p.n;
-  pl.src_seg = true;
-  CFM_TRY(apply_plan(c, pl, prec));
-  // blob forces: Mbb f_b = -v_b (body held fixed, so K V_body = 0)
+  // With overlap, the blob solve starts from the blob rows of M f_seg alone while the
+  // segment rows (the largest product) run on the side stream, joined before
+  // v_seg += Msb f_b; the two touch disjoint rows of v, and the blob solve reads and
+  // writes only blob rows of f and its own vectors.  Every product's source split is
+  // then tuned (first call) on the main stream before the fork, never beside the side
+  // launch.
   const SystemDesc s = system_desc(c, CFM_SYS_BLOB);
-  CFM_TRY(F->rhs_b.alloc(s.L));
-  CFM_TRY(F->fb_prev.alloc(s.L));
-  CFM_TRY(dev_copy_rows(c, p.v.p, (size_t)p.n, p.n_seg, F->rhs_b.p, (size_t)s.m, 0, 3, s.m));
-  CFM_TRY(dev_axpby(c, -1.0, F->rhs_b.p, 0.0, F->rhs_b.p, F->rhs_b.p, s.L));
-  if (F->prm.guess == CFM_GUESS_PROJECTED || F->prm.guess == CFM_GUESS_LEARNED) {
-    History& h = F->hist;
-    if (h.L != s.L) {
-      h.L = s.L;
-      h.count = h.head = 0;
-      CFM_TRY(h.B.alloc((size_t)History::H * s.L));
-      CFM_TRY(h.X.alloc((size_t)History::H * s.L));
-      CFM_TRY(h.g.alloc(History::H));
-      CFM_TRY(h.grow.alloc(History::H));
-      CFM_TRY(h.coef.alloc(History::H));
-      CFM_TRY(h.partial.alloc(dev_dots_partial_size(s.L, History::H)));
-      h.pending = -1;
-      if (!h.host) CFM_HIP(hipHostMalloc((void**)&h.host, sizeof(double) * 3 * History::H, hipHostMallocDefault));
+  ApplyPlan bs;
+  bs.tgt_begin = p.n_seg;
+  bs.tgt_end = p.n;
+  bs.src_seg = true;
+  ApplyPlan ss;
+  ss.tgt_begin = 0;
+  ss.tgt_end = p.n_seg;
+  ss.src_seg = true;
+  ss.side = true;
+  if (F->prm.overlap) {
+    const int saved = c->cfg.precision;
+    c->cfg.precision = prec;
+    (void)choose_split(c, bs);
+    (void)choose_split(c, ss);
+    c->cfg.precision = CFM_FP64;
+    (void)choose_split(c, plan_for(c, CFM_BB));
+    if (F->prm.blob_solve.mode == CFM_SOLVE_MIXED) {
+      c->cfg.precision = CFM_FP32C;
+      (void)choose_split(c, plan_for(c, CFM_BB));
     }
-    bool used = false;
-    CFM_TRY(history_guess(c, h, F->rhs_b.p, F->fb_prev.p, &used));
+    c->cfg.precision = saved;
   }
-  cfm_solve_opts o = F->prm.blob_solve;
-  o.system = CFM_SYS_BLOB;
-  o.use_guess = 1;
-  if (!c->krylov) c->krylov = new Krylov();
-  cfm_solve_stats ss{};
-  // keep the segment velocities: the blob solve overwrites only blob rows of p.v
-  CFM_TRY(solve(c, *c->krylov, s, o, F->rhs_b.p, F->fb_prev.p, &ss, nullptr));
-  if (F->prm.guess == CFM_GUESS_PROJECTED || F->prm.guess == CFM_GUESS_LEARNED)
-    CFM_TRY(history_push(c, F->hist, F->rhs_b.p, F->fb_prev.p));
-  CFM_TRY(gnn_record(c, F->rhs_b.p, F->fb_prev.p));
-  if (st) {
-    st->gmres_iterations += ss.iterations;
-    st->gmres_solves += 1;
-    st->max_gmres_rel_residual = std::max(st->max_gmres_rel_residual, ss.rel_residual);
-  }
+  const bool overlap = F->prm.overlap != 0;
+  if (overlap) CFM_TRY(side_fork(c));
+  auto blob_part = [&]() -> Status {
+    if (overlap) {
+      CFM_TRY(apply_plan(c, bs, prec));
+      CFM_TRY(apply_plan(c, ss, prec));
+    } else {  // segments and blobs from the segment loads, one launch
+      ApplyPlan all;
+      all.tgt_begin = 0;
+      all.tgt_end = p.n;
+      all.src_seg = true;
+      CFM_TRY(apply_plan(c, all, prec));
+    }
+    // blob forces: Mbb f_b = -v_b (body held fixed, so K V_body = 0)
+    CFM_TRY(F->rhs_b.alloc(s.L));
+    CFM_TRY(F->fb_prev.alloc(s.L));
+    CFM_TRY(dev_copy_rows(c, p.v.p, (size_t)p.n, p.n_seg, F->rhs_b.p, (size_t)s.m, 0, 3, s.m));
+    CFM_TRY(dev_axpby(c, -1.0, F->rhs_b.p, 0.0, F->rhs_b.p, F->rhs_b.p, s.L));
+    if (F->prm.guess == CFM_GUESS_PROJECTED || F->prm.guess == CFM_GUESS_LEARNED) {
+      History& h = F->hist;
+      if (h.L != s.L) {
+        h.L = s.L;
+        h.count = h.head = 0;
+        CFM_TRY(h.B.alloc((size_t)History::H * s.L));
+        CFM_TRY(h.X.alloc((size_t)History::H * s.L));
+        CFM_TRY(h.g.alloc(History::H));
+        CFM_TRY(h.grow.alloc(History::H));
+        CFM_TRY(h.coef.alloc(History::H));
+        CFM_TRY(h.partial.alloc(dev_dots_partial_size(s.L, History::H)));
+        h.pending = -1;
+        if (!h.host) CFM_HIP(hipHostMalloc((void**)&h.host, sizeof(double) * 3 * History::H, hipHostMallocDefault));
+      }
+      bool used = false;
+      CFM_TRY(history_guess(c, h, F->rhs_b.p, F->fb_prev.p, &used));
+    }
+    cfm_solve_opts o = F->prm.blob_solve;
+    o.system = CFM_SYS_BLOB;
+    o.use_guess = 1;
+    if (!c->krylov) c->krylov = new Krylov();
+    cfm_solve_stats ss_{};
+    CFM_TRY(solve(c, *c->krylov, s, o, F->rhs_b.p, F->fb_prev.p, &ss_, nullptr));
+    if (F->prm.guess == CFM_GUESS_PROJECTED || F->prm.guess == CFM_GUESS_LEARNED)
+      CFM_TRY(history_push(c, F->hist, F->rhs_b.p, F->fb_prev.p));
+    CFM_TRY(gnn_record(c, F->rhs_b.p, F->fb_prev.p));
+    if (st) {
+      st->gmres_iterations += ss_.iterations;
+      st->gmres_solves += 1;
+      st->max_gmres_rel_residual = std::max(st->max_gmres_rel_residual, ss_.rel_residual);
+    }
+    return dev_copy_rows(c, F->fb_prev.p, (size_t)s.m, 0, p.f.p, (size_t)p.n, p.n_seg, 3, s.m);
+  };
+  const Status sb_st = blob_part();
+  const Status sj = overlap ? side_join(c) : Status{};  // the side stream has finished M_ss f_seg
+  if (!sb_st.ok()) return sb_st;
+  CFM_TRY(sj);
   // v_seg += Msb f_b
-  CFM_TRY(dev_copy_rows(c, F->fb_prev.p, (size_t)s.m, 0, p.f.p, (size_t)p.n, p.n_seg, 3, s.m));
   ApplyPlan sb;
   sb.tgt_begin = 0;
   sb.tgt_end = p.n_seg;
@@ -265,8 +307,9 @@ static Status swim_velocities(cfm_ctx* c, FilState* F, cfm_step_stats* st) {
   }
   CFM_HIP(hipMemcpyAsync(F->S.vel, F->fb_prev.p + 3 * (size_t)s.m, sizeof(double) * s.extra,
                          hipMemcpyDeviceToDevice, c->stream));
-  // v_seg += Msb f_b
+  // v_seg += Msb f_b, after the segment rows of the side stream
   CFM_TRY(dev_copy_rows(c, F->fb_prev.p, (size_t)s.m, 0, p.f.p, (size_t)p.n, p.n_seg, 3, s.m));
+  CFM_TRY(side_join(c));
   ApplyPlan sb;
   sb.tgt_begin = 0;
   sb.tgt_end = p.n_seg;

@@ -36,7 +36,8 @@ class FilamentSystem:
 
     def __init__(self, x_seg, tangents, *, x_blob=None, precision="fp64", solve_mode="fp64",
                  blob_rtol=1e-10, blob_restart=50, blob_max_iter=2000, blob_precond="diag",
-                 guess="previous", pert=None, device=0, tile_split=0, wall=True, **params):
+                 guess="previous", pert=None, device=0, tile_split=0, wall=True, overlap=False,
+                 **params):
         prm = dict(DEFAULTS)
         prm.update(params)
         self.params = prm
@@ -68,6 +69,7 @@ class FilamentSystem:
                                         mode=RpyMobilitySolver._MODE[solve_mode],
                                         restart=blob_restart, max_iter=blob_max_iter,
                                         use_guess=1, rtol=blob_rtol)
+        P.overlap = int(bool(overlap))
         self._P = P
         pert_arr = None if pert is None else L.f64(pert, 3 * self.nfil)
         L.check(self.h, L.lib().cfm_fil_init(self.h, ctypes.byref(P), L.ptr(q), L.ptr(pert_arr)))

@@ -182,6 +182,9 @@ typedef struct cfm_fil_params {
   int32_t pair_precision;  /* CFM_FP64 / CFM_FP32C for the velocity products */
   int32_t guess;           /* CFM_GUESS_* for the blob-force GMRES */
   cfm_solve_opts blob_solve; /* inner GMRES on the body blobs (system field ignored) */
+  int32_t overlap;         /* 1: M_ss f_seg runs on a second stream while the blob solve runs
+                              (same results; the pair kernel shares the GPU meanwhile) */
+  int32_t reserved;
 } cfm_fil_params;
 
 typedef struct cfm_step_stats {

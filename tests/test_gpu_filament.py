@@ -180,3 +180,23 @@ def test_projected_guess_reduces_gmres_work():
     assert out["projected"][0] < out["previous"][0], out
     # both converge to the same state within the Broyden tolerance
     assert np.abs(out["projected"][1] - out["previous"][1]).max() < 1e-3
+
+
+def test_overlapped_segment_product_gives_the_same_steps():
+    """M_ss f_seg on the side stream beside the blob solve: the same step in FP64 (the
+    products' source splits differ from the one-launch path, so agreement is to
+    rounding)."""
+    xs, xb, c, d, t = geo.sphere_over_wall(n_blob=800, n_fil=30, radius=14.0, gap=50.0,
+                                            return_tangents=True)
+    out = {}
+    for overlap in (False, True):
+        with FilamentSystem(xs, t, x_blob=xb, guess="projected", blob_precond="dense",
+                            overlap=overlap) as S:
+            its = []
+            for nt in range(3):
+                st = S.step(nt)
+                assert st["converged"] and st["max_gmres_rel_residual"] <= 1e-10
+                its.append(st["broyden_iterations"])
+            out[overlap] = (its, S.state()["x"])
+    assert out[True][0] == out[False][0]
+    assert np.abs(out[True][1] - out[False][1]).max() < 1e-9
