@@ -231,6 +231,7 @@ def lib():
     L.cfm_fil_initial_guess.argtypes = [_vp, ctypes.c_int32]
     L.cfm_fil_attach_bodies.argtypes = [_vp, _dp]
     L.cfm_fil_get_body_state.argtypes = [_vp, _dp, _dp, _dp, _dp]
+    L.cfm_fil_set_body_sphere.argtypes = [_vp, ctypes.c_double]
     _fp = ctypes.POINTER(ctypes.c_float)
     _ip = ctypes.POINTER(ctypes.c_int32)
     L.cfm_gnn_init.argtypes = [_vp, ctypes.c_uint32, ctypes.c_int32]

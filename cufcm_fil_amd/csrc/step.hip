@@ -766,6 +766,8 @@ int cfm_fil_attach_bodies(cfm_ctx* c, const double* f_ext) {
     S.dt = F->K.dt;
     S.tol = F->K.tol;
     S.end_force = F->K.end_force;
+    S.a_seg = c->cfg.a_seg;
+    S.mu = c->cfg.mu;
     S.fref = F->sw_fref.p;
     S.fext = F->sw_fext.p;
     S.mob = F->sw_mob.p;
@@ -799,6 +801,16 @@ int cfm_fil_attach_bodies(cfm_ctx* c, const double* f_ext) {
     F->swim = false;
     return fail_c(c, st);
   }
+  return CFM_OK;
+}
+
+int cfm_fil_set_body_sphere(cfm_ctx* c, double radius) {
+  if (!c) return CFM_EINVAL;
+  FilState* F = fil_of(c);
+  if (!F || !F->swim) return fail_c(c, make_status(CFM_ESTATE, "cfm_fil_attach_bodies has not been called"));
+  if (!(radius >= 0.0) || !std::isfinite(radius))
+    return fail_c(c, make_status(CFM_EINVAL, "sphere radius must be finite and >= 0"));
+  F->S.sphere_R = radius;
   return CFM_OK;
 }
 

@@ -187,8 +187,9 @@ __device__ __forceinline__ M3 rot_mob(const SwimK& S, int b, int bi, int bj) {
   return mm(mm(Q, blk6(S.mob, bi, bj)), transpose(Q));
 }
 
-// B J^-1 of filament f (swimmer.cpp:744-886 without the sphere correction) and its
-// product with the filament's body columns A (:888-902).  One workgroup per filament.
+// B J^-1 of filament f (swimmer.cpp:744-886, with the sphere correction when the body
+// is a sphere) and its product with the filament's body columns A (:888-902).  One
+// workgroup per filament.
 __global__ __launch_bounds__(256) void swim_bj_kernel(SwimK S, FilK K, int nt) {
   extern __shared__ double lds[];
   const int f = blockIdx.x;
@@ -196,6 +197,7 @@ __global__ __launch_bounds__(256) void swim_bj_kernel(SwimK S, FilK K, int nt) {
   const int s0 = f * n, b = f / S.fpb;
   double* B = lds;          // 6 x N
   double* BJ = lds + 6 * N; // 6 x N
+  double* C = lds + 12 * N; // 6 x N: sphere-segment coupling added to N B
   const int tid = threadIdx.x;
   const double dt_fac = nt < S.num_euler ? -S.dt : -2.0 * S.dt / 3.0;
   const V3 ub = ldv(S.u + 3 * b);
@@ -204,8 +206,52 @@ __global__ __launch_bounds__(256) void swim_bj_kernel(SwimK S, FilK K, int nt) {
   const M3 Dinv = madd(madd(eye3(), blc2, 1.0 / 12.0), blc, -0.5);
   const M3 DinvT = madd(madd(eye3(), blc2, 1.0 / 12.0), blc, 0.5);
   const V3 d = base_disp(S, f);
-  for (int e = tid; e < 6 * N; e += 256) B[e] = 0.0;
+  for (int e = tid; e < 6 * N; e += 256) {
+    B[e] = 0.0;
+    C[e] = 0.0;
+  }
   __syncthreads();
+  if (tid == 1 && S.sphere_R > 0.0) {
+    // RPY of the sphere (radius R) and segment m (radius a): translation-translation,
+    // translation-rotation and rotation-rotation blocks (swimmer.cpp:809-877)
+    auto add = [&](int r0, int c0, const M3& M) {
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) C[(r0 + i) * N + c0 + j] += M.a[i][j];
+    };
+    const double pi = 3.14159265358979323846, R2a2 = S.sphere_R * S.sphere_R + S.a_seg * S.a_seg;
+    for (int m = 0; m < n; ++m) {
+      const V3 rv = v3(K.x[s0 + m], K.y[s0 + m], K.z[s0 + m]) - ldv(S.x + 3 * b);
+      const double r = sqrt(dot(rv, rv));
+      const V3 e = (1.0 / r) * rv;
+      M3 rr;
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) rr.a[i][j] = (&e.x)[i] * (&e.x)[j];
+      const M3 Mtt = mscale(madd(mscale(eye3(), 1.0 + R2a2 / (3.0 * r * r)), rr, 1.0 - R2a2 / (r * r)),
+                            1.0 / (8.0 * pi * S.mu * r));
+      const M3 Mtr = mscale(rcross(e), 1.0 / (8.0 * pi * S.mu * r * r));
+      const M3 Mrr = mscale(madd(mscale(rr, 3.0), eye3(), -1.0), 1.0 / (16.0 * pi * S.mu * r * r * r));
+      const M3 tc = rcross(tangent(segq(K, s0 + m)));
+      const M3 ta = mscale(mm(Mtr, tc), 0.5 * S.dl), tb = mscale(mm(Mrr, tc), 0.5 * S.dl);
+      if (m == 0) {
+        add(0, 0, Mtt);  // tether constraint force
+        add(3, 0, Mtr);
+        add(0, 3 * n, mm(Mtr, DinvT));  // clamp constraint
+        add(3, 3 * n, mm(Mrr, DinvT));
+        add(0, 3, madd(ta, Mtt, -1.0));  // inextensibility force of segment 1
+        add(3, 3, madd(tb, Mtr, -1.0));
+      } else if (m == n - 1) {
+        add(0, N - 3, mscale(mm(Mtt, tc), -S.end_force));  // tip Lie algebra element
+        add(3, N - 3, mscale(mm(Mtr, tc), -S.end_force));
+        add(0, 3 * n - 3, madd(ta, Mtt));
+        add(3, 3 * n - 3, madd(tb, Mtr));
+      } else {
+        add(0, 3 * m, madd(ta, Mtt));
+        add(3, 3 * m, madd(tb, Mtr));
+        add(0, 3 * (m + 1), madd(ta, Mtt, -1.0));
+        add(3, 3 * (m + 1), madd(tb, Mtr, -1.0));
+      }
+    }
+  }
   if (tid == 0) {
     auto put = [&](int r0, int c0, const M3& M) {
       for (int i = 0; i < 3; ++i)
@@ -230,6 +276,7 @@ __global__ __launch_bounds__(256) void swim_bj_kernel(SwimK S, FilK K, int nt) {
     for (int bi = 0; bi < 6; bi += 3) {
       V3 acc = v3(0.0, 0.0, 0.0);
       for (int bj = 0; bj < 6; bj += 3) acc = acc + mv(rot_mob(S, b, bi, bj), v3(col[bj], col[bj + 1], col[bj + 2]));
+      acc = acc + v3(C[bi * N + c], C[(bi + 1) * N + c], C[(bi + 2) * N + c]);
       if (bi == 3) acc = mv(Dinv, acc);
       out[bi] = dt_fac * acc.x;
       out[bi + 1] = dt_fac * acc.y;
@@ -428,7 +475,7 @@ Status swim_errors(cfm_ctx* c, const SwimK& S, int nt, double* err_body, double*
 
 Status swim_jacobian(cfm_ctx* c, const SwimK& S, const FilK& K, int nt, int* status_dev) {
   const int N = 6 * K.nseg;
-  const size_t lds = sizeof(double) * 12 * N;
+  const size_t lds = sizeof(double) * 18 * N;
   hipLaunchKernelGGL(swim_bj_kernel, dim3(S.nfil), dim3(256), lds, c->stream, S, K, nt);
   hipLaunchKernelGGL(swim_schur_kernel, gs(S.nbody, 64), dim3(64), 0, c->stream, S, K, nt, status_dev);
   CFM_CHECK_LAUNCH();

@@ -11,6 +11,9 @@ struct SwimK {
   int nb = 0;                                  // blobs per body
   int num_euler = 1;
   double dl = 2.2, dt = 0.1, tol = 1e-4, end_force = 0.0;
+  // > 0: the body is a sphere of this radius, and the approximate Jacobian carries the
+  // RPY coupling of sphere and segments (swimmer.cpp:800-877); a_seg, mu for it
+  double sphere_R = 0.0, a_seg = 1.0, mu = 1.0;
   // per body, AoS: position 3 (x, x_-1, x_-2), Lie algebra 3 (u, u_-1), quaternion 4
   // (q, q_-1), R(q) at the step's initial guess 9 (row-major)
   double *x = nullptr, *xm1 = nullptr, *xm2 = nullptr, *u = nullptr, *um1 = nullptr;

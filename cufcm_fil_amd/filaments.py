@@ -99,6 +99,11 @@ class FilamentSystem:
         L.check(self.h, L.lib().cfm_fil_attach_bodies(self.h, L.ptr(fe)))
         self.n_body = n_body
 
+    def set_body_sphere(self, radius):
+        """Treat the bodies as spheres of this radius in the approximate Jacobian (the
+        reference's sphere correction); 0 turns it off."""
+        L.check(self.h, L.lib().cfm_fil_set_body_sphere(self.h, float(radius)))
+
     def write_outputs(self, writer, step):
         """One line of the reference's _seg_states.dat (segment quaternions) and, with
         bodies attached, _body_states.dat (centre, quaternion) through a simio.OutputWriter."""

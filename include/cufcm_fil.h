@@ -253,6 +253,12 @@ int cfm_fil_attach_bodies(cfm_ctx* ctx, const double* f_ext);
 /* Body centres (3 n_body), quaternions (4), Lie-algebra elements of the step (3) and the
  * last solve's (V, Omega) (6); each pointer may be null. */
 int cfm_fil_get_body_state(cfm_ctx* ctx, double* x, double* q, double* u, double* vel);
+/* The bodies are spheres of this radius (0: off, the default): the approximate Jacobian's
+ * body rows then also carry the RPY coupling of sphere and segments -- the unequal-sphere
+ * translation-translation, translation-rotation and rotation-rotation blocks of every
+ * segment (swimmer.cpp:800-877, surface-of-revolution bodies whose generatrix is a
+ * sphere).  Changes Broyden's convergence only, not the converged state. */
+int cfm_fil_set_body_sphere(cfm_ctx* ctx, double radius);
 
 /* ---- measurement --------------------------------------------------------- */
 typedef struct cfm_profile_stats {

@@ -67,6 +67,52 @@ class Body:
         return self.x + self.offsets()
 
 
+def sphere_blocks(rv, R, a=1.0, mu=1.0):
+    """(Mtt, Mtr, Mrr) of a sphere of radius R and a segment of radius a at separation
+    rv = x_seg - x_body (swimmer.cpp:809-840)."""
+    r = np.linalg.norm(rv)
+    e = rv / r
+    rr = np.outer(e, e)
+    s2 = R * R + a * a
+    Mtt = ((1 + s2 / (3 * r * r)) * np.eye(3) + (1 - s2 / (r * r)) * rr) / (8 * math.pi * mu * r)
+    Mtr = rcross(e) / (8 * math.pi * mu * r * r)
+    Mrr = (3 * rr - np.eye(3)) / (16 * math.pi * mu * r ** 3)
+    return Mtt, Mtr, Mrr
+
+
+def sphere_coupling(f, xb, R, P, DinvT, a=1.0, mu=1.0):
+    """Body rows of B added for a spherical body (swimmer.cpp:800-877): for every segment
+    the RPY blocks of a sphere of radius R and a segment of radius a at separation r,
+        Mtt = ((1 + (R^2 + a^2) / 3 r^2) I + (1 - (R^2 + a^2) / r^2) r r) / (8 pi mu r),
+        Mtr = rcross(r_hat) / (8 pi mu r^2),   Mrr = (3 r r - I) / (16 pi mu r^3),
+    placed at the columns of the constraint forces and Lie-algebra elements the segment
+    depends on."""
+    n = P.nseg
+    C = np.zeros((6, 6 * n))
+    for m in range(n):
+        Mtt, Mtr, Mrr = sphere_blocks(f.x[m] - xb, R, a, mu)
+        tc = rcross(tangent(f.q[m]))
+        ta, tb = 0.5 * P.dl * Mtr @ tc, 0.5 * P.dl * Mrr @ tc
+        if m == 0:
+            C[0:3, 0:3] += Mtt
+            C[3:6, 0:3] += Mtr
+            C[0:3, 3 * n:3 * n + 3] += Mtr @ DinvT
+            C[3:6, 3 * n:3 * n + 3] += Mrr @ DinvT
+            C[0:3, 3:6] += ta - Mtt
+            C[3:6, 3:6] += tb - Mtr
+        elif m == n - 1:
+            C[0:3, 6 * n - 3:6 * n] += -P.end_force * Mtt @ tc
+            C[3:6, 6 * n - 3:6 * n] += -P.end_force * Mtr @ tc
+            C[0:3, 3 * n - 3:3 * n] += ta + Mtt
+            C[3:6, 3 * n - 3:3 * n] += tb + Mtr
+        else:
+            C[0:3, 3 * m:3 * m + 3] += ta + Mtt
+            C[3:6, 3 * m:3 * m + 3] += tb + Mtr
+            C[0:3, 3 * (m + 1):3 * (m + 1) + 3] += ta - Mtt
+            C[3:6, 3 * (m + 1):3 * (m + 1) + 3] += tb - Mtr
+    return C
+
+
 def blob_barrier_forces(x_seg, x_blob, P, nb_per_body):
     """Repulsion on each blob from the segments and from the blobs of other bodies
     (cuda_functions.cu:1932-2012; the blob pairs of one body cancel in its total force
@@ -157,6 +203,8 @@ class Swimmer:
         return max(bf, np.abs(eb[:3]).max() / (P.dl * P.tol), np.abs(eb[3:]).max() / P.tol)
 
     # -- approximate Jacobian with the body Schur complement (swimmer.cpp:688-917) --
+    sphere_R = 0.0   # > 0: the sphere correction of the body rows of B
+
     def prepare_jacobian(self, nt, F_int):
         P = self.P
         n = P.nseg
@@ -205,6 +253,8 @@ class Swimmer:
                 tm = tangent(f.q[m])
                 B[3:6, 3 * (n + m):3 * (n + m) + 3] = B[0:3, 6 * n - 3:6 * n] @ rcross(P.dl * tm)
             B = N @ B
+            if self.sphere_R > 0.0:
+                B += sphere_coupling(f, b.x, self.sphere_R, P, DinvT)
             B[3:6, :] = Dinv @ B[3:6, :]
             B *= dt_fac
             A = np.zeros((6 * n, 6))

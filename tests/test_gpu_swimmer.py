@@ -54,6 +54,23 @@ def test_free_swimmer_steps_match_oracle(wall, precond):
     assert np.linalg.norm(sw.body.x - np.array([0.0, 0.0, 40.0])) > 1e-6
 
 
+def test_sphere_correction_matches_oracle():
+    """The sphere correction of the approximate Jacobian (swimmer.cpp:800-877) on the
+    GPU and in the oracle: identical Broyden iterations and states."""
+    P, sw, wall = _swimmer()
+    sw.sphere_R = 6.0
+    vel = _velocity(wall)
+    with _gpu(sw, P, wall) as S:
+        S.set_body_sphere(6.0)
+        for nt in range(3):
+            st = S.step(nt)
+            it = so.implicit_step(sw, nt, vel, wall=wall)
+            assert st["converged"] and st["broyden_iterations"] == it, (nt, st, it)
+            xs = np.concatenate([f.x for f in sw.fils])
+            assert np.abs(S.state()["x"].reshape(-1, 3) - xs).max() < 1e-7, nt
+            assert np.abs(S.body_state()["x"] - sw.body.x).max() < 1e-8
+
+
 def test_external_load_moves_the_swimmer_as_the_oracle():
     f_ext = np.array([0.0, 0.0, -50.0, 0.0, 5.0, 0.0])
     P, sw, wall = _swimmer(f_ext=f_ext, end_force=0.0)

@@ -121,3 +121,44 @@ def test_external_force_sets_the_net_force_on_the_fluid():
     assert np.allclose(tot, f_ext[:3], atol=1e-8 * 50.0)
     # it sinks: downward velocity of the body
     assert sw.vel[2] < 0.0
+
+
+def test_sphere_correction_blocks_are_the_unequal_sphere_rpy_blocks():
+    """The blocks of the sphere correction (swimmer.cpp:809-840): translation-
+    translation and rotation-rotation are the RPY blocks of the body's (radius R) velocity
+    due to a load on a segment (radius a); translation-rotation is the segment's
+    translational velocity due to a torque on the body (the orientation the reference
+    uses in the body rows)."""
+    def rpy_block(xi, ai, xj, aj):
+        M = np.zeros((6, 6))
+        for k in range(6):
+            F = np.zeros(6)
+            F[k] = 1.0
+            M[:, k] = oracle.free_pair(xi, ai, xj, aj, F, 6, 6, 1.0)
+        return M
+
+    rng = np.random.default_rng(4)
+    for R, a in ((6.0, 1.0), (20.0, 1.0), (3.0, 0.5)):
+        for _ in range(3):
+            d = rng.normal(size=3)
+            xs = (R + a + rng.uniform(0.5, 30.0)) * d / np.linalg.norm(d)
+            Mtt, Mtr, Mrr = so.sphere_blocks(xs, R, a)
+            Mbs = rpy_block(np.zeros(3), R, xs, a)
+            Msb = rpy_block(xs, a, np.zeros(3), R)
+            assert np.allclose(Mtt, Mbs[:3, :3], rtol=1e-12, atol=1e-18)
+            assert np.allclose(Mrr, Mbs[3:, 3:], rtol=1e-12, atol=1e-18)
+            assert np.allclose(Mtr, Msb[:3, 3:], rtol=1e-12, atol=1e-18)
+
+
+def test_sphere_correction_changes_convergence_not_the_state():
+    out = {}
+    for R in (0.0, 6.0):
+        P, sw, wall = _swimmer()
+        sw.sphere_R = R
+        vel = _velocity(wall)
+        its = [so.implicit_step(sw, nt, vel, wall=wall) for nt in range(2)]
+        out[R] = (its, np.concatenate([f.x for f in sw.fils]), sw.body.x.copy())
+    # both converge to the Broyden tolerance (1e-4) around the same state
+    assert np.abs(out[0.0][1] - out[6.0][1]).max() < 1e-3
+    assert np.abs(out[0.0][2] - out[6.0][2]).max() < 1e-3
+    print("broyden iterations without / with the sphere correction", out[0.0][0], out[6.0][0])
