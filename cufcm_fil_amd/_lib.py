@@ -128,6 +128,9 @@ class cfm_profile_stats(ctypes.Structure):
 
 CFM_BEAT_FULFORD_BLAKE = 1
 CFM_BEAT_FULFORD_BLAKE_ORIGINAL = 2
+CFM_BEAT_VOLVOX = 3
+CFM_BEAT_CORAL_LARVAE = 4
+CFM_BEAT_BUILD_A_BEAT = 5
 
 
 class cfm_cilia_params(ctypes.Structure):

@@ -24,6 +24,16 @@ def wall_lattice(nx, ny, spacing, dl=2.6):
     return np.array(b, dtype=np.float64).reshape(-1)
 
 
+BEATS = ("fulford_blake", "fulford_blake_original", "volvox", "coral_larvae", "build_a_beat")
+
+
+def _beat(name):
+    """Beat name -> CFM_BEAT_* (the reference's SHAPE_SEQUENCE 1, 4, 3, 2 and 0)."""
+    if name not in BEATS:
+        raise ValueError(f"unknown beat {name!r}; one of {BEATS}")
+    return BEATS.index(name) + 1
+
+
 class PrescribedCilia:
     """n cilia of nseg segments; base (3 n), phase and phase_dot (n)."""
 
@@ -40,9 +50,7 @@ class PrescribedCilia:
         o = L.cfm_solve_opts(system=L.CFM_SYS_SEG, precond=RpyMobilitySolver._PC[precond],
                              mode=RpyMobilitySolver._MODE[solve_mode], restart=restart,
                              max_iter=max_iter, use_guess=1, rtol=rtol)
-        beats = {"fulford_blake": L.CFM_BEAT_FULFORD_BLAKE,
-                 "fulford_blake_original": L.CFM_BEAT_FULFORD_BLAKE_ORIGINAL}
-        self.prm = L.cfm_cilia_params(n_cilia=n, nseg=nseg, beat=beats[beat], dl=dl, dt=dt,
+        self.prm = L.cfm_cilia_params(n_cilia=n, nseg=nseg, beat=_beat(beat), dl=dl, dt=dt,
                                       s_tol=s_tol, solve=o)
         L.check(self.mob._h, L.lib().cfm_cilia_init(self.mob._h, ctypes.byref(self.prm),
                                                     L.ptr(base), L.ptr(phase),
@@ -198,9 +206,7 @@ class CiliatedBodies:
         o = L.cfm_solve_opts(system=L.CFM_SYS_SEG, precond=RpyMobilitySolver._PC[precond],
                              mode=RpyMobilitySolver._MODE[solve_mode], restart=restart,
                              max_iter=max_iter, use_guess=1, rtol=rtol)
-        beats = {"fulford_blake": L.CFM_BEAT_FULFORD_BLAKE,
-                 "fulford_blake_original": L.CFM_BEAT_FULFORD_BLAKE_ORIGINAL}
-        self.prm = L.cfm_cilia_params(n_cilia=n, nseg=nseg, beat=beats[beat], dl=dl, dt=dt,
+        self.prm = L.cfm_cilia_params(n_cilia=n, nseg=nseg, beat=_beat(beat), dl=dl, dt=dt,
                                       s_tol=s_tol, solve=o)
         L.check(self.mob._h, L.lib().cfm_cilia_init_bodies(self.mob._h, ctypes.byref(self.prm),
                                                            L.ptr(fil_ref), L.ptr(fil_q),

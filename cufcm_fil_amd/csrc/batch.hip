@@ -1102,8 +1102,8 @@ int cfm_batch_gmres(cfm_batch* b, const double* rhs, double* xout, const cfm_sol
       if (debug) {
         std::vector<double> si((size_t)8 * b->n_sys);
         std::vector<int> ii((size_t)8 * b->n_sys);
-        CFM_HIP(hipMemcpy(si.data(), b->sinfo.p, sizeof(double) * si.size(), hipMemcpyDeviceToHost));
-        CFM_HIP(hipMemcpy(ii.data(), b->iinfo.p, sizeof(int) * ii.size(), hipMemcpyDeviceToHost));
+        CFM_HIP(copy_to_host(b->stream, si.data(), b->sinfo.p, sizeof(double) * si.size()));
+        CFM_HIP(copy_to_host(b->stream, ii.data(), b->iinfo.p, sizeof(int) * ii.size()));
         std::fprintf(stderr, "[batch] pass %d running %d j %d steps %d\n", pass, jd[2], jd[0], jd[1]);
         for (int q = 0; q < b->n_sys; ++q)
           std::fprintf(stderr, "  sys %d st %d it %d k %d nr %.3e tgt %.3e est %.3e nb %.3e\n", q,
@@ -1129,8 +1129,8 @@ int cfm_batch_gmres(cfm_batch* b, const double* rhs, double* xout, const cfm_sol
     // per-system results
     std::vector<double> si((size_t)8 * b->n_sys);
     std::vector<int> ii((size_t)8 * b->n_sys);
-    CFM_HIP(hipMemcpy(si.data(), b->sinfo.p, sizeof(double) * si.size(), hipMemcpyDeviceToHost));
-    CFM_HIP(hipMemcpy(ii.data(), b->iinfo.p, sizeof(int) * ii.size(), hipMemcpyDeviceToHost));
+    CFM_HIP(copy_to_host(b->stream, si.data(), b->sinfo.p, sizeof(double) * si.size()));
+    CFM_HIP(copy_to_host(b->stream, ii.data(), b->iinfo.p, sizeof(int) * ii.size()));
     bool all = true;
     for (int s = 0; s < b->n_sys; ++s) {
       const double nb = si[8 * s + SI_NB];

@@ -1,4 +1,5 @@
-// cilia.hip -- prescribed-shape cilia above the wall (fitted Fulford-Blake beat).
+// cilia.hip -- prescribed-shape cilia above the wall or on free bodies (fitted
+// Fulford-Blake, volvox and coral-larvae beats; the build-a-beat tangent-angle model).
 //
 // Restates the reference's CILIA_TYPE 3 path.  With WRITE_GENERALISED_FORCES (the
 // mode that wrote its input/forcing reference files) every cilium follows the fitted
@@ -9,7 +10,9 @@
 // speeds are unknowns of a coupled system driven by recorded generalised forces
 // (mobility_solver.cpp:242-356 driving, :1891-2115 operator, :1104-1244 block
 // preconditioner).
-//   beat coefficients      filament.cpp:684-758 (Ito, Omori & Ishikawa 2019, Table 1)
+//   beat coefficients      filament.cpp:684-758 (Ito, Omori & Ishikawa 2019, Table 1),
+//                          :748-812 volvox, :813-868 unaltered, :869-893 coral larvae
+//   build-a-beat           filament.cpp:1238-1334 angle law, :1401-1546 placement
 //   shape / derivatives    filament.cpp:964-1045
 //   segment parameters     filament.cpp:1047-1160 (trapezoid arc length, bisection)
 //   placement              filament.cpp:1336-1490, base frame filament.cpp:225-226
@@ -38,12 +41,33 @@ Status download_aos(cfm_ctx* c, double* host, int cnt, int dof, const double* sr
 
 constexpr int kMaxCiliaSeg = 64;
 
-// Fourier modes x polynomial degree (s, s^2, s^3); x along the filament axis of the
-// reference frame, y along the beat direction; cos0 weights the zero cosine mode.
+// Fourier modes x polynomial degree (s, s^2, ... s^ndeg, ndeg <= 4); x along the
+// filament axis of the reference frame, y along the beat direction; cos0 weights the
+// zero cosine mode.  build: the build-a-beat tangent-angle model instead.
+constexpr int kMaxDeg = 4;
 struct BeatCoeffs {
-  double ax[4][3], ay[4][3], bx[3][3], by[3][3];
+  double ax[4][kMaxDeg], ay[4][kMaxDeg], bx[3][kMaxDeg], by[3][kMaxDeg];
   double cos0;
+  int ndeg;
+  int build;
 };
+
+template <int D>
+static void set_table(BeatCoeffs& b, const double (&ax)[4][D], const double (&ay)[4][D],
+                      const double (&bx)[3][D], const double (&by)[3][D]) {
+  b = BeatCoeffs{};
+  for (int m = 0; m < 4; ++m)
+    for (int d = 0; d < D; ++d) {
+      b.ax[m][d] = ax[m][d];
+      b.ay[m][d] = ay[m][d];
+      if (m < 3) {
+        b.bx[m][d] = bx[m][d];
+        b.by[m][d] = by[m][d];
+      }
+    }
+  b.ndeg = D;
+  b.cos0 = 1.0;
+}
 
 // Ito, Omori & Ishikawa (2019) Table 1 as SHAPE_SEQUENCE 1 uses it: rescaled to unit
 // mean length, zero mode halved into the table (filament.cpp:684-758).
@@ -63,11 +87,7 @@ static BeatCoeffs fulford_blake() {
   const double by[3][3] = {{2.9136e-01, 1.0721e+00, -1.0433e+00},
                            {6.1554e-03, 3.2521e-01, -2.8315e-01},
                            {-6.0528e-02, 2.3185e-01, -2.0108e-01}};
-  std::memcpy(b.ax, ax, sizeof ax);
-  std::memcpy(b.ay, ay, sizeof ay);
-  std::memcpy(b.bx, bx, sizeof bx);
-  std::memcpy(b.by, by, sizeof by);
-  b.cos0 = 1.0;
+  set_table(b, ax, ay, bx, by);
   return b;
 }
 
@@ -81,12 +101,93 @@ static BeatCoeffs fulford_blake_original() {
                            {0.079, -0.302, 0.142}};
   const double bx[3][3] = {{0.192, -0.499, 0.339}, {-0.050, 0.423, -0.327}, {0.012, 0.138, -0.114}};
   const double by[3][3] = {{0.284, 1.045, -1.017}, {0.006, 0.317, -0.276}, {-0.059, 0.226, -0.196}};
-  std::memcpy(b.ax, ax, sizeof ax);
-  std::memcpy(b.ay, ay, sizeof ay);
-  std::memcpy(b.bx, bx, sizeof bx);
-  std::memcpy(b.by, by, sizeof by);
+  set_table(b, ax, ay, bx, by);
   b.cos0 = 0.5;
   return b;
+}
+
+// The reference's volvox fit (SHAPE_SEQUENCE 3, filament.cpp:748-812): four polynomial
+// degrees, zero cosine mode at weight 1.
+static BeatCoeffs volvox() {
+  BeatCoeffs b{};
+  const double ax[4][4] = {{1.00490080, -0.19958656, -0.20422343, 0.10887946},
+                           {0.00314934, -0.22239636, 1.18659976, -0.87409016},
+                           {0.18942274, -1.24525818, 1.94063304, -0.78632572},
+                           {-0.11162123, 1.07262966, -2.15142914, 1.20544741}};
+  const double ay[4][4] = {{-0.01665535, -0.27457190, 1.08406999, -0.56102820},
+                           {0.17505739, -2.09412076, 1.99556763, -0.50179439},
+                           {-0.12408600, 1.62438155, -3.21015214, 1.70178281},
+                           {-0.16672661, 0.90896038, -1.50867276, 0.74387795}};
+  const double bx[3][4] = {{0.09140608, -0.54804089, 1.40643137, -0.60109684},
+                           {-0.09670739, 1.14634486, -2.60675971, 1.55525152},
+                           {-0.16451150, 0.98846449, -1.49147276, 0.64896090}};
+  const double by[3][4] = {{-0.06343445, 2.15509751, -4.19609063, 2.05489100},
+                           {-0.28828336, 1.45223183, -1.56580487, 0.44541361},
+                           {0.02187088, -0.33070891, 0.94906955, -0.66157093}};
+  set_table(b, ax, ay, bx, by);
+  return b;
+}
+
+// The reference's coral-larvae fit (SHAPE_SEQUENCE 2, filament.cpp:869-893).
+static BeatCoeffs coral_larvae() {
+  BeatCoeffs b{};
+  const double ax[4][3] = {{9.034276e-01, -6.634681e-01, 2.016123e-01},
+                           {-2.468656e-01, 1.215869e+00, -5.867236e-01},
+                           {3.075836e-01, -6.966322e-01, 4.554729e-01},
+                           {2.378489e-02, -1.355582e-01, 7.315831e-02}};
+  const double ay[4][3] = {{-2.766076e-01, 6.368662e-01, -8.513947e-02},
+                           {3.091205e-01, -1.993712e+00, 1.091847e+00},
+                           {-2.952120e-01, 7.189021e-01, -6.328589e-01},
+                           {-9.868018e-02, 3.312812e-01, -2.578206e-01}};
+  const double bx[3][3] = {{1.336807e-01, -6.807852e-01, 6.008592e-01},
+                           {8.146824e-02, 3.472676e-01, -2.744220e-01},
+                           {3.615272e-02, 8.619119e-02, -6.122992e-02}};
+  const double by[3][3] = {{9.165546e-01, 8.848315e-02, -4.838832e-01},
+                           {-2.946211e-01, 1.057964e+00, -6.335721e-01},
+                           {6.122744e-02, -2.319094e-01, 1.906292e-01}};
+  set_table(b, ax, ay, bx, by);
+  return b;
+}
+
+// Build-a-beat (SHAPE_SEQUENCE 0, filament.cpp:1238-1334) with the reference's
+// configuration (config.hpp:133-139): amplitude 1.99 filament lengths, recovery-stroke
+// window 1/12 of the cycle, effective stroke 0.75 of it, zero-velocity avoidance 0.05.
+constexpr double kBuildAmplitude = 1.99, kBuildRecovery = 1.0 / 12.0, kBuildEffective = 0.75,
+                 kBuildShift = 0.05;
+static BeatCoeffs build_a_beat() {
+  BeatCoeffs b{};
+  b.build = 1;
+  b.ndeg = 0;
+  return b;
+}
+
+// Tangent angle of the build-a-beat model at arc fraction s and phase psi, and its
+// phase derivative.
+__device__ inline void build_angle(double s, double psi, double* theta, double* dtheta) {
+  const double pi = 3.14159265358979323846;
+  const double th_s = acos(0.5 * kBuildAmplitude);
+  const double wR = kBuildRecovery * 2.0 * pi, wE = kBuildEffective * 2.0 * pi;
+  const double trans = wE + s * (2.0 * pi - wE - wR);
+  double sp = psi - s * 2.0 * pi * kBuildShift;
+  sp -= 2.0 * pi * floor(0.5 * sp / pi);
+  double delta, dd;
+  if (sp < wE) {
+    const double t = pi * (wE - sp) / wE;
+    delta = t - sin(t) * cos(t);
+    dd = -sin(t) * sin(t) * 2.0 * pi / wE / pi;  // d delta / d psi
+  } else if (sp < trans) {
+    delta = 0.0;
+    dd = 0.0;
+  } else if (sp < trans + wR) {
+    const double t = pi * (sp - trans) / wR;
+    delta = t - sin(t) * cos(t);
+    dd = sin(t) * sin(t) * 2.0 * pi / wR / pi;
+  } else {
+    delta = pi;
+    dd = 0.0;
+  }
+  *theta = th_s + (pi - 2.0 * th_s) * delta / pi;
+  *dtheta = (pi - 2.0 * th_s) * dd;
 }
 
 struct CiliaK {
@@ -120,11 +221,11 @@ __device__ inline void modes(double psi, double cos0, double cv[4], double sv[3]
   }
 }
 
-// sum_m (cv_m A_m + sv_m B_m) . p over the three polynomial degrees
-__device__ inline double combine(const double A[4][3], const double B[3][3], const double cv[4],
-                                 const double sv[3], const double p[3]) {
+// sum_m (cv_m A_m + sv_m B_m) . p over the polynomial degrees
+__device__ inline double combine(const double A[4][kMaxDeg], const double B[3][kMaxDeg], const double cv[4],
+                                 const double sv[3], const double p[kMaxDeg], int ndeg) {
   double r = 0.0;
-  for (int d = 0; d < 3; ++d) {
+  for (int d = 0; d < ndeg; ++d) {
     double a = 0.0;
     for (int m = 0; m < 4; ++m) a += cv[m] * A[m][d];
     for (int m = 0; m < 3; ++m) a += sv[m] * B[m][d];
@@ -135,9 +236,9 @@ __device__ inline double combine(const double A[4][3], const double B[3][3], con
 
 __device__ inline double tangent_norm(const BeatCoeffs& cf, const double cv[4], const double sv[3],
                                       double s) {
-  const double p[3] = {1.0, 2.0 * s, 3.0 * s * s};
-  const double tx = combine(cf.ax, cf.bx, cv, sv, p);
-  const double ty = combine(cf.ay, cf.by, cv, sv, p);
+  const double p[kMaxDeg] = {1.0, 2.0 * s, 3.0 * s * s, 4.0 * s * s * s};
+  const double tx = combine(cf.ax, cf.bx, cv, sv, p, cf.ndeg);
+  const double ty = combine(cf.ay, cf.by, cv, sv, p, cf.ndeg);
   return sqrt(tx * tx + ty * ty);
 }
 
@@ -186,7 +287,10 @@ __global__ void cilia_geometry_kernel(CiliaK k) {
   double s[kMaxCiliaSeg];
   s[0] = 0.0;
   s[n - 1] = 1.0;
-  if (k.s_table) {
+  if (k.cf.build) {
+    // build-a-beat: segments at uniform arc fraction (filament.cpp:1489-1493)
+    for (int i = 1; i < n - 1; ++i) s[i] = double(i) / double(n - 1);
+  } else if (k.s_table) {
     // recorded parameters, linear in phase between the table's rows and linear in
     // segment position between its columns (filament.cpp:1190-1235)
     double x = 0.5 * psi / 3.14159265358979323846;
@@ -257,6 +361,17 @@ __global__ void cilia_geometry_kernel(CiliaK k) {
     dc[m] = -m * sin(m * psi);
     ds[m - 1] = m * cos(m * psi);
   }
+  // build-a-beat: trapezoid sums of the unit tangent (sin th, cos th) and of its phase
+  // derivative over the segments (filament.cpp:1401-1546), in filament lengths
+  double bt[2] = {0.0, 0.0}, bk[2] = {0.0, 0.0}, sum_t[2] = {0.0, 0.0}, sum_k[2] = {0.0, 0.0};
+  if (k.cf.build) {
+    double th, dth;
+    build_angle(0.0, psi, &th, &dth);
+    bt[0] = sin(th);
+    bt[1] = cos(th);
+    bk[0] = dth * cos(th);
+    bk[1] = -dth * sin(th);
+  }
   for (int i = 0; i < n; ++i) {
     const int g = s0 + i;
     k.svals[(size_t)c * n + i] = s[i];
@@ -270,13 +385,31 @@ __global__ void cilia_geometry_kernel(CiliaK k) {
       }
       continue;
     }
-    const double p[3] = {s[i], s[i] * s[i], s[i] * s[i] * s[i]};
+    double sx, sy, dx, dy;  // shape and its phase derivative in filament lengths
+    if (k.cf.build) {
+      double th, dth;
+      build_angle(s[i], psi, &th, &dth);
+      const double t2[2] = {sin(th), cos(th)}, k2[2] = {dth * cos(th), -dth * sin(th)};
+      const double h = 0.5 / double(n - 1);
+      for (int d = 0; d < 2; ++d) {
+        sum_t[d] += h * (bt[d] + t2[d]);
+        sum_k[d] += h * (bk[d] + k2[d]);
+        bt[d] = t2[d];
+        bk[d] = k2[d];
+      }
+      sx = sum_t[0];
+      sy = sum_t[1];
+      dx = sum_k[0];
+      dy = sum_k[1];
+    } else {
+      const double p[kMaxDeg] = {s[i], s[i] * s[i], s[i] * s[i] * s[i], s[i] * s[i] * s[i] * s[i]};
+      sx = combine(k.cf.ax, k.cf.bx, cv, sv, p, k.cf.ndeg);
+      sy = combine(k.cf.ay, k.cf.by, cv, sv, p, k.cf.ndeg);
+      dx = combine(k.cf.ax, k.cf.bx, dc, ds, p, k.cf.ndeg);
+      dy = combine(k.cf.ay, k.cf.by, dc, ds, p, k.cf.ndeg);
+    }
     // shape in the reference frame, rotated by the beat angle about its z axis
-    const double sx = combine(k.cf.ax, k.cf.bx, cv, sv, p);
-    const double sy = combine(k.cf.ay, k.cf.by, cv, sv, p);
     const double rx = k.len * (ca * sx - sa * sy), ry = k.len * (sa * sx + ca * sy);
-    const double dx = combine(k.cf.ax, k.cf.bx, dc, ds, p);
-    const double dy = combine(k.cf.ay, k.cf.by, dc, ds, p);
     const double ux = k.len * (ca * dx - sa * dy), uy = k.len * (sa * dx + ca * dy);
     if (!k.bx) {  // wall frame written out: lab (0, ry, rx)
       k.x[g] = bxp;
@@ -1137,7 +1270,13 @@ static DriveOpts drive_opts(const CiliaState* S, int nt) {
 
 static CiliaK kargs(cfm_ctx* c, CiliaState* S) {
   CiliaK k{};
-  k.cf = S->prm.beat == CFM_BEAT_FULFORD_BLAKE_ORIGINAL ? fulford_blake_original() : fulford_blake();
+  switch (S->prm.beat) {
+    case CFM_BEAT_FULFORD_BLAKE_ORIGINAL: k.cf = fulford_blake_original(); break;
+    case CFM_BEAT_VOLVOX: k.cf = volvox(); break;
+    case CFM_BEAT_CORAL_LARVAE: k.cf = coral_larvae(); break;
+    case CFM_BEAT_BUILD_A_BEAT: k.cf = build_a_beat(); break;
+    default: k.cf = fulford_blake();
+  }
   k.s_tol = S->prm.s_tol > 0.0 ? S->prm.s_tol : 0.1 / S->prm.nseg;
   k.s_table = S->s_rows > 0 ? S->s_table.p : nullptr;
   k.s_rows = S->s_rows;
@@ -1435,7 +1574,7 @@ extern "C" {
 static Status check_params(const cfm_ctx* c, const cfm_cilia_params* prm, const double* phase,
                            const double* phase_dot) {
   const Particles& p = c->p;
-  if (prm->beat != CFM_BEAT_FULFORD_BLAKE && prm->beat != CFM_BEAT_FULFORD_BLAKE_ORIGINAL)
+  if (prm->beat < CFM_BEAT_FULFORD_BLAKE || prm->beat > CFM_BEAT_BUILD_A_BEAT)
     return make_status(CFM_EINVAL, "unknown beat %d", prm->beat);
   if (!(prm->s_tol >= 0.0) || prm->s_tol >= 0.5)
     return make_status(CFM_EINVAL, "s_tol must lie in [0, 0.5) (0: 0.1 / nseg)");
@@ -1608,9 +1747,9 @@ int cfm_cilia_get_bodies(cfm_ctx* c, double* x, double* q, double* vel, double* 
   const size_t nb = S->n_body;
   auto run = [&]() -> Status {
     CFM_HIP(hipStreamSynchronize(c->stream));
-    if (x) CFM_HIP(hipMemcpy(x, S->bx.p, sizeof(double) * 3 * nb, hipMemcpyDeviceToHost));
-    if (q) CFM_HIP(hipMemcpy(q, S->bq.p, sizeof(double) * 4 * nb, hipMemcpyDeviceToHost));
-    if (vel) CFM_HIP(hipMemcpy(vel, S->vbody.p, sizeof(double) * 6 * nb, hipMemcpyDeviceToHost));
+    if (x) CFM_HIP(copy_to_host(c->stream, x, S->bx.p, sizeof(double) * 3 * nb));
+    if (q) CFM_HIP(copy_to_host(c->stream, q, S->bq.p, sizeof(double) * 4 * nb));
+    if (vel) CFM_HIP(copy_to_host(c->stream, vel, S->vbody.p, sizeof(double) * 6 * nb));
     if (f_blob) CFM_TRY(download_aos(c, f_blob, c->p.n_blob, 3, S->fblob.p, c->p.n_blob, 0));
     return {};
   };
@@ -1704,12 +1843,12 @@ int cfm_cilia_set_driving(cfm_ctx* c, const double* q_phase_table, int32_t n_pha
   const int nc = S->prm.n_cilia;
   auto run = [&]() -> Status {
     CFM_TRY(S->tab_phase.alloc(n_phase));
-    CFM_HIP(hipMemcpy(S->tab_phase.p, q_phase_table, sizeof(double) * n_phase, hipMemcpyHostToDevice));
+    CFM_HIP(copy_to_device(c->stream, S->tab_phase.p, q_phase_table, sizeof(double) * n_phase));
     S->n_tab_phase = n_phase;
     S->dyn_angle = q_angle_table != nullptr;
     if (S->dyn_angle) {
       CFM_TRY(S->tab_angle.alloc(n_angle));
-      CFM_HIP(hipMemcpy(S->tab_angle.p, q_angle_table, sizeof(double) * n_angle, hipMemcpyHostToDevice));
+      CFM_HIP(copy_to_device(c->stream, S->tab_angle.p, q_angle_table, sizeof(double) * n_angle));
       S->n_tab_angle = n_angle;
     }
     S->spring = spring;
@@ -1777,10 +1916,10 @@ int cfm_cilia_get_speeds(cfm_ctx* c, double* phase, double* phase_dot, double* a
   const int nc = S->prm.n_cilia;
   auto run = [&]() -> Status {
     CFM_HIP(hipStreamSynchronize(c->stream));
-    if (phase) CFM_HIP(hipMemcpy(phase, S->phase.p, sizeof(double) * nc, hipMemcpyDeviceToHost));
-    if (phase_dot) CFM_HIP(hipMemcpy(phase_dot, S->pdot.p, sizeof(double) * nc, hipMemcpyDeviceToHost));
-    if (angle) CFM_HIP(hipMemcpy(angle, S->angle.p, sizeof(double) * nc, hipMemcpyDeviceToHost));
-    if (angle_dot) CFM_HIP(hipMemcpy(angle_dot, S->adot.p, sizeof(double) * nc, hipMemcpyDeviceToHost));
+    if (phase) CFM_HIP(copy_to_host(c->stream, phase, S->phase.p, sizeof(double) * nc));
+    if (phase_dot) CFM_HIP(copy_to_host(c->stream, phase_dot, S->pdot.p, sizeof(double) * nc));
+    if (angle) CFM_HIP(copy_to_host(c->stream, angle, S->angle.p, sizeof(double) * nc));
+    if (angle_dot) CFM_HIP(copy_to_host(c->stream, angle_dot, S->adot.p, sizeof(double) * nc));
     return {};
   };
   Status s = run();
@@ -1817,9 +1956,9 @@ int cfm_cilia_get_state(cfm_ctx* c, double* x, double* vel_dir_phase, double* ve
   auto run = [&]() -> Status {
     if (x) {
       std::vector<double> t(3 * (size_t)ns);
-      CFM_HIP(hipMemcpy(t.data(), c->p.x.p, sizeof(double) * ns, hipMemcpyDeviceToHost));
-      CFM_HIP(hipMemcpy(t.data() + ns, c->p.y.p, sizeof(double) * ns, hipMemcpyDeviceToHost));
-      CFM_HIP(hipMemcpy(t.data() + 2 * (size_t)ns, c->p.z.p, sizeof(double) * ns, hipMemcpyDeviceToHost));
+      CFM_HIP(copy_to_host(c->stream, t.data(), c->p.x.p, sizeof(double) * ns));
+      CFM_HIP(copy_to_host(c->stream, t.data() + ns, c->p.y.p, sizeof(double) * ns));
+      CFM_HIP(copy_to_host(c->stream, t.data() + 2 * (size_t)ns, c->p.z.p, sizeof(double) * ns));
       for (int i = 0; i < ns; ++i)
         for (int d = 0; d < 3; ++d) x[3 * i + d] = t[(size_t)d * ns + i];
     }
@@ -1827,8 +1966,8 @@ int cfm_cilia_get_state(cfm_ctx* c, double* x, double* vel_dir_phase, double* ve
     if (vel_dir_angle) CFM_TRY(download_aos(c, vel_dir_angle, ns, 3, S->vangle.p, ns, 0));
     if (f_seg) CFM_TRY(download_aos(c, f_seg, ns, 3, S->f.p, ns, 0));
     if (s_values)
-      CFM_HIP(hipMemcpy(s_values, S->svals.p, sizeof(double) * nc * S->prm.nseg, hipMemcpyDeviceToHost));
-    if (phase) CFM_HIP(hipMemcpy(phase, S->phase.p, sizeof(double) * nc, hipMemcpyDeviceToHost));
+      CFM_HIP(copy_to_host(c->stream, s_values, S->svals.p, sizeof(double) * nc * S->prm.nseg));
+    if (phase) CFM_HIP(copy_to_host(c->stream, phase, S->phase.p, sizeof(double) * nc));
     return {};
   };
   Status s = run();

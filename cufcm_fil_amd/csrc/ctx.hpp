@@ -20,6 +20,18 @@ struct Status {
   bool ok() const { return code == CFM_OK; }
 };
 
+// Copies between host and device ordered after the work already queued on a stream.
+// The library's streams are non-blocking, so a plain hipMemcpy (null stream) neither
+// waits for their kernels nor is waited for by them.
+inline hipError_t copy_to_host(hipStream_t s, void* dst, const void* src, size_t bytes) {
+  hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s);
+  return e != hipSuccess ? e : hipStreamSynchronize(s);
+}
+inline hipError_t copy_to_device(hipStream_t s, void* dst, const void* src, size_t bytes) {
+  hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s);
+  return e != hipSuccess ? e : hipStreamSynchronize(s);
+}
+
 inline Status make_status(int code, const char* fmt, ...) {
   Status s;
   s.code = code;

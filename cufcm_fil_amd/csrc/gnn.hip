@@ -609,7 +609,7 @@ int cfm_gnn_init(cfm_ctx* c, uint32_t seed, int32_t capacity) {
     CFM_TRY(gnn_alloc(c, *G));
     std::vector<float> h;
     init_params(*G, seed, h);
-    CFM_HIP(hipMemcpy(G->theta.p, h.data(), sizeof(float) * P, hipMemcpyHostToDevice));
+    CFM_HIP(copy_to_device(c->stream, G->theta.p, h.data(), sizeof(float) * P));
     hipLaunchKernelGGL(fill_f32_kernel, gl(P), dim3(256), 0, c->stream, P, G->adam_m.p, 0.f);
     hipLaunchKernelGGL(fill_f32_kernel, gl(P), dim3(256), 0, c->stream, P, G->adam_v.p, 0.f);
     CFM_CHECK_LAUNCH();
@@ -720,7 +720,7 @@ int cfm_gnn_get_params(cfm_ctx* c, float* out) {
   Gnn* G = c ? gnn_of(c) : nullptr;
   if (!G || !out) return c ? gfail(c, make_status(CFM_ESTATE, "no network")) : CFM_EINVAL;
   (void)hipSetDevice(c->device);
-  hipError_t e = hipMemcpy(out, G->theta.p, sizeof(float) * G->lay.total, hipMemcpyDeviceToHost);
+  hipError_t e = copy_to_host(c->stream, out, G->theta.p, sizeof(float) * G->lay.total);
   return e == hipSuccess ? CFM_OK : gfail(c, make_status(CFM_EHIP, "%s", hipGetErrorString(e)));
 }
 
@@ -728,7 +728,7 @@ int cfm_gnn_set_params(cfm_ctx* c, const float* in) {
   Gnn* G = c ? gnn_of(c) : nullptr;
   if (!G || !in) return c ? gfail(c, make_status(CFM_ESTATE, "no network")) : CFM_EINVAL;
   (void)hipSetDevice(c->device);
-  hipError_t e = hipMemcpy(G->theta.p, in, sizeof(float) * G->lay.total, hipMemcpyHostToDevice);
+  hipError_t e = copy_to_device(c->stream, G->theta.p, in, sizeof(float) * G->lay.total);
   if (e != hipSuccess) return gfail(c, make_status(CFM_EHIP, "%s", hipGetErrorString(e)));
   Status s = gnn_forward(c, *G);
   if (s.ok() && hipStreamSynchronize(c->stream) != hipSuccess) s = make_status(CFM_EHIP, "sync failed");
@@ -744,9 +744,9 @@ int cfm_gnn_correction(cfm_ctx* c, double* corr, int32_t* nbr) {
   Status s = gnn_forward(c, *G);
   if (!s.ok()) return gfail(c, s);
   if (hipStreamSynchronize(c->stream) != hipSuccess) return gfail(c, make_status(CFM_EHIP, "sync failed"));
-  if (corr && hipMemcpy(corr, G->corr.p, sizeof(double) * G->E * 9, hipMemcpyDeviceToHost) != hipSuccess)
+  if (corr && copy_to_host(c->stream, corr, G->corr.p, sizeof(double) * G->E * 9) != hipSuccess)
     return gfail(c, make_status(CFM_EHIP, "copy failed"));
-  if (nbr && hipMemcpy(nbr, c->nf.nbr.p, sizeof(int) * (size_t)G->m * G_K, hipMemcpyDeviceToHost) != hipSuccess)
+  if (nbr && copy_to_host(c->stream, nbr, c->nf.nbr.p, sizeof(int) * (size_t)G->m * G_K) != hipSuccess)
     return gfail(c, make_status(CFM_EHIP, "copy failed"));
   return CFM_OK;
 }
@@ -766,7 +766,7 @@ int cfm_gnn_gradient(cfm_ctx* c, int32_t s0, int32_t s1, float* grad, double* lo
     CFM_CHECK_LAUNCH();
     CFM_TRY(gnn_backward(c, *G, 1.0 / (s1 - s0)));
     CFM_HIP(hipStreamSynchronize(c->stream));
-    if (grad) CFM_HIP(hipMemcpy(grad, G->grad.p, sizeof(float) * G->lay.total, hipMemcpyDeviceToHost));
+    if (grad) CFM_HIP(copy_to_host(c->stream, grad, G->grad.p, sizeof(float) * G->lay.total));
     if (loss) *loss = l;
     return {};
   };
@@ -784,11 +784,11 @@ int cfm_gemm_bf16(cfm_ctx* c, int32_t M, int32_t N, int32_t K, const float* A, i
     CFM_TRY(a.alloc((size_t)M * K));
     CFM_TRY(b.alloc((size_t)K * N));
     CFM_TRY(cc.alloc((size_t)M * N));
-    CFM_HIP(hipMemcpy(a.p, A, sizeof(float) * M * K, hipMemcpyHostToDevice));
-    CFM_HIP(hipMemcpy(b.p, B, sizeof(float) * K * N, hipMemcpyHostToDevice));
+    CFM_HIP(copy_to_device(c->stream, a.p, A, sizeof(float) * M * K));
+    CFM_HIP(copy_to_device(c->stream, b.p, B, sizeof(float) * K * N));
     CFM_TRY(gemm(c, M, N, K, a.p, transA ? M : K, transA != 0, b.p, transB ? K : N, transB != 0, nullptr, cc.p, N));
     CFM_HIP(hipStreamSynchronize(c->stream));
-    CFM_HIP(hipMemcpy(C, cc.p, sizeof(float) * M * N, hipMemcpyDeviceToHost));
+    CFM_HIP(copy_to_host(c->stream, C, cc.p, sizeof(float) * M * N));
     return {};
   };
   Status s = run();

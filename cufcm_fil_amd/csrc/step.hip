@@ -639,7 +639,7 @@ int cfm_fil_jacobian_inverse(cfm_ctx* c, int32_t nt, double* jinv) {
       const int N = 6 * F->K.nseg;
       const size_t tot = (size_t)F->K.nfil * N * N;
       std::vector<double> t(tot);
-      CFM_HIP(hipMemcpy(t.data(), F->jinv.p, sizeof(double) * tot, hipMemcpyDeviceToHost));
+      CFM_HIP(copy_to_host(c->stream, t.data(), F->jinv.p, sizeof(double) * tot));
       for (int f = 0; f < F->K.nfil; ++f)
         for (int r = 0; r < N; ++r)
           for (int cc = 0; cc < N; ++cc)
@@ -659,7 +659,7 @@ int cfm_fil_errors(cfm_ctx* c, int32_t nt, const double* v_seg, double* err, dou
     double bad = 0.0;
     CFM_TRY(residual(c, F, nt, F->err.p, &bad));
     if (badness) *badness = bad;
-    CFM_HIP(hipMemcpy(err, F->err.p, sizeof(double) * F->nbroy, hipMemcpyDeviceToHost));
+    CFM_HIP(copy_to_host(c->stream, err, F->err.p, sizeof(double) * F->nbroy));
     return {};
   };
   Status s = run();
@@ -670,7 +670,7 @@ int cfm_fil_update(cfm_ctx* c, const double* upd) {
   FIL_OR_FAIL(c);
   if (!upd) return fail_c(c, make_status(CFM_EINVAL, "null argument"));
   auto run = [&]() -> Status {
-    CFM_HIP(hipMemcpy(F->upd.p, upd, sizeof(double) * F->nbroy, hipMemcpyHostToDevice));
+    CFM_HIP(copy_to_device(c->stream, F->upd.p, upd, sizeof(double) * F->nbroy));
     CFM_TRY(update(c, F, F->upd.p));
     CFM_HIP(hipStreamSynchronize(c->stream));
     return {};
@@ -701,7 +701,7 @@ int cfm_fil_attach_bodies(cfm_ctx* c, const double* f_ext) {
   auto run = [&]() -> Status {
     // base offsets in the body frame from the current first segments
     std::vector<double> base(3 * (size_t)nfil), fref(3 * (size_t)nfil);
-    CFM_HIP(hipMemcpy(base.data(), F->base.p, sizeof(double) * base.size(), hipMemcpyDeviceToHost));
+    CFM_HIP(copy_to_host(c->stream, base.data(), F->base.p, sizeof(double) * base.size()));
     for (int f = 0; f < nfil; ++f) {
       const int b = f / (nfil / B);
       const double* q = &bi.q[4 * (size_t)b];

@@ -319,6 +319,15 @@ int cfm_gemm_bf16(cfm_ctx* ctx, int32_t M, int32_t N, int32_t K, const float* A,
 #define CFM_BEAT_FULFORD_BLAKE 1   /* Ito, Omori & Ishikawa (2019) fit, filament.cpp:684-758 */
 #define CFM_BEAT_FULFORD_BLAKE_ORIGINAL 2 /* the fit's unaltered coefficients, zero cosine mode
                                              weighted 0.5 (SHAPE_SEQUENCE 4, filament.cpp:813-868,977-1008) */
+#define CFM_BEAT_VOLVOX 3          /* volvox fit, four polynomial degrees (SHAPE_SEQUENCE 3,
+                                      filament.cpp:748-812) */
+#define CFM_BEAT_CORAL_LARVAE 4    /* coral-larvae fit (SHAPE_SEQUENCE 2, filament.cpp:869-893) */
+#define CFM_BEAT_BUILD_A_BEAT 5    /* tangent-angle model with amplitude 1.99, recovery window
+                                      1/12, effective stroke 0.75, zero-velocity avoidance 0.05
+                                      (SHAPE_SEQUENCE 0, filament.cpp:1238-1334,1401-1546,
+                                      config.hpp:133-139): segments at uniform arc fraction,
+                                      trapezoid sums of the unit tangent; s_tol and the s table
+                                      are unused */
 
 typedef struct {
   int32_t n_cilia;

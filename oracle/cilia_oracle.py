@@ -1,6 +1,9 @@
 """numpy restatement of the prescribed-cilium path -- TEST INFRASTRUCTURE ONLY.
 
-Follows the reference's fitted-beat cilium (CILIA_TYPE 3, SHAPE_SEQUENCE 1):
+Follows the reference's prescribed-shape cilium (CILIA_TYPE 3): the fitted beats of
+SHAPE_SEQUENCE 1 (described below), 4 (unaltered coefficients), 3 (volvox, four
+polynomial degrees) and 2 (coral larvae), and the build-a-beat tangent-angle model of
+SHAPE_SEQUENCE 0 (build_a_beat_angle, build_a_beat_placement).  For SHAPE_SEQUENCE 1:
   * Fourier/polynomial beat of Fulford & Blake as fitted by Ito, Omori & Ishikawa
     (2019, Table 1), rescaled to unit mean length with the zero mode halved
     (filament.cpp:684-758 coefficients, :964-1045 shape, tangent and phase derivative);
@@ -27,6 +30,13 @@ and its recorded segment parameters ..._s_values_NSEG=20_SEP=2.600000.dat
 with its own recorded forces returns its beat speed) -- by the arc-length property of
 the s-values, by finite differences of the shape in the phase, and by node, periodicity
 and slope properties of the interpolation (tests/test_cilia_oracle.py).
+The other beats have no reference output: their shape, tangent and phase direction
+agree by finite differences, the coral-larvae table has the unit period-mean length its
+source states, and the build-a-beat angle is pinned by continuity at its switch points,
+its range [theta_s, pi - theta_s], finite differences in the phase, the chord length of
+two unit tangents and the derivatives of the placement in phase and angle.  The volvox
+coefficient values themselves are parity unpinned: a transcription of
+filament.cpp:748-812, checked by reading only.
 """
 from __future__ import annotations
 
@@ -62,9 +72,43 @@ AY0 = np.array([[-0.654, 0.787, 0.202], [0.393, -1.516, 0.716], [-0.097, 0.032, 
 BX0 = np.array([[0.192, -0.499, 0.339], [-0.050, 0.423, -0.327], [0.012, 0.138, -0.114]])
 BY0 = np.array([[0.284, 1.045, -1.017], [0.006, 0.317, -0.276], [-0.059, 0.226, -0.196]])
 
-# beat name -> (Ax, Ay, Bx, By, weight of the zero cosine mode)
+# The volvox fit (SHAPE_SEQUENCE 3, filament.cpp:748-812): four polynomial degrees.
+AXV = np.array([[1.00490080, -0.19958656, -0.20422343, 0.10887946],
+                [0.00314934, -0.22239636, 1.18659976, -0.87409016],
+                [0.18942274, -1.24525818, 1.94063304, -0.78632572],
+                [-0.11162123, 1.07262966, -2.15142914, 1.20544741]])
+AYV = np.array([[-0.01665535, -0.27457190, 1.08406999, -0.56102820],
+                [0.17505739, -2.09412076, 1.99556763, -0.50179439],
+                [-0.12408600, 1.62438155, -3.21015214, 1.70178281],
+                [-0.16672661, 0.90896038, -1.50867276, 0.74387795]])
+BXV = np.array([[0.09140608, -0.54804089, 1.40643137, -0.60109684],
+                [-0.09670739, 1.14634486, -2.60675971, 1.55525152],
+                [-0.16451150, 0.98846449, -1.49147276, 0.64896090]])
+BYV = np.array([[-0.06343445, 2.15509751, -4.19609063, 2.05489100],
+                [-0.28828336, 1.45223183, -1.56580487, 0.44541361],
+                [0.02187088, -0.33070891, 0.94906955, -0.66157093]])
+
+# The coral-larvae fit (SHAPE_SEQUENCE 2, filament.cpp:869-893), scaled to unit mean length.
+AXC = np.array([[9.034276e-01, -6.634681e-01, 2.016123e-01],
+                [-2.468656e-01, 1.215869e+00, -5.867236e-01],
+                [3.075836e-01, -6.966322e-01, 4.554729e-01],
+                [2.378489e-02, -1.355582e-01, 7.315831e-02]])
+AYC = np.array([[-2.766076e-01, 6.368662e-01, -8.513947e-02],
+                [3.091205e-01, -1.993712e+00, 1.091847e+00],
+                [-2.952120e-01, 7.189021e-01, -6.328589e-01],
+                [-9.868018e-02, 3.312812e-01, -2.578206e-01]])
+BXC = np.array([[1.336807e-01, -6.807852e-01, 6.008592e-01],
+                [8.146824e-02, 3.472676e-01, -2.744220e-01],
+                [3.615272e-02, 8.619119e-02, -6.122992e-02]])
+BYC = np.array([[9.165546e-01, 8.848315e-02, -4.838832e-01],
+                [-2.946211e-01, 1.057964e+00, -6.335721e-01],
+                [6.122744e-02, -2.319094e-01, 1.906292e-01]])
+
+# beat name -> (Ax, Ay, Bx, By, weight of the zero cosine mode); columns = degrees
 BEATS = {"fulford_blake": (AX, AY, BX, BY, 1.0),
-         "fulford_blake_original": (AX0, AY0, BX0, BY0, 0.5)}
+         "fulford_blake_original": (AX0, AY0, BX0, BY0, 0.5),
+         "volvox": (AXV, AYV, BXV, BYV, 1.0),
+         "coral_larvae": (AXC, AYC, BXC, BYC, 1.0)}
 
 
 def _modes(psi, c0=1.0):
@@ -77,7 +121,7 @@ def tangent(s, psi, beat="fulford_blake"):
     """d shape / d s in the beat plane (filament.cpp:964-988)."""
     ax, ay, bx, by, c0 = BEATS[beat]
     cv, sv = _modes(psi, c0)
-    p = np.array([1.0, 2.0 * s, 3.0 * s * s])
+    p = np.array([(d + 1) * s ** d for d in range(ax.shape[1])])
     return (cv @ ax + sv @ bx) @ p, (cv @ ay + sv @ by) @ p
 
 
@@ -85,7 +129,7 @@ def shape(s, psi, beat="fulford_blake"):
     """Reference-frame position at parameter s (filament.cpp:990-1019)."""
     ax, ay, bx, by, c0 = BEATS[beat]
     cv, sv = _modes(psi, c0)
-    p = np.array([s, s * s, s ** 3])
+    p = np.array([s ** (d + 1) for d in range(ax.shape[1])])
     return np.array([(cv @ ax + sv @ bx) @ p, (cv @ ay + sv @ by) @ p, 0.0])
 
 
@@ -94,8 +138,66 @@ def shape_phase_derivative(s, psi, beat="fulford_blake"):
     ax, ay, bx, by, _ = BEATS[beat]
     dc = np.array([0.0] + [-n * math.sin(n * psi) for n in range(1, 4)])
     ds = np.array([n * math.cos(n * psi) for n in range(1, 4)])
-    p = np.array([s, s * s, s ** 3])
+    p = np.array([s ** (d + 1) for d in range(ax.shape[1])])
     return np.array([(dc @ ax + ds @ bx) @ p, (dc @ ay + ds @ by) @ p, 0.0])
+
+
+# Build-a-beat (SHAPE_SEQUENCE 0) with the reference's constants (config.hpp:133-139).
+BUILD_AMPLITUDE, BUILD_RECOVERY, BUILD_EFFECTIVE, BUILD_SHIFT = 1.99, 1.0 / 12.0, 0.75, 0.05
+
+
+def build_a_beat_angle(s, psi):
+    """Tangent angle theta(s, psi) of the build-a-beat model and d theta / d psi: theta
+    moves between theta_s = acos(amplitude / 2) and pi - theta_s through
+    delta = t - sin t cos t on the effective stroke and the recovery window, whose start
+    travels along the filament (filament.cpp:1240-1330)."""
+    th_s = math.acos(0.5 * BUILD_AMPLITUDE)
+    w_r, w_e = 2 * math.pi * BUILD_RECOVERY, 2 * math.pi * BUILD_EFFECTIVE
+    trans = w_e + s * (2 * math.pi - w_e - w_r)
+    sp = psi - 2 * math.pi * BUILD_SHIFT * s
+    sp -= 2 * math.pi * math.floor(sp / (2 * math.pi))
+    if sp < w_e:
+        t = math.pi * (w_e - sp) / w_e
+        delta, d_delta = t - math.sin(t) * math.cos(t), -2 * math.sin(t) ** 2 / w_e
+    elif sp < trans:
+        delta, d_delta = 0.0, 0.0
+    elif sp < trans + w_r:
+        t = math.pi * (sp - trans) / w_r
+        delta, d_delta = t - math.sin(t) * math.cos(t), 2 * math.sin(t) ** 2 / w_r
+    else:
+        delta, d_delta = math.pi, 0.0
+    span = math.pi - 2 * th_s
+    return th_s + span * delta / math.pi, span * d_delta
+
+
+def build_a_beat_placement(psi, base, nseg, dl, angle, frame):
+    """Segments at s = n / (nseg - 1) joined by trapezoid steps of the unit tangent
+    (sin theta, cos theta, 0) turned by frame Rshape; the phase direction the same sum
+    of d tangent / d psi (filament.cpp:1401-1546).  Two departures from the reference,
+    where its placement applies the frame twice to the base tangent and turns the lab
+    displacement by Rshape for the angle direction: the frame enters once, and the angle
+    direction is frame (e_z x frame^T (x_n - x_0)), as for the fitted beats."""
+    ca, sa = math.cos(angle), math.sin(angle)
+    R = frame @ np.array([[ca, -sa, 0.0], [sa, ca, 0.0], [0.0, 0.0, 1.0]])
+
+    def tk(s):
+        th, dth = build_a_beat_angle(s, psi)
+        return (R @ np.array([math.sin(th), math.cos(th), 0.0]),
+                R @ (dth * np.array([math.cos(th), -math.sin(th), 0.0])))
+
+    x = np.zeros((nseg, 3))
+    vphase = np.zeros((nseg, 3))
+    vangle = np.zeros((nseg, 3))
+    x[0] = base
+    t1, k1 = tk(0.0)
+    for n in range(1, nseg):
+        t2, k2 = tk(n / (nseg - 1))
+        x[n] = x[n - 1] + 0.5 * dl * (t1 + t2)
+        vphase[n] = vphase[n - 1] + 0.5 * dl * (k1 + k2)
+        ref = frame.T @ (x[n] - x[0])
+        vangle[n] = frame @ np.array([-ref[1], ref[0], 0.0])
+        t1, k1 = t2, k2
+    return x, vphase, vangle, np.arange(nseg) / (nseg - 1)
 
 
 def curve_length(s, psi, nseg=20, beat="fulford_blake"):
@@ -124,6 +226,9 @@ def s_values(psi, nseg=20, beat="fulford_blake", tol=None, margins=False):
     segment, how close any of its bisection comparisons came to flipping (the smaller
     of | |frac - target| - tol | and |frac - target|)."""
     tol = 0.1 / nseg if tol is None else tol
+    if beat == "build_a_beat":      # uniform arc fraction, no search
+        s = np.arange(nseg) / (nseg - 1)
+        return (s, np.full(nseg, np.inf)) if margins else s
     s = [0.0] * nseg
     s[-1] = 1.0
     m = np.full(nseg, np.inf)
@@ -181,6 +286,10 @@ def placement(psi, base, nseg=20, dl=2.6, angle=0.0, beat="fulford_blake", s=Non
     segment parameters by bisection unless given; frame: the cilium's rotation (3 x 3,
     default the wall frame FRAME)."""
     FRAME_ = FRAME if frame is None else np.asarray(frame, dtype=np.float64)
+    if beat == "build_a_beat":
+        x, vp, va, s = build_a_beat_placement(psi, np.asarray(base, dtype=np.float64), nseg,
+                                              dl, angle, FRAME_)
+        return x.reshape(-1), vp.reshape(-1), va.reshape(-1), s
     L = dl * (nseg - 1)
     s = s_values(psi, nseg, beat) if s is None else np.asarray(s)
     ca, sa = math.cos(angle), math.sin(angle)

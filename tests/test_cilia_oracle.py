@@ -174,3 +174,116 @@ def test_s_table_interpolation_rule():
     fine[:, ::2] = tab
     fine[:, 1::2] = 0.5 * (tab[:, :-1] + tab[:, 1:])
     assert np.allclose(co.s_values_from_table(fine, 2 * math.pi * 5 / 300, 20), tab[5], atol=1e-12)
+
+
+FITTED = ["fulford_blake", "fulford_blake_original", "volvox", "coral_larvae"]
+
+
+@pytest.mark.parametrize("beat", FITTED)
+def test_fitted_beats_evaluate_their_tables_consistently(beat):
+    """Every fitted beat, the four-degree volvox one included: the shape starts at the
+    base, its s-derivative is the tangent and its phase derivative the phase direction
+    (a dropped or mis-weighted polynomial degree breaks the first or second)."""
+    h = 1e-5
+    for psi in (0.4, 2.5, 5.1):
+        assert np.allclose(co.shape(0.0, psi, beat), 0.0)
+        for s in (0.15, 0.5, 0.95):
+            fd = (co.shape(s, psi + h, beat) - co.shape(s, psi - h, beat)) / (2 * h)
+            assert np.allclose(co.shape_phase_derivative(s, psi, beat), fd, atol=1e-8)
+            tfd = (co.shape(s + h, psi, beat) - co.shape(s - h, psi, beat)) / (2 * h)
+            assert np.allclose(co.tangent(s, psi, beat), tfd[:2], atol=1e-8)
+
+
+def test_coral_larvae_fit_has_unit_mean_length():
+    """The reference scales the coral-larvae coefficients to a period-mean length of 1
+    (filament.cpp:871); a sign error in any one coefficient moves it by more than 1e-3."""
+    phases = 2 * math.pi * np.arange(64) / 64
+    mean = lambda b: np.mean([co.curve_length(1.0, p, 100, b) for p in phases])
+    assert abs(mean("coral_larvae") - 1.0) < 2e-4
+    ax, ay, bx, by, c0 = co.BEATS["coral_larvae"]
+    try:
+        for k in range(3):
+            bad = ay.copy()
+            bad[k + 1, 1] *= -1.0
+            co.BEATS["_planted"] = (ax, bad, bx, by, c0)
+            assert abs(mean("_planted") - 1.0) > 1e-3
+    finally:
+        co.BEATS.pop("_planted", None)
+
+
+def _bab_boundaries(s):
+    w_r, w_e = 2 * math.pi * co.BUILD_RECOVERY, 2 * math.pi * co.BUILD_EFFECTIVE
+    trans = w_e + s * (2 * math.pi - w_e - w_r)
+    shift = 2 * math.pi * co.BUILD_SHIFT * s
+    return [b + shift for b in (0.0, w_e, trans, trans + w_r)]
+
+
+@pytest.mark.parametrize("s", [0.0, 0.3, 0.77, 1.0])
+def test_build_a_beat_angle_is_continuous_and_its_phase_derivative_matches(s):
+    th_s = math.acos(0.5 * co.BUILD_AMPLITUDE)
+    eps = 1e-9
+    for b in _bab_boundaries(s):
+        lo, _ = co.build_a_beat_angle(s, b - eps)
+        hi, _ = co.build_a_beat_angle(s, b + eps)
+        assert abs(lo - hi) < 1e-7, (s, b)
+    h = 1e-6
+    for psi in np.linspace(0.05, 2 * math.pi - 0.05, 57):
+        th, dth = co.build_a_beat_angle(s, psi)
+        assert th_s - 1e-15 <= th <= math.pi - th_s + 1e-15
+        fd = (co.build_a_beat_angle(s, psi + h)[0] - co.build_a_beat_angle(s, psi - h)[0]) / (2 * h)
+        assert abs(fd - dth) < 1e-5 * (1 + abs(dth)), (s, psi)
+    # the stroke sweeps the full range: theta_s at the end of the effective stroke,
+    # pi - theta_s between recovery and the next effective stroke
+    b = _bab_boundaries(s)
+    assert abs(co.build_a_beat_angle(s, b[1])[0] - th_s) < 1e-12
+    if s < 1.0:
+        assert abs(co.build_a_beat_angle(s, b[3] + 1e-3)[0] - (math.pi - th_s)) < 1e-12
+
+
+@pytest.mark.parametrize("psi", [0.5, 3.0, 5.0, 5.9])
+def test_build_a_beat_placement_derivatives(psi):
+    """Segments are joined by the chord of two unit tangents (length dl cos of half
+    their angle); the phase and angle directions are the derivatives of the placement in
+    the phase and in the shape angle."""
+    base, nseg, dl, ang = np.array([0.3, -0.2, 1.3]), 20, 2.6, 0.4
+    x, vp, va, s = co.placement(psi, base, nseg, dl, angle=ang, beat="build_a_beat")
+    p = x.reshape(-1, 3)
+    assert np.allclose(s, np.arange(nseg) / (nseg - 1)) and np.allclose(p[0], base)
+    th = np.array([co.build_a_beat_angle(v, psi)[0] for v in s])
+    steps = np.linalg.norm(np.diff(p, axis=0), axis=1)
+    assert np.allclose(steps, dl * np.cos(0.5 * np.diff(th)), rtol=1e-12)
+    h = 1e-6
+    xp = co.placement(psi + h, base, nseg, dl, angle=ang, beat="build_a_beat")[0]
+    xm = co.placement(psi - h, base, nseg, dl, angle=ang, beat="build_a_beat")[0]
+    assert np.allclose((xp - xm) / (2 * h), vp, atol=1e-5 * np.abs(vp).max())
+    xp = co.placement(psi, base, nseg, dl, angle=ang + h, beat="build_a_beat")[0]
+    xm = co.placement(psi, base, nseg, dl, angle=ang - h, beat="build_a_beat")[0]
+    assert np.allclose((xp - xm) / (2 * h), va, atol=1e-6 * np.abs(va).max())
+    # the first chord bisects the base tangent (sin theta, cos theta) of the reference
+    # frame, turned once by the shape angle and once by the frame, and the next one
+    ca, sa = math.cos(ang), math.sin(ang)
+    t = [co.FRAME @ np.array([ca * math.sin(v) - sa * math.cos(v),
+                              sa * math.sin(v) + ca * math.cos(v), 0.0]) for v in th[:2]]
+    d1 = (p[1] - p[0]) / steps[0]
+    assert np.allclose(d1, (t[0] + t[1]) / np.linalg.norm(t[0] + t[1]), atol=1e-12)
+
+
+def test_build_a_beat_tip_sweeps_the_amplitude():
+    """Straight filament at tangent angle theta: the tip lies at L (sin theta, cos theta)
+    in the reference frame; between theta_s and pi - theta_s the tip's beat-direction
+    coordinate moves by amplitude x L (1.99 filament lengths)."""
+    th_s = math.acos(0.5 * co.BUILD_AMPLITUDE)
+    L = 1.0
+    y = [L * math.cos(t) for t in (th_s, math.pi - th_s)]
+    assert abs((y[0] - y[1]) - co.BUILD_AMPLITUDE * L) < 1e-12
+    # the model's tip over a cycle stays within that sweep and reaches most of it
+    nseg, dl = 20, 1.0 / 19
+    tips = []
+    for psi in np.linspace(0, 2 * math.pi, 200, endpoint=False):
+        x, _, _, _ = co.placement(psi, np.zeros(3), nseg, dl, beat="build_a_beat",
+                                  frame=np.eye(3))
+        tips.append(x.reshape(-1, 3)[-1])
+    tips = np.array(tips)
+    assert tips[:, 1].max() <= y[0] + 1e-12 and tips[:, 1].min() >= y[1] - 1e-12
+    assert tips[:, 1].max() - tips[:, 1].min() > 0.8 * co.BUILD_AMPLITUDE * L
+    assert np.all(np.linalg.norm(tips, axis=1) <= L + 1e-12)

@@ -29,6 +29,35 @@ def test_geometry_matches_oracle():
         assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
 
 
+@pytest.mark.parametrize("beat", ["volvox", "coral_larvae", "build_a_beat",
+                                  "fulford_blake_original"])
+def test_other_beats_geometry_matches_oracle(beat):
+    """Three cilia at different phases and shape angles.  Segment parameters: equal to
+    the oracle's where no bisection comparison came within 1e-12 of flipping, and in any
+    case splitting the curve into equal arcs to within the tolerance; the placement
+    against the oracle's at the library's parameters."""
+    base = [0.0, 0.0, 1.3, 40.0, 0.0, 1.3, 0.0, 40.0, 1.3]
+    phases, angles = [0.4, 2.9, 5.6], [0.0, 0.3, -0.7]
+    with PrescribedCilia(base, phase=phases, nseg=12, beat=beat) as C:
+        C.set_phases(angle=angles)
+        st = C.state()
+    tol = 0.1 / 12
+    for c in range(3):
+        s_gpu = st["s_values"][c]
+        s_ref, margin = co.s_values(phases[c], 12, beat, margins=True)
+        safe = margin > 1e-12
+        assert np.array_equal(s_gpu[safe], s_ref[safe]), (beat, c)
+        if beat != "build_a_beat":
+            total = co.curve_length(1.0, phases[c], 12, beat)
+            frac = np.array([co.curve_length(v, phases[c], 12, beat) / total for v in s_gpu])
+            assert np.all(np.abs(frac - np.arange(12) / 11) <= tol + 1e-12)
+        x, vp, va, _ = co.placement(phases[c], base[3 * c:3 * c + 3], 12, 2.6, angles[c], beat,
+                                    s=s_gpu)
+        sl = slice(36 * c, 36 * c + 36)
+        for got, ref in ((st["x"], x), (st["vel_dir_phase"], vp), (st["vel_dir_angle"], va)):
+            assert np.abs(got[sl] - ref).max() <= 1e-11 * np.abs(ref).max(), (beat, c)
+
+
 @pytest.mark.parametrize("mode", ["fp64", "mixed"])
 def test_generalised_forces_match_reference_files(mode):
     """One 20-segment cilium, phase speed 2 pi, 300 steps (dt = 1/300): the reference

@@ -57,6 +57,18 @@ def test_prescribed_phase_speeds_match_oracle(precond):
             _compare(G, sw, nt, qp_o, qa_o, qp_g, qa_g)
 
 
+@pytest.mark.parametrize("beat", ["volvox", "coral_larvae", "build_a_beat"])
+def test_other_beats_on_bodies_match_oracle(beat):
+    sw = swimmer(n_body=1, per=3, seed=14, dt=0.02)
+    sw.beat = beat
+    with _gpu(sw, "diag", beat=beat) as G:
+        for nt in range(2):
+            qp_g, qa_g, stats = G.step(nt)
+            _, qp_o, qa_o = sw.step(nt)
+            assert stats["converged"], stats
+            _compare(G, sw, nt, qp_o, qa_o, qp_g, qa_g)
+
+
 @pytest.mark.parametrize("precond", ["diag", "near", "dense"])
 def test_dynamic_phases_and_angles_match_oracle(precond):
     qp = np.loadtxt(os.path.join(GOLD, "fb_phase_generalised_forces_nseg20_sep2.6.dat"))[1:]
