@@ -27,6 +27,7 @@ plain GPU step performed.
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
 import math
 import os
@@ -462,7 +463,22 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run one device-to-device copy of BYTES bytes (calibrates the "
                          "FETCH_SIZE / WRITE_SIZE counters against a copy of known size)")
+    ap.add_argument("--issue-probe", action="store_true",
+                    help="only measure the FP32 VALU issue rate of the instruction forms the "
+                         "pair kernel is built from (packed FP32 disabled), lane-instructions/s")
     args = ap.parse_args()
+    if args.issue_probe:
+        from cufcm_fil_amd import _lib as L
+        forms = ["v_fmac_f32", "v_mul_f32_sgpr", "v_add_f32_sgpr"]
+        rates = {}
+        for k, name in enumerate(forms):
+            r = ctypes.c_double(0.0)
+            rc = L.lib().cfm_probe_valu_rate(0, k, ctypes.byref(r))
+            if rc != 0:
+                raise RuntimeError(f"cfm_probe_valu_rate({name}) failed: {rc}")
+            rates[name] = r.value
+        print(json.dumps({"valu_issue_rate_lane_instr_per_s": rates}), flush=True)
+        return
     if args.copy_probe > 0:
         import torch
         a = torch.ones(args.copy_probe // 4, dtype=torch.float32, device="cuda")

@@ -274,6 +274,7 @@ def lib():
     L.cfm_batch_gmres.argtypes = [_vp, _dp, _dp, ctypes.POINTER(cfm_solve_opts),
                                   ctypes.POINTER(cfm_batch_sys_stats), _dp]
     L.cfm_profile.argtypes = [_vp, ctypes.c_int32]
+    L.cfm_probe_valu_rate.argtypes = [ctypes.c_int32, ctypes.c_int32, _dp]
     L.cfm_profile_read.argtypes = [_vp, ctypes.POINTER(cfm_profile_stats)]
     _lib = L
     return L

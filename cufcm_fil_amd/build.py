@@ -15,7 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libcufcm_fil.so")
 SOURCES = ["api.hip", "apply.hip", "gmres.hip", "fil.hip", "step.hip", "gnn.hip", "nearfield.hip",
-           "densepc.hip", "batch.hip", "cilia.hip", "body.hip", "swim.hip"]
+           "densepc.hip", "batch.hip", "cilia.hip", "body.hip", "swim.hip", "probe.hip"]
 ARCH = os.environ.get("CFM_OFFLOAD_ARCH", "gfx950")
 
 FLAGS = [
