@@ -469,7 +469,7 @@ def main():
     args = ap.parse_args()
     if args.issue_probe:
         from cufcm_fil_amd import _lib as L
-        forms = ["v_fmac_f32", "v_mul_f32_sgpr", "v_add_f32_sgpr"]
+        forms = ["v_fmac_f32", "v_mul_f32_sgpr", "v_add_f32_sgpr", "v_mul_f32", "v_fma_f32"]
         rates = {}
         for k, name in enumerate(forms):
             r = ctypes.c_double(0.0)

@@ -20,12 +20,10 @@ __global__ void probe_fmac_kernel(float* out, int iters, float a) {
 #pragma unroll
   for (int k = 0; k < 16; ++k) acc[k] = threadIdx.x * 1e-3f + k;
 #pragma unroll
-  for (int k = 0; k < 4; ++k) x[k] = threadIdx.x * 2e-3f + k;
+  for (int k = 0; k < 4; ++k) x[k] = threadIdx.x * 2e-3f + k + a;
   for (int i = 0; i < iters; ++i) {
 #pragma unroll
     for (int k = 0; k < 16; ++k) acc[k] = __builtin_fmaf(x[k & 3], x[(k + 1) & 3], acc[k]);
-#pragma unroll
-    for (int k = 0; k < 4; ++k) x[k] = x[k] * a;
   }
   float s = 0.f;
 #pragma unroll
@@ -63,12 +61,46 @@ __global__ void probe_add_kernel(float* out, int iters, float a) {
   out[blockIdx.x * blockDim.x + threadIdx.x] = s;
 }
 
+// v_mul_f32 with VGPR operands only.
+__global__ void probe_mulvv_kernel(float* out, int iters, float a) {
+  float acc[16], x[4];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) acc[k] = threadIdx.x * 1e-3f + k;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) x[k] = threadIdx.x * 2e-3f + k + a;
+  for (int i = 0; i < iters; ++i) {
+#pragma unroll
+    for (int k = 0; k < 16; ++k) acc[k] = acc[k] * x[k & 3];
+  }
+  float s = 0.f;
+#pragma unroll
+  for (int k = 0; k < 16; ++k) s += acc[k];
+  out[blockIdx.x * blockDim.x + threadIdx.x] = s + x[0] + x[1] + x[2] + x[3];
+}
+
+// v_fma_f32 (three-operand form) with VGPR operands only.
+__global__ void probe_fma3_kernel(float* out, int iters, float a) {
+  float acc[16], x[4];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) acc[k] = threadIdx.x * 1e-3f + k;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) x[k] = threadIdx.x * 2e-3f + k + a;
+  for (int i = 0; i < iters; ++i) {
+#pragma unroll
+    for (int k = 0; k < 16; ++k) acc[k] = __builtin_fmaf(acc[k], x[k & 3], x[(k + 1) & 3]);
+  }
+  float s = 0.f;
+#pragma unroll
+  for (int k = 0; k < 16; ++k) s += acc[k];
+  out[blockIdx.x * blockDim.x + threadIdx.x] = s + x[0] + x[1] + x[2] + x[3];
+}
+
 }  // namespace
 }  // namespace cfm
 
 extern "C" int cfm_probe_valu_rate(int32_t device, int32_t form, double* lane_instr_per_s) {
   using namespace cfm;
-  if (!lane_instr_per_s || form < 0 || form > 2) return CFM_EINVAL;
+  if (!lane_instr_per_s || form < 0 || form > 4) return CFM_EINVAL;
   *lane_instr_per_s = 0.0;
   if (hipSetDevice(device) != hipSuccess) return CFM_EHIP;
   float* out = nullptr;
@@ -76,8 +108,7 @@ extern "C" int cfm_probe_valu_rate(int32_t device, int32_t form, double* lane_in
   hipEvent_t e0 = nullptr, e1 = nullptr;
   int rc = CFM_EHIP;
   const size_t n = (size_t)kProbeBlocks * kProbeThreads;
-  // VALU instructions per loop iteration per lane: 16 (+4 multiplier updates)
-  const double per_iter = form == 0 ? 20.0 : 16.0;
+  const double per_iter = 16.0;  // VALU instructions per loop iteration per lane
   if (hipMalloc(&out, sizeof(float) * n) == hipSuccess &&
       hipStreamCreateWithFlags(&st, hipStreamNonBlocking) == hipSuccess &&
       hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess) {
@@ -88,6 +119,8 @@ extern "C" int cfm_probe_valu_rate(int32_t device, int32_t form, double* lane_in
       switch (form) {
         case 0: hipLaunchKernelGGL(probe_fmac_kernel, dim3(kProbeBlocks), dim3(kProbeThreads), 0, st, out, kProbeIters, 0.999f); break;
         case 1: hipLaunchKernelGGL(probe_mul_kernel, dim3(kProbeBlocks), dim3(kProbeThreads), 0, st, out, kProbeIters, 0.999f); break;
+        case 3: hipLaunchKernelGGL(probe_mulvv_kernel, dim3(kProbeBlocks), dim3(kProbeThreads), 0, st, out, kProbeIters, 0.999f); break;
+        case 4: hipLaunchKernelGGL(probe_fma3_kernel, dim3(kProbeBlocks), dim3(kProbeThreads), 0, st, out, kProbeIters, 0.999f); break;
         default: hipLaunchKernelGGL(probe_add_kernel, dim3(kProbeBlocks), dim3(kProbeThreads), 0, st, out, kProbeIters, 1e-3f); break;
       }
       ok = ok && hipGetLastError() == hipSuccess && hipEventRecord(e1, st) == hipSuccess &&

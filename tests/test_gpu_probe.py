@@ -12,10 +12,10 @@ pytestmark = pytest.mark.gpu
 
 def test_probe_rates_are_plausible_and_bad_forms_refused():
     lib = L.lib()
-    for form in range(3):
+    for form in range(5):
         r = ctypes.c_double(0.0)
         assert lib.cfm_probe_valu_rate(0, form, ctypes.byref(r)) == L.CFM_OK
         assert 1e13 < r.value < 2e14, (form, r.value)
     r = ctypes.c_double(0.0)
-    assert lib.cfm_probe_valu_rate(0, 3, ctypes.byref(r)) == L.CFM_EINVAL
+    assert lib.cfm_probe_valu_rate(0, 5, ctypes.byref(r)) == L.CFM_EINVAL
     assert lib.cfm_probe_valu_rate(0, 0, None) == L.CFM_EINVAL
