@@ -135,3 +135,36 @@ def test_autotuned_split_matches_oracle(prec):
         for got, ref in ((vs, rs), (vb, rb)):
             e, t = _err(got, ref, prec)
             assert e < t, ("FULL", e)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32c"])
+@pytest.mark.parametrize("ns,nb", [(1, 0), (2, 0), (63, 1), (65, 0), (257, 130), (0, 5),
+                                   (0, 257), (300, 64)])
+def test_ragged_sizes_match_oracle(ns, nb, prec):
+    """Sizes on either side of the 64-target and 256-source tiles, single particles
+    (the self term alone) and single-species contexts, every product against the
+    oracle's direct double loop."""
+    rng = np.random.default_rng(ns * 1000 + nb)
+    xs = np.column_stack([rng.uniform(0, 40, ns), rng.uniform(0, 40, ns),
+                          rng.uniform(1.5, 30, ns)]).reshape(-1)
+    xb = np.column_stack([rng.uniform(0, 40, nb), rng.uniform(0, 40, nb),
+                          rng.uniform(1.5, 30, nb)]).reshape(-1)
+    fs, fb = geo.loads(ns, 6, 7) if ns else np.zeros(0), geo.loads(nb, 3, 8) if nb else np.zeros(0)
+    ids, idb = np.arange(ns), np.arange(ns, ns + nb)
+    with RpyMobilitySolver(ns, nb, precision=prec) as m:
+        m.set_positions(xs, xb)
+        if ns:
+            e, t = _err(m.evaluate_segment_segment_mobility(fs), oracle.apply(xs, 6, xs, 6, fs), prec)
+            assert e < t, ("SS", e)
+        if nb:
+            e, t = _err(m.evaluate_blob_blob_mobility(fb), oracle.apply(xb, 3, xb, 3, fb), prec)
+            assert e < t, ("BB", e)
+        if ns and nb:
+            ref = oracle.apply(xb, 3, xs, 6, fs, id_tgt=idb, id_src=ids)
+            e, t = _err(m.evaluate_blob_segment_mobility(fs), ref, prec)
+            assert e < t, ("BS", e)
+            vs, vb = m.evaluate_full_mobility(fs, fb)
+            rs, rb = oracle.apply_full(xs, fs, xb, fb)
+            for got, r in ((vs, rs), (vb, rb)):
+                e, t = _err(got, r, prec)
+                assert e < t, ("FULL", e)
