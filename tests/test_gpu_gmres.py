@@ -5,6 +5,7 @@ import pytest
 
 import oracle
 from cufcm_fil_amd import RpyMobilitySolver
+from cufcm_fil_amd import _lib as L
 from cufcm_fil_amd import geometry as geo
 
 pytestmark = pytest.mark.gpu
@@ -168,3 +169,23 @@ def test_restarted_gmres_iterations_against_unrestarted_reference():
         _, st = m.solve("seg", v, rtol=1e-8, restart=50, precond="self", mode="fp64")
     assert res_ref <= 1e-8 and st["converged"]
     assert st["iterations"] <= 2 * it_ref + 10, (st["iterations"], it_ref)
+
+
+@pytest.mark.parametrize("mode", ["fp64", "mixed"])
+def test_iteration_cap_reports_non_convergence(mode):
+    """A solve stopped by max_iter returns CFM_ENOCONV (raised by default), reports
+    converged = 0 and its iteration count, and leaves in x the iterate whose FP64
+    residual (checked with the oracle's product) is the reported one."""
+    xs = geo.tilted_lattice(nx=3, ny=3, nseg=12, seed=5)      # 108 segments
+    n = xs.size // 3
+    b = geo.loads(n, 6, 11)
+    with RpyMobilitySolver(n, 0) as m:
+        m.set_positions(xs)
+        with pytest.raises(L.CfmError):
+            m.solve("seg", b, rtol=1e-14, restart=10, max_iter=4, precond="diag", mode=mode)
+        x, st = m.solve("seg", b, rtol=1e-14, restart=10, max_iter=4, precond="diag", mode=mode,
+                        raise_on_noconv=False)
+    assert not st["converged"] and 1 <= st["iterations"] <= 4, st
+    rel = _oracle_rel_res(xs, 6, x, b)
+    assert 1e-14 < st["rel_residual"] < 1.0
+    assert abs(rel - st["rel_residual"]) <= 1e-9 + 1e-6 * rel, (rel, st["rel_residual"])
