@@ -529,7 +529,7 @@ def main():
         print(json.dumps({"copy_probe_bytes": a.numel() * 4}), flush=True)
         return
     if args.gpus != 1:
-        print("bench.py measures one GPU only: this build runs the implicit step on a "
+        print("bench.py measures one GPU only: the library runs the implicit step on a "
               "single MI355X (multi-GPU sharding is a design note, not a run path).",
               file=sys.stderr)
         sys.exit(2)

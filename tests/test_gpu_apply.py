@@ -1,6 +1,6 @@
 """GPU parity of the mobility products (through the C ABI) against the FP64 oracle.
 
-Tolerances (north star): FP64 path <= 1e-12 relative (max-norm of the error over
+Tolerances: FP64 path <= 1e-12 relative (max-norm of the error over
 max-norm of the oracle result); FP32-compensated path <= 1e-6 absolute.
 """
 import numpy as np
