@@ -791,11 +791,14 @@ Status fil_errors(cfm_ctx* c, const FilK& K, int nt, double* err, double* bad_pa
 Status fil_jacobian(cfm_ctx* c, const FilK& K, int nt, int* status_dev) {
   const int N = 6 * K.nseg;
   const size_t lds = (size_t)N * N * sizeof(double);
-  static bool attr_set = false;
-  if (!attr_set) {
+  if (lds > 160 * 1024)
+    return make_status(CFM_EINVAL, "approximate Jacobian of %d segments exceeds the LDS of one CU", K.nseg);
+  // the kernel's dynamic-LDS limit is raised to the largest block any context has used
+  static size_t attr_bytes = 0;
+  if (lds > attr_bytes) {
     CFM_HIP(hipFuncSetAttribute((const void*)fil_jacobian_kernel,
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    attr_set = true;
+    attr_bytes = lds;
   }
   hipLaunchKernelGGL(fil_jacobian_kernel, dim3(K.nfil), dim3(256), lds, c->stream, K, nt);
   CFM_CHECK_LAUNCH();
