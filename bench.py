@@ -93,6 +93,7 @@ def run_path(args, label, precision, solve_mode, guess, precond, overlap=False):
         t1 = time.perf_counter()
         prof = S.mob.profile_read()
         S.mob.profile(False)
+        final_x = S.state()["x"]
     finally:
         S.close()
     el = t1 - t0
@@ -103,7 +104,9 @@ def run_path(args, label, precision, solve_mode, guess, precond, overlap=False):
         "gmres_iterations": sum(s["gmres_iterations"] for s in stats),
         "max_gmres_rel_residual": max(s["max_gmres_rel_residual"] for s in stats),
         "all_converged": all(s["converged"] for s in stats),
+        "broyden_per_step": [s["broyden_iterations"] for s in stats],
         "profile": prof,
+        "final_x": final_x,
     }
 
 
@@ -513,7 +516,7 @@ def main():
         rates = {}
         for k, name in enumerate(forms):
             r = ctypes.c_double(0.0)
-            rc = L.lib().cfm_probe_valu_rate(0, k, ctypes.byref(r))
+            rc = L.probe_lib().cfm_probe_valu_rate(0, k, ctypes.byref(r))
             if rc != 0:
                 raise RuntimeError(f"cfm_probe_valu_rate({name}) failed: {rc}")
             rates[name] = r.value
@@ -547,13 +550,23 @@ def main():
                "broyden_iterations": o["broyden_iterations"], "gmres_iterations": o["gmres_iterations"],
                "max_gmres_rel_residual": o["max_gmres_rel_residual"]}
 
+    if plain is not None:
+        # both paths start from the same state and take the same steps: the mixed path's
+        # segment positions after the timed steps against the plain FP64 path's
+        import numpy as np
+        mixed["max_abs_state_diff_vs_plain"] = float(np.abs(mixed["final_x"] - plain["final_x"]).max())
+        mixed["same_broyden_iterations_as_plain"] = mixed["broyden_per_step"] == plain["broyden_per_step"]
     prof = mixed["profile"]
     kms = prof["kernel_ms_fp32"] / max(1, prof["launches_fp32"])
     flops_launch = prof["alg_flops_fp32"] / max(1, prof["launches_fp32"])
     achieved = flops_launch / (kms * 1e-3) / 1e12 if kms > 0 else 0.0
     roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": FP32_VALU_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(achieved / FP32_VALU_PEAK_TFLOPS, 4),
-                "traffic": measured_traffic(), "kernel": "rpy_apply_kernel<float,6,true>",
+                "traffic": measured_traffic(),
+                "traffic_source": os.path.relpath(PMC_SUMMARY, ROOT) + " (FETCH_SIZE and WRITE_SIZE "
+                                  "counter runs of this workload, copy-calibrated; not counted in "
+                                  "this run)",
+                "kernel": "rpy_apply_kernel<float,6,true>",
                 "launches": prof["launches_fp32"], "avg_ms": round(kms, 4),
                 "alg_flops_per_launch": flops_launch}
     batched = None if args.skip_batch else run_batched(args)
@@ -589,8 +602,9 @@ def main():
             "learned_guess": bool(args.learned),
             "parallelism": "single GPU",
         },
-        "mixed": {k: v for k, v in mixed.items() if k != "profile"},
-        "plain_fp64": None if plain is None else {k: v for k, v in plain.items() if k != "profile"},
+        "mixed": {k: v for k, v in mixed.items() if k not in ("profile", "final_x")},
+        "plain_fp64": None if plain is None else {k: v for k, v in plain.items()
+                                                  if k not in ("profile", "final_x")},
         "speedup_vs_plain_fp64": None if plain is None else round(mixed["steps_per_s"] / plain["steps_per_s"], 4),
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -14,6 +14,9 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libcufcm_fil.so")
 HEADER = os.path.join(os.path.dirname(HERE), "include", "cufcm_fil.h")
+# measurement helper beside the mobility library (include/cfm_probe.h)
+PROBE_LIB_PATH = os.path.join(HERE, "libcfm_probe.so")
+PROBE_HEADER = os.path.join(os.path.dirname(HERE), "include", "cfm_probe.h")
 
 CFM_OK, CFM_EINVAL, CFM_EHIP, CFM_ENOMEM, CFM_ESTATE, CFM_ENOCONV, CFM_ENONFINITE = range(7)
 CFM_FP64, CFM_FP32C = 0, 1
@@ -186,9 +189,9 @@ _vp = ctypes.c_void_p
 _lib = None
 
 
-def header_symbols() -> list[str]:
-    """Function names declared in include/cufcm_fil.h."""
-    with open(HEADER) as fh:
+def header_symbols(header: str = HEADER) -> list[str]:
+    """Function names declared in a header (default include/cufcm_fil.h)."""
+    with open(header) as fh:
         text = fh.read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     return sorted(set(re.findall(r"\b(cfm_[a-z0-9_]+)\s*\(", text)))
@@ -274,10 +277,24 @@ def lib():
     L.cfm_batch_gmres.argtypes = [_vp, _dp, _dp, ctypes.POINTER(cfm_solve_opts),
                                   ctypes.POINTER(cfm_batch_sys_stats), _dp]
     L.cfm_profile.argtypes = [_vp, ctypes.c_int32]
-    L.cfm_probe_valu_rate.argtypes = [ctypes.c_int32, ctypes.c_int32, _dp]
     L.cfm_profile_read.argtypes = [_vp, ctypes.POINTER(cfm_profile_stats)]
     _lib = L
     return L
+
+
+_probe = None
+
+
+def probe_lib():
+    """libcfm_probe.so (the FP32 issue-rate probe; not part of the mobility library)."""
+    global _probe
+    if _probe is None:
+        if not os.path.exists(PROBE_LIB_PATH):
+            raise ImportError(f"{PROBE_LIB_PATH} is not built; run `python -m cufcm_fil_amd.build`")
+        P = ctypes.CDLL(PROBE_LIB_PATH)
+        P.cfm_probe_valu_rate.argtypes = [ctypes.c_int32, ctypes.c_int32, _dp]
+        _probe = P
+    return _probe
 
 
 def ptr(a):

@@ -526,7 +526,9 @@ struct cfm_batch {
   int nchunk = 0;
   DevBuf<double> Q, H, cs, sn, beta, h, h2, yv, kpart, sinfo, r, w, zt, xs, dv, rhs, otarget;
   DevBuf<int> iinfo, jdev;
-  int* host_flags = nullptr;  // pinned: running-system count of the last step
+  // pinned: [0, 4) the step counters after a residual pass, [4, 8) a ring of the
+  // running-system count after each graph replay (slot = step % 4)
+  int* host_flags = nullptr;
   hipGraphExec_t graph[2] = {nullptr, nullptr};  // one Arnoldi step, [FP64, FP32C]
   int graph_pc = -1;
   std::vector<hipEvent_t> ev;
@@ -715,7 +717,7 @@ Status reserve_krylov(cfm_batch* b, int m) {
   CFM_TRY(b->iinfo.alloc((size_t)S * 8));
   CFM_TRY(b->jdev.alloc(4));
   for (auto* v : {&b->r, &b->w, &b->zt, &b->xs, &b->dv, &b->rhs}) CFM_TRY(v->alloc(L));
-  if (!b->host_flags) CFM_HIP(hipHostMalloc((void**)&b->host_flags, sizeof(int) * 4, hipHostMallocDefault));
+  if (!b->host_flags) CFM_HIP(hipHostMalloc((void**)&b->host_flags, sizeof(int) * 8, hipHostMallocDefault));
   CFM_HIP(hipStreamSynchronize(b->stream));
   drop_graphs(b);
   return {};
@@ -740,12 +742,13 @@ Status enqueue_step(cfm_batch* b, int prec, int pc, double* x) {
   hipLaunchKernelGGL(bscale_kernel, gb(b->L), dim3(256), 0, b->stream, k, b->w.p);
   hipLaunchKernelGGL(bupdate_kernel, gb(b->N), dim3(256), 0, b->stream, k, pc, wall, a, mu, b->z.p, x);
   CFM_CHECK_LAUNCH();
-  CFM_HIP(hipMemcpyAsync(b->host_flags, b->jdev.p + 2, sizeof(int), hipMemcpyDeviceToHost, b->stream));
   return {};
 }
 
 // Graph of one Arnoldi step for (prec, pc); x is the vector the step updates
 // (fixed per solve mode: the solution in FP64 mode, the correction in mixed mode).
+// The graph holds kernel nodes only: the running count the host polls is copied out
+// by run_cycle after each replay, outside the graph.
 Status step_graph(cfm_batch* b, int prec, int pc, double* x, hipGraphExec_t* out) {
   const int gi = prec == CFM_FP32C ? 1 : 0;
   if (b->graph_pc != pc) drop_graphs(b);
@@ -779,11 +782,18 @@ Status event_at(cfm_batch* b, size_t i, hipEvent_t* e) {
 }
 
 // Run Arnoldi steps (graph replays) until no system is running or the cycle is
-// full.  The running count is read one step late (pinned copy at each step's end).
+// full.  After each replay the running count is copied to its own pinned ring slot
+// (step j -> host_flags[4 + j % 4]) and read one step late, after the event behind
+// that copy: a slot is rewritten only four replays later, by which time the host has
+// read it.
 Status run_cycle(cfm_batch* b, hipGraphExec_t g, size_t* nev, int max_steps) {
   int pending = -1;
+  int pending_slot = 0;
   for (int j = 0; j < b->m && j < max_steps; ++j) {
     CFM_HIP(hipGraphLaunch(g, b->stream));
+    const int slot = 4 + (j % 4);
+    CFM_HIP(hipMemcpyAsync(b->host_flags + slot, b->jdev.p + 2, sizeof(int), hipMemcpyDeviceToHost,
+                           b->stream));
     hipEvent_t e;
     CFM_TRY(event_at(b, *nev, &e));
     CFM_HIP(hipEventRecord(e, b->stream));
@@ -792,9 +802,10 @@ Status run_cycle(cfm_batch* b, hipGraphExec_t g, size_t* nev, int max_steps) {
       hipEvent_t pe;
       CFM_TRY(event_at(b, (size_t)pending, &pe));
       CFM_HIP(hipEventSynchronize(pe));
-      if (b->host_flags[0] == 0) break;
+      if (b->host_flags[pending_slot] == 0) break;
     }
     pending = (int)mine;
+    pending_slot = slot;
   }
   CFM_HIP(hipStreamSynchronize(b->stream));
   return {};
@@ -1057,7 +1068,6 @@ int cfm_batch_gmres(cfm_batch* b, const double* rhs, double* xout, const cfm_sol
   const int max_iter = o->max_iter > 0 ? o->max_iter : 1000;
   const int pc = o->precond;
   const bool mixed = o->mode == CFM_SOLVE_MIXED;
-  const bool debug = std::getenv("CFM_BATCH_DEBUG") != nullptr;
   double elapsed = 0.0;
   auto solve = [&]() -> Status {
     CFM_TRY(reserve_krylov(b, restart));
@@ -1099,17 +1109,6 @@ int cfm_batch_gmres(cfm_batch* b, const double* rhs, double* xout, const cfm_sol
       CFM_HIP(hipMemcpyAsync(b->host_flags, b->jdev.p, sizeof(int) * 4, hipMemcpyDeviceToHost, b->stream));
       CFM_HIP(hipStreamSynchronize(b->stream));
       std::memcpy(jd, b->host_flags, sizeof(jd));
-      if (debug) {
-        std::vector<double> si((size_t)8 * b->n_sys);
-        std::vector<int> ii((size_t)8 * b->n_sys);
-        CFM_HIP(copy_to_host(b->stream, si.data(), b->sinfo.p, sizeof(double) * si.size()));
-        CFM_HIP(copy_to_host(b->stream, ii.data(), b->iinfo.p, sizeof(int) * ii.size()));
-        std::fprintf(stderr, "[batch] pass %d running %d j %d steps %d\n", pass, jd[2], jd[0], jd[1]);
-        for (int q = 0; q < b->n_sys; ++q)
-          std::fprintf(stderr, "  sys %d st %d it %d k %d nr %.3e tgt %.3e est %.3e nb %.3e\n", q,
-                       ii[8 * q], ii[8 * q + II_ITERS], ii[8 * q + II_K], si[8 * q + SI_NR],
-                       si[8 * q + SI_TARGET], si[8 * q + SI_EST], si[8 * q + SI_NB]);
-      }
       if (jd[2] == 0) break;  // nobody left running
       CFM_TRY(run_cycle(b, g, &nev, b->m));
       if (mixed) {

@@ -13,10 +13,12 @@
 // Threading: kinematics and loads use one thread per filament (the chain recursions
 // are sequential along a filament); the residual uses one thread per segment; the
 // 6 NSEG x 6 NSEG approximate Jacobian of a filament is assembled in LDS by one
-// workgroup (a thread per segment block row) and inverted in place by Gauss-Jordan
-// elimination with partial pivoting; its transpose is stored so that the inverse
-// products read HBM coalesced.
+// workgroup (a thread per segment block row) and inverted by a batched LU
+// factorisation; its transpose is stored so that the inverse products read HBM
+// coalesced.
 #include "ctx.hpp"
+#include <mutex>
+#include <vector>
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
 
@@ -788,18 +790,33 @@ Status fil_errors(cfm_ctx* c, const FilK& K, int nt, double* err, double* bad_pa
   return {};
 }
 
+// Raises the Jacobian kernel's dynamic-LDS limit once per device (with that device
+// current) to the largest block cfm_fil_init admits, after checking it against the
+// device's per-workgroup LDS; contexts on several host threads share the setting.
+static Status jacobian_lds_limit(int device) {
+  static std::mutex mu;
+  static std::vector<int> done;  // per device: 0 unset, 1 set
+  const size_t need = (size_t)(6 * kMaxFilSeg) * (6 * kMaxFilSeg) * sizeof(double);
+  std::lock_guard<std::mutex> lock(mu);
+  if ((int)done.size() <= device) done.resize(device + 1, 0);
+  if (done[device]) return {};
+  int limit = 0;
+  CFM_HIP(hipDeviceGetAttribute(&limit, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
+  if ((size_t)limit < need)
+    return make_status(CFM_EINVAL, "the %d-segment Jacobian block needs %zu bytes of LDS; the device offers %d",
+                       kMaxFilSeg, need, limit);
+  CFM_HIP(hipFuncSetAttribute((const void*)fil_jacobian_kernel,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)need));
+  done[device] = 1;
+  return {};
+}
+
 Status fil_jacobian(cfm_ctx* c, const FilK& K, int nt, int* status_dev) {
   const int N = 6 * K.nseg;
   const size_t lds = (size_t)N * N * sizeof(double);
-  if (lds > 160 * 1024)
-    return make_status(CFM_EINVAL, "approximate Jacobian of %d segments exceeds the LDS of one CU", K.nseg);
-  // the kernel's dynamic-LDS limit is raised to the largest block any context has used
-  static size_t attr_bytes = 0;
-  if (lds > attr_bytes) {
-    CFM_HIP(hipFuncSetAttribute((const void*)fil_jacobian_kernel,
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    attr_bytes = lds;
-  }
+  if (K.nseg > kMaxFilSeg)
+    return make_status(CFM_EINVAL, "filaments of %d segments exceed the limit of %d", K.nseg, kMaxFilSeg);
+  CFM_TRY(jacobian_lds_limit(c->device));
   hipLaunchKernelGGL(fil_jacobian_kernel, dim3(K.nfil), dim3(256), lds, c->stream, K, nt);
   CFM_CHECK_LAUNCH();
   // J0^-1 of every filament in place: LU with partial pivoting, then the inverse

@@ -4,6 +4,10 @@
 
 namespace cfm {
 
+// Longest filament the implicit step takes: its (6 NSEG)^2 FP64 approximate Jacobian
+// is assembled in one workgroup's LDS (127,008 bytes at 21 segments).
+constexpr int kMaxFilSeg = 21;
+
 // Kernel-side view of the filament state.  Per-segment arrays are structure of
 // arrays over the nseg_total segments (component-major, e.g. q + k*nseg_total);
 // per-filament arrays over nfil.  lam row 0 of each filament holds the tether

@@ -117,27 +117,34 @@ __global__ void copy_rows_kernel(const double* __restrict__ src, size_t src_ld, 
 }
 
 // h[i] (+)= sum_b partial[b*k+i] (every block reduces the partials in the same
-// order), then w[r] -= sum_{i<k} h_i Q[i][r]; block 0 stores h.
-__global__ __launch_bounds__(256) void cgs_finish_sub_kernel(const double* __restrict__ Q,
+// order), then w[r] -= sum_{i<k} h_i Q[i][r]; block 0 stores h.  The column passes
+// through LDS in chunks of kRB entries, so any restart length fits (the reference
+// keeps up to 350 vectors); the sum over i keeps ascending order.  Launched with
+// kRB threads per block.
+__global__ __launch_bounds__(kRB) void cgs_finish_sub_kernel(const double* __restrict__ Q,
                                                              size_t ldq, int k,
                                                              const double* __restrict__ partial,
                                                              int nb, double* __restrict__ h_out,
                                                              const double* __restrict__ h_prev,
                                                              double* __restrict__ w, int L) {
-  __shared__ double hs[256];
-  for (int i = threadIdx.x; i < k; i += blockDim.x) {
-    double s = 0.0;
-    for (int b = 0; b < nb; ++b) s += partial[(size_t)b * k + i];
-    hs[i] = s;
-  }
-  __syncthreads();
-  if (blockIdx.x == 0)
-    for (int i = threadIdx.x; i < k; i += blockDim.x) h_out[i] = h_prev ? h_prev[i] + hs[i] : hs[i];
+  __shared__ double hs[kRB];
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= L) return;
   double s = 0.0;
-  for (int i = 0; i < k; ++i) s += hs[i] * Q[(size_t)i * ldq + r];
-  w[r] -= s;
+  for (int c0 = 0; c0 < k; c0 += kRB) {
+    const int cn = min(kRB, k - c0);
+    __syncthreads();  // the previous chunk has been read by every thread
+    if ((int)threadIdx.x < cn) {
+      const int i = c0 + (int)threadIdx.x;
+      double a = 0.0;
+      for (int b = 0; b < nb; ++b) a += partial[(size_t)b * k + i];
+      hs[threadIdx.x] = a;
+      if (blockIdx.x == 0) h_out[i] = h_prev ? h_prev[i] + a : a;
+    }
+    __syncthreads();
+    if (r < L)
+      for (int i = 0; i < cn; ++i) s += hs[i] * Q[(size_t)(c0 + i) * ldq + r];
+  }
+  if (r < L) w[r] -= s;
 }
 
 // Finishes ||w|| from its partials, then one Givens step on column j (below).
@@ -237,8 +244,8 @@ __global__ void backsub_kernel(const double* __restrict__ H, int ldh,
 }
 
 __global__ void beta_init_kernel(double* __restrict__ beta, int n, const double* __restrict__ b0) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) beta[i] = (i == 0) ? b0[0] : 0.0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    beta[i] = (i == 0) ? b0[0] : 0.0;
 }
 
 // ---- preconditioners --------------------------------------------------------

@@ -460,8 +460,9 @@ int cfm_fil_init(cfm_ctx* c, const cfm_fil_params* prm, const double* q_seg, con
   if (!c->have_positions) return fail_c(c, make_status(CFM_ESTATE, "positions not set"));
   Particles& p = c->p;
   if (p.seg_dof != 6) return fail_c(c, make_status(CFM_EINVAL, "filaments need seg_dof = 6"));
-  if (prm->nseg < 2 || prm->nseg > 21 || p.n_seg % prm->nseg != 0)
-    return fail_c(c, make_status(CFM_EINVAL, "nseg must be in [2, 21] and divide n_seg (%d)", p.n_seg));
+  if (prm->nseg < 2 || prm->nseg > kMaxFilSeg || p.n_seg % prm->nseg != 0)
+    return fail_c(c, make_status(CFM_EINVAL, "nseg must be in [2, %d] and divide n_seg (%d)", kMaxFilSeg,
+                                 p.n_seg));
   if (prm->max_broyden < 1 || !(prm->dt > 0) || !(prm->dl > 0) || !(prm->tol > 0))
     return fail_c(c, make_status(CFM_EINVAL, "need max_broyden >= 1 and dt, dl, tol > 0"));
   if (prm->pair_precision != CFM_FP64 && prm->pair_precision != CFM_FP32C)

@@ -1,4 +1,5 @@
-// probe.hip -- FP32 VALU issue-rate probes for gfx950.
+// valu_probe.hip -- FP32 VALU issue-rate probes for gfx950 (measurement only; built
+// into libcfm_probe.so, not into the mobility library).
 //
 // Every device build disables packed FP32 instructions, so the ceiling for the FP32
 // pair kernel is the rate at which the GPU issues ordinary FP32 VALU instructions.
@@ -7,9 +8,9 @@
 // store per thread.  The result is lane-instructions per second over the whole GPU.
 #include <hip/hip_runtime.h>
 
-#include "ctx.hpp"
+#include "../../include/cfm_probe.h"
 
-namespace cfm {
+namespace cfm_probe {
 namespace {
 
 constexpr int kProbeBlocks = 256 * 32, kProbeThreads = 256, kProbeIters = 4096;
@@ -96,17 +97,17 @@ __global__ void probe_fma3_kernel(float* out, int iters, float a) {
 }
 
 }  // namespace
-}  // namespace cfm
+}  // namespace cfm_probe
 
 extern "C" int cfm_probe_valu_rate(int32_t device, int32_t form, double* lane_instr_per_s) {
-  using namespace cfm;
-  if (!lane_instr_per_s || form < 0 || form > 4) return CFM_EINVAL;
+  using namespace cfm_probe;
+  if (!lane_instr_per_s || form < 0 || form > 4) return CFM_PROBE_EINVAL;
   *lane_instr_per_s = 0.0;
-  if (hipSetDevice(device) != hipSuccess) return CFM_EHIP;
+  if (hipSetDevice(device) != hipSuccess) return CFM_PROBE_EHIP;
   float* out = nullptr;
   hipStream_t st = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
-  int rc = CFM_EHIP;
+  int rc = CFM_PROBE_EHIP;
   const size_t n = (size_t)kProbeBlocks * kProbeThreads;
   const double per_iter = 16.0;  // VALU instructions per loop iteration per lane
   if (hipMalloc(&out, sizeof(float) * n) == hipSuccess &&
@@ -131,7 +132,7 @@ extern "C" int cfm_probe_valu_rate(int32_t device, int32_t form, double* lane_in
     }
     if (ok && best > 0.0f) {
       *lane_instr_per_s = (double)n * kProbeIters * per_iter / (best * 1e-3);
-      rc = CFM_OK;
+      rc = CFM_PROBE_OK;
     }
   }
   if (e0) (void)hipEventDestroy(e0);

@@ -512,16 +512,6 @@ int cfm_batch_tune(cfm_batch* b, int32_t precision, cfm_batch_tuning* chosen,
 int cfm_batch_gmres(cfm_batch* b, const double* rhs, double* x /* in: guess if use_guess */,
                     const cfm_solve_opts* opts, cfm_batch_sys_stats* per_sys, double* total_ms);
 
-/* ---- measurement helper (no counterpart in the reference) ----------------------------
- * Issue rate of one FP32 VALU instruction form on `device`, in lane-instructions per
- * second over the whole GPU: the ceiling against which the FP32 pair kernel's rate
- * (its VALU instructions per pair x pairs per second) is judged, since every device
- * build disables packed FP32 instructions.  form: 0 v_fmac_f32 (two-operand, VGPR
- * sources), 1 v_mul_f32 with a scalar operand, 2 v_add_f32 with a scalar operand,
- * 3 v_mul_f32 with VGPR operands, 4 v_fma_f32 (three-operand) with VGPR operands.
- * Allocates and frees its own 8 MiB buffer and stream. */
-int cfm_probe_valu_rate(int32_t device, int32_t form, double* lane_instr_per_s);
-
 #ifdef __cplusplus
 }
 #endif

@@ -23,7 +23,11 @@ Parity status: pinned by the identities in tests/test_filament_oracle.py (dexp o
 dexpinv = identity, lie_exp vs Rodrigues, finite-difference checks of the
 elastic-moment derivative, J_approx against a finite-difference Jacobian of the
 BDF residual in the free-draining limit, zero internal force for a straight
-unstressed filament, Newton-Euler balance of the internal loads).
+unstressed filament, Newton-Euler balance of the internal loads) and in
+tests/test_oracle_step_pins.py (the BDF2 / Euler residual vanishes on exactly
+integrated motion and is third order on a general rotation; the barrier's known
+values at chosen separations and heights, its cut-offs, exclusions and pairwise
+balance; planted coefficient errors in this module fail those checks).
 """
 from __future__ import annotations
 
@@ -33,6 +37,13 @@ import numpy as np
 
 PI = math.pi
 SMALL_ANGLE = 1e-5
+
+# BDF2 residual coefficients (mobility_solver.cpp:2767-2769): x + C1 x_m1 + C2 x_m2 + C3 dt v
+BDF2_C1, BDF2_C2, BDF2_C3 = -4.0 / 3.0, 1.0 / 3.0, -2.0 / 3.0
+# barrier (cuda_functions.cu:1856-1903): pairs closer than BARRIER_RANGE (a_i + a_j) repel
+# with magnitude REPULSIVE_FORCE_FACTOR END_FORCE min(1, 1 - chi (d - a_sum))^BARRIER_POWER,
+# chi = 1 / ((BARRIER_RANGE - 1) a_sum) = 10 / a_sum; the wall pushes below BASE_HEIGHT (0.5 DL)
+BARRIER_RANGE, BARRIER_POWER = 1.1, 4
 
 
 # ---------------------------------------------------------------- quaternions
@@ -298,7 +309,7 @@ class Filament:
         P = self.P
         n = P.nseg
         e = np.zeros(6 * n)
-        c1, c2, c3 = -4.0 / 3.0, 1.0 / 3.0, -2.0 / 3.0
+        c1, c2, c3 = BDF2_C1, BDF2_C2, BDF2_C3
         for j in range(n):
             if nt < P.num_euler_steps:
                 e[3 * j:3 * j + 3] = self.x[j] - self.xm1[j] - P.dt * v[j, :3]
@@ -450,16 +461,17 @@ def barrier_forces(x_seg, nseg, P: Params, x_blob=None, wall=True):
         fi = i // nseg
         if wall and xs[i, 2] < 0.5 * P.dl:
             fz = min(1.0, 1.0 - (xs[i, 2] - P.a) / (0.5 * P.dl - P.a))
-            F[i, 2] += fmag * fz ** 4
+            F[i, 2] += fmag * fz ** BARRIER_POWER
         d = xs[i] - allx
         dist = np.sqrt((d * d).sum(1))
         for j in range(allx.shape[0]):
             a_sum = 2.0 * P.a
             if j < ns and fi == j // nseg and abs(i - j) <= 1:
                 continue
-            if dist[j] < 1.1 * a_sum:
-                fac = min(1.0, 1.0 - (10.0 / a_sum) * (dist[j] - a_sum))
-                F[i] += fmag * fac ** 4 * d[j] / dist[j]
+            if dist[j] < BARRIER_RANGE * a_sum:
+                chi = 1.0 / ((BARRIER_RANGE - 1.0) * a_sum)
+                fac = min(1.0, 1.0 - chi * (dist[j] - a_sum))
+                F[i] += fmag * fac ** BARRIER_POWER * d[j] / dist[j]
     return F
 
 

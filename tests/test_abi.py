@@ -22,6 +22,16 @@ def test_library_exports_every_header_symbol():
     assert lib.cfm_abi_version() == 1
 
 
+def test_probe_library_exports_its_header_and_stays_out_of_the_product():
+    psyms = L.header_symbols(L.PROBE_HEADER)
+    assert psyms == ["cfm_probe_valu_rate"]
+    assert hasattr(L.probe_lib(), "cfm_probe_valu_rate")
+    out = subprocess.run(["nm", "-D", "--defined-only", L.LIB_PATH], capture_output=True,
+                         text=True, check=True).stdout
+    assert "cfm_probe_valu_rate" not in out
+    assert not set(psyms) & set(L.header_symbols())
+
+
 def _cfg(**kw):
     base = dict(n_seg=40, n_blob=0, seg_dof=6, wall=1, a_seg=1.0, a_blob=1.0, mu=1.0,
                 precision=0, device=0, tile_split=0, reserved=0)

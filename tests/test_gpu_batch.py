@@ -109,3 +109,30 @@ def test_autotuner_picks_a_tried_candidate():
         assert (best["max_sources"], best["grid"]) in {(t["max_sources"], t["grid"]) for t in tried}
         assert B.tuning()["grid"] == best["grid"]
         assert np.abs(B.apply(f, "fp32c") - ref).max() <= 1e-6
+
+
+def test_sixty_four_systems_of_uneven_size():
+    """The benchmarked ensemble: 64 independent wall systems of 100-4,000 segments
+    (121,460 particles), every resistance problem solved together to 1e-10 in mixed
+    precision.  Every system's reported FP64 residual is at most 1e-10, and the oracle's
+    own product confirms it on eight systems (the smallest, the largest and six drawn
+    at random; 1e-13 covers the rounding difference of the two FP64 products)."""
+    sizes, x, ens = _ensemble(64, 100, 4000, seed=3)
+    assert len(sizes) == 64 and min(sizes) <= 300 and max(sizes) >= 3800
+    rhs = geo.loads(sum(sizes), 6, 1)
+    with BatchedMobility(sizes) as B:
+        B.set_positions(x)
+        B.tune("fp32c")
+        sol, stats, ms = B.solve(rhs, rtol=1e-10, precond="self", mode="mixed")
+    assert ms > 0.0
+    for st in stats:
+        assert st["converged"] and st["rel_residual"] <= 1e-10, st
+        assert 0.0 < st["latency_ms"] <= ms + 1e-9
+    rng = np.random.default_rng(0)
+    pick = {int(np.argmin(sizes)), int(np.argmax(sizes))} | set(rng.choice(64, 6, replace=False).tolist())
+    bounds = np.cumsum([0] + [6 * n for n in sizes])
+    for s in sorted(pick):
+        n, xs = ens[s]
+        b, xk = rhs[bounds[s]:bounds[s + 1]], sol[bounds[s]:bounds[s + 1]]
+        res = np.linalg.norm(b - oracle.apply(xs, 6, xs, 6, xk)) / np.linalg.norm(b)
+        assert res <= 1e-10 + 1e-13, (s, n, res)

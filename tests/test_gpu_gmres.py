@@ -15,6 +15,13 @@ def _oracle_rel_res(x_pos, dof, sol, rhs):
     return np.linalg.norm(rhs - oracle.apply(x_pos, dof, x_pos, dof, sol)) / np.linalg.norm(rhs)
 
 
+# Solves requested at rtol 1e-10 are checked against the oracle's own FP64 residual at
+# 1e-10 itself.  The resident-inverse path may stop on its FP64 recurrence estimate once
+# it is within 0.9 of the target (no verifying product); the 10% margin is the rounding
+# allowance between that estimate and the true residual.
+RTOL = 1e-10
+
+
 @pytest.mark.parametrize("mode", ["fp64", "mixed"])
 @pytest.mark.parametrize("precond", ["diag", "self"])
 def test_small_system_matches_dense_solve(mode, precond):
@@ -63,7 +70,7 @@ def test_blob_system_and_guess_rule():
         m.set_positions(xs, xb)
         f, st = m.solve("blob", rhs, rtol=1e-10, precond="diag", mode="fp64")
         assert st["converged"]
-        assert _oracle_rel_res(xb, 3, f, rhs) < 1e-9
+        assert _oracle_rel_res(xb, 3, f, rhs) <= RTOL
         # a good guess is kept and converges immediately; a bad one is rejected
         f2, st2 = m.solve("blob", rhs, x0=f, rtol=1e-9)
         assert st2["iterations"] == 0 and st2["initial_rel_residual"] < 1e-9
@@ -115,7 +122,7 @@ def test_nearfield_preconditioner_cuts_iterations():
         m.set_positions(xs, xb)
         _, sd = m.solve("blob", rhs, rtol=1e-10, precond="diag", max_iter=3000)
         f, sn = m.solve("blob", rhs, rtol=1e-10, precond="near", max_iter=3000)
-    assert sn["converged"] and _oracle_rel_res(xb, 3, f, rhs) < 1e-9
+    assert sn["converged"] and _oracle_rel_res(xb, 3, f, rhs) <= RTOL
     assert sn["iterations"] * 2 < sd["iterations"], (sn, sd)
 
 
@@ -145,7 +152,7 @@ def test_dense_preconditioner_solves_in_few_iterations(mode):
         m.set_positions(xs, xb)
         f, st = m.solve("blob", rhs, rtol=1e-10, precond="dense", mode=mode, max_iter=200)
         _, sn = m.solve("blob", rhs, rtol=1e-10, precond="near", mode=mode, max_iter=3000)
-    assert st["converged"] and _oracle_rel_res(xb, 3, f, rhs) < 1e-9
+    assert st["converged"] and _oracle_rel_res(xb, 3, f, rhs) <= RTOL
     assert st["iterations"] <= 10 and st["iterations"] * 3 < sn["iterations"], (st, sn)
 
 
@@ -189,3 +196,25 @@ def test_iteration_cap_reports_non_convergence(mode):
     rel = _oracle_rel_res(xs, 6, x, b)
     assert 1e-14 < st["rel_residual"] < 1.0
     assert abs(rel - st["rel_residual"]) <= 1e-9 + 1e-6 * rel, (rel, st["rel_residual"])
+
+
+def test_restart_beyond_256_vectors():
+    """One Krylov cycle of 300 vectors (the reference keeps up to 350,
+    mobility_solver.cpp:151-170, config.hpp:366): the Hessenberg column passes through
+    LDS in chunks, and the 300-step iterate equals the reference's unrestarted MGS
+    GMRES iterate after 300 steps (oracle products, no preconditioner)."""
+    xs = geo.tilted_lattice(nx=4, ny=4, seed=2)               # 320 segments, 1,920 unknowns
+    n = xs.size // 3
+    b = geo.loads(n, 6, 17)
+    with RpyMobilitySolver(n, 0) as m:
+        m.set_positions(xs)
+        x, st = m.solve("seg", b, rtol=1e-300, restart=300, max_iter=300, precond="none",
+                        mode="fp64", raise_on_noconv=False)
+    assert st["iterations"] == 300, st
+    ref, it_ref, res_ref = oracle.gmres_reference(lambda y: oracle.apply(xs, 6, xs, 6, y),
+                                                  lambda r: r, b, tol=1e-300, max_iter=300)
+    assert it_ref == 300
+    rel = _oracle_rel_res(xs, 6, x, b)
+    assert abs(rel - st["rel_residual"]) <= 1e-9 + 1e-6 * rel, (rel, st["rel_residual"])
+    assert rel <= 2.0 * res_ref + 1e-14, (rel, res_ref)
+    assert np.abs(x - ref).max() <= 1e-6 * np.abs(ref).max()
