@@ -215,6 +215,7 @@ def lib():
     L.cfm_set_precision.argtypes = [_vp, ctypes.c_int32]
     L.cfm_apply.argtypes = [_vp, ctypes.c_int32, _dp, _dp, _dp, _dp]
     L.cfm_time_apply.argtypes = [_vp, ctypes.c_int32, ctypes.c_int32, _dp]
+    L.cfm_apply_noalloc.argtypes = [_vp, ctypes.c_int32, _vp, _vp, _vp, _vp]
     L.cfm_gmres.argtypes = [_vp, ctypes.POINTER(cfm_solve_opts), _dp, _dp,
                             ctypes.POINTER(cfm_solve_stats)]
     L.cfm_apply_preconditioner.argtypes = [_vp, ctypes.c_int32, ctypes.c_int32, _dp, _dp]

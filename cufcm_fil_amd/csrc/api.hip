@@ -19,6 +19,7 @@ namespace cfm {
 
 Status launch_apply(cfm_ctx* c, const ApplyPlan& plan);
 int plan_tiles(const cfm_ctx* c, const ApplyPlan& plan);
+Status noalloc_scratch(int device, double** ptr);
 
 // AoS [dof*cnt] (stage) -> SoA rows k of dst (dst + k*ld + off), k < dof.
 __global__ void aos_to_soa_kernel(const double* __restrict__ src, double* __restrict__ dst,
@@ -131,7 +132,7 @@ ApplyPlan plan_for(const cfm_ctx* c, int which) {
 // 256 CUs (about 4 per CU), bounded by the number of 256-source tiles.
 Status tune_split(cfm_ctx* c, const ApplyPlan& pl, int heuristic, int* best);
 
-static int heuristic_split(const cfm_ctx* c, const ApplyPlan& pl) {
+int heuristic_split(const cfm_ctx* c, const ApplyPlan& pl) {
   const int tiles = std::max(1, plan_tiles(c, pl));
   int nsrc = 0;
   if (pl.src_seg) nsrc += c->p.n_seg;
@@ -248,6 +249,8 @@ int cfm_create(const cfm_config* cfg, cfm_ctx** out) {
     CFM_HIP(hipMemsetAsync(p.f.p, 0, sizeof(double) * 6 * p.n, c->stream));
     CFM_HIP(hipMemsetAsync(p.v.p, 0, sizeof(double) * 6 * p.n, c->stream));
     CFM_HIP(hipStreamSynchronize(c->stream));
+    // this device's address of the allocation-free entry point's scratch array
+    CFM_TRY(noalloc_scratch(c->device, &c->scratch));
     return {};
   };
   Status s = init();

@@ -335,9 +335,13 @@ Status launch_apply(cfm_ctx* c, const ApplyPlan& plan) {
   if (plan.side && !c->side) return make_status(CFM_ESTATE, "side stream launch before side_fork");
   const hipStream_t stream = plan.side ? c->side : c->stream;
   if (split > 1) {
-    DevBuf<double>& part = plan.side ? c->partial_side : p.partial;
-    CFM_TRY(part.alloc((size_t)split * 6 * p.n));
-    a.partial = part.p;
+    if (plan.partial_ext) {
+      a.partial = plan.partial_ext;
+    } else {
+      DevBuf<double>& part = plan.side ? c->partial_side : p.partial;
+      CFM_TRY(part.alloc((size_t)split * 6 * p.n));
+      a.partial = part.p;
+    }
   }
   const bool fp32 = c->cfg.precision == CFM_FP32C;
   const bool wall = c->cfg.wall != 0;

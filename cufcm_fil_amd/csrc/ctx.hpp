@@ -152,6 +152,9 @@ struct ApplyPlan {
   // Launch on the context's side stream (with its own split partials), to overlap
   // with work queued on the main stream; the caller orders the two with events.
   bool side = false;
+  // Split partials in device memory the caller provides (split x 6 x n doubles): the
+  // launch then allocates nothing.  Null: the context's own buffer, grown on demand.
+  double* partial_ext = nullptr;
 };
 
 }  // namespace cfm
@@ -198,5 +201,7 @@ struct cfm_ctx {
   void* gnn = nullptr;                  // learned initial guess (gnn.hip)
   void* cilia = nullptr;                // prescribed-shape cilia (cilia.hip)
   void* body = nullptr;                 // free rigid bodies made of the blobs (body.hip)
+  double* scratch = nullptr;            // this device's copy of the allocation-free entry
+                                        // point's static scratch array (noalloc.hip)
   std::string err;
 };

@@ -94,6 +94,22 @@ int cfm_apply(cfm_ctx* ctx, int32_t which, const double* f_seg, const double* f_
  * mean milliseconds per launch.  (Measurement helper; no reference counterpart.) */
 int cfm_time_apply(cfm_ctx* ctx, int32_t which, int32_t reps, double* ms_per_apply);
 
+/* Allocation-free variant of cfm_apply.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  f_seg, f_blob, v_seg, v_blob are
+ * DEVICE arrays in cfm_apply's layout and write/accumulate rules; positions come from
+ * cfm_set_positions as for cfm_apply.  The pair kernel's split partial sums go to one
+ * static device array of CFM_NOALLOC_SCRATCH_BYTES per GPU (held until the process
+ * ends) instead of context memory; the split is the context's fixed tile_split when
+ * > 0 -- rejected with CFM_EINVAL when split x 6 x (n_seg + n_blob) doubles exceed the
+ * array -- and otherwise the size heuristic, lowered to fit.  The call allocates
+ * nothing and returns after the context's stream has finished.  Every call on a GPU
+ * shares that GPU's array: calls that overlap on one GPU (other contexts, other host
+ * threads) race and must be serialised by the caller. */
+#define CFM_NOALLOC_SCRATCH_BYTES (32u << 20)
+int cfm_apply_noalloc(cfm_ctx* ctx, int32_t which, const double* f_seg, const double* f_blob,
+                      double* v_seg, double* v_blob);
+
 
 /* ---- linear solves (GMRES) ------------------------------------------------ */
 
