@@ -495,6 +495,9 @@ def main():
     ap.add_argument("--skip-ciliated", action="store_true")
     ap.add_argument("--skip-ensemble", action="store_true")
     ap.add_argument("--skip-overlap", action="store_true")
+    ap.add_argument("--only", choices=["batched", "learned", "cilia", "body", "swimmer", "ciliated",
+                                       "ensemble"],
+                    help="run one side section alone and print its JSON (for profiling)")
     ap.add_argument("--batch-reps", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--precond", default="dense", choices=["near", "dense"],
@@ -537,6 +540,12 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    if args.only:
+        fn = {"batched": run_batched, "learned": run_learned, "cilia": run_cilia, "body": run_free_body,
+              "swimmer": run_free_swimmer, "ciliated": run_ciliated_swimmer,
+              "ensemble": run_orbit_ensemble}[args.only]
+        print(json.dumps({args.only: fn(args)}), flush=True)
+        return
     guess = "learned" if args.learned else "projected"
     mixed = run_path(args, "mixed", "fp32c", args.mode, guess, args.precond)
     plain = None if args.skip_plain else run_path(args, "plain", "fp64", "fp64", "previous", "diag")
