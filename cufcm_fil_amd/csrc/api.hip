@@ -271,8 +271,6 @@ int cfm_destroy(cfm_ctx* c) {
   cfm_body_release(c);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   if (c->rocblas) (void)rocblas_destroy_handle((rocblas_handle)c->rocblas);
-  c->lu_piv.release();
-  c->lu_info.release();
   c->tune_v.release();
   c->p.x.release();
   c->p.y.release();

@@ -191,7 +191,6 @@ struct cfm_ctx {
   // autotuned source splits (tile_split = -1), keyed on the product's shape
   std::map<std::array<long long, 6>, int> split_cache;
   cfm::DevBuf<double> tune_v;           // scratch output of the tuning launches
-  cfm::DevBuf<int> lu_piv, lu_info;     // batched LU pivots / status (filament Jacobians)
   // pair-kernel profiling (HIP events around each launch)
   bool prof_on = false;
   std::vector<hipEvent_t> prof_events;  // pool, two per launch

@@ -13,15 +13,13 @@
 // Threading: kinematics and loads use one thread per filament (the chain recursions
 // are sequential along a filament); the residual uses one thread per segment; the
 // 6 NSEG x 6 NSEG approximate Jacobian of a filament is assembled in LDS by one
-// workgroup (a thread per segment block row) and inverted by a batched LU
-// factorisation; its transpose is stored so that the inverse products read HBM
-// coalesced.
+// workgroup (a thread per segment block row) and inverted by Gauss-Jordan elimination
+// held in registers (two lanes per row); its transpose is stored so that the inverse
+// products read HBM coalesced.
 #include "ctx.hpp"
+#include <cfloat>
 #include <mutex>
 #include <vector>
-#include <rocblas/rocblas.h>
-#include <rocsolver/rocsolver.h>
-
 #include "fil.hpp"
 #include "fil_device.hpp"
 
@@ -332,8 +330,8 @@ __device__ __forceinline__ M3 frame_sum(V3 Kv, Q4 q) {
 }
 
 // Approximate Jacobian of one filament (filament.cpp:1855-2037 blocks, swimmer.cpp:
-// 539-560 assembly), built in LDS and stored column-major in K.jinv; the batched LU
-// inversion (fil_jacobian) then overwrites it with the inverse.
+// 539-560 assembly), built in LDS and stored row-major in K.jinv; the in-register
+// inversion (fil_jacobian_invert_kernel) then overwrites it with the inverse.
 __global__ __launch_bounds__(256) void fil_jacobian_kernel(FilK K, int nt) {
   extern __shared__ double J[];
   const int f = blockIdx.x;
@@ -442,17 +440,130 @@ __global__ __launch_bounds__(256) void fil_jacobian_kernel(FilK K, int nt) {
     if (k <= n - 2) put_block(J, N, r0, 3 * (n + k + 1), Rb);
   }
   __syncthreads();
-  // column-major copy (element (r, c) at c N + r) for the batched LU inversion
+  // row-major copy (element (r, c) at r N + c) for the in-register inversion
   double* out = K.jinv + (size_t)f * N * N;
-  for (int e = tid; e < N * N; e += nth) {
-    const int c = e / N, r = e - c * N;
-    out[e] = J[r * N + c];
-  }
+  for (int e = tid; e < N * N; e += nth) out[e] = J[e];
 }
 
-__global__ void info_any_kernel(const int* __restrict__ info, int n, int* __restrict__ status) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n && info[i] != 0) status[0] = 1;
+// ---- in-register Gauss-Jordan inversion of the approximate Jacobians ------------------
+// One workgroup of 256 threads per filament; row r of the N x N block (N = 6 NSEG <= 128)
+// belongs to threads 2r (columns 0..63) and 2r + 1 (columns 64..127), each holding its
+// 64 entries in registers.  Step k picks the pivot of column k among the rows not yet
+// used (largest magnitude, lowest row on ties), the pivot row is scaled and broadcast
+// through LDS, and every other row eliminates column k; rows are never moved.  After the
+// N steps the row that pivoted column k holds row k of (P A)^-1 = A^-1 P^T, i.e.
+// A^-1[k][p_j] in its column j (p_j the pivot row of column j), which is stored as the
+// transposed inverse (element (r, c) of A^-1 at c N + r).  This is the in-place
+// Gauss-Jordan elimination with partial pivoting; the reference inverts each block with
+// LAPACK getrf/getri (matrix.cpp:538-571), the same inverse up to rounding.
+constexpr int kGjHalf = 64;
+
+// Register entry c of a thread's half-row, c uniform across the wave (a scalar branch).
+#define CFM_GJ_CASES(X)                                                                          \
+  X(0) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16)   \
+  X(17) X(18) X(19) X(20) X(21) X(22) X(23) X(24) X(25) X(26) X(27) X(28) X(29) X(30) X(31)     \
+  X(32) X(33) X(34) X(35) X(36) X(37) X(38) X(39) X(40) X(41) X(42) X(43) X(44) X(45) X(46)     \
+  X(47) X(48) X(49) X(50) X(51) X(52) X(53) X(54) X(55) X(56) X(57) X(58) X(59) X(60) X(61)     \
+  X(62) X(63)
+__device__ __forceinline__ double gj_get(const double (&a)[kGjHalf], int c) {
+  switch (c) {
+#define CFM_GJ_GET(i) case i: return a[i];
+    CFM_GJ_CASES(CFM_GJ_GET)
+#undef CFM_GJ_GET
+  }
+  return 0.0;
+}
+__device__ __forceinline__ void gj_set(double (&a)[kGjHalf], int c, double v) {
+  switch (c) {
+#define CFM_GJ_SET(i) case i: a[i] = v; break;
+    CFM_GJ_CASES(CFM_GJ_SET)
+#undef CFM_GJ_SET
+  }
+}
+#undef CFM_GJ_CASES
+
+__global__ __launch_bounds__(256) void fil_jacobian_invert_kernel(FilK K, int* __restrict__ status) {
+  __shared__ double prow[2 * kGjHalf];
+  __shared__ double rval[4];
+  __shared__ int rrow[4];
+  __shared__ int perm[2 * kGjHalf];
+  const int f = blockIdx.x;
+  const int N = 6 * K.nseg;
+  const int tid = threadIdx.x;
+  const int r = tid >> 1, h = tid & 1;
+  const bool row_ok = r < N;
+  double* Jf = K.jinv + (size_t)f * N * N;
+  double a[kGjHalf];
+#pragma unroll
+  for (int c = 0; c < kGjHalf; ++c) {
+    const int col = h * kGjHalf + c;
+    a[c] = (row_ok && col < N) ? Jf[(size_t)r * N + col] : 0.0;
+  }
+  bool used = !row_ok;
+  int my_k = 0;
+  bool bad = false;
+  for (int k = 0; k < N; ++k) {
+    const int hk = k >> 6, ck = k & (kGjHalf - 1);
+    // column k of this row (held by the lane of half hk)
+    const double mine = gj_get(a, ck);
+    const double other = __shfl_xor(mine, 1, 64);
+    const double bk = (h == hk) ? mine : other;
+    // pivot: largest |B[r][k]| over unused rows, the lowest row on ties; a non-finite
+    // entry ranks above every finite one (and flags the block), so the pivot is always
+    // an unused row and the pivots form a permutation of the rows
+    double v = used ? -1.0 : (isfinite(bk) ? fabs(bk) : DBL_MAX);
+    int vr = r;
+    for (int off = 32; off > 0; off >>= 1) {
+      const double ov = __shfl_xor(v, off, 64);
+      const int orr = __shfl_xor(vr, off, 64);
+      if (ov > v || (ov == v && orr < vr)) {
+        v = ov;
+        vr = orr;
+      }
+    }
+    if ((tid & 63) == 0) {
+      rval[tid >> 6] = v;
+      rrow[tid >> 6] = vr;
+    }
+    __syncthreads();
+    double pv = rval[0];
+    int p = rrow[0];
+    for (int w = 1; w < 4; ++w)
+      if (rval[w] > pv || (rval[w] == pv && rrow[w] < p)) {
+        pv = rval[w];
+        p = rrow[w];
+      }
+    if (!(pv > 0.0) || pv == DBL_MAX) bad = true;
+    if (r == p) {
+      // pivot row: B[p][k] = 1, then the row scaled by 1 / piv
+      const double inv = 1.0 / bk;
+      if (h == hk) gj_set(a, ck, 1.0);
+#pragma unroll
+      for (int c = 0; c < kGjHalf; ++c) {
+        a[c] *= inv;
+        prow[h * kGjHalf + c] = a[c];
+      }
+      used = true;
+      my_k = k;
+      if (h == 0) perm[k] = r;
+    }
+    __syncthreads();
+    if (r != p && row_ok) {
+      // B[r][k] = 0, then B[r][:] -= B[r][k] (old) x pivot row
+      if (h == hk) gj_set(a, ck, 0.0);
+#pragma unroll
+      for (int c = 0; c < kGjHalf; ++c) a[c] = fma(-bk, prow[h * kGjHalf + c], a[c]);
+    }
+  }
+  if (bad && tid == 0) status[0] = 1;
+  __syncthreads();  // perm complete; every row read its block before any store below
+  if (row_ok) {
+#pragma unroll
+    for (int c = 0; c < kGjHalf; ++c) {
+      const int col = h * kGjHalf + c;
+      if (col < N && perm[col] < N) Jf[(size_t)perm[col] * N + my_k] = a[c];
+    }
+  }
 }
 
 // out_f = scale * J0^-1 in_f for every filament (vectors in the reference layout)
@@ -819,24 +930,9 @@ Status fil_jacobian(cfm_ctx* c, const FilK& K, int nt, int* status_dev) {
   CFM_TRY(jacobian_lds_limit(c->device));
   hipLaunchKernelGGL(fil_jacobian_kernel, dim3(K.nfil), dim3(256), lds, c->stream, K, nt);
   CFM_CHECK_LAUNCH();
-  // J0^-1 of every filament in place: LU with partial pivoting, then the inverse
-  // from the factors (the reference inverts each block with LAPACK getrf/getri,
-  // matrix.cpp:538-571)
-  rocblas_handle h = nullptr;
-  CFM_TRY(ctx_rocblas(c, (void**)&h));
-  CFM_TRY(c->lu_piv.alloc((size_t)K.nfil * N));
-  CFM_TRY(c->lu_info.alloc((size_t)K.nfil));
-  const rocblas_stride sA = (rocblas_stride)N * N, sP = N;
-  if (rocsolver_dgetrf_strided_batched(h, N, N, K.jinv, N, sA, c->lu_piv.p, sP, c->lu_info.p,
-                                       K.nfil) != rocblas_status_success)
-    return make_status(CFM_EHIP, "rocsolver_dgetrf_strided_batched failed");
-  hipLaunchKernelGGL(info_any_kernel, gsz(K.nfil, 256), dim3(256), 0, c->stream, c->lu_info.p,
-                     K.nfil, status_dev);
-  if (rocsolver_dgetri_strided_batched(h, N, K.jinv, N, sA, c->lu_piv.p, sP, c->lu_info.p,
-                                       K.nfil) != rocblas_status_success)
-    return make_status(CFM_EHIP, "rocsolver_dgetri_strided_batched failed");
-  hipLaunchKernelGGL(info_any_kernel, gsz(K.nfil, 256), dim3(256), 0, c->stream, c->lu_info.p,
-                     K.nfil, status_dev);
+  // J0^-1 of every filament in place (the reference inverts each block with LAPACK
+  // getrf/getri, matrix.cpp:538-571)
+  hipLaunchKernelGGL(fil_jacobian_invert_kernel, dim3(K.nfil), dim3(256), 0, c->stream, K, status_dev);
   CFM_CHECK_LAUNCH();
   return {};
 }
