@@ -24,6 +24,7 @@ CFM_SS, CFM_SB, CFM_BB, CFM_BS, CFM_FULL = range(5)
 CFM_SYS_SEG, CFM_SYS_BLOB, CFM_SYS_BODY = 0, 1, 2
 CFM_PC_NONE, CFM_PC_DIAG, CFM_PC_SELF, CFM_PC_NEAR, CFM_PC_DENSE = 0, 1, 2, 3, 4
 CFM_SOLVE_FP64, CFM_SOLVE_MIXED = 0, 1
+CFM_COARSE_NONE, CFM_COARSE_LINEAR, CFM_COARSE_LEARNED = 0, 1, 2
 
 
 class CfmError(RuntimeError):
@@ -254,6 +255,10 @@ def lib():
     L.cfm_gnn_gradient.argtypes = [_vp, ctypes.c_int32, ctypes.c_int32, _fp, _dp]
     L.cfm_gemm_bf16.argtypes = [_vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _fp,
                                 ctypes.c_int32, _fp, ctypes.c_int32, _fp]
+    L.cfm_nearfield_coarse.argtypes = [_vp, ctypes.c_int32]
+    L.cfm_gnn_train_coarse.argtypes = [_vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_double, _dp, _dp]
+    L.cfm_gnn_coarse_gradient.argtypes = [_vp, ctypes.c_int32, ctypes.c_int32, _fp, _dp, _dp]
+    L.cfm_gnn_coarse_vectors.argtypes = [_vp, _dp]
     L.cfm_cilia_init.argtypes = [_vp, ctypes.POINTER(cfm_cilia_params), _dp, _dp, _dp]
     L.cfm_cilia_step.argtypes = [_vp, ctypes.c_int32, _dp, _dp, ctypes.POINTER(cfm_solve_stats)]
     L.cfm_cilia_get_state.argtypes = [_vp, _dp, _dp, _dp, _dp, _dp, _dp]

@@ -280,6 +280,7 @@ int cfm_destroy(cfm_ctx* c) {
   c->p.partial.release();
   c->stage.release();
   c->nf.release();
+  c->coarse.release();
   c->dinv.release();
   c->vec_a.release();
   c->vec_b.release();
@@ -346,6 +347,7 @@ int cfm_set_positions(cfm_ctx* c, const double* x_seg, const double* x_blob) {
     c->blob_pos.assign(x_blob, x_blob + nb3);
     c->nf.valid = false;
     c->dinv.valid = false;
+    c->coarse.valid = false;
   }
   return CFM_OK;
 }

@@ -113,6 +113,26 @@ struct NearField {
   }
 };
 
+// Coarse correction of the near-field preconditioner (coarse.hip): k vectors Z of the
+// blob system ([k][3][m]) -- the 12 linear fields of the blob positions, optionally
+// followed by the learned network's vectors -- with M Z, M^2 Z and the inverse of the
+// regularised Gram matrix (M Z)^T (M Z) + delta I, built once per blob geometry (and
+// per network state).  Applied after the near-field inverse W:
+//   y = W r + Z c,  c = argmin ||(r - M W r) - M Z c||  (+ delta ||c||^2),
+// with (M Z)^T M W r = (M^2 Z)^T W r, so no mobility product is needed per application.
+struct CoarseSpace {
+  int mode = 0;       // CFM_COARSE_*
+  bool valid = false;
+  int k = 0, L = 0;
+  DevBuf<double> Z, MZ, M2Z, Ginv, coef, part;
+  std::vector<double> G;   // host copy of the Gram matrix (k x k)
+  void release() {
+    Z.release(); MZ.release(); M2Z.release(); Ginv.release(); coef.release(); part.release();
+    valid = false;
+    k = 0;
+  }
+};
+
 // Resident inverse of the blob-blob mobility (densepc.hip): an n x n FP32 matrix,
 // n = 3 n_blob, rows and columns in the blob-system vector order [component][blob],
 // leading dimension ld (a multiple of 64 floats, padding zero).  Built once per set
@@ -186,6 +206,7 @@ struct cfm_ctx {
   cfm::NearField nf;                    // near-field preconditioner of the blob system
   const double* nf_corr = nullptr;      // learned correction blocks (same layout as nf.W) or null
   cfm::DenseInverse dinv;               // resident inverse of the blob system (CFM_PC_DENSE)
+  cfm::CoarseSpace coarse;              // coarse correction of CFM_PC_NEAR (cfm_nearfield_coarse)
   std::vector<double> blob_pos;         // blob positions (AoS) of the last cfm_set_positions
   void* rocblas = nullptr;              // rocblas_handle on `stream` (created on first use)
   // autotuned source splits (tile_split = -1), keyed on the product's shape

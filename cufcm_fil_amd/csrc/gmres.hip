@@ -20,6 +20,7 @@
 #include "krylov.hpp"
 #include "nearfield.hpp"
 #include "densepc.hpp"
+#include "coarse.hpp"
 #include "precond.hpp"
 
 namespace cfm {
@@ -383,7 +384,9 @@ Status system_precond(cfm_ctx* c, const SystemDesc& s, int kind, const double* x
   if (kind == CFM_PC_NEAR) {
     if (s.which != CFM_BB) return make_status(CFM_EINVAL, "CFM_PC_NEAR applies to the blob system only");
     if (!c->nf.valid) CFM_TRY(nearfield_build(c, c->nf));
-    return nearfield_apply(c, c->nf, c->nf_corr, x, y);
+    CFM_TRY(nearfield_apply(c, c->nf, c->nf_corr, x, y));
+    if (c->coarse.mode != CFM_COARSE_NONE) CFM_TRY(coarse_apply(c, c->coarse, x, y));
+    return {};
   }
   if (kind == CFM_PC_DENSE) {
     if (s.which != CFM_BB) return make_status(CFM_EINVAL, "CFM_PC_DENSE applies to the blob system only");

@@ -28,6 +28,7 @@
 #include <random>
 #include <vector>
 
+#include "coarse.hpp"
 #include "ctx.hpp"
 #include "krylov.hpp"
 #include "nearfield.hpp"
@@ -42,10 +43,15 @@ constexpr int G_XE = 2 * G_H + G_F;  // 136
 constexpr int G_XN = 2 * G_H;      // 128
 constexpr int G_O = 16;            // output width (9 used)
 constexpr int G_L = 3;             // message-passing layers
+constexpr int G_QF = 16;           // positional monomials of the coarse head (10 used)
+constexpr int G_KC = 16;           // learned coarse vectors
+constexpr int G_XC = G_H + G_QF;   // 80: coarse-head input [h_i, q_i]
+constexpr int G_OC = 3 * G_KC;     // 48: coarse-head output (3 components per vector)
+constexpr int G_TMAX = 128;        // samples per coarse training batch
 
 // parameter offsets in one FP32 buffer
 struct GnnLayout {
-  size_t Wn, bn, W1[G_L], b1[G_L], W2[G_L], b2[G_L], Wo, bo, total;
+  size_t Wn, bn, W1[G_L], b1[G_L], W2[G_L], b2[G_L], Wo, bo, Wc, bc, total;
   GnnLayout() {
     size_t o = 0;
     Wn = o; o += G_F * G_H;
@@ -58,6 +64,8 @@ struct GnnLayout {
     }
     Wo = o; o += G_XE * G_O;
     bo = o; o += G_O;
+    Wc = o; o += G_XC * G_OC;
+    bc = o; o += G_OC;
     total = o;
   }
 };
@@ -75,6 +83,15 @@ struct Gnn {
   DevBuf<double> y, Ay, r, g, scal, partial;
   DevBuf<double> samples_b, samples_x;   // recorded training pairs [n][3m]
   int nsamples = 0, capacity = 0, next = 0;
+  // coarse head: positional monomials, input [h, q], output, learned vectors ([KC][3][m])
+  DevBuf<float> qfeat, Xc, Oc, dOc, dXc;
+  DevBuf<double> zl;
+  // coarse training: s_t = b_t - M W b_t per sample, M Z ([12 + KC][3m]; rows 0..11 the
+  // linear fields, fixed), the descent direction D and dL/dZL ([KC][3m]), host copies
+  DevBuf<double> S, snorm, MZt, Zp, D, dZ, gram, bmat, cmat, wmat, cpart;
+  std::vector<double> s_norm2;
+  int s_ready = 0;          // samples whose s_t is computed
+  double delta = 0.0;       // Tikhonov term of the coarse objective (geometry only)
   bool enabled = false, trained = false, recording = false;
   int adam_t = 0;
   double last_loss = 0.0;
@@ -241,6 +258,73 @@ __global__ void gnn_output_kernel(size_t E, const float* __restrict__ Co, double
   corr[t] = (double)Co[e * G_O + (t - e * 9)];
 }
 
+// ---- coarse head ----------------------------------------------------------------
+// q_i: 1, y, y^2 (per component), y_x y_y, y_x y_z, y_y y_z, 0 x 6; y = (x_i - c) / R
+__global__ void gnn_positional_kernel(int m, int off, const double* __restrict__ x,
+                                      const double* __restrict__ y, const double* __restrict__ z,
+                                      double cx, double cy, double cz, double inv_r,
+                                      float* __restrict__ q) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const double a = (x[off + i] - cx) * inv_r, b = (y[off + i] - cy) * inv_r, c = (z[off + i] - cz) * inv_r;
+  const double v[G_QF] = {1.0, a, b, c, a * a, b * b, c * c, a * b, a * c, b * c, 0, 0, 0, 0, 0, 0};
+  for (int k = 0; k < G_QF; ++k) q[(size_t)i * G_QF + k] = (float)v[k];
+}
+
+// Xc[i] = [h_i, q_i]
+__global__ void gnn_coarse_input_kernel(int m, const float* __restrict__ h, const float* __restrict__ q,
+                                        float* __restrict__ Xc) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)m * G_XC) return;
+  const int i = (int)(t / G_XC), c = (int)(t - (size_t)i * G_XC);
+  Xc[t] = c < G_H ? h[(size_t)i * G_H + c] : q[(size_t)i * G_QF + c - G_H];
+}
+
+// zl[k][d][i] = Oc[i][3k + d] (FP64, blob-system layout)
+__global__ void gnn_coarse_out_kernel(int m, const float* __restrict__ Oc, double* __restrict__ zl) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)m * G_OC) return;
+  const int i = (int)(t / G_OC), c = (int)(t - (size_t)i * G_OC);
+  const int k = c / 3, d = c - 3 * k;
+  zl[((size_t)k * 3 + d) * m + i] = (double)Oc[t];
+}
+
+// dOc[i][3k + d] = scale dZ[k][d][i]
+__global__ void gnn_coarse_dout_kernel(int m, const double* __restrict__ dZ, double scale,
+                                       float* __restrict__ dOc) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)m * G_OC) return;
+  const int i = (int)(t / G_OC), c = (int)(t - (size_t)i * G_OC);
+  const int k = c / 3, d = c - 3 * k;
+  dOc[t] = (float)(scale * dZ[((size_t)k * 3 + d) * m + i]);
+}
+
+// D[kk][e] = sum_t w[t][kk] (S[t][e] - sum_j MZ[j][e] C[t][j]) over the T samples of a
+// batch: C[t] the coarse coefficients of sample t (k values), w[t][kk] = C[t][12 + kk] /
+// ||s_t||^2 for the learned vectors.
+__global__ void gnn_coarse_dir_kernel(int L, int T, int k, const double* __restrict__ S,
+                                      const double* __restrict__ MZ, const double* __restrict__ C,
+                                      const double* __restrict__ w, double* __restrict__ D) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= L) return;
+  double acc[G_KC];
+  for (int q = 0; q < G_KC; ++q) acc[q] = 0.0;
+  for (int t = 0; t < T; ++t) {
+    double r = S[(size_t)t * L + e];
+    for (int j = 0; j < k; ++j) r -= MZ[(size_t)j * L + e] * C[(size_t)t * k + j];
+    for (int q = 0; q < G_KC; ++q) acc[q] += w[(size_t)t * G_KC + q] * r;
+  }
+  for (int q = 0; q < G_KC; ++q) D[(size_t)q * L + e] = acc[q];
+}
+
+// dH[i][c] = dXc[i][c], c < G_H
+__global__ void gnn_copy_dh_kernel(int m, const float* __restrict__ dXc, float* __restrict__ dH) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)m * G_H) return;
+  const int i = (int)(t / G_H), c = (int)(t - (size_t)i * G_H);
+  dH[t] = dXc[(size_t)i * G_XC + c];
+}
+
 // ---- training pieces -------------------------------------------------------------
 // G[e] += -2 alpha / ||b||^2 * g_i b_j^T   (b, g: [3][m]; e = (i, slot));
 // scal = [||b||^2, ., alpha]
@@ -370,6 +454,12 @@ static Status gnn_alloc(cfm_ctx* c, Gnn& G) {
   CFM_TRY(G.dA.alloc(m * G_H));
   CFM_TRY(G.corr.alloc(E * 9));
   CFM_TRY(G.Gacc.alloc(E * 9));
+  CFM_TRY(G.qfeat.alloc(m * G_QF));
+  CFM_TRY(G.Xc.alloc(m * G_XC));
+  CFM_TRY(G.Oc.alloc(m * G_OC));
+  CFM_TRY(G.dOc.alloc(m * G_OC));
+  CFM_TRY(G.dXc.alloc(m * G_XC));
+  CFM_TRY(G.zl.alloc((size_t)G_KC * 3 * m));
   const int L = 3 * G.m;
   CFM_TRY(G.y.alloc(L));
   CFM_TRY(G.Ay.alloc(L));
@@ -414,8 +504,35 @@ static Status gnn_forward(cfm_ctx* c, Gnn& G) {
                G_O));
   hipLaunchKernelGGL(gnn_output_kernel, gl(E * 9), dim3(256), 0, c->stream, E, G.Co.p, G.corr.p);
   CFM_CHECK_LAUNCH();
+  // coarse head: positional monomials about the blob centroid, [h, q] Wc + bc
+  const std::vector<double>& bp = c->blob_pos;
+  if ((int)bp.size() != 3 * m) return make_status(CFM_ESTATE, "blob positions not set");
+  double cx = 0, cy = 0, cz = 0, r2 = 0;
+  for (int i = 0; i < m; ++i) {
+    cx += bp[3 * i];
+    cy += bp[3 * i + 1];
+    cz += bp[3 * i + 2];
+  }
+  cx /= m;
+  cy /= m;
+  cz /= m;
+  for (int i = 0; i < m; ++i) {
+    const double dx = bp[3 * i] - cx, dy = bp[3 * i + 1] - cy, dz = bp[3 * i + 2] - cz;
+    r2 = std::max(r2, dx * dx + dy * dy + dz * dz);
+  }
+  hipLaunchKernelGGL(gnn_positional_kernel, gl(m), dim3(256), 0, c->stream, m, p.n_seg, p.x.p, p.y.p, p.z.p,
+                     cx, cy, cz, r2 > 0 ? 1.0 / std::sqrt(r2) : 1.0, G.qfeat.p);
+  hipLaunchKernelGGL(gnn_coarse_input_kernel, gl((size_t)m * G_XC), dim3(256), 0, c->stream, m, G.H[G_L].p,
+                     G.qfeat.p, G.Xc.p);
+  CFM_CHECK_LAUNCH();
+  CFM_TRY(gemm(c, m, G_OC, G_XC, G.Xc.p, G_XC, false, th + G.lay.Wc, G_OC, false, th + G.lay.bc, G.Oc.p,
+               G_OC));
+  hipLaunchKernelGGL(gnn_coarse_out_kernel, gl((size_t)m * G_OC), dim3(256), 0, c->stream, m, G.Oc.p, G.zl.p);
+  CFM_CHECK_LAUNCH();
   return {};
 }
+
+static Status gnn_trunk_backward(cfm_ctx* c, Gnn& G);
 
 // Back-propagates dL/dC (G.Gacc, FP64 [E][9], scaled by `scale`) into G.grad.
 static Status gnn_backward(cfm_ctx* c, Gnn& G, double scale) {
@@ -433,6 +550,33 @@ static Status gnn_backward(cfm_ctx* c, Gnn& G, double scale) {
   hipLaunchKernelGGL(fill_f32_kernel, gl((size_t)m * G_H), dim3(256), 0, c->stream, (size_t)m * G_H, G.dH.p, 0.f);
   hipLaunchKernelGGL(gnn_scatter_edge_kernel, gl(E * 2 * G_H), dim3(256), 0, c->stream, m, nf.nbr.p, G.dXo.p, G.dH.p);
   CFM_CHECK_LAUNCH();
+  return gnn_trunk_backward(c, G);
+}
+
+// Back-propagates dL/dZL (FP64 [KC][3][m], scaled by `scale`) through the coarse head and
+// the trunk into G.grad.
+static Status gnn_backward_coarse(cfm_ctx* c, Gnn& G, const double* dZ, double scale) {
+  const int m = G.m;
+  const float* th = G.theta.p;
+  float* gr = G.grad.p;
+  hipLaunchKernelGGL(gnn_coarse_dout_kernel, gl((size_t)m * G_OC), dim3(256), 0, c->stream, m, dZ, scale, G.dOc.p);
+  CFM_CHECK_LAUNCH();
+  CFM_TRY(gemm(c, G_XC, G_OC, m, G.Xc.p, G_XC, true, G.dOc.p, G_OC, false, nullptr, gr + G.lay.Wc, G_OC));
+  hipLaunchKernelGGL(col_sum_kernel, dim3(G_OC), dim3(256), 0, c->stream, m, G_OC, G.dOc.p, G_OC, gr + G.lay.bc);
+  CFM_TRY(gemm(c, m, G_XC, G_OC, G.dOc.p, G_OC, false, th + G.lay.Wc, G_OC, true, nullptr, G.dXc.p, G_XC));
+  hipLaunchKernelGGL(gnn_copy_dh_kernel, gl((size_t)m * G_H), dim3(256), 0, c->stream, m, G.dXc.p, G.dH.p);
+  CFM_CHECK_LAUNCH();
+  return gnn_trunk_backward(c, G);
+}
+
+// Back-propagates dL/dh of the last layer (G.dH) through the message-passing trunk and
+// the embedding into G.grad.
+static Status gnn_trunk_backward(cfm_ctx* c, Gnn& G) {
+  const int m = G.m;
+  const size_t E = G.E;
+  const float* th = G.theta.p;
+  float* gr = G.grad.p;
+  const NearField& nf = c->nf;
   for (int l = G_L - 1; l >= 0; --l) {
     // h_{l+1} = h_l + relu(Zn): dZn = dH (masked); dH stays (identity path)
     hipLaunchKernelGGL(relu_mask_kernel, gl((size_t)m * G_H), dim3(256), 0, c->stream, (size_t)m * G_H, G.dH.p,
@@ -502,6 +646,110 @@ static Status gnn_loss_grad(cfm_ctx* c, Gnn& G, int s0, int s1, bool acc, double
   return {};
 }
 
+// ---- coarse objective --------------------------------------------------------------
+// Training data of the coarse head: s_t = b_t - M W b_t for every recorded right-hand
+// side (FP32C products), ||s_t||^2, the linear fields Zp with M Zp (rows 0..11 of MZt),
+// and the Tikhonov term delta = 1e-10 tr((M Zp)^T M Zp) / 12, which depends on the
+// geometry only.
+static Status coarse_prepare(cfm_ctx* c, Gnn& G) {
+  const SystemDesc s = system_desc(c, CFM_SYS_BLOB);
+  const int L = s.L;
+  const int k = kCoarseLinear + G_KC;
+  if (!G.MZt.p) {
+    CFM_TRY(G.Zp.alloc((size_t)kCoarseLinear * L));
+    CFM_TRY(G.MZt.alloc((size_t)k * L));
+    CFM_TRY(G.D.alloc((size_t)G_KC * L));
+    CFM_TRY(G.dZ.alloc((size_t)G_KC * L));
+    CFM_TRY(G.gram.alloc((size_t)k * k));
+    CFM_TRY(G.bmat.alloc((size_t)G_TMAX * k));
+    CFM_TRY(G.cmat.alloc((size_t)G_TMAX * k));
+    CFM_TRY(G.wmat.alloc((size_t)G_TMAX * G_KC));
+    CFM_TRY(G.cpart.alloc(dev_dots_partial_size(L, k)));
+    CFM_TRY(coarse_linear_fields(c, G.Zp.p));
+    for (int j = 0; j < kCoarseLinear; ++j)
+      CFM_TRY(system_apply(c, s, CFM_FP32C, G.Zp.p + (size_t)j * L, G.MZt.p + (size_t)j * L));
+    for (int j = 0; j < kCoarseLinear; ++j)
+      CFM_TRY(dev_dots(c, G.cpart.p, G.MZt.p + (size_t)j * L, 0, 1, G.MZt.p + (size_t)j * L, L, G.gram.p + j));
+    std::vector<double> d(kCoarseLinear);
+    CFM_HIP(copy_to_host(c->stream, d.data(), G.gram.p, sizeof(double) * kCoarseLinear));
+    double tr = 0.0;
+    for (double v : d) tr += v;
+    G.delta = 1e-10 * tr / kCoarseLinear;
+  }
+  CFM_TRY(G.S.alloc((size_t)G.capacity * L));
+  CFM_TRY(G.snorm.alloc((size_t)G.capacity));
+  for (int t = 0; t < G.nsamples; ++t) {
+    const double* b = G.samples_b.p + (size_t)t * L;
+    double* st = G.S.p + (size_t)t * L;
+    CFM_TRY(nearfield_apply(c, c->nf, nullptr, b, G.y.p));
+    CFM_TRY(system_apply(c, s, CFM_FP32C, G.y.p, G.Ay.p));
+    CFM_TRY(dev_axpby(c, 1.0, b, -1.0, G.Ay.p, st, L));
+    CFM_TRY(dev_dots(c, G.cpart.p, st, 0, 1, st, L, G.snorm.p + t));
+  }
+  G.s_norm2.assign(G.nsamples, 0.0);
+  CFM_HIP(copy_to_host(c->stream, G.s_norm2.data(), G.snorm.p, sizeof(double) * G.nsamples));
+  G.s_ready = G.nsamples;
+  return {};
+}
+
+// Coarse objective over samples [s0, s1) (s1 - s0 <= G_TMAX):
+//   L = mean_t (||s_t - M Z c_t||^2 + delta ||c_t||^2) / ||s_t||^2,  Z = [Zp | ZL],
+//   (G + delta I) c_t = (M Z)^T s_t,  G = (M Z)^T M Z,
+// which at the optimum equals 1 - c_t . (M Z)^T s_t / ||s_t||^2.  With acc, dL/dZL =
+// -2/T M sum_t (s_t - M Z c_t) c_t,L^T / ||s_t||^2 (envelope theorem) is back-propagated
+// into G.grad (zero-filled by the caller).
+static Status coarse_loss_grad(cfm_ctx* c, Gnn& G, int s0, int s1, bool acc, double* loss) {
+  const SystemDesc s = system_desc(c, CFM_SYS_BLOB);
+  const int L = s.L;
+  const int k = kCoarseLinear + G_KC;
+  const int T = s1 - s0;
+  CFM_TRY(gnn_forward(c, G));
+  for (int q = 0; q < G_KC; ++q)
+    CFM_TRY(system_apply(c, s, CFM_FP32C, G.zl.p + (size_t)q * L, G.MZt.p + (size_t)(kCoarseLinear + q) * L));
+  for (int j = 0; j < k; ++j)
+    CFM_TRY(dev_dots(c, G.cpart.p, G.MZt.p, (size_t)L, k, G.MZt.p + (size_t)j * L, L, G.gram.p + (size_t)j * k));
+  for (int t = 0; t < T; ++t)
+    CFM_TRY(dev_dots(c, G.cpart.p, G.MZt.p, (size_t)L, k, G.S.p + (size_t)(s0 + t) * L, L,
+                     G.bmat.p + (size_t)t * k));
+  std::vector<double> gram((size_t)k * k), bm((size_t)T * k), inv;
+  CFM_HIP(hipMemcpyAsync(gram.data(), G.gram.p, sizeof(double) * k * k, hipMemcpyDeviceToHost, c->stream));
+  CFM_HIP(copy_to_host(c->stream, bm.data(), G.bmat.p, sizeof(double) * T * k));
+  if (!invert_gram(gram, k, inv, G.delta)) return make_status(CFM_ENONFINITE, "singular coarse Gram matrix");
+  std::vector<double> cm((size_t)T * k), wm((size_t)T * G_KC);
+  double tot = 0.0;
+  for (int t = 0; t < T; ++t) {
+    const double n2 = G.s_norm2[s0 + t];
+    double cb = 0.0;
+    for (int a = 0; a < k; ++a) {
+      double v = 0.0;
+      for (int b = 0; b < k; ++b) v += inv[(size_t)a * k + b] * bm[(size_t)t * k + b];
+      cm[(size_t)t * k + a] = v;
+      cb += v * bm[(size_t)t * k + a];
+    }
+    tot += n2 > 0.0 ? 1.0 - cb / n2 : 0.0;
+    for (int q = 0; q < G_KC; ++q) wm[(size_t)t * G_KC + q] = n2 > 0.0 ? cm[(size_t)t * k + kCoarseLinear + q] / n2 : 0.0;
+  }
+  *loss = tot / std::max(1, T);
+  if (!acc) return {};
+  CFM_HIP(hipMemcpyAsync(G.cmat.p, cm.data(), sizeof(double) * cm.size(), hipMemcpyHostToDevice, c->stream));
+  CFM_HIP(copy_to_device(c->stream, G.wmat.p, wm.data(), sizeof(double) * wm.size()));
+  hipLaunchKernelGGL(gnn_coarse_dir_kernel, gl(L), dim3(256), 0, c->stream, L, T, k, G.S.p + (size_t)s0 * L, G.MZt.p,
+                     G.cmat.p, G.wmat.p, G.D.p);
+  CFM_CHECK_LAUNCH();
+  for (int q = 0; q < G_KC; ++q)
+    CFM_TRY(system_apply(c, s, CFM_FP32C, G.D.p + (size_t)q * L, G.dZ.p + (size_t)q * L));
+  return gnn_backward_coarse(c, G, G.dZ.p, -2.0 / T);
+}
+
+Status gnn_coarse_vectors(cfm_ctx* c, const double** zl, int* kl) {
+  Gnn* G = reinterpret_cast<Gnn*>(c->gnn);
+  if (!G) return make_status(CFM_ESTATE, "cfm_gnn_init has not been called");
+  CFM_TRY(gnn_forward(c, *G));
+  *zl = G->zl.p;
+  *kl = G_KC;
+  return {};
+}
+
 static void init_params(Gnn& G, unsigned seed, std::vector<float>& h) {
   h.assign(G.lay.total, 0.f);
   std::mt19937 rng(seed);
@@ -516,7 +764,9 @@ static void init_params(Gnn& G, unsigned seed, std::vector<float>& h) {
     // keep the residual branches small at the start
     for (int i = 0; i < G_XN * G_H; ++i) h[G.lay.W2[l] + i] *= 0.1f;
   }
-  // output layer zero: the network starts as "no correction"
+  // edge output layer zero: the network starts as "no correction"; the coarse head
+  // starts random (its vectors only extend the linear fields' coarse space)
+  fill(G.lay.Wc, G_XC, G_XC * G_OC);
 }
 
 // Stores one (rhs, solution) pair of a blob solve in the training ring (device copies).
@@ -572,18 +822,24 @@ void cfm_gnn_release(cfm_ctx* c) {
   Gnn* G = gnn_of(c);
   if (!G) return;
   DevBuf<float>* fb[] = {&G->theta, &G->grad, &G->adam_m, &G->adam_v, &G->nfeat, &G->efeat, &G->Z0,
-                         &G->Xo, &G->Co, &G->dH, &G->dXe, &G->dZe, &G->dXn, &G->dZn, &G->dCo, &G->dXo, &G->dA};
+                         &G->Xo, &G->Co, &G->dH, &G->dXe, &G->dZe, &G->dXn, &G->dZn, &G->dCo, &G->dXo, &G->dA,
+                         &G->qfeat, &G->Xc, &G->Oc, &G->dOc, &G->dXc};
   for (auto* b : fb) b->release();
   for (int l = 0; l <= G_L; ++l) G->H[l].release();
   for (int l = 0; l < G_L; ++l) {
     G->Xe[l].release(); G->Ze[l].release(); G->Me[l].release(); G->Xn[l].release(); G->Zn[l].release();
   }
   DevBuf<double>* db[] = {&G->corr, &G->Gacc, &G->y, &G->Ay, &G->r, &G->g, &G->scal, &G->partial,
-                          &G->samples_b, &G->samples_x};
+                          &G->samples_b, &G->samples_x, &G->zl, &G->S, &G->snorm, &G->MZt, &G->Zp, &G->D,
+                          &G->dZ, &G->gram, &G->bmat, &G->cmat, &G->wmat, &G->cpart};
   for (auto* b : db) b->release();
   delete G;
   c->gnn = nullptr;
   c->nf_corr = nullptr;
+  if (c->coarse.mode == CFM_COARSE_LEARNED) {
+    c->coarse.mode = CFM_COARSE_NONE;
+    c->coarse.valid = false;
+  }
 }
 
 extern "C" {
@@ -730,6 +986,7 @@ int cfm_gnn_set_params(cfm_ctx* c, const float* in) {
   (void)hipSetDevice(c->device);
   hipError_t e = copy_to_device(c->stream, G->theta.p, in, sizeof(float) * G->lay.total);
   if (e != hipSuccess) return gfail(c, make_status(CFM_EHIP, "%s", hipGetErrorString(e)));
+  if (c->coarse.mode == CFM_COARSE_LEARNED) c->coarse.valid = false;
   Status s = gnn_forward(c, *G);
   if (s.ok() && hipStreamSynchronize(c->stream) != hipSuccess) s = make_status(CFM_EHIP, "sync failed");
   return s.ok() ? CFM_OK : gfail(c, s);
@@ -771,6 +1028,99 @@ int cfm_gnn_gradient(cfm_ctx* c, int32_t s0, int32_t s1, float* grad, double* lo
     return {};
   };
   Status s = run();
+  return s.ok() ? CFM_OK : gfail(c, s);
+}
+
+// Trains the coarse head on the recorded right-hand sides (coarse objective above);
+// the learned coarse space of CFM_COARSE_LEARNED is rebuilt on its next use.
+int cfm_gnn_train_coarse(cfm_ctx* c, int32_t epochs, int32_t batch, double lr, double* loss_before,
+                         double* loss_after) {
+  if (!c) return CFM_EINVAL;
+  Gnn* G = gnn_of(c);
+  if (!G) return gfail(c, make_status(CFM_ESTATE, "cfm_gnn_init has not been called"));
+  if (G->nsamples < 1) return gfail(c, make_status(CFM_ESTATE, "no training samples"));
+  if (epochs < 0 || batch < 1 || batch > G_TMAX || !(lr > 0))
+    return gfail(c, make_status(CFM_EINVAL, "bad training options (batch 1..%d)", G_TMAX));
+  (void)hipSetDevice(c->device);
+  auto full_loss = [&](double* out) -> Status {
+    double tot = 0.0;
+    for (int s0 = 0; s0 < G->nsamples; s0 += G_TMAX) {
+      const int s1 = std::min(G->nsamples, s0 + G_TMAX);
+      double l = 0.0;
+      CFM_TRY(coarse_loss_grad(c, *G, s0, s1, false, &l));
+      tot += l * (s1 - s0);
+    }
+    *out = tot / G->nsamples;
+    return {};
+  };
+  auto run = [&]() -> Status {
+    CFM_TRY(coarse_prepare(c, *G));
+    double l0 = 0.0;
+    CFM_TRY(full_loss(&l0));
+    if (loss_before) *loss_before = l0;
+    const size_t P = G->lay.total;
+    for (int ep = 0; ep < epochs; ++ep) {
+      for (int s0 = 0; s0 < G->nsamples; s0 += batch) {
+        const int s1 = std::min(G->nsamples, s0 + batch);
+        double l = 0.0;
+        hipLaunchKernelGGL(fill_f32_kernel, gl(P), dim3(256), 0, c->stream, P, G->grad.p, 0.f);
+        CFM_CHECK_LAUNCH();
+        CFM_TRY(coarse_loss_grad(c, *G, s0, s1, true, &l));
+        ++G->adam_t;
+        const float b1 = 0.9f, b2 = 0.999f;
+        const float c1 = 1.f - std::pow(b1, (float)G->adam_t), c2 = 1.f - std::pow(b2, (float)G->adam_t);
+        hipLaunchKernelGGL(adam_kernel, gl(P), dim3(256), 0, c->stream, P, G->theta.p, G->grad.p, G->adam_m.p,
+                           G->adam_v.p, (float)lr, b1, b2, c1, c2);
+        CFM_CHECK_LAUNCH();
+      }
+    }
+    double l1 = 0.0;
+    CFM_TRY(full_loss(&l1));
+    if (loss_after) *loss_after = l1;
+    if (c->coarse.mode == CFM_COARSE_LEARNED) c->coarse.valid = false;
+    return {};
+  };
+  Status s = run();
+  return s.ok() ? CFM_OK : gfail(c, s);
+}
+
+// Gradient of the coarse objective over samples [s0, s1) with respect to the parameters
+// (FP32 out), the objective and its Tikhonov term delta.
+int cfm_gnn_coarse_gradient(cfm_ctx* c, int32_t s0, int32_t s1, float* grad, double* loss, double* delta) {
+  Gnn* G = c ? gnn_of(c) : nullptr;
+  if (!G) return c ? gfail(c, make_status(CFM_ESTATE, "no network")) : CFM_EINVAL;
+  if (s0 < 0 || s1 > G->nsamples || s1 <= s0 || s1 - s0 > G_TMAX)
+    return gfail(c, make_status(CFM_EINVAL, "bad sample range"));
+  (void)hipSetDevice(c->device);
+  auto run = [&]() -> Status {
+    CFM_TRY(coarse_prepare(c, *G));
+    hipLaunchKernelGGL(fill_f32_kernel, gl(G->lay.total), dim3(256), 0, c->stream, G->lay.total, G->grad.p, 0.f);
+    CFM_CHECK_LAUNCH();
+    double l = 0.0;
+    CFM_TRY(coarse_loss_grad(c, *G, s0, s1, true, &l));
+    CFM_HIP(hipStreamSynchronize(c->stream));
+    if (grad) CFM_HIP(copy_to_host(c->stream, grad, G->grad.p, sizeof(float) * G->lay.total));
+    if (loss) *loss = l;
+    if (delta) *delta = G->delta;
+    return {};
+  };
+  Status s = run();
+  return s.ok() ? CFM_OK : gfail(c, s);
+}
+
+// The learned coarse vectors of the current parameters: G_KC vectors of the blob system,
+// each [3][n_blob] (component-major), one after another.
+int cfm_gnn_coarse_vectors(cfm_ctx* c, double* zl) {
+  Gnn* G = c ? gnn_of(c) : nullptr;
+  if (!G || !zl) return c ? gfail(c, make_status(CFM_ESTATE, "no network")) : CFM_EINVAL;
+  (void)hipSetDevice(c->device);
+  const double* d = nullptr;
+  int kl = 0;
+  Status s = gnn_coarse_vectors(c, &d, &kl);
+  if (s.ok()) {
+    hipError_t e = copy_to_host(c->stream, zl, d, sizeof(double) * kl * 3 * (size_t)G->m);
+    if (e != hipSuccess) s = make_status(CFM_EHIP, "%s", hipGetErrorString(e));
+  }
   return s.ok() ? CFM_OK : gfail(c, s);
 }
 

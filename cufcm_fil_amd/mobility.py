@@ -283,6 +283,59 @@ class RpyMobilitySolver:
                                                   ctypes.byref(loss)))
         return loss.value, g
 
+    # -- coarse correction of the near-field preconditioner --------------------------
+    _COARSE = {"none": L.CFM_COARSE_NONE, "linear": L.CFM_COARSE_LINEAR,
+               "learned": L.CFM_COARSE_LEARNED}
+
+    def nearfield_coarse(self, mode="linear"):
+        """Coarse correction added to the 'near' blob preconditioner: 'none', 'linear'
+        (the 12 linear fields of the blob positions) or 'learned' (those plus the
+        network's 16 coarse vectors)."""
+        L.check(self._h, L.lib().cfm_nearfield_coarse(self._h, self._COARSE[mode]))
+
+    def gnn_train_coarse(self, epochs=10, batch=32, lr=1e-3):
+        l0, l1 = ctypes.c_double(), ctypes.c_double()
+        L.check(self._h, L.lib().cfm_gnn_train_coarse(self._h, int(epochs), int(batch), float(lr),
+                                                      ctypes.byref(l0), ctypes.byref(l1)))
+        return l0.value, l1.value
+
+    def gnn_coarse_gradient(self, s0, s1):
+        g = np.zeros(L.lib().cfm_gnn_num_params(self._h), dtype=np.float32)
+        loss, delta = ctypes.c_double(), ctypes.c_double()
+        L.check(self._h, L.lib().cfm_gnn_coarse_gradient(self._h, int(s0), int(s1), L.fptr(g),
+                                                         ctypes.byref(loss), ctypes.byref(delta)))
+        return loss.value, g, delta.value
+
+    def gnn_coarse_vectors(self):
+        z = np.zeros(16 * 3 * self.n_blob)
+        L.check(self._h, L.lib().cfm_gnn_coarse_vectors(self._h, L.ptr(z)))
+        return z.reshape(16, 3 * self.n_blob)
+
+    def gnn_train_coarse_gated(self, validation_rhs, epochs=10, batch=32, lr=1e-3, rtol=1e-10,
+                               max_iter=3000):
+        """Train the coarse head on the recorded right-hand sides, then keep the learned
+        coarse space only if it lowers the total GMRES iterations of the validation blob
+        solves against the linear fields alone (near-field preconditioner, FP64, rtol);
+        otherwise the linear fields stay in use.  Every validation solve must reach rtol."""
+        def iterations(mode):
+            self.nearfield_coarse(mode)
+            its = []
+            for b in validation_rhs:
+                _, st = self.solve("blob", b, rtol=rtol, precond="near", mode="fp64",
+                                   max_iter=max_iter)
+                if not st["converged"]:
+                    raise L.CfmError(L.CFM_ENOCONV, "validation solve did not converge")
+                its.append(st["iterations"])
+            return its
+
+        its0 = iterations("linear")
+        l0, l1 = self.gnn_train_coarse(epochs=epochs, batch=batch, lr=lr)
+        its1 = iterations("learned")
+        keep = sum(its1) < sum(its0)
+        self.nearfield_coarse("learned" if keep else "linear")
+        return {"loss_before": l0, "loss_after": l1, "iterations_linear": its0,
+                "iterations_learned": its1, "kept": keep}
+
     def gemm_bf16(self, A, B, transA=False, transB=False):
         A = np.ascontiguousarray(A, dtype=np.float32)
         B = np.ascontiguousarray(B, dtype=np.float32)

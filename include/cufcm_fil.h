@@ -299,6 +299,22 @@ typedef struct cfm_profile_stats {
 int cfm_profile(cfm_ctx* ctx, int32_t enable);
 int cfm_profile_read(cfm_ctx* ctx, cfm_profile_stats* stats);
 
+/* ---- coarse correction of the near-field preconditioner (no reference counterpart) ----
+ * CFM_PC_NEAR applies the local inverse W of each blob's 17-blob neighbourhood; its
+ * iterations are dominated by collective modes no local block holds (rigid and
+ * linear motions of whole bodies, and the near-null "squeeze" of a closed surface).
+ * A coarse correction over k vectors Z of the blob system adds
+ *     y = W r + Z c,  c = argmin_c ||(r - M W r) - M Z c||^2 + delta ||c||^2
+ * (the residual of W r minimised over span(M Z)); M Z, M^2 Z and the Gram inverse are
+ * built once per blob geometry with FP64 products, so an application costs 2k dot
+ * products and k axpys.  CFM_COARSE_LINEAR: the 12 linear fields of the blob positions
+ * (3 directions x {1, y_x, y_y, y_z}, y = (x - centroid) / radius).  CFM_COARSE_LEARNED:
+ * those plus the learned network's coarse vectors (cfm_gnn_*; rebuilt after training). */
+#define CFM_COARSE_NONE 0
+#define CFM_COARSE_LINEAR 1
+#define CFM_COARSE_LEARNED 2
+int cfm_nearfield_coarse(cfm_ctx* ctx, int32_t mode);
+
 /* ---- learned near-field correction (no reference counterpart) -------------- */
 /* A message-passing network on the blobs' 16-nearest-neighbour graph predicts a
  * 3x3 correction block per edge; with it enabled, CFM_PC_NEAR applies the analytic
@@ -321,6 +337,19 @@ int cfm_gnn_correction(cfm_ctx* ctx, double* corr, int32_t* nbr); /* corr [n_blo
 int cfm_gnn_gradient(cfm_ctx* ctx, int32_t s0, int32_t s1, float* grad, double* loss);
 int cfm_gemm_bf16(cfm_ctx* ctx, int32_t M, int32_t N, int32_t K, const float* A, int32_t transA,
                   const float* B, int32_t transB, float* C);  /* C = op(A) op(B), row-major */
+/* The network's coarse head: per blob, [h_i, positional monomials] (80 -> 48) gives 16
+ * learned coarse vectors of the blob system that extend the linear fields of
+ * CFM_COARSE_LINEAR (CFM_COARSE_LEARNED).  Training minimises, over the recorded
+ * right-hand sides b_t with s_t = b_t - M W b_t (W the near-field inverse),
+ *   L = mean_t (||s_t - M Z c_t||^2 + delta ||c_t||^2) / ||s_t||^2,  Z = [linear | learned],
+ * c_t its minimiser, delta = 1e-10 tr((M Zlin)^T M Zlin) / 12 (geometry only); the
+ * gradient dL/dZ = -2/T M sum_t (s_t - M Z c_t) c_t^T / ||s_t||^2 is back-propagated
+ * through the network (BF16 matrix-core GEMMs) and applied with Adam.  batch <= 128. */
+int cfm_gnn_train_coarse(cfm_ctx* ctx, int32_t epochs, int32_t batch, double lr, double* loss_before,
+                         double* loss_after);
+int cfm_gnn_coarse_gradient(cfm_ctx* ctx, int32_t s0, int32_t s1, float* grad, double* loss,
+                            double* delta);
+int cfm_gnn_coarse_vectors(cfm_ctx* ctx, double* zl); /* 16 x [3][n_blob] */
 
 /* ---- prescribed-shape cilia above the wall ------------------------------------------
  * The reference's CILIA_TYPE 3 path in the mode that writes its generalised-force
