@@ -29,8 +29,6 @@ Status launch_apply(cfm_ctx* c, const ApplyPlan& plan);
 int choose_split(cfm_ctx* c, const ApplyPlan& pl);
 ApplyPlan plan_for(const cfm_ctx* c, int which);
 
-bool gnn_recording(cfm_ctx* c);
-Status gnn_record_rhs(cfm_ctx* c, const double* b);
 
 constexpr int kRB = 256;  // threads per reduction block
 constexpr int kChunk = kRB * 8;
@@ -430,7 +428,6 @@ Status gmres_core(cfm_ctx* c, Krylov& K, const SystemDesc& s, int prec, int pc, 
   double last_est = nr;
   const bool sync_check = pc == CFM_PC_DENSE;
   while (nr > target_abs && *iters < max_iter) {
-    if (s.which == CFM_BB && !s.body && gnn_recording(c)) CFM_TRY(gnn_record_rhs(c, K.r.p));
     // q0 = r / ||r||
     hipLaunchKernelGGL(scale_by_inv_kernel, g1(L), dim3(256), 0, c->stream, K.r.p, K.scal.p, K.Q.p, L);
     hipLaunchKernelGGL(beta_init_kernel, dim3(1), dim3(64), 0, c->stream, K.beta.p, m + 1, K.scal.p);
@@ -459,10 +456,6 @@ Status gmres_core(cfm_ctx* c, Krylov& K, const SystemDesc& s, int prec, int pc, 
       hipLaunchKernelGGL(scale_by_inv_kernel, g1(L), dim3(256), 0, c->stream, K.w.p, K.scal.p + 1, qn, L);
       CFM_CHECK_LAUNCH();
       CFM_HIP(hipMemcpyAsync(K.host_res + 4 + (j % 4), K.res_ring.p + (j % 4), sizeof(double), hipMemcpyDeviceToHost, c->stream));
-      // training data for the learned near-field correction: Krylov directions of
-      // the blob system visited by this solve
-      if (s.which == CFM_BB && !s.body && (j == 3 || j == 9 || j == 19 || j == 39) && gnn_recording(c))
-        CFM_TRY(gnn_record_rhs(c, qn));
       CFM_HIP(hipEventRecord(K.ev[j % 4], c->stream));
       ++*iters;
       k = j + 1;

@@ -782,30 +782,15 @@ Status gnn_record(cfm_ctx* c, const double* b, const double* x) {
   return {};
 }
 
-bool gnn_recording(cfm_ctx* c) {
-  Gnn* G = gnn_of(c);
-  return G && G->recording;
-}
-
-// Stores one right-hand side the solver visited (a GMRES restart residual or a
-// Krylov basis vector of the blob system); its solution is not needed by the loss.
-Status gnn_record_rhs(cfm_ctx* c, const double* b) {
-  Gnn* G = gnn_of(c);
-  if (!G || !G->recording) return {};
-  const size_t L = 3 * (size_t)G->m;
-  const int slot = G->next;
-  CFM_HIP(hipMemcpyAsync(G->samples_b.p + slot * L, b, sizeof(double) * L, hipMemcpyDeviceToDevice, c->stream));
-  CFM_TRY(dev_fill(c, G->samples_x.p + slot * L, 0.0, (int)L));
-  G->next = (G->next + 1) % G->capacity;
-  G->nsamples = std::min(G->nsamples + 1, G->capacity);
-  return {};
-}
-
+// The network's initial guess of a blob solve: the two-level preconditioner with the
+// learned coarse space applied to the right-hand side, x0 = (W + Z c) b, when that
+// space is in use (CFM_COARSE_LEARNED); the solver still checks its FP64 residual and
+// rejects it when it is worse than a zero guess.
 Status gnn_guess(cfm_ctx* c, const double* rhs, double* x0, bool* used) {
-  (void)c;
-  (void)rhs;
-  (void)x0;
   *used = false;
+  if (!gnn_of(c) || c->coarse.mode != CFM_COARSE_LEARNED) return {};
+  CFM_TRY(system_precond(c, system_desc(c, CFM_SYS_BLOB), CFM_PC_NEAR, rhs, x0));
+  *used = true;
   return {};
 }
 

@@ -238,6 +238,8 @@ static Status velocities(cfm_ctx* c, FilState* F, cfm_step_stats* st) {
       }
       bool used = false;
       CFM_TRY(history_guess(c, h, F->rhs_b.p, F->fb_prev.p, &used));
+      // cold start (no history yet): the network's guess
+      if (!used && F->prm.guess == CFM_GUESS_LEARNED) CFM_TRY(gnn_guess(c, F->rhs_b.p, F->fb_prev.p, &used));
     }
     cfm_solve_opts o = F->prm.blob_solve;
     o.system = CFM_SYS_BLOB;

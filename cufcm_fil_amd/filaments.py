@@ -71,8 +71,20 @@ class FilamentSystem:
                                         use_guess=1, rtol=blob_rtol)
         P.overlap = int(bool(overlap))
         self._P = P
+        self._x0 = (x_seg, None if x_blob is None else L.f64(x_blob))
+        self._q0 = q
         pert_arr = None if pert is None else L.f64(pert, 3 * self.nfil)
         L.check(self.h, L.lib().cfm_fil_init(self.h, ctypes.byref(P), L.ptr(q), L.ptr(pert_arr)))
+        self.nt = 0
+
+    def restart(self, pert=None):
+        """Back to the initial configuration (positions, orientations, zero multipliers)
+        with a new tip perturbation; everything else the context holds (blob
+        preconditioners, recorded samples, the network) is kept."""
+        self.mob.set_positions(*self._x0)
+        pert_arr = None if pert is None else L.f64(pert, 3 * self.nfil)
+        L.check(self.h, L.lib().cfm_fil_init(self.h, ctypes.byref(self._P), L.ptr(self._q0),
+                                              L.ptr(pert_arr)))
         self.nt = 0
 
     @property

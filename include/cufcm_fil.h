@@ -179,7 +179,8 @@ int cfm_system_apply(cfm_ctx* ctx, int32_t system, int32_t precision, const doub
  * n_seg = nfil * nseg. */
 
 #define CFM_GUESS_PREVIOUS 0   /* blob-force GMRES guess = previous solution (mobility_solver.cpp:2296-2388) */
-#define CFM_GUESS_LEARNED 1    /* history projection + learned near-field preconditioner correction */
+#define CFM_GUESS_LEARNED 1    /* history projection; before any history exists, the network's guess
+                                  (W + Z c) b with the learned coarse space (CFM_COARSE_LEARNED) */
 #define CFM_GUESS_PROJECTED 2  /* minimal-residual combination of the last 16 (rhs, solution) pairs
                                   (the blob matrix is constant while the body is held fixed) */
 
@@ -325,7 +326,8 @@ int cfm_nearfield_coarse(cfm_ctx* ctx, int32_t mode);
  * solve still checks its FP64 residual against rtol. */
 int cfm_gnn_init(cfm_ctx* ctx, uint32_t seed, int32_t capacity);   /* capacity: training pairs kept */
 int cfm_gnn_num_params(cfm_ctx* ctx);
-int cfm_gnn_record(cfm_ctx* ctx, int32_t on);          /* record every blob solve of cfm_fil_step */
+int cfm_gnn_record(cfm_ctx* ctx, int32_t on);          /* record (rhs, solution) of every blob solve of
+                                                          cfm_fil_step */
 int cfm_gnn_samples(cfm_ctx* ctx);                     /* recorded pairs available */
 int cfm_gnn_add_samples(cfm_ctx* ctx, int32_t n, const double* rhs, const double* sol); /* AoS [n][3 n_blob] */
 int cfm_gnn_train(cfm_ctx* ctx, int32_t epochs, int32_t batch, double lr, double* loss_before,

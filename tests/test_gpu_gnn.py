@@ -85,7 +85,7 @@ def test_training_lowers_loss_and_gate_never_costs_iterations():
         for nt in range(3):
             S.step(nt)
         S.mob.gnn_record(False)
-        assert S.mob.gnn_samples() >= 64
+        assert S.mob.gnn_samples() >= 30     # one (rhs, solution) pair per blob solve
         rng = np.random.default_rng(3)
         rhs = [-S.mob.evaluate_blob_segment_mobility(rng.standard_normal(6 * S.n_seg))
                for _ in range(4)] + [rng.standard_normal(3 * S.n_blob)]
