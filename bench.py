@@ -206,6 +206,9 @@ def run_learned(args):
                 S.restart(perturbation(S.nfil, 7 + r))
             for nt in range(steps):
                 S.step(nt)
+                if (nt + 1) % 50 == 0:
+                    print(f"[bench] learned section: trajectory {r + 1}/{runs} step {nt + 1}, "
+                          f"{m.gnn_samples()} pairs recorded", file=sys.stderr, flush=True)
         m.gnn_record(False)
         out["record_seconds"] = round(time.perf_counter() - t0, 2)
         samples = m.gnn_samples()
