@@ -150,6 +150,17 @@ class cfm_cilia_params(ctypes.Structure):
     ]
 
 
+class cfm_squirmer_params(ctypes.Structure):
+    _fields_ = [
+        ("n_modes", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("A", ctypes.c_double * 8),
+        ("B", ctypes.c_double * 8),
+        ("dt", ctypes.c_double),
+        ("solve", cfm_solve_opts),
+    ]
+
+
 class cfm_batch_config(ctypes.Structure):
     _fields_ = [
         ("n_sys", ctypes.c_int32),
@@ -256,6 +267,8 @@ def lib():
     L.cfm_gemm_bf16.argtypes = [_vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _fp,
                                 ctypes.c_int32, _fp, ctypes.c_int32, _fp]
     L.cfm_nearfield_coarse.argtypes = [_vp, ctypes.c_int32]
+    L.cfm_squirmer_init.argtypes = [_vp, ctypes.POINTER(cfm_squirmer_params)]
+    L.cfm_squirmer_step.argtypes = [_vp, ctypes.c_int32, _dp, ctypes.POINTER(cfm_solve_stats)]
     L.cfm_gnn_train_coarse.argtypes = [_vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_double, _dp, _dp]
     L.cfm_gnn_coarse_gradient.argtypes = [_vp, ctypes.c_int32, ctypes.c_int32, _fp, _dp, _dp]
     L.cfm_gnn_coarse_vectors.argtypes = [_vp, _dp]

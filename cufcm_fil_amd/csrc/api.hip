@@ -169,6 +169,7 @@ void cfm_gnn_release(cfm_ctx* c);
 void cfm_cilia_release(cfm_ctx* c);
 void cfm_body_release(cfm_ctx* c);
 namespace cfm {
+void cfm_squirmer_release(cfm_ctx* c);
 void body_positions_changed(cfm_ctx* c);
 }
 
@@ -268,6 +269,7 @@ int cfm_destroy(cfm_ctx* c) {
   cfm_fil_release(c);
   cfm_gnn_release(c);
   cfm_cilia_release(c);
+  cfm::cfm_squirmer_release(c);
   cfm_body_release(c);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   if (c->rocblas) (void)rocblas_destroy_handle((rocblas_handle)c->rocblas);

@@ -667,6 +667,7 @@ Status body_info(cfm_ctx* c, BodyInfo* out) {
   out->nb = S->nb;
   out->x = S->x;
   out->q = S->q;
+  out->ref = S->ref;
   std::copy(S->mob, S->mob + 36, out->mob);
   return {};
 }

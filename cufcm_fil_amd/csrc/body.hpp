@@ -9,6 +9,7 @@ namespace cfm {
 struct BodyInfo {
   int n_body = 0, nb = 0;    // bodies, blobs per body
   std::vector<double> x, q;  // centres (3 n_body), quaternions (4 n_body) of the last placement
+  std::vector<double> ref;   // body-frame offsets of one body's blobs (3 nb, AoS)
   double mob[36];            // body-frame body mobility (identity until made or set)
 };
 

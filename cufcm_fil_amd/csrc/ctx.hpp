@@ -221,6 +221,7 @@ struct cfm_ctx {
   void* gnn = nullptr;                  // learned initial guess (gnn.hip)
   void* cilia = nullptr;                // prescribed-shape cilia (cilia.hip)
   void* body = nullptr;                 // free rigid bodies made of the blobs (body.hip)
+  void* squirmer = nullptr;             // squirmer slip driving of the bodies (squirmer.hip)
   double* scratch = nullptr;            // this device's copy of the allocation-free entry
                                         // point's static scratch array (noalloc.hip)
   std::string err;

@@ -283,6 +283,35 @@ class RpyMobilitySolver:
                                                   ctypes.byref(loss)))
         return loss.value, g
 
+    # -- squirmer bodies (blobs only; bodies from body_init / body_set_state) ---------
+    def squirmer_init(self, B, A=None, *, dt=1e-3, precond="dense", mode="fp64", rtol=1e-10,
+                      restart=50, max_iter=2000):
+        """Slip modes B_n (tangential) and A_n (radial), n = 1.. (at most 8)."""
+        B = list(np.atleast_1d(np.asarray(B, dtype=np.float64)))
+        A = [0.0] * len(B) if A is None else list(np.atleast_1d(np.asarray(A, dtype=np.float64)))
+        n = max(len(A), len(B))
+        P = L.cfm_squirmer_params()
+        P.n_modes = n
+        for k in range(n):
+            P.A[k] = A[k] if k < len(A) else 0.0
+            P.B[k] = B[k] if k < len(B) else 0.0
+        P.dt = dt
+        P.solve = L.cfm_solve_opts(system=L.CFM_SYS_BODY, precond=self._PC[precond],
+                                   mode=self._MODE[mode], restart=restart, max_iter=max_iter,
+                                   use_guess=1, rtol=rtol)
+        L.check(self._h, L.lib().cfm_squirmer_init(self._h, ctypes.byref(P)))
+
+    def squirmer_step(self, nt):
+        """One step; returns ((V, Omega) per body as an (n_body, 6) array, solve stats)."""
+        nb = self._n_body_cached()
+        vel = np.zeros(6 * nb)
+        st = L.cfm_solve_stats()
+        L.check(self._h, L.lib().cfm_squirmer_step(self._h, int(nt), L.ptr(vel), ctypes.byref(st)))
+        return vel.reshape(nb, 6), st.as_dict()
+
+    def _n_body_cached(self):
+        return getattr(self, "_n_body", 1)
+
     # -- coarse correction of the near-field preconditioner --------------------------
     _COARSE = {"none": L.CFM_COARSE_NONE, "linear": L.CFM_COARSE_LINEAR,
                "learned": L.CFM_COARSE_LEARNED}

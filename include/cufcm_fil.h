@@ -500,6 +500,31 @@ int cfm_body_reference_matrices(cfm_ctx* ctx, int32_t body, const cfm_solve_opts
 int cfm_body_set_reference_matrices(cfm_ctx* ctx, const double* ktk_inv, const double* mobility);
 int cfm_body_get_reference_matrices(cfm_ctx* ctx, double* ktk_inv, double* mobility);
 
+/* ---- squirmer bodies ------------------------------------------------------------------
+ * The reference's CILIA_TYPE 4 (NO_CILIA_SQUIRMER, config.hpp:87,407): bodies of blobs
+ * without cilia, driven by a surface slip velocity (mobility_solver.cpp:470-505,
+ * util.cu:69-115).  For a blob at body-frame offset r (polar angle theta from the body's
+ * z axis, azimuth phi): u_r = sum_n A_n P_n(cos theta), u_theta = sum_n B_n V_n(cos theta)
+ * with V_n(x) = 2 sin(theta) P_n'(x) / (n (n + 1)), in Cartesian components turned by
+ * R(q_b).  (The reference evaluates this slip but leaves its addition to the right-hand
+ * side commented out; here it is applied.)  A lone spherical squirmer swims along its
+ * axis at 2 B_1 / 3.  The context holds blobs only, with the bodies of cfm_body_init /
+ * cfm_body_set_state.  Each cfm_squirmer_step solves the force- and torque-free body
+ * system M F - K U = u_s, -K^T F = 0 (CFM_SYS_BODY with opts `solve`; the previous
+ * solution is the guess) and, from the second call on, first advances every body with
+ * the previous velocities, x += dt V, q = exp(dt Omega) q (cilia_sim_main.cu:724-740);
+ * vel receives (V, Omega) per body (6 n_body, may be null). */
+typedef struct {
+  int32_t n_modes;      /* 1..8 */
+  int32_t reserved;
+  double A[8];          /* radial modes A_n, n = 1..n_modes */
+  double B[8];          /* tangential modes B_n */
+  double dt;
+  cfm_solve_opts solve;
+} cfm_squirmer_params;
+int cfm_squirmer_init(cfm_ctx* ctx, const cfm_squirmer_params* p);
+int cfm_squirmer_step(cfm_ctx* ctx, int32_t nt, double* vel, cfm_solve_stats* st);
+
 /* ---- batched ensembles of independent wall systems ----------------------------------
  * No counterpart inside the reference's mobility_solver: the reference runs each
  * member of an ensemble (parameter sweep, bisection, periodic-orbit search) as its

@@ -141,3 +141,51 @@ def faxen_brenner_ratios(R, h):
     par = 1.0 - 9.0 / 16.0 * s + 1.0 / 8.0 * s ** 3 - 45.0 / 256.0 * s ** 4 - 1.0 / 16.0 * s ** 5
     nor = 1.0 - 9.0 / 8.0 * s + 1.0 / 2.0 * s ** 3 - 57.0 / 100.0 * s ** 4 + 1.0 / 5.0 * s ** 5
     return par, nor
+
+
+# ---------------------------------------------------------------- squirmer bodies
+def legendre(n, x):
+    """P_n(x) and P_n'(x) by the three-term recurrences (util.cu:87-100 restates P_n;
+    the derivative here is exact where the reference differences P_n numerically)."""
+    p0, p1 = 1.0, x
+    d0, d1 = 0.0, 1.0
+    if n == 0:
+        return p0, d0
+    for k in range(2, n + 1):
+        p0, p1 = p1, ((2 * k - 1) * x * p1 - (k - 1) * p0) / k
+        d0, d1 = d1, d0 + (2 * k - 1) * p0
+    return p1, d1
+
+
+def squirmer_slip(ref, A, B):
+    """Body-frame slip velocity of each blob (mobility_solver.cpp:470-505 with
+    util.cu:69-115): theta, phi the polar and azimuthal angles of the body-frame offset,
+    u_r = sum_n A_n P_n(cos theta), u_theta = sum_n B_n V_n(cos theta) with
+    V_n(x) = 2 sin(theta) P_n'(x) / (n (n + 1)), mapped to Cartesian components."""
+    ref = np.asarray(ref, dtype=np.float64).reshape(-1, 3)
+    out = np.zeros_like(ref)
+    for i, r in enumerate(ref):
+        phi = math.atan2(r[1], r[0])
+        th = math.acos(r[2] / np.linalg.norm(r))
+        x, s = math.cos(th), math.sin(th)
+        ur = ut = 0.0
+        for n in range(1, len(A) + 1):
+            p, dp = legendre(n, x)
+            ur += A[n - 1] * p
+            ut += B[n - 1] * 2.0 * s * dp / (n * (n + 1))
+        out[i] = [ur * s * math.cos(phi) + ut * x * math.cos(phi),
+                  ur * s * math.sin(phi) + ut * x * math.sin(phi),
+                  ur * x - ut * s]
+    return out
+
+
+def squirmer_velocity(M, K, slip_lab):
+    """Force- and torque-free bodies whose blobs move with the rigid velocity plus the slip:
+    M F - K U = u_s, -K^T F = 0 (the body system with the slip as blob right-hand side).
+    Returns the body velocities (V, Omega) per body and the blob forces."""
+    A = saddle_matrix(M, K)
+    n = K.shape[0]
+    rhs = np.zeros(A.shape[0])
+    rhs[:n] = np.asarray(slip_lab).reshape(-1)
+    z = np.linalg.solve(A, rhs)
+    return z[n:].reshape(-1, 6), z[:n]

@@ -102,3 +102,49 @@ def test_blob_sphere_near_the_wall_follows_faxen_and_brenner():
     assert abs(Nw[0, 0] / N0[0, 0] / par - 1) < 0.01
     assert abs(Nw[1, 1] / N0[1, 1] / par - 1) < 0.01
     assert abs(Nw[2, 2] / N0[2, 2] / nor - 1) < 0.03
+
+
+def test_legendre_helpers_match_closed_forms():
+    """P_n and P_n' against the closed forms of degrees 2-4, and the squirmer's tangential
+    modes V_1 = sin(theta), V_2 = sin(theta) cos(theta)."""
+    for x in (-0.9, -0.3, 0.2, 0.75):
+        for n, p, dp in ((2, 0.5 * (3 * x * x - 1), 3 * x), (3, 0.5 * (5 * x ** 3 - 3 * x), 0.5 * (15 * x * x - 3)),
+                         (4, (35 * x ** 4 - 30 * x * x + 3) / 8, (140 * x ** 3 - 60 * x) / 8)):
+            got = bo.legendre(n, x)
+            assert np.allclose(got, (p, dp), rtol=1e-13, atol=1e-14)
+    th = 0.7
+    ref = np.array([[math.sin(th), 0.0, math.cos(th)]])
+    u1 = bo.squirmer_slip(ref, [0.0], [1.0])[0]          # u_theta = sin(theta) along e_theta
+    e_th = np.array([math.cos(th), 0.0, -math.sin(th)])
+    assert np.allclose(u1, math.sin(th) * e_th, atol=1e-15)
+    u2 = bo.squirmer_slip(ref, [0.0, 0.0], [0.0, 1.0])[0]
+    assert np.allclose(u2, math.sin(th) * math.cos(th) * e_th, atol=1e-15)
+    ur = bo.squirmer_slip(ref, [2.0], [0.0])[0]          # u_r = A_1 cos(theta) along e_r
+    assert np.allclose(ur, 2.0 * math.cos(th) * ref[0], atol=1e-15)
+
+
+def test_blob_squirmer_swims_at_two_thirds_b1():
+    """A spherical squirmer with tangential slip B_1 sin(theta) swims along its axis at
+    2 B_1 / 3 (Lighthill 1952, Blake 1971), without rotating, whatever B_2 is; B_2 alone
+    does not swim.  A blob sphere of radius R reproduces this up to a correction of order
+    a / R (the slip sits at the blob centres): U = 0.808, 0.757 at R = 8, 12.5 (400, 1,000
+    blobs), and the 1/R extrapolation lands within 1% of 2/3 (0.665)."""
+    Us = []
+    for n, R in [(400, 8.0), (1000, 12.5)]:
+        ref = geo.fibonacci_sphere(n, R)
+        pos, off = bo.place(ref, np.zeros((1, 3)), np.array([[1.0, 0, 0, 0]]))
+        M = oracle.mobility_matrix(pos, 3, wall=False)
+        K = bo.k_matrix(off, 1)
+        U1, _ = bo.squirmer_velocity(M, K, bo.squirmer_slip(ref, [0.0, 0.0], [1.0, 0.0]))
+        U12, _ = bo.squirmer_velocity(M, K, bo.squirmer_slip(ref, [0.0, 0.0], [1.0, 3.0]))
+        U2, _ = bo.squirmer_velocity(M, K, bo.squirmer_slip(ref, [0.0, 0.0], [0.0, 1.0]))
+        assert abs(U1[0, 2] - U12[0, 2]) < 1e-3
+        assert np.abs(U2[0]).max() < 2e-4
+        assert np.abs(U1[0, [0, 1, 3, 4, 5]]).max() < 5e-4
+        Us.append((R, U1[0, 2]))
+    (R1, u1), (R2, u2) = Us
+    c = (u1 - u2) / (1.0 / R1 - 1.0 / R2)
+    u_inf = u2 - c / R2
+    assert abs(u_inf / (2.0 / 3.0) - 1.0) < 0.01, (Us, u_inf)
+    # the sign of e_theta is pinned: the body swims towards its north pole (+z)
+    assert u1 > u2 > 2.0 / 3.0
