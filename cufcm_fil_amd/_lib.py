@@ -25,6 +25,7 @@ CFM_SYS_SEG, CFM_SYS_BLOB, CFM_SYS_BODY = 0, 1, 2
 CFM_PC_NONE, CFM_PC_DIAG, CFM_PC_SELF, CFM_PC_NEAR, CFM_PC_DENSE = 0, 1, 2, 3, 4
 CFM_SOLVE_FP64, CFM_SOLVE_MIXED = 0, 1
 CFM_COARSE_NONE, CFM_COARSE_LINEAR, CFM_COARSE_LEARNED = 0, 1, 2
+CFM_BODY_FREE, CFM_BODY_PRESCRIBED_VELOCITY, CFM_BODY_PRESCRIBED_ROTATION = 0, 1, 2
 
 
 class CfmError(RuntimeError):
@@ -238,6 +239,8 @@ def lib():
                                               _dp, _dp, ctypes.POINTER(cfm_solve_stats)]
     L.cfm_body_set_reference_matrices.argtypes = [_vp, _dp, _dp]
     L.cfm_body_get_reference_matrices.argtypes = [_vp, _dp, _dp]
+    L.cfm_body_set_motion.argtypes = [_vp, ctypes.c_int32]
+    L.cfm_body_get_motion.argtypes = [_vp, ctypes.POINTER(ctypes.c_int32)]
     L.cfm_fil_init.argtypes = [_vp, ctypes.POINTER(cfm_fil_params), _dp, _dp]
     L.cfm_fil_step.argtypes = [_vp, ctypes.c_int32, ctypes.POINTER(cfm_step_stats)]
     L.cfm_fil_get_state.argtypes = [_vp, _dp, _dp, _dp, _dp, _dp]

@@ -179,12 +179,14 @@ class CiliatedBodies:
     cilia; body_x, body_q: centres and unit quaternions (w, x, y, z).  Phases and phase
     speeds cover all n_body x per cilia.  Every step solves for the forces, the bodies'
     velocities and, with set_driving, the phase (and shape-angle) speeds; later steps
-    first move phases and bodies explicitly with the previous step's speeds."""
+    first move phases and bodies explicitly with the previous step's speeds.
+    body_motion "prescribed_rotation" holds the bodies' rotation at zero (the reference's
+    BODY_VELOCITY_TYPE 2)."""
 
     def __init__(self, blob_ref, fil_ref, fil_q, body_x, body_q, phase=None, phase_dot=None, *,
                  nseg=20, dl=2.6, dt=1.0 / 300, wall=False, precision="fp64", solve_mode="fp64",
                  precond="diag", rtol=1e-10, restart=50, max_iter=2000, device=0, tile_split=0,
-                 beat="fulford_blake", s_tol=0.0):
+                 beat="fulford_blake", s_tol=0.0, body_motion="free"):
         blob_ref = L.f64(blob_ref)
         fil_ref, fil_q = L.f64(fil_ref), L.f64(fil_q)
         body_x, body_q = L.f64(body_x), L.f64(body_q)
@@ -203,6 +205,7 @@ class CiliatedBodies:
         self.mob.set_positions(x_seg, x_blob)
         self.mob.body_init(blob_ref, nbody)
         self.mob.body_set_state(body_x, body_q)
+        self.mob.body_set_motion(body_motion)
         o = L.cfm_solve_opts(system=L.CFM_SYS_SEG, precond=RpyMobilitySolver._PC[precond],
                              mode=RpyMobilitySolver._MODE[solve_mode], restart=restart,
                              max_iter=max_iter, use_guess=1, rtol=rtol)

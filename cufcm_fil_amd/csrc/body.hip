@@ -21,6 +21,17 @@
 // make_body_reference_matrices (mobility_solver.cpp:3049-3146) is
 // cfm_body_reference_matrices: K^T K from the offsets, and N from six solves of one
 // body alone with a unit force or torque.
+//
+// Prescribed rotation (BODY_VELOCITY_TYPE 2, cfm_body_set_motion): the reference drops
+// the Omega columns from the blob rows and the torque rows from the product
+// (mobility_solver.cpp:1947-1994, 2177-2181) and from the preconditioner (:1268-1291,
+// 1506-1512).  Here the unknown vector keeps its 6 entries per body; the blob rows see
+// only V_b, and the torque rows are identity rows on Omega_b, so the solve returns
+// Omega_b = the torque entries of the right-hand side (zero, as the reference
+// prescribes) and the blob forces and V_b of the reduced system [[M, -K_V], [-K_V^T, 0]].
+// The preconditioners invert the same reduction: CFM_PC_DIAG / _SELF with
+// (K_V^T K_V)^-1 = I / nb and the clamped translational mobility ((N^-1)_VV)^-1,
+// CFM_PC_DENSE with the Schur complement's translation block (K_V^T P K_V)^-1.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -66,6 +77,9 @@ struct BodyState {
   DevBuf<double> rin, rout;       // rotated system vectors
   DevBuf<double> st;              // staging of an embedded blob block (3 n_blob + 6 n_body)
   std::vector<double> R0;         // K^T P K at the build, (6 n_body)^2 row-major
+  // Prescribed rotation (the reference's BODY_VELOCITY_TYPE 2): Omega_b = 0 is not an
+  // unknown; its columns leave the blob rows and each torque row reads Omega_b itself.
+  bool rot_fixed = false;
 };
 
 BodyState* body_of(cfm_ctx* c) { return reinterpret_cast<BodyState*>(c->body); }
@@ -158,12 +172,19 @@ void ktk(const std::vector<double>& ref, double A[36]) {
 // ---------------------------------------------------------------- kernels
 
 // y_F -= K U for every blob: V_b + Omega_b x r_i (mobility_solver.cpp:1838-1866, -K mult).
-__global__ __launch_bounds__(256) void body_k_kernel(int m, int nb, const double* __restrict__ r,
+// rot: prescribed rotation, V_b only.
+__global__ __launch_bounds__(256) void body_k_kernel(int m, int nb, int rot, const double* __restrict__ r,
                                                      const double* __restrict__ x,
                                                      double* __restrict__ y) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m) return;
   const double* U = x + 3 * (size_t)m + 6 * (size_t)(i / nb);
+  if (rot) {
+    y[i] -= U[0];
+    y[m + i] -= U[1];
+    y[2 * (size_t)m + i] -= U[2];
+    return;
+  }
   const double rx = r[i], ry = r[m + i], rz = r[2 * (size_t)m + i];
   y[i] -= U[0] + (U[4] * rz - U[5] * ry);
   y[m + i] -= U[1] + (U[5] * rx - U[3] * rz);
@@ -175,7 +196,9 @@ __global__ __launch_bounds__(256) void body_k_kernel(int m, int nb, const double
 //   mode 0: outU = -t                                   (system product, -K^T mult)
 //   mode 1: outU = Kinv_b (-t) - N_b inU                (block preconditioner)
 //   mode 2: outU = -inU - t                             (Schur right-hand side)
-__global__ __launch_bounds__(256) void body_kt_kernel(int mode, int m, int nb,
+// With rot (prescribed rotation) the torque rows of modes 0 and 2 are inU_Omega and
+// -inU_Omega (mode 1 takes the reduced matrices, whose Omega rows give inU_Omega).
+__global__ __launch_bounds__(256) void body_kt_kernel(int mode, int rot, int m, int nb,
                                                       const double* __restrict__ r,
                                                       const double* __restrict__ F,
                                                       const double* __restrict__ inU,
@@ -209,7 +232,7 @@ __global__ __launch_bounds__(256) void body_kt_kernel(int mode, int m, int nb,
 #pragma unroll
   for (int j = 0; j < 6; ++j) t[j] = (red[0][j] + red[1][j]) + (red[2][j] + red[3][j]);
   if (mode == 0) {
-    outU[6 * b + k] = -t[k];
+    outU[6 * b + k] = (rot && k >= 3) ? inU[6 * b + k] : -t[k];
   } else if (mode == 1) {
     const double* Ki = mats + 72 * (size_t)b;
     const double* N = Ki + 36;
@@ -217,12 +240,13 @@ __global__ __launch_bounds__(256) void body_kt_kernel(int mode, int m, int nb,
     for (int j = 0; j < 6; ++j) v -= Ki[6 * k + j] * t[j] + N[6 * k + j] * inU[6 * b + j];
     outU[6 * b + k] = v;
   } else {
-    outU[6 * b + k] = -inU[6 * b + k] - t[k];
+    outU[6 * b + k] = (rot && k >= 3) ? -inU[6 * b + k] : -inU[6 * b + k] - t[k];
   }
 }
 
-// out_F = P_i (f_i + V_b + Omega_b x r_i), P_i the particle preconditioner of blob i.
-__global__ __launch_bounds__(256) void body_pc_blob_kernel(int kind, int wall, int m, int nb,
+// out_F = P_i (f_i + V_b + Omega_b x r_i), P_i the particle preconditioner of blob i
+// (rot: P_i (f_i + V_b)).
+__global__ __launch_bounds__(256) void body_pc_blob_kernel(int kind, int wall, int rot, int m, int nb,
                                                            const double* __restrict__ r,
                                                            const double* __restrict__ z,
                                                            double a, double mu,
@@ -232,10 +256,11 @@ __global__ __launch_bounds__(256) void body_pc_blob_kernel(int kind, int wall, i
   if (i >= m) return;
   const double* U = out + 3 * (size_t)m + 6 * (size_t)(i / nb);
   const double rx = r[i], ry = r[m + i], rz = r[2 * (size_t)m + i];
+  const double w0 = rot ? 0.0 : U[3], w1 = rot ? 0.0 : U[4], w2 = rot ? 0.0 : U[5];
   double v[3], o[3];
-  v[0] = in[i] + U[0] + (U[4] * rz - U[5] * ry);
-  v[1] = in[m + i] + U[1] + (U[5] * rx - U[3] * rz);
-  v[2] = in[2 * (size_t)m + i] + U[2] + (U[3] * ry - U[4] * rx);
+  v[0] = in[i] + U[0] + (w1 * rz - w2 * ry);
+  v[1] = in[m + i] + U[1] + (w2 * rx - w0 * rz);
+  v[2] = in[2 * (size_t)m + i] + U[2] + (w0 * ry - w1 * rx);
   particle_precond(kind, wall, 3, z[i], a, mu, v, o);
   out[i] = o[0];
   out[m + i] = o[1];
@@ -276,8 +301,8 @@ __global__ __launch_bounds__(256) void body_small_gemv_kernel(int n, const doubl
   out[row] = s;
 }
 
-// out_F[j] = w[j] + sum_c PK[c][j] U[c].
-__global__ __launch_bounds__(256) void body_dense_finish_kernel(int n3, int ncol,
+// out_F[j] = w[j] + sum_c PK[c][j] U[c] (rot: over the V columns only).
+__global__ __launch_bounds__(256) void body_dense_finish_kernel(int n3, int ncol, int rot,
                                                                 const double* __restrict__ w,
                                                                 const double* __restrict__ PK,
                                                                 const double* __restrict__ U,
@@ -285,7 +310,8 @@ __global__ __launch_bounds__(256) void body_dense_finish_kernel(int n3, int ncol
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n3) return;
   double s = w[j];
-  for (int cc = 0; cc < ncol; ++cc) s += PK[(size_t)cc * n3 + j] * U[cc];
+  for (int cc = 0; cc < ncol; ++cc)
+    if (!rot || cc % 6 < 3) s += PK[(size_t)cc * n3 + j] * U[cc];
   out[j] = s;
 }
 
@@ -388,19 +414,42 @@ __global__ void body_mats_kernel(int nbody, const double* __restrict__ frame,
   mats[72 * (size_t)b + e] = v;
 }
 
+// Body-frame matrices of the block preconditioner: (K^T K)^-1 and N, or with the
+// rotation prescribed, diag(I / nb, 0) and diag(((N^-1)_VV)^-1, -I) (mode 1 of
+// body_kt_kernel then returns V from the translation rows and Omega = inU_Omega).
+Status frame_mats(const BodyState* S, double fr[72]) {
+  if (!S->rot_fixed) {
+    std::copy(S->ktk_inv, S->ktk_inv + 36, fr);
+    std::copy(S->mob, S->mob + 36, fr + 36);
+    return {};
+  }
+  std::fill(fr, fr + 72, 0.0);
+  for (int k = 0; k < 3; ++k) fr[7 * k] = 1.0 / S->nb;
+  std::vector<double> Ni(S->mob, S->mob + 36);
+  if (!invert(Ni, 6)) return make_status(CFM_ENONFINITE, "singular body mobility");
+  std::vector<double> Nv(9);
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) Nv[3 * i + j] = Ni[6 * i + j];
+  if (!invert(Nv, 3)) return make_status(CFM_ENONFINITE, "singular translational resistance");
+  for (int i = 0; i < 3; ++i) {
+    for (int j = 0; j < 3; ++j) fr[36 + 6 * i + j] = Nv[3 * i + j];
+    fr[36 + 6 * (3 + i) + 3 + i] = -1.0;
+  }
+  return {};
+}
+
 Status upload_mats(cfm_ctx* c, BodyState* S) {
+  double fr[72];
+  CFM_TRY(frame_mats(S, fr));
   std::vector<double> h((size_t)72 * S->n_body);
   for (int b = 0; b < S->n_body; ++b) {
     double R[9];
     rot_mat(&S->q[4 * b], R);
-    rotate6(S->ktk_inv, R, false, &h[72 * (size_t)b]);
-    rotate6(S->mob, R, false, &h[72 * (size_t)b + 36]);
+    rotate6(fr, R, false, &h[72 * (size_t)b]);
+    rotate6(fr + 36, R, false, &h[72 * (size_t)b + 36]);
   }
   CFM_TRY(S->mats.alloc(h.size()));
   CFM_HIP(hipMemcpyAsync(S->mats.p, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice, c->stream));
-  double fr[72];
-  std::copy(S->ktk_inv, S->ktk_inv + 36, fr);
-  std::copy(S->mob, S->mob + 36, fr + 36);
   CFM_TRY(S->frame.alloc(72));
   CFM_HIP(hipMemcpyAsync(S->frame.p, fr, sizeof(fr), hipMemcpyHostToDevice, c->stream));
   CFM_HIP(hipStreamSynchronize(c->stream));
@@ -427,7 +476,7 @@ Status build_schur(cfm_ctx* c, BodyState* S) {
     CFM_CHECK_LAUNCH();
     CFM_TRY(dense_apply(c, c->dinv, S->kcol.p, S->PK.p + (size_t)col * n3));
     // column col of -K^T P K
-    hipLaunchKernelGGL(body_kt_kernel, dim3(S->n_body), dim3(256), 0, c->stream, 0, m, S->nb, S->r.p,
+    hipLaunchKernelGGL(body_kt_kernel, dim3(S->n_body), dim3(256), 0, c->stream, 0, 0, m, S->nb, S->r.p,
                        S->PK.p + (size_t)col * n3, (const double*)nullptr, (const double*)nullptr,
                        St.p + (size_t)col * nc);
     CFM_CHECK_LAUNCH();
@@ -441,7 +490,21 @@ Status build_schur(cfm_ctx* c, BodyState* S) {
     for (int j = 0; j < nc; ++j)
       Sm[(size_t)i * nc + j] = -0.5 * (A[(size_t)j * nc + i] + A[(size_t)i * nc + j]);
   S->R0 = Sm;
-  if (!invert(Sm, nc)) return make_status(CFM_ENONFINITE, "singular body Schur complement K^T P K");
+  if (S->rot_fixed) {
+    // translation block (K_V^T P K_V)^-1; Omega rows -I against the -inU_Omega of mode 2
+    const int nv = 3 * S->n_body;
+    std::vector<double> T((size_t)nv * nv);
+    for (int i = 0; i < nv; ++i)
+      for (int j = 0; j < nv; ++j) T[(size_t)i * nv + j] = Sm[(size_t)(6 * (i / 3) + i % 3) * nc + 6 * (j / 3) + j % 3];
+    if (!invert(T, nv)) return make_status(CFM_ENONFINITE, "singular body Schur complement K_V^T P K_V");
+    std::fill(Sm.begin(), Sm.end(), 0.0);
+    for (int i = 0; i < nv; ++i)
+      for (int j = 0; j < nv; ++j) Sm[(size_t)(6 * (i / 3) + i % 3) * nc + 6 * (j / 3) + j % 3] = T[(size_t)i * nv + j];
+    for (int b = 0; b < S->n_body; ++b)
+      for (int k = 3; k < 6; ++k) Sm[(size_t)(6 * b + k) * nc + 6 * b + k] = -1.0;
+  } else if (!invert(Sm, nc)) {
+    return make_status(CFM_ENONFINITE, "singular body Schur complement K^T P K");
+  }
   CFM_HIP(hipMemcpyAsync(S->Sinv.p, Sm.data(), sizeof(double) * Sm.size(), hipMemcpyHostToDevice, c->stream));
   CFM_HIP(hipStreamSynchronize(c->stream));
   S->schur_gen = c->dinv.generation;
@@ -470,10 +533,10 @@ Status body_system_desc(cfm_ctx* c, SystemDesc* s) {
 Status body_couple_apply(cfm_ctx* c, const double* x, double* y) {
   BodyState* S = body_of(c);
   if (!S || !S->placed) return make_status(CFM_ESTATE, "no placed bodies");
-  const int m = c->p.n_blob;
-  hipLaunchKernelGGL(body_kt_kernel, dim3(S->n_body), dim3(256), 0, c->stream, 0, m, S->nb, S->r.p, x,
-                     (const double*)nullptr, (const double*)nullptr, y + 3 * (size_t)m);
-  hipLaunchKernelGGL(body_k_kernel, g1(m), dim3(256), 0, c->stream, m, S->nb, S->r.p, x, y);
+  const int m = c->p.n_blob, rot = S->rot_fixed;
+  hipLaunchKernelGGL(body_kt_kernel, dim3(S->n_body), dim3(256), 0, c->stream, 0, rot, m, S->nb, S->r.p, x,
+                     x + 3 * (size_t)m, (const double*)nullptr, y + 3 * (size_t)m);
+  hipLaunchKernelGGL(body_k_kernel, g1(m), dim3(256), 0, c->stream, m, S->nb, rot, S->r.p, x, y);
   CFM_CHECK_LAUNCH();
   return {};
 }
@@ -533,7 +596,7 @@ Status body_dense_blob_apply(cfm_ctx* c, const double* x, size_t ldx, double* y,
   return dev_copy_rows(c, out, m, 0, y, ldy, 0, 3, m);
 }
 
-// y_b += P K U (U: 6 per body, in the current frame).
+// y_b += P K U (U: 6 per body, in the current frame; prescribed rotation: P K_V V).
 Status body_dense_rigid_add(cfm_ctx* c, const double* U, double* y, size_t ldy) {
   BodyState* S = body_of(c);
   const int m = c->p.n_blob, nrot = m + 2 * S->n_body, nc = 6 * S->n_body;
@@ -550,8 +613,8 @@ Status body_dense_rigid_add(cfm_ctx* c, const double* U, double* y, size_t ldy) 
     in = S->rin.p;
   }
   // blob part of `in` is zero: out = P K U'
-  hipLaunchKernelGGL(body_dense_finish_kernel, g1(3 * m), dim3(256), 0, c->stream, 3 * m, nc, in, S->PK.p,
-                     in + 3 * (size_t)m, S->rout.p);
+  hipLaunchKernelGGL(body_dense_finish_kernel, g1(3 * m), dim3(256), 0, c->stream, 3 * m, nc,
+                     (int)S->rot_fixed, in, S->PK.p, in + 3 * (size_t)m, S->rout.p);
   CFM_TRY(dev_fill(c, S->rout.p + 3 * (size_t)m, 0.0, nc));
   const double* out = S->rout.p;
   if (S->frozen) {
@@ -617,10 +680,11 @@ Status body_precond(cfm_ctx* c, int kind, const double* x, double* y) {
     CFM_HIP(hipMemcpyAsync(y, x, sizeof(double) * L, hipMemcpyDeviceToDevice, c->stream));
     return {};
   }
+  const int rot = S->rot_fixed;
   if (kind == CFM_PC_DIAG || kind == CFM_PC_SELF) {
-    hipLaunchKernelGGL(body_kt_kernel, dim3(S->n_body), dim3(256), 0, c->stream, 1, m, S->nb, S->r.p, x,
+    hipLaunchKernelGGL(body_kt_kernel, dim3(S->n_body), dim3(256), 0, c->stream, 1, rot, m, S->nb, S->r.p, x,
                        xU, S->mats.p, yU);
-    hipLaunchKernelGGL(body_pc_blob_kernel, g1(m), dim3(256), 0, c->stream, kind, c->cfg.wall, m, S->nb,
+    hipLaunchKernelGGL(body_pc_blob_kernel, g1(m), dim3(256), 0, c->stream, kind, c->cfg.wall, rot, m, S->nb,
                        S->r.p, c->p.z.p + c->p.n_seg, c->cfg.a_blob, c->cfg.mu, x, y);
     CFM_CHECK_LAUNCH();
     return {};
@@ -644,11 +708,11 @@ Status body_precond(cfm_ctx* c, int kind, const double* x, double* y) {
     CFM_TRY(S->w.alloc(3 * (size_t)m));
     CFM_TRY(S->aux.alloc(nc));
     CFM_TRY(dense_apply(c, c->dinv, xin, S->w.p));  // w = P f
-    hipLaunchKernelGGL(body_kt_kernel, dim3(S->n_body), dim3(256), 0, c->stream, 2, m, S->nb, roff,
+    hipLaunchKernelGGL(body_kt_kernel, dim3(S->n_body), dim3(256), 0, c->stream, 2, rot, m, S->nb, roff,
                        S->w.p, xin + 3 * (size_t)m, (const double*)nullptr, S->aux.p);  // -u - K^T w
     hipLaunchKernelGGL(body_small_gemv_kernel, g1(nc), dim3(256), 0, c->stream, nc, S->Sinv.p, S->aux.p,
                        yout + 3 * (size_t)m);
-    hipLaunchKernelGGL(body_dense_finish_kernel, g1(3 * m), dim3(256), 0, c->stream, 3 * m, nc, S->w.p,
+    hipLaunchKernelGGL(body_dense_finish_kernel, g1(3 * m), dim3(256), 0, c->stream, 3 * m, nc, rot, S->w.p,
                        S->PK.p, yout + 3 * (size_t)m, yout);
     if (S->frozen)  // back to the current frame
       hipLaunchKernelGGL(body_rotate_kernel, g1(nrot), dim3(256), 0, c->stream, m, S->nb, S->n_body,
@@ -711,6 +775,27 @@ void body_freeze_dense(cfm_ctx* c, bool on) {
   if (!S) return;
   S->frozen = on;
   S->built_frozen = false;
+}
+
+bool body_rotation_prescribed(cfm_ctx* c) {
+  BodyState* S = body_of(c);
+  return S && S->rot_fixed;
+}
+
+// Prescribed rotation: the torque rows of a body-system right-hand side hold the
+// prescribed Omega_b, zero (the reference has no torque rows).
+__global__ void body_zero_torque_kernel(int nbody, double* __restrict__ u) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < 3 * nbody) u[6 * (t / 3) + 3 + t % 3] = 0.0;
+}
+
+Status body_prescribed_rhs(cfm_ctx* c, double* rhs) {
+  BodyState* S = body_of(c);
+  if (!S || !S->rot_fixed) return {};
+  hipLaunchKernelGGL(body_zero_torque_kernel, g1(3 * S->n_body), dim3(256), 0, c->stream, S->n_body,
+                     rhs + 3 * (size_t)c->p.n_blob);
+  CFM_CHECK_LAUNCH();
+  return {};
 }
 
 // Blob positions changed behind the bodies' back (cfm_set_positions): the body
@@ -836,6 +921,40 @@ int cfm_body_set_state(cfm_ctx* c, const double* x, const double* q) {
   S->schur_gen = -1;  // K changed with the offsets
   S->built_frozen = false;
   S->placed = true;
+  return CFM_OK;
+}
+
+int cfm_body_set_motion(cfm_ctx* c, int32_t type) {
+  if (!c) return CFM_EINVAL;
+  BodyState* S = body_of(c);
+  if (!S) return bfail(c, make_status(CFM_ESTATE, "cfm_body_init has not been called"));
+  if (type == CFM_BODY_PRESCRIBED_VELOCITY)
+    return bfail(c, make_status(CFM_EINVAL, "bodies with prescribed (zero) velocities are the blob system "
+                                            "CFM_SYS_BLOB, not a body motion"));
+  if (type != CFM_BODY_FREE && type != CFM_BODY_PRESCRIBED_ROTATION)
+    return bfail(c, make_status(CFM_EINVAL, "unknown body motion %d", type));
+  const bool rot = type == CFM_BODY_PRESCRIBED_ROTATION;
+  if (rot == S->rot_fixed) return CFM_OK;
+  const bool was = S->rot_fixed;
+  S->rot_fixed = rot;
+  if (S->placed) {
+    (void)hipSetDevice(c->device);
+    Status st = upload_mats(c, S);
+    if (!st.ok()) {
+      S->rot_fixed = was;
+      return bfail(c, st);
+    }
+  }
+  S->schur_gen = -1;  // the dense Schur inverse depends on the motion
+  S->built_frozen = false;
+  return CFM_OK;
+}
+
+int cfm_body_get_motion(cfm_ctx* c, int32_t* type) {
+  if (!c || !type) return CFM_EINVAL;
+  BodyState* S = body_of(c);
+  if (!S) return bfail(c, make_status(CFM_ESTATE, "cfm_body_init has not been called"));
+  *type = S->rot_fixed ? CFM_BODY_PRESCRIBED_ROTATION : CFM_BODY_FREE;
   return CFM_OK;
 }
 

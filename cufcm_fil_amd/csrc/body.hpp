@@ -23,6 +23,11 @@ Status body_place_device(cfm_ctx* c, const double* bx, const double* bq);
 void body_freeze_dense(cfm_ctx* c, bool on);
 // Rotated blob offsets R(q_b) ref of the last placement ([3][n_blob], device), or null.
 const double* body_offsets(cfm_ctx* c);
+// Prescribed rotation (cfm_body_set_motion): Omega_b is held at zero.
+bool body_rotation_prescribed(cfm_ctx* c);
+// Prescribed rotation: zero the torque rows (the prescribed Omega_b) of a body-system
+// right-hand side in device memory; nothing for free bodies.
+Status body_prescribed_rhs(cfm_ctx* c, double* rhs);
 
 // Dense blob block for systems that embed the bodies (cilia on free bodies): the
 // resident inverse P of M_bb, its rigid columns P K and R = K^T P K, built for the

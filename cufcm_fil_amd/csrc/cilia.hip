@@ -659,11 +659,16 @@ __global__ void cilia_advance_angle_kernel(int ncil, double dt, const double* __
 // with r_i = x_i - x_b.  The block preconditioner inverts this saddle matrix exactly
 // with M^-1 replaced by the particle preconditioner P (6 pi mu a: the reference's
 // mobility_solver.cpp:1250-1515 with make_precon_mat, swimmer.cpp:1175-1347).
+// With the bodies' rotation prescribed (cfm_body_set_motion, BODY_VELOCITY_TYPE 2) the
+// Omega_b columns leave the particle rows and the torque rows read Omega_b
+// (mobility_solver.cpp:1947-1994, 2177-2181); the preconditioner keeps the translation
+// block of each body's Schur complement and returns Omega_b = r_Omega.
 
 struct BodyCiliaK {
   int ncil, nseg, nseg_total, nb, nbody, per_body;
   int M;  // particles of the system: segments, then blobs
   int dynamic, dyn_angle;
+  int rot;  // prescribed rotation: Omega_b held at the torque rows' right-hand side (zero)
   const double *px, *py, *pz;  // particle positions
   const double* bx;            // body centres, 3 per body
   const double* vphase;        // [3][n_seg]
@@ -730,9 +735,10 @@ __global__ __launch_bounds__(256) void cb_couple_particles_kernel(BodyCiliaK k, 
   const int b = cb_body(k, i);
   const double* U = x + 3 * M + 6 * (size_t)b;
   const double r0 = k.px[i] - k.bx[3 * b], r1 = k.py[i] - k.bx[3 * b + 1], r2 = k.pz[i] - k.bx[3 * b + 2];
-  double u0 = U[0] + (U[4] * r2 - U[5] * r1);
-  double u1 = U[1] + (U[5] * r0 - U[3] * r2);
-  double u2 = U[2] + (U[3] * r1 - U[4] * r0);
+  const double w0 = k.rot ? 0.0 : U[3], w1 = k.rot ? 0.0 : U[4], w2 = k.rot ? 0.0 : U[5];
+  double u0 = U[0] + (w1 * r2 - w2 * r1);
+  double u1 = U[1] + (w2 * r0 - w0 * r2);
+  double u2 = U[2] + (w0 * r1 - w1 * r0);
   if (k.dynamic && i < k.nseg_total) {
     const size_t ph = 3 * M + 6 * (size_t)k.nbody, ns = k.nseg_total;
     const int c = i / k.nseg;
@@ -791,7 +797,10 @@ __global__ __launch_bounds__(256) void cb_couple_body_kernel(BodyCiliaK k, const
   }
   cb_block_sum<6>(acc);
   if (threadIdx.x == 0)
-    for (int d = 0; d < 6; ++d) y[3 * M + 6 * (size_t)b + d] = -acc[d];
+    for (int d = 0; d < 6; ++d) {
+      const size_t o = 3 * M + 6 * (size_t)b + d;
+      y[o] = (k.rot && d >= 3) ? x[o] : -acc[d];
+    }
 }
 
 // Per cilium: S_cc = (k_phi, k_theta)^T P (k_phi, k_theta) inverted, and
@@ -912,6 +921,10 @@ __global__ __launch_bounds__(256) void cb_pc_body_prep_kernel(BodyCiliaK k) {
           A[6 * a + e] -= o[0] * Bp[a] * Bp[e] + o[1] * (Bp[a] * Ba[e] + Ba[a] * Bp[e]) + o[2] * Ba[a] * Ba[e];
     }
   }
+  if (k.rot && !k.dense)  // translation block; Omega rows -I against t_Omega = -r_Omega
+    for (int a = 0; a < 6; ++a)
+      for (int e = 0; e < 6; ++e)
+        if (a >= 3 || e >= 3) A[6 * a + e] = (a == e) ? -1.0 : 0.0;
   if (!k.dense) cb_invert6(A);
   for (int j = 0; j < 36; ++j) k.pc_b[36 * (size_t)b + j] = A[j];
 }
@@ -931,7 +944,8 @@ __global__ __launch_bounds__(256) void cb_pc_particles_kernel(BodyCiliaK k, int 
   const int b = cb_body(k, i);
   const double* U = out + 3 * M + 6 * (size_t)b;
   const double r0 = k.px[i] - k.bx[3 * b], r1 = k.py[i] - k.bx[3 * b + 1], r2 = k.pz[i] - k.bx[3 * b + 2];
-  double u[3] = {U[0] + (U[4] * r2 - U[5] * r1), U[1] + (U[5] * r0 - U[3] * r2), U[2] + (U[3] * r1 - U[4] * r0)};
+  const double w0 = k.rot ? 0.0 : U[3], w1 = k.rot ? 0.0 : U[4], w2 = k.rot ? 0.0 : U[5];
+  double u[3] = {U[0] + (w1 * r2 - w2 * r1), U[1] + (w2 * r0 - w0 * r2), U[2] + (w0 * r1 - w1 * r0)};
   if (k.dynamic && i < k.nseg_total) {
     const size_t ph = 3 * M + 6 * (size_t)k.nbody, ns = k.nseg_total;
     const int c = i / k.nseg;
@@ -964,7 +978,7 @@ __global__ void cb_pc_cilia_kernel(BodyCiliaK k, int mode, const double* __restr
   const int b = c / k.per_body;
   const double* U = out + 3 * M + 6 * (size_t)b;
   double bp = 0.0, ba = 0.0;
-  for (int d = 0; d < 6; ++d) {
+  for (int d = 0; d < (k.rot ? 3 : 6); ++d) {
     bp += o[3 + d] * U[d];
     ba += o[9 + d] * U[d];
   }
@@ -1004,6 +1018,8 @@ __global__ __launch_bounds__(256) void cb_pc_body_kernel(BodyCiliaK k, const dou
       for (int d = 0; d < 6; ++d) t[d] -= o[3 + d] * wp + o[9 + d] * wa;
     }
   }
+  if (k.rot)
+    for (int d = 3; d < 6; ++d) t[d] = -r[ub + d];
   if (k.dense) {  // the bodies couple through P: one full solve (cb_full_gemv_kernel)
     for (int d = 0; d < 6; ++d) k.tU[6 * (size_t)b + d] = t[d];
     return;
@@ -1216,7 +1232,7 @@ __global__ __launch_bounds__(256) void cb_block_apply_kernel(BodyCiliaK k, int m
   const double* U = out + 3 * M + 6 * (size_t)b;
   const double* W = k.pkc + (size_t)c * 8 * n;
   double v = 0.0;
-  for (int a = 0; a < 6; ++a) v += W[(size_t)a * n + row] * U[a];
+  for (int a = 0; a < (k.rot ? 3 : 6); ++a) v += W[(size_t)a * n + row] * U[a];
   if (k.dynamic) {
     const size_t ph = 3 * M + 6 * (size_t)k.nbody;
     v += W[(size_t)6 * n + row] * out[ph + c];
@@ -1248,7 +1264,7 @@ struct CiliaState {
   DevBuf<double> tU, pc_full;  // CFM_PC_DENSE: body right-hand sides, full Schur inverse
   DevBuf<double> pblk, pkc, dcil;  // CFM_PC_NEAR / _DENSE: per-cilium blocks
   bool pc_ready = false;
-  int pc_kind = -1;
+  int pc_kind = -1, pc_rot = -1;
 };
 
 // First phase-speed slot of the system vector, and the system length.
@@ -1338,6 +1354,7 @@ static BodyCiliaK body_cilia_args(cfm_ctx* c, CiliaState* S, int pc_kind) {
   k.M = c->p.n_seg + c->p.n_blob;
   k.dynamic = S->dynamic ? 1 : 0;
   k.dyn_angle = S->dyn_angle ? 1 : 0;
+  k.rot = body_rotation_prescribed(c) ? 1 : 0;
   k.px = c->p.x.p;
   k.py = c->p.y.p;
   k.pz = c->p.z.p;
@@ -1403,6 +1420,10 @@ static Status body_cilia_prep(cfm_ctx* c, CiliaState* S, int kind) {
     for (int b = 0; b < k.nbody; ++b)
       for (int i = 0; i < 6; ++i)
         for (int j = 0; j < 6; ++j) R[(size_t)(6 * b + i) * n + 6 * b + j] += D[36 * (size_t)b + 6 * i + j];
+    if (k.rot)  // translation block; Omega rows -I against t_Omega = -r_Omega
+      for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j)
+          if (i % 6 >= 3 || j % 6 >= 3) R[(size_t)i * n + j] = (i == j) ? -1.0 : 0.0;
     // symmetric positive definite: Gauss-Jordan with partial pivoting
     std::vector<double> I((size_t)n * n, 0.0);
     for (int i = 0; i < n; ++i) I[(size_t)i * n + i] = 1.0;
@@ -1438,6 +1459,7 @@ static Status body_cilia_prep(cfm_ctx* c, CiliaState* S, int kind) {
   }
   S->pc_ready = true;
   S->pc_kind = kind;
+  S->pc_rot = k.rot;
   return {};
 }
 
@@ -1467,7 +1489,8 @@ Status cilia_precond(cfm_ctx* c, int kind, const double* x, double* y) {
     }
     if (kind != CFM_PC_DIAG && kind != CFM_PC_SELF && kind != CFM_PC_NEAR && kind != CFM_PC_DENSE)
       return make_status(CFM_EINVAL, "cilia on free bodies take CFM_PC_NONE, _DIAG, _SELF, _NEAR or _DENSE");
-    if (!S->pc_ready || S->pc_kind != kind) CFM_TRY(body_cilia_prep(c, S, kind));
+    if (!S->pc_ready || S->pc_kind != kind || S->pc_rot != (body_rotation_prescribed(c) ? 1 : 0))
+      CFM_TRY(body_cilia_prep(c, S, kind));
     const BodyCiliaK k = body_cilia_args(c, S, kind);
     const dim3 gp((k.M + 255) / 256);
     const size_t M = k.M, ns = k.nseg_total;

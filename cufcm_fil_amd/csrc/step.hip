@@ -296,6 +296,7 @@ static Status swim_velocities(cfm_ctx* c, FilState* F, cfm_step_stats* st) {
   CFM_HIP(hipMemcpyAsync(F->rhs_b.p + 3 * (size_t)s.m, F->S.force, sizeof(double) * s.extra,
                          hipMemcpyDeviceToDevice, c->stream));
   CFM_TRY(dev_axpby(c, -1.0, F->rhs_b.p, 0.0, F->rhs_b.p, F->rhs_b.p, s.L));
+  CFM_TRY(body_prescribed_rhs(c, F->rhs_b.p));  // prescribed rotation: no torque balance
   cfm_solve_opts o = F->prm.blob_solve;
   o.system = CFM_SYS_BODY;
   o.use_guess = 1;  // previous blob forces and body velocities

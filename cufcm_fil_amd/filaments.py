@@ -93,16 +93,18 @@ class FilamentSystem:
 
     # -- free swimmers (swimmer.cpp, rigid_body.cpp) ---------------------------------
     def attach_bodies(self, blob_ref, x_body, q_body, *, f_ext=None, mobility=None,
-                      ref_precond="dense", ref_rtol=1e-10):
+                      ref_precond="dense", ref_rtol=1e-10, body_motion="free"):
         """The blobs become n_body free rigid bodies (blob_ref: body-frame offsets of one
         body, x_body 3 n_body, q_body 4 n_body) carrying the filaments (n_fil / n_body
         per body); their positions and orientations join the Broyden unknowns.  The body
         mobility for the approximate Jacobian is computed (make_body_reference_matrices)
-        unless given (6 x 6, body frame)."""
+        unless given (6 x 6, body frame).  body_motion "prescribed_rotation" holds the
+        bodies' rotation at zero (the reference's BODY_VELOCITY_TYPE 2)."""
         x_body = L.f64(x_body)
         n_body = x_body.size // 3
         self.mob.body_init(blob_ref, n_body)
         self.mob.body_set_state(x_body, q_body)
+        self.mob.body_set_motion(body_motion)
         if mobility is None:
             self.mob.make_body_reference_matrices(0, precond=ref_precond, rtol=ref_rtol)
         else:

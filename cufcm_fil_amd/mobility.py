@@ -199,6 +199,21 @@ class RpyMobilitySolver:
         L.check(self._h, L.lib().cfm_body_get_reference_matrices(self._h, L.ptr(ktk), L.ptr(mob)))
         return ktk.reshape(6, 6), mob.reshape(6, 6)
 
+    _MOTION = {"free": L.CFM_BODY_FREE, "prescribed_velocity": L.CFM_BODY_PRESCRIBED_VELOCITY,
+               "prescribed_rotation": L.CFM_BODY_PRESCRIBED_ROTATION}
+
+    def body_set_motion(self, motion):
+        """The reference's BODY_VELOCITY_TYPE (config.hpp:94-101): "free" (0) or
+        "prescribed_rotation" (2: Omega held at zero, the torque rows of CFM_SYS_BODY read
+        Omega; mobility_solver.cpp:1947-1994, 2177-2181).  "prescribed_velocity" (1) is
+        the blob system of bodies held fixed and is refused."""
+        L.check(self._h, L.lib().cfm_body_set_motion(self._h, int(self._MOTION[motion])))
+
+    def body_motion(self):
+        t = ctypes.c_int32(-1)
+        L.check(self._h, L.lib().cfm_body_get_motion(self._h, ctypes.byref(t)))
+        return {v: k for k, v in self._MOTION.items()}[t.value]
+
     # -- measurement --------------------------------------------------------------
     def profile(self, enable=True):
         L.check(self._h, L.lib().cfm_profile(self._h, int(bool(enable))))

@@ -499,6 +499,23 @@ int cfm_body_reference_matrices(cfm_ctx* ctx, int32_t body, const cfm_solve_opts
  * cilia_sim_main.cu:293-297); either pointer may be null to keep the current one. */
 int cfm_body_set_reference_matrices(cfm_ctx* ctx, const double* ktk_inv, const double* mobility);
 int cfm_body_get_reference_matrices(cfm_ctx* ctx, double* ktk_inv, double* mobility);
+/* How the bodies move: the reference's BODY_VELOCITY_TYPE (config.hpp:94-101).
+ * CFM_BODY_FREE (0, the default) solves for V and Omega.  CFM_BODY_PRESCRIBED_VELOCITY
+ * (1, all velocities zero, rigid_body.cpp:599-614) is the blob system CFM_SYS_BLOB and is
+ * refused here.  CFM_BODY_PRESCRIBED_ROTATION (2) holds Omega_b at zero: the reference
+ * drops the Omega columns and the torque rows of CFM_SYS_BODY (mobility_solver.cpp:
+ * 1947-1994, 2177-2181; preconditioner :1268-1291, 1506-1512); here the unknown vector
+ * keeps 6 entries per body, the blob rows see V_b only and each torque row is the
+ * identity on Omega_b, so a solve returns the reduced system's F and V and Omega = the
+ * right-hand side's torque entries (pass zeros; the library's own right-hand sides --
+ * free swimmers, squirmers, cilia on bodies -- zero them).  CFM_PC_DIAG / _SELF / _DENSE
+ * invert the reduced system as they do the free one, with (K_V^T K_V)^-1 = I / nb, the
+ * clamped mobility ((N^-1)_VV)^-1 of the reference matrices and (K_V^T P K_V)^-1. */
+#define CFM_BODY_FREE 0
+#define CFM_BODY_PRESCRIBED_VELOCITY 1
+#define CFM_BODY_PRESCRIBED_ROTATION 2
+int cfm_body_set_motion(cfm_ctx* ctx, int32_t type);
+int cfm_body_get_motion(cfm_ctx* ctx, int32_t* type);
 
 /* ---- squirmer bodies ------------------------------------------------------------------
  * The reference's CILIA_TYPE 4 (NO_CILIA_SQUIRMER, config.hpp:87,407): bodies of blobs

@@ -14,6 +14,8 @@ PRESCRIBED_BODY_VELOCITIES 0):
     body mobility from six resistance solves with a unit force or torque on the
     body (column 5 - n from right-hand side entry -1 at row 6 n_body - 1 - n)
                                                       mobility_solver.cpp:3049-3146
+  * prescribed rotation (BODY_VELOCITY_TYPE 2): the Omega columns and the torque
+    rows dropped, [[M, -K_V], [-K_V^T, 0]]      mobility_solver.cpp:1947-1994, 2177-2181
 Unknown ordering as the reference: 3 n_blob blob forces (AoS), then 6 per body
 (V then Omega).
 
@@ -22,7 +24,12 @@ K^T K against the Gram matrix of an explicitly assembled K, the preconditioner a
 the exact inverse of the saddle matrix with M_bb replaced by its scalar
 approximation, the body mobility against (K^T M_bb^-1 K)^-1 from a dense solve, its
 symmetry, Stokes drag of a blob sphere (1/(6 pi mu R), 1/(8 pi mu R^3)) and the
-Faxen/Brenner wall corrections of a sphere's translational mobility.
+Faxen/Brenner wall corrections of a sphere's translational mobility.  The
+prescribed-rotation solve is pinned against the free system: its velocity under a unit
+force equals the clamped mobility ((N^-1)_VV)^-1 of the body mobility N from six free
+solves (the Schur complement N_VV - N_VO N_OO^-1 N_OV), and in free space a sphere's
+translation is unchanged by holding its rotation; its preconditioner is the exact inverse
+of the reduced saddle matrix with M replaced by 6 pi mu a.
 """
 from __future__ import annotations
 
@@ -189,3 +196,68 @@ def squirmer_velocity(M, K, slip_lab):
     rhs[:n] = np.asarray(slip_lab).reshape(-1)
     z = np.linalg.solve(A, rhs)
     return z[n:].reshape(-1, 6), z[:n]
+
+
+# ---------------------------------------------------------------- prescribed rotation
+def translation_columns(n_body):
+    """Indices of the V entries among the 6 n_body body unknowns."""
+    return np.array([6 * b + k for b in range(n_body) for k in range(3)])
+
+
+def prescribed_rotation_solve(M, K, rhs_f, rhs_v):
+    """BODY_VELOCITY_TYPE 2: Omega columns dropped from the blob rows and the torque rows
+    dropped (mobility_solver.cpp:1947-1994, 2177-2181), i.e. the reduced saddle system
+    [[M, -K_V], [-K_V^T, 0]] [F; V] = [rhs_f; rhs_v].  Returns F and (V, Omega = 0) per
+    body."""
+    nbody = K.shape[1] // 6
+    Kv = K[:, translation_columns(nbody)]
+    A = saddle_matrix(M, Kv)
+    n = K.shape[0]
+    z = np.linalg.solve(A, np.concatenate([np.asarray(rhs_f, float), np.asarray(rhs_v, float)]))
+    U = np.zeros((nbody, 6))
+    U[:, :3] = z[n:].reshape(nbody, 3)
+    return z[:n], U
+
+
+def clamped_mobility(N):
+    """Translational mobility of a body whose rotation is held: ((N^-1)_VV)^-1."""
+    return np.linalg.inv(np.linalg.inv(N)[:3, :3])
+
+
+def precondition_prescribed_rotation(v, offsets, q, mob_ref, mu=1.0, a=1.0):
+    """The block preconditioner of the reduced system: V = -(1/nb) sum f - N'_VV u_V with
+    the clamped mobility N' of the rotated reference mobility, Omega = u_Omega, F = 6 pi mu
+    a (f + K_V V) (the reference's prescribed-rotation branches
+    mobility_solver.cpp:1268-1291, 1506-1512 drop the Omega terms the same way)."""
+    q = np.asarray(q).reshape(-1, 4)
+    nbody = q.shape[0]
+    r = np.asarray(offsets).reshape(-1, 3)
+    n = r.shape[0]
+    nb = n // nbody
+    minv = 6.0 * math.pi * mu * a
+    out = np.zeros_like(v)
+    for b in range(nbody):
+        R = rot_mat(q[b])
+        Nc = R @ clamped_mobility(mob_ref) @ R.T
+        fb = v[3 * b * nb:3 * (b + 1) * nb].reshape(nb, 3)
+        ub = v[3 * n + 6 * b:3 * n + 6 * b + 6]
+        V = -fb.sum(axis=0) / nb - Nc @ ub[:3]
+        out[3 * n + 6 * b:3 * n + 6 * b + 3] = V
+        out[3 * n + 6 * b + 3:3 * n + 6 * b + 6] = ub[3:]
+        out[3 * b * nb:3 * (b + 1) * nb] = (minv * (fb + V)).reshape(-1)
+    return out
+
+
+def prescribed_rotation_matrix(M, K):
+    """The library's form of the prescribed-rotation product: the saddle matrix with the
+    Omega columns of the blob rows zeroed and each torque row the identity on Omega_b,
+    in the unknown order (F, then (V, Omega) per body)."""
+    A = saddle_matrix(M, K)
+    n = K.shape[0]
+    for b in range(K.shape[1] // 6):
+        for k in range(3, 6):
+            j = n + 6 * b + k
+            A[:, j] = 0.0
+            A[j, :] = 0.0
+            A[j, j] = 1.0
+    return A

@@ -20,6 +20,9 @@ Follows the reference's PRESCRIBED_CILIA swimmers with free bodies
   * the block preconditioner with 6 pi mu a on every particle
     (mobility_solver.cpp:1104-1515) and make_precon_mat's 6 x 6 matrix
     (swimmer.cpp:1175-1347), restated step by step;
+  * prescribed rotation (BODY_VELOCITY_TYPE 2, rot_prescribed): Omega_b = 0, its
+    columns and the torque rows dropped (mobility_solver.cpp:1947-1994, 2177-2181),
+    written as identity rows on Omega_b with a zero right-hand side;
   * the explicit time integration (filament.cpp:1340-1356, cilia_sim_main.cu:724-740):
     phase += dt phase_dot before a step, x_b += dt V_b and q_b = exp(dt Omega_b) q_b after.
 
@@ -30,7 +33,9 @@ preconditioner against the exact inverse of the saddle matrix with M replaced by
 (6 pi mu a)^-1 I, make_precon_mat against that matrix's Schur complement, the solve
 against force- and torque-free balance and the rigid-body identities (a swimmer
 translated moves identically; rotated, its velocities rotate), and the dynamic-phase
-solve by each cilium's generalised force equalling its interpolated drive.
+solve by each cilium's generalised force equalling its interpolated drive.  The
+prescribed-rotation solve is pinned against the free system: applying to the free swimmer
+the torque that holds its rotation reproduces the same forces and V with Omega = 0.
 """
 from __future__ import annotations
 
@@ -87,6 +92,9 @@ class Swimmers:
         self.mu, self.a = mu, a
         self.driving = None   # (q_phase_table, q_angle_table or None, spring)
         self.fene, self.noise, self.seed, self.nt = False, 0.0, 0, 0
+        # BODY_VELOCITY_TYPE 2: Omega_b held at zero, the Omega columns and torque rows
+        # dropped (mobility_solver.cpp:1947-1994, 2177-2181), written with identity rows
+        self.rot_prescribed = False
 
     def geometry(self):
         """Segment positions (n_seg x 3), blob positions (n_blob x 3) and per-cilium
@@ -142,6 +150,13 @@ class Swimmers:
                 v[3 * c * self.nseg:3 * (c + 1) * self.nseg] = vp[c] if j < self.nc else va[c]
                 A[:n3, nu + j] = -v
                 A[nu + j, :n3] = -v
+        if self.rot_prescribed:
+            for b in range(self.nbody):
+                for k in range(3, 6):
+                    j = n3 + 6 * b + k
+                    A[:, j] = 0.0
+                    A[j, :] = 0.0
+                    A[j, j] = 1.0
         return A
 
     def mobility(self, xs, xb):

@@ -148,3 +148,83 @@ def test_blob_squirmer_swims_at_two_thirds_b1():
     assert abs(u_inf / (2.0 / 3.0) - 1.0) < 0.01, (Us, u_inf)
     # the sign of e_theta is pinned: the body swims towards its north pole (+z)
     assert u1 > u2 > 2.0 / 3.0
+
+
+def _two_bodies_over_wall():
+    ref = geo.fibonacci_sphere(60, 4.5)
+    x = np.array([[0.0, 0.0, 8.0], [11.0, 2.0, 9.0]])
+    q = np.stack([_quat(21), _quat(22)])
+    pos, off = bo.place(ref, x, q)
+    return ref, pos, off, q, bo.k_matrix(off, 2), oracle.mobility_matrix(pos, 3, wall=True)
+
+
+def test_prescribed_rotation_velocity_is_the_clamped_mobility():
+    """Under a unit force the prescribed-rotation solve moves a body above the wall with
+    ((N^-1)_VV)^-1 e_k, N the body mobility of six free solves -- the Schur complement
+    N_VV - N_VO N_OO^-1 N_OV, which differs from the torque-free N_VV by the wall's
+    translation-rotation coupling (a solve that kept the Omega columns or the torque
+    rows would return N_VV)."""
+    ref = geo.fibonacci_sphere(60, 4.5)
+    pos, off = bo.place(ref, np.array([[0.0, 0.0, 7.0]]), _quat(11)[None])
+    K = bo.k_matrix(off, 1)
+    M = oracle.mobility_matrix(pos, 3, wall=True)
+    N = bo.reference_mobility_by_solves(M, K)
+    Nc = bo.clamped_mobility(N)
+    assert np.allclose(Nc, N[:3, :3] - N[:3, 3:] @ np.linalg.solve(N[3:, 3:], N[3:, :3]),
+                       rtol=0, atol=1e-12 * np.abs(Nc).max())
+    got = np.zeros((3, 3))
+    for k in range(3):
+        rhs_v = np.zeros(3)
+        rhs_v[k] = -1.0                      # -K^T F = -e_k: unit force on the body
+        F, U = bo.prescribed_rotation_solve(M, K, np.zeros(3 * len(pos)), rhs_v)
+        got[:, k] = U[0, :3]
+        assert np.all(U[0, 3:] == 0.0)
+    assert np.allclose(got, Nc, rtol=0, atol=1e-10 * np.abs(Nc).max())
+    assert np.abs(Nc - N[:3, :3]).max() > 1e-4 * np.abs(Nc).max()     # 3e-4 at this height
+
+
+def test_prescribed_rotation_equals_free_bodies_held_by_their_torques():
+    """Two bodies above the wall with a blob right-hand side: the reduced solve's blob
+    torques tau_b = -K_O^T F, given to the free system as its torque rows, reproduce the
+    same F and V with Omega = 0; the identity-row product the library uses solves to the
+    same answer."""
+    ref, pos, off, q, K, M = _two_bodies_over_wall()
+    rng = np.random.default_rng(23)
+    rf, rv = rng.normal(size=3 * len(pos)), rng.normal(size=6)
+    F, U = bo.prescribed_rotation_solve(M, K, rf, rv)
+    tau = -K.T @ F
+    rhs = np.concatenate([rf, tau])
+    for b in range(2):
+        rhs[3 * len(pos) + 6 * b:3 * len(pos) + 6 * b + 3] = rv[3 * b:3 * b + 3]
+    z = np.linalg.solve(bo.saddle_matrix(M, K), rhs)
+    n = 3 * len(pos)
+    assert np.abs(z[n:].reshape(2, 6)[:, 3:]).max() < 1e-10 * np.abs(U).max()
+    assert np.allclose(z[:n], F, rtol=0, atol=1e-10 * np.abs(F).max())
+    assert np.allclose(z[n:].reshape(2, 6)[:, :3], U[:, :3], rtol=1e-9)
+    rhs_id = np.concatenate([rf, np.zeros(12)])
+    for b in range(2):
+        rhs_id[n + 6 * b:n + 6 * b + 3] = rv[3 * b:3 * b + 3]
+    zi = np.linalg.solve(bo.prescribed_rotation_matrix(M, K), rhs_id)
+    assert np.allclose(zi[:n], F, rtol=0, atol=1e-10 * np.abs(F).max())
+    assert np.allclose(zi[n:].reshape(2, 6), U, rtol=0, atol=1e-10 * np.abs(U).max())
+
+
+def test_prescribed_rotation_preconditioner_inverts_the_reduced_scalar_saddle():
+    """With M replaced by m I and a body mobility whose clamped block is m / nb I, the
+    reduced block preconditioner is the exact inverse of the identity-row matrix."""
+    mu, a = 0.7, 1.3
+    m = 1.0 / (6.0 * math.pi * mu * a)
+    ref = geo.fibonacci_sphere(40, 4.0)
+    x = np.array([[0.0, 0.0, 10.0], [12.0, 0.0, 10.0]])
+    q = np.stack([_quat(7), _quat(8)])
+    pos, off = bo.place(ref, x, q)
+    K = bo.k_matrix(off, 2)
+    A = bo.prescribed_rotation_matrix(m * np.eye(3 * len(pos)), K)
+    rng = np.random.default_rng(9)
+    G = rng.normal(size=(3, 3))
+    N = np.zeros((6, 6))
+    N[:3, :3] = m / len(ref) * np.eye(3)
+    N[3:, 3:] = G @ G.T + np.eye(3)               # the rotation block does not enter
+    v = rng.normal(size=A.shape[0])
+    got = bo.precondition_prescribed_rotation(v, off, q, N, mu=mu, a=a)
+    assert np.allclose(got, np.linalg.solve(A, v), rtol=0, atol=1e-10 * np.abs(got).max())

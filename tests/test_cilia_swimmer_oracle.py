@@ -161,3 +161,29 @@ def test_driving_options_fene_spring_and_seeded_noise():
     z = np.array([co.normal_deviate(11, nt, c) for nt in range(200) for c in range(250)])
     assert abs(z.mean()) < 0.015 and abs(z.var() - 1.0) < 0.02
     assert abs(np.mean(z ** 4) - 3.0) < 0.08 and abs(np.mean(np.abs(z) < 1.0) - 0.6827) < 0.01
+
+
+def test_prescribed_rotation_is_the_free_swimmer_held_by_its_torque():
+    """With the rotation prescribed (Omega_b = 0, Omega columns and torque rows dropped)
+    the fluid's torque on the body, tau = -sum r x F, is not balanced.  Applying -tau to
+    the free swimmer as the right-hand side of its torque rows must reproduce the same
+    forces and V with Omega = 0 -- a dropped column, a wrong sign or a torque row left
+    in the reduced system would break the equality."""
+    sw = swimmer(per=3, seed=5)
+    sw.rot_prescribed = True
+    z, qp, qa = sw.step(0)
+    ns, n3, nu, L = sw.layout(False, False)
+    assert np.abs(sw.V[:, 3:]).max() == 0.0
+    xs, xb, vp, va = sw.last["xs"], sw.last["xb"], sw.last["vp"], sw.last["va"]
+    F = z[:n3].reshape(-1, 3)
+    tau = np.cross(np.vstack([xs, xb]) - sw.x[0], F).sum(0)
+    assert np.linalg.norm(tau) > 1e-3 * np.abs(F).max() * R_BODY   # the hold takes a torque
+    assert np.abs(F.sum(0)).max() < 1e-9 * np.abs(F).max()           # still force-free
+    free = swimmer(per=3, seed=5)
+    A = free.system_matrix(free.mobility(xs, xb), xs, xb, vp, va, False, False)
+    rhs = free.rhs(vp, False, False)
+    rhs[n3 + 3:n3 + 6] = -tau                                         # -K^T F rows
+    zf = np.linalg.solve(A, rhs)
+    assert np.abs(zf[n3 + 3:n3 + 6]).max() < 1e-10 * np.abs(zf[n3:n3 + 3]).max()
+    assert np.allclose(zf[:n3], z[:n3], rtol=0, atol=1e-9 * np.abs(F).max())
+    assert np.allclose(zf[n3:n3 + 3], z[n3:n3 + 3], rtol=1e-9)

@@ -17,7 +17,7 @@ from test_swimmer_oracle import _swimmer, _velocity
 pytestmark = pytest.mark.gpu
 
 
-def _gpu(sw, P, wall, f_ext=None, precond="self"):
+def _gpu(sw, P, wall, f_ext=None, precond="self", **attach):
     xs = np.concatenate([f.x for f in sw.fils]).reshape(-1)
     t = np.concatenate([[fo.tangent(q) for q in f.q] for f in sw.fils]).reshape(-1)
     pert = np.concatenate([f.pert for f in sw.fils])
@@ -25,7 +25,7 @@ def _gpu(sw, P, wall, f_ext=None, precond="self"):
                        dimensionless_force=P.end_force * P.dl ** 2 * P.nseg ** 2 / P.kb,
                        blob_rtol=1e-12, blob_precond=precond, pert=pert)
     S.attach_bodies(sw.body.ref.reshape(-1), sw.body.x, sw.body.q, f_ext=f_ext,
-                    mobility=sw.mob_ref.reshape(-1))
+                    mobility=sw.mob_ref.reshape(-1), **attach)
     return S
 
 
