@@ -41,13 +41,15 @@ FP32_VALU_PEAK_TFLOPS = 157.3   # MI355X vector FP32 (published)
 PMC_SUMMARY = os.path.join(ROOT, "profiles", "rpy_apply_fp32c_sphere32k_pmc.md")
 
 
-def measured_traffic():
-    """HBM bytes per launch of the FP32C pair kernel from the committed counter
-    summary (FETCH_SIZE and WRITE_SIZE runs calibrated on a device copy)."""
+def measured_traffic(kernel):
+    """HBM bytes per launch of the named FP32C pair kernel from the committed counter
+    summary (FETCH_SIZE and WRITE_SIZE runs calibrated on a device copy); None when the
+    summary holds no figure for it."""
+    key = "traffic_bytes_per_launch_sym:" if kernel.startswith("rpy_sym") else "traffic_bytes_per_launch:"
     try:
         with open(PMC_SUMMARY) as fh:
             for line in fh:
-                if line.startswith("traffic_bytes_per_launch:"):
+                if line.startswith(key):
                     return float(line.split(":")[1].split()[0])
     except OSError:
         pass
@@ -80,8 +82,9 @@ def build_system(args, precision, solve_mode, guess, precond, overlap=False, per
                           overlap=overlap)
 
 
-def run_path(args, label, precision, solve_mode, guess, precond, overlap=False):
+def run_path(args, label, precision, solve_mode, guess, precond, overlap=False, symmetry=True):
     S = build_system(args, precision, solve_mode, guess, precond, overlap)
+    S.mob.set_pair_symmetry(symmetry and not args.no_pair_symmetry)
     try:
         for nt in range(args.warmup):
             st = S.step(nt)
@@ -543,6 +546,10 @@ def main():
     ap.add_argument("--skip-ciliated", action="store_true")
     ap.add_argument("--skip-ensemble", action="store_true")
     ap.add_argument("--skip-overlap", action="store_true")
+    ap.add_argument("--no-pair-symmetry", action="store_true",
+                    help="one-directional pair kernel for the segment-segment product")
+    ap.add_argument("--symmetry-ab", action="store_true",
+                    help="also time the fast path with the one-directional pair kernel")
     ap.add_argument("--only", choices=["batched", "learned", "cilia", "body", "swimmer", "ciliated",
                                        "ensemble"],
                     help="run one side section alone and print its JSON (for profiling)")
@@ -618,18 +625,37 @@ def main():
         mixed["max_abs_state_diff_vs_plain"] = float(np.abs(mixed["final_x"] - plain["final_x"]).max())
         mixed["same_broyden_iterations_as_plain"] = mixed["broyden_per_step"] == plain["broyden_per_step"]
     prof = mixed["profile"]
-    kms = prof["kernel_ms_fp32"] / max(1, prof["launches_fp32"])
-    flops_launch = prof["alg_flops_fp32"] / max(1, prof["launches_fp32"])
+    # the dominant pair kernel: the symmetric segment-segment product when it runs
+    # (kernel + finalisation between the events; flops of both pair directions), else
+    # the one-directional tiled kernel
+    if prof["kernel_ms_sym"] >= prof["kernel_ms_fp32"]:
+        launches, ms_tot, fl_tot = prof["launches_sym"], prof["kernel_ms_sym"], prof["alg_flops_sym"]
+        kname = "rpy_sym_kernel + rpy_sym_finalize_kernel"
+    else:
+        launches, ms_tot, fl_tot = prof["launches_fp32"], prof["kernel_ms_fp32"], prof["alg_flops_fp32"]
+        kname = "rpy_apply_kernel<float,6,true>"
+    kms = ms_tot / max(1, launches)
+    flops_launch = fl_tot / max(1, launches)
     achieved = flops_launch / (kms * 1e-3) / 1e12 if kms > 0 else 0.0
     roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": FP32_VALU_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(achieved / FP32_VALU_PEAK_TFLOPS, 4),
-                "traffic": measured_traffic(),
+                "traffic": measured_traffic(kname),
                 "traffic_source": os.path.relpath(PMC_SUMMARY, ROOT) + " (FETCH_SIZE and WRITE_SIZE "
                                   "counter runs of this workload, copy-calibrated; not counted in "
                                   "this run)",
-                "kernel": "rpy_apply_kernel<float,6,true>",
-                "launches": prof["launches_fp32"], "avg_ms": round(kms, 4),
-                "alg_flops_per_launch": flops_launch}
+                "kernel": kname,
+                "launches": launches, "avg_ms": round(kms, 4),
+                "alg_flops_per_launch": flops_launch,
+                "other_pair_kernel_ms_per_step": round(
+                    (prof["kernel_ms_fp32"] if launches == prof["launches_sym"] else prof["kernel_ms_sym"])
+                    / args.steps, 3)}
+    sym_ab = None
+    if args.symmetry_ab:
+        o = run_path(args, "one-directional", "fp32c", args.mode, guess, args.precond, symmetry=False)
+        sym_ab = {"steps_per_s": round(o["steps_per_s"], 4), "ms_per_step": round(o["ms_per_step"], 3),
+                  "broyden_iterations": o["broyden_iterations"],
+                  "pair_kernel_ms_per_step": round(o["profile"]["kernel_ms_fp32"] / args.steps, 3),
+                  "max_abs_state_diff_vs_symmetric": float(abs(o["final_x"] - mixed["final_x"]).max())}
     batched = None if args.skip_batch else run_batched(args)
     learned = None if args.skip_learned else run_learned(args)
     cilia = None if args.skip_cilia else run_cilia(args)
@@ -677,6 +703,7 @@ def main():
         "ciliated_swimmer": ciliated,
         "orbit_ensemble": ensemble,
         "overlap": ovl,
+        "one_directional_pair_kernel": sym_ab,
     }
     print(json.dumps(out), flush=True)
 

@@ -125,6 +125,9 @@ class cfm_profile_stats(ctypes.Structure):
         ("launches_fp32", ctypes.c_int64),
         ("kernel_ms_fp32", ctypes.c_double),
         ("alg_flops_fp32", ctypes.c_double),
+        ("launches_sym", ctypes.c_int64),
+        ("kernel_ms_sym", ctypes.c_double),
+        ("alg_flops_sym", ctypes.c_double),
     ]
 
     def as_dict(self):
@@ -300,6 +303,7 @@ def lib():
                                   ctypes.POINTER(cfm_batch_sys_stats), _dp]
     L.cfm_profile.argtypes = [_vp, ctypes.c_int32]
     L.cfm_profile_read.argtypes = [_vp, ctypes.POINTER(cfm_profile_stats)]
+    L.cfm_set_pair_symmetry.argtypes = [_vp, ctypes.c_int32]
     _lib = L
     return L
 

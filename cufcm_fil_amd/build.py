@@ -25,7 +25,7 @@ PROBE_SRC = os.path.join(HERE, "probe", "valu_probe.hip")
 PROBE_LIB = os.path.join(HERE, "libcfm_probe.so")
 SOURCES = ["api.hip", "apply.hip", "gmres.hip", "fil.hip", "step.hip", "gnn.hip", "nearfield.hip",
            "densepc.hip", "batch.hip", "cilia.hip", "body.hip", "swim.hip", "noalloc.hip",
-           "coarse.hip", "squirmer.hip"]
+           "coarse.hip", "squirmer.hip", "apply_sym.hip"]
 ARCH = os.environ.get("CFM_OFFLOAD_ARCH", "gfx950")
 
 FLAGS = [

@@ -301,6 +301,10 @@ int cfm_destroy(cfm_ctx* c) {
     (void)hipStreamDestroy(c->side);
   }
   c->partial_side.release();
+  c->sym_own.release();
+  c->sym_own_side.release();
+  c->sym_src.release();
+  c->sym_src_side.release();
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return CFM_OK;
@@ -516,6 +520,13 @@ int cfm_profile(cfm_ctx* c, int32_t enable) {
   return CFM_OK;
 }
 
+int cfm_set_pair_symmetry(cfm_ctx* c, int32_t on) {
+  if (!c) return CFM_EINVAL;
+  c->sym_pairs = on != 0;
+  c->split_cache.clear();  // the remaining launches change shape
+  return CFM_OK;
+}
+
 int cfm_profile_read(cfm_ctx* c, cfm_profile_stats* st) {
   if (!c || !st) return CFM_EINVAL;
   (void)hipSetDevice(c->device);
@@ -530,10 +541,14 @@ int cfm_profile_read(cfm_ctx* c, cfm_profile_stats* st) {
     S.kernel_ms += ms;
     S.alg_flops += c->prof_flops[k];
     S.alg_bytes += c->prof_bytes[k];
-    if (c->prof_fp32[k]) {
+    if (c->prof_fp32[k] == 1) {
       S.launches_fp32 += 1;
       S.kernel_ms_fp32 += ms;
       S.alg_flops_fp32 += c->prof_flops[k];
+    } else if (c->prof_fp32[k] == 2) {
+      S.launches_sym += 1;
+      S.kernel_ms_sym += ms;
+      S.alg_flops_sym += c->prof_flops[k];
     }
   }
   *st = S;

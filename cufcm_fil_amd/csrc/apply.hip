@@ -287,8 +287,16 @@ Status side_join(cfm_ctx* c) {
   return {};
 }
 
-Status launch_apply(cfm_ctx* c, const ApplyPlan& plan) {
+bool sym_applicable(const cfm_ctx* c, const ApplyPlan& plan);
+Status launch_sym(cfm_ctx* c, const ApplyPlan& plan, hipStream_t stream);
+
+Status launch_apply(cfm_ctx* c, const ApplyPlan& plan_in) {
   Particles& p = c->p;
+  // FP32C wall products over all segment targets from segment sources: the symmetric
+  // kernel takes the segment targets, this launch the rest (blob targets)
+  const bool sym = sym_applicable(c, plan_in);
+  ApplyPlan plan = plan_in;
+  if (sym) plan.tgt_begin = std::max(plan_in.tgt_begin, p.n_seg);
   KArgs a{};
   a.x = p.x.p;
   a.y = p.y.p;
@@ -329,11 +337,28 @@ Status launch_apply(cfm_ctx* c, const ApplyPlan& plan) {
   a.seg_tiles = seg_hi > seg_lo ? (seg_hi - seg_lo + kTT - 1) / kTT : 0;
   const int blob_tiles = blob_hi > blob_lo ? (blob_hi - blob_lo + kTT - 1) / kTT : 0;
   const int tiles = a.seg_tiles + blob_tiles;
-  if (tiles == 0) return {};
+  if (tiles == 0 && !sym) return {};
   int split = std::max(1, c->split);
   a.split = split;
   if (plan.side && !c->side) return make_status(CFM_ESTATE, "side stream launch before side_fork");
   const hipStream_t stream = plan.side ? c->side : c->stream;
+  if (sym) {
+    hipEvent_t se0 = nullptr, se1 = nullptr;
+    if (c->prof_on) CFM_TRY(prof_begin(c, &se0, &se1, stream));
+    CFM_TRY(launch_sym(c, plan_in, stream));
+    if (c->prof_on) {
+      CFM_HIP(hipEventRecord(se1, stream));
+      ApplyPlan ss = plan_in;
+      ss.tgt_begin = 0;
+      ss.tgt_end = p.n_seg;
+      double fl = 0.0, by = 0.0;
+      plan_cost(c, ss, true, &fl, &by);
+      c->prof_flops.push_back(fl);
+      c->prof_bytes.push_back(by);
+      c->prof_fp32.push_back(2);  // the symmetric segment product
+    }
+    if (tiles == 0) return {};
+  }
   if (split > 1) {
     if (plan.partial_ext) {
       a.partial = plan.partial_ext;

@@ -212,6 +212,10 @@ struct cfm_ctx {
   // autotuned source splits (tile_split = -1), keyed on the product's shape
   std::map<std::array<long long, 6>, int> split_cache;
   cfm::DevBuf<double> tune_v;           // scratch output of the tuning launches
+  // segment-segment products evaluating each unordered pair once (apply_sym.hip)
+  bool sym_pairs = true;
+  cfm::DevBuf<double> sym_own, sym_own_side;  // per-workgroup target partials
+  cfm::DevBuf<float> sym_src, sym_src_side;   // per-block-pair source partials
   // pair-kernel profiling (HIP events around each launch)
   bool prof_on = false;
   std::vector<hipEvent_t> prof_events;  // pool, two per launch

@@ -223,6 +223,11 @@ class RpyMobilitySolver:
         L.check(self._h, L.lib().cfm_profile_read(self._h, ctypes.byref(st)))
         return st.as_dict()
 
+    def set_pair_symmetry(self, on=True):
+        """FP32C wall segment products evaluate each unordered pair once (default) or,
+        off, with the one-directional kernel."""
+        L.check(self._h, L.lib().cfm_set_pair_symmetry(self._h, int(bool(on))))
+
     # -- learned near-field correction of the blob preconditioner ----------------
     def gnn_init(self, seed=0, capacity=256):
         L.check(self._h, L.lib().cfm_gnn_init(self._h, int(seed), int(capacity)))
