@@ -305,6 +305,8 @@ int cfm_destroy(cfm_ctx* c) {
   c->sym_own_side.release();
   c->sym_src.release();
   c->sym_src_side.release();
+  c->sym_src64.release();
+  c->sym_src64_side.release();
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return CFM_OK;

@@ -287,16 +287,18 @@ Status side_join(cfm_ctx* c) {
   return {};
 }
 
-bool sym_applicable(const cfm_ctx* c, const ApplyPlan& plan);
-Status launch_sym(cfm_ctx* c, const ApplyPlan& plan, hipStream_t stream);
+int sym_species(const cfm_ctx* c, const ApplyPlan& plan);
+Status launch_sym(cfm_ctx* c, const ApplyPlan& plan, int species, hipStream_t stream);
 
 Status launch_apply(cfm_ctx* c, const ApplyPlan& plan_in) {
   Particles& p = c->p;
-  // FP32C wall products over all segment targets from segment sources: the symmetric
-  // kernel takes the segment targets, this launch the rest (blob targets)
-  const bool sym = sym_applicable(c, plan_in);
+  // Wall products of one species from itself: the symmetric kernels take that species'
+  // targets, this launch the rest (the other species' targets, if any)
+  const int species = sym_species(c, plan_in);
+  const bool sym = species != 0;
   ApplyPlan plan = plan_in;
-  if (sym) plan.tgt_begin = std::max(plan_in.tgt_begin, p.n_seg);
+  if (species == 1) plan.tgt_begin = std::max(plan_in.tgt_begin, p.n_seg);
+  if (species == 2) plan.tgt_end = std::min(plan_in.tgt_end, p.n_seg);
   KArgs a{};
   a.x = p.x.p;
   a.y = p.y.p;
@@ -345,17 +347,18 @@ Status launch_apply(cfm_ctx* c, const ApplyPlan& plan_in) {
   if (sym) {
     hipEvent_t se0 = nullptr, se1 = nullptr;
     if (c->prof_on) CFM_TRY(prof_begin(c, &se0, &se1, stream));
-    CFM_TRY(launch_sym(c, plan_in, stream));
+    CFM_TRY(launch_sym(c, plan_in, species, stream));
     if (c->prof_on) {
       CFM_HIP(hipEventRecord(se1, stream));
       ApplyPlan ss = plan_in;
-      ss.tgt_begin = 0;
-      ss.tgt_end = p.n_seg;
+      ss.tgt_begin = species == 1 ? 0 : p.n_seg;
+      ss.tgt_end = species == 1 ? p.n_seg : p.n;
       double fl = 0.0, by = 0.0;
-      plan_cost(c, ss, true, &fl, &by);
+      const bool f32 = c->cfg.precision == CFM_FP32C;
+      plan_cost(c, ss, f32, &fl, &by);
       c->prof_flops.push_back(fl);
       c->prof_bytes.push_back(by);
-      c->prof_fp32.push_back(2);  // the symmetric segment product
+      c->prof_fp32.push_back(f32 ? 2 : 3);  // symmetric products: FP32C 2, FP64 3
     }
     if (tiles == 0) return {};
   }

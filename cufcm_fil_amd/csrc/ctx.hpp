@@ -212,10 +212,11 @@ struct cfm_ctx {
   // autotuned source splits (tile_split = -1), keyed on the product's shape
   std::map<std::array<long long, 6>, int> split_cache;
   cfm::DevBuf<double> tune_v;           // scratch output of the tuning launches
-  // segment-segment products evaluating each unordered pair once (apply_sym.hip)
+  // single-species wall products evaluating each unordered pair once (apply_sym.hip)
   bool sym_pairs = true;
   cfm::DevBuf<double> sym_own, sym_own_side;  // per-workgroup target partials
-  cfm::DevBuf<float> sym_src, sym_src_side;   // per-block-pair source partials
+  cfm::DevBuf<float> sym_src, sym_src_side;   // per-block-pair source partials (FP32C)
+  cfm::DevBuf<double> sym_src64, sym_src64_side;  // the same, FP64
   // pair-kernel profiling (HIP events around each launch)
   bool prof_on = false;
   std::vector<hipEvent_t> prof_events;  // pool, two per launch

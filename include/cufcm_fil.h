@@ -286,9 +286,9 @@ typedef struct cfm_profile_stats {
   int64_t launches_fp32;   /* of which FP32C, one-directional tiled kernel */
   double kernel_ms_fp32;
   double alg_flops_fp32;
-  int64_t launches_sym;    /* of which the FP32C segment-segment products that evaluate each
-                              unordered pair once (kernel and finalisation; flops counted
-                              for both directions, as the one-directional kernel would) */
+  int64_t launches_sym;    /* of which FP32C products that evaluate each unordered pair once
+                              (kernel and finalisation; flops counted for both directions,
+                              as the one-directional kernel would) */
   double kernel_ms_sym;
   double alg_flops_sym;
 } cfm_profile_stats;
@@ -304,9 +304,10 @@ typedef struct cfm_profile_stats {
  * (enable = 1), stop (0).  Enabling resets the counters. (No reference counterpart.) */
 int cfm_profile(cfm_ctx* ctx, int32_t enable);
 int cfm_profile_read(cfm_ctx* ctx, cfm_profile_stats* stats);
-/* FP32C wall products of all segments from the segments (6 DOF) evaluate each unordered
- * pair once, applying its 6 x 6 block both ways (on = 1, the default); 0 restores the
- * one-directional kernel (for comparison).  (No reference counterpart.) */
+/* Wall products of one species from itself over all of its targets (segments from
+ * segments, blobs from blobs; FP32C and FP64) evaluate each unordered pair once,
+ * applying its block both ways (on = 1, the default); 0 restores the one-directional
+ * kernel (for comparison).  (No reference counterpart.) */
 int cfm_set_pair_symmetry(cfm_ctx* ctx, int32_t on);
 
 /* ---- coarse correction of the near-field preconditioner (no reference counterpart) ----
