@@ -38,7 +38,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 FP32_VALU_PEAK_TFLOPS = 157.3   # MI355X vector FP32 (published)
-PMC_SUMMARY = os.path.join(ROOT, "profiles", "rpy_apply_fp32c_sphere32k_pmc.md")
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "rpy_pair_fp32c_sphere32k_pmc.md")
 
 
 def measured_traffic(kernel):
