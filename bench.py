@@ -408,10 +408,13 @@ def run_free_swimmer(args):
 
 def run_orbit_ensemble(args):
     """Outer-loop flow maps (the periodic-orbit / bisection / Floquet drivers' unit of
-    work): 8 members of a 4 x 4 lattice of dynamic-phase Fulford-Blake cilia driven by
-    their recorded generalised forces, 30 steps each (a tenth of a beat), run as one
-    concurrent ensemble (one context and HIP stream per member) and one member at a
-    time."""
+    work): members of a 4 x 4 lattice of dynamic-phase Fulford-Blake cilia driven by
+    their recorded generalised forces, 30 steps each (a tenth of a beat).  Eight members
+    run batched (one CiliaEnsemble: one product launch and one lockstep GMRES per step for
+    all of them), as a concurrent ensemble (one context and HIP stream per member) and one
+    member at a time (near-synchronous phases); 32 members -- the central-difference states of the
+    monodromy matrix around the first of them (phases +- 1e-4) -- batched and
+    concurrent."""
     import numpy as np
 
     from cufcm_fil_amd.cilia import wall_lattice
@@ -421,27 +424,59 @@ def run_orbit_ensemble(args):
     flow = CiliaFlow(wall_lattice(4, 4, spacing=30.0), qp, rtol=1e-10)
     n = flow.nfil
     rng = np.random.default_rng(3)
-    xs = [np.concatenate([[1.0], rng.uniform(0, 2 * math.pi, n)]) for _ in range(8)]
+    # near-synchronous states (a common phase +- 0.3): with independent random phases,
+    # neighbours 30a apart (cilia 49a long) can overlap, where the phase speeds spike and
+    # the flow map is not continuous at the scale of the solve tolerance
+    xs = [np.concatenate([[1.0], rng.uniform(0, 2 * math.pi) + rng.uniform(-0.3, 0.3, n)])
+          for _ in range(8)]
+    # the monodromy matrix's 2 n = 32 central-difference states around the first one
+    ms = []
+    for j in range(n):
+        for sgn in (1.0, -1.0):
+            x = xs[0].copy()
+            x[1 + j] += sgn * 1e-4
+            ms.append(x)
     ndts = 30
-    flow.ensemble(xs[:2], 2, workers=2)          # warm-up (library load, first contexts)
-    t0 = time.perf_counter()
-    conc = flow.ensemble(xs, ndts, workers=8)
-    t_conc = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    seq = [flow.run(x, ndts) for x in xs]
-    t_seq = time.perf_counter() - t0
-    diff = max(float(np.abs(a - b).max()) for a, b in zip(conc, seq))
-    print(f"[bench] ensemble: concurrent {t_conc:.2f}s, sequential {t_seq:.2f}s", file=sys.stderr,
-          flush=True)
+    flow.ensemble(xs[:2], 2, workers=2, batched=False)   # warm-up (library load, first contexts)
+    flow.ensemble(xs[:2], 2)
+
+    def timed(fn):
+        t0 = time.perf_counter()
+        r = fn()
+        return r, time.perf_counter() - t0
+
+    bat, t_bat = timed(lambda: flow.ensemble(xs, ndts))
+    conc, t_conc = timed(lambda: flow.ensemble(xs, ndts, workers=8, batched=False))
+    seq, t_seq = timed(lambda: [flow.run(x, ndts) for x in xs])
+    bat32, t_bat32 = timed(lambda: flow.ensemble(ms, ndts))
+    conc32, t_conc32 = timed(lambda: flow.ensemble(ms, ndts, workers=8, batched=False))
+    diff_c = max(float(np.abs(a - b).max()) for a, b in zip(conc, seq))
+    diff_b = max(float(np.abs(a - b).max()) for a, b in zip(bat, seq))
+    diff_b32 = max(float(np.abs(a - b).max()) for a, b in zip(bat32, conc32))
+    print(f"[bench] ensemble: 8 members batched {t_bat:.2f}s, concurrent {t_conc:.2f}s, "
+          f"sequential {t_seq:.2f}s; 32 members batched {t_bat32:.2f}s, concurrent {t_conc32:.2f}s",
+          file=sys.stderr, flush=True)
     return {
-        "workload": "8 flow maps of a 4 x 4 wall lattice of dynamic-phase Fulford-Blake cilia "
+        "workload": "flow maps of a 4 x 4 wall lattice of dynamic-phase Fulford-Blake cilia "
                     "(20 segments, spacing 30a) driven by recorded generalised forces, 30 "
-                    "steps each, one coupled GMRES solve per step (rtol 1e-10)",
-        "members": len(xs), "steps_per_member": ndts,
-        "concurrent_flow_maps_per_s": round(len(xs) / t_conc, 3),
-        "sequential_flow_maps_per_s": round(len(xs) / t_seq, 3),
-        "speedup": round(t_seq / t_conc, 3),
-        "max_abs_diff_concurrent_vs_sequential": diff,
+                    "steps each, one coupled FP64 GMRES solve per step (rtol 1e-10)",
+        "steps_per_member": ndts,
+        "members_8": {
+            "batched_flow_maps_per_s": round(8 / t_bat, 3),
+            "concurrent_flow_maps_per_s": round(8 / t_conc, 3),
+            "sequential_flow_maps_per_s": round(8 / t_seq, 3),
+            "batched_speedup_vs_sequential": round(t_seq / t_bat, 3),
+            "concurrent_speedup_vs_sequential": round(t_seq / t_conc, 3),
+            "max_abs_diff_concurrent_vs_sequential": diff_c,
+            "max_abs_diff_batched_vs_sequential": diff_b,
+        },
+        "members_32": {
+            "states": "monodromy matrix: phases of one state +- 1e-4",
+            "batched_flow_maps_per_s": round(32 / t_bat32, 3),
+            "concurrent_flow_maps_per_s": round(32 / t_conc32, 3),
+            "batched_speedup_vs_concurrent": round(t_conc32 / t_bat32, 3),
+            "max_abs_diff_batched_vs_concurrent": diff_b32,
+        },
     }
 
 

@@ -200,6 +200,20 @@ class cfm_batch_sys_stats(ctypes.Structure):
     ]
 
 
+class cfm_cilia_batch_config(ctypes.Structure):
+    _fields_ = [
+        ("n_members", ctypes.c_int32),
+        ("device", ctypes.c_int32),
+        ("spring_model", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("mu", ctypes.c_double),
+        ("spring", ctypes.c_double),
+        ("noise", ctypes.c_double),
+        ("seed", ctypes.c_uint64),
+        ("prm", cfm_cilia_params),
+    ]
+
+
 _dp = ctypes.POINTER(ctypes.c_double)
 _vp = ctypes.c_void_p
 _lib = None
@@ -304,6 +318,16 @@ def lib():
     L.cfm_profile.argtypes = [_vp, ctypes.c_int32]
     L.cfm_profile_read.argtypes = [_vp, ctypes.POINTER(cfm_profile_stats)]
     L.cfm_set_pair_symmetry.argtypes = [_vp, ctypes.c_int32]
+    L.cfm_cilia_batch_create.argtypes = [ctypes.POINTER(cfm_cilia_batch_config), _dp, _dp, ctypes.c_int32,
+                                         _dp, ctypes.c_int32, ctypes.POINTER(_vp)]
+    L.cfm_cilia_batch_destroy.argtypes = [_vp]
+    L.cfm_cilia_batch_last_error.argtypes = [_vp]
+    L.cfm_cilia_batch_last_error.restype = ctypes.c_char_p
+    L.cfm_cilia_batch_set_state.argtypes = [_vp, _dp, _dp, _dp, _dp]
+    L.cfm_cilia_batch_step.argtypes = [_vp, ctypes.c_int32, _dp, _dp, ctypes.POINTER(cfm_batch_sys_stats)]
+    L.cfm_cilia_batch_get_state.argtypes = [_vp, _dp, _dp, _dp, _dp]
+    L.cfm_cilia_batch_get_positions.argtypes = [_vp, _dp]
+    L.cfm_cilia_batch_get_solution.argtypes = [_vp, _dp]
     _lib = L
     return L
 
@@ -345,6 +369,12 @@ def check(ctx, code):
 def check_batch(b, code):
     if code != CFM_OK:
         msg = lib().cfm_batch_last_error(b)
+        raise CfmError(code, msg.decode() if msg else "")
+
+
+def check_cilia_batch(b, code):
+    if code != CFM_OK:
+        msg = lib().cfm_cilia_batch_last_error(b)
         raise CfmError(code, msg.decode() if msg else "")
 
 

@@ -253,3 +253,92 @@ def _rot(q):
     return np.array([[1 - 2 * (b * b + c * c), 2 * (a * b - s * c), 2 * (a * c + s * b)],
                      [2 * (a * b + s * c), 1 - 2 * (a * a + c * c), 2 * (b * c - s * a)],
                      [2 * (a * c - s * b), 2 * (b * c + s * a), 1 - 2 * (a * a + b * b)]])
+
+
+class CiliaEnsemble:
+    """n_members copies of one lattice of dynamic-phase cilia above the wall, stepped
+    together (cfm_cilia_batch_*): the same bases, beat and driving tables, each member
+    with its own phases, natural frequencies, shape angles and time step.  One step
+    advances every member as PrescribedCilia.step does for one, with one product launch
+    and one lockstep GMRES (a captured hipGraph per Arnoldi step) for all of them."""
+
+    def __init__(self, base, n_members, q_phase_table, q_angle_table=None, *, spring=0.0,
+                 nseg=20, dl=2.6, precond="diag", rtol=1e-10, restart=50, max_iter=1000,
+                 beat="fulford_blake", s_tol=0.0, mu=1.0, fene=False, noise=0.0, seed=0,
+                 device=0):
+        base = L.f64(base)
+        self.n_cilia, self.nseg, self.n_members = base.size // 3, nseg, int(n_members)
+        o = L.cfm_solve_opts(system=L.CFM_SYS_SEG, precond=RpyMobilitySolver._PC[precond],
+                             mode=L.CFM_SOLVE_FP64, restart=restart, max_iter=max_iter,
+                             use_guess=1, rtol=rtol)
+        prm = L.cfm_cilia_params(n_cilia=self.n_cilia, nseg=nseg, beat=_beat(beat), dl=dl,
+                                 dt=0.0, s_tol=s_tol, solve=o)
+        cfg = L.cfm_cilia_batch_config(n_members=self.n_members, device=device,
+                                       spring_model=int(bool(fene)), mu=mu, spring=float(spring),
+                                       noise=float(noise), seed=int(seed), prm=prm)
+        qp = L.f64(q_phase_table)
+        qa = None if q_angle_table is None else L.f64(q_angle_table)
+        self.angles = qa is not None
+        h = ctypes.c_void_p()
+        rc = L.lib().cfm_cilia_batch_create(ctypes.byref(cfg), L.ptr(base), L.ptr(qp), qp.size,
+                                            L.ptr(qa), 0 if qa is None else qa.size, ctypes.byref(h))
+        L.check_cilia_batch(None, rc)
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            L.lib().cfm_cilia_batch_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        self.close()
+
+    def set_state(self, phase, omega0, dt, angle=None):
+        """phase, omega0, angle: (n_members, n_cilia); dt: n_members time steps."""
+        n = self.n_members * self.n_cilia
+        ph, om = L.f64(phase, n), L.f64(omega0, n)
+        an = None if angle is None else L.f64(angle, n)
+        d = L.f64(dt, self.n_members)
+        L.check_cilia_batch(self._h, L.lib().cfm_cilia_batch_set_state(
+            self._h, L.ptr(ph), L.ptr(om), L.ptr(an), L.ptr(d)))
+
+    def step(self, nt):
+        """One step of every member; returns the generalised forces (q_phase, q_angle),
+        each (n_members, n_cilia), and per-member solve statistics."""
+        n = self.n_members * self.n_cilia
+        qp, qa = np.zeros(n), np.zeros(n)
+        st = (L.cfm_batch_sys_stats * self.n_members)()
+        L.check_cilia_batch(self._h, L.lib().cfm_cilia_batch_step(self._h, int(nt), L.ptr(qp), L.ptr(qa), st))
+        stats = [{"iterations": s.iterations, "converged": bool(s.converged),
+                  "rel_residual": s.rel_residual, "guess_rejected": bool(s.reserved0)} for s in st]
+        shape = (self.n_members, self.n_cilia)
+        return qp.reshape(shape), qa.reshape(shape), stats
+
+    def speeds(self):
+        n = self.n_members * self.n_cilia
+        ph, pd, an, ad = np.zeros(n), np.zeros(n), np.zeros(n), np.zeros(n)
+        L.check_cilia_batch(self._h, L.lib().cfm_cilia_batch_get_state(
+            self._h, L.ptr(ph), L.ptr(pd), L.ptr(an), L.ptr(ad)))
+        shape = (self.n_members, self.n_cilia)
+        return {"phase": ph.reshape(shape), "phase_dot": pd.reshape(shape),
+                "angle": an.reshape(shape), "angle_dot": ad.reshape(shape)}
+
+    def solution(self):
+        """Per member: segment forces (n_cilia nseg, 3), phase speeds, angle speeds."""
+        ns = self.n_cilia * self.nseg
+        lm = 3 * ns + self.n_cilia * (2 if self.angles else 1)
+        z = np.zeros(self.n_members * lm)
+        L.check_cilia_batch(self._h, L.lib().cfm_cilia_batch_get_solution(self._h, L.ptr(z)))
+        z = z.reshape(self.n_members, lm)
+        return z[:, :3 * ns].reshape(self.n_members, ns, 3), z[:, 3 * ns:]
+
+    def positions(self):
+        x = np.zeros(3 * self.n_members * self.n_cilia * self.nseg)
+        L.check_cilia_batch(self._h, L.lib().cfm_cilia_batch_get_positions(self._h, L.ptr(x)))
+        return x.reshape(self.n_members, -1, 3)

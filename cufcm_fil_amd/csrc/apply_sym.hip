@@ -55,7 +55,27 @@ struct SymArgs {
   size_t fld, vld;
   int acc;
   double mu;
+  // ensembles of equal members (blockIdx.z = member): strides of the positions, of the
+  // load / velocity vectors and of the partial sets, and a per-member status filter
+  size_t mpos, mvec, mown, msrc;
+  const int* status;  // member e runs when status[e * sstride] == want (null: all)
+  int sstride, want;
 };
+
+// The arguments of member blockIdx.z, or false when that member is not evaluated.
+template <typename T>
+__device__ __forceinline__ bool member_args(SymArgs<T>& a) {
+  const int e = blockIdx.z;
+  if (a.status && a.status[(size_t)e * a.sstride] != a.want) return false;
+  a.x += e * a.mpos;
+  a.y += e * a.mpos;
+  a.z += e * a.mpos;
+  a.f += e * a.mvec;
+  a.v += e * a.mvec;
+  a.own += e * a.mown;
+  a.src += e * a.msrc;
+  return true;
+}
 
 // Does workgroup row I take ring offset d?
 __device__ __forceinline__ bool takes(int I, int d, int B) {
@@ -244,6 +264,7 @@ __device__ __forceinline__ void stage(const SymArgs<T>& a, SymTile<T>& t, int j,
 
 template <typename T, int D>
 __global__ __launch_bounds__(256) void rpy_sym_kernel(SymArgs<T> a) {
+  if (!member_args(a)) return;
   __shared__ SymTile<T> tile[kWaves];
   __shared__ double red[kWaves * D][kBlk];
   constexpr bool kF32 = sizeof(T) == 4;
@@ -353,6 +374,7 @@ __global__ __launch_bounds__(256) void rpy_sym_kernel(SymArgs<T> a) {
 // accumulated.  One workgroup per block; wave w sums partial slots w, w+4, ...
 template <typename T, int D>
 __global__ __launch_bounds__(256) void rpy_sym_finalize_kernel(SymArgs<T> a) {
+  if (!member_args(a)) return;
   __shared__ double red[kWaves][D][kBlk];
   const int I = blockIdx.x;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -453,6 +475,45 @@ static Status launch_sym_t(cfm_ctx* c, const ApplyPlan& plan, int base, int cnt,
   hipLaunchKernelGGL((rpy_sym_kernel<T, D>), dim3(a.nblk, a.S), dim3(256), 0, stream, a);
   CFM_CHECK_LAUNCH();
   hipLaunchKernelGGL((rpy_sym_finalize_kernel<T, D>), dim3(a.nblk), dim3(256), 0, stream, a);
+  CFM_CHECK_LAUNCH();
+  return {};
+}
+
+// Ensembles of equal members (wall, 3 DOF, FP64): member e's particles sit at
+// [e ns, (e+1) ns) of the position arrays and its loads / velocities at f + e mvec,
+// v + e mvec with component stride ns; members whose status differs from `want` are
+// skipped.  Velocities are overwritten (scaled by 1/mu, self term included).
+Status sym_members_apply(hipStream_t stream, const double* x, const double* y, const double* z, int ns,
+                         int members, const double* f, size_t mvec, double* v, double mu,
+                         const int* status, int sstride, int want, DevBuf<double>& own,
+                         DevBuf<double>& src) {
+  if (ns <= 0 || members <= 0) return {};
+  SymArgs<double> a{};
+  a.x = x;
+  a.y = y;
+  a.z = z;
+  a.f = f;
+  a.v = v;
+  a.fld = ns;
+  a.vld = ns;
+  a.acc = 0;
+  a.mu = mu;
+  a.cnt = ns;
+  sym_shape(ns, &a.nblk, &a.Dn, &a.S, &a.dpw);
+  a.mpos = ns;
+  a.mvec = mvec;
+  a.mown = (size_t)a.nblk * a.S * 3 * kBlk;
+  a.msrc = (size_t)a.nblk * std::max(a.Dn, 1) * 3 * kBlk;
+  a.status = status;
+  a.sstride = sstride;
+  a.want = want;
+  CFM_TRY(own.alloc(a.mown * members));
+  CFM_TRY(src.alloc(a.msrc * members));
+  a.own = own.p;
+  a.src = src.p;
+  hipLaunchKernelGGL((rpy_sym_kernel<double, 3>), dim3(a.nblk, a.S, members), dim3(256), 0, stream, a);
+  CFM_CHECK_LAUNCH();
+  hipLaunchKernelGGL((rpy_sym_finalize_kernel<double, 3>), dim3(a.nblk, 1, members), dim3(256), 0, stream, a);
   CFM_CHECK_LAUNCH();
   return {};
 }

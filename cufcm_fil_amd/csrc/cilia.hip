@@ -1998,3 +1998,959 @@ int cfm_cilia_get_state(cfm_ctx* c, double* x, double* vel_dir_phase, double* ve
 }
 
 }  // extern "C"
+
+// ---- ensembles of dynamic-phase cilia lattices -----------------------------------------
+// The reference's outer loops (pyfile/find_periodic/newton_solver.py:49-74,
+// pyfile/bisection/bisection.py:150-162) run each flow map as its own process.  Here the
+// members of an ensemble -- the same lattice of cilia above the wall, the same beat and
+// driving tables, each member with its own phases, shape angles, natural frequencies and
+// time step -- step together: one geometry launch for every cilium of every member, the
+// wall-RPY product of every member by the symmetric kernel with one grid slice per
+// member, and one GMRES in lockstep with per-member Krylov bases, Hessenbergs, Givens
+// rotations and convergence.  One Arnoldi step of the ensemble is a fixed kernel
+// sequence whose step index lives in device memory, captured once into a hipGraph and
+// replayed.
+//
+// Member e's system vector is the slice [e Lm, (e+1) Lm) of the ensemble vector with
+// the single-system layout: forces [3][ns] (component-major), then n_cilia phase
+// speeds and, with shape rotation, n_cilia angle speeds.  Geometry arrays (positions,
+// vel_dir_phase / vel_dir_angle) are SoA over all members' segments.
+
+namespace cfm {
+
+Status sym_members_apply(hipStream_t stream, const double* x, const double* y, const double* z, int ns,
+                         int members, const double* f, size_t mvec, double* v, double mu,
+                         const int* status, int sstride, int want, DevBuf<double>& own,
+                         DevBuf<double>& src);
+
+namespace {
+
+// member status (slot 0 of iinfo)
+constexpr int CE_DONE = 0, CE_RUN = 1, CE_CHECK = 2;
+constexpr int CE_ST = 0, CE_K = 1, CE_ITERS = 2, CE_UPD = 3, CE_DONE_AT = 4, CE_REJ = 5;
+constexpr int CE_TARGET = 0, CE_EST = 1, CE_INV = 2, CE_NR = 3, CE_NB = 4;
+
+struct CEK {
+  int E, ncil, nseg, ns, ne, Lm, dyn_angle, m;
+  size_t L, Nseg;         // E Lm; E ns (stride of the SoA geometry arrays)
+  const double* vphase;   // 3 x Nseg
+  const double* vangle;   // 3 x Nseg
+  const double* z;        // segment heights (Nseg)
+  double* Q;              // (m + 1) x L
+  double* H;              // E x (m + 1) x (m + 1), column j at H[e][j][0..m]
+  double *cs, *sn, *beta, *h, *h2, *y;  // E x (m + 2)
+  double* part;           // (m + 1) x E
+  double* sinfo;          // E x 8
+  int* iinfo;             // E x 8
+  int* jdev;              // [0] step j, [1] steps taken, [2] members running
+};
+
+__device__ __forceinline__ int ce_status(const CEK& k, int e) { return k.iinfo[e * 8 + CE_ST]; }
+
+// Block preconditioner of one cilium (cilia_precond_kernel for member slices): mode 0
+// on running members from Q_j, mode 1 on members whose cycle ended from `in`.
+__global__ void ce_precond_kernel(CEK k, int kind, double mu, int mode, const double* __restrict__ in,
+                                  double* __restrict__ out) {
+  const int gcil = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gcil >= k.E * k.ncil) return;
+  const int e = gcil / k.ncil, c = gcil - e * k.ncil;
+  if (mode == 0 ? ce_status(k, e) != CE_RUN : !k.iinfo[e * 8 + CE_UPD]) return;
+  const double* r = (mode == 0 ? k.Q + (size_t)k.jdev[0] * k.L : in) + (size_t)e * k.Lm;
+  double* o = out + (size_t)e * k.Lm;
+  const size_t ns = k.ns, N = k.Nseg;
+  double s11 = 0.0, s12 = 0.0, s22 = 0.0, dp = 0.0, da = 0.0;
+  for (int i = 0; i < k.nseg; ++i) {
+    const size_t g = (size_t)c * k.nseg + i, gg = (size_t)e * ns + g;
+    double rv[3], vp[3], va[3], pr[3], pp[3], pa[3];
+    for (int d = 0; d < 3; ++d) {
+      rv[d] = r[d * ns + g];
+      vp[d] = k.vphase[d * N + gg];
+      va[d] = k.vangle[d * N + gg];
+    }
+    particle_precond(kind, 1, 3, k.z[gg], 1.0, mu, rv, pr);
+    particle_precond(kind, 1, 3, k.z[gg], 1.0, mu, vp, pp);
+    particle_precond(kind, 1, 3, k.z[gg], 1.0, mu, va, pa);
+    for (int d = 0; d < 3; ++d) {
+      s11 += vp[d] * pp[d];
+      s12 += vp[d] * pa[d];
+      s22 += va[d] * pa[d];
+      dp += vp[d] * pr[d];
+      da += va[d] * pr[d];
+      o[d * ns + g] = pr[d];
+    }
+  }
+  const double tp = -r[3 * ns + c] - dp;
+  double wp, wa = 0.0;
+  if (k.dyn_angle) {
+    const double ta = -r[3 * ns + k.ncil + c] - da;
+    const double det = s11 * s22 - s12 * s12;
+    wp = (s22 * tp - s12 * ta) / det;
+    wa = (s11 * ta - s12 * tp) / det;
+    o[3 * ns + k.ncil + c] = wa;
+  } else {
+    wp = tp / s11;
+  }
+  o[3 * ns + c] = wp;
+  for (int i = 0; i < k.nseg; ++i) {
+    const size_t g = (size_t)c * k.nseg + i, gg = (size_t)e * ns + g;
+    double vp[3], va[3], pp[3], pa[3];
+    for (int d = 0; d < 3; ++d) {
+      vp[d] = k.vphase[d * N + gg];
+      va[d] = k.vangle[d * N + gg];
+    }
+    particle_precond(kind, 1, 3, k.z[gg], 1.0, mu, vp, pp);
+    particle_precond(kind, 1, 3, k.z[gg], 1.0, mu, va, pa);
+    for (int d = 0; d < 3; ++d) o[d * ns + g] += wp * pp[d] + (k.dyn_angle ? wa * pa[d] : 0.0);
+  }
+}
+
+// Coupling rows of the members with status `want` (after y_F = M x_F):
+// y_F -= phi_dot K + psi_dot A, y_phi = -K . F, y_psi = -A . F (cilia_couple_kernel).
+__global__ void ce_couple_kernel(CEK k, int want, const double* __restrict__ x, double* __restrict__ y) {
+  const int gcil = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gcil >= k.E * k.ncil) return;
+  const int e = gcil / k.ncil, c = gcil - e * k.ncil;
+  if (ce_status(k, e) != want) return;
+  const double* xe = x + (size_t)e * k.Lm;
+  double* ye = y + (size_t)e * k.Lm;
+  const size_t ns = k.ns, N = k.Nseg;
+  const double wp = xe[3 * ns + c];
+  const double wa = k.dyn_angle ? xe[3 * ns + k.ncil + c] : 0.0;
+  double dp = 0.0, da = 0.0;
+  for (int i = 0; i < k.nseg; ++i) {
+    const size_t g = (size_t)c * k.nseg + i, gg = (size_t)e * ns + g;
+    for (int d = 0; d < 3; ++d) {
+      const double vp = k.vphase[d * N + gg], va = k.vangle[d * N + gg];
+      const double fv = xe[d * ns + g];
+      dp += vp * fv;
+      da += va * fv;
+      ye[d * ns + g] -= wp * vp + (k.dyn_angle ? wa * va : 0.0);
+    }
+  }
+  ye[3 * ns + c] = -dp;
+  if (k.dyn_angle) ye[3 * ns + k.ncil + c] = -da;
+}
+
+// part[i][e] = <Q_i, w> over member e (i <= j; or <w, w> with norm), one workgroup per
+// (member, vector), summed in a fixed order.
+__global__ __launch_bounds__(256) void ce_dots_kernel(CEK k, const double* __restrict__ w, int norm, int want) {
+  __shared__ double red[4];
+  const int e = blockIdx.x, i = blockIdx.y;
+  if (ce_status(k, e) != want) return;
+  if (norm ? i > 0 : i > k.jdev[0]) return;
+  const double* q = (norm ? w : k.Q + (size_t)i * k.L) + (size_t)e * k.Lm;
+  const double* we = w + (size_t)e * k.Lm;
+  double s = 0.0;
+  for (int t = threadIdx.x; t < k.Lm; t += 256) s += q[t] * we[t];
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) k.part[(size_t)i * k.E + e] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+// pass 0: h = part; pass 1: h2 = part, H[:, j] = h + h2.
+__global__ void ce_finish_kernel(CEK k, int pass) {
+  const int e = blockIdx.x;
+  if (ce_status(k, e) != CE_RUN) return;
+  const int j = k.jdev[0];
+  const int ld = k.m + 2;
+  for (int i = threadIdx.x; i <= j; i += blockDim.x) {
+    const double v = k.part[(size_t)i * k.E + e];
+    if (pass == 0) {
+      k.h[(size_t)e * ld + i] = v;
+    } else {
+      k.h2[(size_t)e * ld + i] = v;
+      k.H[((size_t)e * (k.m + 1) + j) * (k.m + 1) + i] = k.h[(size_t)e * ld + i] + v;
+    }
+  }
+}
+
+// w -= sum_{i <= j} h[e][i] Q_i over running members.
+__global__ void ce_sub_kernel(CEK k, double* __restrict__ w, int pass) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= k.L) return;
+  const int e = (int)(t / k.Lm);
+  if (ce_status(k, e) != CE_RUN) return;
+  const int j = k.jdev[0];
+  const double* hh = (pass == 0 ? k.h : k.h2) + (size_t)e * (k.m + 2);
+  double acc = 0.0;
+  for (int i = 0; i <= j; ++i) acc += hh[i] * k.Q[(size_t)i * k.L + t];
+  w[t] -= acc;
+}
+
+__device__ void ce_backsub(const CEK& k, int e, int kk) {
+  const int ldh = k.m + 1;
+  const double* H = k.H + (size_t)e * ldh * ldh;
+  const double* beta = k.beta + (size_t)e * (k.m + 2);
+  double* y = k.y + (size_t)e * (k.m + 2);
+  for (int i = kk - 1; i >= 0; --i) {
+    double acc = beta[i];
+    for (int c = i + 1; c < kk; ++c) acc -= H[(size_t)c * ldh + i] * y[c];
+    y[i] = acc / H[(size_t)i * ldh + i];
+  }
+}
+
+// Norm of w, Givens rotations of column j, residual estimate and the per-member
+// decisions of this step (mobility_solver.cpp:2467-2510 rotation formulas); thread 0
+// then advances the step counters.
+__global__ __launch_bounds__(256) void ce_givens_kernel(CEK k) {
+  __shared__ int run_count;
+  if (threadIdx.x == 0) run_count = 0;
+  __syncthreads();
+  const int j = k.jdev[0];
+  const int ldh = k.m + 1, ld = k.m + 2;
+  for (int e = threadIdx.x; e < k.E; e += blockDim.x) {
+    int* ii = k.iinfo + e * 8;
+    ii[CE_UPD] = 0;
+    if (ii[CE_ST] != CE_RUN) continue;
+    const double nrm = sqrt(k.part[e]);
+    double* col = k.H + ((size_t)e * ldh + j) * ldh;
+    col[j + 1] = nrm;
+    double* cs = k.cs + (size_t)e * ld;
+    double* sn = k.sn + (size_t)e * ld;
+    for (int i = 0; i < j; ++i) {
+      const double t = cs[i] * col[i] + sn[i] * col[i + 1];
+      col[i + 1] = -sn[i] * col[i] + cs[i] * col[i + 1];
+      col[i] = t;
+    }
+    const double a = col[j], b = col[j + 1];
+    const double den = sqrt(a * a + b * b);
+    const double c_ = den > 0.0 ? a / den : 1.0, s_ = den > 0.0 ? b / den : 0.0;
+    cs[j] = c_;
+    sn[j] = s_;
+    col[j] = c_ * a + s_ * b;
+    col[j + 1] = 0.0;
+    double* beta = k.beta + (size_t)e * ld;
+    beta[j + 1] = -s_ * beta[j];
+    beta[j] = c_ * beta[j];
+    const double est = fabs(beta[j + 1]);
+    double* si = k.sinfo + e * 8;
+    si[CE_EST] = est;
+    si[CE_INV] = nrm > 0.0 ? 1.0 / nrm : 0.0;
+    ii[CE_K] = j + 1;
+    ii[CE_ITERS] += 1;
+    const bool conv = est <= si[CE_TARGET];
+    if (conv) ii[CE_DONE_AT] = k.jdev[1];
+    if (conv || j + 1 >= k.m || !(nrm > 0.0) || !isfinite(est)) {
+      ce_backsub(k, e, j + 1);
+      ii[CE_UPD] = 1;
+      ii[CE_ST] = CE_CHECK;
+    } else {
+      atomicAdd(&run_count, 1);
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    k.jdev[0] = j + 1;
+    k.jdev[1] += 1;
+    k.jdev[2] = run_count;
+  }
+}
+
+// Q_{j+1} = w / ||w|| for members still running (j already advanced).
+__global__ void ce_scale_kernel(CEK k, const double* __restrict__ w) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= k.L) return;
+  const int e = (int)(t / k.Lm);
+  if (ce_status(k, e) != CE_RUN) return;
+  k.Q[(size_t)k.jdev[0] * k.L + t] = w[t] * k.sinfo[e * 8 + CE_INV];
+}
+
+// t = Q y over the members whose cycle ended at this step.
+__global__ void ce_qy_kernel(CEK k, double* __restrict__ out) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= k.L) return;
+  const int e = (int)(t / k.Lm);
+  const int* ii = k.iinfo + e * 8;
+  if (!ii[CE_UPD]) return;
+  const double* y = k.y + (size_t)e * (k.m + 2);
+  double acc = 0.0;
+  for (int i = 0; i < ii[CE_K]; ++i) acc += y[i] * k.Q[(size_t)i * k.L + t];
+  out[t] = acc;
+}
+
+// x += d over the members whose cycle ended at this step.
+__global__ void ce_add_kernel(CEK k, double* __restrict__ x, const double* __restrict__ d) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= k.L) return;
+  const int e = (int)(t / k.Lm);
+  if (!k.iinfo[e * 8 + CE_UPD]) return;
+  x[t] += d[t];
+}
+
+// r = b - w over members with status `want`.
+__global__ void ce_resid_kernel(CEK k, int want, const double* __restrict__ b, const double* __restrict__ w,
+                                double* __restrict__ r) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= k.L) return;
+  const int e = (int)(t / k.Lm);
+  if (ce_status(k, e) != want) return;
+  r[t] = b[t] - w[t];
+}
+
+// ||b|| per member (part[0][e] holds <b, b>); every member enters CE_CHECK with target
+// rtol ||b||; a zero right-hand side gives the zero solution at once
+// (mobility_solver.cpp:2221-2294).
+__global__ void ce_setup_kernel(CEK k, double rtol) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= k.E) return;
+  double* si = k.sinfo + e * 8;
+  int* ii = k.iinfo + e * 8;
+  si[CE_NB] = sqrt(k.part[e]);
+  si[CE_TARGET] = rtol * si[CE_NB];
+  si[CE_NR] = 0.0;
+  ii[CE_ST] = si[CE_NB] > 0.0 ? CE_CHECK : CE_DONE;
+  ii[CE_ITERS] = 0;
+  ii[CE_UPD] = 0;
+  ii[CE_K] = 0;
+  ii[CE_DONE_AT] = -1;
+  ii[CE_REJ] = si[CE_NB] > 0.0 ? 0 : 1;
+}
+
+// The guess is rejected when its residual exceeds ||b|| (mobility_solver.cpp:2296-2388):
+// part[0][e] holds ||b - A x0||^2.
+__global__ void ce_guess_kernel(CEK k) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= k.E) return;
+  if (ce_status(k, e) != CE_CHECK) return;
+  const double* si = k.sinfo + e * 8;
+  k.iinfo[e * 8 + CE_REJ] = sqrt(k.part[e]) > si[CE_NB] ? 1 : 0;
+}
+
+// Rejected guesses (and zero right-hand sides): x = 0, r = b.
+__global__ void ce_reject_kernel(CEK k, const double* __restrict__ b, double* __restrict__ x,
+                                 double* __restrict__ r) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= k.L) return;
+  const int e = (int)(t / k.Lm);
+  if (!k.iinfo[e * 8 + CE_REJ]) return;
+  x[t] = 0.0;
+  r[t] = b[t];
+}
+
+// After the residual norms of CE_CHECK members (part[0][e]): converged -> CE_DONE, else
+// restart with beta = (||r||, 0, ...).
+__global__ void ce_restart_kernel(CEK k, int max_iter) {
+  __shared__ int run_count;
+  if (threadIdx.x == 0) run_count = 0;
+  __syncthreads();
+  for (int e = threadIdx.x; e < k.E; e += blockDim.x) {
+    int* ii = k.iinfo + e * 8;
+    if (ii[CE_ST] != CE_CHECK) continue;
+    const double nr = sqrt(k.part[e]);
+    double* si = k.sinfo + e * 8;
+    si[CE_NR] = nr;
+    if (nr <= si[CE_TARGET] || ii[CE_ITERS] >= max_iter || !isfinite(nr)) {
+      ii[CE_ST] = CE_DONE;
+      continue;
+    }
+    si[CE_INV] = 1.0 / nr;
+    double* beta = k.beta + (size_t)e * (k.m + 2);
+    beta[0] = nr;
+    for (int i = 1; i < k.m + 2; ++i) beta[i] = 0.0;
+    ii[CE_K] = 0;
+    ii[CE_ST] = CE_RUN;
+    atomicAdd(&run_count, 1);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    k.jdev[0] = 0;
+    k.jdev[2] = run_count;
+  }
+}
+
+// Q_0 = r / ||r|| for members that restarted.
+__global__ void ce_q0_kernel(CEK k, const double* __restrict__ r) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= k.L) return;
+  const int e = (int)(t / k.Lm);
+  if (ce_status(k, e) != CE_RUN) return;
+  k.Q[t] = r[t] * k.sinfo[e * 8 + CE_INV];
+}
+
+// rhs of every member: [0 ; -q_phase ; -q_angle] (cilia_drive_kernel per member slice;
+// the noise deviate is keyed on the cilium's index within its member).
+__global__ void ce_drive_kernel(CEK k, double len, double spring, const double* __restrict__ phase,
+                                const double* __restrict__ angle, const double* __restrict__ omega0,
+                                const double* __restrict__ tab_p, int np, const double* __restrict__ tab_a,
+                                int na, DriveOpts dopt, double* __restrict__ rhs) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= k.L) return;
+  const int e = (int)(t / k.Lm);
+  const int loc = (int)(t - (size_t)e * k.Lm);
+  const int off = 3 * k.ns;
+  if (loc < off) {
+    rhs[t] = 0.0;
+    return;
+  }
+  const int c = (loc - off) % k.ncil;
+  const int gc_ = e * k.ncil + c;
+  const double sc = 0.5 * omega0[gc_] / 3.14159265358979323846;
+  if (loc < off + k.ncil) {
+    double q = cyclic_hermite(tab_p, np, phase[gc_]) * sc;
+    if (dopt.noise != 0.0) q += dopt.noise * normal_deviate(dopt.seed, dopt.nt, c);
+    rhs[t] = -q;
+  } else {
+    const double a = angle[gc_], amax = 0.78;
+    double fac = a;
+    if (dopt.fene && fabs(a) > amax) fac = 0.5 * a * (1.0 + a * a / (amax * amax) * exp(fabs(a) - amax));
+    const double q = cyclic_hermite(tab_a, na, phase[gc_]) * sc - spring * omega0[gc_] * len * len * len * fac;
+    rhs[t] = -q;
+  }
+}
+
+// Speeds and generalised forces of every cilium from the members' solutions.
+__global__ void ce_unpack_kernel(CEK k, const double* __restrict__ sol, double* __restrict__ pdot,
+                                 double* __restrict__ adot, double* __restrict__ q) {
+  const int gcil = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gcil >= k.E * k.ncil) return;
+  const int e = gcil / k.ncil, c = gcil - e * k.ncil;
+  const double* xe = sol + (size_t)e * k.Lm;
+  const size_t ns = k.ns, N = k.Nseg;
+  pdot[gcil] = xe[3 * ns + c];
+  adot[gcil] = k.dyn_angle ? xe[3 * ns + k.ncil + c] : 0.0;
+  double qp = 0.0, qa = 0.0;
+  for (int i = 0; i < k.nseg; ++i) {
+    const size_t g = (size_t)c * k.nseg + i, gg = (size_t)e * ns + g;
+    for (int d = 0; d < 3; ++d) {
+      const double fv = xe[d * ns + g];
+      qp += k.vphase[d * N + gg] * fv;
+      qa += k.vangle[d * N + gg] * fv;
+    }
+  }
+  q[gcil] = qp;
+  q[(size_t)k.E * k.ncil + gcil] = qa;
+}
+
+// phase += dt_e phase_dot (and the shape angle) before a step, per member time step.
+__global__ void ce_advance_kernel(int n, int ncil, int dyn_angle, const double* __restrict__ dt,
+                                  const double* __restrict__ pdot, const double* __restrict__ adot,
+                                  double* __restrict__ phase, double* __restrict__ angle) {
+  const int gcil = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gcil >= n) return;
+  const double h = dt[gcil / ncil];
+  phase[gcil] += pdot[gcil] * h;
+  if (dyn_angle) angle[gcil] += adot[gcil] * h;
+}
+
+// the initial guess of every member: forces zero, phase speeds = natural frequencies,
+// shape-angle speeds zero
+__global__ void ce_init_guess_kernel(CEK k, const double* __restrict__ omega0, double* __restrict__ x) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= k.L) return;
+  const int e = (int)(t / k.Lm);
+  const int loc = (int)(t - (size_t)e * k.Lm), off = 3 * k.ns;
+  x[t] = (loc >= off && loc < off + k.ncil) ? omega0[e * k.ncil + (loc - off)] : 0.0;
+}
+
+__global__ void bfill_cilia_kernel(double* __restrict__ x, double v, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = v;
+}
+
+__global__ void ce_zero_int_kernel(int* __restrict__ p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+__global__ void ce_status_fill_kernel(int* __restrict__ iinfo, int E, int st) {
+  for (int e = threadIdx.x; e < E; e += blockDim.x) iinfo[e * 8 + CE_ST] = st;
+}
+
+inline dim3 g256(size_t n) { return dim3((unsigned)((n + 255) / 256)); }
+
+}  // namespace
+}  // namespace cfm
+
+using namespace cfm;
+
+struct cfm_cilia_batch {
+  cfm_cilia_batch_config cfg{};
+  hipStream_t stream = nullptr;
+  int E = 0, nc = 0, nseg = 0, ns = 0, ne = 0, Lm = 0, dyn_angle = 0, m = 0;
+  size_t L = 0, N = 0;
+  CiliaK geo{};
+  DevBuf<double> base, phase, angle, pdot, adot, omega0, dt, x, y, z, vphase, vangle, svals, q;
+  DevBuf<double> tab_phase, tab_angle;
+  int n_tab_phase = 0, n_tab_angle = 0;
+  DevBuf<double> sol, rhs, r, w, zt, t;
+  DevBuf<double> Q, H, cs, sn, beta, h, h2, yv, part, sinfo;
+  DevBuf<int> iinfo, jdev;
+  DevBuf<double> own, src;  // partials of the symmetric product
+  int* host_flags = nullptr;  // pinned: [0, 4) counters after a residual pass, [4, 8) ring
+  double* host = nullptr;     // pinned read-back of q
+  hipGraphExec_t graph = nullptr;
+  std::vector<hipEvent_t> ev;
+  bool have_state = false;
+  std::string err;
+};
+
+namespace {
+
+int cefail(cfm_cilia_batch* b, const Status& s) {
+  if (b) b->err = s.msg;
+  return s.code;
+}
+
+CEK cek_of(cfm_cilia_batch* b) {
+  CEK k{};
+  k.E = b->E;
+  k.ncil = b->nc;
+  k.nseg = b->nseg;
+  k.ns = b->ns;
+  k.ne = b->ne;
+  k.Lm = b->Lm;
+  k.dyn_angle = b->dyn_angle;
+  k.m = b->m;
+  k.L = b->L;
+  k.Nseg = b->N;
+  k.vphase = b->vphase.p;
+  k.vangle = b->vangle.p;
+  k.z = b->z.p;
+  k.Q = b->Q.p;
+  k.H = b->H.p;
+  k.cs = b->cs.p;
+  k.sn = b->sn.p;
+  k.beta = b->beta.p;
+  k.h = b->h.p;
+  k.h2 = b->h2.p;
+  k.y = b->yv.p;
+  k.part = b->part.p;
+  k.sinfo = b->sinfo.p;
+  k.iinfo = b->iinfo.p;
+  k.jdev = b->jdev.p;
+  return k;
+}
+
+// out_F = M x_F (members with status want, FP64 symmetric product), then the coupling
+// rows of those members.
+Status ce_product(cfm_cilia_batch* b, int want, const double* x, double* out) {
+  const CEK k = cek_of(b);
+  CFM_TRY(sym_members_apply(b->stream, b->x.p, b->y.p, b->z.p, b->ns, b->E, x, (size_t)b->Lm, out, b->cfg.mu,
+                            b->iinfo.p, 8, want, b->own, b->src));
+  hipLaunchKernelGGL(ce_couple_kernel, gc(b->E * b->nc), dim3(64), 0, b->stream, k, want, x, out);
+  CFM_CHECK_LAUNCH();
+  return {};
+}
+
+// One Arnoldi step of every running member (captured into the graph).
+Status ce_enqueue_step(cfm_cilia_batch* b) {
+  const CEK k = cek_of(b);
+  const int pc = b->cfg.prm.solve.precond;
+  const double mu = b->cfg.mu;
+  const int ncl = b->E * b->nc;
+  hipLaunchKernelGGL(ce_precond_kernel, gc(ncl), dim3(64), 0, b->stream, k, pc, mu, 0, (const double*)nullptr,
+                     b->zt.p);
+  CFM_CHECK_LAUNCH();
+  CFM_TRY(ce_product(b, CE_RUN, b->zt.p, b->w.p));
+  for (int pass = 0; pass < 2; ++pass) {
+    hipLaunchKernelGGL(ce_dots_kernel, dim3(b->E, b->m + 1), dim3(256), 0, b->stream, k, b->w.p, 0, CE_RUN);
+    hipLaunchKernelGGL(ce_finish_kernel, dim3(b->E), dim3(64), 0, b->stream, k, pass);
+    hipLaunchKernelGGL(ce_sub_kernel, g256(b->L), dim3(256), 0, b->stream, k, b->w.p, pass);
+  }
+  hipLaunchKernelGGL(ce_dots_kernel, dim3(b->E, 1), dim3(256), 0, b->stream, k, b->w.p, 1, CE_RUN);
+  hipLaunchKernelGGL(ce_givens_kernel, dim3(1), dim3(256), 0, b->stream, k);
+  hipLaunchKernelGGL(ce_scale_kernel, g256(b->L), dim3(256), 0, b->stream, k, b->w.p);
+  // x += P^-1 (Q y) for members whose cycle ended
+  hipLaunchKernelGGL(ce_qy_kernel, g256(b->L), dim3(256), 0, b->stream, k, b->t.p);
+  hipLaunchKernelGGL(ce_precond_kernel, gc(ncl), dim3(64), 0, b->stream, k, pc, mu, 1, (const double*)b->t.p,
+                     b->zt.p);
+  hipLaunchKernelGGL(ce_add_kernel, g256(b->L), dim3(256), 0, b->stream, k, b->sol.p, (const double*)b->zt.p);
+  CFM_CHECK_LAUNCH();
+  return {};
+}
+
+Status ce_event(cfm_cilia_batch* b, size_t i, hipEvent_t* e) {
+  while (b->ev.size() <= i) {
+    hipEvent_t t;
+    CFM_HIP(hipEventCreate(&t));
+    b->ev.push_back(t);
+  }
+  *e = b->ev[i];
+  return {};
+}
+
+Status ce_graph(cfm_cilia_batch* b, hipGraphExec_t* out) {
+  if (!b->graph) {
+    hipGraph_t g = nullptr;
+    CFM_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    Status s = ce_enqueue_step(b);
+    hipError_t e = hipStreamEndCapture(b->stream, &g);
+    if (!s.ok()) {
+      if (g) (void)hipGraphDestroy(g);
+      return s;
+    }
+    if (e != hipSuccess) return make_status(CFM_EHIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return make_status(CFM_EHIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+  }
+  *out = b->graph;
+  return {};
+}
+
+// Graph replays until no member is running or the cycle is full; the running count of
+// step j is copied to ring slot 4 + j % 4 after the replay and read one step late.
+Status ce_cycle(cfm_cilia_batch* b, hipGraphExec_t g, size_t* nev) {
+  int pending = -1, pending_slot = 0;
+  for (int j = 0; j < b->m; ++j) {
+    CFM_HIP(hipGraphLaunch(g, b->stream));
+    const int slot = 4 + (j % 4);
+    CFM_HIP(hipMemcpyAsync(b->host_flags + slot, b->jdev.p + 2, sizeof(int), hipMemcpyDeviceToHost, b->stream));
+    hipEvent_t e;
+    CFM_TRY(ce_event(b, *nev, &e));
+    CFM_HIP(hipEventRecord(e, b->stream));
+    const size_t mine = (*nev)++;
+    if (pending >= 0) {
+      hipEvent_t pe;
+      CFM_TRY(ce_event(b, (size_t)pending, &pe));
+      CFM_HIP(hipEventSynchronize(pe));
+      if (b->host_flags[pending_slot] == 0) break;
+    }
+    pending = (int)mine;
+    pending_slot = slot;
+  }
+  CFM_HIP(hipStreamSynchronize(b->stream));
+  return {};
+}
+
+// Every member's solve of M x = rhs from the guess in b->sol (the single-system solve's
+// rule: restarted GMRES(m) with the true residual at each cycle's end, the guess
+// rejected when worse than zero).
+Status ce_solve(cfm_cilia_batch* b, std::vector<double>* si, std::vector<int>* ii) {
+  const CEK k = cek_of(b);
+  const cfm_solve_opts& o = b->cfg.prm.solve;
+  const int max_iter = o.max_iter > 0 ? o.max_iter : 1000;
+  hipGraphExec_t g = nullptr;  // captured before the first cycle, after the first product
+                               // has sized the product's partial buffers
+  hipLaunchKernelGGL(ce_zero_int_kernel, dim3(1), dim3(64), 0, b->stream, b->jdev.p, 4);
+  CFM_CHECK_LAUNCH();
+  // ||b||: every member in CE_DONE for the norm pass, then targets
+  hipLaunchKernelGGL(ce_status_fill_kernel, dim3(1), dim3(256), 0, b->stream, b->iinfo.p, b->E, CE_DONE);
+  hipLaunchKernelGGL(ce_dots_kernel, dim3(b->E, 1), dim3(256), 0, b->stream, k, b->rhs.p, 1, CE_DONE);
+  hipLaunchKernelGGL(ce_setup_kernel, gc(b->E), dim3(64), 0, b->stream, k, o.rtol);
+  CFM_CHECK_LAUNCH();
+  size_t nev = 0;
+  for (int pass = 0; pass < 4 * (max_iter / std::max(1, b->m) + 2); ++pass) {
+    // FP64 residual of the members to check: r = b - A x
+    CFM_TRY(ce_product(b, CE_CHECK, b->sol.p, b->w.p));
+    hipLaunchKernelGGL(ce_resid_kernel, g256(b->L), dim3(256), 0, b->stream, k, CE_CHECK, b->rhs.p, b->w.p, b->r.p);
+    hipLaunchKernelGGL(ce_dots_kernel, dim3(b->E, 1), dim3(256), 0, b->stream, k, b->r.p, 1, CE_CHECK);
+    if (pass == 0) {  // reject guesses worse than zero
+      hipLaunchKernelGGL(ce_guess_kernel, gc(b->E), dim3(64), 0, b->stream, k);
+      hipLaunchKernelGGL(ce_reject_kernel, g256(b->L), dim3(256), 0, b->stream, k, b->rhs.p, b->sol.p, b->r.p);
+      hipLaunchKernelGGL(ce_dots_kernel, dim3(b->E, 1), dim3(256), 0, b->stream, k, b->r.p, 1, CE_CHECK);
+    }
+    hipLaunchKernelGGL(ce_restart_kernel, dim3(1), dim3(256), 0, b->stream, k, max_iter);
+    hipLaunchKernelGGL(ce_q0_kernel, g256(b->L), dim3(256), 0, b->stream, k, b->r.p);
+    CFM_CHECK_LAUNCH();
+    CFM_HIP(hipMemcpyAsync(b->host_flags, b->jdev.p, sizeof(int) * 4, hipMemcpyDeviceToHost, b->stream));
+    CFM_HIP(hipStreamSynchronize(b->stream));
+    if (b->host_flags[2] == 0) break;  // nobody left running
+    if (!g) CFM_TRY(ce_graph(b, &g));
+    CFM_TRY(ce_cycle(b, g, &nev));
+  }
+  si->resize((size_t)8 * b->E);
+  ii->resize((size_t)8 * b->E);
+  CFM_HIP(copy_to_host(b->stream, si->data(), b->sinfo.p, sizeof(double) * si->size()));
+  CFM_HIP(copy_to_host(b->stream, ii->data(), b->iinfo.p, sizeof(int) * ii->size()));
+  return {};
+}
+
+}  // namespace
+
+extern "C" {
+
+static thread_local std::string g_ce_create_err;  // message of the last failed create
+
+const char* cfm_cilia_batch_last_error(const cfm_cilia_batch* b) {
+  return b ? b->err.c_str() : g_ce_create_err.c_str();
+}
+
+int cfm_cilia_batch_destroy(cfm_cilia_batch* b) {
+  if (!b) return CFM_EINVAL;
+  (void)hipSetDevice(b->cfg.device);
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  if (b->graph) (void)hipGraphExecDestroy(b->graph);
+  for (hipEvent_t e : b->ev) (void)hipEventDestroy(e);
+  for (auto* v : {&b->base, &b->phase, &b->angle, &b->pdot, &b->adot, &b->omega0, &b->dt, &b->x, &b->y, &b->z,
+                  &b->vphase, &b->vangle, &b->svals, &b->q, &b->tab_phase, &b->tab_angle, &b->sol, &b->rhs, &b->r,
+                  &b->w, &b->zt, &b->t, &b->Q, &b->H, &b->cs, &b->sn, &b->beta, &b->h, &b->h2, &b->yv, &b->part,
+                  &b->sinfo, &b->own, &b->src})
+    v->release();
+  b->iinfo.release();
+  b->jdev.release();
+  if (b->host_flags) (void)hipHostFree(b->host_flags);
+  if (b->host) (void)hipHostFree(b->host);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+  return CFM_OK;
+}
+
+int cfm_cilia_batch_create(const cfm_cilia_batch_config* cfg, const double* base, const double* q_phase_table,
+                           int32_t n_phase, const double* q_angle_table, int32_t n_angle, cfm_cilia_batch** out) {
+  if (!out) return CFM_EINVAL;
+  *out = nullptr;
+  auto fail = [&](int code, const char* msg) {
+    g_ce_create_err = msg;
+    return code;
+  };
+  if (!cfg || !base || !q_phase_table) return fail(CFM_EINVAL, "null argument");
+  const cfm_cilia_params& prm = cfg->prm;
+  if (cfg->n_members <= 0 || prm.n_cilia <= 0) return fail(CFM_EINVAL, "n_members and n_cilia must be positive");
+  if (prm.nseg < 2 || prm.nseg > kMaxCiliaSeg) return fail(CFM_EINVAL, "nseg must be in 2..64");
+  if (prm.beat < CFM_BEAT_FULFORD_BLAKE || prm.beat > CFM_BEAT_BUILD_A_BEAT) return fail(CFM_EINVAL, "unknown beat");
+  if (!(prm.dl > 0.0) || !(cfg->mu > 0.0)) return fail(CFM_EINVAL, "dl and mu must be positive");
+  if (prm.solve.precond != CFM_PC_DIAG && prm.solve.precond != CFM_PC_SELF)
+    return fail(CFM_EINVAL, "the coupled cilia system takes CFM_PC_DIAG or CFM_PC_SELF");
+  if (prm.solve.mode != CFM_SOLVE_FP64) return fail(CFM_EINVAL, "ensembles solve in FP64 (CFM_SOLVE_FP64)");
+  if (!(prm.solve.rtol > 0.0)) return fail(CFM_EINVAL, "rtol must be positive");
+  if (prm.solve.restart < 0 || prm.solve.max_iter < 0) return fail(CFM_EINVAL, "restart and max_iter must be >= 0");
+  if (n_phase < 2 || (q_angle_table && n_angle < 2)) return fail(CFM_EINVAL, "driving tables need at least 2 values");
+  if (cfg->spring_model != 0 && cfg->spring_model != 1) return fail(CFM_EINVAL, "spring_model is 0 or 1");
+  if (!std::isfinite(cfg->spring) || !std::isfinite(cfg->noise)) return fail(CFM_EINVAL, "non-finite spring or noise");
+  for (int i = 0; i < n_phase; ++i)
+    if (!std::isfinite(q_phase_table[i])) return fail(CFM_EINVAL, "non-finite table");
+  if (q_angle_table)
+    for (int i = 0; i < n_angle; ++i)
+      if (!std::isfinite(q_angle_table[i])) return fail(CFM_EINVAL, "non-finite table");
+  const int nc = prm.n_cilia;
+  for (int i = 0; i < nc; ++i) {
+    for (int d = 0; d < 3; ++d)
+      if (!std::isfinite(base[3 * i + d])) return fail(CFM_EINVAL, "non-finite base");
+    if (!(base[3 * i + 2] > 0.0)) return fail(CFM_EINVAL, "a cilium base is not above the wall");
+  }
+  if (hipSetDevice(cfg->device) != hipSuccess) return fail(CFM_EHIP, "hipSetDevice failed");
+  cfm_cilia_batch* b = new (std::nothrow) cfm_cilia_batch();
+  if (!b) return CFM_ENOMEM;
+  b->cfg = *cfg;
+  b->E = cfg->n_members;
+  b->nc = nc;
+  b->nseg = prm.nseg;
+  b->ns = nc * prm.nseg;
+  b->dyn_angle = q_angle_table ? 1 : 0;
+  b->ne = nc * (b->dyn_angle ? 2 : 1);
+  b->Lm = 3 * b->ns + b->ne;
+  b->L = (size_t)b->E * b->Lm;
+  b->N = (size_t)b->E * b->ns;
+  b->m = prm.solve.restart > 0 ? prm.solve.restart : 50;
+  auto run = [&]() -> Status {
+    CFM_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    const size_t ncl = (size_t)b->E * nc;
+    // bases replicated for every member, SoA over all cilia
+    std::vector<double> hb(3 * ncl);
+    for (int e = 0; e < b->E; ++e)
+      for (int c = 0; c < nc; ++c)
+        for (int d = 0; d < 3; ++d) hb[d * ncl + (size_t)e * nc + c] = base[3 * c + d];
+    CFM_TRY(b->base.alloc(hb.size()));
+    CFM_HIP(copy_to_device(b->stream, b->base.p, hb.data(), sizeof(double) * hb.size()));
+    for (auto* v : {&b->phase, &b->angle, &b->pdot, &b->adot, &b->omega0}) CFM_TRY(v->alloc(ncl));
+    CFM_TRY(b->q.alloc(2 * ncl));
+    CFM_TRY(b->dt.alloc(b->E));
+    for (auto* v : {&b->x, &b->y, &b->z}) CFM_TRY(v->alloc(b->N));
+    CFM_TRY(b->vphase.alloc(3 * b->N));
+    CFM_TRY(b->vangle.alloc(3 * b->N));
+    CFM_TRY(b->svals.alloc(ncl * prm.nseg));
+    CFM_TRY(b->tab_phase.alloc(n_phase));
+    CFM_HIP(copy_to_device(b->stream, b->tab_phase.p, q_phase_table, sizeof(double) * n_phase));
+    b->n_tab_phase = n_phase;
+    if (q_angle_table) {
+      CFM_TRY(b->tab_angle.alloc(n_angle));
+      CFM_HIP(copy_to_device(b->stream, b->tab_angle.p, q_angle_table, sizeof(double) * n_angle));
+      b->n_tab_angle = n_angle;
+    }
+    for (auto* v : {&b->sol, &b->rhs, &b->r, &b->w, &b->zt, &b->t}) CFM_TRY(v->alloc(b->L));
+    const int m = b->m, E = b->E;
+    CFM_TRY(b->Q.alloc((size_t)(m + 1) * b->L));
+    CFM_TRY(b->H.alloc((size_t)E * (m + 1) * (m + 1)));
+    for (auto* v : {&b->cs, &b->sn, &b->beta, &b->h, &b->h2, &b->yv}) CFM_TRY(v->alloc((size_t)E * (m + 2)));
+    CFM_TRY(b->part.alloc((size_t)(m + 1) * E));
+    CFM_TRY(b->sinfo.alloc((size_t)8 * E));
+    CFM_TRY(b->iinfo.alloc((size_t)8 * E));
+    CFM_TRY(b->jdev.alloc(4));
+    CFM_HIP(hipHostMalloc((void**)&b->host_flags, sizeof(int) * 8, hipHostMallocDefault));
+    CFM_HIP(hipHostMalloc((void**)&b->host, sizeof(double) * 2 * ncl, hipHostMallocDefault));
+    // geometry of every cilium of every member (the single system's kernel over the
+    // concatenated cilia)
+    CiliaK& k = b->geo;
+    switch (prm.beat) {
+      case CFM_BEAT_FULFORD_BLAKE_ORIGINAL: k.cf = fulford_blake_original(); break;
+      case CFM_BEAT_VOLVOX: k.cf = volvox(); break;
+      case CFM_BEAT_CORAL_LARVAE: k.cf = coral_larvae(); break;
+      case CFM_BEAT_BUILD_A_BEAT: k.cf = build_a_beat(); break;
+      default: k.cf = fulford_blake();
+    }
+    k.s_tol = prm.s_tol > 0.0 ? prm.s_tol : 0.1 / prm.nseg;
+    k.ncil = (int)ncl;
+    k.nseg = prm.nseg;
+    k.nseg_total = (int)b->N;
+    k.dl = prm.dl;
+    k.len = prm.dl * (prm.nseg - 1);
+    k.base = b->base.p;
+    k.phase = b->phase.p;
+    k.angle = b->angle.p;
+    k.x = b->x.p;
+    k.y = b->y.p;
+    k.z = b->z.p;
+    k.vphase = b->vphase.p;
+    k.vangle = b->vangle.p;
+    k.svals = b->svals.p;
+    CFM_HIP(hipStreamSynchronize(b->stream));
+    return {};
+  };
+  Status s = run();
+  if (!s.ok()) {
+    g_ce_create_err = s.msg;
+    const int code = s.code;
+    cfm_cilia_batch_destroy(b);
+    return code;
+  }
+  *out = b;
+  return CFM_OK;
+}
+
+int cfm_cilia_batch_set_state(cfm_cilia_batch* b, const double* phase, const double* omega0, const double* angle,
+                              const double* dt) {
+  if (!b) return CFM_EINVAL;
+  if (!phase || !omega0 || !dt) return cefail(b, make_status(CFM_EINVAL, "null argument"));
+  const size_t ncl = (size_t)b->E * b->nc;
+  for (size_t i = 0; i < ncl; ++i)
+    if (!std::isfinite(phase[i]) || !std::isfinite(omega0[i]) || (angle && !std::isfinite(angle[i])))
+      return cefail(b, make_status(CFM_EINVAL, "non-finite phase, natural frequency or angle"));
+  for (int e = 0; e < b->E; ++e)
+    if (!std::isfinite(dt[e])) return cefail(b, make_status(CFM_EINVAL, "non-finite time step"));
+  (void)hipSetDevice(b->cfg.device);
+  auto run = [&]() -> Status {
+    CFM_HIP(copy_to_device(b->stream, b->phase.p, phase, sizeof(double) * ncl));
+    CFM_HIP(copy_to_device(b->stream, b->omega0.p, omega0, sizeof(double) * ncl));
+    CFM_HIP(copy_to_device(b->stream, b->pdot.p, omega0, sizeof(double) * ncl));
+    if (angle) {
+      CFM_HIP(copy_to_device(b->stream, b->angle.p, angle, sizeof(double) * ncl));
+    } else {
+      hipLaunchKernelGGL(bfill_cilia_kernel, g256(ncl), dim3(256), 0, b->stream, b->angle.p, 0.0, (int)ncl);
+    }
+    hipLaunchKernelGGL(bfill_cilia_kernel, g256(ncl), dim3(256), 0, b->stream, b->adot.p, 0.0, (int)ncl);
+    CFM_HIP(copy_to_device(b->stream, b->dt.p, dt, sizeof(double) * b->E));
+    const CEK k = cek_of(b);
+    hipLaunchKernelGGL(ce_init_guess_kernel, g256(b->L), dim3(256), 0, b->stream, k, (const double*)b->omega0.p,
+                       b->sol.p);
+    hipLaunchKernelGGL(cilia_geometry_kernel, gc(b->E * b->nc), dim3(64), 0, b->stream, b->geo);
+    CFM_CHECK_LAUNCH();
+    CFM_HIP(hipStreamSynchronize(b->stream));
+    return {};
+  };
+  Status s = run();
+  if (!s.ok()) return cefail(b, s);
+  b->have_state = true;
+  return CFM_OK;
+}
+
+int cfm_cilia_batch_step(cfm_cilia_batch* b, int32_t nt, double* q_phase, double* q_angle,
+                         cfm_batch_sys_stats* per_member) {
+  if (!b) return CFM_EINVAL;
+  if (!b->have_state) return cefail(b, make_status(CFM_ESTATE, "cfm_cilia_batch_set_state has not been called"));
+  (void)hipSetDevice(b->cfg.device);
+  const size_t ncl = (size_t)b->E * b->nc;
+  auto run = [&]() -> Status {
+    const CEK k = cek_of(b);
+    if (nt > 0)  // phase += dt phase_dot before the step (filament.cpp:1340-1356)
+      hipLaunchKernelGGL(ce_advance_kernel, g256(ncl), dim3(256), 0, b->stream, (int)ncl, b->nc, b->dyn_angle,
+                         (const double*)b->dt.p, (const double*)b->pdot.p, (const double*)b->adot.p, b->phase.p,
+                         b->angle.p);
+    hipLaunchKernelGGL(cilia_geometry_kernel, gc((int)ncl), dim3(64), 0, b->stream, b->geo);
+    DriveOpts d{b->cfg.spring_model, b->cfg.noise, b->cfg.seed, nt};
+    hipLaunchKernelGGL(ce_drive_kernel, g256(b->L), dim3(256), 0, b->stream, k, b->cfg.prm.dl * (b->nseg - 1),
+                       b->cfg.spring, (const double*)b->phase.p, (const double*)b->angle.p,
+                       (const double*)b->omega0.p, (const double*)b->tab_phase.p, b->n_tab_phase,
+                       (const double*)b->tab_angle.p, b->n_tab_angle, d, b->rhs.p);
+    CFM_CHECK_LAUNCH();
+    std::vector<double> si;
+    std::vector<int> ii;
+    CFM_TRY(ce_solve(b, &si, &ii));
+    hipLaunchKernelGGL(ce_unpack_kernel, gc((int)ncl), dim3(64), 0, b->stream, k, (const double*)b->sol.p,
+                       b->pdot.p, b->adot.p, b->q.p);
+    CFM_CHECK_LAUNCH();
+    CFM_HIP(hipMemcpyAsync(b->host, b->q.p, sizeof(double) * 2 * ncl, hipMemcpyDeviceToHost, b->stream));
+    CFM_HIP(hipStreamSynchronize(b->stream));
+    if (q_phase) std::memcpy(q_phase, b->host, sizeof(double) * ncl);
+    if (q_angle) std::memcpy(q_angle, b->host + ncl, sizeof(double) * ncl);
+    bool all = true;
+    for (int e = 0; e < b->E; ++e) {
+      const double nb = si[8 * e + CE_NB];
+      const double rel = nb > 0.0 ? si[8 * e + CE_NR] / nb : 0.0;
+      const bool conv = rel <= b->cfg.prm.solve.rtol;
+      all = all && conv;
+      if (per_member) {
+        cfm_batch_sys_stats& P = per_member[e];
+        P.iterations = ii[8 * e + CE_ITERS];
+        P.converged = conv ? 1 : 0;
+        P.rel_residual = rel;
+        P.latency_ms = 0.0;
+        P.reserved0 = ii[8 * e + CE_REJ];  // 1: the guess was rejected
+        P.reserved1 = 0;
+      }
+    }
+    if (!all) return make_status(CFM_ENOCONV, "ensemble GMRES: a member stopped above rtol %.3e",
+                                 b->cfg.prm.solve.rtol);
+    return {};
+  };
+  Status s = run();
+  return s.ok() ? CFM_OK : cefail(b, s);
+}
+
+int cfm_cilia_batch_get_state(cfm_cilia_batch* b, double* phase, double* phase_dot, double* angle,
+                              double* angle_dot) {
+  if (!b) return CFM_EINVAL;
+  (void)hipSetDevice(b->cfg.device);
+  const size_t n = sizeof(double) * (size_t)b->E * b->nc;
+  auto run = [&]() -> Status {
+    CFM_HIP(hipStreamSynchronize(b->stream));
+    if (phase) CFM_HIP(copy_to_host(b->stream, phase, b->phase.p, n));
+    if (phase_dot) CFM_HIP(copy_to_host(b->stream, phase_dot, b->pdot.p, n));
+    if (angle) CFM_HIP(copy_to_host(b->stream, angle, b->angle.p, n));
+    if (angle_dot) CFM_HIP(copy_to_host(b->stream, angle_dot, b->adot.p, n));
+    return {};
+  };
+  Status s = run();
+  return s.ok() ? CFM_OK : cefail(b, s);
+}
+
+int cfm_cilia_batch_get_solution(cfm_cilia_batch* b, double* out) {
+  if (!b || !out) return CFM_EINVAL;
+  (void)hipSetDevice(b->cfg.device);
+  auto run = [&]() -> Status {
+    std::vector<double> t(b->L);
+    CFM_HIP(hipStreamSynchronize(b->stream));
+    CFM_HIP(copy_to_host(b->stream, t.data(), b->sol.p, sizeof(double) * b->L));
+    const size_t ns = b->ns;
+    for (int e = 0; e < b->E; ++e) {
+      const double* src = t.data() + (size_t)e * b->Lm;
+      double* dst = out + (size_t)e * b->Lm;
+      for (size_t i = 0; i < ns; ++i)
+        for (int d = 0; d < 3; ++d) dst[3 * i + d] = src[d * ns + i];
+      for (int j = 0; j < b->ne; ++j) dst[3 * ns + j] = src[3 * ns + j];
+    }
+    return {};
+  };
+  Status s = run();
+  return s.ok() ? CFM_OK : cefail(b, s);
+}
+
+int cfm_cilia_batch_get_positions(cfm_cilia_batch* b, double* x) {
+  if (!b || !x) return CFM_EINVAL;
+  (void)hipSetDevice(b->cfg.device);
+  auto run = [&]() -> Status {
+    std::vector<double> t(3 * b->N);
+    CFM_HIP(hipStreamSynchronize(b->stream));
+    CFM_HIP(copy_to_host(b->stream, t.data(), b->x.p, sizeof(double) * b->N));
+    CFM_HIP(copy_to_host(b->stream, t.data() + b->N, b->y.p, sizeof(double) * b->N));
+    CFM_HIP(copy_to_host(b->stream, t.data() + 2 * b->N, b->z.p, sizeof(double) * b->N));
+    for (size_t i = 0; i < b->N; ++i)
+      for (int d = 0; d < 3; ++d) x[3 * i + d] = t[d * b->N + i];
+    return {};
+  };
+  Status s = run();
+  return s.ok() ? CFM_OK : cefail(b, s);
+}
+
+}  // extern "C"

@@ -22,7 +22,7 @@ from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 
-from .cilia import PrescribedCilia
+from .cilia import CiliaEnsemble, PrescribedCilia
 
 
 def box(x, period):
@@ -96,10 +96,32 @@ class CiliaFlow:
         y = np.concatenate([[T], sp["phase"]] + ([sp["angle"]] if self.angles else []))
         return (y, np.array(rows)) if record else y
 
-    def ensemble(self, xs, ndts, workers=4):
-        """Flow maps of several states at once (one context and stream per member)."""
-        with ThreadPoolExecutor(max_workers=max(1, int(workers))) as ex:
-            return list(ex.map(lambda x: self.run(x, ndts), xs))
+    def ensemble(self, xs, ndts, workers=4, batched=True):
+        """Flow maps of several states at once.  batched: every member in one
+        CiliaEnsemble (one product launch and one lockstep GMRES per step for all of them;
+        FP64 pair sums and solve); otherwise one context and stream per member on
+        `workers` host threads."""
+        xs = [np.asarray(x, float) for x in xs]
+        kw = getattr(self, "kw", None)   # flows that override run() step on their own
+        fp64 = kw is not None and kw["precision"] == "fp64" and kw["solve_mode"] == "fp64"
+        if not batched or not fp64 or not xs:
+            with ThreadPoolExecutor(max_workers=max(1, int(workers))) as ex:
+                return list(ex.map(lambda x: self.run(x, ndts), xs))
+        E, n = len(xs), self.nfil
+        parts = [self._split(x) for x in xs]
+        T = np.array([p[0] for p in parts])
+        ph = np.stack([p[1] for p in parts])
+        an = np.stack([p[2] for p in parts]) if self.angles else None
+        with CiliaEnsemble(self.base, E, self.q_phase, self.q_angle, spring=self.spring,
+                           nseg=kw["nseg"], dl=kw["dl"], precond=kw["precond"], rtol=kw["rtol"],
+                           s_tol=kw["s_tol"]) as C:
+            C.set_state(ph, np.full((E, n), self.omega0), T / self.spp, angle=an)
+            C.step(0)          # solve at the initial states (the first step does not advance)
+            for nt in range(1, ndts + 1):
+                C.step(nt)
+            sp = C.speeds()
+        return [np.concatenate([[T[e]], sp["phase"][e]] + ([sp["angle"][e]] if self.angles else []))
+                for e in range(E)]
 
     # -- the periodic-orbit residual of find_periodic/newton_solver.py:80-126 --------
     def steporbit(self, ndts, x):
@@ -352,8 +374,9 @@ def floquet_arnoldi(flow, x_star, n_evals=7, eps=1e-2, tol=1e-3, seed=0, start=N
 
 def monodromy(flow, x_star, eps=1e-4, workers=8):
     """The whole linearised one-period map at x_star by central differences, its 2 n
-    flow maps run as one concurrent ensemble (the batched counterpart of the Arnoldi
-    iteration for lattices small enough to form it).  Returns (matrix, exponents)."""
+    flow maps run as one ensemble (batched when the flow solves in FP64, else concurrent
+    contexts on `workers` threads; the batched counterpart of the Arnoldi iteration for
+    lattices small enough to form it).  Returns (matrix, exponents)."""
     x_star = np.asarray(x_star, float)
     T = float(x_star[0])
     n = x_star.size - 1

@@ -611,6 +611,58 @@ int cfm_batch_tune(cfm_batch* b, int32_t precision, cfm_batch_tuning* chosen,
 int cfm_batch_gmres(cfm_batch* b, const double* rhs, double* x /* in: guess if use_guess */,
                     const cfm_solve_opts* opts, cfm_batch_sys_stats* per_sys, double* total_ms);
 
+/* ---- ensembles of dynamic-phase cilia lattices ---------------------------------------
+ * The reference's outer loops run one process per flow map (newton_solver.py:49-74,
+ * bisection.py:150-162, pyfile/driver/func.py:544).  An ensemble holds n_members copies
+ * of one lattice of prescribed cilia above the wall (bases `base`, 3 n_cilia AoS; beat,
+ * nseg, dl, s_tol and the segment solve of cfg.prm) driven by the same recorded
+ * generalised-force tables (cfm_cilia_set_driving's meaning; q_angle_table may be null:
+ * no shape rotation), each member with its own phases, natural frequencies, shape angles
+ * and time step.  A step advances every member as cfm_cilia_step does for one context
+ * -- phase += dt phase_dot (from nt = 1), geometry, the coupled solve of forces and
+ * speeds (restarted GMRES, true residual at each cycle's end, the previous solution as
+ * guess unless it is worse than zero), generalised forces -- with one product launch for
+ * all members (each unordered pair of a member's segments once, FP64) and one GMRES in
+ * lockstep whose Arnoldi step is a captured hipGraph.  cfg.prm.solve: precond
+ * CFM_PC_DIAG or CFM_PC_SELF, mode CFM_SOLVE_FP64; cfg.prm.dt and n_cilia's
+ * phase_dot are not used (per-member values come with cfm_cilia_batch_set_state). */
+typedef struct cfm_cilia_batch cfm_cilia_batch;
+typedef struct {
+  int32_t n_members;
+  int32_t device;
+  int32_t spring_model;   /* 0 linear torsional spring, 1 FENE (cfm_cilia_set_driving_options) */
+  int32_t reserved;
+  double mu;
+  double spring;          /* the shape angle's torsional spring constant */
+  double noise;           /* FORCE_NOISE_MAG, deviates keyed on (seed, step, cilium of the member) */
+  uint64_t seed;
+  cfm_cilia_params prm;
+} cfm_cilia_batch_config;
+
+int cfm_cilia_batch_create(const cfm_cilia_batch_config* cfg, const double* base,
+                           const double* q_phase_table, int32_t n_phase,
+                           const double* q_angle_table, int32_t n_angle, cfm_cilia_batch** out);
+int cfm_cilia_batch_destroy(cfm_cilia_batch* b);
+const char* cfm_cilia_batch_last_error(const cfm_cilia_batch* b);
+/* Member states (n_members x n_cilia each, member-major): phases, natural frequencies
+ * (the phase speeds of the first guess and the drive's scale), shape angles (null: 0);
+ * dt: one time step per member.  Resets every member's guess (forces 0, speeds =
+ * natural frequencies) and places the cilia. */
+int cfm_cilia_batch_set_state(cfm_cilia_batch* b, const double* phase, const double* omega0,
+                              const double* angle, const double* dt);
+/* One step nt of every member; q_phase, q_angle (n_members x n_cilia, may be null)
+ * receive the generalised forces, per_member (n_members, may be null) the solve
+ * statistics (reserved0 = 1 when the member's guess was rejected). */
+int cfm_cilia_batch_step(cfm_cilia_batch* b, int32_t nt, double* q_phase, double* q_angle,
+                         cfm_batch_sys_stats* per_member);
+int cfm_cilia_batch_get_state(cfm_cilia_batch* b, double* phase, double* phase_dot, double* angle,
+                              double* angle_dot);
+/* Segment positions of every member (3 per segment, AoS, member-major). */
+int cfm_cilia_batch_get_positions(cfm_cilia_batch* b, double* x);
+/* The last step's solution of every member, member-major: segment forces (3 per segment,
+ * AoS), then n_cilia phase speeds and, with shape rotation, n_cilia angle speeds. */
+int cfm_cilia_batch_get_solution(cfm_cilia_batch* b, double* z);
+
 #ifdef __cplusplus
 }
 #endif

@@ -200,23 +200,38 @@ def build_a_beat_placement(psi, base, nseg, dl, angle, frame):
     return x, vphase, vangle, np.arange(nseg) / (nseg - 1)
 
 
-def curve_length(s, psi, nseg=20, beat="fulford_blake"):
+class _Panels:
+    """Tangent norms at the panel nodes k / (10 nseg) of one phase and their running
+    trapezoid sums, accumulated in the same order as a loop over the panels, so that
+    curve_length gives the same bits with or without the table."""
+
+    def __init__(self, psi, nseg, beat):
+        self.psi, self.beat = psi, beat
+        self.dl = 1.0 / (10 * nseg)
+        self.f = [math.hypot(*tangent(0.0, psi, beat))]
+        self.P = [0.0]
+
+    def upto(self, k):
+        while len(self.f) <= k:
+            n = len(self.f)
+            self.f.append(math.hypot(*tangent(n * self.dl, psi=self.psi, beat=self.beat)))
+            self.P.append(self.P[-1] + 0.5 * self.dl * (self.f[n - 1] + self.f[n]))
+
+
+def curve_length(s, psi, nseg=20, beat="fulford_blake", panels=None):
     """Trapezoid arc length from 0 to s on 10 nseg panels, the last panel
-    interpolated (filament.cpp:1047-1094)."""
+    interpolated (filament.cpp:1047-1094).  panels: a _Panels of the same phase, reused
+    between calls."""
     if s <= 0.0:
         return 0.0
-    dl = 1.0 / (10 * nseg)
-    f0 = math.hypot(*tangent(0.0, psi, beat))
+    t = _Panels(psi, nseg, beat) if panels is None else panels
+    dl = t.dl
     fl = math.floor(s / dl)
-    length = 0.0
-    for n in range(1, int(fl) + 1):
-        f1 = math.hypot(*tangent(n * dl, psi, beat))
-        length += 0.5 * dl * (f0 + f1)
-        f0 = f1
     ce = math.ceil(s / dl)
+    t.upto(max(fl, ce))
+    length = t.P[fl]
     if ce > fl:
-        f1 = math.hypot(*tangent(ce * dl, psi, beat))
-        length += ((s / dl - fl) / (ce - fl)) * 0.5 * dl * (f0 + f1)
+        length += ((s / dl - fl) / (ce - fl)) * 0.5 * dl * (t.f[fl] + t.f[ce])
     return length
 
 
@@ -232,12 +247,13 @@ def s_values(psi, nseg=20, beat="fulford_blake", tol=None, margins=False):
     s = [0.0] * nseg
     s[-1] = 1.0
     m = np.full(nseg, np.inf)
-    total = curve_length(1.0, psi, nseg, beat)
+    panels = _Panels(psi, nseg, beat)
+    total = curve_length(1.0, psi, nseg, beat, panels)
     for n in range(1, nseg - 1):
         target = n / (nseg - 1)
         lo, hi = s[n - 1], 1.0
         est = 0.5 * (lo + hi)
-        frac = curve_length(est, psi, nseg, beat) / total
+        frac = curve_length(est, psi, nseg, beat, panels) / total
         m[n] = min(abs(abs(frac - target) - tol), abs(frac - target))
         while abs(frac - target) > tol:
             if frac > target:
@@ -245,7 +261,7 @@ def s_values(psi, nseg=20, beat="fulford_blake", tol=None, margins=False):
             else:
                 lo = est
             est = 0.5 * (lo + hi)
-            frac = curve_length(est, psi, nseg, beat) / total
+            frac = curve_length(est, psi, nseg, beat, panels) / total
             m[n] = min(m[n], abs(abs(frac - target) - tol), abs(frac - target))
         s[n] = est
     return (np.array(s), m) if margins else np.array(s)

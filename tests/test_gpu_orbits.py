@@ -27,7 +27,7 @@ def test_concurrent_flow_maps_equal_sequential_runs():
     rng = np.random.default_rng(1)
     xs = [np.concatenate([[1.0], rng.uniform(0, 2 * math.pi, 4)]) for _ in range(4)]
     seq = [flow.run(x, 20) for x in xs]
-    par = flow.ensemble(xs, 20, workers=4)
+    par = flow.ensemble(xs, 20, workers=4, batched=False)
     for a, b in zip(seq, par):
         assert np.array_equal(a, b)
     # an isolated cilium driven by its own recorded forces advances by 2 pi per period
