@@ -102,7 +102,10 @@ int cfm_time_apply(cfm_ctx* ctx, int32_t which, int32_t reps, double* ms_per_app
  * static device array of CFM_NOALLOC_SCRATCH_BYTES per GPU (held until the process
  * ends) instead of context memory; the split is the context's fixed tile_split when
  * > 0 -- rejected with CFM_EINVAL when split x 6 x (n_seg + n_blob) doubles exceed the
- * array -- and otherwise the size heuristic, lowered to fit.  The call allocates
+ * array -- and otherwise the size heuristic, lowered to fit.  Single-species wall
+ * products run the one-directional kernels (those of cfm_set_pair_symmetry(ctx, 0)),
+ * whose partial sums fit the array; they agree with cfm_apply's symmetric default to
+ * rounding.  The call allocates
  * nothing and returns after the context's stream has finished.  Every call on a GPU
  * shares that GPU's array: calls that overlap on one GPU (other contexts, other host
  * threads) race and must be serialised by the caller. */

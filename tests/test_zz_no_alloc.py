@@ -2,9 +2,12 @@
 existing cfm_apply: every block of a segment + blob context, from device arrays, on two
 calls in a row on one GPU -- the second from a smaller context, so the shared static
 scratch array still holds the first call's larger partial sums.  Tolerances as for
-every product (FP64 <= 1e-12 relative; FP32C <= 1e-6 absolute); with a fixed split
-both entry points run the same launches, so the results are bitwise equal.  Device
-memory for the test comes from the HIP runtime the library itself links."""
+every product (FP64 <= 1e-12 relative; FP32C <= 1e-6 absolute) against cfm_apply's
+default, which forms single-species wall products with the symmetric kernels; the
+allocation-free entry point runs the one-directional kernels (their partial sums fit
+its scratch array), so with a fixed split it is bitwise equal to cfm_apply with pair
+symmetry off, which runs the same launches.  Device memory for the test comes from the
+HIP runtime the library itself links."""
 import ctypes
 
 import numpy as np
@@ -74,10 +77,16 @@ def _compare(xs, xb, fs, fb, prec, split):
         v0s = np.random.default_rng(9).standard_normal(6 * ns)  # accumulated into by SB
         try:
             for which in (CFM_SS, CFM_SB, CFM_BB, CFM_BS, CFM_FULL):
-                ref_s = v0s.copy() if which == CFM_SB else np.zeros(6 * ns)
-                ref_b = np.zeros(3 * nb)
-                L.check(m.handle, lib.cfm_apply(m.handle, which, L.ptr(fs), L.ptr(fb),
-                                                L.ptr(ref_s), L.ptr(ref_b)))
+                refs = []
+                for sym in (1, 0):   # cfm_apply's default, then the one-directional kernels
+                    L.check(m.handle, lib.cfm_set_pair_symmetry(m.handle, sym))
+                    r_s = v0s.copy() if which == CFM_SB else np.zeros(6 * ns)
+                    r_b = np.zeros(3 * nb)
+                    L.check(m.handle, lib.cfm_apply(m.handle, which, L.ptr(fs), L.ptr(fb),
+                                                    L.ptr(r_s), L.ptr(r_b)))
+                    refs.append((r_s, r_b))
+                L.check(m.handle, lib.cfm_set_pair_symmetry(m.handle, 1))
+                (ref_s, ref_b), (dir_s, dir_b) = refs
                 dvs = Dev(v0s.copy() if which == CFM_SB else np.zeros(6 * ns))
                 dvb = Dev(np.zeros(3 * nb))
                 try:
@@ -88,12 +97,12 @@ def _compare(xs, xb, fs, fb, prec, split):
                     dvb.free()
                 if which in (CFM_SS, CFM_SB, CFM_FULL):
                     if split > 0:
-                        assert np.array_equal(got_s, ref_s), which
+                        assert np.array_equal(got_s, dir_s), which
                     e, t = _err(got_s, ref_s, prec)
                     assert e < t, (which, e)
                 if which in (CFM_BB, CFM_BS, CFM_FULL):
                     if split > 0:
-                        assert np.array_equal(got_b, ref_b), which
+                        assert np.array_equal(got_b, dir_b), which
                     e, t = _err(got_b, ref_b, prec)
                     assert e < t, (which, e)
         finally:
