@@ -97,7 +97,7 @@ class cfm_fil_params(ctypes.Structure):
         ("guess", ctypes.c_int32),
         ("blob_solve", cfm_solve_opts),
         ("overlap", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("broyden_only", ctypes.c_int32),
     ]
 
 
@@ -270,6 +270,7 @@ def lib():
     L.cfm_fil_initial_guess.argtypes = [_vp, ctypes.c_int32]
     L.cfm_fil_attach_bodies.argtypes = [_vp, _dp]
     L.cfm_fil_get_body_state.argtypes = [_vp, _dp, _dp, _dp, _dp]
+    L.cfm_fil_get_blob_forces.argtypes = [_vp, _dp]
     L.cfm_fil_set_body_sphere.argtypes = [_vp, ctypes.c_double]
     _fp = ctypes.POINTER(ctypes.c_float)
     _ip = ctypes.POINTER(ctypes.c_int32)

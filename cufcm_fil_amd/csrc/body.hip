@@ -741,6 +741,11 @@ const double* body_offsets(cfm_ctx* c) {
   return S ? S->r.p : nullptr;
 }
 
+const double* body_ref_device(cfm_ctx* c) {
+  BodyState* S = body_of(c);
+  return S ? S->ref_dev.p : nullptr;
+}
+
 Status body_place_device(cfm_ctx* c, const double* bx, const double* bq) {
   BodyState* S = body_of(c);
   if (!S) return make_status(CFM_ESTATE, "cfm_body_init has not been called");

@@ -23,6 +23,9 @@ Status body_place_device(cfm_ctx* c, const double* bx, const double* bq);
 void body_freeze_dense(cfm_ctx* c, bool on);
 // Rotated blob offsets R(q_b) ref of the last placement ([3][n_blob], device), or null.
 const double* body_offsets(cfm_ctx* c);
+// Body-frame offsets of one body's blobs on the device (3 nb, AoS; set by the first
+// placement from device-held body states), or null.
+const double* body_ref_device(cfm_ctx* c);
 // Prescribed rotation (cfm_body_set_motion): Omega_b is held at zero.
 bool body_rotation_prescribed(cfm_ctx* c);
 // Prescribed rotation: zero the torque rows (the prescribed Omega_b) of a body-system

@@ -128,6 +128,24 @@ __device__ __forceinline__ V3 dexpinv_transpose(V3 u, V3 v) {
   const V3 uv = cross(u, v);
   return v + 0.5 * uv - f * cross(u, uv);
 }
+// dexp(u) v = v + alpha u x v + beta u x (u x v); below the small angle the
+// coefficients invert dexpinv's series exactly (quaternion.cpp:552-586)
+__device__ __forceinline__ V3 dexp(V3 u, V3 v) {
+  const double th2 = dot(u, u);
+  const double th = sqrt(th2);
+  double alpha, beta;
+  if (th < 1e-5) {
+    const double th4 = th2 * th2;
+    beta = (1.0 / 3.0 + th2 / 72.0) / (2.0 + th2 / 6.0 + th4 / 72.0);
+    alpha = 1.0 / 6.0 + 2.0 * beta * (1.0 - th2 / 12.0);
+  } else {
+    alpha = sin(0.5 * th);
+    alpha *= 2.0 * alpha / th2;
+    beta = (th - sin(th)) / (th * th2);
+  }
+  const V3 uv = cross(u, v);
+  return v + alpha * uv + beta * cross(u, uv);
+}
 
 }  // namespace fil
 }  // namespace cfm

@@ -136,6 +136,11 @@ struct FilState {
   bool swim = false;
   SwimK S;
   DevBuf<double> sw_state, sw_fref, sw_fext, sw_mob, sw_BJ, sw_Sf, sw_clamp, sw_rep, sw_tmp;
+  // blob forces among the Broyden unknowns (broyden_only): [3][n_blob] and two previous steps
+  bool bx = false;
+  BxK X;
+  int nt = 0;  // the step in progress
+  DevBuf<double> bx_fb, bx_fb_m1, bx_fb_m2;
 };
 
 static Status ensure_host(FilState* F) {
@@ -163,7 +168,30 @@ static Status apply_plan(cfm_ctx* c, const ApplyPlan& pl, int prec) {
 // v = M f for the current loads (segments from the filament loads, blob forces by GMRES)
 static Status swim_velocities(cfm_ctx* c, FilState* F, cfm_step_stats* st);
 
+// Blob forces among the Broyden unknowns: every product from the current loads
+// (mobility_solver.cpp:2697-2730); with free bodies also the body loads of the body rows.
+static Status bx_velocities(cfm_ctx* c, FilState* F) {
+  Particles& p = c->p;
+  const double* rep = nullptr;
+  if (F->prm.barrier) {
+    CFM_TRY(fil_barrier(c, F->K, F->grid));
+    if (F->swim) {
+      CFM_TRY(fil_blob_barrier(c, F->K, F->grid, F->S.nb, F->sw_rep.p));
+      rep = F->sw_rep.p;
+    }
+  }
+  if (F->swim) CFM_TRY(swim_force(c, F->S, F->K, rep, body_offsets(c)));
+  CFM_TRY(dev_copy_rows(c, F->X.fb, (size_t)p.n_blob, 0, p.f.p, (size_t)p.n, p.n_seg, 3, p.n_blob));
+  ApplyPlan all;
+  all.tgt_begin = 0;
+  all.tgt_end = p.n;
+  all.src_seg = true;
+  all.src_blob = true;
+  return apply_plan(c, all, F->prm.pair_precision);
+}
+
 static Status velocities(cfm_ctx* c, FilState* F, cfm_step_stats* st) {
+  if (F->bx) return bx_velocities(c, F);
   if (F->swim) return swim_velocities(c, F, st);
   Particles& p = c->p;
   const int prec = F->prm.pair_precision;
@@ -323,7 +351,12 @@ static Status swim_velocities(cfm_ctx* c, FilState* F, cfm_step_stats* st) {
 
 static Status residual(cfm_ctx* c, FilState* F, int nt, double* e, double* bad) {
   int extra = 0;
-  if (F->swim) {
+  if (F->bx) {
+    const int nb = (c->p.n_seg + 255) / 256;
+    const size_t L0 = 6 * (size_t)c->p.n_seg;
+    CFM_TRY(bx_errors(c, F->X, F->S, nt, e + L0, e + L0 + 6 * (size_t)F->X.nbody, F->bad_part.p + nb));
+    extra = F->X.nbody + (F->X.m + 255) / 256;
+  } else if (F->swim) {
     const int nb = (c->p.n_seg + 255) / 256;
     CFM_TRY(swim_errors(c, F->S, nt, e + 6 * (size_t)c->p.n_seg, F->bad_part.p + nb));
     extra = F->S.nbody;
@@ -334,6 +367,7 @@ static Status residual(cfm_ctx* c, FilState* F, int nt, double* e, double* bad) 
 
 // out = scale J0^-1 in (block diagonal over filaments, plus the body Schur complement)
 static Status jinv_mult(cfm_ctx* c, FilState* F, const double* in, double* out, double scale) {
+  if (F->bx) return bx_jinv_mult(c, F->X, F->S, F->K, F->nt, in, out, scale, F->sw_tmp.p);
   if (F->swim) return swim_jinv_mult(c, F->S, F->K, in, out, scale, F->sw_tmp.p);
   return fil_jinv_mult(c, F->K, in, out, scale);
 }
@@ -341,12 +375,14 @@ static Status jinv_mult(cfm_ctx* c, FilState* F, const double* in, double* out, 
 static Status jacobian(cfm_ctx* c, FilState* F, int nt) {
   CFM_HIP(hipMemsetAsync(F->status.p, 0, sizeof(int), c->stream));
   CFM_TRY(fil_jacobian(c, F->K, nt, F->status.p));
+  if (F->bx) return bx_jacobian(c, F->X, F->S, F->K, nt, F->status.p);
   if (F->swim) CFM_TRY(swim_jacobian(c, F->S, F->K, nt, F->status.p));
   return {};
 }
 
 static Status update(cfm_ctx* c, FilState* F, const double* upd) {
   if (F->swim) CFM_TRY(swim_update(c, F->S, upd + 6 * (size_t)c->p.n_seg));
+  if (F->bx) CFM_TRY(bx_update(c, F->X, upd + 6 * ((size_t)c->p.n_seg + F->X.nbody)));
   CFM_TRY(fil_update(c, F->K, upd));
   if (F->swim) CFM_TRY(body_place_device(c, F->S.x, F->S.q));
   return {};
@@ -354,6 +390,7 @@ static Status update(cfm_ctx* c, FilState* F, const double* upd) {
 
 static Status initial_guess(cfm_ctx* c, FilState* F, int nt) {
   if (F->swim) CFM_TRY(swim_guess(c, F->S, nt));
+  if (F->bx) CFM_TRY(bx_guess(c, F->X, nt));
   CFM_TRY(fil_initial_guess(c, F->K, nt));
   if (F->swim) CFM_TRY(body_place_device(c, F->S.x, F->S.q));
   return {};
@@ -372,6 +409,9 @@ static Status low_rank_apply(cfm_ctx* c, FilState* F, const double* e, double* o
 
 static Status step(cfm_ctx* c, FilState* F, int nt, cfm_step_stats* st) {
   const auto t0 = std::chrono::steady_clock::now();
+  if (F->bx && F->swim && body_rotation_prescribed(c))
+    return make_status(CFM_EINVAL, "broyden_only takes free bodies, not a prescribed rotation");
+  F->nt = nt;
   cfm_step_stats S{};
   const int L = F->nbroy;
   const int maxit = F->prm.max_broyden;
@@ -440,7 +480,8 @@ void cfm_fil_release(cfm_ctx* c) {
                             &F->base, &F->pert, &F->fint, &F->jinv, &F->err, &F->nerr, &F->upd,
                             &F->C, &F->D, &F->coef, &F->partial, &F->bad_part, &F->scal,
                             &F->fb_prev, &F->rhs_b, &F->sw_state, &F->sw_fref, &F->sw_fext,
-                            &F->sw_mob, &F->sw_BJ, &F->sw_Sf, &F->sw_clamp, &F->sw_rep, &F->sw_tmp};
+                            &F->sw_mob, &F->sw_BJ, &F->sw_Sf, &F->sw_clamp, &F->sw_rep, &F->sw_tmp,
+                            &F->bx_fb, &F->bx_fb_m1, &F->bx_fb_m2};
   for (auto* b : bufs) b->release();
   F->status.release();
   F->grid.release();
@@ -491,7 +532,8 @@ int cfm_fil_init(cfm_ctx* c, const cfm_fil_params* prm, const double* q_seg, con
   K.rep_factor = prm->repulsive_factor;
   K.tol = prm->tol;
   const size_t ns = p.n_seg, nf = K.nfil;
-  F->nbroy = 6 * p.n_seg;
+  F->bx = prm->broyden_only != 0 && p.n_blob > 0;
+  F->nbroy = 6 * p.n_seg + (F->bx ? 3 * p.n_blob : 0);
   auto run = [&]() -> Status {
     CFM_TRY(F->q.alloc(4 * ns)); CFM_TRY(F->qm1.alloc(4 * ns));
     CFM_TRY(F->u.alloc(3 * ns)); CFM_TRY(F->um1.alloc(3 * ns));
@@ -507,7 +549,14 @@ int cfm_fil_init(cfm_ctx* c, const cfm_fil_params* prm, const double* q_seg, con
     CFM_TRY(F->D.alloc((size_t)L * prm->max_broyden));
     CFM_TRY(F->coef.alloc(prm->max_broyden + 1));
     CFM_TRY(F->partial.alloc(dev_dots_partial_size(L, prm->max_broyden + 1)));
-    CFM_TRY(F->bad_part.alloc((p.n_seg + 255) / 256 + 1));
+    CFM_TRY(F->bad_part.alloc((p.n_seg + 255) / 256 + 1 + (F->bx ? (p.n_blob + 255) / 256 : 0)));
+    if (F->bx) {
+      for (auto* b : {&F->bx_fb, &F->bx_fb_m1, &F->bx_fb_m2}) {
+        CFM_TRY(b->alloc(3 * (size_t)p.n_blob));
+        CFM_TRY(dev_fill(c, b->p, 0.0, 3 * p.n_blob));
+      }
+      CFM_TRY(F->sw_tmp.alloc(6 * ns));
+    }
     CFM_TRY(F->scal.alloc(8));
     CFM_TRY(F->grid.ensure(p.n_seg + p.n_blob));
     CFM_TRY(F->status.alloc(4));
@@ -546,6 +595,18 @@ int cfm_fil_init(cfm_ctx* c, const cfm_fil_params* prm, const double* q_seg, con
   K.lam = F->lam.p; K.lam_m1 = F->lam_m1.p; K.lam_m2 = F->lam_m2.p;
   K.clamp = F->clamp.p; K.clamp_m1 = F->clamp_m1.p; K.clamp_m2 = F->clamp_m2.p;
   K.base = F->base.p; K.pert = F->pert.p; K.fint = F->fint.p; K.jinv = F->jinv.p;
+  if (F->bx) {
+    BxK& X = F->X;
+    X.m = p.n_blob;
+    X.num_euler = prm->num_euler_steps;
+    X.dt = prm->dt;
+    X.tol = prm->tol;
+    X.mu = c->cfg.mu;
+    X.a = c->cfg.a_blob;
+    X.fb = F->bx_fb.p;
+    X.fb_m1 = F->bx_fb_m1.p;
+    X.fb_m2 = F->bx_fb_m2.p;
+  }
   return CFM_OK;
 }
 
@@ -658,6 +719,8 @@ int cfm_fil_jacobian_inverse(cfm_ctx* c, int32_t nt, double* jinv) {
 int cfm_fil_errors(cfm_ctx* c, int32_t nt, const double* v_seg, double* err, double* badness) {
   FIL_OR_FAIL(c);
   if (!v_seg || !err) return fail_c(c, make_status(CFM_EINVAL, "null argument"));
+  if (F->bx) return fail_c(c, make_status(CFM_EINVAL, "with broyden_only the residual also needs the blob "
+                                                      "velocities; it is formed inside cfm_fil_step"));
   auto run = [&]() -> Status {
     CFM_TRY(upload_aos(c, v_seg, c->p.n_seg, 6, c->p.v.p, c->p.n, 0));
     double bad = 0.0;
@@ -777,14 +840,15 @@ int cfm_fil_attach_bodies(cfm_ctx* c, const double* f_ext) {
     S.mob = F->sw_mob.p;
     S.BJ = F->sw_BJ.p;
     S.Sf = F->sw_Sf.p;
-    // Broyden vectors grow by 6 per body; the blob solve by the body velocities
-    const int L = 6 * ns + 6 * B;
+    // Broyden vectors grow by 6 per body (and keep the blob forces with broyden_only);
+    // the blob solve by the body velocities
+    const int L = 6 * ns + 6 * B + (F->bx ? 3 * m : 0);
     F->nbroy = L;
     for (auto* buf : {&F->err, &F->nerr, &F->upd}) CFM_TRY(buf->alloc(L));
     CFM_TRY(F->C.alloc((size_t)L * F->prm.max_broyden));
     CFM_TRY(F->D.alloc((size_t)L * F->prm.max_broyden));
     CFM_TRY(F->partial.alloc(dev_dots_partial_size(L, F->prm.max_broyden + 1)));
-    CFM_TRY(F->bad_part.alloc((ns + 255) / 256 + 1 + B));
+    CFM_TRY(F->bad_part.alloc((ns + 255) / 256 + 1 + B + (F->bx ? (m + 255) / 256 : 0)));
     CFM_TRY(F->fb_prev.alloc(3 * (size_t)m + 6 * B));
     CFM_TRY(dev_fill(c, F->fb_prev.p, 0.0, 3 * m + 6 * B));
     F->K.bx = S.x;
@@ -797,6 +861,12 @@ int cfm_fil_attach_bodies(cfm_ctx* c, const double* f_ext) {
     body_freeze_dense(c, pc == CFM_PC_DENSE);
     F->swim = true;
     CFM_TRY(body_place_device(c, S.x, S.q));
+    if (F->bx) {
+      F->X.nbody = B;
+      F->X.nb = bi.nb;
+      F->X.ref = body_ref_device(c);
+      if (!F->X.ref) return make_status(CFM_ESTATE, "no body-frame blob offsets on the device");
+    }
     CFM_HIP(hipStreamSynchronize(c->stream));
     return {};
   };
@@ -831,6 +901,15 @@ int cfm_fil_get_body_state(cfm_ctx* c, double* x, double* q, double* u, double* 
     return {};
   };
   Status st = run();
+  return st.ok() ? CFM_OK : fail_c(c, st);
+}
+
+int cfm_fil_get_blob_forces(cfm_ctx* c, double* f_blob) {
+  FIL_OR_FAIL(c);
+  if (!f_blob) return fail_c(c, make_status(CFM_EINVAL, "null output"));
+  const int m = c->p.n_blob;
+  if (m == 0) return CFM_OK;
+  Status st = download_aos(c, f_blob, m, 3, F->bx ? F->bx_fb.p : F->fb_prev.p, m, 0);
   return st.ok() ? CFM_OK : fail_c(c, st);
 }
 

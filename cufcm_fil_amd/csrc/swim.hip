@@ -14,6 +14,7 @@
 #include "fil.hpp"
 #include "fil_device.hpp"
 #include "swim.hpp"
+#include "body.hpp"
 
 namespace cfm {
 using namespace fil;
@@ -190,7 +191,10 @@ __device__ __forceinline__ M3 rot_mob(const SwimK& S, int b, int bi, int bj) {
 // B J^-1 of filament f (swimmer.cpp:744-886, with the sphere correction when the body
 // is a sphere) and its product with the filament's body columns A (:888-902).  One
 // workgroup per filament.
-__global__ __launch_bounds__(256) void swim_bj_kernel(SwimK S, FilK K, int nt) {
+// type0 (blob forces among the Broyden unknowns, swimmer.cpp:624-668): the body rows are
+// the force and torque balance, whose block against the filament unknowns is D = -B with
+// B the raw block below (no body mobility, time-step factor or sphere correction).
+__global__ __launch_bounds__(256) void swim_bj_kernel(SwimK S, FilK K, int nt, int type0) {
   extern __shared__ double lds[];
   const int f = blockIdx.x;
   const int n = K.nseg, N = 6 * n;
@@ -211,7 +215,7 @@ __global__ __launch_bounds__(256) void swim_bj_kernel(SwimK S, FilK K, int nt) {
     C[e] = 0.0;
   }
   __syncthreads();
-  if (tid == 1 && S.sphere_R > 0.0) {
+  if (tid == 1 && S.sphere_R > 0.0 && !type0) {
     // RPY of the sphere (radius R) and segment m (radius a): translation-translation,
     // translation-rotation and rotation-rotation blocks (swimmer.cpp:809-877)
     auto add = [&](int r0, int c0, const M3& M) {
@@ -268,8 +272,9 @@ __global__ __launch_bounds__(256) void swim_bj_kernel(SwimK S, FilK K, int nt) {
     for (int m = 1; m < n - 1; ++m) put(3, 3 * (n + m), mm(E, rcross(S.dl * tangent(segq(K, s0 + m)))));
   }
   __syncthreads();
-  // B <- dt_fac diag(I, Dinv) N B, column by column
-  if (tid < N) {
+  if (type0) {
+    for (int e = tid; e < 6 * N; e += 256) B[e] = -B[e];
+  } else if (tid < N) {  // B <- dt_fac diag(I, Dinv) N B, column by column
     const int c = tid;
     double col[6], out[6];
     for (int r = 0; r < 6; ++r) col[r] = B[r * N + c];
@@ -315,6 +320,42 @@ __global__ __launch_bounds__(256) void swim_bj_kernel(SwimK S, FilK K, int nt) {
   }
 }
 
+// In-place Gauss-Jordan with partial pivoting of a row-major 6 x 6: I6 <- A^-1; false
+// when a pivot vanishes or is not finite.
+__device__ bool invert6(double* A, double* I6) {
+  for (int i = 0; i < 36; ++i) I6[i] = (i % 7 == 0) ? 1.0 : 0.0;
+  for (int c = 0; c < 6; ++c) {
+    int p = c;
+    for (int r = c + 1; r < 6; ++r)
+      if (fabs(A[r * 6 + c]) > fabs(A[p * 6 + c])) p = r;
+    const double piv = A[p * 6 + c];
+    if (!(fabs(piv) > 0.0) || piv != piv) return false;
+    if (p != c)
+      for (int j = 0; j < 6; ++j) {
+        double t = A[p * 6 + j];
+        A[p * 6 + j] = A[c * 6 + j];
+        A[c * 6 + j] = t;
+        t = I6[p * 6 + j];
+        I6[p * 6 + j] = I6[c * 6 + j];
+        I6[c * 6 + j] = t;
+      }
+    const double ip = 1.0 / piv;
+    for (int j = 0; j < 6; ++j) {
+      A[c * 6 + j] *= ip;
+      I6[c * 6 + j] *= ip;
+    }
+    for (int r = 0; r < 6; ++r) {
+      if (r == c) continue;
+      const double g = A[r * 6 + c];
+      for (int j = 0; j < 6; ++j) {
+        A[r * 6 + j] -= g * A[c * 6 + j];
+        I6[r * 6 + j] -= g * I6[c * 6 + j];
+      }
+    }
+  }
+  return true;
+}
+
 // Body block of J0 and its Schur complement (swimmer.cpp:690-742, 902-909), inverted.
 __global__ void swim_schur_kernel(SwimK S, FilK K, int nt, int* __restrict__ status) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
@@ -347,43 +388,8 @@ __global__ void swim_schur_kernel(SwimK S, FilK K, int nt, int* __restrict__ sta
   for (int i = 0; i < 36; ++i) S.S0[36 * b + i] = A[i];
   for (int f = b * S.fpb; f < (b + 1) * S.fpb; ++f)
     for (int i = 0; i < 36; ++i) A[i] -= S.Sf[36 * (size_t)f + i];
-  // Gauss-Jordan with partial pivoting
   double I6[36];
-  for (int i = 0; i < 36; ++i) I6[i] = (i % 7 == 0) ? 1.0 : 0.0;
-  bool ok = true;
-  for (int c = 0; c < 6 && ok; ++c) {
-    int p = c;
-    for (int r = c + 1; r < 6; ++r)
-      if (fabs(A[r * 6 + c]) > fabs(A[p * 6 + c])) p = r;
-    const double piv = A[p * 6 + c];
-    if (!(fabs(piv) > 0.0) || piv != piv) {
-      ok = false;
-      break;
-    }
-    if (p != c)
-      for (int j = 0; j < 6; ++j) {
-        double t = A[p * 6 + j];
-        A[p * 6 + j] = A[c * 6 + j];
-        A[c * 6 + j] = t;
-        t = I6[p * 6 + j];
-        I6[p * 6 + j] = I6[c * 6 + j];
-        I6[c * 6 + j] = t;
-      }
-    const double ip = 1.0 / piv;
-    for (int j = 0; j < 6; ++j) {
-      A[c * 6 + j] *= ip;
-      I6[c * 6 + j] *= ip;
-    }
-    for (int r = 0; r < 6; ++r) {
-      if (r == c) continue;
-      const double g = A[r * 6 + c];
-      for (int j = 0; j < 6; ++j) {
-        A[r * 6 + j] -= g * A[c * 6 + j];
-        I6[r * 6 + j] -= g * I6[c * 6 + j];
-      }
-    }
-  }
-  if (!ok) status[0] = 1;
+  if (!invert6(A, I6)) status[0] = 1;
   for (int i = 0; i < 36; ++i) S.Sinv[36 * b + i] = I6[i];
 }
 
@@ -444,6 +450,259 @@ __global__ void swim_rhs_kernel(SwimK S, FilK K, const double* __restrict__ in, 
   tmp[il + 2] = in[il + 2] - pl.z;
 }
 
+
+// ---- blob forces among the Broyden unknowns (the reference's SOLVER_TYPE 0) ----------
+
+// Offset of blob j from its body's centre with the orientation the step started from,
+// Q_init ref (swimmer.cpp:582, 951).
+__device__ __forceinline__ V3 bx_disp0(const BxK& X, const SwimK& S, int j) {
+  const int b = j / X.nb, k = j - b * X.nb;
+  return mv(ldm(S.Qinit + 9 * b), ldv(X.ref + 3 * k));
+}
+
+// rigid_body::initial_guess, blob-force branch (rigid_body.cpp:459-490).
+__global__ void bx_guess_kernel(BxK X, int nt) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= 3 * X.m || nt < 1) return;
+  const double f = X.fb[e], f1 = X.fb_m1[e], f2 = X.fb_m2[e];
+  if (nt == 1) {
+    X.fb_m1[e] = f;
+  } else if (nt == 2) {
+    X.fb_m2[e] = f1;
+    X.fb_m1[e] = f;
+    X.fb[e] = 2.0 * f - f1;
+  } else {
+    X.fb_m2[e] = f1;
+    X.fb_m1[e] = f;
+    X.fb[e] = 3.0 * (f - f1) + f2;
+  }
+}
+
+// blob forces += their block of a Broyden increment (3 per blob) (rigid_body.cpp:521-527)
+__global__ void bx_update_kernel(BxK X, const double* __restrict__ upd) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= 3 * X.m) return;
+  const int k = e / X.m, j = e - k * X.m;
+  X.fb[e] += upd[3 * (size_t)j + k];
+}
+
+// Six per-thread sums reduced over a 256-thread workgroup in a fixed order.
+__device__ __forceinline__ void bx_reduce6(double* s, double (*red)[6]) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    double v = s[k];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if (lane == 0) red[wave][k] = v;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 6; ++k) s[k] = (red[0][k] + red[1][k]) + (red[2][k] + red[3][k]);
+}
+
+// Body rows: -(body load) + sum of the blob forces and of their torques about the centre
+// with the current orientation (mobility_solver.cpp:2770-2781, 2930-2941); badness
+// max |e| / TOL.  One workgroup per body.
+__global__ __launch_bounds__(256) void bx_body_err_kernel(BxK X, SwimK S, const double* __restrict__ boff,
+                                                          double* __restrict__ err, double* __restrict__ bad) {
+  __shared__ double red[4][6];
+  const int b = blockIdx.x;
+  double s[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int j = b * X.nb + threadIdx.x; j < (b + 1) * X.nb; j += 256) {
+    const V3 F = {X.fb[j], X.fb[X.m + j], X.fb[2 * (size_t)X.m + j]};
+    const V3 d = {boff[j], boff[X.m + j], boff[2 * (size_t)X.m + j]};
+    const V3 t = cross(d, F);
+    s[0] += F.x;
+    s[1] += F.y;
+    s[2] += F.z;
+    s[3] += t.x;
+    s[4] += t.y;
+    s[5] += t.z;
+  }
+  bx_reduce6(s, red);
+  if (threadIdx.x == 0) {
+    double v = 0.0;
+    bool nan = false;
+    for (int k = 0; k < 6; ++k) {
+      const double e = s[k] - S.force[6 * b + k];
+      err[6 * b + k] = e;
+      v = fmax(v, fabs(e) / X.tol);
+      nan = nan || e != e;
+    }
+    bad[b] = nan ? __builtin_nan("") : v;
+  }
+}
+
+// Blob rows: V + Omega x d - v_blob, V and Omega from the body's BDF2 (Euler) relation
+// (mobility_solver.cpp:2873-2922); -v_blob for a body held fixed.  Badness per
+// workgroup, max |e| / TOL.
+__global__ __launch_bounds__(256) void bx_blob_err_kernel(BxK X, SwimK S, int nt, const double* __restrict__ v,
+                                                          size_t ld, int off, const double* __restrict__ boff,
+                                                          double* __restrict__ err, double* __restrict__ bad) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  double bj = 0.0;
+  if (j < X.m) {
+    V3 e = {-v[off + j], -v[ld + off + j], -v[2 * ld + off + j]};
+    if (X.nbody > 0) {
+      const int b = j / X.nb;
+      const V3 x = ldv(S.x + 3 * b), xm1 = ldv(S.xm1 + 3 * b), xm2 = ldv(S.xm2 + 3 * b);
+      const V3 u = ldv(S.u + 3 * b), um1 = ldv(S.um1 + 3 * b);
+      V3 Vb, Om;
+      if (nt < X.num_euler) {
+        Vb = (1.0 / X.dt) * (x - xm1);
+        Om = (1.0 / X.dt) * u;
+      } else {
+        const double c1 = -4.0 / 3.0, c2 = 1.0 / 3.0, c3 = -2.0 / 3.0;
+        Vb = (-1.0 / (c3 * X.dt)) * (x + c1 * xm1 + c2 * xm2);
+        Om = dexp(u, (1.0 / (c3 * X.dt)) * (c2 * um1 - u));
+      }
+      const V3 d = {boff[j], boff[X.m + j], boff[2 * (size_t)X.m + j]};
+      e = e + Vb + cross(Om, d);
+    }
+    err[3 * (size_t)j] = e.x;
+    err[3 * (size_t)j + 1] = e.y;
+    err[3 * (size_t)j + 2] = e.z;
+    bj = fmax(fabs(e.x), fmax(fabs(e.y), fabs(e.z))) / X.tol;
+    if (!(e.x == e.x && e.y == e.y && e.z == e.z)) bj = __builtin_nan("");
+  }
+  __shared__ double red[256];
+  red[threadIdx.x] = bj;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) {
+      const double a = red[threadIdx.x], c = red[threadIdx.x + o];
+      red[threadIdx.x] = (a != a || c != c) ? __builtin_nan("") : fmax(a, c);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) bad[blockIdx.x] = red[0];
+}
+
+// Body block of J0 with the blob forces among the unknowns and its Schur complement
+// (swimmer.cpp:575-695), inverted.  One workgroup per body: the blob sums
+// sum rcross(d), sum rcross(d)^2 and sum rcross(f) rcross(d) (d = Q_init ref, f the blob
+// force) in a fixed order, then thread 0 adds the filaments' terms.
+__global__ __launch_bounds__(256) void bx_schur_kernel(BxK X, SwimK S, FilK K, int nt, int* __restrict__ status) {
+  __shared__ double red[4][27];
+  const int b = blockIdx.x;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  double acc[27];
+  for (int i = 0; i < 27; ++i) acc[i] = 0.0;
+  for (int j = b * X.nb + threadIdx.x; j < (b + 1) * X.nb; j += 256) {
+    const M3 Rc = rcross(bx_disp0(X, S, j));
+    const M3 Rc2 = mm(Rc, Rc);
+    const M3 FR = mm(rcross(v3(X.fb[j], X.fb[X.m + j], X.fb[2 * (size_t)X.m + j])), Rc);
+    for (int r = 0; r < 3; ++r)
+      for (int q = 0; q < 3; ++q) {
+        acc[3 * r + q] += Rc.a[r][q];
+        acc[9 + 3 * r + q] += Rc2.a[r][q];
+        acc[18 + 3 * r + q] += FR.a[r][q];
+      }
+  }
+  for (int i = 0; i < 27; ++i) {
+    double v = acc[i];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if (lane == 0) red[wave][i] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  for (int i = 0; i < 27; ++i) acc[i] = (red[0][i] + red[1][i]) + (red[2][i] + red[3][i]);
+  const double bmf = (nt < X.num_euler ? -6.0 : -9.0) * 3.14159265358979323846 * X.mu * X.a / X.dt;
+  double A[36];
+  for (int i = 0; i < 36; ++i) A[i] = 0.0;
+  for (int i = 0; i < 3; ++i) {
+    A[i * 6 + i] = -X.nb * bmf;
+    for (int j = 0; j < 3; ++j) {
+      A[i * 6 + 3 + j] = -bmf * acc[3 * i + j];
+      A[(3 + i) * 6 + j] = bmf * acc[3 * i + j];
+      A[(3 + i) * 6 + 3 + j] = acc[18 + 3 * i + j] + bmf * acc[9 + 3 * i + j];
+    }
+  }
+  const int n = K.nseg;
+  const size_t ns = K.nseg_total;
+  const V3 u = ldv(S.u + 3 * b);
+  M3 T = zero3();
+  for (int f = b * S.fpb; f < (b + 1) * S.fpb; ++f) {
+    const V3 d = base_disp(S, f);
+    const int s0 = f * n;
+    const V3 lt = {K.lam[s0], K.lam[ns + s0], K.lam[2 * ns + s0]};
+    const V3 cl = {K.clamp[f], K.clamp[K.nfil + f], K.clamp[2 * K.nfil + f]};
+    T = madd(T, mm(rcross(lt), rcross(d)));
+    T = madd(T, rcross(cl), 0.5);
+    T = madd(T, madd(rcross(cross(u, cl)), mm(rcross(u), rcross(cl)), -1.0), 1.0 / 12.0);
+    const V3 t0 = tangent(segq(K, s0)), tl = tangent(segq(K, s0 + n - 1));
+    T = madd(T, mm(rcross((-S.end_force) * tl), rcross((0.5 * S.dl) * t0)));
+  }
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) A[(3 + i) * 6 + 3 + j] += T.a[i][j];
+  for (int i = 0; i < 36; ++i) S.S0[36 * b + i] = A[i];
+  for (int f = b * S.fpb; f < (b + 1) * S.fpb; ++f)
+    for (int i = 0; i < 36; ++i) A[i] -= S.Sf[36 * (size_t)f + i];
+  double I6[36];
+  if (!invert6(A, I6)) status[0] = 1;
+  for (int i = 0; i < 36; ++i) S.Sinv[36 * b + i] = I6[i];
+}
+
+// Body block of J0^-1 in with the blob rows eliminated (swimmer.cpp:946-971):
+// t = S^-1 (in_b - sum_f D J_f^-1 in_f + 6 pi mu a (sum_j in_j, sum_j d_j x in_j));
+// out_b = scale t.  One workgroup per body.
+__global__ __launch_bounds__(256) void bx_b2_kernel(BxK X, SwimK S, int N, const double* __restrict__ in,
+                                                    double* __restrict__ out, double scale) {
+  __shared__ double red[4][6];
+  const int b = blockIdx.x;
+  const size_t L0 = (size_t)S.nfil * N, G0 = L0 + 6 * (size_t)S.nbody;
+  double s[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  const size_t e0 = (size_t)b * S.fpb * N, e1 = e0 + (size_t)S.fpb * N;
+  for (size_t e = e0 + threadIdx.x; e < e1; e += 256) {
+    const size_t f = e / N, c = e - f * N;
+    const double* bj = S.BJ + f * 6 * N + c;
+    const double v = in[e];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) s[k] += bj[(size_t)k * N] * v;
+  }
+  bx_reduce6(s, red);
+  double g[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  __syncthreads();  // red is reused
+  for (int j = b * X.nb + threadIdx.x; j < (b + 1) * X.nb; j += 256) {
+    const V3 r = ldv(in + G0 + 3 * (size_t)j);
+    const V3 t = cross(bx_disp0(X, S, j), r);
+    g[0] += r.x;
+    g[1] += r.y;
+    g[2] += r.z;
+    g[3] += t.x;
+    g[4] += t.y;
+    g[5] += t.z;
+  }
+  bx_reduce6(g, red);
+  if (threadIdx.x == 0) {
+    const double bmf0 = -6.0 * 3.14159265358979323846 * X.mu * X.a;
+    double b3[6];
+    for (int k = 0; k < 6; ++k) b3[k] = in[L0 + 6 * b + k] - s[k] - bmf0 * g[k];
+    for (int k = 0; k < 6; ++k) {
+      double t = 0.0;
+      for (int j = 0; j < 6; ++j) t += S.Sinv[36 * b + 6 * k + j] * b3[j];
+      S.tb[6 * b + k] = t;
+      out[L0 + 6 * b + k] = scale * t;
+    }
+  }
+}
+
+// Blob block of J0^-1 in: -6 pi mu a (in_j - dt_fac (t_V + t_Omega x d_j)) for free
+// bodies (swimmer.cpp:973-989), -6 pi mu a in_j with the body fixed (:940-944).
+__global__ void bx_blob_out_kernel(BxK X, SwimK S, int nt, const double* __restrict__ in,
+                                   double* __restrict__ out, double scale) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= X.m) return;
+  V3 r = ldv(in + 3 * (size_t)j);
+  if (X.nbody > 0) {
+    const int b = j / X.nb;
+    const double dtf = (nt < X.num_euler ? 1.0 : 1.5) / X.dt;
+    r = r - dtf * (ldv(S.tb + 6 * b) + cross(ldv(S.tb + 6 * b + 3), bx_disp0(X, S, j)));
+  }
+  const double c = scale * (-6.0 * 3.14159265358979323846 * X.mu * X.a);
+  stv(out + 3 * (size_t)j, c * r);
+}
+
 inline dim3 gs(int n, int b) { return dim3((unsigned)((n + b - 1) / b)); }
 
 }  // namespace
@@ -476,7 +735,7 @@ Status swim_errors(cfm_ctx* c, const SwimK& S, int nt, double* err_body, double*
 Status swim_jacobian(cfm_ctx* c, const SwimK& S, const FilK& K, int nt, int* status_dev) {
   const int N = 6 * K.nseg;
   const size_t lds = sizeof(double) * 18 * N;
-  hipLaunchKernelGGL(swim_bj_kernel, dim3(S.nfil), dim3(256), lds, c->stream, S, K, nt);
+  hipLaunchKernelGGL(swim_bj_kernel, dim3(S.nfil), dim3(256), lds, c->stream, S, K, nt, 0);
   hipLaunchKernelGGL(swim_schur_kernel, gs(S.nbody, 64), dim3(64), 0, c->stream, S, K, nt, status_dev);
   CFM_CHECK_LAUNCH();
   return {};
@@ -489,6 +748,64 @@ Status swim_jinv_mult(cfm_ctx* c, const SwimK& S, const FilK& K, const double* i
   hipLaunchKernelGGL(swim_rhs_kernel, gs((int)K.nseg_total, 256), dim3(256), 0, c->stream, S, K, in, tmp);
   CFM_CHECK_LAUNCH();
   return fil_jinv_mult(c, K, tmp, out, scale);
+}
+
+
+Status bx_guess(cfm_ctx* c, const BxK& X, int nt) {
+  if (X.m == 0) return {};
+  hipLaunchKernelGGL(bx_guess_kernel, gs(3 * X.m, 256), dim3(256), 0, c->stream, X, nt);
+  CFM_CHECK_LAUNCH();
+  return {};
+}
+
+Status bx_update(cfm_ctx* c, const BxK& X, const double* upd) {
+  if (X.m == 0) return {};
+  hipLaunchKernelGGL(bx_update_kernel, gs(3 * X.m, 256), dim3(256), 0, c->stream, X, upd);
+  CFM_CHECK_LAUNCH();
+  return {};
+}
+
+Status bx_errors(cfm_ctx* c, const BxK& X, const SwimK& S, int nt, double* err_body, double* err_blob,
+                 double* bad) {
+  const Particles& p = c->p;
+  const double* boff = X.nbody > 0 ? body_offsets(c) : nullptr;
+  if (X.nbody > 0) {
+    if (!boff) return make_status(CFM_ESTATE, "no body placement for the blob rows");
+    hipLaunchKernelGGL(bx_body_err_kernel, dim3(X.nbody), dim3(256), 0, c->stream, X, S, boff, err_body, bad);
+  }
+  hipLaunchKernelGGL(bx_blob_err_kernel, gs(X.m, 256), dim3(256), 0, c->stream, X, S, nt, (const double*)p.v.p,
+                     (size_t)p.n, p.n_seg, boff, err_blob, bad + X.nbody);
+  CFM_CHECK_LAUNCH();
+  return {};
+}
+
+Status bx_jacobian(cfm_ctx* c, const BxK& X, const SwimK& S, const FilK& K, int nt, int* status_dev) {
+  if (X.nbody == 0) return {};
+  const int N = 6 * K.nseg;
+  const size_t lds = sizeof(double) * 18 * N;
+  hipLaunchKernelGGL(swim_bj_kernel, dim3(S.nfil), dim3(256), lds, c->stream, S, K, nt, 1);
+  hipLaunchKernelGGL(bx_schur_kernel, dim3(S.nbody), dim3(256), 0, c->stream, X, S, K, nt, status_dev);
+  CFM_CHECK_LAUNCH();
+  return {};
+}
+
+Status bx_jinv_mult(cfm_ctx* c, const BxK& X, const SwimK& S, const FilK& K, int nt, const double* in,
+                    double* out, double scale, double* tmp) {
+  const size_t L0 = (size_t)K.nseg_total * 6;
+  const size_t G0 = L0 + 6 * (size_t)X.nbody;
+  if (X.nbody > 0) {
+    const int N = 6 * K.nseg;
+    hipLaunchKernelGGL(bx_b2_kernel, dim3(S.nbody), dim3(256), 0, c->stream, X, S, N, in, out, scale);
+    hipLaunchKernelGGL(bx_blob_out_kernel, gs(X.m, 256), dim3(256), 0, c->stream, X, S, nt, in + G0, out + G0,
+                       scale);
+    hipLaunchKernelGGL(swim_rhs_kernel, gs((int)K.nseg_total, 256), dim3(256), 0, c->stream, S, K, in, tmp);
+    CFM_CHECK_LAUNCH();
+    return fil_jinv_mult(c, K, tmp, out, scale);
+  }
+  hipLaunchKernelGGL(bx_blob_out_kernel, gs(X.m, 256), dim3(256), 0, c->stream, X, S, nt, in + G0, out + G0,
+                     scale);
+  CFM_CHECK_LAUNCH();
+  return fil_jinv_mult(c, K, in, out, scale);
 }
 
 }  // namespace cfm

@@ -30,6 +30,18 @@ struct SwimK {
   double* tb = nullptr;          // 6 per body: body part of the last J0^-1 product
 };
 
+// Blob forces among the Broyden unknowns (the reference's SOLVER_TYPE 0,
+// config.hpp:360-364): every blob's force ([3][m]: component k of blob j at k m + j) and
+// its two previous steps; free bodies (nbody > 0, nb blobs each, their unknowns and rows
+// as for SwimK) or a body held fixed (nbody = 0).  Broyden vector: the filaments, then
+// 6 per free body, then 3 per blob.
+struct BxK {
+  int m = 0, nbody = 0, nb = 0, num_euler = 1;
+  double dt = 0.1, tol = 1e-4, mu = 1.0, a = 1.0;
+  double *fb = nullptr, *fb_m1 = nullptr, *fb_m2 = nullptr;
+  const double* ref = nullptr;  // free bodies: body-frame offsets of one body's blobs (AoS)
+};
+
 Status swim_guess(cfm_ctx* c, const SwimK& S, int nt);
 Status swim_update(cfm_ctx* c, const SwimK& S, const double* upd_body);
 Status swim_force(cfm_ctx* c, const SwimK& S, const FilK& K, const double* rep, const double* boff);
@@ -37,5 +49,15 @@ Status swim_errors(cfm_ctx* c, const SwimK& S, int nt, double* err_body, double*
 Status swim_jacobian(cfm_ctx* c, const SwimK& S, const FilK& K, int nt, int* status_dev);
 Status swim_jinv_mult(cfm_ctx* c, const SwimK& S, const FilK& K, const double* in, double* out,
                       double scale, double* tmp);
+
+Status bx_guess(cfm_ctx* c, const BxK& X, int nt);
+Status bx_update(cfm_ctx* c, const BxK& X, const double* upd);  // upd: 3 per blob
+// Body rows (6 per free body) into err_body, blob rows (3 per blob) into err_blob; the
+// badness of each body and of each 256 blobs into bad[0 .. nbody + ceil(m / 256)).
+Status bx_errors(cfm_ctx* c, const BxK& X, const SwimK& S, int nt, double* err_body, double* err_blob,
+                 double* bad);
+Status bx_jacobian(cfm_ctx* c, const BxK& X, const SwimK& S, const FilK& K, int nt, int* status_dev);
+Status bx_jinv_mult(cfm_ctx* c, const BxK& X, const SwimK& S, const FilK& K, int nt, const double* in,
+                    double* out, double scale, double* tmp);
 
 }  // namespace cfm

@@ -37,7 +37,7 @@ class FilamentSystem:
     def __init__(self, x_seg, tangents, *, x_blob=None, precision="fp64", solve_mode="fp64",
                  blob_rtol=1e-10, blob_restart=50, blob_max_iter=2000, blob_precond="diag",
                  guess="previous", pert=None, device=0, tile_split=0, wall=True, overlap=False,
-                 **params):
+                 broyden_only=False, **params):
         prm = dict(DEFAULTS)
         prm.update(params)
         self.params = prm
@@ -70,6 +70,10 @@ class FilamentSystem:
                                         restart=blob_restart, max_iter=blob_max_iter,
                                         use_guess=1, rtol=blob_rtol)
         P.overlap = int(bool(overlap))
+        # blob forces (and attached bodies' states) among the Broyden unknowns: the
+        # reference's SOLVER_TYPE 0
+        P.broyden_only = int(bool(broyden_only))
+        self.broyden_only = bool(broyden_only) and n_blob > 0
         self._P = P
         self._x0 = (x_seg, None if x_blob is None else L.f64(x_blob))
         self._q0 = q
@@ -188,8 +192,16 @@ class FilamentSystem:
 
     @property
     def n_unknowns(self):
-        """Length of the Broyden vectors: 6 per segment, plus 6 per attached body."""
-        return 6 * self.n_seg + 6 * getattr(self, "n_body", 0)
+        """Length of the Broyden vectors: 6 per segment, plus 6 per attached body, plus 3
+        per blob with broyden_only."""
+        return 6 * self.n_seg + 6 * getattr(self, "n_body", 0) + (3 * self.n_blob if self.broyden_only else 0)
+
+    def blob_forces(self):
+        """Blob forces (n_blob, 3): the Broyden state with broyden_only, else the last
+        blob solve's."""
+        f = np.zeros(3 * self.n_blob)
+        L.check(self.h, L.lib().cfm_fil_get_blob_forces(self.h, L.ptr(f)))
+        return f.reshape(-1, 3)
 
     def errors(self, nt, v_seg):
         e = np.zeros(self.n_unknowns)

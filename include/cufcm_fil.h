@@ -204,7 +204,13 @@ typedef struct cfm_fil_params {
   cfm_solve_opts blob_solve; /* inner GMRES on the body blobs (system field ignored) */
   int32_t overlap;         /* 1: M_ss f_seg runs on a second stream while the blob solve runs
                               (same results; the pair kernel shares the GPU meanwhile) */
-  int32_t reserved;
+  int32_t broyden_only;    /* 1: the blob forces -- and, after cfm_fil_attach_bodies, the bodies'
+                              positions and orientations -- are Broyden unknowns, every product
+                              is taken from the current loads and no blob system is solved (the
+                              reference's SOLVER_TYPE 0, config.hpp:360-364, 443-447); the
+                              Broyden vector is the filaments, 6 per free body, 3 per blob.
+                              blob_solve, guess and overlap are then unused.  0: GMRES for the
+                              blob system inside every Broyden iteration (SOLVER_TYPE 1). */
 } cfm_fil_params;
 
 typedef struct cfm_step_stats {
@@ -251,6 +257,7 @@ int cfm_fil_internal_forces(cfm_ctx* ctx, int32_t nt, double* f_seg);
 int cfm_barrier_forces(cfm_ctx* ctx, double* f_seg);
 int cfm_fil_jacobian_inverse(cfm_ctx* ctx, int32_t nt, double* jinv);
 int cfm_fil_errors(cfm_ctx* ctx, int32_t nt, const double* v_seg, double* err, double* badness);
+/*   (cfm_fil_errors refuses a broyden_only context: its residual needs the blob velocities) */
 int cfm_fil_update(cfm_ctx* ctx, const double* upd);
 int cfm_fil_initial_guess(cfm_ctx* ctx, int32_t nt);
 
@@ -268,11 +275,19 @@ int cfm_fil_initial_guess(cfm_ctx* ctx, int32_t nt);
  * Broyden vectors of cfm_fil_errors / cfm_fil_update then hold 6 n_seg + 6 n_body
  * values (bodies last: position increment, Lie-algebra increment).  Needs equal blob
  * and segment radii and a blob preconditioner NONE, DIAG or SELF; the blob-force guess
- * becomes the previous solution. */
+ * becomes the previous solution.  With cfm_fil_params.broyden_only the blob forces are
+ * Broyden unknowns too (SOLVER_TYPE 0): the body rows become the force and torque
+ * balance -f + sum of the blob forces and their torques about the centre, 3 blob rows
+ * V + Omega x (R r) - v_blob follow (mobility_solver.cpp:2768-2999), J0^-1 takes the body
+ * Schur complement of swimmer.cpp:575-695 and the blob block -6 pi mu a
+ * (swimmer.cpp:929-1018), and the vectors hold 6 n_seg + 6 n_body + 3 n_blob values. */
 int cfm_fil_attach_bodies(cfm_ctx* ctx, const double* f_ext);
 /* Body centres (3 n_body), quaternions (4), Lie-algebra elements of the step (3) and the
  * last solve's (V, Omega) (6); each pointer may be null. */
 int cfm_fil_get_body_state(cfm_ctx* ctx, double* x, double* q, double* u, double* vel);
+/* Blob forces (3 n_blob, AoS): the Broyden state with broyden_only, else the last blob
+ * solve's (the reference's f_blobs_host / rigid_body::blob_forces). */
+int cfm_fil_get_blob_forces(cfm_ctx* ctx, double* f_blob);
 /* The bodies are spheres of this radius (0: off, the default): the approximate Jacobian's
  * body rows then also carry the RPY coupling of sphere and segments -- the unequal-sphere
  * translation-translation, translation-rotation and rotation-rotation blocks of every
