@@ -274,6 +274,7 @@ int cfm_destroy(cfm_ctx* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   if (c->rocblas) (void)rocblas_destroy_handle((rocblas_handle)c->rocblas);
   c->tune_v.release();
+  c->red_count.release();
   c->p.x.release();
   c->p.y.release();
   c->p.z.release();

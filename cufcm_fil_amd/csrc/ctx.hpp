@@ -212,6 +212,9 @@ struct cfm_ctx {
   // autotuned source splits (tile_split = -1), keyed on the product's shape
   std::map<std::array<long long, 6>, int> split_cache;
   cfm::DevBuf<double> tune_v;           // scratch output of the tuning launches
+  // arrival counter of the single-launch reductions on `stream` (zero between launches:
+  // the last workgroup of each reduction resets it)
+  cfm::DevBuf<unsigned> red_count;
   // single-species wall products evaluating each unordered pair once (apply_sym.hip)
   bool sym_pairs = true;
   cfm::DevBuf<double> sym_own, sym_own_side;  // per-workgroup target partials

@@ -55,14 +55,57 @@ __global__ __launch_bounds__(kRB) void dots_partial_kernel(const double* __restr
   if (threadIdx.x == 0) partial[(size_t)b * k + i] = red[0];
 }
 
-// out[i] (+)= sum_b partial[b*k + i], fixed order
-__global__ void dots_finish_kernel(const double* __restrict__ partial, int nb, int k,
-                                   double* __restrict__ out, int accumulate) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= k) return;
+// Q[0..k)^T w in one launch.  Block (i, b) stores the partial of chunk b for column i
+// (as dots_partial_kernel), publishes it with an agent-scope release fence and takes a
+// ticket from `counter`; the block holding the last ticket (after an acquire fence)
+// reduces every column's partials -- a wave per column: lane l adds chunks l, l + 64,
+// ... in order, then the 64 lane sums are added by a butterfly in a fixed pattern --
+// and resets the counter for the next reduction queued on the stream.  No block waits
+// for another, and the order of every addition is fixed (deterministic).
+// mode 0: out[i] = s; 1: out[i] += s; 2: out[i] = sqrt(max(s, 0)).  out2, when given,
+// receives the same value at out2[i * ld2].
+__global__ __launch_bounds__(kRB) void dots_fused_kernel(const double* __restrict__ Q, size_t ldq,
+                                                         int k, const double* __restrict__ w, int L,
+                                                         double* partial, unsigned* counter,
+                                                         double* out, int mode, double* out2,
+                                                         int ld2) {
+  __shared__ double red[kRB];
+  __shared__ int last;
+  const int i = blockIdx.x;
+  const int b = blockIdx.y;
+  const double* q = Q + (size_t)i * ldq;
+  const int r0 = b * kChunk;
+  const int r1 = min(L, r0 + kChunk);
   double s = 0.0;
-  for (int b = 0; b < nb; ++b) s += partial[(size_t)b * k + i];
-  out[i] = accumulate ? out[i] + s : s;
+  for (int r = r0 + threadIdx.x; r < r1; r += kRB) s += q[r] * w[r];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int off = kRB / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    partial[(size_t)b * k + i] = red[0];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    last = atomicAdd(counter, 1u) == gridDim.x * gridDim.y - 1u;
+  }
+  __syncthreads();
+  if (!last) return;
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  const int nb = gridDim.y;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int col = wave; col < k; col += kRB / 64) {
+    double t = 0.0;
+    for (int bb = lane; bb < nb; bb += 64) t += partial[(size_t)bb * k + col];
+    for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
+    if (lane == 0) {
+      if (mode == 2) t = sqrt(fmax(t, 0.0));
+      const double v = mode == 1 ? out[col] + t : t;
+      out[col] = v;
+      if (out2) out2[(size_t)col * ld2] = v;
+    }
+  }
+  if (threadIdx.x == 0) *counter = 0u;
 }
 
 // w[r] -= sum_{i<k} h[i] Q[i][r]
@@ -103,8 +146,6 @@ __global__ void fill_kernel(double* __restrict__ x, double v, int L) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r < L) x[r] = v;
 }
-
-__global__ void sqrt_kernel(double* x) { x[0] = sqrt(fmax(x[0], 0.0)); }
 
 // Copies `dof` rows of length m between two row-major blocks.
 __global__ void copy_rows_kernel(const double* __restrict__ src, size_t src_ld, int src_off,
@@ -303,30 +344,56 @@ void Krylov::release() {
   L = m = 0;
 }
 
-// out[0..k) = Q[0..k)^T w
-Status dots(cfm_ctx* c, Krylov& K, const double* Q, size_t ldq, int k, const double* w, int L,
-            double* out, bool accumulate) {
+// The context's reduction counter, created (zeroed on the context's stream) on first use.
+static Status red_counter(cfm_ctx* c, unsigned** out) {
+  if (!c->red_count.p) {
+    CFM_TRY(c->red_count.alloc(1));
+    CFM_HIP(hipMemsetAsync(c->red_count.p, 0, sizeof(unsigned), c->stream));
+  }
+  *out = c->red_count.p;
+  return {};
+}
+
+// One-launch reduction of Q[0..k)^T w into out (mode as dots_fused_kernel).
+static Status dots_launch(cfm_ctx* c, double* partial, const double* Q, size_t ldq, int k,
+                          const double* w, int L, double* out, int mode, double* out2 = nullptr,
+                          int ld2 = 0) {
+  if (k <= 0) return {};
+  unsigned* cnt = nullptr;
+  CFM_TRY(red_counter(c, &cnt));
   const int nb = (L + kChunk - 1) / kChunk;
-  hipLaunchKernelGGL(dots_partial_kernel, dim3(k, nb), dim3(kRB), 0, c->stream, Q, ldq, k, w, L,
-                     K.partial.p);
-  CFM_CHECK_LAUNCH();
-  hipLaunchKernelGGL(dots_finish_kernel, dim3((k + 63) / 64), dim3(64), 0, c->stream,
-                     K.partial.p, nb, k, out, accumulate ? 1 : 0);
+  hipLaunchKernelGGL(dots_fused_kernel, dim3(k, nb), dim3(kRB), 0, c->stream, Q, ldq, k, w, L,
+                     partial, cnt, out, mode, out2, ld2);
   CFM_CHECK_LAUNCH();
   return {};
 }
 
+// out[0..k) = Q[0..k)^T w
+Status dots(cfm_ctx* c, Krylov& K, const double* Q, size_t ldq, int k, const double* w, int L,
+            double* out, bool accumulate) {
+  return dots_launch(c, K.partial.p, Q, ldq, k, w, L, out, accumulate ? 1 : 0);
+}
+
 Status norm2(cfm_ctx* c, Krylov& K, const double* w, int L, double* out) {
-  CFM_TRY(dots(c, K, w, 0, 1, w, L, out, false));
-  hipLaunchKernelGGL(sqrt_kernel, dim3(1), dim3(1), 0, c->stream, out);
-  CFM_CHECK_LAUNCH();
-  return {};
+  return dots_launch(c, K.partial.p, w, 0, 1, w, L, out, 2);
 }
 
 // Copies n <= 4 consecutive device doubles to K.host_res[0..n) with one wait.
 Status read_scalars(cfm_ctx* c, Krylov& K, const double* dev, int n) {
   CFM_HIP(hipMemcpyAsync(K.host_res, dev, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
   CFM_HIP(hipStreamSynchronize(c->stream));
+  return {};
+}
+
+// As read_scalars, running hooks->before_first_wait (when given) between queueing the
+// copy and waiting for it.
+static Status read_scalars_hooked(cfm_ctx* c, Krylov& K, const double* dev, int n,
+                                  const GmresHooks* hooks) {
+  if (!hooks || !hooks->before_first_wait) return read_scalars(c, K, dev, n);
+  CFM_HIP(hipMemcpyAsync(K.host_res, dev, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+  CFM_HIP(hipEventRecord(K.ev[0], c->stream));
+  CFM_TRY(hooks->before_first_wait(c, hooks->wait_user));
+  CFM_HIP(hipEventSynchronize(K.ev[0]));
   return {};
 }
 
@@ -522,11 +589,12 @@ Status solve(cfm_ctx* c, Krylov& K, const SystemDesc& s, const cfm_solve_opts& o
     hipLaunchKernelGGL(axpby_kernel, g1(L), dim3(256), 0, c->stream, 1.0, b, -1.0, K.w.p, K.d.p, L);
     CFM_CHECK_LAUNCH();
     CFM_TRY(norm2(c, K, K.d.p, L, K.scal.p));
-    CFM_TRY(read_scalars(c, K, K.scal.p, 4));
+    CFM_TRY(read_scalars_hooked(c, K, K.scal.p, 4, hooks));
     nr = K.host_res[0];
     nb = K.host_res[3];
   } else {
-    CFM_TRY(read_scalar(c, K, K.scal.p + 3, &nb));
+    CFM_TRY(read_scalars_hooked(c, K, K.scal.p + 3, 1, hooks));
+    nb = K.host_res[0];
     nr = nb;
   }
   if (!std::isfinite(nb)) return make_status(CFM_ENONFINITE, "non-finite right-hand side");
@@ -615,12 +683,11 @@ namespace cfm {
 // ---- BLAS-1/2 helpers shared with the Broyden driver -------------------------
 Status dev_dots(cfm_ctx* c, double* partial, const double* Q, size_t ldq, int k, const double* w,
                 int L, double* out) {
-  const int nb = (L + kChunk - 1) / kChunk;
-  hipLaunchKernelGGL(dots_partial_kernel, dim3(k, nb), dim3(kRB), 0, c->stream, Q, ldq, k, w, L, partial);
-  CFM_CHECK_LAUNCH();
-  hipLaunchKernelGGL(dots_finish_kernel, dim3((k + 63) / 64), dim3(64), 0, c->stream, partial, nb, k, out, 0);
-  CFM_CHECK_LAUNCH();
-  return {};
+  return dots_launch(c, partial, Q, ldq, k, w, L, out, 0);
+}
+Status dev_dots2(cfm_ctx* c, double* partial, const double* Q, size_t ldq, int k, const double* w,
+                 int L, double* out, double* out2, int ld2) {
+  return dots_launch(c, partial, Q, ldq, k, w, L, out, 0, out2, ld2);
 }
 size_t dev_dots_partial_size(int L, int k) { return (size_t)((L + kChunk - 1) / kChunk) * k; }
 
@@ -650,10 +717,7 @@ Status dev_scale_by_inv(cfm_ctx* c, const double* x, const double* s, double* ou
   return {};
 }
 Status dev_norm2(cfm_ctx* c, double* partial, const double* w, int L, double* out) {
-  CFM_TRY(dev_dots(c, partial, w, 0, 1, w, L, out));
-  hipLaunchKernelGGL(sqrt_kernel, dim3(1), dim3(1), 0, c->stream, out);
-  CFM_CHECK_LAUNCH();
-  return {};
+  return dots_launch(c, partial, w, 0, 1, w, L, out, 2);
 }
 Status dev_copy_rows(cfm_ctx* c, const double* src, size_t src_ld, int src_off, double* dst,
                      size_t dst_ld, int dst_off, int dof, int m) {

@@ -42,11 +42,17 @@ Status body_couple_apply(cfm_ctx* c, const double* x, double* y);
 Status body_precond(cfm_ctx* c, int kind, const double* x, double* y);
 Status body_system_desc(cfm_ctx* c, SystemDesc* s);
 
-// Optional correction added to the preconditioner output (learned near-field term).
+// Optional correction added to the preconditioner output (learned near-field term),
+// and optional work queued once the solve has queued its first read-back (||b|| and the
+// residual of the guess) and before the host waits for it: work independent of the
+// solve that keeps the device busy meanwhile.  solve() calls before_first_wait at most
+// once, and always when it returns CFM_OK.
 struct GmresHooks {
   Status (*precond_correction)(cfm_ctx*, const SystemDesc&, const double* in, double* out,
                                void* user) = nullptr;
   void* user = nullptr;
+  Status (*before_first_wait)(cfm_ctx*, void* user) = nullptr;
+  void* wait_user = nullptr;
 };
 
 SystemDesc system_desc(const cfm_ctx* c, int system);
@@ -64,6 +70,9 @@ Status read_scalar(cfm_ctx* c, Krylov& K, const double* dev, double* host);
 namespace cfm {
 Status dev_dots(cfm_ctx* c, double* partial, const double* Q, size_t ldq, int k, const double* w,
                 int L, double* out);
+// As dev_dots, also writing out2[i * ld2] = out[i].
+Status dev_dots2(cfm_ctx* c, double* partial, const double* Q, size_t ldq, int k, const double* w,
+                 int L, double* out, double* out2, int ld2);
 size_t dev_dots_partial_size(int L, int k);
 Status dev_gemv_sub(cfm_ctx* c, const double* Q, size_t ldq, int k, const double* h, double* w, int L);
 Status dev_axpby(cfm_ctx* c, double a, const double* x, double b, const double* y, double* out, int L);

@@ -32,91 +32,169 @@ Status download_aos(cfm_ctx* c, double* host, int cnt, int dof, const double* sr
 Status gnn_guess(cfm_ctx* c, const double* rhs, double* x0, bool* used);
 Status gnn_record(cfm_ctx* c, const double* b, const double* x);
 
-// Ring of past (rhs, solution) pairs of the blob system and the Gram matrix of
-// the stored right-hand sides; the guess minimises ||b - sum_i c_i b_i|| (with a
-// tiny Tikhonov term) and returns sum_i c_i x_i.  Because A x_i = b_i to the
-// solver tolerance, its residual is the projection residual of b.
+// Ring of past (rhs, solution) pairs of the blob system and, on the device, the Gram
+// matrix of the stored right-hand sides (H x H, row-major).  The guess minimises
+// ||b - sum_i c_i b_i|| (with a tiny Tikhonov term) and returns sum_i c_i x_i.
+// Because A x_i = b_i to the solver tolerance, its residual is the projection residual
+// of b.  The coefficients are solved on the device, so the guess costs the host no wait.
 struct History {
   static constexpr int H = 16;
   int L = 0, count = 0, head = 0;
-  DevBuf<double> B, X, g, grow, coef, partial;
-  double G[H][H] = {};
-  double* host = nullptr;  // pinned: [0,H) dots with b, [H,2H) new Gram row, [2H,3H) coefficients
-  int pending = -1;        // slot whose Gram row is computed on the device but not yet read
+  DevBuf<double> B, X, G, g, coef, partial;
+  DevBuf<int> used;  // 1 when the last guess replaced x0
 };
 
-// Folds a Gram row that has reached h.host[H..] into G.
-static void history_fold_row(History& h) {
-  if (h.pending < 0) return;
-  for (int i = 0; i < h.count; ++i) {
-    h.G[h.pending][i] = h.host[History::H + i];
-    h.G[i][h.pending] = h.host[History::H + i];
+// Solves (G + d I) c = g, d = 1e-13 tr(G) / k, for the k <= H coefficients of the
+// projected guess: Gaussian elimination with partial pivoting (the first row holding
+// the largest magnitude), then column-oriented back substitution.  *used = 0,
+// coefficients untouched, when a pivot is not positive in magnitude or a coefficient
+// is not finite.  One wave: the augmented matrix lives in LDS, lane 0 searches the
+// pivot, the lanes share the row updates; every operation has a fixed order.
+__global__ __launch_bounds__(64) void history_solve_kernel(const double* __restrict__ G,
+                                                           const double* __restrict__ g, int k,
+                                                           double* __restrict__ coef,
+                                                           int* __restrict__ used) {
+  constexpr int H = History::H;
+  constexpr int W = H + 1;
+  __shared__ double A[H][W];
+  __shared__ double fr[H];
+  __shared__ double dshift;
+  __shared__ int piv, ok;
+  const int t = threadIdx.x;
+  // every lane issues its loads before any is used (device-memory latency paid once)
+  double ld[(H * W + 63) / 64];
+#pragma unroll
+  for (int q = 0; q < (H * W + 63) / 64; ++q) {
+    const int e = t + 64 * q, i = e / W, j = e % W;
+    ld[q] = (e < H * W && i < k && (j < k || j == H)) ? (j < k ? G[i * H + j] : g[i]) : 0.0;
   }
-  h.pending = -1;
+#pragma unroll
+  for (int q = 0; q < (H * W + 63) / 64; ++q) {
+    const int e = t + 64 * q;
+    if (e < H * W) A[e / W][e % W] = ld[q];
+  }
+  __syncthreads();
+  if (t == 0) {
+    double tr = 0.0;
+    for (int i = 0; i < k; ++i) tr += A[i][i];
+    dshift = 1e-13 * tr / k;
+  }
+  __syncthreads();
+  if (t < k) A[t][t] += dshift;
+  __syncthreads();
+  for (int col = 0; col < k; ++col) {
+    if (t == 0) {
+      int p = col;
+      double best = fabs(A[col][col]);
+      for (int r = col + 1; r < k; ++r) {
+        const double v = fabs(A[r][col]);
+        if (v > best) {
+          best = v;
+          p = r;
+        }
+      }
+      piv = p;
+      ok = best > 0.0;
+    }
+    __syncthreads();
+    if (!ok) {
+      if (t == 0) *used = 0;
+      return;
+    }
+    const int p = piv;
+    if (p != col)
+      for (int j = t; j < W; j += 64) {  // columns col..k and the right-hand side (H)
+        const double a = A[p][j];
+        A[p][j] = A[col][j];
+        A[col][j] = a;
+      }
+    __syncthreads();
+    if (t > col && t < k) fr[t] = A[t][col] / A[col][col];
+    __syncthreads();
+    const int w = k - col;  // columns col..k-1, then the right-hand side
+    for (int e = t; e < (k - 1 - col) * (w + 1); e += 64) {
+      const int r = col + 1 + e / (w + 1), jj = e % (w + 1);
+      const int j = jj < w ? col + jj : H;
+      A[r][j] -= fr[r] * A[col][j];
+    }
+    __syncthreads();
+  }
+  // back substitution by columns: x_i = b_i / A_ii, then b_r -= A_ri x_i for r < i
+  for (int i = k - 1; i >= 0; --i) {
+    if (t == 0) fr[i] = A[i][H] / A[i][i];
+    __syncthreads();
+    if (t < i) A[t][H] -= A[t][i] * fr[i];
+    __syncthreads();
+  }
+  if (t != 0) return;
+  for (int i = 0; i < k; ++i)
+    if (!isfinite(fr[i])) {
+      *used = 0;
+      return;
+    }
+  for (int i = 0; i < k; ++i) coef[i] = fr[i];
+  *used = 1;
 }
 
-static Status history_guess(cfm_ctx* c, History& h, const double* b, double* x0, bool* used) {
-  *used = false;
+// out[r] = sum_{i<k} y[i] Q[i][r] when *flag is set; out untouched otherwise.
+__global__ void gemv_if_kernel(const double* __restrict__ Q, size_t ldq, int k,
+                               const double* __restrict__ y, const int* __restrict__ flag,
+                               double* __restrict__ out, int L) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= L || !flag[0]) return;
+  double s = 0.0;
+  for (int i = 0; i < k; ++i) s += y[i] * Q[(size_t)i * ldq + r];
+  out[r] = s;
+}
+
+// dst_a = a, dst_b = b (length L each)
+__global__ void copy2_kernel(const double* __restrict__ a, double* __restrict__ dst_a,
+                             const double* __restrict__ b, double* __restrict__ dst_b, int L) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= L) return;
+  dst_a[r] = a[r];
+  dst_b[r] = b[r];
+}
+
+static Status history_reset(History& h, int L) {
+  h.L = L;
+  h.count = h.head = 0;
+  CFM_TRY(h.B.alloc((size_t)History::H * L));
+  CFM_TRY(h.X.alloc((size_t)History::H * L));
+  CFM_TRY(h.G.alloc((size_t)History::H * History::H));
+  CFM_TRY(h.g.alloc(History::H));
+  CFM_TRY(h.coef.alloc(History::H));
+  CFM_TRY(h.used.alloc(1));
+  return h.partial.alloc(dev_dots_partial_size(L, History::H));
+}
+
+// x0 = the projected guess once two solves are stored (*queued = true); x0 keeps its
+// value (the previous solution) when the device finds the Gram system singular.
+static Status history_guess(cfm_ctx* c, History& h, const double* b, double* x0, bool* queued) {
+  *queued = false;
   if (h.count < 2) return {};
   const int k = h.count;
   CFM_TRY(dev_dots(c, h.partial.p, h.B.p, (size_t)h.L, k, b, h.L, h.g.p));
-  CFM_HIP(hipMemcpyAsync(h.host, h.g.p, sizeof(double) * k, hipMemcpyDeviceToHost, c->stream));
-  if (h.pending >= 0)  // the last push's Gram row comes back with the same wait
-    CFM_HIP(hipMemcpyAsync(h.host + History::H, h.grow.p, sizeof(double) * k, hipMemcpyDeviceToHost, c->stream));
-  CFM_HIP(hipStreamSynchronize(c->stream));
-  history_fold_row(h);
-  // solve (G + d I) c = g by Gaussian elimination with partial pivoting (k <= 16)
-  double A[History::H][History::H + 1];
-  double tr = 0.0;
-  for (int i = 0; i < k; ++i) tr += h.G[i][i];
-  const double d = 1e-13 * tr / k;
-  for (int i = 0; i < k; ++i) {
-    for (int j = 0; j < k; ++j) A[i][j] = h.G[i][j] + (i == j ? d : 0.0);
-    A[i][k] = h.host[i];
-  }
-  for (int col = 0; col < k; ++col) {
-    int p = col;
-    for (int r = col + 1; r < k; ++r)
-      if (std::fabs(A[r][col]) > std::fabs(A[p][col])) p = r;
-    if (!(std::fabs(A[p][col]) > 0.0)) return {};
-    if (p != col)
-      for (int j = 0; j <= k; ++j) std::swap(A[p][j], A[col][j]);
-    for (int r = col + 1; r < k; ++r) {
-      const double f = A[r][col] / A[col][col];
-      for (int j = col; j <= k; ++j) A[r][j] -= f * A[col][j];
-    }
-  }
-  double cvec[History::H];
-  for (int i = k - 1; i >= 0; --i) {
-    double s = A[i][k];
-    for (int j = i + 1; j < k; ++j) s -= A[i][j] * cvec[j];
-    cvec[i] = s / A[i][i];
-    if (!std::isfinite(cvec[i])) return {};
-  }
-  // pinned staging: rewritten only after the next wait on this stream
-  double* pc = h.host + 2 * History::H;
-  for (int i = 0; i < k; ++i) pc[i] = cvec[i];
-  CFM_HIP(hipMemcpyAsync(h.coef.p, pc, sizeof(double) * k, hipMemcpyHostToDevice, c->stream));
-  CFM_TRY(dev_gemv(c, h.X.p, (size_t)h.L, k, h.coef.p, x0, h.L));
-  *used = true;
+  hipLaunchKernelGGL(history_solve_kernel, dim3(1), dim3(64), 0, c->stream, h.G.p, h.g.p, k,
+                     h.coef.p, h.used.p);
+  CFM_CHECK_LAUNCH();
+  hipLaunchKernelGGL(gemv_if_kernel, dim3((h.L + 255) / 256), dim3(256), 0, c->stream, h.X.p,
+                     (size_t)h.L, k, h.coef.p, h.used.p, x0, h.L);
+  CFM_CHECK_LAUNCH();
+  *queued = true;
   return {};
 }
 
+// Stores (b, x) in the next slot and its row and column of the Gram matrix.
 static Status history_push(cfm_ctx* c, History& h, const double* b, const double* x) {
-  if (h.pending >= 0) {  // a row no guess has read yet: fetch it before it is overwritten
-    CFM_HIP(hipMemcpyAsync(h.host + History::H, h.grow.p, sizeof(double) * h.count, hipMemcpyDeviceToHost, c->stream));
-    CFM_HIP(hipStreamSynchronize(c->stream));
-    history_fold_row(h);
-  }
   const int slot = h.head;
-  CFM_HIP(hipMemcpyAsync(h.B.p + (size_t)slot * h.L, b, sizeof(double) * h.L, hipMemcpyDeviceToDevice, c->stream));
-  CFM_HIP(hipMemcpyAsync(h.X.p + (size_t)slot * h.L, x, sizeof(double) * h.L, hipMemcpyDeviceToDevice, c->stream));
+  hipLaunchKernelGGL(copy2_kernel, dim3((h.L + 255) / 256), dim3(256), 0, c->stream, b,
+                     h.B.p + (size_t)slot * h.L, x, h.X.p + (size_t)slot * h.L, h.L);
+  CFM_CHECK_LAUNCH();
   h.head = (h.head + 1) % History::H;
   h.count = std::min(h.count + 1, History::H);
-  // new row/column of the Gram matrix, read back by the next guess
-  CFM_TRY(dev_dots(c, h.partial.p, h.B.p, (size_t)h.L, h.count, h.B.p + (size_t)slot * h.L, h.L, h.grow.p));
-  h.pending = slot;
-  return {};
+  return dev_dots2(c, h.partial.p, h.B.p, (size_t)h.L, h.count, h.B.p + (size_t)slot * h.L, h.L,
+                   h.G.p + (size_t)slot * History::H, h.G.p + slot, History::H);
 }
 
 struct FilState {
@@ -234,17 +312,29 @@ static Status velocities(cfm_ctx* c, FilState* F, cfm_step_stats* st) {
   }
   const bool overlap = F->prm.overlap != 0;
   if (overlap) CFM_TRY(side_fork(c));
+  // Without overlap the segment rows M_ss f_seg (the largest product) are queued on the
+  // main stream behind the blob solve's first read-back, so the device computes them
+  // while the host waits for that read; they touch only segment rows of v.
+  struct SegRows {
+    ApplyPlan pl;
+    int prec = 0;
+    bool queued = false;
+  } seg_rows{ss, prec, false};
+  seg_rows.pl.side = false;
+  auto queue_seg_rows = [](cfm_ctx* cc, void* u) -> Status {
+    SegRows* sr = static_cast<SegRows*>(u);
+    if (sr->queued) return {};
+    sr->queued = true;
+    return apply_plan(cc, sr->pl, sr->prec);
+  };
+  GmresHooks hooks;
+  if (!overlap) {
+    hooks.before_first_wait = queue_seg_rows;
+    hooks.wait_user = &seg_rows;
+  }
   auto blob_part = [&]() -> Status {
-    if (overlap) {
-      CFM_TRY(apply_plan(c, bs, prec));
-      CFM_TRY(apply_plan(c, ss, prec));
-    } else {  // segments and blobs from the segment loads, one launch
-      ApplyPlan all;
-      all.tgt_begin = 0;
-      all.tgt_end = p.n;
-      all.src_seg = true;
-      CFM_TRY(apply_plan(c, all, prec));
-    }
+    CFM_TRY(apply_plan(c, bs, prec));  // blob rows from the segment loads
+    if (overlap) CFM_TRY(apply_plan(c, ss, prec));
     // blob forces: Mbb f_b = -v_b (body held fixed, so K V_body = 0)
     CFM_TRY(F->rhs_b.alloc(s.L));
     CFM_TRY(F->fb_prev.alloc(s.L));
@@ -252,21 +342,10 @@ static Status velocities(cfm_ctx* c, FilState* F, cfm_step_stats* st) {
     CFM_TRY(dev_axpby(c, -1.0, F->rhs_b.p, 0.0, F->rhs_b.p, F->rhs_b.p, s.L));
     if (F->prm.guess == CFM_GUESS_PROJECTED || F->prm.guess == CFM_GUESS_LEARNED) {
       History& h = F->hist;
-      if (h.L != s.L) {
-        h.L = s.L;
-        h.count = h.head = 0;
-        CFM_TRY(h.B.alloc((size_t)History::H * s.L));
-        CFM_TRY(h.X.alloc((size_t)History::H * s.L));
-        CFM_TRY(h.g.alloc(History::H));
-        CFM_TRY(h.grow.alloc(History::H));
-        CFM_TRY(h.coef.alloc(History::H));
-        CFM_TRY(h.partial.alloc(dev_dots_partial_size(s.L, History::H)));
-        h.pending = -1;
-        if (!h.host) CFM_HIP(hipHostMalloc((void**)&h.host, sizeof(double) * 3 * History::H, hipHostMallocDefault));
-      }
+      if (h.L != s.L) CFM_TRY(history_reset(h, s.L));
       bool used = false;
       CFM_TRY(history_guess(c, h, F->rhs_b.p, F->fb_prev.p, &used));
-      // cold start (no history yet): the network's guess
+      // cold start (fewer than two stored solves): the network's guess
       if (!used && F->prm.guess == CFM_GUESS_LEARNED) CFM_TRY(gnn_guess(c, F->rhs_b.p, F->fb_prev.p, &used));
     }
     cfm_solve_opts o = F->prm.blob_solve;
@@ -274,7 +353,8 @@ static Status velocities(cfm_ctx* c, FilState* F, cfm_step_stats* st) {
     o.use_guess = 1;
     if (!c->krylov) c->krylov = new Krylov();
     cfm_solve_stats ss_{};
-    CFM_TRY(solve(c, *c->krylov, s, o, F->rhs_b.p, F->fb_prev.p, &ss_, nullptr));
+    CFM_TRY(solve(c, *c->krylov, s, o, F->rhs_b.p, F->fb_prev.p, &ss_, overlap ? nullptr : &hooks));
+    if (!overlap) CFM_TRY(queue_seg_rows(c, &seg_rows));  // already queued by the solve
     if (F->prm.guess == CFM_GUESS_PROJECTED || F->prm.guess == CFM_GUESS_LEARNED)
       CFM_TRY(history_push(c, F->hist, F->rhs_b.p, F->fb_prev.p));
     CFM_TRY(gnn_record(c, F->rhs_b.p, F->fb_prev.p));
@@ -487,11 +567,11 @@ void cfm_fil_release(cfm_ctx* c) {
   F->grid.release();
   F->hist.B.release();
   F->hist.X.release();
+  F->hist.G.release();
   F->hist.g.release();
-  F->hist.grow.release();
   F->hist.coef.release();
+  F->hist.used.release();
   F->hist.partial.release();
-  if (F->hist.host) (void)hipHostFree(F->hist.host);
   if (F->host) (void)hipHostFree(F->host);
   delete F;
   c->fil = nullptr;
