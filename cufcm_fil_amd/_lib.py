@@ -139,6 +139,8 @@ CFM_BEAT_FULFORD_BLAKE_ORIGINAL = 2
 CFM_BEAT_VOLVOX = 3
 CFM_BEAT_CORAL_LARVAE = 4
 CFM_BEAT_BUILD_A_BEAT = 5
+CFM_BEAT_BICILIA = 6
+CFM_BEAT_BICILIA_LONGT = 7
 
 
 class cfm_cilia_params(ctypes.Structure):
@@ -301,6 +303,7 @@ def lib():
     L.cfm_cilia_get_speeds.argtypes = [_vp, _dp, _dp, _dp, _dp]
     L.cfm_cilia_set_s_table.argtypes = [_vp, _dp, ctypes.c_int32, ctypes.c_int32]
     L.cfm_cilia_set_phases.argtypes = [_vp, _dp, _dp]
+    L.cfm_cilia_set_pair_dp.argtypes = [_vp, ctypes.c_double]
     L.cfm_cilia_init_bodies.argtypes = [_vp, ctypes.POINTER(cfm_cilia_params), _dp, _dp, _dp, _dp]
     L.cfm_cilia_get_bodies.argtypes = [_vp, _dp, _dp, _dp, _dp]
     L.cfm_cilia_set_driving_options.argtypes = [_vp, ctypes.c_int32, ctypes.c_double, ctypes.c_uint64]

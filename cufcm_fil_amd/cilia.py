@@ -24,14 +24,25 @@ def wall_lattice(nx, ny, spacing, dl=2.6):
     return np.array(b, dtype=np.float64).reshape(-1)
 
 
-BEATS = ("fulford_blake", "fulford_blake_original", "volvox", "coral_larvae", "build_a_beat")
+BEATS = ("fulford_blake", "fulford_blake_original", "volvox", "coral_larvae", "build_a_beat",
+         "bicilia", "bicilia_longt")
 
 
 def _beat(name):
-    """Beat name -> CFM_BEAT_* (the reference's SHAPE_SEQUENCE 1, 4, 3, 2 and 0)."""
+    """Beat name -> CFM_BEAT_* (the reference's SHAPE_SEQUENCE 1, 4, 3, 2, 0, 5 and 6;
+    the last two are paired cilia: each unit of nseg segments is two cilia of nseg / 2)."""
     if name not in BEATS:
         raise ValueError(f"unknown beat {name!r}; one of {BEATS}")
     return BEATS.index(name) + 1
+
+
+def pair_phase_speeds(n_cilia, omega0=2.0 * math.pi, pair_dp=1.0):
+    """Intrinsic phase speeds of the reference's PAIR 1 mode (filament.cpp:240-244): the
+    second half of the filaments beats at pair_dp times the frequency of the first."""
+    w = np.full(int(n_cilia), float(omega0))
+    npair = int(n_cilia) // 2           # NPAIR = NFIL / 2; filament id with id / NPAIR == 1
+    w[npair:2 * npair] *= pair_dp
+    return w
 
 
 class PrescribedCilia:
@@ -97,6 +108,11 @@ class PrescribedCilia:
         ph = None if phase is None else L.f64(phase, n)
         an = None if angle is None else L.f64(angle, n)
         L.check(self.mob._h, L.lib().cfm_cilia_set_phases(self.mob._h, L.ptr(ph), L.ptr(an)))
+
+    def set_pair_dp(self, pair_dp):
+        """PAIR_DP of the paired beats (globals.ini pair_dp): the second cilium's phase
+        offset in cycles ("bicilia") or phase factor ("bicilia_longt")."""
+        L.check(self.mob._h, L.lib().cfm_cilia_set_pair_dp(self.mob._h, float(pair_dp)))
 
     def write_outputs(self, writer, step, period=1.0):
         """One line of the reference's _seg_states.dat (positions) and _true_states.dat
@@ -227,6 +243,7 @@ class CiliatedBodies:
     set_driving = PrescribedCilia.set_driving
     set_driving_options = PrescribedCilia.set_driving_options
     set_phases = PrescribedCilia.set_phases
+    set_pair_dp = PrescribedCilia.set_pair_dp
     speeds = PrescribedCilia.speeds
     step = PrescribedCilia.step
     state = PrescribedCilia.state

@@ -161,6 +161,14 @@ static BeatCoeffs build_a_beat() {
   return b;
 }
 
+// Segments of one cilium: half the unit for the paired beats.
+static int cilium_segments(const cfm_cilia_params& prm) {
+  const bool pair = prm.beat == CFM_BEAT_BICILIA || prm.beat == CFM_BEAT_BICILIA_LONGT;
+  return pair ? prm.nseg / 2 : prm.nseg;
+}
+// The reference's FIL_LENGTH = DL (NSEG_PER_CILIA - 1) (config.hpp:579).
+static double cilium_length(const cfm_cilia_params& prm) { return prm.dl * (cilium_segments(prm) - 1); }
+
 // Tangent angle of the build-a-beat model at arc fraction s and phase psi, and its
 // phase derivative.
 __device__ inline void build_angle(double s, double psi, double* theta, double* dtheta) {
@@ -193,7 +201,12 @@ __device__ inline void build_angle(double s, double psi, double* theta, double* 
 struct CiliaK {
   BeatCoeffs cf;
   int ncil, nseg, nseg_total;
-  double dl, len;  // segment spacing, filament length dl (nseg - 1)
+  double dl, len;  // segment spacing, length dl (m - 1) of one cilium of m segments
+  // paired cilia (CFM_BEAT_BICILIA / _LONGT): each unit of nseg segments is two cilia of
+  // m = nseg / 2, the second at phase psi + 2 pi pair_dp (pair 1) or pair_dp psi (pair 2),
+  // its base moved by zdis along the frame's third axis (filament.cpp:371-375,1429-1466)
+  int pair;
+  double pair_dp, zdis;
   double s_tol;    // bisection tolerance on the arc-length fraction
   const double* s_table;  // recorded segment parameters (rows x cols) or null
   int s_rows, s_cols;
@@ -211,6 +224,26 @@ struct CiliaK {
   double* vangle;        // 3 x nseg_total
   double* svals;         // ncil x nseg
 };
+
+// Beat table, pairing, filament length and bisection tolerance of a parameter set; the
+// paired beats use the volvox fit (filament.cpp:895-960 holds the same values) and put
+// the second cilium 2.2 segment radii from the first (filament.cpp:1429-1437).
+static void beat_args(const cfm_cilia_params& prm, double pair_dp, double a_seg, CiliaK& k) {
+  switch (prm.beat) {
+    case CFM_BEAT_FULFORD_BLAKE_ORIGINAL: k.cf = fulford_blake_original(); break;
+    case CFM_BEAT_VOLVOX:
+    case CFM_BEAT_BICILIA:
+    case CFM_BEAT_BICILIA_LONGT: k.cf = volvox(); break;
+    case CFM_BEAT_CORAL_LARVAE: k.cf = coral_larvae(); break;
+    case CFM_BEAT_BUILD_A_BEAT: k.cf = build_a_beat(); break;
+    default: k.cf = fulford_blake();
+  }
+  k.pair = prm.beat == CFM_BEAT_BICILIA ? 1 : prm.beat == CFM_BEAT_BICILIA_LONGT ? 2 : 0;
+  k.pair_dp = pair_dp;
+  k.zdis = 2.2 * a_seg;
+  k.s_tol = prm.s_tol > 0.0 ? prm.s_tol : 0.1 / cilium_segments(prm);
+  k.len = cilium_length(prm);
+}
 
 // Mode vectors (cos0, cos psi, cos 2psi, cos 3psi) and (sin psi, sin 2psi, sin 3psi).
 __device__ inline void modes(double psi, double cos0, double cv[4], double sv[3]) {
@@ -263,6 +296,40 @@ __device__ double curve_length(const BeatCoeffs& cf, const double cv[4], const d
   return length;
 }
 
+// Running trapezoid sums P[j] (j = 0 .. 10 nseg) of curve_length's panels at one phase,
+// accumulated in the order of curve_length's loop (same bits); P lives in LDS.
+__device__ void curve_table(const BeatCoeffs& cf, const double cv[4], const double sv[3],
+                            int nseg, double* P) {
+  const double dl = 1.0 / (10.0 * nseg);
+  double f0 = tangent_norm(cf, cv, sv, 0.0);
+  double length = 0.0;
+  P[0] = 0.0;
+  for (int n = 1; n <= 10 * nseg; ++n) {
+    const double f1 = tangent_norm(cf, cv, sv, n * dl);
+    length += 0.5 * dl * (f0 + f1);
+    P[n] = length;
+    f0 = f1;
+  }
+}
+
+// curve_length from the table: the whole panels read from P, the last panel evaluated
+// as curve_length evaluates it.
+__device__ double curve_length_tab(const BeatCoeffs& cf, const double cv[4], const double sv[3],
+                                   double s, int nseg, const double* P) {
+  if (s <= 0.0) return 0.0;
+  const double dl = 1.0 / (10.0 * nseg);
+  const double fl = floor(s / dl);
+  if (fl > 10.0 * nseg) return curve_length(cf, cv, sv, s, nseg);  // s > 1: not tabulated
+  double length = P[(int)fl];
+  const double ce = ceil(s / dl);
+  if (ce > fl) {
+    const double f0 = tangent_norm(cf, cv, sv, (int)fl * dl);
+    const double f1 = tangent_norm(cf, cv, sv, ce * dl);
+    length += ((s / dl - fl) / (ce - fl)) * 0.5 * dl * (f0 + f1);
+  }
+  return length;
+}
+
 // Rotation matrix (row-major) of a unit quaternion (w, x, y, z) (quaternion.cpp:305-347).
 __device__ inline void quat_to_mat(const double* q, double R[9]) {
   const double s = q[0], a = q[1], b = q[2], c = q[3];
@@ -277,11 +344,13 @@ __device__ inline void quat_to_mat(const double* q, double R[9]) {
   R[8] = 1.0 - 2.0 * (a * a + b * b);
 }
 
-__global__ void cilia_geometry_kernel(CiliaK k) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= k.ncil) return;
-  const int n = k.nseg, s0 = c * n;
-  const double psi = k.phase[c];
+// Places one cilium of m segments: unit c's segments [first, first + m) (its segment
+// parameters at svals[c nseg + first ...]) at phase psi, base (bxp, byp, bzp) and frame
+// axes e1 (reference x) and e2 (reference y); `wall`: the wall frame e1 = e_z, e2 = e_y.
+__device__ __forceinline__ void place_cilium(const CiliaK& k, int c, double psi, int first, int m, double bxp,
+                             double byp, double bzp, const double e1[3], const double e2[3],
+                             bool wall, double* P) {
+  const int n = m, s0 = c * k.nseg + first;
   double cv[4], sv[3];
   modes(psi, k.cf.cos0, cv, sv);
   double s[kMaxCiliaSeg];
@@ -311,55 +380,33 @@ __global__ void cilia_geometry_kernel(CiliaK k) {
       s[i] = vj + (si - j) * (vj1 - vj);
     }
   } else {
-    // bisection on the arc-length fraction (filament.cpp:1096-1185)
-    const double total = curve_length(k.cf, cv, sv, 1.0, n);
+    // bisection on the arc-length fraction (filament.cpp:1096-1185), the panel sums
+    // tabulated once for this phase
+    curve_table(k.cf, cv, sv, n, P);
+    const double total = curve_length_tab(k.cf, cv, sv, 1.0, n, P);
     for (int i = 1; i < n - 1; ++i) {
       const double target = double(i) / double(n - 1);
       double lo = s[i - 1], hi = 1.0;
       double est = 0.5 * (lo + hi);
-      double frac = curve_length(k.cf, cv, sv, est, n) / total;
+      double frac = curve_length_tab(k.cf, cv, sv, est, n, P) / total;
       for (int it = 0; it < 200 && fabs(frac - target) > k.s_tol; ++it) {
         if (frac > target)
           hi = est;
         else
           lo = est;
         est = 0.5 * (lo + hi);
-        frac = curve_length(k.cf, cv, sv, est, n) / total;
+        frac = curve_length_tab(k.cf, cv, sv, est, n, P) / total;
       }
       s[i] = est;
     }
-  }
-  // frame of the cilium: its first two columns carry the reference x (filament axis)
-  // and y (beat direction) axes.  On the wall: reference x -> lab z, y -> lab y
-  // (filament.cpp:225-226 for dir = e_z).  On a body: R(q_body) R(q_ref) with the base
-  // at x_body + R(q_body) ref (swimmer.cpp:448-462, filament.cpp:653-663).
-  double e1[3] = {0.0, 0.0, 1.0}, e2[3] = {0.0, 1.0, 0.0};
-  double bxp, byp, bzp;
-  if (k.bx) {
-    const int b = c / k.per_body, l = c - b * k.per_body;
-    double Rb[9], Rf[9];
-    quat_to_mat(k.bq + 4 * b, Rb);
-    quat_to_mat(k.fq + 4 * l, Rf);
-    for (int r = 0; r < 3; ++r) {
-      e1[r] = Rb[3 * r] * Rf[0] + Rb[3 * r + 1] * Rf[3] + Rb[3 * r + 2] * Rf[6];
-      e2[r] = Rb[3 * r] * Rf[1] + Rb[3 * r + 1] * Rf[4] + Rb[3 * r + 2] * Rf[7];
-    }
-    const double* o = k.fref + 3 * l;
-    bxp = k.bx[3 * b] + Rb[0] * o[0] + Rb[1] * o[1] + Rb[2] * o[2];
-    byp = k.bx[3 * b + 1] + Rb[3] * o[0] + Rb[4] * o[1] + Rb[5] * o[2];
-    bzp = k.bx[3 * b + 2] + Rb[6] * o[0] + Rb[7] * o[1] + Rb[8] * o[2];
-  } else {
-    bxp = k.base[c];
-    byp = k.base[k.ncil + c];
-    bzp = k.base[2 * k.ncil + c];
   }
   const double ca = cos(k.angle[c]), sa = sin(k.angle[c]);
   const size_t ns = k.nseg_total;
   double dc[4], ds[3];
   dc[0] = 0.0;
-  for (int m = 1; m < 4; ++m) {
-    dc[m] = -m * sin(m * psi);
-    ds[m - 1] = m * cos(m * psi);
+  for (int mo = 1; mo < 4; ++mo) {
+    dc[mo] = -mo * sin(mo * psi);
+    ds[mo - 1] = mo * cos(mo * psi);
   }
   // build-a-beat: trapezoid sums of the unit tangent (sin th, cos th) and of its phase
   // derivative over the segments (filament.cpp:1401-1546), in filament lengths
@@ -374,7 +421,7 @@ __global__ void cilia_geometry_kernel(CiliaK k) {
   }
   for (int i = 0; i < n; ++i) {
     const int g = s0 + i;
-    k.svals[(size_t)c * n + i] = s[i];
+    k.svals[(size_t)c * k.nseg + first + i] = s[i];
     if (i == 0) {
       k.x[g] = bxp;
       k.y[g] = byp;
@@ -411,7 +458,7 @@ __global__ void cilia_geometry_kernel(CiliaK k) {
     // shape in the reference frame, rotated by the beat angle about its z axis
     const double rx = k.len * (ca * sx - sa * sy), ry = k.len * (sa * sx + ca * sy);
     const double ux = k.len * (ca * dx - sa * dy), uy = k.len * (sa * dx + ca * dy);
-    if (!k.bx) {  // wall frame written out: lab (0, ry, rx)
+    if (wall) {  // wall frame written out: lab (0, ry, rx)
       k.x[g] = bxp;
       k.y[g] = byp + ry;
       k.z[g] = bzp + rx;
@@ -432,6 +479,67 @@ __global__ void cilia_geometry_kernel(CiliaK k) {
       k.vphase[d * ns + g] = e1[d] * ux + e2[d] * uy;
     }
   }
+}
+
+// One thread per cilium: per unit, or per half of a pair (the second half of unit c is
+// thread 2 c + 1).  Dynamic LDS: 10 m + 1 doubles per thread (the arc-length table).
+__global__ void cilia_geometry_kernel(CiliaK k) {
+  extern __shared__ double cilia_lds[];
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int halves = k.pair ? 2 : 1;
+  const int c = t / halves, h = t - c * halves;
+  if (c >= k.ncil) return;
+  // frame of the cilium: its first two columns carry the reference x (filament axis)
+  // and y (beat direction) axes.  On the wall: reference x -> lab z, y -> lab y
+  // (filament.cpp:225-226 for dir = e_z).  On a body: R(q_body) R(q_ref) with the base
+  // at x_body + R(q_body) ref (swimmer.cpp:448-462, filament.cpp:653-663).
+  double e1[3] = {0.0, 0.0, 1.0}, e2[3] = {0.0, 1.0, 0.0};
+  double bxp, byp, bzp;
+  if (k.bx) {
+    const int b = c / k.per_body, l = c - b * k.per_body;
+    double Rb[9], Rf[9];
+    quat_to_mat(k.bq + 4 * b, Rb);
+    quat_to_mat(k.fq + 4 * l, Rf);
+    for (int r = 0; r < 3; ++r) {
+      e1[r] = Rb[3 * r] * Rf[0] + Rb[3 * r + 1] * Rf[3] + Rb[3 * r + 2] * Rf[6];
+      e2[r] = Rb[3 * r] * Rf[1] + Rb[3 * r + 1] * Rf[4] + Rb[3 * r + 2] * Rf[7];
+    }
+    const double* o = k.fref + 3 * l;
+    bxp = k.bx[3 * b] + Rb[0] * o[0] + Rb[1] * o[1] + Rb[2] * o[2];
+    byp = k.bx[3 * b + 1] + Rb[3] * o[0] + Rb[4] * o[1] + Rb[5] * o[2];
+    bzp = k.bx[3 * b + 2] + Rb[6] * o[0] + Rb[7] * o[1] + Rb[8] * o[2];
+  } else {
+    bxp = k.base[c];
+    byp = k.base[k.ncil + c];
+    bzp = k.base[2 * k.ncil + c];
+  }
+  const bool wall = !k.bx;
+  double psi = k.phase[c];
+  if (h == 1) {
+    // the second cilium of a pair at its own phase, its base zdis along the frame's third
+    // axis e1 x e2 (the reference's z axis: lab -x on the wall)
+    psi = k.pair == 1 ? psi + k.pair_dp * 2.0 * 3.14159265358979323846 : k.pair_dp * psi;
+    bxp += k.zdis * (e1[1] * e2[2] - e1[2] * e2[1]);
+    byp += k.zdis * (e1[2] * e2[0] - e1[0] * e2[2]);
+    bzp += k.zdis * (e1[0] * e2[1] - e1[1] * e2[0]);
+  }
+  const int m = k.nseg / halves;
+  place_cilium(k, c, psi, h * m, m, bxp, byp, bzp, e1, e2, wall,
+               cilia_lds + (size_t)threadIdx.x * (10 * m + 1));
+}
+
+// Launch shape of the geometry kernel: threads, threads per workgroup (the arc-length
+// tables within 64 KiB of LDS) and the LDS bytes per workgroup.
+struct GeoLaunch {
+  dim3 grid, block;
+  size_t lds;
+};
+static GeoLaunch geometry_launch(const CiliaK& k) {
+  const int threads = k.ncil * (k.pair ? 2 : 1);
+  const int m = k.pair ? k.nseg / 2 : k.nseg;
+  const size_t per = sizeof(double) * (10 * (size_t)m + 1);
+  const int tpb = (int)std::max<size_t>(1, std::min<size_t>(64, 65536 / per));
+  return {dim3((unsigned)((threads + tpb - 1) / tpb)), dim3((unsigned)tpb), per * tpb};
 }
 
 // rhs = phase_dot vel_dir_phase ([3][n_seg] system layout)
@@ -1256,6 +1364,7 @@ struct CiliaState {
   // recorded segment parameters (cfm_cilia_set_s_table)
   DevBuf<double> s_table;
   int s_rows = 0, s_cols = 0;
+  double pair_dp = 0.0;  // phase offset of the second cilium of a pair (cfm_cilia_set_pair_dp)
   // cilia carried by free bodies (cfm_cilia_init_bodies): body centres, quaternions and
   // velocities, the cilia's body-frame bases and frames, blob forces, and the block
   // preconditioner's per-cilium and per-body matrices
@@ -1286,14 +1395,7 @@ static DriveOpts drive_opts(const CiliaState* S, int nt) {
 
 static CiliaK kargs(cfm_ctx* c, CiliaState* S) {
   CiliaK k{};
-  switch (S->prm.beat) {
-    case CFM_BEAT_FULFORD_BLAKE_ORIGINAL: k.cf = fulford_blake_original(); break;
-    case CFM_BEAT_VOLVOX: k.cf = volvox(); break;
-    case CFM_BEAT_CORAL_LARVAE: k.cf = coral_larvae(); break;
-    case CFM_BEAT_BUILD_A_BEAT: k.cf = build_a_beat(); break;
-    default: k.cf = fulford_blake();
-  }
-  k.s_tol = S->prm.s_tol > 0.0 ? S->prm.s_tol : 0.1 / S->prm.nseg;
+  beat_args(S->prm, S->pair_dp, c->cfg.a_seg, k);
   k.s_table = S->s_rows > 0 ? S->s_table.p : nullptr;
   k.s_rows = S->s_rows;
   k.s_cols = S->s_cols;
@@ -1301,7 +1403,6 @@ static CiliaK kargs(cfm_ctx* c, CiliaState* S) {
   k.nseg = S->prm.nseg;
   k.nseg_total = c->p.n_seg;
   k.dl = S->prm.dl;
-  k.len = S->prm.dl * (S->prm.nseg - 1);
   k.base = S->base.p;
   if (S->n_body > 0) {
     k.bx = S->bx.p;
@@ -1324,7 +1425,9 @@ static CiliaK kargs(cfm_ctx* c, CiliaState* S) {
 static inline dim3 gc(int n) { return dim3((unsigned)((n + 63) / 64)); }
 
 static Status geometry(cfm_ctx* c, CiliaState* S) {
-  hipLaunchKernelGGL(cilia_geometry_kernel, gc(S->prm.n_cilia), dim3(64), 0, c->stream, kargs(c, S));
+  const CiliaK gk = kargs(c, S);
+  const GeoLaunch gl = geometry_launch(gk);
+  hipLaunchKernelGGL(cilia_geometry_kernel, gl.grid, gl.block, gl.lds, c->stream, gk);
   CFM_CHECK_LAUNCH();
   c->have_positions = true;
   S->have_geometry = true;
@@ -1543,7 +1646,7 @@ static Status step_bodies(cfm_ctx* c, CiliaState* S, int nt, double* q_phase, do
   if (o.precond != CFM_PC_NONE) CFM_TRY(body_cilia_prep(c, S, o.precond));
   if (S->dynamic) {
     hipLaunchKernelGGL(cilia_drive_kernel, dim3((sd.L + 255) / 256), dim3(256), 0, c->stream, nc,
-                       (int)phase_slot(c, S), S->dyn_angle ? 1 : 0, S->prm.dl * (S->prm.nseg - 1),
+                       (int)phase_slot(c, S), S->dyn_angle ? 1 : 0, cilium_length(S->prm),
                        S->spring, S->phase.p, S->angle.p, S->omega0.p, S->tab_phase.p, S->n_tab_phase,
                        S->tab_angle.p, S->n_tab_angle, drive_opts(S, nt), S->zrhs.p);
   } else {
@@ -1597,8 +1700,10 @@ extern "C" {
 static Status check_params(const cfm_ctx* c, const cfm_cilia_params* prm, const double* phase,
                            const double* phase_dot) {
   const Particles& p = c->p;
-  if (prm->beat < CFM_BEAT_FULFORD_BLAKE || prm->beat > CFM_BEAT_BUILD_A_BEAT)
+  if (prm->beat < CFM_BEAT_FULFORD_BLAKE || prm->beat > CFM_BEAT_BICILIA_LONGT)
     return make_status(CFM_EINVAL, "unknown beat %d", prm->beat);
+  if (cilium_segments(*prm) != prm->nseg && (prm->nseg % 2 != 0 || prm->nseg < 4))
+    return make_status(CFM_EINVAL, "paired beats take an even nseg >= 4 (two cilia of nseg / 2)");
   if (!(prm->s_tol >= 0.0) || prm->s_tol >= 0.5)
     return make_status(CFM_EINVAL, "s_tol must lie in [0, 0.5) (0: 0.1 / nseg)");
   if (prm->nseg < 2 || prm->nseg > kMaxCiliaSeg || prm->n_cilia <= 0 ||
@@ -1803,7 +1908,7 @@ int cfm_cilia_step(cfm_ctx* c, int32_t nt, double* q_phase, double* q_angle, cfm
       sd.extra = nc * (S->dyn_angle ? 2 : 1);
       sd.L = 3 * c->p.n_seg + sd.extra;
       hipLaunchKernelGGL(cilia_drive_kernel, dim3((sd.L + 255) / 256), dim3(256), 0, c->stream, nc,
-                         3 * c->p.n_seg, S->dyn_angle ? 1 : 0, S->prm.dl * (S->prm.nseg - 1), S->spring,
+                         3 * c->p.n_seg, S->dyn_angle ? 1 : 0, cilium_length(S->prm), S->spring,
                          S->phase.p, S->angle.p, S->omega0.p, S->tab_phase.p, S->n_tab_phase,
                          S->tab_angle.p, S->n_tab_angle, drive_opts(S, nt), S->zrhs.p);
       CFM_CHECK_LAUNCH();
@@ -1961,6 +2066,22 @@ int cfm_cilia_set_phases(cfm_ctx* c, const double* phase, const double* angle) {
   auto run = [&]() -> Status {
     if (phase) CFM_HIP(hipMemcpyAsync(S->phase.p, phase, sizeof(double) * nc, hipMemcpyHostToDevice, c->stream));
     if (angle) CFM_HIP(hipMemcpyAsync(S->angle.p, angle, sizeof(double) * nc, hipMemcpyHostToDevice, c->stream));
+    CFM_TRY(geometry(c, S));
+    CFM_HIP(hipStreamSynchronize(c->stream));
+    return {};
+  };
+  Status s = run();
+  return s.ok() ? CFM_OK : cfail(c, s);
+}
+
+int cfm_cilia_set_pair_dp(cfm_ctx* c, double pair_dp) {
+  if (!c) return CFM_EINVAL;
+  CiliaState* S = cil_of(c);
+  if (!S) return cfail(c, make_status(CFM_ESTATE, "cfm_cilia_init has not been called"));
+  if (!std::isfinite(pair_dp)) return cfail(c, make_status(CFM_EINVAL, "non-finite pair_dp"));
+  (void)hipSetDevice(c->device);
+  auto run = [&]() -> Status {
+    S->pair_dp = pair_dp;
     CFM_TRY(geometry(c, S));
     CFM_HIP(hipStreamSynchronize(c->stream));
     return {};
@@ -2699,7 +2820,8 @@ int cfm_cilia_batch_create(const cfm_cilia_batch_config* cfg, const double* base
   const cfm_cilia_params& prm = cfg->prm;
   if (cfg->n_members <= 0 || prm.n_cilia <= 0) return fail(CFM_EINVAL, "n_members and n_cilia must be positive");
   if (prm.nseg < 2 || prm.nseg > kMaxCiliaSeg) return fail(CFM_EINVAL, "nseg must be in 2..64");
-  if (prm.beat < CFM_BEAT_FULFORD_BLAKE || prm.beat > CFM_BEAT_BUILD_A_BEAT) return fail(CFM_EINVAL, "unknown beat");
+  if (prm.beat < CFM_BEAT_FULFORD_BLAKE || prm.beat > CFM_BEAT_BUILD_A_BEAT)
+    return fail(CFM_EINVAL, "unknown beat (paired beats run in single contexts)");
   if (!(prm.dl > 0.0) || !(cfg->mu > 0.0)) return fail(CFM_EINVAL, "dl and mu must be positive");
   if (prm.solve.precond != CFM_PC_DIAG && prm.solve.precond != CFM_PC_SELF)
     return fail(CFM_EINVAL, "the coupled cilia system takes CFM_PC_DIAG or CFM_PC_SELF");
@@ -2773,19 +2895,11 @@ int cfm_cilia_batch_create(const cfm_cilia_batch_config* cfg, const double* base
     // geometry of every cilium of every member (the single system's kernel over the
     // concatenated cilia)
     CiliaK& k = b->geo;
-    switch (prm.beat) {
-      case CFM_BEAT_FULFORD_BLAKE_ORIGINAL: k.cf = fulford_blake_original(); break;
-      case CFM_BEAT_VOLVOX: k.cf = volvox(); break;
-      case CFM_BEAT_CORAL_LARVAE: k.cf = coral_larvae(); break;
-      case CFM_BEAT_BUILD_A_BEAT: k.cf = build_a_beat(); break;
-      default: k.cf = fulford_blake();
-    }
-    k.s_tol = prm.s_tol > 0.0 ? prm.s_tol : 0.1 / prm.nseg;
+    beat_args(prm, 0.0, 1.0, k);
     k.ncil = (int)ncl;
     k.nseg = prm.nseg;
     k.nseg_total = (int)b->N;
     k.dl = prm.dl;
-    k.len = prm.dl * (prm.nseg - 1);
     k.base = b->base.p;
     k.phase = b->phase.p;
     k.angle = b->angle.p;
@@ -2834,7 +2948,10 @@ int cfm_cilia_batch_set_state(cfm_cilia_batch* b, const double* phase, const dou
     const CEK k = cek_of(b);
     hipLaunchKernelGGL(ce_init_guess_kernel, g256(b->L), dim3(256), 0, b->stream, k, (const double*)b->omega0.p,
                        b->sol.p);
-    hipLaunchKernelGGL(cilia_geometry_kernel, gc(b->E * b->nc), dim3(64), 0, b->stream, b->geo);
+    {
+      const GeoLaunch gl = geometry_launch(b->geo);
+      hipLaunchKernelGGL(cilia_geometry_kernel, gl.grid, gl.block, gl.lds, b->stream, b->geo);
+    }
     CFM_CHECK_LAUNCH();
     CFM_HIP(hipStreamSynchronize(b->stream));
     return {};
@@ -2857,9 +2974,12 @@ int cfm_cilia_batch_step(cfm_cilia_batch* b, int32_t nt, double* q_phase, double
       hipLaunchKernelGGL(ce_advance_kernel, g256(ncl), dim3(256), 0, b->stream, (int)ncl, b->nc, b->dyn_angle,
                          (const double*)b->dt.p, (const double*)b->pdot.p, (const double*)b->adot.p, b->phase.p,
                          b->angle.p);
-    hipLaunchKernelGGL(cilia_geometry_kernel, gc((int)ncl), dim3(64), 0, b->stream, b->geo);
+    {
+      const GeoLaunch gl = geometry_launch(b->geo);
+      hipLaunchKernelGGL(cilia_geometry_kernel, gl.grid, gl.block, gl.lds, b->stream, b->geo);
+    }
     DriveOpts d{b->cfg.spring_model, b->cfg.noise, b->cfg.seed, nt};
-    hipLaunchKernelGGL(ce_drive_kernel, g256(b->L), dim3(256), 0, b->stream, k, b->cfg.prm.dl * (b->nseg - 1),
+    hipLaunchKernelGGL(ce_drive_kernel, g256(b->L), dim3(256), 0, b->stream, k, cilium_length(b->cfg.prm),
                        b->cfg.spring, (const double*)b->phase.p, (const double*)b->angle.p,
                        (const double*)b->omega0.p, (const double*)b->tab_phase.p, b->n_tab_phase,
                        (const double*)b->tab_angle.p, b->n_tab_angle, d, b->rhs.p);

@@ -403,6 +403,15 @@ int cfm_gnn_coarse_vectors(cfm_ctx* ctx, double* zl); /* 16 x [3][n_blob] */
                                       config.hpp:133-139): segments at uniform arc fraction,
                                       trapezoid sums of the unit tangent; s_tol and the s table
                                       are unused */
+#define CFM_BEAT_BICILIA 6         /* paired cilia (SHAPE_SEQUENCE 5, deprecated in the reference):
+                                      each unit of nseg segments is two cilia of nseg / 2 beating
+                                      the volvox fit (filament.cpp:895-960), the second at phase
+                                      psi + 2 pi pair_dp with its base 2.2 a_seg along the frame's
+                                      third axis (lab -x on the wall); the unit's phase and angle
+                                      generalised forces sum over both (filament.cpp:371-375,
+                                      1096-1185, 1341-1490) */
+#define CFM_BEAT_BICILIA_LONGT 7   /* the same with the second phase pair_dp psi
+                                      (SHAPE_SEQUENCE 6) */
 
 typedef struct {
   int32_t n_cilia;
@@ -455,6 +464,12 @@ int cfm_cilia_set_s_table(cfm_ctx* ctx, const double* table, int32_t rows, int32
  * re-places the segments: a run restarted from a recorded state (the psi.dat initial
  * state the reference's drivers write, read at swimmer.cpp:43-60). */
 int cfm_cilia_set_phases(cfm_ctx* ctx, const double* phase, const double* angle);
+/* PAIR_DP of the paired beats (globals.ini "pair_dp", read at cilia_sim_main.cu:69;
+ * default 0): the second cilium's phase offset (CFM_BEAT_BICILIA, in cycles) or phase
+ * factor (CFM_BEAT_BICILIA_LONGT).  Re-places the segments.  (The reference's other pairing,
+ * PAIR 1, only multiplies the intrinsic frequency of the second half of the filaments by
+ * PAIR_DP (filament.cpp:240-244): pass such phase speeds to cfm_cilia_init.) */
+int cfm_cilia_set_pair_dp(cfm_ctx* ctx, double pair_dp);
 /* Phases, phase speeds, shape angles and their speeds (n_cilia each, may be null). */
 int cfm_cilia_get_speeds(cfm_ctx* ctx, double* phase, double* phase_dot, double* angle,
                          double* angle_dot);

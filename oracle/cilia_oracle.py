@@ -37,6 +37,12 @@ its range [theta_s, pi - theta_s], finite differences in the phase, the chord le
 two unit tangents and the derivatives of the placement in phase and angle.  The volvox
 coefficient values themselves are parity unpinned: a transcription of
 filament.cpp:748-812, checked by reading only.
+Paired cilia (pair_placement) are two single-cilium placements; the pairing is pinned by
+the reference's phase rules at the values that make the two cilia coincide in shape
+(pair_dp 0 or 1 for BICILIA, 1 for BICILIA_LONGT: the second cilium is the first moved
+by 2.2 a), by the second cilium equalling a lone volvox cilium at the second phase, and
+by the offset being 2.2 a along the wall frame's third axis, rebuilt from the
+quaternion of filament.cpp:225-226.
 """
 from __future__ import annotations
 
@@ -321,6 +327,40 @@ def placement(psi, base, nseg=20, dl=2.6, angle=0.0, beat="fulford_blake", s=Non
         vangle[n] = FRAME_ @ np.array([-ref[1], ref[0], 0.0])   # e_z x ref
         vphase[n] = FRAME_ @ (L * (Rshape @ shape_phase_derivative(s[n], psi, beat)))
     return x.reshape(-1), vphase.reshape(-1), vangle.reshape(-1), s
+
+
+# ---- paired cilia (SHAPE_SEQUENCE 5 / 6, BICILIA / BICILIA_LONGT) ---------------------
+# Each filament of NSEG segments is two cilia of NSEG_PER_CILIA = NSEG / 2
+# (cilia_sim_main.cu:110-116) beating the volvox fit (filament.cpp:895-960 holds the
+# same table as :748-812); the second cilium at phase psi2 (filament.cpp:371-375,
+# 1344-1348) is placed by fitted_shape(s, psi2, z_dis / FIL_LENGTH), whose third
+# component z_dis = 2.2 RSEG moves it along the frame's z axis (filament.cpp:990-1019,
+# 1429-1466); FIL_LENGTH, the bisection panels and tolerance use NSEG_PER_CILIA
+# (filament.cpp:1057,1145, config.hpp:579).
+PAIR_SEP = 2.2
+
+
+def pair_phase(psi, pair_dp, longt=False):
+    """Phase of the second cilium: psi + 2 pi pair_dp (BICILIA) or pair_dp psi (LONGT)."""
+    return pair_dp * psi if longt else psi + pair_dp * 2.0 * math.pi
+
+
+def pair_placement(psi, base, nseg=20, dl=2.6, angle=0.0, pair_dp=0.0, longt=False, a=1.0,
+                   frame=None, s=None):
+    """Positions, phase and angle velocity directions (AoS, 3 nseg) and segment parameters
+    (nseg) of one pair of cilia of nseg // 2 segments: the first at phase psi from base,
+    the second at pair_phase(psi) from base + PAIR_SEP a frame[:, 2]; s: both cilia's
+    parameters (nseg), by bisection when None."""
+    m = nseg // 2
+    F = FRAME if frame is None else np.asarray(frame, dtype=np.float64)
+    base = np.asarray(base, dtype=np.float64)
+    s1 = None if s is None else np.asarray(s)[:m]
+    s2 = None if s is None else np.asarray(s)[m:]
+    x1, vp1, va1, s1 = placement(psi, base, m, dl, angle, "volvox", s=s1, frame=F)
+    x2, vp2, va2, s2 = placement(pair_phase(psi, pair_dp, longt), base + PAIR_SEP * a * F[:, 2],
+                                 m, dl, angle, "volvox", s=s2, frame=F)
+    return (np.concatenate([x1, x2]), np.concatenate([vp1, vp2]), np.concatenate([va1, va2]),
+            np.concatenate([s1, s2]))
 
 
 def generalised_forces(mobility, psi, phase_dot=2.0 * math.pi, base=(0.0, 0.0, 1.3), nseg=20,

@@ -90,6 +90,7 @@ class Swimmers:
         self.V = np.zeros((self.nbody, 6))
         self.nseg, self.dl, self.dt, self.beat, self.s_tol = nseg, dl, dt, beat, s_tol
         self.mu, self.a = mu, a
+        self.pair_dp = 0.0    # PAIR_DP of the paired beats (BICILIA, BICILIA_LONGT)
         self.driving = None   # (q_phase_table, q_angle_table or None, spring)
         self.fene, self.noise, self.seed, self.nt = False, 0.0, 0, 0
         # BODY_VELOCITY_TYPE 2: Omega_b held at zero, the Omega columns and torque rows
@@ -106,9 +107,19 @@ class Swimmers:
                 c = b * self.per + l
                 R = Rb @ rot_mat(self.fil_q[l])
                 base = self.x[b] + Rb @ self.fil_ref[l]
-                s = co.s_values(self.phase[c], self.nseg, self.beat, tol=self.s_tol)
-                x, p, a, _ = co.placement(self.phase[c], base, self.nseg, self.dl, self.angle[c],
-                                          self.beat, s=s, frame=R)
+                if self.beat in ("bicilia", "bicilia_longt"):   # pairs of nseg / 2
+                    longt = self.beat == "bicilia_longt"
+                    m = self.nseg // 2
+                    psi2 = co.pair_phase(self.phase[c], self.pair_dp, longt)
+                    s = np.concatenate([co.s_values(self.phase[c], m, "volvox", tol=self.s_tol),
+                                        co.s_values(psi2, m, "volvox", tol=self.s_tol)])
+                    x, p, a, _ = co.pair_placement(self.phase[c], base, self.nseg, self.dl,
+                                                   self.angle[c], self.pair_dp, longt, self.a,
+                                                   frame=R, s=s)
+                else:
+                    s = co.s_values(self.phase[c], self.nseg, self.beat, tol=self.s_tol)
+                    x, p, a, _ = co.placement(self.phase[c], base, self.nseg, self.dl,
+                                              self.angle[c], self.beat, s=s, frame=R)
                 xs.append(x.reshape(-1, 3))
                 vp.append(p)
                 va.append(a)

@@ -287,3 +287,52 @@ def test_build_a_beat_tip_sweeps_the_amplitude():
     assert tips[:, 1].max() <= y[0] + 1e-12 and tips[:, 1].min() >= y[1] - 1e-12
     assert tips[:, 1].max() - tips[:, 1].min() > 0.8 * co.BUILD_AMPLITUDE * L
     assert np.all(np.linalg.norm(tips, axis=1) <= L + 1e-12)
+
+
+def _wall_frame_from_quaternion():
+    """The wall cilium's frame rebuilt from filament.cpp:225-226 for dir = e_z: the square
+    root of the quaternion (0, 0, -1, 0) (quaternion.cpp:197-223), as a rotation matrix."""
+    w, v = 0.0, np.array([0.0, -1.0, 0.0])
+    w = math.sqrt(0.5 * (1.0 + w))
+    v = v / (2.0 * w)
+    n = math.sqrt(w * w + v @ v)
+    w, (a, b, c) = w / n, v / n
+    return np.array([[1 - 2 * (b * b + c * c), 2 * (a * b - w * c), 2 * (a * c + w * b)],
+                     [2 * (a * b + w * c), 1 - 2 * (a * a + c * c), 2 * (b * c - w * a)],
+                     [2 * (a * c - w * b), 2 * (b * c + w * a), 1 - 2 * (a * a + b * b)]])
+
+
+@pytest.mark.parametrize("pair_dp,longt", [(0.0, False), (1.0, False), (1.0, True)])
+def test_pair_whose_phases_coincide_is_one_cilium_moved_by_the_separation(pair_dp, longt):
+    """BICILIA at pair_dp 0 or 1 and BICILIA_LONGT at 1 give the second cilium the first
+    one's phase (modulo 2 pi): its segments are the first cilium's moved by 2.2 a along
+    the wall frame's third axis (lab -x), with the same velocity directions."""
+    F = _wall_frame_from_quaternion()
+    assert np.allclose(F, co.FRAME, atol=1e-15)
+    assert np.allclose(F[:, 2], [-1.0, 0.0, 0.0], atol=1e-15)
+    base, a = np.array([3.0, -2.0, 1.3]), 1.25
+    x, vp, va, s = co.pair_placement(1.7, base, nseg=16, angle=0.4, pair_dp=pair_dp,
+                                     longt=longt, a=a)
+    p, vp, va = x.reshape(2, 8, 3), vp.reshape(2, 8, 3), va.reshape(2, 8, 3)
+    assert np.allclose(p[1] - p[0], 2.2 * a * np.array([-1.0, 0.0, 0.0]), atol=1e-12)
+    assert np.allclose(vp[1], vp[0], atol=1e-12) and np.allclose(va[1], va[0], atol=1e-12)
+    assert np.allclose(s[8:], s[:8], atol=1e-12)
+    # each cilium is one volvox cilium of nseg / 2 segments (panels and tolerance of 8)
+    x1, _, _, s1 = co.placement(1.7, base, 8, 2.6, 0.4, "volvox")
+    assert np.array_equal(s[:8], s1) and np.array_equal(x[:24], x1)
+
+
+@pytest.mark.parametrize("longt", [False, True])
+def test_second_cilium_is_a_lone_cilium_at_the_second_phase(longt):
+    psi, dp = 2.3, 0.37
+    psi2 = dp * psi if longt else psi + 2.0 * math.pi * dp
+    assert co.pair_phase(psi, dp, longt) == psi2
+    base = np.array([0.0, 5.0, 1.3])
+    x, vp, va, s = co.pair_placement(psi, base, nseg=20, pair_dp=dp, longt=longt)
+    x2, vp2, va2, s2 = co.placement(psi2, base + np.array([-2.2, 0.0, 0.0]), 10, 2.6, 0.0, "volvox")
+    assert np.array_equal(s[10:], s2)
+    assert np.allclose(x[30:], x2, atol=1e-13) and np.allclose(vp[30:], vp2, atol=1e-13)
+    assert np.allclose(va[30:], va2, atol=1e-13)
+    # a planted error: the second cilium at the first cilium's phase is far off
+    x_bad, _, _, _ = co.placement(psi, base + np.array([-2.2, 0.0, 0.0]), 10, 2.6, 0.0, "volvox")
+    assert np.abs(x_bad - x[30:]).max() > 1.0
