@@ -25,10 +25,14 @@ tolerance, and the blob rows and the force and torque balance hold at convergenc
 (tests/test_broyden_only_oracle.py); the elimination in jinv_mult is pinned against a
 dense solve of the assembled block matrix, and the blob terms of the body's Schur
 complement against the elimination of the blob rows.  The remaining rotation-rotation
-terms of the body block (the blob torques' dependence on the orientation, the
-filaments' tether, clamp and end-force terms, swimmer.cpp:582-621) are an approximation
-that changes only how fast Broyden converges: restated from the reference, parity
-unpinned for those terms.
+terms of the body block (swimmer.cpp:582-621) are the reference's approximation of the
+Jacobian, which changes only how fast Broyden converges.  The blob-force, tether and
+end-force terms are each the exact derivative of a torque (exp(delta) a) x f with the
+force held, and the clamp term the derivative of the moment map dexpinv^T(u) lam through
+second order in u: both pinned by finite differences (torque_rotation_block, with the
+factors' order checked; clamp_block, with the error shrinking as |u|^3).  What is
+approximate is where the reference evaluates them (the step's initial orientation, the
+forces held), which moves Broyden's iterates but not the converged state, pinned above.
 """
 from __future__ import annotations
 
@@ -62,6 +66,20 @@ def body_velocity(x, xm1, xm2, u, um1, dt, nt, num_euler):
     V = -(x + c1 * xm1 + c2 * xm2) / (c3 * dt)
     w = (c2 * um1 - u) / (c3 * dt)
     return V, dexp(u, w)
+
+
+def torque_rotation_block(f, a):
+    """d/d(delta) of (exp(delta) a) x f at delta = 0 with f held: rcross(f) rcross(a).  The
+    body block's blob-force, tether and end-force terms (swimmer.cpp:588-603,607,614-621)
+    all have this form."""
+    return rcross(f) @ rcross(a)
+
+
+def clamp_block(u, lam):
+    """The clamp moment's term of the body block (swimmer.cpp:609-612): 0.5 rcross(lam)
+    plus the second-order correction (rcross(u x lam) - rcross(u) rcross(lam)) / 12 -- the
+    derivative of the moment map dexpinv^T(u) lam through second order in u."""
+    return 0.5 * rcross(lam) + (rcross(np.cross(u, lam)) - rcross(u) @ rcross(lam)) / 12.0
 
 
 class FreeSwimmer(Swimmer):
@@ -119,18 +137,17 @@ class FreeSwimmer(Swimmer):
         S[:3, :3] = -nb * bmf * np.eye(3)
         for m in range(nb):
             Rc = rcross(Q @ b.ref[m])
-            S[3:, 3:] += rcross(self.fb[m]) @ Rc
+            S[3:, 3:] += torque_rotation_block(self.fb[m], Q @ b.ref[m])
             S[:3, 3:] -= bmf * Rc
             S[3:, :3] += bmf * Rc
             S[3:, 3:] += bmf * Rc @ Rc
         self.jinv, self.JB, self.A, self.D, self.Ja = [], [], [], [], []
         for i, f in enumerate(self.fils):
             d = Q @ self.fref[i]
-            S[3:, 3:] += rcross(f.lam[0]) @ rcross(d)
-            uxl = np.cross(u, f.clamp)
-            S[3:, 3:] += 0.5 * rcross(f.clamp) + (rcross(uxl) - rcross(u) @ rcross(f.clamp)) / 12.0
+            S[3:, 3:] += torque_rotation_block(f.lam[0], d)
+            S[3:, 3:] += clamp_block(u, f.clamp)
             t0, tl = tangent(f.q[0]), tangent(f.q[n - 1])
-            S[3:, 3:] += rcross(-P.end_force * tl) @ rcross(0.5 * P.dl * t0)
+            S[3:, 3:] += torque_rotation_block(-P.end_force * tl, 0.5 * P.dl * t0)
             Dm = np.zeros((6, 6 * n))
             Dm[0:3, 0:3] = np.eye(3)
             Dm[3:6, 0:3] = rcross(-d)

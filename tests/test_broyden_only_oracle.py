@@ -215,3 +215,41 @@ def test_fixed_blobs_reach_the_gmres_variant_state():
     F[:, :3] += fo.barrier_forces(xseg, P.nseg, P, xb)
     _, vb = vel_full(xseg, F, S.fb)
     assert np.abs(vb).max() <= P.tol
+
+
+def test_body_block_torque_terms_are_derivatives_of_their_torques():
+    """The blob-force, tether and end-force terms of the body block are
+    d/d(delta) [(exp(delta) a) x f] at delta = 0 with f held (central differences, 1e-8);
+    a swapped product rcross(a) rcross(f) fails.  The clamp term is the derivative of the
+    moment map dexpinv^T(u) lam through second order in u: the difference shrinks as
+    |u|^3, while the first-order part alone (or a sign error in the correction) leaves an
+    O(|u|) difference."""
+    rng = np.random.default_rng(4)
+    h = 1e-6
+
+    def fd(fun, x0):
+        J = np.zeros((3, 3))
+        for k in range(3):
+            e = np.zeros(3)
+            e[k] = h
+            J[:, k] = (fun(x0 + e) - fun(x0 - e)) / (2 * h)
+        return J
+
+    for _ in range(5):
+        a, f = rng.standard_normal(3), rng.standard_normal(3)
+        J = bx.torque_rotation_block(f, a)
+        ref = fd(lambda d: np.cross(fo.rot_mat(fo.lie_exp(d)) @ a, f), np.zeros(3))
+        assert np.abs(J - ref).max() <= 1e-8 * max(1.0, np.abs(ref).max())
+        assert np.abs(fo.rcross(a) @ fo.rcross(f) - ref).max() > 1e-3
+        lam, dirn = rng.standard_normal(3), rng.standard_normal(3)
+        dirn /= np.linalg.norm(dirn)
+        errs, first = [], []
+        for s in (0.1, 0.05, 0.025):
+            u = s * dirn
+            ex = fd(lambda v: fo.dexpinv_transpose(v, lam), u)
+            errs.append(np.abs(bx.clamp_block(u, lam) - ex).max())
+            first.append(np.abs(0.5 * fo.rcross(lam) - ex).max())
+            wrong = 0.5 * fo.rcross(lam) - (fo.rcross(np.cross(u, lam)) - fo.rcross(u) @ fo.rcross(lam)) / 12.0
+            assert np.abs(wrong - ex).max() > 10 * errs[-1]
+        assert errs[0] / errs[2] > 30 and errs[2] < 1e-5 * np.abs(lam).max()   # O(|u|^3)
+        assert first[0] / first[2] < 5                                         # O(|u|)
