@@ -34,9 +34,9 @@ The other beats have no reference output: their shape, tangent and phase directi
 agree by finite differences, the coral-larvae table has the unit period-mean length its
 source states, and the build-a-beat angle is pinned by continuity at its switch points,
 its range [theta_s, pi - theta_s], finite differences in the phase, the chord length of
-two unit tangents and the derivatives of the placement in phase and angle.  The volvox
-coefficient values themselves are parity unpinned: a transcription of
-filament.cpp:748-812, checked by reading only.
+two unit tangents and the derivatives of the placement in phase and angle.  Every
+coefficient table equals, value for value, the assignments of filament.cpp:684-960 as
+tests/golden/extract_beat_tables.py reads them into tests/golden/beat_tables.json.
 Paired cilia (pair_placement) are two single-cilium placements; the pairing is pinned by
 the reference's phase rules at the values that make the two cilia coincide in shape
 (pair_dp 0 or 1 for BICILIA, 1 for BICILIA_LONGT: the second cilium is the first moved

@@ -336,3 +336,18 @@ def test_second_cilium_is_a_lone_cilium_at_the_second_phase(longt):
     # a planted error: the second cilium at the first cilium's phase is far off
     x_bad, _, _, _ = co.placement(psi, base + np.array([-2.2, 0.0, 0.0]), 10, 2.6, 0.0, "volvox")
     assert np.abs(x_bad - x[30:]).max() > 1.0
+
+
+@pytest.mark.parametrize("beat", ["fulford_blake", "fulford_blake_original", "volvox",
+                                  "coral_larvae", "bicilia"])
+def test_oracle_tables_equal_the_reference_assignments(beat):
+    """Every coefficient of the oracle's hand-typed beat tables equals the value the
+    reference assigns (filament.cpp:684-960), read mechanically into
+    tests/golden/beat_tables.json by tests/golden/extract_beat_tables.py; the paired
+    beats' table is the volvox table."""
+    import json
+    with open(os.path.join(GOLD, "beat_tables.json")) as fh:
+        ref = json.load(fh)[beat]
+    ax, ay, bx, by, _ = co.BEATS["volvox" if beat == "bicilia" else beat]
+    for name, mine in (("Ax", ax), ("Ay", ay), ("Bx", bx), ("By", by)):
+        assert np.array_equal(np.asarray(ref[name]), mine), (beat, name)
