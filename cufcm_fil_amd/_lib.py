@@ -322,6 +322,7 @@ def lib():
     L.cfm_profile.argtypes = [_vp, ctypes.c_int32]
     L.cfm_profile_read.argtypes = [_vp, ctypes.POINTER(cfm_profile_stats)]
     L.cfm_set_pair_symmetry.argtypes = [_vp, ctypes.c_int32]
+    L.cfm_cilia_batch_set_pair_dp.argtypes = [_vp, ctypes.c_double]
     L.cfm_cilia_batch_create.argtypes = [ctypes.POINTER(cfm_cilia_batch_config), _dp, _dp, ctypes.c_int32,
                                          _dp, ctypes.c_int32, ctypes.POINTER(_vp)]
     L.cfm_cilia_batch_destroy.argtypes = [_vp]

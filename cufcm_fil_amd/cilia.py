@@ -316,6 +316,10 @@ class CiliaEnsemble:
     def __del__(self):
         self.close()
 
+    def set_pair_dp(self, pair_dp):
+        """PAIR_DP of the paired beats, for every member."""
+        L.check_cilia_batch(self._h, L.lib().cfm_cilia_batch_set_pair_dp(self._h, float(pair_dp)))
+
     def set_state(self, phase, omega0, dt, angle=None):
         """phase, omega0, angle: (n_members, n_cilia); dt: n_members time steps."""
         n = self.n_members * self.n_cilia

@@ -2820,8 +2820,9 @@ int cfm_cilia_batch_create(const cfm_cilia_batch_config* cfg, const double* base
   const cfm_cilia_params& prm = cfg->prm;
   if (cfg->n_members <= 0 || prm.n_cilia <= 0) return fail(CFM_EINVAL, "n_members and n_cilia must be positive");
   if (prm.nseg < 2 || prm.nseg > kMaxCiliaSeg) return fail(CFM_EINVAL, "nseg must be in 2..64");
-  if (prm.beat < CFM_BEAT_FULFORD_BLAKE || prm.beat > CFM_BEAT_BUILD_A_BEAT)
-    return fail(CFM_EINVAL, "unknown beat (paired beats run in single contexts)");
+  if (prm.beat < CFM_BEAT_FULFORD_BLAKE || prm.beat > CFM_BEAT_BICILIA_LONGT) return fail(CFM_EINVAL, "unknown beat");
+  if (cilium_segments(prm) != prm.nseg && (prm.nseg % 2 != 0 || prm.nseg < 4))
+    return fail(CFM_EINVAL, "paired beats take an even nseg >= 4 (two cilia of nseg / 2)");
   if (!(prm.dl > 0.0) || !(cfg->mu > 0.0)) return fail(CFM_EINVAL, "dl and mu must be positive");
   if (prm.solve.precond != CFM_PC_DIAG && prm.solve.precond != CFM_PC_SELF)
     return fail(CFM_EINVAL, "the coupled cilia system takes CFM_PC_DIAG or CFM_PC_SELF");
@@ -2960,6 +2961,23 @@ int cfm_cilia_batch_set_state(cfm_cilia_batch* b, const double* phase, const dou
   if (!s.ok()) return cefail(b, s);
   b->have_state = true;
   return CFM_OK;
+}
+
+int cfm_cilia_batch_set_pair_dp(cfm_cilia_batch* b, double pair_dp) {
+  if (!b) return CFM_EINVAL;
+  if (!std::isfinite(pair_dp)) return cefail(b, make_status(CFM_EINVAL, "non-finite pair_dp"));
+  (void)hipSetDevice(b->cfg.device);
+  b->geo.pair_dp = pair_dp;
+  if (!b->have_state) return CFM_OK;  // placed by the next cfm_cilia_batch_set_state
+  auto run = [&]() -> Status {
+    const GeoLaunch gl = geometry_launch(b->geo);
+    hipLaunchKernelGGL(cilia_geometry_kernel, gl.grid, gl.block, gl.lds, b->stream, b->geo);
+    CFM_CHECK_LAUNCH();
+    CFM_HIP(hipStreamSynchronize(b->stream));
+    return {};
+  };
+  Status s = run();
+  return s.ok() ? CFM_OK : cefail(b, s);
 }
 
 int cfm_cilia_batch_step(cfm_cilia_batch* b, int32_t nt, double* q_phase, double* q_angle,

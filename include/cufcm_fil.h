@@ -683,6 +683,9 @@ const char* cfm_cilia_batch_last_error(const cfm_cilia_batch* b);
  * natural frequencies) and places the cilia. */
 int cfm_cilia_batch_set_state(cfm_cilia_batch* b, const double* phase, const double* omega0,
                               const double* angle, const double* dt);
+/* PAIR_DP of the paired beats for every member (cfm_cilia_set_pair_dp; default 0);
+ * re-places the cilia when the members have a state. */
+int cfm_cilia_batch_set_pair_dp(cfm_cilia_batch* b, double pair_dp);
 /* One step nt of every member; q_phase, q_angle (n_members x n_cilia, may be null)
  * receive the generalised forces, per_member (n_members, may be null) the solve
  * statistics (reserved0 = 1 when the member's guess was rejected). */

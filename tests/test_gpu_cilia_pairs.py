@@ -97,3 +97,40 @@ def test_pair_beats_take_an_even_number_of_segments():
         rc = L.lib().cfm_cilia_init(m._h, ctypes.byref(prm), L.ptr(b), L.ptr(p), L.ptr(w))
         assert rc == L.CFM_EINVAL
         assert b"even nseg" in L.lib().cfm_last_error(m._h)
+
+
+def test_batched_pairs_match_single_contexts():
+    """Two members of a 2 x 2 lattice of dynamic-phase pairs (two 10-segment cilia each)
+    stepped together by the ensemble: each member's positions, phases and phase speeds
+    after two steps against the same member in its own context."""
+    import os
+    from cufcm_fil_amd.cilia import CiliaEnsemble
+    gold = os.path.join(os.path.dirname(__file__), "golden")
+    qp_tab = np.loadtxt(os.path.join(gold, "fb_phase_generalised_forces_nseg20_sep2.6.dat"))[1:]
+    base = wall_lattice(2, 2, spacing=30.0)
+    n = base.size // 3
+    rng = np.random.default_rng(12)
+    phase = rng.uniform(0.0, 2.0 * math.pi, (2, n))
+    omega = np.full((2, n), 2.0 * math.pi)
+    with CiliaEnsemble(base, 2, qp_tab, beat="bicilia", precond="diag", rtol=1e-11,
+                       s_tol=1e-12) as E:
+        E.set_pair_dp(0.3)
+        E.set_state(phase, omega, np.full(2, 1.0 / 40))
+        for nt in range(2):
+            E.step(nt)
+        xe = E.positions()
+        st_e = E.speeds()
+    for m in range(2):
+        with PrescribedCilia(base, phase[m], omega[m], dt=1.0 / 40, beat="bicilia",
+                             precond="diag", rtol=1e-11, s_tol=1e-12) as C:
+            C.set_pair_dp(0.3)
+            C.set_driving(qp_tab)
+            for nt in range(2):
+                C.step(nt)
+            x1 = C.state()["x"]
+            sp = C.speeds()
+        # both solve to rtol 1e-11 with different reduction orders: speeds agree to the
+        # solve tolerance times the conditioning, positions through the phases they move
+        assert np.abs(xe[m].reshape(-1) - x1).max() <= 1e-7, m
+        assert np.allclose(st_e["phase_dot"][m], sp["phase_dot"], rtol=1e-7, atol=0), m
+        assert np.allclose(st_e["phase"][m], sp["phase"], rtol=0, atol=1e-9), m
