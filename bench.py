@@ -480,6 +480,92 @@ def run_orbit_ensemble(args):
     }
 
 
+def run_filament_ensemble(args):
+    """Ensembles of elastic-filament runs (parameter sweeps launch one executable per run in
+    the reference, pyfile/driver/func.py:544): 16 members, each the 64 x 20-segment wall
+    lattice of tilted filaments (8 x 8, spacing 10a, tilt <= 0.3 rad, its own seed and tip
+    perturbation), FP32C pair sums; one warm-up step, then 4 timed implicit steps of every
+    member.  Batched: one context with cfm_fil_set_members (one product launch for all
+    members, a Broyden iteration per member); one member at a time in its own context;
+    concurrent: one context and stream per member on 8 host threads.  Every member's state
+    is compared with its lone run.  Only the timed steps are timed (contexts and warm-up
+    steps are made first; the concurrent members start together, eight at a time)."""
+    import numpy as np
+    from concurrent.futures import ThreadPoolExecutor
+
+    from cufcm_fil_amd import geometry as geo
+    from cufcm_fil_amd.filaments import DEFAULTS, FilamentSystem, end_force
+    E, steps, nseg = 16, 4, 20
+    f_end = end_force(DEFAULTS["dimensionless_force"], DEFAULTS["kb"], DEFAULTS["dl"], nseg)
+    xs, ts, ps = [], [], []
+    for e in range(E):
+        x, t = geo.tilted_lattice(nx=8, ny=8, spacing=10.0, nseg=nseg, seed=200 + e,
+                                  return_tangents=True, min_sep=2.5)
+        rng = np.random.default_rng(300 + e)
+        xs.append(x)
+        ts.append(np.asarray(t).reshape(-1, 3))
+        ps.append((0.02 + 0.01 * (e % 4)) * f_end * rng.standard_normal(3 * 64))
+
+    def lone(e, barrier=None):
+        """Member e alone: (start, end) of its timed steps, iterations, positions; with a
+        barrier, the timed steps start once every thread has its context and warm-up step."""
+        with FilamentSystem(xs[e], ts[e], pert=ps[e], precision="fp32c") as S:
+            S.step(0)
+            if barrier is not None:
+                barrier.wait()
+            t0 = time.perf_counter()
+            its = [S.step(nt)["broyden_iterations"] for nt in range(1, steps + 1)]
+            return t0, time.perf_counter(), its, S.state()["x"]
+
+    lone(0)   # warm-up: library and first context
+    t_seq, its_seq, x_seq = 0.0, [], []
+    for e in range(E):
+        a, b, its, x = lone(e)
+        t_seq += b - a
+        its_seq.append(its)
+        x_seq.append(x)
+    import threading
+    conc = []
+    for g in range(0, E, 8):   # eight members at a time, one context and stream each
+        bar = threading.Barrier(8)
+        with ThreadPoolExecutor(8) as pool:
+            conc += list(pool.map(lambda e: lone(e, bar), range(g, g + 8)))
+    t_conc_wall = sum(max(r[1] for r in conc[g:g + 8]) - min(r[0] for r in conc[g:g + 8])
+                      for g in range(0, E, 8))
+    with FilamentSystem(np.concatenate(xs), np.concatenate(ts), pert=np.concatenate(ps),
+                        precision="fp32c") as S:
+        S.set_members(E)
+        S.step(0)
+        t0 = time.perf_counter()
+        its_bat = []
+        for nt in range(1, steps + 1):
+            S.step(nt)
+            its_bat.append([m["broyden_iterations"] for m in S.member_stats()])
+        t_bat = time.perf_counter() - t0
+        x_bat = S.state()["x"].reshape(E, -1)
+    same_its = all([its_bat[k][e] for k in range(steps)] == its_seq[e] for e in range(E))
+    diff = max(float(np.abs(x_bat[e] - x_seq[e]).max()) for e in range(E))
+    diff_c = max(float(np.abs(conc[e][3] - x_seq[e]).max()) for e in range(E))
+    member_steps = E * steps
+    print(f"[bench] filament ensemble: batched {t_bat:.3f}s, concurrent {t_conc_wall:.3f}s, "
+          f"one at a time {t_seq:.3f}s", file=sys.stderr, flush=True)
+    return {
+        "workload": "16 members, each 64 x 20-segment tilted filaments on an 8 x 8 wall lattice "
+                    "(spacing 10a), FP32C pair sums, 4 implicit steps per member after one "
+                    "warm-up step",
+        "members": E,
+        "max_broyden_iterations_per_step": [max(its_bat[k]) for k in range(steps)],
+        "batched_member_steps_per_s": round(member_steps / t_bat, 2),
+        "concurrent_member_steps_per_s": round(member_steps / t_conc_wall, 2),
+        "sequential_member_steps_per_s": round(member_steps / t_seq, 2),
+        "batched_speedup_vs_sequential": round(t_seq / t_bat, 3),
+        "batched_speedup_vs_concurrent": round(t_conc_wall / t_bat, 3),
+        "same_broyden_iterations_as_lone_runs": bool(same_its),
+        "max_abs_diff_batched_vs_lone": diff,
+        "max_abs_diff_concurrent_vs_lone": diff_c,
+    }
+
+
 def run_ciliated_swimmer(args):
     """Prescribed cilia on a free body (the reference's PRESCRIBED_CILIA swimmers): a
     sphere of 6,000 blobs (R = 40a) carrying 128 Fulford-Blake cilia of 20 segments at
@@ -580,13 +666,14 @@ def main():
     ap.add_argument("--skip-swimmer", action="store_true")
     ap.add_argument("--skip-ciliated", action="store_true")
     ap.add_argument("--skip-ensemble", action="store_true")
+    ap.add_argument("--skip-filament-ensemble", action="store_true")
     ap.add_argument("--skip-overlap", action="store_true")
     ap.add_argument("--no-pair-symmetry", action="store_true",
                     help="one-directional pair kernel for the segment-segment product")
     ap.add_argument("--symmetry-ab", action="store_true",
                     help="also time the fast path with the one-directional pair kernel")
     ap.add_argument("--only", choices=["batched", "learned", "cilia", "body", "swimmer", "ciliated",
-                                       "ensemble"],
+                                       "ensemble", "filaments"],
                     help="run one side section alone and print its JSON (for profiling)")
     ap.add_argument("--batch-reps", type=int, default=3)
     ap.add_argument("--learned-runs", type=int, default=1,
@@ -637,7 +724,7 @@ def main():
     if args.only:
         fn = {"batched": run_batched, "learned": run_learned, "cilia": run_cilia, "body": run_free_body,
               "swimmer": run_free_swimmer, "ciliated": run_ciliated_swimmer,
-              "ensemble": run_orbit_ensemble}[args.only]
+              "ensemble": run_orbit_ensemble, "filaments": run_filament_ensemble}[args.only]
         print(json.dumps({args.only: fn(args)}), flush=True)
         return
     guess = "learned" if args.learned else "projected"
@@ -698,6 +785,7 @@ def main():
     swimmer = None if args.skip_swimmer else run_free_swimmer(args)
     ciliated = None if args.skip_ciliated else run_ciliated_swimmer(args)
     ensemble = None if args.skip_ensemble else run_orbit_ensemble(args)
+    fil_ensemble = None if args.skip_filament_ensemble else run_filament_ensemble(args)
     cpu = None
     if not args.skip_cpu:
         ref = plain if plain else mixed
@@ -737,6 +825,7 @@ def main():
         "free_swimmer": swimmer,
         "ciliated_swimmer": ciliated,
         "orbit_ensemble": ensemble,
+        "filament_ensemble": fil_ensemble,
         "overlap": ovl,
         "one_directional_pair_kernel": sym_ab,
     }

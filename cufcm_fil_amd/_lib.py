@@ -295,6 +295,8 @@ def lib():
     L.cfm_gnn_train_coarse.argtypes = [_vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_double, _dp, _dp]
     L.cfm_gnn_coarse_gradient.argtypes = [_vp, ctypes.c_int32, ctypes.c_int32, _fp, _dp, _dp]
     L.cfm_gnn_coarse_vectors.argtypes = [_vp, _dp]
+    L.cfm_fil_set_members.argtypes = [_vp, ctypes.c_int32]
+    L.cfm_fil_member_stats.argtypes = [_vp, ctypes.POINTER(cfm_step_stats)]
     L.cfm_cilia_init.argtypes = [_vp, ctypes.POINTER(cfm_cilia_params), _dp, _dp, _dp]
     L.cfm_cilia_step.argtypes = [_vp, ctypes.c_int32, _dp, _dp, ctypes.POINTER(cfm_solve_stats)]
     L.cfm_cilia_get_state.argtypes = [_vp, _dp, _dp, _dp, _dp, _dp, _dp]

@@ -479,31 +479,33 @@ static Status launch_sym_t(cfm_ctx* c, const ApplyPlan& plan, int base, int cnt,
   return {};
 }
 
-// Ensembles of equal members (wall, 3 DOF, FP64): member e's particles sit at
-// [e ns, (e+1) ns) of the position arrays and its loads / velocities at f + e mvec,
-// v + e mvec with component stride ns; members whose status differs from `want` are
-// skipped.  Velocities are overwritten (scaled by 1/mu, self term included).
-Status sym_members_apply(hipStream_t stream, const double* x, const double* y, const double* z, int ns,
-                         int members, const double* f, size_t mvec, double* v, double mu,
-                         const int* status, int sstride, int want, DevBuf<double>& own,
-                         DevBuf<double>& src) {
+// Ensembles of equal members (wall): member e's particles sit at [e ns, (e+1) ns) of the
+// position arrays and its loads / velocities at f + e mvec, v + e mvec with component
+// strides fld, vld; members whose status differs from `want` are skipped (status may be
+// null).  Velocities are overwritten (scaled by 1/mu, self term included).  Each member
+// is decomposed as a lone system of ns particles would be, so its sums are the same.
+template <typename T, int D>
+static Status sym_members_t(hipStream_t stream, const double* x, const double* y, const double* z,
+                            int ns, int members, const double* f, size_t fld, size_t mvec,
+                            double* v, size_t vld, double mu, const int* status, int sstride,
+                            int want, DevBuf<double>& own, DevBuf<T>& src) {
   if (ns <= 0 || members <= 0) return {};
-  SymArgs<double> a{};
+  SymArgs<T> a{};
   a.x = x;
   a.y = y;
   a.z = z;
   a.f = f;
   a.v = v;
-  a.fld = ns;
-  a.vld = ns;
+  a.fld = fld;
+  a.vld = vld;
   a.acc = 0;
   a.mu = mu;
   a.cnt = ns;
   sym_shape(ns, &a.nblk, &a.Dn, &a.S, &a.dpw);
   a.mpos = ns;
   a.mvec = mvec;
-  a.mown = (size_t)a.nblk * a.S * 3 * kBlk;
-  a.msrc = (size_t)a.nblk * std::max(a.Dn, 1) * 3 * kBlk;
+  a.mown = (size_t)a.nblk * a.S * D * kBlk;
+  a.msrc = (size_t)a.nblk * std::max(a.Dn, 1) * D * kBlk;
   a.status = status;
   a.sstride = sstride;
   a.want = want;
@@ -511,11 +513,43 @@ Status sym_members_apply(hipStream_t stream, const double* x, const double* y, c
   CFM_TRY(src.alloc(a.msrc * members));
   a.own = own.p;
   a.src = src.p;
-  hipLaunchKernelGGL((rpy_sym_kernel<double, 3>), dim3(a.nblk, a.S, members), dim3(256), 0, stream, a);
+  hipLaunchKernelGGL((rpy_sym_kernel<T, D>), dim3(a.nblk, a.S, members), dim3(256), 0, stream, a);
   CFM_CHECK_LAUNCH();
-  hipLaunchKernelGGL((rpy_sym_finalize_kernel<double, 3>), dim3(a.nblk, 1, members), dim3(256), 0, stream, a);
+  hipLaunchKernelGGL((rpy_sym_finalize_kernel<T, D>), dim3(a.nblk, 1, members), dim3(256), 0, stream, a);
   CFM_CHECK_LAUNCH();
   return {};
+}
+
+// 3 DOF, FP64, member vectors contiguous (component stride ns).
+Status sym_members_apply(hipStream_t stream, const double* x, const double* y, const double* z, int ns,
+                         int members, const double* f, size_t mvec, double* v, double mu,
+                         const int* status, int sstride, int want, DevBuf<double>& own,
+                         DevBuf<double>& src) {
+  return sym_members_t<double, 3>(stream, x, y, z, ns, members, f, ns, mvec, v, ns, mu, status,
+                                  sstride, want, own, src);
+}
+
+// The context's segments as `members` equal systems of ns segments each (6 or 3 DOF, in
+// the context's precision): v_seg = M_ss f_seg per member, no coupling between members;
+// members with active[e] == 0 keep their velocities.
+Status sym_members_segments(cfm_ctx* c, int members, const int* active) {
+  const Particles& p = c->p;
+  if (members <= 0 || p.n_seg % members != 0)
+    return make_status(CFM_EINVAL, "members must divide the segments");
+  const int ns = p.n_seg / members;
+  const bool fp32 = c->cfg.precision == CFM_FP32C;
+  const size_t fld = p.n, vld = p.n;
+  if (fp32)
+    return p.seg_dof == 6
+               ? sym_members_t<float, 6>(c->stream, p.x.p, p.y.p, p.z.p, ns, members, p.f.p, fld, ns,
+                                         p.v.p, vld, c->cfg.mu, active, 1, 1, c->sym_own, c->sym_src)
+               : sym_members_t<float, 3>(c->stream, p.x.p, p.y.p, p.z.p, ns, members, p.f.p, fld, ns,
+                                         p.v.p, vld, c->cfg.mu, active, 1, 1, c->sym_own, c->sym_src);
+  return p.seg_dof == 6
+             ? sym_members_t<double, 6>(c->stream, p.x.p, p.y.p, p.z.p, ns, members, p.f.p, fld, ns,
+                                        p.v.p, vld, c->cfg.mu, active, 1, 1, c->sym_own, c->sym_src64)
+             : sym_members_t<double, 3>(c->stream, p.x.p, p.y.p, p.z.p, ns, members, p.f.p, fld, ns,
+                                        p.v.p, vld, c->cfg.mu, active, 1, 1, c->sym_own, c->sym_src64);
 }
 
 Status launch_sym(cfm_ctx* c, const ApplyPlan& plan, int species, hipStream_t stream) {

@@ -269,6 +269,31 @@ __global__ void max_finish_kernel(const double* __restrict__ part, int nb, doubl
   if (threadIdx.x == 0) out[0] = red[0];
 }
 
+// bad[e] = max over rows r of system e of |err[e Lm + r]| times 1/(DL TOL) (position
+// rows: the first 3 nseg of each filament's 6 nseg) or 1/TOL (Lie-algebra rows).
+__global__ void member_badness_kernel(FilK K, const double* __restrict__ err, int Lm,
+                                      double* __restrict__ bad) {
+  const double* x = err + (size_t)blockIdx.x * Lm;
+  const int per = 6 * K.nseg, half = 3 * K.nseg;
+  const double sx = 1.0 / (K.dl * K.tol), su = 1.0 / K.tol;
+  double b = 0.0;
+  for (int r = threadIdx.x; r < Lm; r += 256) {
+    const double a = fabs(x[r]) * ((r % per) < half ? sx : su);
+    b = (a != a || b != b) ? __builtin_nan("") : fmax(a, b);
+  }
+  __shared__ double red[256];
+  red[threadIdx.x] = b;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (threadIdx.x < off) {
+      const double a = red[threadIdx.x], c = red[threadIdx.x + off];
+      red[threadIdx.x] = (a != a || c != c) ? __builtin_nan("") : fmax(a, c);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) bad[blockIdx.x] = red[0];
+}
+
 // ---- approximate Jacobian -----------------------------------------------------
 struct M43 {
   double a[4][3];
@@ -759,6 +784,7 @@ __global__ __launch_bounds__(256) void barrier_kernel(FilK K, GridPos P, double 
               cell_coord(zj, inv_h) != nz)
             continue;
           if (j < P.ns && j / K.nseg == fi && abs(i - j) <= 1) continue;
+          if (K.fpm && j < P.ns && (j / K.nseg) / K.fpm != fi / K.fpm) continue;  // other system
           const double ex = xi - xj, ey = yi - yj, ez = zi - zj;
           const double d2 = ex * ex + ey * ey + ez * ez;
           if (d2 < cut * cut) {
@@ -886,6 +912,12 @@ Status fil_blob_barrier(cfm_ctx* c, const FilK& K, BarrierGrid& g, int nb_per_bo
   const double inv_h = 1.0 / (1.1 * 2.0 * K.a);
   hipLaunchKernelGGL(blob_barrier_kernel, gsz(p.n_blob, 256), dim3(256), 0, c->stream, K, P, inv_h,
                      g.nbucket - 1, g.start.p, g.ids.p, nb_per_body, rep);
+  CFM_CHECK_LAUNCH();
+  return {};
+}
+
+Status fil_member_badness(cfm_ctx* c, const FilK& K, const double* err, int Lm, int E, double* bad) {
+  hipLaunchKernelGGL(member_badness_kernel, dim3(E), dim3(256), 0, c->stream, K, err, Lm, bad);
   CFM_CHECK_LAUNCH();
   return {};
 }

@@ -36,6 +36,9 @@ struct FilK {
   const double *bx = nullptr, *bq = nullptr, *bu = nullptr, *fref = nullptr;
   int fpb = 0;
   double* clampblk = nullptr;  // 18 per filament: Lie rows 0 and 1 of J against u_0
+  // Equal independent systems of fpm consecutive filaments each (0: one system): the
+  // barrier skips pairs of segments of different systems.
+  int fpm = 0;
 };
 
 // Hashed uniform cell list over segments and blobs for the barrier: cells of edge
@@ -69,5 +72,9 @@ Status fil_blob_barrier(cfm_ctx* c, const FilK& K, BarrierGrid& grid, int nb_per
 Status fil_jacobian(cfm_ctx* c, const FilK& K, int nt, int* status_dev);
 Status fil_jinv_mult(cfm_ctx* c, const FilK& K, const double* in, double* out, double scale);
 Status fil_update(cfm_ctx* c, const FilK& K, const double* upd);
+// Badness of each of E equal systems from the residual err (Lm rows each, laid out as
+// fil_errors writes them): the max of |e_x| / (DL TOL) and |e_u| / TOL, NaN if a row is
+// not finite; bad[e].
+Status fil_member_badness(cfm_ctx* c, const FilK& K, const double* err, int Lm, int E, double* bad);
 
 }  // namespace cfm

@@ -64,13 +64,23 @@ __global__ __launch_bounds__(kRB) void dots_partial_kernel(const double* __restr
 // for another, and the order of every addition is fixed (deterministic).
 // mode 0: out[i] = s; 1: out[i] += s; 2: out[i] = sqrt(max(s, 0)).  out2, when given,
 // receives the same value at out2[i * ld2].
+// Members (blockIdx.z = e, independent reductions of equal length L): Q + e mq, w + e mw,
+// out + e ldo, counter e, their own partials; members with active[e] == 0 are skipped.
 __global__ __launch_bounds__(kRB) void dots_fused_kernel(const double* __restrict__ Q, size_t ldq,
                                                          int k, const double* __restrict__ w, int L,
                                                          double* partial, unsigned* counter,
                                                          double* out, int mode, double* out2,
-                                                         int ld2) {
+                                                         int ld2, size_t mq, size_t mw, int ldo,
+                                                         const int* __restrict__ active) {
   __shared__ double red[kRB];
   __shared__ int last;
+  const int e = blockIdx.z;
+  if (active && !active[e]) return;
+  Q += e * mq;
+  w += e * mw;
+  out += (size_t)e * ldo;
+  counter += e;
+  partial += (size_t)e * gridDim.x * gridDim.y;
   const int i = blockIdx.x;
   const int b = blockIdx.y;
   const double* q = Q + (size_t)i * ldq;
@@ -344,26 +354,28 @@ void Krylov::release() {
   L = m = 0;
 }
 
-// The context's reduction counter, created (zeroed on the context's stream) on first use.
-static Status red_counter(cfm_ctx* c, unsigned** out) {
-  if (!c->red_count.p) {
-    CFM_TRY(c->red_count.alloc(1));
-    CFM_HIP(hipMemsetAsync(c->red_count.p, 0, sizeof(unsigned), c->stream));
+// The context's reduction counters (at least n), created zeroed on the context's stream.
+static Status red_counter(cfm_ctx* c, unsigned** out, int n = 1) {
+  if (!c->red_count.p || c->red_count.n < (size_t)n) {
+    CFM_TRY(c->red_count.alloc((size_t)std::max(n, 1)));
+    CFM_HIP(hipMemsetAsync(c->red_count.p, 0, sizeof(unsigned) * c->red_count.n, c->stream));
   }
   *out = c->red_count.p;
   return {};
 }
 
-// One-launch reduction of Q[0..k)^T w into out (mode as dots_fused_kernel).
+// One-launch reduction of Q[0..k)^T w into out (mode as dots_fused_kernel); with
+// members > 1, one reduction per member (strides mq, mw, ldo; active may be null).
 static Status dots_launch(cfm_ctx* c, double* partial, const double* Q, size_t ldq, int k,
                           const double* w, int L, double* out, int mode, double* out2 = nullptr,
-                          int ld2 = 0) {
-  if (k <= 0) return {};
+                          int ld2 = 0, int members = 1, size_t mq = 0, size_t mw = 0, int ldo = 0,
+                          const int* active = nullptr) {
+  if (k <= 0 || members <= 0) return {};
   unsigned* cnt = nullptr;
-  CFM_TRY(red_counter(c, &cnt));
+  CFM_TRY(red_counter(c, &cnt, members));
   const int nb = (L + kChunk - 1) / kChunk;
-  hipLaunchKernelGGL(dots_fused_kernel, dim3(k, nb), dim3(kRB), 0, c->stream, Q, ldq, k, w, L,
-                     partial, cnt, out, mode, out2, ld2);
+  hipLaunchKernelGGL(dots_fused_kernel, dim3(k, nb, members), dim3(kRB), 0, c->stream, Q, ldq, k,
+                     w, L, partial, cnt, out, mode, out2, ld2, mq, mw, ldo, active);
   CFM_CHECK_LAUNCH();
   return {};
 }
@@ -718,6 +730,60 @@ Status dev_scale_by_inv(cfm_ctx* c, const double* x, const double* s, double* ou
 }
 Status dev_norm2(cfm_ctx* c, double* partial, const double* w, int L, double* out) {
   return dots_launch(c, partial, w, 0, 1, w, L, out, 2);
+}
+
+// ---- members: E independent vectors of length Lm laid end to end ------------------
+// w[r] -= sum_{i<k} h[m ldh + i] Q[i][r], m = r / Lm (the sum in the order of
+// gemv_sub_kernel)
+__global__ void member_gemv_sub_kernel(const double* __restrict__ Q, size_t ldq, int k,
+                                       const double* __restrict__ h, int ldh,
+                                       double* __restrict__ w, int L, int Lm) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= L) return;
+  const double* hm = h + (size_t)(r / Lm) * ldh;
+  double s = 0.0;
+  for (int i = 0; i < k; ++i) s += hm[i] * Q[(size_t)i * ldq + r];
+  w[r] -= s;
+}
+// out[r] = x[r] / s[m] (0 when s[m] is 0)
+__global__ void member_scale_by_inv_kernel(const double* __restrict__ x, const double* __restrict__ s,
+                                           double* __restrict__ out, int L, int Lm) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= L) return;
+  const double d = s[r / Lm];
+  out[r] = d != 0.0 ? x[r] / d : 0.0;
+}
+// x[r] = 0 where member m is inactive
+__global__ void member_mask_kernel(double* __restrict__ x, int L, int Lm, const int* __restrict__ active) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < L && !active[r / Lm]) x[r] = 0.0;
+}
+
+Status dev_member_dots(cfm_ctx* c, double* partial, const double* Q, size_t ldq, int k,
+                       const double* w, int Lm, int E, double* out, int ldo, bool norm,
+                       const int* active) {
+  return dots_launch(c, partial, Q, ldq, k, w, Lm, out, norm ? 2 : 0, nullptr, 0, E, (size_t)Lm,
+                     (size_t)Lm, ldo, active);
+}
+size_t dev_member_dots_partial_size(int Lm, int k, int E) {
+  return (size_t)E * dev_dots_partial_size(Lm, k);
+}
+Status dev_member_gemv_sub(cfm_ctx* c, const double* Q, size_t ldq, int k, const double* h, int ldh,
+                           double* w, int L, int Lm) {
+  if (k <= 0) return {};
+  hipLaunchKernelGGL(member_gemv_sub_kernel, g1(L), dim3(256), 0, c->stream, Q, ldq, k, h, ldh, w, L, Lm);
+  CFM_CHECK_LAUNCH();
+  return {};
+}
+Status dev_member_scale_by_inv(cfm_ctx* c, const double* x, const double* s, double* out, int L, int Lm) {
+  hipLaunchKernelGGL(member_scale_by_inv_kernel, g1(L), dim3(256), 0, c->stream, x, s, out, L, Lm);
+  CFM_CHECK_LAUNCH();
+  return {};
+}
+Status dev_member_mask(cfm_ctx* c, double* x, int L, int Lm, const int* active) {
+  hipLaunchKernelGGL(member_mask_kernel, g1(L), dim3(256), 0, c->stream, x, L, Lm, active);
+  CFM_CHECK_LAUNCH();
+  return {};
 }
 Status dev_copy_rows(cfm_ctx* c, const double* src, size_t src_ld, int src_off, double* dst,
                      size_t dst_ld, int dst_off, int dof, int m) {

@@ -74,6 +74,19 @@ Status dev_dots(cfm_ctx* c, double* partial, const double* Q, size_t ldq, int k,
 Status dev_dots2(cfm_ctx* c, double* partial, const double* Q, size_t ldq, int k, const double* w,
                  int L, double* out, double* out2, int ld2);
 size_t dev_dots_partial_size(int L, int k);
+// E independent vectors of length Lm laid end to end (members): out[e ldo + i] =
+// Q[i][e Lm ..]^T w[e Lm ..] (norm: the square root, k = 1), members with active[e] == 0
+// skipped (active may be null); the partials hold dev_member_dots_partial_size values.
+Status dev_member_dots(cfm_ctx* c, double* partial, const double* Q, size_t ldq, int k,
+                       const double* w, int Lm, int E, double* out, int ldo, bool norm,
+                       const int* active);
+size_t dev_member_dots_partial_size(int Lm, int k, int E);
+// w[r] -= sum_{i<k} h[m ldh + i] Q[i][r] with m = r / Lm
+Status dev_member_gemv_sub(cfm_ctx* c, const double* Q, size_t ldq, int k, const double* h, int ldh,
+                           double* w, int L, int Lm);
+// out[r] = x[r] / s[r / Lm]; x[r] = 0 where active[r / Lm] == 0
+Status dev_member_scale_by_inv(cfm_ctx* c, const double* x, const double* s, double* out, int L, int Lm);
+Status dev_member_mask(cfm_ctx* c, double* x, int L, int Lm, const int* active);
 Status dev_gemv_sub(cfm_ctx* c, const double* Q, size_t ldq, int k, const double* h, double* w, int L);
 Status dev_axpby(cfm_ctx* c, double a, const double* x, double b, const double* y, double* out, int L);
 Status dev_fill(cfm_ctx* c, double* x, double v, int L);

@@ -30,6 +30,7 @@ Status side_join(cfm_ctx* c);
 Status upload_aos(cfm_ctx* c, const double* host, int cnt, int dof, double* dst, int ld, int off);
 Status download_aos(cfm_ctx* c, double* host, int cnt, int dof, const double* src, int ld, int off);
 Status gnn_guess(cfm_ctx* c, const double* rhs, double* x0, bool* used);
+Status sym_members_segments(cfm_ctx* c, int members, const int* active);
 Status gnn_record(cfm_ctx* c, const double* b, const double* x);
 
 // Ring of past (rhs, solution) pairs of the blob system and, on the device, the Gram
@@ -219,6 +220,15 @@ struct FilState {
   BxK X;
   int nt = 0;  // the step in progress
   DevBuf<double> bx_fb, bx_fb_m1, bx_fb_m2;
+  // equal independent systems stepped together (cfm_fil_set_members): per member the
+  // Broyden badness, low-rank coefficients (members x (max_broyden + 1)), norms and the
+  // active flag (1 while its iteration continues)
+  int members = 1;
+  DevBuf<double> m_bad, m_coef, m_norm, m_partial;
+  DevBuf<int> m_active;
+  double* m_host = nullptr;  // pinned read-back of the members' badness
+  int* m_host_flags = nullptr;  // pinned staging of the active flags
+  std::vector<cfm_step_stats> m_stats;
 };
 
 static Status ensure_host(FilState* F) {
@@ -535,6 +545,139 @@ static Status step(cfm_ctx* c, FilState* F, int nt, cfm_step_stats* st) {
   return {};
 }
 
+// ---- equal independent systems in one context (members) ----------------------------
+// Each member is a wall-bounded set of fpm filaments.  One product launch evaluates every
+// active member's M_ss f (no coupling between members), the barrier skips pairs of
+// different members, and each member runs its own Broyden iteration: its low-rank
+// coefficients and norms are reductions over its own rows, it stops when its own
+// badness meets TOL, and from then on its state is held (zero update).  Every member
+// takes the arithmetic a lone system of fpm filaments would (the same product
+// decomposition and reduction orders), so its states equal those of a lone run.
+
+static Status member_velocities(cfm_ctx* c, FilState* F) {
+  if (F->prm.barrier) CFM_TRY(fil_barrier(c, F->K, F->grid));
+  const int saved = c->cfg.precision;
+  c->cfg.precision = F->prm.pair_precision;
+  const Status s = sym_members_segments(c, F->members, F->m_active.p);
+  c->cfg.precision = saved;
+  return s;
+}
+
+// err = the residual of every member; bad[e] read back to the host.
+static Status member_residual(cfm_ctx* c, FilState* F, int nt, double* err, std::vector<double>& bad) {
+  const int E = F->members, Lm = F->nbroy / E;
+  CFM_TRY(fil_errors(c, F->K, nt, err, F->bad_part.p, F->scal.p, 0));
+  CFM_TRY(fil_member_badness(c, F->K, err, Lm, E, F->m_bad.p));
+  CFM_HIP(hipMemcpyAsync(F->m_host, F->m_bad.p, sizeof(double) * E, hipMemcpyDeviceToHost, c->stream));
+  CFM_HIP(hipStreamSynchronize(c->stream));
+  bad.assign(F->m_host, F->m_host + E);
+  return {};
+}
+
+// The members' active flags to the device (through pinned memory, waited for: the next
+// call may rewrite the staging words).
+static Status member_flags(cfm_ctx* c, FilState* F, const std::vector<int>& act) {
+  std::memcpy(F->m_host_flags, act.data(), sizeof(int) * act.size());
+  CFM_HIP(hipMemcpyAsync(F->m_active.p, F->m_host_flags, sizeof(int) * act.size(),
+                         hipMemcpyHostToDevice, c->stream));
+  CFM_HIP(hipStreamSynchronize(c->stream));
+  return {};
+}
+
+// out = -gamma J0^-1 e - sum_{i<it} (e . D_i) C_i, member by member (active members)
+static Status member_low_rank(cfm_ctx* c, FilState* F, const double* e, double* out, int it) {
+  const int E = F->members, L = F->nbroy, Lm = L / E, ldc = F->prm.max_broyden + 1;
+  CFM_TRY(fil_jinv_mult(c, F->K, e, out, -F->prm.gamma));
+  if (it > 0) {
+    CFM_TRY(dev_member_dots(c, F->m_partial.p, F->D.p, (size_t)L, it, e, Lm, E, F->m_coef.p, ldc,
+                            false, F->m_active.p));
+    CFM_TRY(dev_member_gemv_sub(c, F->C.p, (size_t)L, it, F->m_coef.p, ldc, out, L, Lm));
+  }
+  return {};
+}
+
+static Status member_step(cfm_ctx* c, FilState* F, int nt, cfm_step_stats* st) {
+  const auto t0 = std::chrono::steady_clock::now();
+  F->nt = nt;
+  const int E = F->members, L = F->nbroy, Lm = L / E;
+  const int maxit = F->prm.max_broyden;
+  std::vector<int> act(E, 1), iters(E, 0);
+  std::vector<double> bad(E, 0.0), last(E, 0.0);
+  CFM_TRY(initial_guess(c, F, nt));
+  CFM_TRY(fil_forces(c, F->K, nt));
+  CFM_TRY(member_flags(c, F, act));
+  CFM_TRY(member_velocities(c, F));
+  CFM_TRY(member_residual(c, F, nt, F->err.p, bad));
+  bool any = false;
+  for (int e = 0; e < E; ++e) {
+    if (!std::isfinite(bad[e]))
+      return make_status(CFM_ENONFINITE, "non-finite residual of member %d at step %d", e, nt);
+    act[e] = bad[e] > 1.0;
+    last[e] = bad[e];
+    any = any || act[e];
+  }
+  CFM_TRY(member_flags(c, F, act));
+  if (any) CFM_TRY(jacobian(c, F, nt));
+  int it = 0;
+  while (any && it < maxit) {
+    CFM_TRY(member_low_rank(c, F, F->err.p, F->upd.p, it));
+    CFM_TRY(dev_member_mask(c, F->upd.p, L, Lm, F->m_active.p));  // held members: zero update
+    CFM_TRY(update(c, F, F->upd.p));
+    CFM_TRY(fil_forces(c, F->K, nt));
+    CFM_TRY(member_velocities(c, F));
+    CFM_TRY(member_residual(c, F, nt, F->nerr.p, bad));
+    std::vector<int> next(E, 0);
+    bool cont = false;
+    for (int e = 0; e < E; ++e) {
+      if (!act[e]) continue;
+      if (!std::isfinite(bad[e]))
+        return make_status(CFM_ENONFINITE,
+                           "Broyden's method diverged for member %d after %d iterations during step %d",
+                           e, it + 1, nt + 1);
+      last[e] = bad[e];
+      next[e] = bad[e] > 1.0;
+      if (!next[e]) iters[e] = it + 1;
+      cont = cont || next[e];
+    }
+    if (cont && it < maxit - 1) {
+      // the low-rank pair of the members that continue (broyden_solver.cpp:140-174)
+      CFM_TRY(member_flags(c, F, next));
+      double* Ci = F->C.p + (size_t)it * L;
+      double* Di = F->D.p + (size_t)it * L;
+      CFM_TRY(member_low_rank(c, F, F->nerr.p, Ci, it));
+      CFM_TRY(dev_axpby(c, 1.0, F->nerr.p, -1.0, F->err.p, Di, L));
+      CFM_TRY(dev_member_dots(c, F->m_partial.p, Di, 0, 1, Di, Lm, E, F->m_norm.p, 1, true,
+                              F->m_active.p));
+      CFM_TRY(dev_member_scale_by_inv(c, Ci, F->m_norm.p, Ci, L, Lm));
+      CFM_TRY(dev_member_scale_by_inv(c, Di, F->m_norm.p, Di, L, Lm));
+      std::swap(F->err.p, F->nerr.p);
+    } else if (cont) {
+      CFM_TRY(member_flags(c, F, next));
+    }
+    act = next;
+    any = cont;
+    ++it;
+  }
+  CFM_HIP(hipStreamSynchronize(c->stream));
+  const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  F->m_stats.assign(E, cfm_step_stats{});
+  cfm_step_stats S{};
+  S.converged = 1;
+  for (int e = 0; e < E; ++e) {
+    cfm_step_stats& m = F->m_stats[e];
+    m.broyden_iterations = act[e] ? it : iters[e];
+    m.converged = act[e] ? 0 : 1;
+    m.final_error = last[e];
+    m.seconds = secs;
+    S.broyden_iterations = std::max(S.broyden_iterations, m.broyden_iterations);
+    S.converged = S.converged && m.converged;
+    S.final_error = std::max(S.final_error, m.final_error);
+  }
+  S.seconds = secs;
+  if (st) *st = S;
+  return {};
+}
+
 static FilState* fil_of(cfm_ctx* c) { return reinterpret_cast<FilState*>(c->fil); }
 
 }  // namespace cfm
@@ -573,6 +716,13 @@ void cfm_fil_release(cfm_ctx* c) {
   F->hist.used.release();
   F->hist.partial.release();
   if (F->host) (void)hipHostFree(F->host);
+  if (F->m_host) (void)hipHostFree(F->m_host);
+  if (F->m_host_flags) (void)hipHostFree(F->m_host_flags);
+  F->m_bad.release();
+  F->m_norm.release();
+  F->m_coef.release();
+  F->m_partial.release();
+  F->m_active.release();
   delete F;
   c->fil = nullptr;
 }
@@ -699,8 +849,48 @@ int cfm_fil_init(cfm_ctx* c, const cfm_fil_params* prm, const double* q_seg, con
 int cfm_fil_step(cfm_ctx* c, int32_t nt, cfm_step_stats* st) {
   FIL_OR_FAIL(c);
   if (nt < 0) return fail_c(c, make_status(CFM_EINVAL, "nt must be >= 0"));
-  Status s = step(c, F, nt, st);
+  Status s = F->members > 1 ? member_step(c, F, nt, st) : step(c, F, nt, st);
   return s.ok() ? CFM_OK : fail_c(c, s);
+}
+
+int cfm_fil_set_members(cfm_ctx* c, int32_t members) {
+  FIL_OR_FAIL(c);
+  const Particles& p = c->p;
+  if (members < 1 || F->K.nfil % members != 0)
+    return fail_c(c, make_status(CFM_EINVAL, "members must divide the number of filaments (%d)", F->K.nfil));
+  if (members > 1 && (p.n_blob > 0 || F->swim || F->bx || !c->cfg.wall))
+    return fail_c(c, make_status(CFM_EINVAL, "members: wall-bounded filaments only (no blobs or bodies)"));
+  const int L = F->nbroy, Lm = L / members, ldc = F->prm.max_broyden + 1;
+  auto run = [&]() -> Status {
+    F->members = members;
+    F->K.fpm = members > 1 ? F->K.nfil / members : 0;
+    if (members == 1) return {};
+    CFM_TRY(F->m_bad.alloc(members));
+    CFM_TRY(F->m_norm.alloc(members));
+    CFM_TRY(F->m_coef.alloc((size_t)members * ldc));
+    CFM_TRY(dev_fill(c, F->m_coef.p, 0.0, members * ldc));
+    CFM_TRY(F->m_partial.alloc(dev_member_dots_partial_size(Lm, ldc, members)));
+    CFM_TRY(F->m_active.alloc(members));
+    if (F->m_host) (void)hipHostFree(F->m_host);
+    if (F->m_host_flags) (void)hipHostFree(F->m_host_flags);
+    F->m_host = nullptr;
+    F->m_host_flags = nullptr;
+    CFM_HIP(hipHostMalloc((void**)&F->m_host, sizeof(double) * members, hipHostMallocDefault));
+    CFM_HIP(hipHostMalloc((void**)&F->m_host_flags, sizeof(int) * members, hipHostMallocDefault));
+    CFM_HIP(hipStreamSynchronize(c->stream));
+    return {};
+  };
+  Status s = run();
+  return s.ok() ? CFM_OK : fail_c(c, s);
+}
+
+int cfm_fil_member_stats(cfm_ctx* c, cfm_step_stats* per_member) {
+  FIL_OR_FAIL(c);
+  if (!per_member) return fail_c(c, make_status(CFM_EINVAL, "null argument"));
+  if ((int)F->m_stats.size() != F->members || F->members < 2)
+    return fail_c(c, make_status(CFM_ESTATE, "no member step has run"));
+  for (int e = 0; e < F->members; ++e) per_member[e] = F->m_stats[e];
+  return CFM_OK;
 }
 
 int cfm_fil_get_state(cfm_ctx* c, double* x, double* q, double* u, double* lam, double* clamp) {

@@ -80,6 +80,20 @@ class FilamentSystem:
         pert_arr = None if pert is None else L.f64(pert, 3 * self.nfil)
         L.check(self.h, L.lib().cfm_fil_init(self.h, ctypes.byref(P), L.ptr(q), L.ptr(pert_arr)))
         self.nt = 0
+        self.members = 1
+
+    def set_members(self, members):
+        """The filaments as `members` equal independent wall-bounded systems stepped
+        together (cfm_fil_set_members): member e owns filaments [e F, (e+1) F) of the
+        stacked configuration, F = n_fil / members, and each evolves as if alone."""
+        L.check(self.h, L.lib().cfm_fil_set_members(self.h, int(members)))
+        self.members = int(members)
+
+    def member_stats(self):
+        """Per-member statistics of the last step (cfm_fil_member_stats)."""
+        st = (L.cfm_step_stats * self.members)()
+        L.check(self.h, L.lib().cfm_fil_member_stats(self.h, st))
+        return [s.as_dict() for s in st]
 
     def restart(self, pert=None):
         """Back to the initial configuration (positions, orientations, zero multipliers)
@@ -90,6 +104,8 @@ class FilamentSystem:
         L.check(self.h, L.lib().cfm_fil_init(self.h, ctypes.byref(self._P), L.ptr(self._q0),
                                               L.ptr(pert_arr)))
         self.nt = 0
+        if self.members > 1:
+            L.check(self.h, L.lib().cfm_fil_set_members(self.h, self.members))
 
     @property
     def h(self):

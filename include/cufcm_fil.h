@@ -237,6 +237,20 @@ int cfm_fil_init(cfm_ctx* ctx, const cfm_fil_params* params, const double* q_seg
  * device.  Returns CFM_ENONFINITE if the iteration diverges. */
 int cfm_fil_step(cfm_ctx* ctx, int32_t nt, cfm_step_stats* stats);
 
+/* Ensembles: the context's filaments form `members` equal independent systems (member e
+ * owns filaments [e F, (e+1) F), F = nfil / members) -- the runs of a parameter sweep,
+ * bisection or periodic-orbit search (pyfile/driver/func.py:544, one executable per run in
+ * the reference).  Wall-bounded filaments only (no blobs or bodies).  From the next
+ * cfm_fil_step on, one product launch gives every member its own M_ss f (no coupling
+ * between members; the barrier skips pairs of different members) and each member runs its
+ * own Broyden iteration (its own low-rank pairs, its own convergence test; a converged
+ * member is held while the others iterate): every member's state is that of the same
+ * system stepped alone.  The step's stats report the largest iteration count, converged
+ * when every member converged and the largest final error; cfm_fil_member_stats gives
+ * each member's.  members = 1 returns to one system. */
+int cfm_fil_set_members(cfm_ctx* ctx, int32_t members);
+int cfm_fil_member_stats(cfm_ctx* ctx, cfm_step_stats* per_member /* members */);
+
 /* Read / overwrite the filament state (any pointer may be NULL).  Arrays are AoS:
  * x[3*n_seg], q/qm1[4*n_seg], u/um1/xm1/xm2/lam[3*n_seg] (lam row 0 of a filament
  * is its tether multiplier), clamp[3*nfil]. */
