@@ -1019,6 +1019,8 @@ int cfm_fil_update(cfm_ctx* c, const double* upd) {
 int cfm_fil_attach_bodies(cfm_ctx* c, const double* f_ext) {
   FIL_OR_FAIL(c);
   if (F->swim) return fail_c(c, make_status(CFM_ESTATE, "bodies are already attached"));
+  if (F->members > 1)
+    return fail_c(c, make_status(CFM_ESTATE, "members are wall-bounded filaments only (cfm_fil_set_members)"));
   BodyInfo bi;
   Status st = body_info(c, &bi);
   if (!st.ok()) return fail_c(c, st);
