@@ -141,62 +141,60 @@ __device__ V3 body_frame_moment(const FilK& K, Q4 q1, Q4 q2) {
   return {K.kt * 2.0 * m.x, K.kb * 2.0 * m.y, K.kb * 2.0 * m.z};
 }
 
+// One thread per segment: the loads of segment i are the contribution of link (i-1, i)
+// (its multiplier and the moment about its midpoint, the part a loop along the filament
+// carries from the previous segment) followed by that of link (i, i+1), the tether and
+// clamp on the base segment, the follower force at the tip and the step-0 perturbation
+// of segment 4 -- added in that order, with the same expressions as the loop.
 __global__ void fil_forces_kernel(FilK K, int nt, double* __restrict__ fout, int ld) {
-  const int f = blockIdx.x * blockDim.x + threadIdx.x;
-  if (f >= K.nfil) return;
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= (int)K.nseg_total) return;
   const size_t n = K.nseg_total;
-  const int s0 = f * K.nseg;
-  double F[6] = {0, 0, 0, 0, 0, 0};   // loads of segment i (being accumulated)
-  Q4 qi = ldq(K.q, n, s0);
-  V3 ti = tangent(qi);
-  for (int i = 0; i < K.nseg; ++i) {
-    double G[6] = {0, 0, 0, 0, 0, 0};  // contributions to segment i+1
-    if (i < K.nseg - 1) {
-      const Q4 qn = ldq(K.q, n, s0 + i + 1);
-      const V3 tn = tangent(qn);
-      const V3 lam = ld3(K.lam, n, s0 + i + 1);
-      const V3 c1 = cross(ti, lam), c2 = cross(tn, lam);
-      const V3 M = mv(rot_mat(midpoint(qi, qn)), body_frame_moment(K, qi, qn));
-      F[0] -= lam.x; F[1] -= lam.y; F[2] -= lam.z;
-      G[0] += lam.x; G[1] += lam.y; G[2] += lam.z;
-      F[3] += -0.5 * K.dl * c1.x + M.x;
-      F[4] += -0.5 * K.dl * c1.y + M.y;
-      F[5] += -0.5 * K.dl * c1.z + M.z;
-      G[3] += -0.5 * K.dl * c2.x - M.x;
-      G[4] += -0.5 * K.dl * c2.y - M.y;
-      G[5] += -0.5 * K.dl * c2.z - M.z;
-      if (i == 0) {
-        // tether force and clamping torque on the base segment
-        const V3 tl = ld3(K.lam, n, s0);
-        F[0] += tl.x; F[1] += tl.y; F[2] += tl.z;
-        const V3 ct = dexpinv_transpose(ld3(K.u, n, s0), ld3(K.clamp, K.nfil, f));
-        F[3] += ct.x; F[4] += ct.y; F[5] += ct.z;
-      }
-      qi = qn;
-      ti = tn;
-    } else {
-      if (i == 0) {
-        const V3 tl = ld3(K.lam, n, s0);
-        F[0] += tl.x; F[1] += tl.y; F[2] += tl.z;
-        const V3 ct = dexpinv_transpose(ld3(K.u, n, s0), ld3(K.clamp, K.nfil, f));
-        F[3] += ct.x; F[4] += ct.y; F[5] += ct.z;
-      }
-      // follower force at the tip
-      F[0] -= K.end_force * ti.x;
-      F[1] -= K.end_force * ti.y;
-      F[2] -= K.end_force * ti.z;
-    }
-    if (nt == 0 && i == 4 && K.nseg > 4) {
-      F[0] += K.pert[f];
-      F[1] += K.pert[K.nfil + f];
-      F[2] += K.pert[2 * K.nfil + f];
-    }
-    const int s = s0 + i;
-    for (int c = 0; c < 6; ++c) {
-      K.fint[c * n + s] = F[c];
-      fout[(size_t)c * ld + s] = F[c];
-      F[c] = G[c];
-    }
+  const int f = s / K.nseg, i = s - f * K.nseg, s0 = f * K.nseg;
+  double F[6] = {0, 0, 0, 0, 0, 0};
+  const Q4 qi = ldq(K.q, n, s);
+  const V3 ti = tangent(qi);
+  if (i > 0) {
+    const Q4 qp = ldq(K.q, n, s - 1);
+    const V3 lam = ld3(K.lam, n, s);
+    const V3 c2 = cross(ti, lam);
+    const V3 M = mv(rot_mat(midpoint(qp, qi)), body_frame_moment(K, qp, qi));
+    F[0] += lam.x; F[1] += lam.y; F[2] += lam.z;
+    F[3] += -0.5 * K.dl * c2.x - M.x;
+    F[4] += -0.5 * K.dl * c2.y - M.y;
+    F[5] += -0.5 * K.dl * c2.z - M.z;
+  }
+  if (i < K.nseg - 1) {
+    const Q4 qn = ldq(K.q, n, s + 1);
+    const V3 lam = ld3(K.lam, n, s + 1);
+    const V3 c1 = cross(ti, lam);
+    const V3 M = mv(rot_mat(midpoint(qi, qn)), body_frame_moment(K, qi, qn));
+    F[0] -= lam.x; F[1] -= lam.y; F[2] -= lam.z;
+    F[3] += -0.5 * K.dl * c1.x + M.x;
+    F[4] += -0.5 * K.dl * c1.y + M.y;
+    F[5] += -0.5 * K.dl * c1.z + M.z;
+  }
+  if (i == 0) {
+    // tether force and clamping torque on the base segment
+    const V3 tl = ld3(K.lam, n, s0);
+    F[0] += tl.x; F[1] += tl.y; F[2] += tl.z;
+    const V3 ct = dexpinv_transpose(ld3(K.u, n, s0), ld3(K.clamp, K.nfil, f));
+    F[3] += ct.x; F[4] += ct.y; F[5] += ct.z;
+  }
+  if (i == K.nseg - 1) {
+    // follower force at the tip
+    F[0] -= K.end_force * ti.x;
+    F[1] -= K.end_force * ti.y;
+    F[2] -= K.end_force * ti.z;
+  }
+  if (nt == 0 && i == 4 && K.nseg > 4) {
+    F[0] += K.pert[f];
+    F[1] += K.pert[K.nfil + f];
+    F[2] += K.pert[2 * K.nfil + f];
+  }
+  for (int c = 0; c < 6; ++c) {
+    K.fint[c * n + s] = F[c];
+    fout[(size_t)c * ld + s] = F[c];
   }
 }
 
@@ -608,25 +606,30 @@ __global__ __launch_bounds__(128) void fil_jinv_mult_kernel(FilK K, const double
   }
 }
 
-__global__ void fil_update_kernel(FilK K, const double* __restrict__ upd) {
-  const int f = blockIdx.x * blockDim.x + threadIdx.x;
-  if (f >= K.nfil) return;
+// One workgroup per group of 256 / nseg filaments: first one thread per segment (its
+// multiplier, the clamp on the base segment, the Lie-algebra element and quaternion of
+// the others; with free bodies the base attached to its moved body, swimmer.cpp:1076-1100),
+// then one thread per filament places the segments by the robot arm.
+__global__ __launch_bounds__(256) void fil_update_kernel(FilK K, const double* __restrict__ upd) {
+  const int n = K.nseg, per = 256 / n;
+  const int f0 = blockIdx.x * per;
+  const int fl = threadIdx.x / n, i = threadIdx.x - fl * n, f = f0 + fl;
   const size_t ns = K.nseg_total;
-  const int n = K.nseg, s0 = f * n;
-  const double* u = upd + (size_t)f * 6 * n;
-  if (K.bx) attach_to_body(K, f);  // the body has moved first (swimmer.cpp:1076-1100)
-  for (int i = 0; i < n; ++i) {
-    const int s = s0 + i;
+  if (fl < per && f < K.nfil) {
+    const int s = f * n + i;
+    const double* u = upd + (size_t)f * 6 * n;
+    if (i == 0 && K.bx) attach_to_body(K, f);
     st3(K.lam, ns, s, ld3(K.lam, ns, s) + v3(u[3 * i], u[3 * i + 1], u[3 * i + 2]));
+    if (i == 0) {
+      st3(K.clamp, K.nfil, f, ld3(K.clamp, K.nfil, f) + v3(u[3 * n], u[3 * n + 1], u[3 * n + 2]));
+    } else {
+      const V3 un = ld3(K.u, ns, s) + v3(u[3 * (n + i)], u[3 * (n + i) + 1], u[3 * (n + i) + 2]);
+      st3(K.u, ns, s, un);
+      stq(K.q, ns, s, qmul(lie_exp(un), ldq(K.qm1, ns, s)));
+    }
   }
-  st3(K.clamp, K.nfil, f, ld3(K.clamp, K.nfil, f) + v3(u[3 * n], u[3 * n + 1], u[3 * n + 2]));
-  for (int i = 1; i < n; ++i) {
-    const int s = s0 + i;
-    const V3 un = ld3(K.u, ns, s) + v3(u[3 * (n + i)], u[3 * (n + i) + 1], u[3 * (n + i) + 2]);
-    st3(K.u, ns, s, un);
-    stq(K.q, ns, s, qmul(lie_exp(un), ldq(K.qm1, ns, s)));
-  }
-  robot_arm(K, f);
+  __syncthreads();  // the group's quaternions and bases are written
+  if ((int)threadIdx.x < per && f0 + (int)threadIdx.x < K.nfil) robot_arm(K, f0 + threadIdx.x);
 }
 
 // ---- barrier forces over a hashed cell list -----------------------------------------
@@ -875,8 +878,8 @@ Status fil_initial_guess(cfm_ctx* c, const FilK& K, int nt) {
 }
 
 Status fil_forces(cfm_ctx* c, const FilK& K, int nt) {
-  hipLaunchKernelGGL(fil_forces_kernel, gsz(K.nfil, 64), dim3(64), 0, c->stream, K, nt, c->p.f.p,
-                     c->p.n);
+  hipLaunchKernelGGL(fil_forces_kernel, gsz((int)K.nseg_total, 256), dim3(256), 0, c->stream, K, nt,
+                     c->p.f.p, c->p.n);
   CFM_CHECK_LAUNCH();
   return {};
 }
@@ -976,7 +979,8 @@ Status fil_jinv_mult(cfm_ctx* c, const FilK& K, const double* in, double* out, d
 }
 
 Status fil_update(cfm_ctx* c, const FilK& K, const double* upd) {
-  hipLaunchKernelGGL(fil_update_kernel, gsz(K.nfil, 64), dim3(64), 0, c->stream, K, upd);
+  const int per = 256 / K.nseg;  // filaments per workgroup (nseg <= kMaxFilSeg)
+  hipLaunchKernelGGL(fil_update_kernel, dim3((K.nfil + per - 1) / per), dim3(256), 0, c->stream, K, upd);
   CFM_CHECK_LAUNCH();
   return {};
 }
