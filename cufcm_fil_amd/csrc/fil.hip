@@ -756,50 +756,58 @@ __global__ void grid_sort_kernel(int nb, const int* __restrict__ start, int* __r
 // (cuda_functions.cu:1825-1930): wall repulsion below half a segment length, and a
 // (1 - chi (r - 2a))^4 / r repulsion from every particle within 1.1 x 2a except the
 // segment itself and its neighbours along the same filament.
+// 32 lanes per segment, lane c < 27 walking neighbour cell c (dz, dy, dx = c / 9 - 1,
+// c / 3 % 3 - 1, c % 3 - 1) and its bucket's particles in ascending id; the 27 cell sums
+// are added by a butterfly in a fixed pattern, then the wall term.  (One thread walking
+// the 27 cells in turn was bound by the chain of dependent loads of each cell.)
 __global__ __launch_bounds__(256) void barrier_kernel(FilK K, GridPos P, double inv_h, int mask,
                                                       const int* __restrict__ start,
                                                       const int* __restrict__ ids, int wall,
                                                       double* __restrict__ fout, int ld) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= (int)K.nseg_total) return;
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = g >> 5, c = g & 31;
+  if (i >= (int)K.nseg_total) return;  // all 32 lanes of a segment leave together
   const double xi = K.x[i], yi = K.y[i], zi = K.z[i];
   const int fi = i / K.nseg;
   const double fmag = K.rep_factor * K.end_force;
   double fx = 0.0, fy = 0.0, fz = 0.0;
+  const double a_sum = 2.0 * K.a;
+  const double cut = 1.1 * a_sum, chi = 10.0 / a_sum;
+  if (c < 27) {
+    const int nx = cell_coord(xi, inv_h) + c % 3 - 1, ny = cell_coord(yi, inv_h) + (c / 3) % 3 - 1,
+              nz = cell_coord(zi, inv_h) + c / 9 - 1;
+    const int b = cell_bucket(nx, ny, nz, mask);
+    const int t1 = start[b + 1];
+    for (int t = start[b]; t < t1; ++t) {
+      const int j = ids[t];
+      double xj, yj, zj;
+      P.get(j, xj, yj, zj);
+      if (cell_coord(xj, inv_h) != nx || cell_coord(yj, inv_h) != ny || cell_coord(zj, inv_h) != nz)
+        continue;
+      if (j < P.ns && j / K.nseg == fi && abs(i - j) <= 1) continue;
+      if (K.fpm && j < P.ns && (j / K.nseg) / K.fpm != fi / K.fpm) continue;  // other system
+      const double ex = xi - xj, ey = yi - yj, ez = zi - zj;
+      const double d2 = ex * ex + ey * ey + ez * ez;
+      if (d2 < cut * cut) {
+        const double dist = sqrt(d2);
+        double fac = fmin(1.0, 1.0 - chi * (dist - a_sum));
+        fac = fmag * fac * fac * fac * fac / dist;
+        fx += fac * ex;
+        fy += fac * ey;
+        fz += fac * ez;
+      }
+    }
+  }
+  for (int off = 16; off > 0; off >>= 1) {
+    fx += __shfl_xor(fx, off, 32);
+    fy += __shfl_xor(fy, off, 32);
+    fz += __shfl_xor(fz, off, 32);
+  }
+  if (c != 0) return;
   if (wall && zi < 0.5 * K.dl) {
     double w = fmin(1.0, 1.0 - (zi - K.a) / (0.5 * K.dl - K.a));
     fz += fmag * w * w * w * w;
   }
-  const double a_sum = 2.0 * K.a;
-  const double cut = 1.1 * a_sum, chi = 10.0 / a_sum;
-  const int cx = cell_coord(xi, inv_h), cy = cell_coord(yi, inv_h), cz = cell_coord(zi, inv_h);
-  for (int dz = -1; dz <= 1; ++dz)
-    for (int dy = -1; dy <= 1; ++dy)
-      for (int dx = -1; dx <= 1; ++dx) {
-        const int nx = cx + dx, ny = cy + dy, nz = cz + dz;
-        const int b = cell_bucket(nx, ny, nz, mask);
-        const int t1 = start[b + 1];
-        for (int t = start[b]; t < t1; ++t) {
-          const int j = ids[t];
-          double xj, yj, zj;
-          P.get(j, xj, yj, zj);
-          if (cell_coord(xj, inv_h) != nx || cell_coord(yj, inv_h) != ny ||
-              cell_coord(zj, inv_h) != nz)
-            continue;
-          if (j < P.ns && j / K.nseg == fi && abs(i - j) <= 1) continue;
-          if (K.fpm && j < P.ns && (j / K.nseg) / K.fpm != fi / K.fpm) continue;  // other system
-          const double ex = xi - xj, ey = yi - yj, ez = zi - zj;
-          const double d2 = ex * ex + ey * ey + ez * ez;
-          if (d2 < cut * cut) {
-            const double dist = sqrt(d2);
-            double fac = fmin(1.0, 1.0 - chi * (dist - a_sum));
-            fac = fmag * fac * fac * fac * fac / dist;
-            fx += fac * ex;
-            fy += fac * ey;
-            fz += fac * ez;
-          }
-        }
-      }
   fout[i] += fx;
   fout[(size_t)ld + i] += fy;
   fout[2 * (size_t)ld + i] += fz;
@@ -901,7 +909,7 @@ Status fil_barrier(cfm_ctx* c, const FilK& K, BarrierGrid& g) {
   hipLaunchKernelGGL(grid_scatter_kernel, gsz(n, 256), dim3(256), 0, c->stream, n, g.key.p,
                      g.cursor.p, g.ids.p);
   hipLaunchKernelGGL(grid_sort_kernel, gsz(nb, 256), dim3(256), 0, c->stream, nb, g.start.p, g.ids.p);
-  hipLaunchKernelGGL(barrier_kernel, gsz((int)K.nseg_total, 256), dim3(256), 0, c->stream, K, P,
+  hipLaunchKernelGGL(barrier_kernel, gsz((int)K.nseg_total * 32, 256), dim3(256), 0, c->stream, K, P,
                      inv_h, mask, g.start.p, g.ids.p, c->cfg.wall, p.f.p, p.n);
   CFM_CHECK_LAUNCH();
   return {};
