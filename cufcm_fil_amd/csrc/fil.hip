@@ -821,42 +821,47 @@ __global__ __launch_bounds__(256) void blob_barrier_kernel(FilK K, GridPos P, do
                                                            const int* __restrict__ ids,
                                                            int nb_per_body,
                                                            double* __restrict__ rep) {
-  const int ib = blockIdx.x * blockDim.x + threadIdx.x;
+  // 32 lanes per blob, one neighbour cell each, as barrier_kernel
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  const int ib = g >> 5, c = g & 31;
   const int nblob = P.n - P.ns;
-  if (ib >= nblob) return;
+  if (ib >= nblob) return;  // all 32 lanes of a blob leave together
   const double xi = P.bx[ib], yi = P.by[ib], zi = P.bz[ib];
   const int bi = ib / nb_per_body;
   const double fmag = K.rep_factor * K.end_force;
   const double a_sum = 2.0 * K.a;
   const double cut = 1.1 * a_sum, chi = 10.0 / a_sum;
   double fx = 0.0, fy = 0.0, fz = 0.0;
-  const int cx = cell_coord(xi, inv_h), cy = cell_coord(yi, inv_h), cz = cell_coord(zi, inv_h);
-  for (int dz = -1; dz <= 1; ++dz)
-    for (int dy = -1; dy <= 1; ++dy)
-      for (int dx = -1; dx <= 1; ++dx) {
-        const int nx = cx + dx, ny = cy + dy, nz = cz + dz;
-        const int b = cell_bucket(nx, ny, nz, mask);
-        const int t1 = start[b + 1];
-        for (int t = start[b]; t < t1; ++t) {
-          const int j = ids[t];
-          if (j >= P.ns && (j - P.ns) / nb_per_body == bi) continue;
-          double xj, yj, zj;
-          P.get(j, xj, yj, zj);
-          if (cell_coord(xj, inv_h) != nx || cell_coord(yj, inv_h) != ny ||
-              cell_coord(zj, inv_h) != nz)
-            continue;
-          const double ex = xi - xj, ey = yi - yj, ez = zi - zj;
-          const double d2 = ex * ex + ey * ey + ez * ez;
-          if (d2 < cut * cut) {
-            const double dist = sqrt(d2);
-            double fac = fmin(1.0, 1.0 - chi * (dist - a_sum));
-            fac = fmag * fac * fac * fac * fac / dist;
-            fx += fac * ex;
-            fy += fac * ey;
-            fz += fac * ez;
-          }
-        }
+  if (c < 27) {
+    const int nx = cell_coord(xi, inv_h) + c % 3 - 1, ny = cell_coord(yi, inv_h) + (c / 3) % 3 - 1,
+              nz = cell_coord(zi, inv_h) + c / 9 - 1;
+    const int b = cell_bucket(nx, ny, nz, mask);
+    const int t1 = start[b + 1];
+    for (int t = start[b]; t < t1; ++t) {
+      const int j = ids[t];
+      if (j >= P.ns && (j - P.ns) / nb_per_body == bi) continue;
+      double xj, yj, zj;
+      P.get(j, xj, yj, zj);
+      if (cell_coord(xj, inv_h) != nx || cell_coord(yj, inv_h) != ny || cell_coord(zj, inv_h) != nz)
+        continue;
+      const double ex = xi - xj, ey = yi - yj, ez = zi - zj;
+      const double d2 = ex * ex + ey * ey + ez * ez;
+      if (d2 < cut * cut) {
+        const double dist = sqrt(d2);
+        double fac = fmin(1.0, 1.0 - chi * (dist - a_sum));
+        fac = fmag * fac * fac * fac * fac / dist;
+        fx += fac * ex;
+        fy += fac * ey;
+        fz += fac * ez;
       }
+    }
+  }
+  for (int off = 16; off > 0; off >>= 1) {
+    fx += __shfl_xor(fx, off, 32);
+    fy += __shfl_xor(fy, off, 32);
+    fz += __shfl_xor(fz, off, 32);
+  }
+  if (c != 0) return;
   rep[ib] = fx;
   rep[(size_t)nblob + ib] = fy;
   rep[2 * (size_t)nblob + ib] = fz;
@@ -921,7 +926,7 @@ Status fil_blob_barrier(cfm_ctx* c, const FilK& K, BarrierGrid& g, int nb_per_bo
   if (p.n_blob == 0) return {};
   const GridPos P{K.x, K.y, K.z, p.x.p + p.n_seg, p.y.p + p.n_seg, p.z.p + p.n_seg, p.n_seg, n};
   const double inv_h = 1.0 / (1.1 * 2.0 * K.a);
-  hipLaunchKernelGGL(blob_barrier_kernel, gsz(p.n_blob, 256), dim3(256), 0, c->stream, K, P, inv_h,
+  hipLaunchKernelGGL(blob_barrier_kernel, gsz(p.n_blob * 32, 256), dim3(256), 0, c->stream, K, P, inv_h,
                      g.nbucket - 1, g.start.p, g.ids.p, nb_per_body, rep);
   CFM_CHECK_LAUNCH();
   return {};
