@@ -120,6 +120,34 @@ def run_path(args, label, precision, solve_mode, guess, precond, overlap=False, 
     }
 
 
+def pair_roofline(prof, steps):
+    """Roofline line of the dominant pair kernel from the library's HIP-event profile of
+    the timed steps (events on the stream each launch ran on): the symmetric
+    segment-segment product when it runs (kernel + finalisation between the events;
+    flops of both pair directions), else the one-directional tiled kernel."""
+    if prof["kernel_ms_sym"] >= prof["kernel_ms_fp32"]:
+        launches, ms_tot, fl_tot = prof["launches_sym"], prof["kernel_ms_sym"], prof["alg_flops_sym"]
+        kname = "rpy_sym_kernel + rpy_sym_finalize_kernel"
+    else:
+        launches, ms_tot, fl_tot = prof["launches_fp32"], prof["kernel_ms_fp32"], prof["alg_flops_fp32"]
+        kname = "rpy_apply_kernel<float,6,true>"
+    kms = ms_tot / max(1, launches)
+    flops_launch = fl_tot / max(1, launches)
+    achieved = flops_launch / (kms * 1e-3) / 1e12 if kms > 0 else 0.0
+    return {"bound": "valu", "achieved": round(achieved, 3), "peak": FP32_VALU_PEAK_TFLOPS,
+            "unit": "TFLOP/s", "frac": round(achieved / FP32_VALU_PEAK_TFLOPS, 4),
+            "traffic": measured_traffic(kname),
+            "traffic_source": os.path.relpath(PMC_SUMMARY, ROOT) + " (FETCH_SIZE and WRITE_SIZE "
+                              "counter runs of this workload, copy-calibrated; not counted in "
+                              "this run)",
+            "kernel": kname,
+            "launches": launches, "avg_ms": round(kms, 4),
+            "alg_flops_per_launch": flops_launch,
+            "other_pair_kernel_ms_per_step": round(
+                (prof["kernel_ms_fp32"] if launches == prof["launches_sym"] else prof["kernel_ms_sym"])
+                / steps, 3)}
+
+
 def run_batched(args):
     """Batched ensemble: 64 independent wall systems of 100-4,000 segments (filaments
     x 20 on lattices as the 1,280-segment case, random tilts), every system's
@@ -667,7 +695,11 @@ def main():
     ap.add_argument("--skip-ciliated", action="store_true")
     ap.add_argument("--skip-ensemble", action="store_true")
     ap.add_argument("--skip-filament-ensemble", action="store_true")
-    ap.add_argument("--skip-overlap", action="store_true")
+    ap.add_argument("--skip-overlap", action="store_true",
+                    help="skip the comparison run with every product on one stream")
+    ap.add_argument("--serial", action="store_true",
+                    help="headline path with every product on one stream (no second stream "
+                         "beside the blob solve)")
     ap.add_argument("--no-pair-symmetry", action="store_true",
                     help="one-directional pair kernel for the segment-segment product")
     ap.add_argument("--symmetry-ab", action="store_true",
@@ -728,17 +760,20 @@ def main():
         print(json.dumps({args.only: fn(args)}), flush=True)
         return
     guess = "learned" if args.learned else "projected"
-    mixed = run_path(args, "mixed", "fp32c", args.mode, guess, args.precond)
+    # the fast path: the segment-segment product on a second stream beside the blob solve
+    # (cfm_fil_params.overlap), unless --serial
+    mixed = run_path(args, "mixed", "fp32c", args.mode, guess, args.precond, overlap=not args.serial)
     plain = None if args.skip_plain else run_path(args, "plain", "fp64", "fp64", "previous", "diag")
-    # the fast path with the segment-segment product on a second stream beside the blob
-    # solve (same results; reported beside the headline, whose roofline is measured
-    # without the overlap)
-    ovl = None
-    if not args.skip_overlap:
-        o = run_path(args, "overlap", "fp32c", args.mode, guess, args.precond, overlap=True)
-        ovl = {"steps_per_s": round(o["steps_per_s"], 4), "ms_per_step": round(o["ms_per_step"], 3),
+    # the same fast path with every product on one stream: the same results, and the
+    # dominant kernel's rate with nothing running beside it
+    ser = ser_run = None
+    if not args.skip_overlap and not args.serial:
+        ser_run = run_path(args, "serial", "fp32c", args.mode, guess, args.precond)
+        o = ser_run
+        ser = {"steps_per_s": round(o["steps_per_s"], 4), "ms_per_step": round(o["ms_per_step"], 3),
                "broyden_iterations": o["broyden_iterations"], "gmres_iterations": o["gmres_iterations"],
-               "max_gmres_rel_residual": o["max_gmres_rel_residual"]}
+               "max_gmres_rel_residual": o["max_gmres_rel_residual"],
+               "max_abs_state_diff_vs_overlapped": float(abs(o["final_x"] - mixed["final_x"]).max())}
 
     if plain is not None:
         # both paths start from the same state and take the same steps: the mixed path's
@@ -746,31 +781,18 @@ def main():
         import numpy as np
         mixed["max_abs_state_diff_vs_plain"] = float(np.abs(mixed["final_x"] - plain["final_x"]).max())
         mixed["same_broyden_iterations_as_plain"] = mixed["broyden_per_step"] == plain["broyden_per_step"]
-    prof = mixed["profile"]
-    # the dominant pair kernel: the symmetric segment-segment product when it runs
-    # (kernel + finalisation between the events; flops of both pair directions), else
-    # the one-directional tiled kernel
-    if prof["kernel_ms_sym"] >= prof["kernel_ms_fp32"]:
-        launches, ms_tot, fl_tot = prof["launches_sym"], prof["kernel_ms_sym"], prof["alg_flops_sym"]
-        kname = "rpy_sym_kernel + rpy_sym_finalize_kernel"
+    # The dominant kernel's rate is taken where it runs alone (the serial pass: the same
+    # steps, every product on one stream); in the overlapped headline pass it shares the
+    # device with the blob solve, so its launches last longer while the step gets shorter
+    # (that pass's mean is reported beside it).
+    if ser_run is not None:
+        roofline = pair_roofline(ser_run["profile"], args.steps)
+        roofline["measured_in"] = "serial pass (same steps, every product on one stream)"
+        ovl_roof = pair_roofline(mixed["profile"], args.steps)
+        roofline["avg_ms_in_overlapped_headline_pass"] = ovl_roof["avg_ms"]
     else:
-        launches, ms_tot, fl_tot = prof["launches_fp32"], prof["kernel_ms_fp32"], prof["alg_flops_fp32"]
-        kname = "rpy_apply_kernel<float,6,true>"
-    kms = ms_tot / max(1, launches)
-    flops_launch = fl_tot / max(1, launches)
-    achieved = flops_launch / (kms * 1e-3) / 1e12 if kms > 0 else 0.0
-    roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": FP32_VALU_PEAK_TFLOPS,
-                "unit": "TFLOP/s", "frac": round(achieved / FP32_VALU_PEAK_TFLOPS, 4),
-                "traffic": measured_traffic(kname),
-                "traffic_source": os.path.relpath(PMC_SUMMARY, ROOT) + " (FETCH_SIZE and WRITE_SIZE "
-                                  "counter runs of this workload, copy-calibrated; not counted in "
-                                  "this run)",
-                "kernel": kname,
-                "launches": launches, "avg_ms": round(kms, 4),
-                "alg_flops_per_launch": flops_launch,
-                "other_pair_kernel_ms_per_step": round(
-                    (prof["kernel_ms_fp32"] if launches == prof["launches_sym"] else prof["kernel_ms_sym"])
-                    / args.steps, 3)}
+        roofline = pair_roofline(mixed["profile"], args.steps)
+        roofline["measured_in"] = "headline pass" + ("" if args.serial else " (overlapped)")
     sym_ab = None
     if args.symmetry_ab:
         o = run_path(args, "one-directional", "fp32c", args.mode, guess, args.precond, symmetry=False)
@@ -826,7 +848,8 @@ def main():
         "ciliated_swimmer": ciliated,
         "orbit_ensemble": ensemble,
         "filament_ensemble": fil_ensemble,
-        "overlap": ovl,
+        "overlap": not args.serial,
+        "serial": ser,
         "one_directional_pair_kernel": sym_ab,
     }
     print(json.dumps(out), flush=True)

@@ -970,17 +970,18 @@ static Status jacobian_lds_limit(int device) {
   return {};
 }
 
-Status fil_jacobian(cfm_ctx* c, const FilK& K, int nt, int* status_dev) {
+Status fil_jacobian(cfm_ctx* c, const FilK& K, int nt, int* status_dev, hipStream_t s) {
+  const hipStream_t st = s ? s : c->stream;
   const int N = 6 * K.nseg;
   const size_t lds = (size_t)N * N * sizeof(double);
   if (K.nseg > kMaxFilSeg)
     return make_status(CFM_EINVAL, "filaments of %d segments exceed the limit of %d", K.nseg, kMaxFilSeg);
   CFM_TRY(jacobian_lds_limit(c->device));
-  hipLaunchKernelGGL(fil_jacobian_kernel, dim3(K.nfil), dim3(256), lds, c->stream, K, nt);
+  hipLaunchKernelGGL(fil_jacobian_kernel, dim3(K.nfil), dim3(256), lds, st, K, nt);
   CFM_CHECK_LAUNCH();
   // J0^-1 of every filament in place (the reference inverts each block with LAPACK
   // getrf/getri, matrix.cpp:538-571)
-  hipLaunchKernelGGL(fil_jacobian_invert_kernel, dim3(K.nfil), dim3(256), 0, c->stream, K, status_dev);
+  hipLaunchKernelGGL(fil_jacobian_invert_kernel, dim3(K.nfil), dim3(256), 0, st, K, status_dev);
   CFM_CHECK_LAUNCH();
   return {};
 }

@@ -69,7 +69,9 @@ Status fil_errors(cfm_ctx* c, const FilK& K, int nt, double* err, double* bad_pa
 // Repulsion on every blob from the segments and from the blobs of other bodies, on the
 // cell list fil_barrier built (rep: 3 x n_blob).
 Status fil_blob_barrier(cfm_ctx* c, const FilK& K, BarrierGrid& grid, int nb_per_body, double* rep);
-Status fil_jacobian(cfm_ctx* c, const FilK& K, int nt, int* status_dev);
+// Builds and inverts every filament's approximate Jacobian on stream s (the context's
+// stream when null).
+Status fil_jacobian(cfm_ctx* c, const FilK& K, int nt, int* status_dev, hipStream_t s = nullptr);
 Status fil_jinv_mult(cfm_ctx* c, const FilK& K, const double* in, double* out, double scale);
 Status fil_update(cfm_ctx* c, const FilK& K, const double* upd);
 // Badness of each of E equal systems from the residual err (Lm rows each, laid out as

@@ -229,6 +229,9 @@ struct FilState {
   double* m_host = nullptr;  // pinned read-back of the members' badness
   int* m_host_flags = nullptr;  // pinned staging of the active flags
   std::vector<cfm_step_stats> m_stats;
+  // the approximate Jacobian built on the side stream (jacobian_fork), joined by ev_jac
+  hipEvent_t ev_jac = nullptr;
+  bool stepped = false;  // a step of this state has completed
 };
 
 static Status ensure_host(FilState* F) {
@@ -470,6 +473,26 @@ static Status jacobian(cfm_ctx* c, FilState* F, int nt) {
   return {};
 }
 
+// The approximate Jacobian depends only on the state after the initial guess and its
+// internal loads, so for plain filaments (no free bodies, no blob forces among the
+// unknowns) it is built and inverted on the side stream while the main stream evaluates
+// the first velocities and residual: the inversion is bound by its per-pivot latency and
+// leaves most issue slots to the pair kernels beside it.  The main stream waits for
+// ev_jac before J0^-1 is first used (and before the step ends when no iteration is
+// needed).  Not on the first step of a state, whose products tune their source splits
+// by timing launches, which a concurrent launch would disturb.
+static Status jacobian_fork(cfm_ctx* c, FilState* F, int nt, bool* forked) {
+  *forked = false;
+  if (F->swim || F->bx || !F->stepped) return {};
+  if (!F->ev_jac) CFM_HIP(hipEventCreateWithFlags(&F->ev_jac, hipEventDisableTiming));
+  CFM_TRY(side_fork(c));
+  CFM_HIP(hipMemsetAsync(F->status.p, 0, sizeof(int), c->side));
+  CFM_TRY(fil_jacobian(c, F->K, nt, F->status.p, c->side));
+  CFM_HIP(hipEventRecord(F->ev_jac, c->side));
+  *forked = true;
+  return {};
+}
+
 static Status update(cfm_ctx* c, FilState* F, const double* upd) {
   if (F->swim) CFM_TRY(swim_update(c, F->S, upd + 6 * (size_t)c->p.n_seg));
   if (F->bx) CFM_TRY(bx_update(c, F->X, upd + 6 * ((size_t)c->p.n_seg + F->X.nbody)));
@@ -507,12 +530,15 @@ static Status step(cfm_ctx* c, FilState* F, int nt, cfm_step_stats* st) {
   const int maxit = F->prm.max_broyden;
   CFM_TRY(initial_guess(c, F, nt));
   CFM_TRY(fil_forces(c, F->K, nt));
+  bool jac_side = false;
+  CFM_TRY(jacobian_fork(c, F, nt, &jac_side));
   CFM_TRY(velocities(c, F, &S));
   double bad = 0.0;
   CFM_TRY(residual(c, F, nt, F->err.p, &bad));
+  if (jac_side) CFM_HIP(hipStreamWaitEvent(c->stream, F->ev_jac, 0));
   if (!std::isfinite(bad)) return make_status(CFM_ENONFINITE, "non-finite residual at step %d", nt);
   bool too_large = bad > 1.0;
-  if (too_large) CFM_TRY(jacobian(c, F, nt));
+  if (too_large && !jac_side) CFM_TRY(jacobian(c, F, nt));
   int it = 0;
   while (too_large && it < maxit) {
     CFM_TRY(low_rank_apply(c, F, F->err.p, F->upd.p, it));
@@ -537,6 +563,7 @@ static Status step(cfm_ctx* c, FilState* F, int nt, cfm_step_stats* st) {
     ++it;
   }
   CFM_HIP(hipStreamSynchronize(c->stream));
+  F->stepped = true;
   S.broyden_iterations = it;
   S.converged = !too_large;
   S.final_error = bad;
@@ -605,9 +632,12 @@ static Status member_step(cfm_ctx* c, FilState* F, int nt, cfm_step_stats* st) {
   std::vector<double> bad(E, 0.0), last(E, 0.0);
   CFM_TRY(initial_guess(c, F, nt));
   CFM_TRY(fil_forces(c, F->K, nt));
+  bool jac_side = false;
+  CFM_TRY(jacobian_fork(c, F, nt, &jac_side));
   CFM_TRY(member_flags(c, F, act));
   CFM_TRY(member_velocities(c, F));
   CFM_TRY(member_residual(c, F, nt, F->err.p, bad));
+  if (jac_side) CFM_HIP(hipStreamWaitEvent(c->stream, F->ev_jac, 0));
   bool any = false;
   for (int e = 0; e < E; ++e) {
     if (!std::isfinite(bad[e]))
@@ -617,7 +647,7 @@ static Status member_step(cfm_ctx* c, FilState* F, int nt, cfm_step_stats* st) {
     any = any || act[e];
   }
   CFM_TRY(member_flags(c, F, act));
-  if (any) CFM_TRY(jacobian(c, F, nt));
+  if (any && !jac_side) CFM_TRY(jacobian(c, F, nt));
   int it = 0;
   while (any && it < maxit) {
     CFM_TRY(member_low_rank(c, F, F->err.p, F->upd.p, it));
@@ -659,6 +689,7 @@ static Status member_step(cfm_ctx* c, FilState* F, int nt, cfm_step_stats* st) {
     ++it;
   }
   CFM_HIP(hipStreamSynchronize(c->stream));
+  F->stepped = true;
   const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   F->m_stats.assign(E, cfm_step_stats{});
   cfm_step_stats S{};
@@ -718,6 +749,7 @@ void cfm_fil_release(cfm_ctx* c) {
   if (F->host) (void)hipHostFree(F->host);
   if (F->m_host) (void)hipHostFree(F->m_host);
   if (F->m_host_flags) (void)hipHostFree(F->m_host_flags);
+  if (F->ev_jac) (void)hipEventDestroy(F->ev_jac);
   F->m_bad.release();
   F->m_norm.release();
   F->m_coef.release();
