@@ -55,24 +55,63 @@ __global__ __launch_bounds__(kRB) void dots_partial_kernel(const double* __restr
   if (threadIdx.x == 0) partial[(size_t)b * k + i] = red[0];
 }
 
-// Q[0..k)^T w in one launch.  Block (i, b) stores the partial of chunk b for column i
-// (as dots_partial_kernel), publishes it with an agent-scope release fence and takes a
-// ticket from `counter`; the block holding the last ticket (after an acquire fence)
-// reduces every column's partials -- a wave per column: lane l adds chunks l, l + 64,
-// ... in order, then the 64 lane sums are added by a butterfly in a fixed pattern --
-// and resets the counter for the next reduction queued on the stream.  No block waits
-// for another, and the order of every addition is fixed (deterministic).
+// Q[0..k)^T w in one launch.  Block b covers rows [b kChunk, (b+1) kChunk) of every
+// column, each thread the eight rows r0 + 256 j with its entries of w in registers.
+// Columns go four at a time: a thread's four running sums take all 32 loads at once,
+// each sum is reduced by a butterfly within the wave, and the four wave sums of a column
+// are added in wave order through LDS by wave 0, which stores the block's partials.  The
+// block then publishes them with an agent-scope release fence (by wave 0, the writer)
+// and takes a ticket from `counter`; the block holding the last ticket (after an acquire
+// fence) reduces every column's partials -- a wave per column: lane l adds chunks l,
+// l + 64, ... in order, then the 64 lane sums are added by a butterfly in a fixed
+// pattern -- and resets the counter for the next reduction queued on the stream.  No
+// block waits for another, one ticket is taken per block whatever k is, and the order of
+// every addition is fixed (deterministic).
 // mode 0: out[i] = s; 1: out[i] += s; 2: out[i] = sqrt(max(s, 0)).  out2, when given,
 // receives the same value at out2[i * ld2].
 // Members (blockIdx.z = e, independent reductions of equal length L): Q + e mq, w + e mw,
 // out + e ldo, counter e, their own partials; members with active[e] == 0 are skipped.
+constexpr int kPerThread = kChunk / kRB;
+constexpr int kColBatch = 4;
+
+// NC columns of Q (stride ldq) against this thread's rows r0 + 256 j: the products of
+// all columns are issued together, each column's sums reduced by a butterfly within the
+// wave; lane 0 writes the wave's sums to red_w[0 .. NC).  FULL: every row is below L.
+template <int NC, bool FULL>
+__device__ __forceinline__ void dots_columns(const double* __restrict__ Q, size_t ldq,
+                                             const double (&wv)[kPerThread], int r0, int L, int lane,
+                                             double* red_w) {
+  double v[NC][kPerThread];
+#pragma unroll
+  for (int j = 0; j < kPerThread; ++j) {
+    const int r = r0 + j * kRB;
+#pragma unroll
+    for (int i = 0; i < NC; ++i) v[i][j] = (FULL || r < L) ? Q[(size_t)i * ldq + r] : 0.0;
+  }
+  double s[NC];
+#pragma unroll
+  for (int i = 0; i < NC; ++i) {
+    s[i] = 0.0;
+#pragma unroll
+    for (int j = 0; j < kPerThread; ++j) s[i] += v[i][j] * wv[j];
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+    for (int i = 0; i < NC; ++i) s[i] += __shfl_xor(s[i], off, 64);
+  }
+  if (lane == 0) {
+#pragma unroll
+    for (int i = 0; i < NC; ++i) red_w[i] = s[i];
+  }
+}
 __global__ __launch_bounds__(kRB) void dots_fused_kernel(const double* __restrict__ Q, size_t ldq,
                                                          int k, const double* __restrict__ w, int L,
                                                          double* partial, unsigned* counter,
                                                          double* out, int mode, double* out2,
                                                          int ld2, size_t mq, size_t mw, int ldo,
                                                          const int* __restrict__ active) {
-  __shared__ double red[kRB];
+  __shared__ double red[kRB / 64][64];
   __shared__ int last;
   const int e = blockIdx.z;
   if (active && !active[e]) return;
@@ -80,30 +119,46 @@ __global__ __launch_bounds__(kRB) void dots_fused_kernel(const double* __restric
   w += e * mw;
   out += (size_t)e * ldo;
   counter += e;
-  partial += (size_t)e * gridDim.x * gridDim.y;
-  const int i = blockIdx.x;
-  const int b = blockIdx.y;
-  const double* q = Q + (size_t)i * ldq;
-  const int r0 = b * kChunk;
-  const int r1 = min(L, r0 + kChunk);
-  double s = 0.0;
-  for (int r = r0 + threadIdx.x; r < r1; r += kRB) s += q[r] * w[r];
-  red[threadIdx.x] = s;
-  __syncthreads();
-  for (int off = kRB / 2; off > 0; off >>= 1) {
-    if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+  partial += (size_t)e * gridDim.x * k;
+  const int b = blockIdx.x;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int r0 = b * kChunk + threadIdx.x;
+  const bool full = (b + 1) * kChunk <= L;
+  double wv[kPerThread];
+#pragma unroll
+  for (int j = 0; j < kPerThread; ++j) {
+    const int r = r0 + j * kRB;
+    wv[j] = r < L ? w[r] : 0.0;
+  }
+  for (int g0 = 0; g0 < k; g0 += 64) {
+    const int gn = min(64, k - g0);
+    int c0 = 0;
+    if (full) {
+      for (; c0 + kColBatch <= gn; c0 += kColBatch)
+        dots_columns<kColBatch, true>(Q + (size_t)(g0 + c0) * ldq, ldq, wv, r0, L, lane, red[wave] + c0);
+      for (; c0 < gn; ++c0)
+        dots_columns<1, true>(Q + (size_t)(g0 + c0) * ldq, ldq, wv, r0, L, lane, red[wave] + c0);
+    } else {
+      for (; c0 + kColBatch <= gn; c0 += kColBatch)
+        dots_columns<kColBatch, false>(Q + (size_t)(g0 + c0) * ldq, ldq, wv, r0, L, lane, red[wave] + c0);
+      for (; c0 < gn; ++c0)
+        dots_columns<1, false>(Q + (size_t)(g0 + c0) * ldq, ldq, wv, r0, L, lane, red[wave] + c0);
+    }
     __syncthreads();
+    if ((int)threadIdx.x < gn) {
+      const int i = threadIdx.x;
+      partial[(size_t)b * k + g0 + i] = (red[0][i] + red[1][i]) + (red[2][i] + red[3][i]);
+    }
+    __syncthreads();  // red is rewritten by the next group of columns
   }
   if (threadIdx.x == 0) {
-    partial[(size_t)b * k + i] = red[0];
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    last = atomicAdd(counter, 1u) == gridDim.x * gridDim.y - 1u;
+    last = atomicAdd(counter, 1u) == gridDim.x - 1u;
   }
   __syncthreads();
   if (!last) return;
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-  const int nb = gridDim.y;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int nb = gridDim.x;
   for (int col = wave; col < k; col += kRB / 64) {
     double t = 0.0;
     for (int bb = lane; bb < nb; bb += 64) t += partial[(size_t)bb * k + col];
@@ -374,7 +429,7 @@ static Status dots_launch(cfm_ctx* c, double* partial, const double* Q, size_t l
   unsigned* cnt = nullptr;
   CFM_TRY(red_counter(c, &cnt, members));
   const int nb = (L + kChunk - 1) / kChunk;
-  hipLaunchKernelGGL(dots_fused_kernel, dim3(k, nb, members), dim3(kRB), 0, c->stream, Q, ldq, k,
+  hipLaunchKernelGGL(dots_fused_kernel, dim3(nb, 1, members), dim3(kRB), 0, c->stream, Q, ldq, k,
                      w, L, partial, cnt, out, mode, out2, ld2, mq, mw, ldo, active);
   CFM_CHECK_LAUNCH();
   return {};

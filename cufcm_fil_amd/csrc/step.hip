@@ -157,6 +157,21 @@ __global__ void copy2_kernel(const double* __restrict__ a, double* __restrict__ 
   dst_b[r] = b[r];
 }
 
+// out = x (n[0] - s[0]) (the scalars on the device)
+__global__ void scale_by_diff_kernel(const double* __restrict__ x, const double* __restrict__ n,
+                                     const double* __restrict__ s, double* __restrict__ out, int L) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < L) out[r] = x[r] * (n[0] - s[0]);
+}
+
+// out[r] = x[r] (n[m] - s[m]), m = r / Lm
+__global__ void member_scale_by_diff_kernel(const double* __restrict__ x, const double* __restrict__ n,
+                                            const double* __restrict__ s, double* __restrict__ out,
+                                            int L, int Lm) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < L) out[r] = x[r] * (n[r / Lm] - s[r / Lm]);
+}
+
 static Status history_reset(History& h, int L) {
   h.L = L;
   h.count = h.head = 0;
@@ -224,7 +239,7 @@ struct FilState {
   // Broyden badness, low-rank coefficients (members x (max_broyden + 1)), norms and the
   // active flag (1 while its iteration continues)
   int members = 1;
-  DevBuf<double> m_bad, m_coef, m_norm, m_partial;
+  DevBuf<double> m_bad, m_coef, m_norm, m_dot, m_partial;
   DevBuf<int> m_active;
   double* m_host = nullptr;  // pinned read-back of the members' badness
   int* m_host_flags = nullptr;  // pinned staging of the active flags
@@ -540,8 +555,10 @@ static Status step(cfm_ctx* c, FilState* F, int nt, cfm_step_stats* st) {
   bool too_large = bad > 1.0;
   if (too_large && !jac_side) CFM_TRY(jacobian(c, F, nt));
   int it = 0;
+  bool upd_ready = false;  // the update was formed with the last low-rank column
   while (too_large && it < maxit) {
-    CFM_TRY(low_rank_apply(c, F, F->err.p, F->upd.p, it));
+    if (!upd_ready) CFM_TRY(low_rank_apply(c, F, F->err.p, F->upd.p, it));
+    upd_ready = false;
     CFM_TRY(update(c, F, F->upd.p));
     CFM_TRY(fil_forces(c, F->K, nt));
     CFM_TRY(velocities(c, F, &S));
@@ -558,6 +575,14 @@ static Status step(cfm_ctx* c, FilState* F, int nt, cfm_step_stats* st) {
       CFM_TRY(dev_norm2(c, F->partial.p, Di, L, F->scal.p + 1));
       CFM_TRY(dev_scale_by_inv(c, Ci, F->scal.p + 1, Ci, L));
       CFM_TRY(dev_scale_by_inv(c, Di, F->scal.p + 1, Di, L));
+      // The next update -gamma J0^-1 e - sum_{i<=it} (e . D_i) C_i (e = the new error)
+      // is C_it's unnormalised column minus its own last term: C_it (n - e . D_it) with
+      // n = |e - e_old|.  Formed here, it saves a J0^-1 product and it dot products.
+      CFM_TRY(dev_dots(c, F->partial.p, Di, (size_t)L, 1, F->nerr.p, L, F->scal.p + 2));
+      hipLaunchKernelGGL(scale_by_diff_kernel, dim3((L + 255) / 256), dim3(256), 0, c->stream, Ci,
+                         F->scal.p + 1, F->scal.p + 2, F->upd.p, L);
+      CFM_CHECK_LAUNCH();
+      upd_ready = true;
       std::swap(F->err.p, F->nerr.p);
     }
     ++it;
@@ -649,8 +674,10 @@ static Status member_step(cfm_ctx* c, FilState* F, int nt, cfm_step_stats* st) {
   CFM_TRY(member_flags(c, F, act));
   if (any && !jac_side) CFM_TRY(jacobian(c, F, nt));
   int it = 0;
+  bool upd_ready = false;  // the update was formed with the last low-rank column (as step())
   while (any && it < maxit) {
-    CFM_TRY(member_low_rank(c, F, F->err.p, F->upd.p, it));
+    if (!upd_ready) CFM_TRY(member_low_rank(c, F, F->err.p, F->upd.p, it));
+    upd_ready = false;
     CFM_TRY(dev_member_mask(c, F->upd.p, L, Lm, F->m_active.p));  // held members: zero update
     CFM_TRY(update(c, F, F->upd.p));
     CFM_TRY(fil_forces(c, F->K, nt));
@@ -680,6 +707,13 @@ static Status member_step(cfm_ctx* c, FilState* F, int nt, cfm_step_stats* st) {
                               F->m_active.p));
       CFM_TRY(dev_member_scale_by_inv(c, Ci, F->m_norm.p, Ci, L, Lm));
       CFM_TRY(dev_member_scale_by_inv(c, Di, F->m_norm.p, Di, L, Lm));
+      // the next update of each continuing member, C_it (n - e . D_it) (as step())
+      CFM_TRY(dev_member_dots(c, F->m_partial.p, Di, 0, 1, F->nerr.p, Lm, E, F->m_dot.p, 1, false,
+                              F->m_active.p));
+      hipLaunchKernelGGL(member_scale_by_diff_kernel, dim3((L + 255) / 256), dim3(256), 0, c->stream,
+                         Ci, F->m_norm.p, F->m_dot.p, F->upd.p, L, Lm);
+      CFM_CHECK_LAUNCH();
+      upd_ready = true;
       std::swap(F->err.p, F->nerr.p);
     } else if (cont) {
       CFM_TRY(member_flags(c, F, next));
@@ -752,6 +786,7 @@ void cfm_fil_release(cfm_ctx* c) {
   if (F->ev_jac) (void)hipEventDestroy(F->ev_jac);
   F->m_bad.release();
   F->m_norm.release();
+  F->m_dot.release();
   F->m_coef.release();
   F->m_partial.release();
   F->m_active.release();
@@ -899,6 +934,7 @@ int cfm_fil_set_members(cfm_ctx* c, int32_t members) {
     if (members == 1) return {};
     CFM_TRY(F->m_bad.alloc(members));
     CFM_TRY(F->m_norm.alloc(members));
+    CFM_TRY(F->m_dot.alloc(members));
     CFM_TRY(F->m_coef.alloc((size_t)members * ldc));
     CFM_TRY(dev_fill(c, F->m_coef.p, 0.0, members * ldc));
     CFM_TRY(F->m_partial.alloc(dev_member_dots_partial_size(Lm, ldc, members)));
