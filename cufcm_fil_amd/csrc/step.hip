@@ -13,6 +13,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <functional>
 
 #include "ctx.hpp"
 #include "body.hpp"
@@ -246,6 +247,7 @@ struct FilState {
   std::vector<cfm_step_stats> m_stats;
   // the approximate Jacobian built on the side stream (jacobian_fork), joined by ev_jac
   hipEvent_t ev_jac = nullptr;
+  hipEvent_t ev_res = nullptr;  // the Broyden residual's read-back
   bool stepped = false;  // a step of this state has completed
 };
 
@@ -457,7 +459,11 @@ static Status swim_velocities(cfm_ctx* c, FilState* F, cfm_step_stats* st) {
   return apply_plan(c, sb, prec);
 }
 
-static Status residual(cfm_ctx* c, FilState* F, int nt, double* e, double* bad) {
+// The residual e of the Broyden rows and its badness read to *bad.  before_wait, when
+// given, is queued after the read-back and before the host waits for it (work that the
+// outcome may make useless but never wrong).
+static Status residual(cfm_ctx* c, FilState* F, int nt, double* e, double* bad,
+                       const std::function<Status()>* before_wait = nullptr) {
   int extra = 0;
   if (F->bx) {
     const int nb = (c->p.n_seg + 255) / 256;
@@ -470,7 +476,15 @@ static Status residual(cfm_ctx* c, FilState* F, int nt, double* e, double* bad) 
     extra = F->S.nbody;
   }
   CFM_TRY(fil_errors(c, F->K, nt, e, F->bad_part.p, F->scal.p, extra));
-  return read_dev(c, F, F->scal.p, bad);
+  if (!before_wait) return read_dev(c, F, F->scal.p, bad);
+  CFM_TRY(ensure_host(F));
+  if (!F->ev_res) CFM_HIP(hipEventCreateWithFlags(&F->ev_res, hipEventDisableTiming));
+  CFM_HIP(hipMemcpyAsync(F->host, F->scal.p, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  CFM_HIP(hipEventRecord(F->ev_res, c->stream));
+  CFM_TRY((*before_wait)());
+  CFM_HIP(hipEventSynchronize(F->ev_res));
+  *bad = F->host[0];
+  return {};
 }
 
 // out = scale J0^-1 in (block diagonal over filaments, plus the body Schur complement)
@@ -556,32 +570,39 @@ static Status step(cfm_ctx* c, FilState* F, int nt, cfm_step_stats* st) {
   if (too_large && !jac_side) CFM_TRY(jacobian(c, F, nt));
   int it = 0;
   bool upd_ready = false;  // the update was formed with the last low-rank column
+  // The low-rank pair of iteration `it` and the next update, from the new error
+  // (broyden_solver.cpp:69-125).  Queued before the host reads the residual: when the
+  // step has converged they are not used (the next step starts from column 0).
+  const std::function<Status()> next_column = [&]() -> Status {
+    double* Ci = F->C.p + (size_t)it * L;
+    double* Di = F->D.p + (size_t)it * L;
+    CFM_TRY(low_rank_apply(c, F, F->nerr.p, Ci, it));
+    CFM_TRY(dev_axpby(c, 1.0, F->nerr.p, -1.0, F->err.p, Di, L));
+    CFM_TRY(dev_norm2(c, F->partial.p, Di, L, F->scal.p + 1));
+    CFM_TRY(dev_scale_by_inv(c, Ci, F->scal.p + 1, Ci, L));
+    CFM_TRY(dev_scale_by_inv(c, Di, F->scal.p + 1, Di, L));
+    // The next update -gamma J0^-1 e - sum_{i<=it} (e . D_i) C_i (e = the new error) is
+    // C_it's unnormalised column minus its own last term: C_it (n - e . D_it) with
+    // n = |e - e_old|.  Formed here, it saves a J0^-1 product and it dot products.
+    CFM_TRY(dev_dots(c, F->partial.p, Di, (size_t)L, 1, F->nerr.p, L, F->scal.p + 2));
+    hipLaunchKernelGGL(scale_by_diff_kernel, dim3((L + 255) / 256), dim3(256), 0, c->stream, Ci,
+                       F->scal.p + 1, F->scal.p + 2, F->upd.p, L);
+    CFM_CHECK_LAUNCH();
+    return {};
+  };
   while (too_large && it < maxit) {
     if (!upd_ready) CFM_TRY(low_rank_apply(c, F, F->err.p, F->upd.p, it));
     upd_ready = false;
     CFM_TRY(update(c, F, F->upd.p));
     CFM_TRY(fil_forces(c, F->K, nt));
     CFM_TRY(velocities(c, F, &S));
-    CFM_TRY(residual(c, F, nt, F->nerr.p, &bad));
+    const bool more = it < maxit - 1;  // a further iteration would be allowed
+    CFM_TRY(residual(c, F, nt, F->nerr.p, &bad, more ? &next_column : nullptr));
     if (!std::isfinite(bad))
       return make_status(CFM_ENONFINITE, "Broyden's method diverged after %d iterations during step %d",
                          it + 1, nt + 1);
     too_large = bad > 1.0;
-    if (too_large && it < maxit - 1) {
-      double* Ci = F->C.p + (size_t)it * L;
-      double* Di = F->D.p + (size_t)it * L;
-      CFM_TRY(low_rank_apply(c, F, F->nerr.p, Ci, it));
-      CFM_TRY(dev_axpby(c, 1.0, F->nerr.p, -1.0, F->err.p, Di, L));
-      CFM_TRY(dev_norm2(c, F->partial.p, Di, L, F->scal.p + 1));
-      CFM_TRY(dev_scale_by_inv(c, Ci, F->scal.p + 1, Ci, L));
-      CFM_TRY(dev_scale_by_inv(c, Di, F->scal.p + 1, Di, L));
-      // The next update -gamma J0^-1 e - sum_{i<=it} (e . D_i) C_i (e = the new error)
-      // is C_it's unnormalised column minus its own last term: C_it (n - e . D_it) with
-      // n = |e - e_old|.  Formed here, it saves a J0^-1 product and it dot products.
-      CFM_TRY(dev_dots(c, F->partial.p, Di, (size_t)L, 1, F->nerr.p, L, F->scal.p + 2));
-      hipLaunchKernelGGL(scale_by_diff_kernel, dim3((L + 255) / 256), dim3(256), 0, c->stream, Ci,
-                         F->scal.p + 1, F->scal.p + 2, F->upd.p, L);
-      CFM_CHECK_LAUNCH();
+    if (too_large && more) {
       upd_ready = true;
       std::swap(F->err.p, F->nerr.p);
     }
@@ -784,6 +805,7 @@ void cfm_fil_release(cfm_ctx* c) {
   if (F->m_host) (void)hipHostFree(F->m_host);
   if (F->m_host_flags) (void)hipHostFree(F->m_host_flags);
   if (F->ev_jac) (void)hipEventDestroy(F->ev_jac);
+  if (F->ev_res) (void)hipEventDestroy(F->ev_res);
   F->m_bad.release();
   F->m_norm.release();
   F->m_dot.release();
