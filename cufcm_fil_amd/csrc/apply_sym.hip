@@ -262,14 +262,24 @@ __device__ __forceinline__ void stage(const SymArgs<T>& a, SymTile<T>& t, int j,
   }
 }
 
+// Waves that share one block pair: in FP64 two waves take 32 steps each of a block
+// pair's 64 (twice the waves of half the length: the FP64 blob products have few block
+// pairs, and short waves fill the CUs evenly); in FP32 one wave takes all 64 steps.
+template <typename T>
+constexpr int sym_halves() { return sizeof(T) == 8 ? 2 : 1; }
+
 template <typename T, int D>
 __global__ __launch_bounds__(256) void rpy_sym_kernel(SymArgs<T> a) {
   if (!member_args(a)) return;
-  __shared__ SymTile<T> tile[kWaves];
+  constexpr int H = sym_halves<T>();
+  constexpr int kOff = kWaves / H;    // ring offsets per round
+  constexpr int kSteps = kBlk / H;    // steps of a block pair per wave
+  __shared__ SymTile<T> tile[kOff];
   __shared__ double red[kWaves * D][kBlk];
   constexpr bool kF32 = sizeof(T) == 4;
   const int I = blockIdx.x, c = blockIdx.y;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int half = wave % H, slot = wave / H;
   const int B = a.nblk;
   // this lane's target (placeholder past the end, as staged)
   const int i = I * kBlk + lane;
@@ -297,61 +307,87 @@ __global__ __launch_bounds__(256) void rpy_sym_kernel(SymArgs<T> a) {
     c_acc[k] = T(0);
   }
   const int d_begin = c * a.dpw, d_end = min(d_begin + a.dpw, a.Dn + 1);
-  SymTile<T>& Tl = tile[wave];
-  for (int d0 = d_begin; d0 < d_end; d0 += kWaves) {
-    const int d = d0 + wave;
+  SymTile<T>& Tl = tile[slot];
+  // H = 2: the second half's source sums, [slot][k][lane] (red is free until the end)
+  T* wsum = reinterpret_cast<T*>(&red[0][0]);
+  for (int d0 = d_begin; d0 < d_end; d0 += kOff) {
+    const int d = d0 + slot;
     const bool mine = d < d_end && takes(I, d, B);
     const int J = (I + d) % B;
     __syncthreads();
-    if (mine) stage<T, D>(a, Tl, J * kBlk + lane, lane);
+    if (mine && half == 0) stage<T, D>(a, Tl, J * kBlk + lane, lane);
     __syncthreads();
-    if (!mine) continue;
-    T part[D];
+    T W[D];
 #pragma unroll
-    for (int k = 0; k < D; ++k) part[k] = T(0);
-    if (d == 0) {
-      // the block with itself, one way round, without the self pair
-      for (int s = 1; s < kBlk; ++s) {
-        const int js = (lane + s) & (kBlk - 1);
-        const T dx = T(xi - Tl.x[js]), dy = T(yi - Tl.y[js]), dz = T(zi - Tl.z[js]);
-        const T zj = Tl.zf[js];
-        Blk66<T> M;
-        wall_block<T, D>(dx, dy, dz, zit + zj, zj, M);
-        T F[D];
+    for (int k = 0; k < D; ++k) W[k] = T(0);
+    if (mine) {
+      T part[D];
 #pragma unroll
-        for (int k = 0; k < D; ++k) F[k] = Tl.f[k][js];
-        blk_apply<T, D>(M, F, part);
+      for (int k = 0; k < D; ++k) part[k] = T(0);
+      const int s0 = half * kSteps, s1 = s0 + kSteps;
+      if (d == 0) {
+        // the block with itself, one way round, without the self pair
+        for (int s = max(s0, 1); s < s1; ++s) {
+          const int js = (lane + s) & (kBlk - 1);
+          const T dx = T(xi - Tl.x[js]), dy = T(yi - Tl.y[js]), dz = T(zi - Tl.z[js]);
+          const T zj = Tl.zf[js];
+          Blk66<T> M;
+          wall_block<T, D>(dx, dy, dz, zit + zj, zj, M);
+          T F[D];
+#pragma unroll
+          for (int k = 0; k < D; ++k) F[k] = Tl.f[k][js];
+          blk_apply<T, D>(M, F, part);
+        }
+      } else {
+        for (int s = s0; s < s1; ++s) {
+          const int js = (lane + s) & (kBlk - 1);
+          const T dx = T(xi - Tl.x[js]), dy = T(yi - Tl.y[js]), dz = T(zi - Tl.z[js]);
+          const T zj = Tl.zf[js];
+          Blk66<T> M;
+          wall_block<T, D>(dx, dy, dz, zit + zj, zj, M);
+          T F[D];
+#pragma unroll
+          for (int k = 0; k < D; ++k) F[k] = Tl.f[k][js];
+          blk_apply<T, D>(M, F, part);
+          blk_apply_t<T, D>(M, fi, W);
+          // hand the running source sum to the lane that meets this source next
+#pragma unroll
+          for (int k = 0; k < D; ++k) W[k] = __shfl(W[k], (lane + 1) & (kBlk - 1), 64);
+        }
+        if constexpr (H > 1) {
+          // lane l holds source (l + s1) mod 64: rotate so that lane l holds source l
+          const int r = s1 & (kBlk - 1);
+#pragma unroll
+          for (int k = 0; k < D; ++k) W[k] = __shfl(W[k], (lane - r) & (kBlk - 1), 64);
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < D; ++k) {
+        if constexpr (kF32)
+          two_sum_add(s_acc[k], c_acc[k], part[k]);
+        else
+          s_acc[k] += part[k];
+      }
+    }
+    // lane l of the first half now holds source l of block J (plus, for H = 2, the second
+    // half's sum for it, added after the first's)
+    if constexpr (H > 1) {
+      if (mine && d != 0 && half == 1) {
+#pragma unroll
+        for (int k = 0; k < D; ++k) wsum[(slot * D + k) * kBlk + lane] = W[k];
+      }
+      __syncthreads();
+      if (mine && d != 0 && half == 0) {
+        T* dst = a.src + ((size_t)J * a.Dn + (d - 1)) * D * kBlk;
+#pragma unroll
+        for (int k = 0; k < D; ++k) dst[k * kBlk + lane] = W[k] + wsum[(slot * D + k) * kBlk + lane];
       }
     } else {
-      T W[D];
+      if (mine && d != 0) {
+        T* dst = a.src + ((size_t)J * a.Dn + (d - 1)) * D * kBlk;
 #pragma unroll
-      for (int k = 0; k < D; ++k) W[k] = T(0);
-      for (int s = 0; s < kBlk; ++s) {
-        const int js = (lane + s) & (kBlk - 1);
-        const T dx = T(xi - Tl.x[js]), dy = T(yi - Tl.y[js]), dz = T(zi - Tl.z[js]);
-        const T zj = Tl.zf[js];
-        Blk66<T> M;
-        wall_block<T, D>(dx, dy, dz, zit + zj, zj, M);
-        T F[D];
-#pragma unroll
-        for (int k = 0; k < D; ++k) F[k] = Tl.f[k][js];
-        blk_apply<T, D>(M, F, part);
-        blk_apply_t<T, D>(M, fi, W);
-        // hand the running source sum to the lane that meets this source next
-#pragma unroll
-        for (int k = 0; k < D; ++k) W[k] = __shfl(W[k], (lane + 1) & (kBlk - 1), 64);
+        for (int k = 0; k < D; ++k) dst[k * kBlk + lane] = W[k];
       }
-      // lane l now holds source l of block J
-      T* dst = a.src + ((size_t)J * a.Dn + (d - 1)) * D * kBlk;
-#pragma unroll
-      for (int k = 0; k < D; ++k) dst[k * kBlk + lane] = W[k];
-    }
-#pragma unroll
-    for (int k = 0; k < D; ++k) {
-      if constexpr (kF32)
-        two_sum_add(s_acc[k], c_acc[k], part[k]);
-      else
-        s_acc[k] += part[k];
     }
   }
   // the four waves' target sums, added in wave order
@@ -429,18 +465,21 @@ int sym_species(const cfm_ctx* c, const ApplyPlan& plan) {
   return (plan.tgt_begin <= p.n_seg && plan.tgt_end >= p.n && p.n_blob >= 2 * kBlk) ? 2 : 0;
 }
 
-// Workgroups per block row: enough for a few workgroups per CU, offsets per workgroup a
-// multiple of the four waves and at most eight (two rounds).  Short workgroups (about
+// Workgroups per block row: enough for a few workgroups per CU (twice as many in FP64,
+// whose waves take half a block pair each), offsets per workgroup a multiple of the
+// offsets of one round and at most eight.  Short workgroups (about
 // 25 us at 26,000 segments) leave no long tail at the end of a launch and, when a
 // product runs on a second stream, free a slot for the other stream's kernels every few
 // microseconds: 36 offsets per workgroup ran the 26,000-segment product 4% slower and
 // held the blob solve beside it waiting for slots.
+template <typename T>
 static void sym_shape(int cnt, int* B, int* Dn, int* S, int* dpw) {
+  constexpr int kOff = kWaves / sym_halves<T>();  // ring offsets per round
   *B = (cnt + kBlk - 1) / kBlk;
   *Dn = *B / 2;
-  const int want = std::max(1, (8 * 256 + *B - 1) / *B);
+  const int want = std::max(1, (8 * 256 * (kWaves / kOff) + *B - 1) / *B);
   int per = (*Dn + 1 + want - 1) / want;
-  per = std::max(kWaves, (per + kWaves - 1) / kWaves * kWaves);
+  per = std::max(kOff, (per + kOff - 1) / kOff * kOff);
   per = std::min(per, 2 * kWaves);
   *dpw = per;
   *S = (*Dn + 1 + per - 1) / per;
@@ -471,7 +510,7 @@ static Status launch_sym_t(cfm_ctx* c, const ApplyPlan& plan, int base, int cnt,
   a.acc = (base == 0 ? plan.acc_seg : plan.acc_blob) ? 1 : 0;
   a.mu = c->cfg.mu;
   a.cnt = cnt;
-  sym_shape(cnt, &a.nblk, &a.Dn, &a.S, &a.dpw);
+  sym_shape<T>(cnt, &a.nblk, &a.Dn, &a.S, &a.dpw);
   DevBuf<double>& own = plan.side ? c->sym_own_side : c->sym_own;
   CFM_TRY(own.alloc((size_t)a.nblk * a.S * D * kBlk));
   CFM_TRY(src.alloc((size_t)a.nblk * std::max(a.Dn, 1) * D * kBlk));
@@ -506,7 +545,7 @@ static Status sym_members_t(hipStream_t stream, const double* x, const double* y
   a.acc = 0;
   a.mu = mu;
   a.cnt = ns;
-  sym_shape(ns, &a.nblk, &a.Dn, &a.S, &a.dpw);
+  sym_shape<T>(ns, &a.nblk, &a.Dn, &a.S, &a.dpw);
   a.mpos = ns;
   a.mvec = mvec;
   a.mown = (size_t)a.nblk * a.S * D * kBlk;
