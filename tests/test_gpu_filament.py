@@ -129,6 +129,25 @@ def test_implicit_steps_match_oracle(nsteps):
             assert np.abs(g["lam"].reshape(-1, 3) - np.concatenate([f.lam for f in fils])).max() < 1e-6
 
 
+@pytest.mark.parametrize("cap", [1, 2, 3])
+def test_capped_broyden_step_matches_oracle(cap):
+    """A step stopped by the Broyden iteration cap (MAX_BROYDEN_ITER) before it converges:
+    the library forms each next low-rank column before it reads the residual, except on
+    the last allowed iteration; the capped state must still be the oracle's after the same
+    number of iterations (broyden_solver.cpp:69-125)."""
+    x, t = _system(nx=2, ny=1)
+    P = _params()
+    pert = 0.01 * P.end_force * np.array([1.0, 0.0, 0.0, 0.0, 1.0, 0.0])
+    with FilamentSystem(x, t, pert=pert, max_broyden=cap) as S:
+        fils = _oracle_fils(x, t, 20, P, pert)
+        st = S.step(0)
+        it = fo.implicit_step(fils, 0, _oracle_velocity, max_iter=cap)
+        assert it == cap and st["broyden_iterations"] == cap and not st["converged"], (st, it)
+        g = S.state()
+        assert np.abs(g["x"].reshape(-1, 3) - np.concatenate([f.x for f in fils])).max() < 1e-8
+        assert np.abs(g["lam"].reshape(-1, 3) - np.concatenate([f.lam for f in fils])).max() < 1e-6
+
+
 def test_step_with_fixed_body_matches_oracle():
     xs, xb, c, d, t = geo.sphere_over_wall(n_blob=200, n_fil=6, radius=8.0, gap=50.0,
                                             return_tangents=True)
